@@ -1,0 +1,314 @@
+/*
+ * RTGL1 extensions of the MI355X implementation.
+ *
+ * RTGL1 has no readback and no stage-level entry points: its passes run
+ * inside rgDrawFrame (Source/VulkanDevice.cpp:484-764) and results only reach
+ * the swapchain.  These functions expose each hot-path stage on device
+ * buffers so that an application (or a test harness) can inject inputs,
+ * run one stage and read its output.  Every function returns RgResult and
+ * never throws; a stream argument is a hipStream_t passed as void* (null =
+ * the default stream).  All pointers marked d_ are device pointers.
+ *
+ * The reference interface each entry point stands in for is cited beside it.
+ */
+#ifndef RTGL1_EXT_H_
+#define RTGL1_EXT_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "RTGL1.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------- device memory */
+/* Plain HIP memory helpers so that callers without their own HIP runtime
+ * (ctypes, cgo, JNI) can stage inputs.  No counterpart in the reference
+ * (its staging is VMA-mapped memory, Source/MemoryAllocator.cpp). */
+RGAPI RgResult RGCONV rgExtDeviceAlloc( size_t bytes, void** pOut );
+RGAPI RgResult RGCONV rgExtDeviceFree( void* d_ptr );
+RGAPI RgResult RGCONV rgExtCopyHostToDevice( void* d_dst, const void* src, size_t bytes );
+RGAPI RgResult RGCONV rgExtCopyDeviceToHost( void* dst, const void* d_src, size_t bytes );
+RGAPI RgResult RGCONV rgExtMemsetDevice( void* d_dst, int value, size_t bytes );
+RGAPI RgResult RGCONV rgExtSynchronize( void );
+/* Number of visible HIP devices (0 without a GPU). */
+RGAPI int RGCONV rgExtGetDeviceCount( void );
+/* Last error message of this thread ("" if none). */
+RGAPI const char* RGCONV rgExtGetLastErrorMessage( void );
+
+/* --------------------------------------------------------- stage timing */
+/* Elapsed GPU time between two points of a stream, measured with HIP events
+ * recorded on that stream. */
+typedef struct RgExtTimer_T* RgExtTimer;
+RGAPI RgResult RGCONV rgExtTimerCreate( RgExtTimer* pOut );
+RGAPI RgResult RGCONV rgExtTimerDestroy( RgExtTimer timer );
+RGAPI RgResult RGCONV rgExtTimerStart( RgExtTimer timer, void* stream );
+RGAPI RgResult RGCONV rgExtTimerStop( RgExtTimer timer, void* stream );
+/* Waits for the stop event and returns start->stop in milliseconds. */
+RGAPI RgResult RGCONV rgExtTimerElapsedMs( RgExtTimer timer, float* pMs );
+
+/* ------------------------------------------------------------------ LBVH */
+/* Replaces the driver acceleration-structure build behind
+ * ASBuilder::BuildBottomLevel / BuildTopLevel (Source/ASBuilder.cpp:141-155,
+ * 193-206; vkCmdBuildAccelerationStructuresKHR at :147,:199).
+ *
+ * Input: n world-space triangles, 9 floats each (v0.xyz v1.xyz v2.xyz),
+ * with transforms already applied (the reference bakes them into the BLAS,
+ * Source/VertexCollector.cpp:259-261).  The primitive id of a triangle is
+ * its index in this array.
+ *
+ * Build: FP64 centroid bounds and 21-bit-per-axis quantisation to 63-bit
+ * Morton codes, stable LSD radix sort of (code, index), Karras hierarchy
+ * over the sorted codes (ties broken by index), bottom-up AABB refit, then
+ * collapse into 8-wide nodes.  The result is deterministic: the same input
+ * gives the same nodes bit for bit. */
+typedef struct RgExtBvh_T* RgExtBvh;
+
+#define RG_EXT_BVH_NODE_FP32 0u /* 256-byte nodes with FP32 child boxes     */
+#define RG_EXT_BVH_NODE_FP16 1u /* 128-byte nodes, boxes rounded outward to FP16 */
+
+typedef struct RgExtBvhBuildParams
+{
+    uint32_t maxLeafSize; /* triangles per leaf child, 1..8 (default 3) */
+    uint32_t nodeFormat;  /* RG_EXT_BVH_NODE_* */
+} RgExtBvhBuildParams;
+
+/* Child reference encoding inside an 8-wide node. */
+#define RG_EXT_BVH_CHILD_EMPTY     0xFFFFFFFFu
+#define RG_EXT_BVH_CHILD_LEAF_BIT  0x80000000u
+/* leaf: LEAF_BIT | (firstSortedTriangle << 3) | (count - 1) */
+
+/* View of a built BVH: device pointers owned by the RgExtBvh object. */
+typedef struct RgExtBvhView
+{
+    uint32_t           triangleCount;
+    uint32_t           nodeCount;        /* 8-wide nodes; node 0 is the root  */
+    uint32_t           nodeFormat;
+    uint32_t           nodeStrideBytes;  /* 256 (FP32) or 128 (FP16)          */
+    const void*        d_nodes;
+    /* Sorted triangles, 3 float4 each: (v0, primId bits) (v1, 0) (v2, 0). */
+    const float*       d_sortedTriangles;
+    const uint64_t*    d_mortonCodes;    /* n codes, in input order           */
+    const uint32_t*    d_sortedIndices;  /* n primitive ids, in Morton order  */
+    /* Binary Karras tree: n-1 internal nodes; child ids < n-1 are internal,
+     * child ids >= n-1 are leaves (leaf k = sorted triangle k - (n-1)).   */
+    const uint32_t*    d_binaryChildren; /* 2 per internal node               */
+    const float*       d_binaryBounds;   /* 6 floats per node, internal then leaves */
+    double             sceneCentroidMin[ 3 ];
+    double             sceneCentroidMax[ 3 ];
+} RgExtBvhView;
+
+RGAPI RgResult RGCONV rgExtBvhCreate( RgExtBvh* pOut );
+RGAPI RgResult RGCONV rgExtBvhDestroy( RgExtBvh bvh );
+RGAPI RgResult RGCONV rgExtBvhBuild( RgExtBvh                   bvh,
+                                     const float*               d_triangles,
+                                     uint32_t                   triangleCount,
+                                     const RgExtBvhBuildParams* pParams,
+                                     void*                      stream );
+RGAPI RgResult RGCONV rgExtBvhGetView( RgExtBvh bvh, RgExtBvhView* pOut );
+
+/* ------------------------------------------------------------- traversal */
+/* Replaces traceRayEXT (Source/Shaders/RaygenCommon.h:223,241,259,338):
+ * closest hit with [tmin, tmax], or any hit for shadow rays
+ * (gl_RayFlagsSkipClosestHitShaderEXT, RaygenCommon.h:337-344). */
+typedef struct RgExtRay
+{
+    float origin[ 3 ];
+    float tmin;
+    float direction[ 3 ];
+    float tmax;
+} RgExtRay;
+
+/* u, v: barycentric weights of v1 and v2 (hit = (1-u-v) v0 + u v1 + v v2,
+ * as Source/Shaders/HitInfo.inl:239-240).  primitiveId = RG_EXT_MISS and
+ * t = tmax on a miss.  For any-hit rays only primitiveId is meaningful. */
+typedef struct RgExtHit
+{
+    float    t;
+    float    u;
+    float    v;
+    uint32_t primitiveId;
+} RgExtHit;
+
+#define RG_EXT_MISS 0xFFFFFFFFu
+
+#define RG_EXT_TRACE_CLOSEST         0u
+#define RG_EXT_TRACE_ANY_HIT         1u /* stop at the first hit (occlusion) */
+#define RG_EXT_TRACE_CULL_FRONT_FACE 2u /* gl_RayFlagsCullFrontFacingTrianglesEXT, CCW = front */
+
+RGAPI RgResult RGCONV rgExtTraceRays( RgExtBvh        bvh,
+                                      const RgExtRay* d_rays,
+                                      uint32_t        rayCount,
+                                      RgExtHit*       d_hits,
+                                      uint32_t        flags,
+                                      void*           stream );
+
+/* -------------------------------------------------------------- tonemap */
+/* CmLuminanceHistogram.comp:41-86: 256-bin log2-luminance histogram of a
+ * width x height FP32 luminance image, accumulated (added) into d_histogram. */
+RGAPI RgResult RGCONV rgExtLuminanceHistogram( const float* d_luminance,
+                                               uint32_t     width,
+                                               uint32_t     height,
+                                               float        minLogLuminance,
+                                               float        maxLogLuminance,
+                                               uint32_t*    d_histogram,
+                                               void*        stream );
+
+/* CmLuminanceAvg.comp:39-90: weighted log average, exponential adaptation
+ * of *d_avgLuminance (read-modify-write) and clearing of d_histogram. */
+RGAPI RgResult RGCONV rgExtLuminanceAverage( uint32_t* d_histogram,
+                                             uint32_t  pixelCount,
+                                             float     minLogLuminance,
+                                             float     maxLogLuminance,
+                                             float     timeDelta,
+                                             RgBool32  stopEyeAdaptation,
+                                             float*    d_avgLuminance,
+                                             void*     stream );
+
+/* --------------------------------------------------------- framebuffers */
+/* The per-pixel images of the hot path (Source/Generated/
+ * GenerateShaderCommon.py:802-887), stored as planes in HBM.  Planes marked
+ * PREV are double-buffered and swapped by rgExtFramebuffersSwap
+ * (Source/Framebuffers.cpp:35-47).  Layouts: C = checkerboard W x H
+ * (Source/Shaders/ShaderCommonGLSLFunc.h:306-357), R = regular W x H,
+ * G = gradient strata ceil(W/3) x ceil(H/3).  Element types in the
+ * FP32 storage mode are given per plane. */
+typedef enum RgExtPlane
+{
+    RG_EXT_PLANE_NORMAL,                 /* C u32  encoded normal, PREV       */
+    RG_EXT_PLANE_METALLIC_ROUGHNESS,     /* C f32x2, PREV                     */
+    RG_EXT_PLANE_DEPTH_WORLD,            /* C f32 linear depth, PREV          */
+    RG_EXT_PLANE_DEPTH_GRAD,             /* C f32                             */
+    RG_EXT_PLANE_MOTION,                 /* C f32x4 (x, y, z, 0)              */
+    RG_EXT_PLANE_UNFILTERED_DIRECT,      /* C f32x4                           */
+    RG_EXT_PLANE_UNFILTERED_SPECULAR,    /* C f32x4                           */
+    RG_EXT_PLANE_UNFILTERED_INDIR,       /* C f32x4                           */
+    RG_EXT_PLANE_SURFACE_POSITION,       /* C f32x4, PREV                     */
+    RG_EXT_PLANE_VISIBILITY_BUFFER,      /* C u32x4, PREV                     */
+    RG_EXT_PLANE_VIEW_DIRECTION,         /* C f32x4 (dir, hit distance)       */
+    RG_EXT_PLANE_THROUGHPUT,             /* C f32x4                           */
+    RG_EXT_PLANE_IS_SKY,                 /* C u32                             */
+    RG_EXT_PLANE_ACCUM_HISTORY_LENGTH,   /* C f32x4, PREV                     */
+    RG_EXT_PLANE_DIFF_TEMPORARY,         /* C f32x4                           */
+    RG_EXT_PLANE_DIFF_ACCUM_COLOR,       /* C f32x4                           */
+    RG_EXT_PLANE_DIFF_ACCUM_MOMENTS,     /* C f32x2, PREV                     */
+    RG_EXT_PLANE_DIFF_COLOR_HISTORY,     /* C f32x4 (color, variance)         */
+    RG_EXT_PLANE_DIFF_PING,              /* C f32x4 (color, variance)         */
+    RG_EXT_PLANE_DIFF_PONG,              /* C f32x4 (color, variance)         */
+    RG_EXT_PLANE_SPEC_ACCUM_COLOR,       /* C f32x4, PREV                     */
+    RG_EXT_PLANE_SPEC_PING,              /* C f32x4                           */
+    RG_EXT_PLANE_SPEC_PONG,              /* C f32x4                           */
+    RG_EXT_PLANE_INDIR_ACCUM,            /* C f32x4, PREV                     */
+    RG_EXT_PLANE_INDIR_PING,             /* C f32x4                           */
+    RG_EXT_PLANE_INDIR_PONG,             /* C f32x4                           */
+    RG_EXT_PLANE_ATROUS_FILTERED_VARIANCE, /* C f32                           */
+    RG_EXT_PLANE_RESERVOIRS,             /* C u32x4, PREV                     */
+    RG_EXT_PLANE_RESERVOIRS_INITIAL,     /* C u32x4                           */
+    RG_EXT_PLANE_GRADIENT_INPUTS,        /* C f32x2, PREV                     */
+    RG_EXT_PLANE_ALBEDO,                 /* R f32x4                           */
+    RG_EXT_PLANE_HISTOGRAM_INPUT,        /* R f32                             */
+    RG_EXT_PLANE_PRE_FINAL,              /* R f32x4                           */
+    RG_EXT_PLANE_DIS_PING_GRADIENT,      /* G f32x4                           */
+    RG_EXT_PLANE_DIS_PONG_GRADIENT,      /* G f32x4                           */
+    RG_EXT_PLANE_DIS_GRADIENT_HISTORY,   /* G f32x4                           */
+    RG_EXT_PLANE_GRADIENT_PREV_PIX,      /* G u32                             */
+    RG_EXT_PLANE_COUNT
+} RgExtPlane;
+
+typedef struct RgExtFramebuffers_T* RgExtFramebuffers;
+
+RGAPI RgResult RGCONV rgExtFramebuffersCreate( uint32_t width, uint32_t height, RgExtFramebuffers* pOut );
+RGAPI RgResult RGCONV rgExtFramebuffersDestroy( RgExtFramebuffers fb );
+/* Current (prev = 0) or previous-frame (prev = 1) copy of a plane.  Planes
+ * without PREV return the same pointer for both. */
+RGAPI RgResult RGCONV rgExtFramebuffersGetPlane( RgExtFramebuffers fb,
+                                                 RgExtPlane        plane,
+                                                 RgBool32          prev,
+                                                 void**            d_pOut,
+                                                 size_t*           pBytes );
+RGAPI RgResult RGCONV rgExtFramebuffersSwap( RgExtFramebuffers fb );
+
+/* Per-frame constants of the hot path: the fields of ShGlobalUniform
+ * (Source/Generated/ShaderCommonC.h:209-304) that its kernels read.
+ * Matrices are column-major like the reference's. */
+typedef struct RgExtFrameParams
+{
+    uint32_t renderWidth;  /* must be even (Source/VulkanDevice.cpp:186-187) */
+    uint32_t renderHeight;
+    uint32_t frameId;
+    float    timeDelta;
+
+    float    view[ 16 ];
+    float    invView[ 16 ];
+    float    viewPrev[ 16 ];
+    float    projection[ 16 ];
+    float    invProjection[ 16 ];
+    float    projectionPrev[ 16 ];
+    float    cameraPosition[ 4 ];
+    float    cameraPositionPrev[ 4 ];
+
+    float    minLogLuminance;
+    float    maxLogLuminance;
+    uint32_t stopEyeAdaptation;
+    uint32_t antiFireflyEnabled;
+
+    float    jitterX;
+    float    jitterY;
+    float    primaryRayMinDist;
+    float    rayLength;
+
+    uint32_t rayCullBackFaces;
+    uint32_t lightCount;          /* regular (non-directional) lights         */
+    uint32_t lightCountPrev;
+    uint32_t directionalLightExists;
+
+    float    skyColorDefault[ 4 ];
+    float    skyColorMultiplier;
+    float    skyColorSaturation;
+    float    cellWorldSize;
+    uint32_t maxBounceShadowsLights;
+
+    float    gradientMultDiffuse;
+    float    gradientMultIndirect;
+    float    gradientMultSpecular;
+    uint32_t indirSecondBounce;
+
+    float    minRoughness;
+    float    polyLightSpotlightFactor;
+    uint32_t atrousIterations;    /* 4 as the reference; 5 adds step 16       */
+    uint32_t restirInitialCandidates; /* 8 as the reference                   */
+
+    uint32_t restirTemporalSamples;   /* 1 as the reference                   */
+    uint32_t restirSpatialSamples;    /* 8 as the reference                   */
+    uint32_t debugShowFlags;
+    uint32_t _pad0;
+} RgExtFrameParams;
+
+/* Fills an RgExtFrameParams with the reference defaults
+ * (Source/DrawFrameInfo.h:34-214) for a width x height render. */
+RGAPI RgResult RGCONV rgExtFrameParamsDefaults( uint32_t width, uint32_t height, RgExtFrameParams* pOut );
+
+/* A-SVGF passes, in the order of Denoiser::Denoise (Source/Denoiser.cpp:64-255). */
+/* CmASVGFGradientAtrous.comp:42-105, iteration 0..3 (steps 1,2,4,8). */
+RGAPI RgResult RGCONV rgExtDenoiseGradientAtrous( RgExtFramebuffers fb, const RgExtFrameParams* p, uint32_t iteration, void* stream );
+/* CmSVGFTemporalAccumulation.comp:101-385. */
+RGAPI RgResult RGCONV rgExtDenoiseTemporal( RgExtFramebuffers fb, const RgExtFrameParams* p, void* stream );
+/* CmAntiFirefly.comp:96-159. */
+RGAPI RgResult RGCONV rgExtDenoiseAntiFirefly( RgExtFramebuffers fb, const RgExtFrameParams* p, void* stream );
+/* CmSVGFEstimateVariance.comp:129-206. */
+RGAPI RgResult RGCONV rgExtDenoiseVariance( RgExtFramebuffers fb, const RgExtFrameParams* p, void* stream );
+/* CmSVGFAtrous_Iter0.comp:196-335 (iteration 0) and CmSVGFAtrous.comp:71-349
+ * (iterations 1..3; the last one also composites into PRE_FINAL and
+ * HISTOGRAM_INPUT).  Iteration 4 (step 16) exists only with
+ * atrousIterations = 5. */
+RGAPI RgResult RGCONV rgExtDenoiseAtrous( RgExtFramebuffers fb, const RgExtFrameParams* p, uint32_t iteration, void* stream );
+/* The whole chain as Denoiser::Denoise runs it. */
+RGAPI RgResult RGCONV rgExtDenoise( RgExtFramebuffers fb, const RgExtFrameParams* p, void* stream );
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RTGL1_EXT_H_ */

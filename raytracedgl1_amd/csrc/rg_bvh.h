@@ -1,0 +1,98 @@
+// LBVH object shared by the build (rg_lbvh.hip) and traversal (rg_trace.hip).
+#pragma once
+
+#include "rg_internal.h"
+
+namespace rg
+{
+
+// 8-wide node with FP32 child boxes, 256 bytes (SoA over the 8 children so
+// one child's slab test reads 6 floats at a fixed stride).
+struct alignas( 256 ) BvhNode8
+{
+    float    lo_x[ 8 ];
+    float    hi_x[ 8 ];
+    float    lo_y[ 8 ];
+    float    hi_y[ 8 ];
+    float    lo_z[ 8 ];
+    float    hi_z[ 8 ];
+    uint32_t child[ 8 ];
+    uint32_t pad[ 8 ];
+};
+static_assert( sizeof( BvhNode8 ) == 256, "node size" );
+
+// 8-wide node with FP16 child boxes rounded outward, 128 bytes.
+struct alignas( 128 ) BvhNode8H
+{
+    uint16_t lo_x[ 8 ];
+    uint16_t hi_x[ 8 ];
+    uint16_t lo_y[ 8 ];
+    uint16_t hi_y[ 8 ];
+    uint16_t lo_z[ 8 ];
+    uint16_t hi_z[ 8 ];
+    uint32_t child[ 8 ];
+};
+static_assert( sizeof( BvhNode8H ) == 128, "node size" );
+
+constexpr uint32_t BVH_MAX_LEVELS = 128;
+
+struct BvhObject
+{
+    uint32_t n          = 0;
+    uint32_t nodeCount  = 0;
+    uint32_t nodeFormat = RG_EXT_BVH_NODE_FP32;
+    uint32_t leafMax    = 3;
+    double   sceneMin[ 3 ] = { 0, 0, 0 };
+    double   sceneMax[ 3 ] = { 0, 0, 0 };
+
+    DeviceBuffer sortedTris;   // n * 3 float4
+    DeviceBuffer codes;        // n u64, input order
+    DeviceBuffer keysA, keysB; // n u64
+    DeviceBuffer valsA, valsB; // n u32
+    DeviceBuffer hist;         // 256 * radix blocks u32
+    DeviceBuffer scanScratch;
+    DeviceBuffer boundsPartials; // 6 doubles per block + 6 final
+    DeviceBuffer children;     // (n-1) * 2 u32
+    DeviceBuffer parents;      // (2n-1) u32
+    DeviceBuffer rangeFirst;   // (n-1) u32
+    DeviceBuffer rangeLast;    // (n-1) u32
+    DeviceBuffer nodeBounds;   // (2n-1) * 6 f32
+    DeviceBuffer refitCounter; // (n-1) u32
+    DeviceBuffer expansion;    // max(1,n-1) * 8 u32
+    DeviceBuffer isNode8;      // max(1,n-1) u32
+    DeviceBuffer node8Index;   // max(1,n-1) u32
+    DeviceBuffer frontier[ 2 ];
+    DeviceBuffer levelCounts;  // BVH_MAX_LEVELS + 2 u32 (last two: node count, error flag)
+    DeviceBuffer nodes;        // nodeCount * stride
+    uint32_t*    hostCounts = nullptr; // pinned
+
+    // Index of the sorted-order permutation after the sort (keysA/valsA or B).
+    const uint32_t* sortedIdx = nullptr;
+    const uint64_t* sortedKeys = nullptr;
+
+    ~BvhObject()
+    {
+        if( hostCounts )
+        {
+            ( void )hipHostFree( hostCounts );
+        }
+    }
+
+    uint32_t NodeStride() const { return nodeFormat == RG_EXT_BVH_NODE_FP16 ? 128u : 256u; }
+};
+
+void BuildBvh( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtBvhBuildParams& params, hipStream_t stream );
+
+// Per-caller traversal scratch: the ray counter, an overflow-flag word and
+// the global overflow stacks of the persistent grid.  Launches that share a
+// context must not overlap on the device.
+struct TraceContext
+{
+    DeviceBuffer counter;
+    DeviceBuffer overflow;
+    uint32_t     gridBlocks = 0;
+};
+
+void TraceRays( const BvhObject& bvh, TraceContext& ctx, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags, hipStream_t stream );
+
+} // namespace rg

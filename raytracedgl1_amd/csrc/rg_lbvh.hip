@@ -1,0 +1,689 @@
+// Linear BVH build for gfx950: FP64 bounds and Morton quantisation, stable
+// LSD radix sort with wave-ballot ranking, Karras (2012) hierarchy, bottom-up
+// refit with agent-scope acquire/release counters, and a deterministic
+// collapse into 8-wide nodes.
+//
+// This replaces the driver-side acceleration-structure build of the reference
+// (ASBuilder::BuildBottomLevel / BuildTopLevel, Source/ASBuilder.cpp:141-206);
+// the reference has no BVH code of its own, so the build follows the
+// published algorithms named above.
+#include "rg_bvh.h"
+#include "rg_device_math.h"
+#include "rg_scan.h"
+
+#include <cfloat>
+#include <cmath>
+
+namespace rg
+{
+namespace
+{
+
+constexpr int      THREADS     = 256;
+constexpr int      RS_ITEMS    = 16;
+constexpr int      RS_TILE     = THREADS * RS_ITEMS;
+constexpr uint32_t MORTON_BITS = 21;
+constexpr double   MORTON_CELLS = 2097152.0; // 2^21
+
+// ------------------------------------------------------------- bounds
+
+__device__ __forceinline__ void LoadTri( const float* tris, uint32_t i, v3& a, v3& b, v3& c )
+{
+    const float* t = tris + size_t( i ) * 9;
+    a              = mk3( t[ 0 ], t[ 1 ], t[ 2 ] );
+    b              = mk3( t[ 3 ], t[ 4 ], t[ 5 ] );
+    c              = mk3( t[ 6 ], t[ 7 ], t[ 8 ] );
+}
+
+// Centroid in FP64: ((v0 + v1) + v2) / 3 per axis.
+__device__ __forceinline__ void Centroid( const float* tris, uint32_t i, double c[ 3 ] )
+{
+    const float* t = tris + size_t( i ) * 9;
+    for( int k = 0; k < 3; k++ )
+    {
+        c[ k ] = ( ( double( t[ k ] ) + double( t[ 3 + k ] ) ) + double( t[ 6 + k ] ) ) / 3.0;
+    }
+}
+
+__global__ __launch_bounds__( THREADS ) void k_centroid_bounds( const float* tris, uint32_t n, double* partials )
+{
+    __shared__ double s[ 6 ][ THREADS ];
+    double            lo[ 3 ] = { DBL_MAX, DBL_MAX, DBL_MAX };
+    double            hi[ 3 ] = { -DBL_MAX, -DBL_MAX, -DBL_MAX };
+    for( uint32_t i = blockIdx.x * THREADS + threadIdx.x; i < n; i += gridDim.x * THREADS )
+    {
+        double c[ 3 ];
+        Centroid( tris, i, c );
+        for( int k = 0; k < 3; k++ )
+        {
+            lo[ k ] = fmin( lo[ k ], c[ k ] );
+            hi[ k ] = fmax( hi[ k ], c[ k ] );
+        }
+    }
+    for( int k = 0; k < 3; k++ )
+    {
+        s[ k ][ threadIdx.x ]     = lo[ k ];
+        s[ 3 + k ][ threadIdx.x ] = hi[ k ];
+    }
+    __syncthreads();
+    for( int stride = THREADS / 2; stride > 0; stride >>= 1 )
+    {
+        if( int( threadIdx.x ) < stride )
+        {
+            for( int k = 0; k < 3; k++ )
+            {
+                s[ k ][ threadIdx.x ]     = fmin( s[ k ][ threadIdx.x ], s[ k ][ threadIdx.x + stride ] );
+                s[ 3 + k ][ threadIdx.x ] = fmax( s[ 3 + k ][ threadIdx.x ], s[ 3 + k ][ threadIdx.x + stride ] );
+            }
+        }
+        __syncthreads();
+    }
+    if( threadIdx.x < 6 )
+    {
+        partials[ blockIdx.x * 6 + threadIdx.x ] = s[ threadIdx.x ][ 0 ];
+    }
+}
+
+__global__ void k_bounds_final( const double* partials, uint32_t blocks, double* out )
+{
+    if( threadIdx.x < 6 )
+    {
+        const bool isMin = threadIdx.x < 3;
+        double     v     = isMin ? DBL_MAX : -DBL_MAX;
+        for( uint32_t b = 0; b < blocks; b++ )
+        {
+            double p = partials[ b * 6 + threadIdx.x ];
+            v        = isMin ? fmin( v, p ) : fmax( v, p );
+        }
+        out[ threadIdx.x ] = v;
+    }
+}
+
+// ------------------------------------------------------------- morton
+
+__device__ __forceinline__ uint64_t ExpandBits21( uint32_t v )
+{
+    uint64_t x = v & 0x1fffffu;
+    x          = ( x | x << 32 ) & 0x1f00000000ffffull;
+    x          = ( x | x << 16 ) & 0x1f0000ff0000ffull;
+    x          = ( x | x << 8 ) & 0x100f00f00f00f00full;
+    x          = ( x | x << 4 ) & 0x10c30c30c30c30c3ull;
+    x          = ( x | x << 2 ) & 0x1249249249249249ull;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t Quantize21( double c, double lo, double hi )
+{
+    const double extent = hi - lo;
+    if( !( extent > 0.0 ) )
+    {
+        return 0;
+    }
+    const double f = floor( ( ( c - lo ) / extent ) * MORTON_CELLS );
+    return f >= MORTON_CELLS - 1.0 ? ( 1u << MORTON_BITS ) - 1u : ( f <= 0.0 ? 0u : uint32_t( f ) );
+}
+
+__global__ __launch_bounds__( THREADS ) void k_morton( const float* tris, uint32_t n, const double* bounds, uint64_t* codes, uint64_t* keys, uint32_t* vals )
+{
+    uint32_t i = blockIdx.x * THREADS + threadIdx.x;
+    if( i >= n )
+    {
+        return;
+    }
+    double c[ 3 ];
+    Centroid( tris, i, c );
+    uint32_t q[ 3 ];
+    for( int k = 0; k < 3; k++ )
+    {
+        q[ k ] = Quantize21( c[ k ], bounds[ k ], bounds[ 3 + k ] );
+    }
+    uint64_t code = ( ExpandBits21( q[ 0 ] ) << 2 ) | ( ExpandBits21( q[ 1 ] ) << 1 ) | ExpandBits21( q[ 2 ] );
+    codes[ i ]    = code;
+    keys[ i ]     = code;
+    vals[ i ]     = i;
+}
+
+// --------------------------------------------------------- radix sort
+
+__global__ __launch_bounds__( THREADS ) void k_radix_hist( const uint64_t* keys, uint32_t n, int shift, uint32_t* hist, uint32_t numBlocks )
+{
+    __shared__ uint32_t cnt[ 256 ];
+    cnt[ threadIdx.x ] = 0;
+    __syncthreads();
+    const uint64_t base = uint64_t( blockIdx.x ) * RS_TILE;
+    for( int k = 0; k < RS_ITEMS; k++ )
+    {
+        uint64_t i = base + uint64_t( k ) * THREADS + threadIdx.x;
+        if( i < n )
+        {
+            atomicAdd( &cnt[ uint32_t( keys[ i ] >> shift ) & 255u ], 1u );
+        }
+    }
+    __syncthreads();
+    hist[ threadIdx.x * numBlocks + blockIdx.x ] = cnt[ threadIdx.x ];
+}
+
+// Stable scatter: keys of a tile are ranked in (round, thread) order, which is
+// their input order; equal digits are ranked within a wave by ballot matching.
+__global__ __launch_bounds__( THREADS ) void k_radix_scatter( const uint64_t* keysIn,
+                                                              const uint32_t* valsIn,
+                                                              uint64_t*       keysOut,
+                                                              uint32_t*       valsOut,
+                                                              uint32_t        n,
+                                                              int             shift,
+                                                              const uint32_t* offsets,
+                                                              uint32_t        numBlocks )
+{
+    constexpr int       WAVES = THREADS / 64;
+    __shared__ uint32_t s_base[ 256 ];
+    __shared__ uint32_t s_waveCnt[ WAVES ][ 256 ];
+
+    const int      tid  = threadIdx.x;
+    const int      lane = tid & 63;
+    const int      wave = tid >> 6;
+    const uint64_t ltMask = ( lane == 0 ) ? 0ull : ( ~0ull >> ( 64 - lane ) );
+
+    s_base[ tid ] = offsets[ tid * numBlocks + blockIdx.x ];
+    for( int w = 0; w < WAVES; w++ )
+    {
+        s_waveCnt[ w ][ tid ] = 0;
+    }
+    __syncthreads();
+
+    const uint64_t base = uint64_t( blockIdx.x ) * RS_TILE;
+    for( int round = 0; round < RS_ITEMS; round++ )
+    {
+        const uint64_t i     = base + uint64_t( round ) * THREADS + tid;
+        const bool     valid = i < n;
+        uint64_t       key   = valid ? keysIn[ i ] : 0ull;
+        uint32_t       val   = valid ? valsIn[ i ] : 0u;
+        uint32_t       digit = uint32_t( key >> shift ) & 255u;
+
+        uint64_t peers = __ballot( valid );
+#pragma unroll
+        for( int b = 0; b < 8; b++ )
+        {
+            const bool     bit = ( digit >> b ) & 1u;
+            const uint64_t m   = __ballot( bit );
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t rank = __popcll( peers & ltMask );
+        if( valid && rank == 0 )
+        {
+            s_waveCnt[ wave ][ digit ] = __popcll( peers );
+        }
+        __syncthreads();
+        if( valid )
+        {
+            uint32_t pos = s_base[ digit ] + rank;
+            for( int w = 0; w < wave; w++ )
+            {
+                pos += s_waveCnt[ w ][ digit ];
+            }
+            keysOut[ pos ] = key;
+            valsOut[ pos ] = val;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for( int w = 0; w < WAVES; w++ )
+        {
+            add += s_waveCnt[ w ][ tid ];
+            s_waveCnt[ w ][ tid ] = 0;
+        }
+        s_base[ tid ] += add;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------- Karras
+
+__device__ __forceinline__ int Delta( const uint64_t* keys, int64_t n, int64_t i, int64_t j )
+{
+    if( j < 0 || j >= n )
+    {
+        return -1;
+    }
+    const uint64_t a = keys[ i ], b = keys[ j ];
+    if( a == b )
+    {
+        return 64 + __clz( uint32_t( i ) ^ uint32_t( j ) );
+    }
+    return __clzll( a ^ b );
+}
+
+// Internal node i of n-1; child ids >= n-1 denote leaves (sorted position id-(n-1)).
+__global__ __launch_bounds__( THREADS ) void k_karras( const uint64_t* keys,
+                                                       uint32_t        n,
+                                                       uint32_t*       children,
+                                                       uint32_t*       parents,
+                                                       uint32_t*       rangeFirst,
+                                                       uint32_t*       rangeLast )
+{
+    const int64_t i = int64_t( blockIdx.x ) * THREADS + threadIdx.x;
+    const int64_t N = n;
+    if( i >= N - 1 )
+    {
+        return;
+    }
+    const int d    = ( Delta( keys, N, i, i + 1 ) - Delta( keys, N, i, i - 1 ) ) >= 0 ? 1 : -1;
+    const int dmin = Delta( keys, N, i, i - d );
+
+    int64_t lmax = 2;
+    while( Delta( keys, N, i, i + lmax * d ) > dmin )
+    {
+        lmax <<= 1;
+    }
+    int64_t l = 0;
+    for( int64_t t = lmax >> 1; t >= 1; t >>= 1 )
+    {
+        if( Delta( keys, N, i, i + ( l + t ) * d ) > dmin )
+        {
+            l += t;
+        }
+    }
+    const int64_t j     = i + l * d;
+    const int     dnode = Delta( keys, N, i, j );
+
+    int64_t s = 0, t = l;
+    do
+    {
+        t = ( t + 1 ) >> 1;
+        if( Delta( keys, N, i, i + ( s + t ) * d ) > dnode )
+        {
+            s += t;
+        }
+    } while( t > 1 );
+
+    const int64_t gamma = i + s * d + ( d < 0 ? d : 0 );
+    const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+
+    const uint32_t left  = uint32_t( lo == gamma ? ( N - 1 ) + gamma : gamma );
+    const uint32_t right = uint32_t( hi == gamma + 1 ? ( N - 1 ) + gamma + 1 : gamma + 1 );
+
+    children[ i * 2 + 0 ] = left;
+    children[ i * 2 + 1 ] = right;
+    parents[ left ]       = uint32_t( i );
+    parents[ right ]      = uint32_t( i );
+    rangeFirst[ i ]       = uint32_t( lo );
+    rangeLast[ i ]        = uint32_t( hi );
+    if( i == 0 )
+    {
+        parents[ 0 ] = 0xFFFFFFFFu;
+    }
+}
+
+// --------------------------------------------------------------- refit
+
+__device__ __forceinline__ void StoreBox( float* b, v3 lo, v3 hi )
+{
+    b[ 0 ] = lo.x;
+    b[ 1 ] = lo.y;
+    b[ 2 ] = lo.z;
+    b[ 3 ] = hi.x;
+    b[ 4 ] = hi.y;
+    b[ 5 ] = hi.z;
+}
+
+__global__ __launch_bounds__( THREADS ) void k_refit( const float*    tris,
+                                                      const uint32_t* sortedIdx,
+                                                      uint32_t        n,
+                                                      const uint32_t* children,
+                                                      const uint32_t* parents,
+                                                      float*          bounds,
+                                                      uint32_t*       counters )
+{
+    const uint32_t k = blockIdx.x * THREADS + threadIdx.x;
+    if( k >= n )
+    {
+        return;
+    }
+    v3 a, b, c;
+    LoadTri( tris, sortedIdx[ k ], a, b, c );
+    const uint32_t leafId = ( n - 1 ) + k;
+    StoreBox( bounds + size_t( leafId ) * 6, vmin( vmin( a, b ), c ), vmax( vmax( a, b ), c ) );
+    if( n == 1 )
+    {
+        return;
+    }
+
+    uint32_t node = parents[ leafId ];
+    // A Karras tree is at most 96 levels deep (63-bit codes + 32-bit index
+    // tie-break); the cap only guarantees termination.
+    for( int guard = 0; node != 0xFFFFFFFFu && guard < 128; guard++ )
+    {
+        // The first child to arrive stops; the second one (which sees the
+        // first one's box through the acquire) builds the union.
+        const uint32_t prev = __hip_atomic_fetch_add( &counters[ node ], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT );
+        if( prev == 0 )
+        {
+            return;
+        }
+        const float* l = bounds + size_t( children[ node * 2 + 0 ] ) * 6;
+        const float* r = bounds + size_t( children[ node * 2 + 1 ] ) * 6;
+        v3           lo = vmin( mk3( l[ 0 ], l[ 1 ], l[ 2 ] ), mk3( r[ 0 ], r[ 1 ], r[ 2 ] ) );
+        v3           hi = vmax( mk3( l[ 3 ], l[ 4 ], l[ 5 ] ), mk3( r[ 3 ], r[ 4 ], r[ 5 ] ) );
+        StoreBox( bounds + size_t( node ) * 6, lo, hi );
+        node = parents[ node ];
+    }
+}
+
+// ------------------------------------------------------------ collapse
+
+__device__ __forceinline__ float HalfArea( const float* b )
+{
+    const float dx = b[ 3 ] - b[ 0 ], dy = b[ 4 ] - b[ 1 ], dz = b[ 5 ] - b[ 2 ];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+__device__ __forceinline__ uint32_t NodeCount( uint32_t id, uint32_t n, const uint32_t* rf, const uint32_t* rl )
+{
+    return id >= n - 1 ? 1u : rl[ id ] - rf[ id ] + 1u;
+}
+
+// Expands binary node b into up to 8 children by repeatedly opening the
+// child with the largest surface area (first one on ties) among those that
+// are internal and hold more than leafMax triangles.
+__global__ __launch_bounds__( THREADS ) void k_collapse_level( const uint32_t* frontierIn,
+                                                               const uint32_t* countIn,
+                                                               uint32_t*       frontierOut,
+                                                               uint32_t*       countOut,
+                                                               uint32_t        n,
+                                                               uint32_t        leafMax,
+                                                               const uint32_t* children,
+                                                               const uint32_t* rf,
+                                                               const uint32_t* rl,
+                                                               const float*    bounds,
+                                                               uint32_t*       expansion,
+                                                               uint32_t*       isNode8 )
+{
+    const uint32_t count = *countIn;
+    for( uint32_t t = blockIdx.x * THREADS + threadIdx.x; t < count; t += gridDim.x * THREADS )
+    {
+        const uint32_t b = frontierIn[ t ];
+        uint32_t       ch[ 8 ];
+        int            nch = 0;
+        if( b >= n - 1 || NodeCount( b, n, rf, rl ) <= leafMax )
+        {
+            ch[ 0 ] = b; // a root that is small enough is a single leaf
+            nch     = 1;
+        }
+        else
+        {
+            ch[ 0 ] = children[ b * 2 + 0 ];
+            ch[ 1 ] = children[ b * 2 + 1 ];
+            nch     = 2;
+            while( nch < 8 )
+            {
+                int   best     = -1;
+                float bestArea = -1.0f;
+                for( int k = 0; k < nch; k++ )
+                {
+                    const uint32_t c = ch[ k ];
+                    if( c < n - 1 && NodeCount( c, n, rf, rl ) > leafMax )
+                    {
+                        const float a = HalfArea( bounds + size_t( c ) * 6 );
+                        if( a > bestArea )
+                        {
+                            bestArea = a;
+                            best     = k;
+                        }
+                    }
+                }
+                if( best < 0 )
+                {
+                    break;
+                }
+                const uint32_t c = ch[ best ];
+                for( int k = nch; k > best + 1; k-- )
+                {
+                    ch[ k ] = ch[ k - 1 ];
+                }
+                ch[ best ]     = children[ c * 2 + 0 ];
+                ch[ best + 1 ] = children[ c * 2 + 1 ];
+                nch++;
+            }
+        }
+        const uint32_t slot = b < n - 1 ? b : 0;
+        for( int k = 0; k < 8; k++ )
+        {
+            expansion[ size_t( slot ) * 8 + k ] = k < nch ? ch[ k ] : 0xFFFFFFFFu;
+        }
+        isNode8[ slot ] = 1u;
+        for( int k = 0; k < nch; k++ )
+        {
+            const uint32_t c = ch[ k ];
+            if( c < n - 1 && NodeCount( c, n, rf, rl ) > leafMax && !( nch == 1 ) )
+            {
+                frontierOut[ atomicAdd( countOut, 1u ) ] = c;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t LeafRef( uint32_t first, uint32_t count )
+{
+    return RG_EXT_BVH_CHILD_LEAF_BIT | ( first << 3 ) | ( count - 1u );
+}
+
+__global__ __launch_bounds__( THREADS ) void k_write_nodes( uint32_t        slots,
+                                                            uint32_t        n,
+                                                            uint32_t        leafMax,
+                                                            const uint32_t* isNode8,
+                                                            const uint32_t* node8Index,
+                                                            const uint32_t* expansion,
+                                                            const uint32_t* rf,
+                                                            const uint32_t* rl,
+                                                            const float*    bounds,
+                                                            BvhNode8*       nodes )
+{
+    const uint32_t b = blockIdx.x * THREADS + threadIdx.x;
+    if( b >= slots || !isNode8[ b ] )
+    {
+        return;
+    }
+    BvhNode8& node = nodes[ node8Index[ b ] ];
+    for( int k = 0; k < 8; k++ )
+    {
+        const uint32_t c = expansion[ size_t( b ) * 8 + k ];
+        if( c == 0xFFFFFFFFu )
+        {
+            node.lo_x[ k ] = node.lo_y[ k ] = node.lo_z[ k ] = INFINITY;
+            node.hi_x[ k ] = node.hi_y[ k ] = node.hi_z[ k ] = -INFINITY;
+            node.child[ k ] = RG_EXT_BVH_CHILD_EMPTY;
+            node.pad[ k ]   = 0;
+            continue;
+        }
+        const float* bb = bounds + size_t( c ) * 6;
+        node.lo_x[ k ]  = bb[ 0 ];
+        node.lo_y[ k ]  = bb[ 1 ];
+        node.lo_z[ k ]  = bb[ 2 ];
+        node.hi_x[ k ]  = bb[ 3 ];
+        node.hi_y[ k ]  = bb[ 4 ];
+        node.hi_z[ k ]  = bb[ 5 ];
+        node.pad[ k ]   = 0;
+        if( c >= n - 1 )
+        {
+            node.child[ k ] = LeafRef( c - ( n - 1 ), 1 );
+        }
+        else if( NodeCount( c, n, rf, rl ) <= leafMax || isNode8[ c ] == 0u || c == b )
+        {
+            node.child[ k ] = LeafRef( rf[ c ], rl[ c ] - rf[ c ] + 1u );
+        }
+        else
+        {
+            node.child[ k ] = node8Index[ c ];
+        }
+    }
+}
+
+__global__ __launch_bounds__( THREADS ) void k_sorted_tris( const float* tris, const uint32_t* sortedIdx, uint32_t n, float4* out )
+{
+    const uint32_t k = blockIdx.x * THREADS + threadIdx.x;
+    if( k >= n )
+    {
+        return;
+    }
+    const uint32_t id = sortedIdx[ k ];
+    v3             a, b, c;
+    LoadTri( tris, id, a, b, c );
+    out[ size_t( k ) * 3 + 0 ] = make_float4( a.x, a.y, a.z, __uint_as_float( id ) );
+    out[ size_t( k ) * 3 + 1 ] = make_float4( b.x, b.y, b.z, 0.0f );
+    out[ size_t( k ) * 3 + 2 ] = make_float4( c.x, c.y, c.z, 0.0f );
+}
+
+__global__ void k_seed_frontier( uint32_t* frontier, uint32_t* counts )
+{
+    frontier[ 0 ] = 0;
+    counts[ 0 ]   = 1;
+}
+
+} // namespace
+
+void BuildBvh( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtBvhBuildParams& params, hipStream_t stream )
+{
+    RG_REQUIRE( n > 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "triangle count must be > 0" );
+    RG_REQUIRE( n < ( 1u << 28 ), RG_RESULT_WRONG_FUNCTION_ARGUMENT, "triangle count must be < 2^28" );
+    RG_REQUIRE( params.maxLeafSize >= 1 && params.maxLeafSize <= 8, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "maxLeafSize must be in [1, 8]" );
+    RG_REQUIRE( params.nodeFormat == RG_EXT_BVH_NODE_FP32, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "only FP32 nodes are built by this version" );
+
+    bvh.n          = n;
+    bvh.leafMax    = params.maxLeafSize;
+    bvh.nodeFormat = params.nodeFormat;
+
+    const uint32_t internal   = n - 1;
+    const uint32_t slots      = internal > 0 ? internal : 1;
+    const uint32_t rsBlocks   = DivUp( n, RS_TILE );
+    const uint32_t bndBlocks  = DivUp( n, THREADS ) < 1024 ? DivUp( n, THREADS ) : 1024;
+    const uint32_t histCount  = 256 * rsBlocks;
+    const uint32_t scanElems  = histCount > slots ? histCount : slots;
+
+    bvh.sortedTris.Reserve( size_t( n ) * 48 );
+    bvh.codes.Reserve( size_t( n ) * 8 );
+    bvh.keysA.Reserve( size_t( n ) * 8 );
+    bvh.keysB.Reserve( size_t( n ) * 8 );
+    bvh.valsA.Reserve( size_t( n ) * 4 );
+    bvh.valsB.Reserve( size_t( n ) * 4 );
+    bvh.hist.Reserve( size_t( histCount ) * 4 );
+    bvh.scanScratch.Reserve( ScanScratchBytes( scanElems ) );
+    bvh.boundsPartials.Reserve( size_t( bndBlocks + 1 ) * 6 * 8 );
+    bvh.children.Reserve( size_t( slots ) * 8 );
+    bvh.parents.Reserve( size_t( 2 * n - 1 ) * 4 );
+    bvh.rangeFirst.Reserve( size_t( slots ) * 4 );
+    bvh.rangeLast.Reserve( size_t( slots ) * 4 );
+    bvh.nodeBounds.Reserve( size_t( 2 * n - 1 ) * 24 );
+    bvh.refitCounter.Reserve( size_t( slots ) * 4 );
+    bvh.expansion.Reserve( size_t( slots ) * 32 );
+    bvh.isNode8.Reserve( size_t( slots ) * 4 );
+    bvh.node8Index.Reserve( size_t( slots ) * 4 );
+    bvh.frontier[ 0 ].Reserve( size_t( slots ) * 4 );
+    bvh.frontier[ 1 ].Reserve( size_t( slots ) * 4 );
+    bvh.levelCounts.Reserve( ( BVH_MAX_LEVELS + 2 ) * 4 );
+    if( !bvh.hostCounts )
+    {
+        RG_HIP_CHECK( hipHostMalloc( reinterpret_cast< void** >( &bvh.hostCounts ), ( BVH_MAX_LEVELS + 2 ) * 4, hipHostMallocDefault ) );
+    }
+
+    double* partials    = bvh.boundsPartials.as< double >();
+    double* sceneBounds = partials + size_t( bndBlocks ) * 6;
+
+    // 1. FP64 centroid bounds.
+    k_centroid_bounds<<< bndBlocks, THREADS, 0, stream >>>( d_tris, n, partials );
+    CheckLaunch( "k_centroid_bounds" );
+    k_bounds_final<<< 1, 64, 0, stream >>>( partials, bndBlocks, sceneBounds );
+    CheckLaunch( "k_bounds_final" );
+
+    // 2. Morton codes (input order) and the identity permutation.
+    k_morton<<< DivUp( n, THREADS ), THREADS, 0, stream >>>( d_tris, n, sceneBounds, bvh.codes.as< uint64_t >(), bvh.keysA.as< uint64_t >(), bvh.valsA.as< uint32_t >() );
+    CheckLaunch( "k_morton" );
+
+    // 3. Stable LSD radix sort, 8 passes of 8 bits.
+    uint64_t* kin  = bvh.keysA.as< uint64_t >();
+    uint64_t* kout = bvh.keysB.as< uint64_t >();
+    uint32_t* vin  = bvh.valsA.as< uint32_t >();
+    uint32_t* vout = bvh.valsB.as< uint32_t >();
+    for( int pass = 0; pass < 8; pass++ )
+    {
+        const int shift = pass * 8;
+        k_radix_hist<<< rsBlocks, THREADS, 0, stream >>>( kin, n, shift, bvh.hist.as< uint32_t >(), rsBlocks );
+        CheckLaunch( "k_radix_hist" );
+        ExclusiveScan( bvh.hist.as< uint32_t >(), histCount, bvh.hist.as< uint32_t >(), nullptr, bvh.scanScratch.as< uint32_t >(), stream );
+        k_radix_scatter<<< rsBlocks, THREADS, 0, stream >>>( kin, vin, kout, vout, n, shift, bvh.hist.as< uint32_t >(), rsBlocks );
+        CheckLaunch( "k_radix_scatter" );
+        std::swap( kin, kout );
+        std::swap( vin, vout );
+    }
+    bvh.sortedKeys = kin;
+    bvh.sortedIdx  = vin;
+
+    // 4. Karras hierarchy.
+    uint32_t* children = bvh.children.as< uint32_t >();
+    uint32_t* parents  = bvh.parents.as< uint32_t >();
+    uint32_t* rf       = bvh.rangeFirst.as< uint32_t >();
+    uint32_t* rl       = bvh.rangeLast.as< uint32_t >();
+    if( internal > 0 )
+    {
+        k_karras<<< DivUp( internal, THREADS ), THREADS, 0, stream >>>( kin, n, children, parents, rf, rl );
+        CheckLaunch( "k_karras" );
+    }
+
+    // 5. Refit.
+    float* bounds = bvh.nodeBounds.as< float >();
+    RG_HIP_CHECK( hipMemsetAsync( bvh.refitCounter.ptr, 0, size_t( slots ) * 4, stream ) );
+    k_refit<<< DivUp( n, THREADS ), THREADS, 0, stream >>>( d_tris, vin, n, children, parents, bounds, bvh.refitCounter.as< uint32_t >() );
+    CheckLaunch( "k_refit" );
+
+    // 6. Collapse to 8-wide nodes, level by level.
+    uint32_t* counts = bvh.levelCounts.as< uint32_t >();
+    RG_HIP_CHECK( hipMemsetAsync( counts, 0, ( BVH_MAX_LEVELS + 2 ) * 4, stream ) );
+    RG_HIP_CHECK( hipMemsetAsync( bvh.isNode8.ptr, 0, size_t( slots ) * 4, stream ) );
+    k_seed_frontier<<< 1, 1, 0, stream >>>( bvh.frontier[ 0 ].as< uint32_t >(), counts );
+    CheckLaunch( "k_seed_frontier" );
+
+    const uint32_t maxFrontier   = n / ( bvh.leafMax + 1 ) + 1;
+    const uint32_t collapseGrid  = DivUp( maxFrontier, THREADS ) < 4096 ? DivUp( maxFrontier, THREADS ) : 4096;
+    uint32_t       level         = 0;
+    bool           done          = false;
+    while( !done )
+    {
+        constexpr uint32_t BATCH = 8;
+        for( uint32_t k = 0; k < BATCH; k++, level++ )
+        {
+            RG_REQUIRE( level + 1 < BVH_MAX_LEVELS, RG_RESULT_INTERNAL_ERROR, "BVH deeper than the level limit" );
+            k_collapse_level<<< collapseGrid, THREADS, 0, stream >>>( bvh.frontier[ level & 1 ].as< uint32_t >(),
+                                                                      counts + level,
+                                                                      bvh.frontier[ ( level + 1 ) & 1 ].as< uint32_t >(),
+                                                                      counts + level + 1,
+                                                                      n,
+                                                                      bvh.leafMax,
+                                                                      children,
+                                                                      rf,
+                                                                      rl,
+                                                                      bounds,
+                                                                      bvh.expansion.as< uint32_t >(),
+                                                                      bvh.isNode8.as< uint32_t >() );
+            CheckLaunch( "k_collapse_level" );
+        }
+        RG_HIP_CHECK( hipMemcpyAsync( bvh.hostCounts, counts + level, 4, hipMemcpyDeviceToHost, stream ) );
+        RG_HIP_CHECK( hipStreamSynchronize( stream ) );
+        done = bvh.hostCounts[ 0 ] == 0;
+    }
+
+    uint32_t* nodeTotal = counts + BVH_MAX_LEVELS;
+    ExclusiveScan( bvh.isNode8.as< uint32_t >(), slots, bvh.node8Index.as< uint32_t >(), nodeTotal, bvh.scanScratch.as< uint32_t >(), stream );
+    RG_HIP_CHECK( hipMemcpyAsync( bvh.hostCounts, nodeTotal, 4, hipMemcpyDeviceToHost, stream ) );
+    RG_HIP_CHECK( hipStreamSynchronize( stream ) );
+    bvh.nodeCount = bvh.hostCounts[ 0 ];
+    RG_REQUIRE( bvh.nodeCount >= 1, RG_RESULT_INTERNAL_ERROR, "BVH collapse produced no nodes" );
+
+    bvh.nodes.Reserve( size_t( bvh.nodeCount ) * sizeof( BvhNode8 ) );
+    k_write_nodes<<< DivUp( slots, THREADS ), THREADS, 0, stream >>>( slots, n, bvh.leafMax, bvh.isNode8.as< uint32_t >(), bvh.node8Index.as< uint32_t >(), bvh.expansion.as< uint32_t >(), rf, rl, bounds, bvh.nodes.as< BvhNode8 >() );
+    CheckLaunch( "k_write_nodes" );
+    k_sorted_tris<<< DivUp( n, THREADS ), THREADS, 0, stream >>>( d_tris, vin, n, bvh.sortedTris.as< float4 >() );
+    CheckLaunch( "k_sorted_tris" );
+
+    RG_HIP_CHECK( hipMemcpyAsync( bvh.sceneMin, sceneBounds, 3 * sizeof( double ), hipMemcpyDeviceToHost, stream ) );
+    RG_HIP_CHECK( hipMemcpyAsync( bvh.sceneMax, sceneBounds + 3, 3 * sizeof( double ), hipMemcpyDeviceToHost, stream ) );
+    RG_HIP_CHECK( hipStreamSynchronize( stream ) );
+}
+
+} // namespace rg

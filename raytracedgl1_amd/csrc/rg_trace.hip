@@ -1,0 +1,289 @@
+// Software ray traversal of the 8-wide LBVH on gfx950.
+//
+// Stands in for traceRayEXT (Source/Shaders/RaygenCommon.h:217-269 closest
+// hit; :322-359 shadow rays with gl_RayFlagsSkipClosestHitShaderEXT).
+//
+// Execution model: a persistent grid of 256-thread blocks.  Each lane owns
+// one ray at a time; whenever lanes of a wave go idle, the wave takes that
+// many new rays from a global counter with a single atomic, so long rays do
+// not hold the wave's SIMD lanes empty.  The traversal stack is a short LDS
+// stack (24 entries per lane, interleaved by lane to avoid bank conflicts)
+// backed by a per-lane overflow area in global memory.
+//
+// Closest hit is order-independent: a hit replaces the best one if its t is
+// smaller, or equal with a smaller primitive id, so the result equals a
+// brute-force loop over all triangles with the same intersection routine.
+#include "rg_bvh.h"
+#include "rg_trace_common.h"
+
+namespace rg
+{
+namespace
+{
+
+constexpr int      TR_THREADS    = 256;
+constexpr int      STACK_LDS     = 24;
+constexpr int      STACK_GLOBAL  = 104;
+constexpr int      STACK_TOTAL   = STACK_LDS + STACK_GLOBAL;
+constexpr uint32_t MAX_STEPS_PER_RAY = 1u << 22;
+
+struct TraceArgs
+{
+    const BvhNode8* nodes;
+    const float4*   tris;
+    const RgExtRay* rays;
+    RgExtHit*       hits;
+    uint32_t        count;
+    uint32_t        flags;
+    uint32_t*       rayCounter;
+    uint32_t*       overflowStack;
+    uint32_t*       errorFlag;
+};
+
+__device__ __forceinline__ void CmpSwap( float& da, uint32_t& ra, float& db, uint32_t& rb )
+{
+    if( db < da )
+    {
+        float    td = da;
+        uint32_t tr = ra;
+        da          = db;
+        ra          = rb;
+        db          = td;
+        rb          = tr;
+    }
+}
+
+// Batcher odd-even merge sort network for 8 keys (19 compare-exchanges).
+__device__ __forceinline__ void Sort8( float d[ 8 ], uint32_t r[ 8 ] )
+{
+#define CS( a, b ) CmpSwap( d[ a ], r[ a ], d[ b ], r[ b ] )
+    CS( 0, 1 ); CS( 2, 3 ); CS( 4, 5 ); CS( 6, 7 );
+    CS( 0, 2 ); CS( 1, 3 ); CS( 4, 6 ); CS( 5, 7 );
+    CS( 1, 2 ); CS( 5, 6 );
+    CS( 0, 4 ); CS( 1, 5 ); CS( 2, 6 ); CS( 3, 7 );
+    CS( 2, 4 ); CS( 3, 5 );
+    CS( 1, 2 ); CS( 3, 4 ); CS( 5, 6 );
+#undef CS
+}
+
+__global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
+{
+    __shared__ uint32_t s_stack[ STACK_LDS * TR_THREADS ];
+
+    const int      tid       = threadIdx.x;
+    const int      lane      = tid & 63;
+    const uint64_t ltMask    = lane == 0 ? 0ull : ( ~0ull >> ( 64 - lane ) );
+    const uint32_t gthread   = blockIdx.x * TR_THREADS + tid;
+    uint32_t*      ovf       = a.overflowStack + size_t( gthread ) * STACK_GLOBAL;
+    const bool     anyHit    = ( a.flags & RG_EXT_TRACE_ANY_HIT ) != 0;
+    const bool     cullFront = ( a.flags & RG_EXT_TRACE_CULL_FRONT_FACE ) != 0;
+
+    bool     active = false, exhausted = false;
+    uint32_t rayId  = 0;
+    RayPre   ray;
+    float    tmin = 0.0f, bestT = 0.0f, bestU = 0.0f, bestV = 0.0f;
+    uint32_t bestPrim = RG_EXT_MISS;
+    uint32_t node     = 0;
+    int      sp       = 0;
+    uint32_t steps    = 0;
+
+    while( true )
+    {
+        // Refill idle lanes with new rays, one atomic per wave.
+        const bool     need = !active && !exhausted;
+        const uint64_t mask = __ballot( need );
+        if( mask != 0ull )
+        {
+            const uint32_t cnt    = __popcll( mask );
+            const int      leader = __ffsll( ( long long )mask ) - 1;
+            uint32_t       base   = 0;
+            if( lane == leader )
+            {
+                base = atomicAdd( a.rayCounter, cnt );
+            }
+            base = __shfl( base, leader, 64 );
+            if( need )
+            {
+                const uint32_t my = base + __popcll( mask & ltMask );
+                if( my < a.count )
+                {
+                    const RgExtRay r = a.rays[ my ];
+                    rayId            = my;
+                    ray              = MakeRayPre( mk3( r.origin[ 0 ], r.origin[ 1 ], r.origin[ 2 ] ),
+                                                   mk3( r.direction[ 0 ], r.direction[ 1 ], r.direction[ 2 ] ) );
+                    tmin             = r.tmin;
+                    bestT            = r.tmax;
+                    bestU = bestV = 0.0f;
+                    bestPrim      = RG_EXT_MISS;
+                    node          = 0; // root
+                    sp            = 0;
+                    steps         = 0;
+                    active        = true;
+                }
+                else
+                {
+                    exhausted = true;
+                }
+            }
+        }
+        if( __ballot( active ) == 0ull )
+        {
+            break;
+        }
+        if( !active )
+        {
+            continue;
+        }
+
+        bool finished = false;
+        if( ++steps > MAX_STEPS_PER_RAY )
+        {
+            // Only a malformed tree can get here; end the ray instead of spinning.
+            atomicOr( a.errorFlag, 2u );
+            node = RG_EXT_BVH_CHILD_EMPTY;
+            sp   = 0;
+        }
+        else if( ( node & RG_EXT_BVH_CHILD_LEAF_BIT ) == 0u )
+        {
+            // Interior node: slab-test the 8 children, visit near to far.
+            const BvhNode8& nd = a.nodes[ node ];
+            float           dist[ 8 ];
+            uint32_t        ref[ 8 ];
+            int             hitCount = 0;
+#pragma unroll
+            for( int k = 0; k < 8; k++ )
+            {
+                const uint32_t c = nd.child[ k ];
+                float          tn;
+                const bool     h = c != RG_EXT_BVH_CHILD_EMPTY &&
+                               SlabTest( ray, nd.lo_x[ k ], nd.lo_y[ k ], nd.lo_z[ k ], nd.hi_x[ k ], nd.hi_y[ k ], nd.hi_z[ k ], tmin, bestT, &tn );
+                dist[ k ] = h ? tn : INFINITY;
+                ref[ k ]  = h ? c : RG_EXT_BVH_CHILD_EMPTY;
+                hitCount += h ? 1 : 0;
+            }
+            if( hitCount > 0 )
+            {
+                Sort8( dist, ref );
+#pragma unroll
+                for( int k = 7; k >= 1; k-- )
+                {
+                    if( k < hitCount )
+                    {
+                        if( sp < STACK_LDS )
+                        {
+                            s_stack[ sp * TR_THREADS + tid ] = ref[ k ];
+                        }
+                        else if( sp < STACK_TOTAL )
+                        {
+                            ovf[ sp - STACK_LDS ] = ref[ k ];
+                        }
+                        else
+                        {
+                            atomicOr( a.errorFlag, 1u ); // dropped: stack overflow
+                            continue;
+                        }
+                        sp++;
+                    }
+                }
+                node = ref[ 0 ];
+            }
+            else
+            {
+                node = RG_EXT_BVH_CHILD_EMPTY;
+            }
+        }
+        else
+        {
+            // Leaf: test its triangles.
+            const uint32_t first = ( node >> 3 ) & 0x0FFFFFFFu;
+            const uint32_t cnt   = ( node & 7u ) + 1u;
+            for( uint32_t k = 0; k < cnt; k++ )
+            {
+                const float4 q0 = a.tris[ size_t( first + k ) * 3 + 0 ];
+                const float4 q1 = a.tris[ size_t( first + k ) * 3 + 1 ];
+                const float4 q2 = a.tris[ size_t( first + k ) * 3 + 2 ];
+                float        t, u, v;
+                if( IntersectTri( ray, mk3( q0.x, q0.y, q0.z ), mk3( q1.x, q1.y, q1.z ), mk3( q2.x, q2.y, q2.z ), tmin, bestT, cullFront, &t, &u, &v ) )
+                {
+                    // IntersectTri guarantees t <= bestT; on equal t the smaller id wins.
+                    const uint32_t prim = __float_as_uint( q0.w );
+                    if( t < bestT || prim < bestPrim )
+                    {
+                        bestT    = t;
+                        bestU    = u;
+                        bestV    = v;
+                        bestPrim = prim;
+                        if( anyHit )
+                        {
+                            finished = true;
+                            break;
+                        }
+                    }
+                }
+            }
+            node = RG_EXT_BVH_CHILD_EMPTY;
+        }
+
+        if( !finished && node == RG_EXT_BVH_CHILD_EMPTY )
+        {
+            if( sp == 0 )
+            {
+                finished = true;
+            }
+            else
+            {
+                sp--;
+                node = sp < STACK_LDS ? s_stack[ sp * TR_THREADS + tid ] : ovf[ sp - STACK_LDS ];
+            }
+        }
+        if( finished )
+        {
+            RgExtHit h;
+            h.t                = bestT;
+            h.u                = bestU;
+            h.v                = bestV;
+            h.primitiveId      = bestPrim;
+            a.hits[ rayId ]    = h;
+            active             = false;
+        }
+    }
+}
+
+} // namespace
+
+void TraceRays( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags, hipStream_t stream )
+{
+    RG_REQUIRE( bvh.nodeCount > 0, RG_RESULT_WRONG_FUNCTION_CALL, "BVH is not built" );
+    RG_REQUIRE( bvh.nodeFormat == RG_EXT_BVH_NODE_FP32, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "unsupported node format" );
+    if( count == 0 )
+    {
+        return;
+    }
+    if( s.gridBlocks == 0 )
+    {
+        int dev = 0, cus = 0, perCu = 0;
+        RG_HIP_CHECK( hipGetDevice( &dev ) );
+        RG_HIP_CHECK( hipDeviceGetAttribute( &cus, hipDeviceAttributeMultiprocessorCount, dev ) );
+        RG_HIP_CHECK( hipOccupancyMaxActiveBlocksPerMultiprocessor( &perCu, k_trace, TR_THREADS, 0 ) );
+        s.gridBlocks = uint32_t( cus * ( perCu > 0 ? perCu : 1 ) );
+    }
+    const uint32_t blocks = DivUp( count, TR_THREADS ) < s.gridBlocks ? DivUp( count, TR_THREADS ) : s.gridBlocks;
+    s.counter.Reserve( 2 * sizeof( uint32_t ) );
+    s.overflow.Reserve( size_t( s.gridBlocks ) * TR_THREADS * STACK_GLOBAL * sizeof( uint32_t ) );
+    RG_HIP_CHECK( hipMemsetAsync( s.counter.ptr, 0, 2 * sizeof( uint32_t ), stream ) );
+
+    TraceArgs a;
+    a.nodes         = reinterpret_cast< const BvhNode8* >( bvh.nodes.ptr );
+    a.tris          = reinterpret_cast< const float4* >( bvh.sortedTris.ptr );
+    a.rays          = d_rays;
+    a.hits          = d_hits;
+    a.count         = count;
+    a.flags         = flags;
+    a.rayCounter    = s.counter.as< uint32_t >();
+    a.errorFlag     = s.counter.as< uint32_t >() + 1;
+    a.overflowStack = s.overflow.as< uint32_t >();
+    k_trace<<< blocks, TR_THREADS, 0, stream >>>( a );
+    CheckLaunch( "k_trace" );
+}
+
+} // namespace rg

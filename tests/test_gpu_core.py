@@ -1,0 +1,185 @@
+"""GPU parity of the LBVH build, traversal and luminance histogram against the
+CPU oracle, called through the C ABI (librtgl1_amd.so).
+
+Integer and index outputs (Morton codes, sort order, topology, node child
+references, primitive ids, histogram bins) must be bit-exact; FP32 outputs of
+the same IEEE operation sequence (boxes, hit t/u/v, average luminance) must be
+equal as floats.
+"""
+import numpy as np
+import pytest
+
+from raytracedgl1_amd import ffi, scenes
+from raytracedgl1_amd.ffi import (HIT_DTYPE, RAY_DTYPE, RG_EXT_MISS, RG_EXT_TRACE_ANY_HIT,
+                                  RG_EXT_TRACE_CULL_FRONT_FACE, Bvh, DeviceArray)
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_tris(n, seed, spread=10.0, size=1.0):
+    r = np.random.default_rng(seed)
+    c = r.uniform(-spread, spread, (n, 1, 3))
+    return (c + r.uniform(-size, size, (n, 3, 3))).reshape(n, 9).astype(np.float32)
+
+
+def _gpu_bvh(tris, leaf=3):
+    d = DeviceArray.from_numpy(tris)
+    b = Bvh()
+    b.build(d, len(tris), leaf)
+    return b, d
+
+
+def _assert_same_bvh(g, o):
+    assert g["n"] == o["n"]
+    assert np.array_equal(g["codes"], o["codes"])
+    assert np.array_equal(g["scene_min"], o["scene_min"])
+    assert np.array_equal(g["scene_max"], o["scene_max"])
+    assert np.array_equal(g["sorted_idx"], o["sorted_idx"])
+    assert np.array_equal(g["children"], o["children"])
+    assert np.array_equal(g["bounds"], o["bounds"])  # float equality (+0 == -0)
+    assert g["node_count"] == o["node_count"]
+    for f in ("lo_x", "hi_x", "lo_y", "hi_y", "lo_z", "hi_z"):
+        assert np.array_equal(g["nodes"][f], o["nodes"][f]), f
+    assert np.array_equal(g["nodes"]["child"], o["nodes"]["child"])
+
+
+@pytest.mark.parametrize("n,leaf", [(1, 3), (2, 1), (3, 3), (7, 1), (64, 3), (1000, 4), (4097, 2), (70000, 3)])
+def test_lbvh_bit_exact_random(gpu, oracle, n, leaf):
+    print(f"lbvh random n={n} leaf={leaf}", flush=True)
+    tris = _random_tris(n, seed=n)
+    b, _ = _gpu_bvh(tris, leaf)
+    _assert_same_bvh(b.download(), oracle.bvh(tris, leaf).download())
+
+
+def test_lbvh_bit_exact_duplicates(gpu, oracle):
+    base = _random_tris(1, seed=5)
+    tris = np.repeat(base, 5000, axis=0)
+    tris[:, 0] += np.linspace(-1e-3, 1e-3, 5000, dtype=np.float32)
+    b, _ = _gpu_bvh(tris, 3)
+    _assert_same_bvh(b.download(), oracle.bvh(tris, 3).download())
+
+
+def test_lbvh_bit_exact_config1(gpu, oracle):
+    s = scenes.config1_scene()
+    b, _ = _gpu_bvh(s.tris, 3)
+    _assert_same_bvh(b.download(), oracle.bvh(s.tris, 3).download())
+
+
+def test_lbvh_bit_exact_config2_full_size(gpu, oracle):
+    print("lbvh config2 1M tris", flush=True)
+    s = scenes.config2_scene()
+    b, _ = _gpu_bvh(s.tris, 3)
+    _assert_same_bvh(b.download(), oracle.bvh(s.tris, 3).download())
+
+
+def test_lbvh_rebuild_is_deterministic_and_order_invariant(gpu):
+    # Tests/RtglTest.cpp:510-511 shuffles primitives because upload order must
+    # not change the image: the set of leaf triangles and the tree shape are
+    # the same after a permutation that keeps ids stable by construction.
+    s = scenes.config1_scene()
+    b, d = _gpu_bvh(s.tris, 3)
+    first = b.download()
+    b.build(d, s.n, 3)
+    again = b.download()
+    _assert_same_bvh(first, again)
+
+
+def _trace_gpu(b, rays, flags):
+    dr = DeviceArray.from_numpy(rays)
+    dh = DeviceArray(len(rays), HIT_DTYPE)
+    b.trace(dr, len(rays), dh, flags)
+    return dh.download()
+
+
+def _assert_same_hits(g, o, any_hit=False):
+    assert np.array_equal(g["prim"] == RG_EXT_MISS, o["prim"] == RG_EXT_MISS)
+    if any_hit:
+        return
+    assert np.array_equal(g["prim"], o["prim"])
+    for f in ("t", "u", "v"):
+        assert np.array_equal(g[f], o[f]), f
+
+
+@pytest.mark.parametrize("flags", [0, RG_EXT_TRACE_CULL_FRONT_FACE, RG_EXT_TRACE_ANY_HIT])
+def test_trace_bit_exact_config1_primary(gpu, oracle, flags):
+    print(f"trace config1 flags={flags}", flush=True)
+    s = scenes.config1_scene()
+    rays = scenes.primary_rays(scenes.config1_camera(), 640, 360)
+    b, _ = _gpu_bvh(s.tris, 3)
+    _assert_same_hits(_trace_gpu(b, rays, flags), oracle.bvh(s.tris, 3).trace(rays, flags),
+                      any_hit=bool(flags & RG_EXT_TRACE_ANY_HIT))
+
+
+def test_trace_matches_bruteforce_fp64(gpu, oracle):
+    s = scenes.config1_scene()
+    rays = scenes.primary_rays(scenes.config1_camera(), 640, 360)[::29]
+    b, _ = _gpu_bvh(s.tris, 3)
+    g = _trace_gpu(b, rays, 0)
+    t, u, v, p = oracle.bruteforce(s.tris, rays, 0)
+    miss = p == RG_EXT_MISS
+    assert np.array_equal(g["prim"] == RG_EXT_MISS, miss)
+    h = ~miss
+    assert (np.abs(g["t"][h] - t[h]) / t[h]).max() < 1e-5
+    diff = h & (g["prim"] != p)
+    assert np.all(np.abs(g["t"][diff] - t[diff]) <= 1e-6 * t[diff] + 1e-7)
+
+
+def test_trace_random_rays_and_empty(gpu, oracle):
+    s = scenes.config1_scene()
+    r = np.random.default_rng(3)
+    rays = np.zeros(50000, RAY_DTYPE)
+    rays["origin"] = r.uniform(-40, 40, (50000, 3))
+    d = r.normal(size=(50000, 3))
+    rays["direction"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays["tmin"] = r.uniform(0, 1, 50000)
+    rays["tmax"] = r.uniform(1, 100, 50000)
+    b, _ = _gpu_bvh(s.tris, 3)
+    _assert_same_hits(_trace_gpu(b, rays, 0), oracle.bvh(s.tris, 3).trace(rays, 0))
+    # zero rays is a no-op
+    dh = DeviceArray(1, HIT_DTYPE)
+    dr = DeviceArray(1, RAY_DTYPE)
+    b.trace(dr, 0, dh, 0)
+
+
+def test_trace_config2_full_size(gpu, oracle):
+    print("trace config2 1080p", flush=True)
+    s = scenes.config2_scene()
+    rays = scenes.primary_rays(scenes.config2_camera(0), 1920, 1080)
+    b, _ = _gpu_bvh(s.tris, 3)
+    g = _trace_gpu(b, rays, 0)
+    ob = oracle.bvh(s.tris, 3)
+    sub = np.arange(0, len(rays), 7)
+    _assert_same_hits(g[sub], ob.trace(rays[sub], 0))
+    # size-independent property over the full image: hit points lie on the hit triangle
+    h = g["prim"] != RG_EXT_MISS
+    assert h.mean() > 0.5
+    tri = s.tris.reshape(-1, 3, 3)[g["prim"][h]].astype(np.float64)
+    u, v = g["u"][h, None].astype(np.float64), g["v"][h, None].astype(np.float64)
+    pb = (1 - u - v) * tri[:, 0] + u * tri[:, 1] + v * tri[:, 2]
+    pr = rays["origin"][h] + rays["direction"][h] * g["t"][h, None]
+    assert np.abs(pb - pr).max() < 1e-2
+
+
+def test_histogram_bit_exact(gpu, oracle):
+    r = np.random.default_rng(11)
+    w, h = 1920, 1080
+    lum = np.exp(r.uniform(-12, 12, w * h)).astype(np.float32)
+    lum[::97] = 0.0
+    lum[::101] = np.nan
+    lum[::103] = np.inf
+    lum[1::89] = 0.001
+    min_log, max_log = np.log2(np.exp2(-4.0) * 12.5 / 100), np.log2(np.exp2(12.0) * 12.5 / 100)
+    d = DeviceArray.from_numpy(lum)
+    dh = DeviceArray(256, np.uint32, zero=True)
+    ffi.call("rgExtLuminanceHistogram", d.ptr, w, h, float(min_log), float(max_log), dh.ptr, None)
+    g = dh.download()
+    o = oracle.lum_histogram(lum, float(np.float32(min_log)), float(np.float32(max_log)))
+    assert np.array_equal(g, o)
+    assert g.sum() == w * h
+    # average: equal to the oracle's FP32 evaluation, and the histogram is cleared
+    davg = DeviceArray.from_numpy(np.array([0.5], np.float32))
+    ffi.call("rgExtLuminanceAverage", dh.ptr, w * h, float(min_log), float(max_log), 0.016, 0, davg.ptr, None)
+    ga = davg.download()[0]
+    oa = oracle.lum_average(o, w * h, float(np.float32(min_log)), float(np.float32(max_log)), 0.016, False, 0.5)
+    assert ga == np.float32(oa)
+    assert np.all(dh.download() == 0)
