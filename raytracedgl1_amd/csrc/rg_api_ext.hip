@@ -1,9 +1,11 @@
 // extern "C" entry points of RTGL1_ext.h: device memory helpers, timers, the
 // LBVH and traversal stages, and eye adaptation.
 #include "rg_bvh.h"
+#include "rg_denoise.h"
 #include "rg_internal.h"
 #include "rg_kernels.h"
 
+#include <cmath>
 #include <memory>
 
 namespace rg
@@ -25,6 +27,11 @@ struct RgExtTimer_T
 {
     hipEvent_t start = nullptr;
     hipEvent_t stop  = nullptr;
+};
+
+struct RgExtFramebuffers_T
+{
+    FramebufferSet fb;
 };
 
 struct RgExtBvh_T
@@ -238,6 +245,121 @@ RgResult rgExtLuminanceAverage( uint32_t* d_histogram, uint32_t pixelCount, floa
         RG_REQUIRE( d_histogram && d_avgLuminance, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
         LuminanceAverage( d_histogram, pixelCount, minLogLuminance, maxLogLuminance, timeDelta, stopEyeAdaptation != 0, d_avgLuminance, AsStream( stream ) );
     } );
+}
+
+// ------------------------------------------------------- framebuffers
+
+RgResult rgExtFramebuffersCreate( uint32_t width, uint32_t height, RgExtFramebuffers* pOut )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( pOut, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "pOut is null" );
+        auto f = std::make_unique< RgExtFramebuffers_T >();
+        f->fb.Create( width, height );
+        *pOut = f.release();
+    } );
+}
+
+RgResult rgExtFramebuffersDestroy( RgExtFramebuffers fb )
+{
+    return Guard( [ & ] { delete fb; } );
+}
+
+RgResult rgExtFramebuffersGetPlane( RgExtFramebuffers fb, RgExtPlane plane, RgBool32 prev, void** d_pOut, size_t* pBytes )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( fb && d_pOut, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        RG_REQUIRE( int( plane ) >= 0 && plane < RG_EXT_PLANE_COUNT, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "unknown plane" );
+        *d_pOut = prev ? fb->fb.Prev( plane ) : fb->fb.Cur( plane );
+        if( pBytes )
+        {
+            *pBytes = fb->fb.PlaneBytes( plane );
+        }
+    } );
+}
+
+RgResult rgExtFramebuffersSwap( RgExtFramebuffers fb )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( fb, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        fb->fb.Swap();
+    } );
+}
+
+RgResult rgExtFrameParamsDefaults( uint32_t width, uint32_t height, RgExtFrameParams* p )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( p, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        std::memset( p, 0, sizeof( *p ) );
+        p->renderWidth  = width;
+        p->renderHeight = height;
+        p->timeDelta    = 1.0f / 60.0f;
+        for( int i = 0; i < 4; i++ )
+        {
+            p->view[ i * 5 ] = p->invView[ i * 5 ] = p->viewPrev[ i * 5 ] = 1.0f;
+            p->projection[ i * 5 ] = p->invProjection[ i * 5 ] = p->projectionPrev[ i * 5 ] = 1.0f;
+        }
+        // Tonemapping defaults ev100 in [0, 10] (DrawFrameInfo.h:95-108),
+        // converted as FillUniform does (VulkanDevice.cpp:202-212).
+        p->minLogLuminance          = std::log2( std::exp2( 0.0f ) * 12.5f / 100.0f );
+        p->maxLogLuminance          = std::log2( std::exp2( 10.0f ) * 12.5f / 100.0f );
+        p->antiFireflyEnabled       = 1;
+        p->primaryRayMinDist        = 0.1f;
+        p->rayLength                = 10000.0f;
+        p->skyColorDefault[ 0 ]     = 199.0f / 255.0f;
+        p->skyColorDefault[ 1 ]     = 233.0f / 255.0f;
+        p->skyColorDefault[ 2 ]     = 255.0f / 255.0f;
+        p->skyColorMultiplier       = 1000.0f;
+        p->skyColorSaturation       = 1.0f;
+        p->cellWorldSize            = 1.0f;
+        p->maxBounceShadowsLights   = 2;
+        p->gradientMultDiffuse      = 0.5f;
+        p->gradientMultIndirect     = 0.2f;
+        p->gradientMultSpecular     = 0.5f;
+        p->indirSecondBounce        = 1;
+        p->polyLightSpotlightFactor = 2.0f;
+        p->atrousIterations         = 4;
+        p->restirInitialCandidates  = 8;
+        p->restirTemporalSamples    = 1;
+        p->restirSpatialSamples     = 8;
+    } );
+}
+
+// ----------------------------------------------------------- denoiser
+
+#define RG_DENOISE_ENTRY( expr )                                                                 \
+    return Guard( [ & ] {                                                                        \
+        RG_REQUIRE( fb && p, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );               \
+        expr;                                                                                    \
+    } )
+
+RgResult rgExtDenoiseGradientAtrous( RgExtFramebuffers fb, const RgExtFrameParams* p, uint32_t iteration, void* stream )
+{
+    RG_DENOISE_ENTRY( DenoiseGradientAtrous( fb->fb, *p, iteration, AsStream( stream ) ) );
+}
+
+RgResult rgExtDenoiseTemporal( RgExtFramebuffers fb, const RgExtFrameParams* p, void* stream )
+{
+    RG_DENOISE_ENTRY( DenoiseTemporal( fb->fb, *p, AsStream( stream ) ) );
+}
+
+RgResult rgExtDenoiseAntiFirefly( RgExtFramebuffers fb, const RgExtFrameParams* p, void* stream )
+{
+    RG_DENOISE_ENTRY( DenoiseAntiFirefly( fb->fb, *p, AsStream( stream ) ) );
+}
+
+RgResult rgExtDenoiseVariance( RgExtFramebuffers fb, const RgExtFrameParams* p, void* stream )
+{
+    RG_DENOISE_ENTRY( DenoiseVariance( fb->fb, *p, AsStream( stream ) ) );
+}
+
+RgResult rgExtDenoiseAtrous( RgExtFramebuffers fb, const RgExtFrameParams* p, uint32_t iteration, void* stream )
+{
+    RG_DENOISE_ENTRY( DenoiseAtrous( fb->fb, *p, iteration, AsStream( stream ) ) );
+}
+
+RgResult rgExtDenoise( RgExtFramebuffers fb, const RgExtFrameParams* p, void* stream )
+{
+    RG_DENOISE_ENTRY( Denoise( fb->fb, *p, AsStream( stream ) ) );
 }
 
 } // extern "C"

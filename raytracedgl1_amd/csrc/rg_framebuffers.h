@@ -1,0 +1,52 @@
+// Per-pixel planes of the hot path (the images of Source/Generated/
+// GenerateShaderCommon.py:802-887), stored in HBM as one linear array per
+// plane, row-major, components interleaved (FP32 storage mode: 4 bytes per
+// component).  Planes with a previous-frame copy are double-buffered; Swap()
+// exchanges the copies at the start of a frame like FrameIndexToFBIndex
+// (Source/Framebuffers.cpp:35-47).
+#pragma once
+
+#include "rg_internal.h"
+
+namespace rg
+{
+
+enum PlaneLayout : uint8_t
+{
+    LAYOUT_CHECKERBOARD = 0, // W x H, odd pixels in the right half
+    LAYOUT_REGULAR      = 1, // W x H
+    LAYOUT_STRATA       = 2, // max(1,(W+1)/3) x max(1,(H+1)/3)
+};
+
+struct PlaneDesc
+{
+    PlaneLayout layout;
+    uint8_t     comps; // 4-byte components per pixel
+    bool        prev;
+};
+
+const PlaneDesc& GetPlaneDesc( int plane );
+
+// Device pointers of every plane, passed to kernels by value.
+struct FbPtrs
+{
+    void* cur[ RG_EXT_PLANE_COUNT ];
+    void* prev[ RG_EXT_PLANE_COUNT ];
+};
+
+struct FramebufferSet
+{
+    uint32_t     width = 0, height = 0, gw = 0, gh = 0;
+    DeviceBuffer planes[ RG_EXT_PLANE_COUNT ][ 2 ];
+    int          parity = 0;
+
+    void   Create( uint32_t w, uint32_t h );
+    size_t PlaneBytes( int plane ) const;
+    void*  Cur( int plane );
+    void*  Prev( int plane );
+    void   Swap() { parity ^= 1; }
+    FbPtrs Ptrs();
+    size_t TotalBytes() const;
+};
+
+} // namespace rg

@@ -120,6 +120,17 @@ _SIGS = {
     "rgExtTraceRays": (_RES, [_P, _P, _U32, _P, _U32, _P]),
     "rgExtLuminanceHistogram": (_RES, [_P, _U32, _U32, _F, _F, _P, _P]),
     "rgExtLuminanceAverage": (_RES, [_P, _U32, _F, _F, _F, _U32, _P, _P]),
+    "rgExtFramebuffersCreate": (_RES, [_U32, _U32, C.POINTER(_P)]),
+    "rgExtFramebuffersDestroy": (_RES, [_P]),
+    "rgExtFramebuffersGetPlane": (_RES, [_P, C.c_int, _U32, C.POINTER(_P), C.POINTER(C.c_size_t)]),
+    "rgExtFramebuffersSwap": (_RES, [_P]),
+    "rgExtFrameParamsDefaults": (_RES, [_U32, _U32, C.POINTER(RgExtFrameParams)]),
+    "rgExtDenoiseGradientAtrous": (_RES, [_P, C.POINTER(RgExtFrameParams), _U32, _P]),
+    "rgExtDenoiseTemporal": (_RES, [_P, C.POINTER(RgExtFrameParams), _P]),
+    "rgExtDenoiseAntiFirefly": (_RES, [_P, C.POINTER(RgExtFrameParams), _P]),
+    "rgExtDenoiseVariance": (_RES, [_P, C.POINTER(RgExtFrameParams), _P]),
+    "rgExtDenoiseAtrous": (_RES, [_P, C.POINTER(RgExtFrameParams), _U32, _P]),
+    "rgExtDenoise": (_RES, [_P, C.POINTER(RgExtFrameParams), _P]),
 }
 
 
@@ -278,5 +289,102 @@ class Bvh:
     def __del__(self):
         try:
             lib().rgExtBvhDestroy(self.h)
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------ framebuffers
+
+# RgExtPlane: (name, layout, components, dtype, has_prev); layout 0 =
+# checkerboard W x H, 1 = regular W x H, 2 = strata max(1,(W+1)/3) x max(1,(H+1)/3)
+PLANES = [
+    ("NORMAL", 0, 1, np.uint32, True),
+    ("METALLIC_ROUGHNESS", 0, 2, np.float32, True),
+    ("DEPTH_WORLD", 0, 1, np.float32, True),
+    ("DEPTH_GRAD", 0, 1, np.float32, False),
+    ("MOTION", 0, 4, np.float32, False),
+    ("UNFILTERED_DIRECT", 0, 4, np.float32, False),
+    ("UNFILTERED_SPECULAR", 0, 4, np.float32, False),
+    ("UNFILTERED_INDIR", 0, 4, np.float32, False),
+    ("SURFACE_POSITION", 0, 4, np.float32, True),
+    ("VISIBILITY_BUFFER", 0, 4, np.uint32, True),
+    ("VIEW_DIRECTION", 0, 4, np.float32, False),
+    ("THROUGHPUT", 0, 4, np.float32, False),
+    ("IS_SKY", 0, 1, np.uint32, False),
+    ("ACCUM_HISTORY_LENGTH", 0, 4, np.float32, True),
+    ("DIFF_TEMPORARY", 0, 4, np.float32, False),
+    ("DIFF_ACCUM_COLOR", 0, 4, np.float32, False),
+    ("DIFF_ACCUM_MOMENTS", 0, 2, np.float32, True),
+    ("DIFF_COLOR_HISTORY", 0, 4, np.float32, False),
+    ("DIFF_PING", 0, 4, np.float32, False),
+    ("DIFF_PONG", 0, 4, np.float32, False),
+    ("SPEC_ACCUM_COLOR", 0, 4, np.float32, True),
+    ("SPEC_PING", 0, 4, np.float32, False),
+    ("SPEC_PONG", 0, 4, np.float32, False),
+    ("INDIR_ACCUM", 0, 4, np.float32, True),
+    ("INDIR_PING", 0, 4, np.float32, False),
+    ("INDIR_PONG", 0, 4, np.float32, False),
+    ("ATROUS_FILTERED_VARIANCE", 0, 1, np.float32, False),
+    ("RESERVOIRS", 0, 4, np.uint32, True),
+    ("RESERVOIRS_INITIAL", 0, 4, np.uint32, False),
+    ("GRADIENT_INPUTS", 0, 2, np.float32, True),
+    ("ALBEDO", 1, 4, np.float32, False),
+    ("HISTOGRAM_INPUT", 1, 1, np.float32, False),
+    ("PRE_FINAL", 1, 4, np.float32, False),
+    ("DIS_PING_GRADIENT", 2, 4, np.float32, False),
+    ("DIS_PONG_GRADIENT", 2, 4, np.float32, False),
+    ("DIS_GRADIENT_HISTORY", 2, 4, np.float32, False),
+    ("GRADIENT_PREV_PIX", 2, 1, np.uint32, False),
+]
+PLANE = {name: i for i, (name, *_rest) in enumerate(PLANES)}
+
+
+def plane_shape(plane: int, width: int, height: int) -> tuple:
+    _, layout, comps, _, _ = PLANES[plane]
+    if layout == 2:
+        w, h = max(1, (width + 1) // 3), max(1, (height + 1) // 3)
+    else:
+        w, h = width, height
+    return (h, w, comps)
+
+
+def frame_params(width: int, height: int) -> RgExtFrameParams:
+    p = RgExtFrameParams()
+    call("rgExtFrameParamsDefaults", width, height, C.byref(p))
+    return p
+
+
+class FramebufferSet:
+    """rgExtFramebuffers*: the device planes of one render size."""
+
+    def __init__(self, width: int, height: int):
+        p = _P()
+        call("rgExtFramebuffersCreate", width, height, C.byref(p))
+        self.h = p.value
+        self.width, self.height = width, height
+
+    def plane_ptr(self, plane: int, prev: bool = False) -> tuple:
+        d = _P()
+        nbytes = C.c_size_t()
+        call("rgExtFramebuffersGetPlane", self.h, plane, int(prev), C.byref(d), C.byref(nbytes))
+        return d.value, nbytes.value
+
+    def upload(self, plane: int, arr: np.ndarray, prev: bool = False) -> None:
+        ptr, nbytes = self.plane_ptr(plane, prev)
+        a = np.ascontiguousarray(arr, dtype=PLANES[plane][3])
+        assert a.nbytes == nbytes, (PLANES[plane][0], a.nbytes, nbytes)
+        call("rgExtCopyHostToDevice", ptr, a.ctypes.data, nbytes)
+
+    def download(self, plane: int, prev: bool = False) -> np.ndarray:
+        ptr, nbytes = self.plane_ptr(plane, prev)
+        shape = plane_shape(plane, self.width, self.height)
+        return download_raw(ptr, nbytes, PLANES[plane][3], shape)
+
+    def swap(self) -> None:
+        call("rgExtFramebuffersSwap", self.h)
+
+    def __del__(self):
+        try:
+            lib().rgExtFramebuffersDestroy(self.h)
         except Exception:
             pass

@@ -54,6 +54,15 @@ class Oracle:
             "or_trace_bruteforce_f64": (None, [_P, _U32, _P, _U32, _U32, _P, _P, _P, _P, C.c_int]),
             "or_lum_histogram": (None, [_P, _U32, _F, _F, _P]),
             "or_lum_average": (_F, [_P, _U32, _F, _F, _F, C.c_int, _F]),
+            "or_plane_components": (C.c_int, [C.c_int]),
+            "or_plane_layout": (C.c_int, [C.c_int]),
+            "or_plane_has_prev": (C.c_int, [C.c_int]),
+            "or_denoise_gradient_atrous": (None, [_P, _P, _U32]),
+            "or_denoise_temporal": (None, [_P, _P]),
+            "or_denoise_antifirefly": (None, [_P, _P]),
+            "or_denoise_variance": (None, [_P, _P]),
+            "or_denoise_atrous": (None, [_P, _P, _U32]),
+            "or_denoise": (None, [_P, _P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(l, name)
@@ -169,3 +178,39 @@ class OracleBvh:
         if stats:
             return hits, int(nv.value), int(tt.value)
         return hits
+
+
+PLANE_COUNT = 37
+
+
+class OrFramebuffersC(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int),
+                ("cur", C.c_void_p * PLANE_COUNT), ("prev", C.c_void_p * PLANE_COUNT)]
+
+
+class OracleFramebuffers:
+    """FP64 host planes for the denoiser oracle (components interleaved)."""
+
+    def __init__(self, o: Oracle, width: int, height: int):
+        self.o, self.width, self.height = o, width, height
+        self.cur, self.prev = {}, {}
+        gw, gh = max(1, (width + 1) // 3), max(1, (height + 1) // 3)
+        self.c = OrFramebuffersC()
+        self.c.width, self.c.height = width, height
+        for p in range(PLANE_COUNT):
+            comps = o.l.or_plane_components(p)
+            layout = o.l.or_plane_layout(p)
+            shape = (gh, gw, comps) if layout == 2 else (height, width, comps)
+            self.cur[p] = np.zeros(shape, np.float64)
+            self.c.cur[p] = self.cur[p].ctypes.data
+            if o.l.or_plane_has_prev(p):
+                self.prev[p] = np.zeros(shape, np.float64)
+                self.c.prev[p] = self.prev[p].ctypes.data
+
+    def set(self, plane: int, arr, prev: bool = False):
+        dst = self.prev[plane] if prev else self.cur[plane]
+        dst[...] = np.asarray(arr).reshape(dst.shape).astype(np.float64)
+
+    def run(self, name: str, params, *args):
+        fn = getattr(self.o.l, "or_denoise" + ("_" + name if name else ""))
+        fn(C.byref(self.c), C.byref(params), *args)

@@ -1,0 +1,320 @@
+"""A-SVGF: the FP64 oracle pinned by closed forms (CPU), and each GPU pass
+compared with the oracle on the same seeded inputs (GPU).
+
+Tolerance: every output channel within 1e-3 relative (denominator
+|ref| + 1e-6 max|ref|) of the FP64 oracle, the bound stated for the denoised
+radiance.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from raytracedgl1_amd import ffi
+from raytracedgl1_amd.ffi import PLANE, FramebufferSet, frame_params
+from denoise_inputs import encode_normal, fill_planes, make_gbuffer, rel_err
+from oracle_lib import OracleFramebuffers
+
+TOL = 1e-3
+
+
+def _flat(o, W, H, color=(1.0, 2.0, 0.5), depth=5.0, hist=10.0):
+    """Constant colour on a flat surface: every filter must return it."""
+    fb = OracleFramebuffers(o, W, H)
+    n = encode_normal(np.array([[0.0, 1.0, 0.0]]))[0]
+    fb.set(PLANE["NORMAL"], np.full((H, W, 1), n))
+    fb.set(PLANE["NORMAL"], np.full((H, W, 1), n), prev=True)
+    fb.set(PLANE["DEPTH_WORLD"], np.full((H, W, 1), depth))
+    fb.set(PLANE["DEPTH_WORLD"], np.full((H, W, 1), depth), prev=True)
+    fb.set(PLANE["DEPTH_GRAD"], np.full((H, W, 1), 0.1))
+    c4 = np.zeros((H, W, 4))
+    c4[..., :3] = color
+    for name in ("DIFF_PING", "SPEC_PING", "INDIR_PING"):
+        fb.set(PLANE[name], c4)
+    mr = np.zeros((H, W, 2))
+    mr[..., 1] = 0.6
+    fb.set(PLANE["METALLIC_ROUGHNESS"], mr)
+    hl = np.full((H, W, 4), hist)
+    fb.set(PLANE["ACCUM_HISTORY_LENGTH"], hl)
+    return fb
+
+
+# ----------------------------------------------------------- CPU pinning
+
+def test_atrous_constant_field_is_identity(oracle):
+    W, H = 32, 16
+    fb = _flat(oracle, W, H)
+    p = frame_params(W, H)
+    c4 = fb.cur[PLANE["DIFF_PING"]].copy()
+    c4[..., 3] = 0.5
+    fb.set(PLANE["DIFF_PING"], c4)
+    fb.run("atrous", p, 0)
+    out = fb.cur[PLANE["DIFF_COLOR_HISTORY"]]
+    assert np.allclose(out[..., :3], [1.0, 2.0, 0.5], rtol=1e-12)
+    assert np.allclose(fb.cur[PLANE["SPEC_PONG"]][..., :3], [1.0, 2.0, 0.5], rtol=1e-12)
+    assert np.allclose(fb.cur[PLANE["INDIR_PONG"]][..., :3], [1.0, 2.0, 0.5], rtol=1e-12)
+    # variance: sum(w^2 var) / (sum w)^2 with w = kernel * inside (w_z = w_n = w_l = 1);
+    # interior pixel: 1 + 4*0.5 + 4*0.25 = 4 and 1 + 4*0.25 + 4*0.0625 = 2.25
+    assert math.isclose(out[8, 4, 3], 0.5 * 2.25 / 16.0, rel_tol=1e-12)
+    # prefiltered variance: sqrt of the 3x3 Gaussian of a constant = sqrt(0.5)
+    assert math.isclose(fb.cur[PLANE["ATROUS_FILTERED_VARIANCE"]][8, 4, 0], math.sqrt(0.5), rel_tol=1e-12)
+
+
+def test_atrous_depth_edge_stops_filter(oracle):
+    W, H = 32, 16
+    fb = _flat(oracle, W, H)
+    d = np.full((H, W, 1), 5.0)
+    d[:, 8:] = 50.0  # w_z = 45 / 0.1 -> exp(-w_z^2) = 0 across the edge
+    fb.set(PLANE["DEPTH_WORLD"], d)
+    c4 = fb.cur[PLANE["DIFF_PING"]].copy()
+    c4[:, 8:, :3] = 100.0
+    fb.set(PLANE["DIFF_PING"], c4)
+    fb.run("atrous", frame_params(W, H), 0)
+    out = fb.cur[PLANE["DIFF_COLOR_HISTORY"]]
+    assert np.allclose(out[4, 7, :3], [1.0, 2.0, 0.5])
+    assert np.allclose(out[4, 8, :3], 100.0)
+
+
+def test_atrous_checkerboard_halves_do_not_mix(oracle):
+    W, H = 16, 8
+    fb = _flat(oracle, W, H)
+    c4 = fb.cur[PLANE["DIFF_PING"]].copy()
+    c4[:, W // 2:, :3] = 7.0
+    fb.set(PLANE["DIFF_PING"], c4)
+    fb.run("atrous", frame_params(W, H), 0)
+    out = fb.cur[PLANE["DIFF_COLOR_HISTORY"]]
+    assert np.allclose(out[3, W // 2 - 1, :3], [1.0, 2.0, 0.5])
+    assert np.allclose(out[3, W // 2, :3], 7.0)
+
+
+def test_temporal_static_closed_form(oracle):
+    # W = 14: the strata image is (W+1)/3 = 5 wide, so x / 3 <= 4 stays inside
+    W, H = 14, 8
+    fb = _flat(oracle, W, H, hist=4.0)
+    p = frame_params(W, H)
+    cur = np.zeros((H, W, 4)); cur[..., :3] = 3.0
+    prv = np.zeros((H, W, 4)); prv[..., :3] = 1.0
+    fb.set(PLANE["UNFILTERED_DIRECT"], cur)
+    fb.set(PLANE["DIFF_COLOR_HISTORY"], prv)
+    fb.set(PLANE["ACCUM_HISTORY_LENGTH"], np.full((H, W, 4), 4.0), prev=True)
+    # motion = 0: posPrev is the pixel centre, subPix = 0, one tap of weight 1
+    fb.run("temporal", p)
+    a = max(0.01, 1.0 / min(4.0 + 1.0, 256.0))
+    out = fb.cur[PLANE["DIFF_TEMPORARY"]]
+    assert np.allclose(out[..., :3], 1.0 * (1 - a) + 3.0 * a, rtol=1e-12)
+    assert np.allclose(fb.cur[PLANE["ACCUM_HISTORY_LENGTH"]][..., 0], 5.0)
+    # with antilag alpha 1 the history is dropped: alpha -> 1
+    g = np.zeros_like(fb.cur[PLANE["DIS_PING_GRADIENT"]]); g[..., 0] = 1.0
+    fb.set(PLANE["DIS_PING_GRADIENT"], g)
+    fb.run("temporal", p)
+    assert np.allclose(fb.cur[PLANE["DIFF_TEMPORARY"]][..., :3], 3.0, rtol=1e-12)
+    assert np.allclose(fb.cur[PLANE["ACCUM_HISTORY_LENGTH"]][..., 0], 1.0)
+
+
+def test_temporal_rejects_inconsistent_history(oracle):
+    W, H = 16, 8
+    fb = _flat(oracle, W, H)
+    fb.set(PLANE["NORMAL"], np.full((H, W, 1), encode_normal(np.array([[1.0, 0, 0]]))[0]), prev=True)
+    cur = np.zeros((H, W, 4)); cur[..., :3] = 3.0
+    fb.set(PLANE["UNFILTERED_DIRECT"], cur)
+    fb.set(PLANE["UNFILTERED_INDIR"], cur * 2)
+    fb.run("temporal", frame_params(W, H))
+    assert np.allclose(fb.cur[PLANE["DIFF_TEMPORARY"]][..., :3], 3.0)
+    assert np.allclose(fb.cur[PLANE["INDIR_ACCUM"]][..., :3], 6.0)
+    assert np.allclose(fb.cur[PLANE["ACCUM_HISTORY_LENGTH"]][..., :3], 1.0)
+    l = 0.2125 * 3 + 0.7154 * 3 + 0.0721 * 3
+    assert np.allclose(fb.cur[PLANE["DIFF_ACCUM_MOMENTS"]], [l, l * l])
+
+
+def test_antifirefly_clamps_an_outlier(oracle):
+    W, H = 16, 8
+    fb = OracleFramebuffers(oracle, W, H)
+    c = np.ones((H, W, 4))
+    c[4, 5, :3] = 1000.0
+    c[4, 9, :3] = 0.5     # below its neighbours: raised to their minimum
+    for name in ("DIFF_TEMPORARY", "SPEC_ACCUM_COLOR", "INDIR_ACCUM"):
+        fb.set(PLANE[name], c)
+    p = frame_params(W, H)
+    fb.run("antifirefly", p)
+    out = fb.cur[PLANE["DIFF_ACCUM_COLOR"]]
+    assert np.allclose(out[4, 5, :3], 1.0)
+    assert np.allclose(out[4, 9, :3], 1.0)
+    # at the border the zero texels outside the image lower the minimum
+    assert np.allclose(out[0, 0, :3], 1.0)
+    p.antiFireflyEnabled = 0
+    fb.run("antifirefly", p)
+    assert np.allclose(fb.cur[PLANE["SPEC_PING"]][4, 5, :3], 1000.0)
+
+
+def test_variance_branches(oracle):
+    W, H = 16, 8
+    fb = _flat(oracle, W, H, hist=9.0)
+    c = np.zeros((H, W, 4)); c[..., :3] = 2.0
+    fb.set(PLANE["DIFF_ACCUM_COLOR"], c)
+    fb.set(PLANE["DIFF_ACCUM_MOMENTS"], np.stack([np.full((H, W), 1.0), np.full((H, W), 3.0)], -1))
+    fb.run("variance", frame_params(W, H))
+    # temporal branch (history 9 > 4): (3 - 1^2) / (9 - 4)
+    assert np.allclose(fb.cur[PLANE["DIFF_PING"]][..., 3], 2.0 / 5.0)
+    # spatial branch (history 2): constant field, w = 1 for 8 interior neighbours
+    fb.set(PLANE["ACCUM_HISTORY_LENGTH"], np.full((H, W, 4), 2.0))
+    fb.run("variance", frame_params(W, H))
+    l = 2.0
+    s0 = (l + 1.0 + 8 * l) / 9
+    s1 = (l * l + 3.0 + 8 * l * l) / 9
+    expect = max(0.0, s1 - s0 * s0) * (1 + 2 * (1 - 2.0 / 4.0))
+    assert math.isclose(fb.cur[PLANE["DIFF_PING"]][4, 4, 3], expect, rel_tol=1e-12)
+
+
+def test_gradient_atrous_constant(oracle):
+    W, H = 60, 30
+    fb = OracleFramebuffers(oracle, W, H)
+    g = np.zeros_like(fb.cur[PLANE["DIS_PING_GRADIENT"]]); g[..., :3] = [0.25, 0.5, 0.75]
+    fb.set(PLANE["DIS_PING_GRADIENT"], g)
+    p = frame_params(W, H)
+    fb.run("gradient_atrous", p, 0)
+    assert np.allclose(fb.cur[PLANE["DIS_PONG_GRADIENT"]][2:8, 1:8, :3], [0.25, 0.5, 0.75])
+    assert np.array_equal(fb.cur[PLANE["DIS_GRADIENT_HISTORY"]], fb.cur[PLANE["DIS_PONG_GRADIENT"]])
+
+
+def test_compose_closed_form(oracle):
+    W, H = 8, 4
+    fb = _flat(oracle, W, H)
+    p = frame_params(W, H)
+    for name in ("DIFF_PONG", "SPEC_PONG", "INDIR_PONG"):
+        c = np.zeros((H, W, 4)); c[..., :3] = {"DIFF_PONG": 2.0, "SPEC_PONG": 5.0, "INDIR_PONG": 1.0}[name]
+        fb.set(PLANE[name], c)
+    alb = np.zeros((H, W, 4)); alb[..., :3] = 0.5
+    fb.set(PLANE["ALBEDO"], alb)
+    th = np.zeros((H, W, 4)); th[..., :3] = 1.0
+    fb.set(PLANE["THROUGHPUT"], th)
+    mr = np.zeros((H, W, 2)); mr[..., 1] = 0.3  # smooth: specular stays specular
+    fb.set(PLANE["METALLIC_ROUGHNESS"], mr)
+    fb.set(PLANE["DEPTH_WORLD"], np.full((H, W, 1), 20000.0))  # sky depth: filter passes through
+    sky = np.zeros((H, W, 1)); sky[0, 0] = 1
+    fb.set(PLANE["IS_SKY"], sky)
+    fb.run("atrous", p, 3)
+    pre = fb.cur[PLANE["PRE_FINAL"]]
+    # (2 + 1) * 0.5 + 5 * 0.04 = 1.7; ambient term 1 (albedo luminance 0.5 > 0.02)
+    assert np.allclose(pre[1, 1, :3], 1.7)
+    assert math.isclose(fb.cur[PLANE["HISTOGRAM_INPUT"]][1, 1, 0], 1.7, rel_tol=1e-12)
+    # checkerboard (0, 0) is regular (0, 0): a sky pixel shows albedo * throughput
+    assert np.allclose(pre[0, 0, :3], 0.5)
+
+
+# ------------------------------------------------------------ GPU parity
+
+def _pair(oracle, W, H, seed, **kw):
+    cur, prev = make_gbuffer(W, H, seed, **kw)
+    g = FramebufferSet(W, H)
+    o = OracleFramebuffers(oracle, W, H)
+    fill_planes(cur, prev, g, o)
+    return g, o
+
+
+# Channels holding a variance: E[l^2] - E[l]^2 cancels in FP32 (as in the
+# reference's FP32 shaders), so their error is bounded relative to the
+# channel's scale (1e-3 of |ref| plus 1e-4 of max|ref|), not to each value.
+VARIANCE_CHANNELS = {"DIFF_PING": 3, "DIFF_PONG": 3, "DIFF_COLOR_HISTORY": 3, "ATROUS_FILTERED_VARIANCE": 0}
+
+
+def _check(g, o, planes, prefix=""):
+    for name in planes:
+        p = PLANE[name]
+        got = g.download(p).astype(np.float64)
+        ref = o.cur[p]
+        assert np.all(np.isfinite(got)), name
+        var_c = VARIANCE_CHANNELS.get(name)
+        chans = [c for c in range(ref.shape[-1]) if c != var_c]
+        if chans:
+            e = rel_err(got[..., chans], ref[..., chans])
+            assert e < TOL, f"{prefix}{name}: rel err {e:.3e}"
+        if var_c is not None:
+            gv, rv = got[..., var_c], ref[..., var_c]
+            err = np.abs(gv - rv) / (np.abs(rv) + 1e-4 * np.abs(rv).max() + 1e-30)
+            assert err.max(initial=0) < TOL, f"{prefix}{name} variance: err {err.max():.3e}"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("W,H", [(64, 32), (640, 360)])
+def test_gpu_gradient_atrous(gpu, oracle, W, H):
+    g, o = _pair(oracle, W, H, seed=W)
+    p = frame_params(W, H)
+    for it in range(4):
+        ffi.call("rgExtDenoiseGradientAtrous", g.h, p, it, None)
+        o.run("gradient_atrous", p, it)
+        _check(g, o, ["DIS_PONG_GRADIENT" if it % 2 == 0 else "DIS_PING_GRADIENT"], f"iter{it} ")
+    _check(g, o, ["DIS_GRADIENT_HISTORY"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("W,H", [(64, 32), (640, 360)])
+def test_gpu_temporal(gpu, oracle, W, H):
+    print(f"temporal {W}x{H}", flush=True)
+    g, o = _pair(oracle, W, H, seed=W + 1)
+    p = frame_params(W, H)
+    ffi.call("rgExtDenoiseTemporal", g.h, p, None)
+    o.run("temporal", p)
+    _check(g, o, ["DIFF_TEMPORARY", "DIFF_ACCUM_MOMENTS", "ACCUM_HISTORY_LENGTH", "SPEC_ACCUM_COLOR", "INDIR_ACCUM"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("enabled", [1, 0])
+def test_gpu_antifirefly(gpu, oracle, enabled):
+    W, H = 640, 360
+    g, o = _pair(oracle, W, H, seed=5)
+    for name in ("DIFF_TEMPORARY",):
+        a = g.download(PLANE["UNFILTERED_DIRECT"])
+        g.upload(PLANE[name], a)
+        o.set(PLANE[name], a)
+    p = frame_params(W, H)
+    p.antiFireflyEnabled = enabled
+    ffi.call("rgExtDenoiseAntiFirefly", g.h, p, None)
+    o.run("antifirefly", p)
+    _check(g, o, ["DIFF_ACCUM_COLOR", "SPEC_PING", "INDIR_PING"])
+
+
+@pytest.mark.gpu
+def test_gpu_variance(gpu, oracle):
+    W, H = 640, 360
+    g, o = _pair(oracle, W, H, seed=6)
+    a = g.download(PLANE["UNFILTERED_DIRECT"])
+    g.upload(PLANE["DIFF_ACCUM_COLOR"], a)
+    o.set(PLANE["DIFF_ACCUM_COLOR"], a)
+    p = frame_params(W, H)
+    ffi.call("rgExtDenoiseVariance", g.h, p, None)
+    o.run("variance", p)
+    _check(g, o, ["DIFF_PING"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("iters", [4, 5])
+def test_gpu_atrous_chain(gpu, oracle, iters):
+    W, H = 640, 360
+    g, o = _pair(oracle, W, H, seed=7 + iters)
+    for name in ("DIFF_PING", "SPEC_PING", "INDIR_PING"):
+        a = g.download(PLANE["UNFILTERED_DIRECT" if name == "DIFF_PING" else "UNFILTERED_SPECULAR"]).copy()
+        if name == "DIFF_PING":
+            a[..., 3] = np.random.default_rng(1).uniform(0, 2, a.shape[:2])
+        g.upload(PLANE[name], a)
+        o.set(PLANE[name], a)
+    p = frame_params(W, H)
+    p.atrousIterations = iters
+    for k in range(iters):
+        ffi.call("rgExtDenoiseAtrous", g.h, p, k, None)
+        o.run("atrous", p, k)
+        if k == 0:
+            _check(g, o, ["DIFF_COLOR_HISTORY", "SPEC_PONG", "INDIR_PONG", "ATROUS_FILTERED_VARIANCE"], "iter0 ")
+    _check(g, o, ["PRE_FINAL", "HISTOGRAM_INPUT"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("W,H", [(640, 360), (1920, 1080)])
+def test_gpu_full_chain(gpu, oracle, W, H):
+    """Denoiser::Denoise end to end: the denoised radiance (PRE_FINAL) within
+    1e-3 of the FP64 oracle."""
+    print(f"full A-SVGF chain {W}x{H}", flush=True)
+    g, o = _pair(oracle, W, H, seed=W * 7)
+    p = frame_params(W, H)
+    ffi.call("rgExtDenoise", g.h, p, None)
+    o.run("", p)
+    _check(g, o, ["PRE_FINAL", "HISTOGRAM_INPUT", "DIFF_TEMPORARY", "ACCUM_HISTORY_LENGTH"])
