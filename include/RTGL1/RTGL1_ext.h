@@ -283,7 +283,7 @@ typedef struct RgExtFrameParams
     uint32_t restirTemporalSamples;   /* 1 as the reference                   */
     uint32_t restirSpatialSamples;    /* 8 as the reference                   */
     uint32_t debugShowFlags;
-    uint32_t _pad0;
+    float    emissionMapBoost;    /* 100 as the reference                   */
 } RgExtFrameParams;
 
 /* Fills an RgExtFrameParams with the reference defaults
@@ -306,6 +306,56 @@ RGAPI RgResult RGCONV rgExtDenoiseVariance( RgExtFramebuffers fb, const RgExtFra
 RGAPI RgResult RGCONV rgExtDenoiseAtrous( RgExtFramebuffers fb, const RgExtFrameParams* p, uint32_t iteration, void* stream );
 /* The whole chain as Denoiser::Denoise runs it. */
 RGAPI RgResult RGCONV rgExtDenoise( RgExtFramebuffers fb, const RgExtFrameParams* p, void* stream );
+
+/* ------------------------------------------------------ instance extras */
+/* Build-only knobs, passed in the RgDrawFrameInfo::pParams chain with this
+ * sType (the reference walks the chain in AccessParams, Source/
+ * DrawFrameInfo.h:282-345, and logs unknown sTypes). */
+#define RG_EXT_STRUCTURE_TYPE_PATH_PARAMS ( ( RgStructureType )0x52470001 )
+
+typedef struct RgExtDrawFramePathParams
+{
+    RgStructureType sType;
+    void*           pNext;
+    uint32_t        atrousIterations;        /* 4 (reference) or 5; 0 = default */
+    uint32_t        restirInitialCandidates; /* 0 = default (8)                 */
+    uint32_t        restirTemporalSamples;   /* 0xFFFFFFFF = default (1)        */
+    uint32_t        restirSpatialSamples;    /* 0xFFFFFFFF = default (8)        */
+    uint32_t        renderWidth;             /* 0 = RENDER_RESOLUTION params / 1280 */
+    uint32_t        renderHeight;            /* 0 = RENDER_RESOLUTION params / 720  */
+} RgExtDrawFramePathParams;
+
+/* Per-frame statistics of the last rgDrawFrame; GPU times from HIP events
+ * recorded on the instance stream around each stage. */
+typedef struct RgExtFrameStats
+{
+    uint32_t frameId;
+    uint32_t renderWidth;
+    uint32_t renderHeight;
+    uint32_t triangleCount;
+    uint32_t bvhRebuilt;
+    uint32_t bvhNodeCount;
+    uint64_t raysTraced;
+    float    gpuMsTotal;
+    float    gpuMsAssemble;
+    float    gpuMsBvh;
+    float    gpuMsTrace;
+    float    gpuMsDenoise;
+    float    gpuMsTonemap;
+    float    avgLuminance;
+    float    _pad0;
+} RgExtFrameStats;
+
+RGAPI RgResult RGCONV rgExtGetFrameStats( RgInstance instance, RgExtFrameStats* pOut );
+/* The instance's framebuffer set (valid after the first rgDrawFrame); owned
+ * by the instance, never pass it to rgExtFramebuffersDestroy. */
+RGAPI RgResult RGCONV rgExtGetFramebuffers( RgInstance instance, RgExtFramebuffers* pOut );
+/* The frame constants used by the last rgDrawFrame. */
+RGAPI RgResult RGCONV rgExtGetFrameParams( RgInstance instance, RgExtFrameParams* pOut );
+/* The luminance histogram of the last frame (256 bins) before it is cleared. */
+RGAPI RgResult RGCONV rgExtReadbackHistogram( RgInstance instance, uint32_t* pOut256 );
+/* The instance's stream (hipStream_t). */
+RGAPI RgResult RGCONV rgExtGetStream( RgInstance instance, void** pOut );
 
 #ifdef __cplusplus
 }

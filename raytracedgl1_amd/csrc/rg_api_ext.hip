@@ -29,11 +29,6 @@ struct RgExtTimer_T
     hipEvent_t stop  = nullptr;
 };
 
-struct RgExtFramebuffers_T
-{
-    FramebufferSet fb;
-};
-
 struct RgExtBvh_T
 {
     BvhObject    bvh;
@@ -321,6 +316,7 @@ RgResult rgExtFrameParamsDefaults( uint32_t width, uint32_t height, RgExtFramePa
         p->restirInitialCandidates  = 8;
         p->restirTemporalSamples    = 1;
         p->restirSpatialSamples     = 8;
+        p->emissionMapBoost         = 100.0f;
     } );
 }
 

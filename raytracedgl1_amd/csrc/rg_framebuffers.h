@@ -50,3 +50,9 @@ struct FramebufferSet
 };
 
 } // namespace rg
+
+// Handle type of RTGL1_ext.h's RgExtFramebuffers.
+struct RgExtFramebuffers_T
+{
+    rg::FramebufferSet fb;
+};

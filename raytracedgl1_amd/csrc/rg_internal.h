@@ -129,7 +129,7 @@ struct DeviceBuffer
     }
 
     template< typename T >
-    T* as()
+    T* as() const
     {
         return reinterpret_cast< T* >( ptr );
     }

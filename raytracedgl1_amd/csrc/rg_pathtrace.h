@@ -1,0 +1,40 @@
+// Ray-traced passes of a frame: primary G-buffer, light grid, ReSTIR DI
+// (initial reservoirs, temporal + spatial reuse, shadow rays) and indirect
+// bounces, scheduled as wavefronts around the shared traversal kernel.
+#pragma once
+
+#include "rg_bvh.h"
+#include "rg_framebuffers.h"
+#include "rg_scene.h"
+
+namespace rg
+{
+
+struct FrameTimings
+{
+    float assembleMs = 0, bvhMs = 0, primaryMs = 0, directMs = 0, indirectMs = 0, denoiseMs = 0, tonemapMs = 0, totalMs = 0;
+    uint64_t rays = 0;
+};
+
+struct PathTracer
+{
+    DeviceBuffer rays, hits, rays2, hits2; // W*H wavefront buffers
+    DeviceBuffer pendA, pendB, pendC;      // per-pixel state between wavefronts (W*H float4 / uint4)
+    DeviceBuffer lights, prevToCur;
+    DeviceBuffer grid[ 2 ];                // 16^3 * 128 light-grid reservoirs, ping-pong
+    DeviceBuffer blueNoise;                // 128 layers of 128x128 RGBA8
+    int          gridParity = 0;
+    uint32_t     lightCount = 0, lightArraySize = 0, prevToCurCount = 0;
+    uint32_t     W = 0, H = 0;
+
+    void Init( hipStream_t s );
+    void Resize( uint32_t w, uint32_t h );
+    void UploadLights( const SceneCollector& sc, hipStream_t s );
+
+    // One frame of ray tracing into fb (G-buffer, unfiltered direct /
+    // specular / indirect, reservoirs).  Returns the number of rays traced.
+    uint64_t Trace( const RgExtFrameParams& p, const BvhObject& bvh, TraceContext& tc, const DeviceScene& scene,
+                    FramebufferSet& fb, uint32_t traceFlags, hipStream_t s, FrameTimings* t );
+};
+
+} // namespace rg

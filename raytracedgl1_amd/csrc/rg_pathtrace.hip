@@ -1,0 +1,1115 @@
+// Ray-traced passes of a frame, as wavefronts around the traversal kernel.
+//
+//   primary      RtRaygenPrimary.rgen -> RaygenPrimary.inl:280-348 (G-buffer)
+//   light grid   CmLightGridBuild.comp:46-105
+//   initial      RtInitialReservoirs.rgen -> calcInitialReservoir
+//                (RaygenCommon.h:396-486), directional-light visibility ray
+//   direct       RtRaygenDirect.rgen -> selectLight_Direct / traceDirect-
+//                Illumination (RaygenCommon.h:503-674), shadow ray
+//   indirect     RtRaygenIndirect.inl:66-240 (GGX-VNDF bounce, shadowed
+//                direct light at the hit, second diffuse bounce) and the
+//                shading of RtRaygenIndirect.inl:402-433
+//
+// Each pass writes rays into a buffer, the persistent traversal kernel
+// resolves them, and a shading kernel consumes the hits; per-pixel state
+// between the two halves of a pass lives in "pending" planes.  Pixels that
+// need no ray emit an empty ray (tmax < tmin) that the traversal rejects at
+// the root.
+#include "rg_denoise.h"
+#include "rg_pathtrace.h"
+#include "rg_shading.h"
+
+namespace rg
+{
+namespace
+{
+
+constexpr int BX = 32, BY = 8, THREADS = 256;
+constexpr int GRID_X = 16, GRID_Y = 16, GRID_Z = 16, GRID_CELL = 128; // LIGHT_GRID_SIZE_*, LIGHT_GRID_CELL_SIZE
+constexpr uint32_t GRID_ENTRIES = GRID_X * GRID_Y * GRID_Z * GRID_CELL;
+
+struct SceneView
+{
+    const float*      tris;
+    const float*      normals;
+    const uint32_t*   triGeom;
+    const GeomRecord* geoms;
+};
+
+struct LightView
+{
+    const LightEncoded* lights;
+    uint32_t            arraySize;
+    const uint32_t*     prevToCur;
+    uint32_t            prevCount;
+    const uint4*        grid;
+    const uint4*        gridPrev;
+    const uint32_t*     blueNoise;
+};
+
+__device__ __forceinline__ LightEncoded GetLight( const LightView& lv, uint32_t i )
+{
+    if( i < lv.arraySize )
+    {
+        return lv.lights[ i ];
+    }
+    LightEncoded none{};
+    return none;
+}
+
+// ------------------------------------------------------------ helpers
+
+template< typename T >
+__device__ __forceinline__ T* Plane( const FbPtrs& P, int plane, bool prev = false )
+{
+    return reinterpret_cast< T* >( prev ? P.prev[ plane ] : P.cur[ plane ] );
+}
+
+__device__ __forceinline__ v3 DecodeNormal( uint32_t p )
+{
+    float sp, cp, st, ct;
+    sincospif( float( p >> 16 ) / 65535.0f, &sp, &cp );
+    sincospif( float( p & 0xFFFFu ) / 32768.0f, &st, &ct );
+    return mk3( sp * ct, sp * st, cp );
+}
+
+// encodeNormal (Utils.h:108-124)
+__device__ __forceinline__ uint32_t EncodeNormal( v3 n )
+{
+    const float phi   = acosf( clampf( n.z, -1.0f, 1.0f ) );
+    float       theta = atan2f( n.y, n.x );
+    theta             = theta < 0.0f ? theta + 2.0f * RG_PI : theta;
+    const uint32_t j  = uint32_t( roundf( phi * 65535.0f / RG_PI ) );
+    const uint32_t k  = uint32_t( roundf( theta * 65536.0f / ( 2.0f * RG_PI ) ) ) % 65536u;
+    return ( j << 16 ) | k;
+}
+
+__device__ __forceinline__ v3 MulM4( const float* m, v3 v, float w, float* wOut )
+{
+    const v3 r = mk3( m[ 0 ] * v.x + m[ 4 ] * v.y + m[ 8 ] * v.z + m[ 12 ] * w,
+                      m[ 1 ] * v.x + m[ 5 ] * v.y + m[ 9 ] * v.z + m[ 13 ] * w,
+                      m[ 2 ] * v.x + m[ 6 ] * v.y + m[ 10 ] * v.z + m[ 14 ] * w );
+    *wOut      = m[ 3 ] * v.x + m[ 7 ] * v.y + m[ 11 ] * v.z + m[ 15 ] * w;
+    return r;
+}
+
+// getRayDir (ShaderCommonGLSLFunc.h:460-468)
+__device__ __forceinline__ v3 GetRayDir( const RgExtFrameParams& p, float u, float v )
+{
+    u = u * 2.0f - 1.0f;
+    v = v * 2.0f - 1.0f;
+    float    tw;
+    const v3 t     = MulM4( p.invProjection, mk3( u, v, 1.0f ), 1.0f, &tw );
+    const v3 local = fabsf( tw ) < 0.001f ? t : t / tw;
+    float    w;
+    return MulM4( p.invView, normalize( local ), 0.0f, &w );
+}
+
+__device__ __forceinline__ v3 UnpackColor( uint32_t c )
+{
+    return mk3( float( c & 0xFFu ) / 255.0f, float( ( c >> 8 ) & 0xFFu ) / 255.0f, float( ( c >> 16 ) & 0xFFu ) / 255.0f );
+}
+
+// getSky / adjustSky (RaygenCommon.h:272-306) for RG_SKY_TYPE_COLOR
+__device__ __forceinline__ v3 SkyColor( const RgExtFrameParams& p )
+{
+    const v3    c = mk3( p.skyColorDefault[ 0 ], p.skyColorDefault[ 1 ], p.skyColorDefault[ 2 ] );
+    const float g = luminance( c );
+    return mix3( mk3( g, g, g ), c, p.skyColorSaturation ) * p.skyColorMultiplier;
+}
+
+__device__ __forceinline__ v3 SpecularColor( v3 albedo, float metallic )
+{
+    return mix3( mk3( 0.04f, 0.04f, 0.04f ), albedo, metallic );
+}
+
+__device__ __forceinline__ v3 Xform34( const float* m, v3 p )
+{
+    return mk3( m[ 0 ] * p.x + m[ 1 ] * p.y + m[ 2 ] * p.z + m[ 3 ],
+                m[ 4 ] * p.x + m[ 5 ] * p.y + m[ 6 ] * p.z + m[ 7 ],
+                m[ 8 ] * p.x + m[ 9 ] * p.y + m[ 10 ] * p.z + m[ 11 ] );
+}
+
+__device__ __forceinline__ RgExtRay MakeRay( v3 o, v3 d, float tmin, float tmax )
+{
+    RgExtRay r;
+    r.origin[ 0 ] = o.x, r.origin[ 1 ] = o.y, r.origin[ 2 ] = o.z, r.tmin = tmin;
+    r.direction[ 0 ] = d.x, r.direction[ 1 ] = d.y, r.direction[ 2 ] = d.z, r.tmax = tmax;
+    return r;
+}
+__device__ __forceinline__ RgExtRay NullRay()
+{
+    return MakeRay( mk3( 0, 0, 0 ), mk3( 0, 1, 0 ), 1.0f, -1.0f );
+}
+
+struct HitInfo
+{
+    v3       position;
+    v3       normal;
+    v3       albedo;
+    v3       emission;
+    float    roughness;
+    float    metallic;
+    uint32_t geom;
+    uint32_t localPrim;
+};
+
+// getHitInfo* (HitInfo.inl:204-466) for untextured geometry: albedo is the
+// primitive colour (layer 0), normals interpolated or exact.
+__device__ HitInfo GetHit( const SceneView& sc, const RgExtFrameParams& p, uint32_t prim, float u, float v, v3 rayOrigin )
+{
+    HitInfo           h;
+    const uint32_t    g  = sc.triGeom[ prim ];
+    const GeomRecord& gr = sc.geoms[ g ];
+    const float*      t  = sc.tris + size_t( prim ) * 9;
+    const float*      n  = sc.normals + size_t( prim ) * 9;
+    const float       b0 = 1.0f - u - v;
+    const v3          p0 = mk3( t[ 0 ], t[ 1 ], t[ 2 ] ), p1 = mk3( t[ 3 ], t[ 4 ], t[ 5 ] ), p2 = mk3( t[ 6 ], t[ 7 ], t[ 8 ] );
+    h.position           = p0 * b0 + p1 * u + p2 * v;
+    if( ( gr.flags & RG_MESH_PRIMITIVE_FORCE_EXACT_NORMALS ) == 0 )
+    {
+        h.normal = normalize( mk3( n[ 0 ], n[ 1 ], n[ 2 ] ) * b0 + mk3( n[ 3 ], n[ 4 ], n[ 5 ] ) * u + mk3( n[ 6 ], n[ 7 ], n[ 8 ] ) * v );
+    }
+    else
+    {
+        h.normal = safeNormalize( cross( p1 - p0, p2 - p0 ) );
+        if( dot( h.normal, h.position - rayOrigin ) > 0.0f )
+        {
+            h.normal = -h.normal;
+        }
+    }
+    const v3 c  = UnpackColor( gr.color );
+    h.albedo    = mk3( saturate( c.x ), saturate( c.y ), saturate( c.z ) );
+    h.roughness = fmaxf( gr.roughness, fmaxf( p.minRoughness, MIN_GGX_ROUGH ) );
+    h.metallic  = gr.metallic;
+    h.emission  = h.albedo * gr.emissive;
+    h.geom      = g;
+    h.localPrim = prim - gr.firstTri;
+    return h;
+}
+
+// fetchGbufferSurface (Surface.inl:44-69)
+__device__ Surface FetchSurface( const FbPtrs& P, int W, int x, int y )
+{
+    Surface        s;
+    const size_t   i = size_t( y ) * W + x;
+    s.isSky          = Plane< uint32_t >( P, RG_EXT_PLANE_IS_SKY )[ i ] != 0u;
+    if( s.isSky )
+    {
+        return s;
+    }
+    int rx, ry;
+    cbToRegular( x, y, W, &rx, &ry );
+    const float4 alb   = Plane< float4 >( P, RG_EXT_PLANE_ALBEDO )[ size_t( ry ) * W + rx ];
+    const float4 pos   = Plane< float4 >( P, RG_EXT_PLANE_SURFACE_POSITION )[ i ];
+    const float2 mr    = Plane< float2 >( P, RG_EXT_PLANE_METALLIC_ROUGHNESS )[ i ];
+    const float4 vd    = Plane< float4 >( P, RG_EXT_PLANE_VIEW_DIRECTION )[ i ];
+    s.albedo           = mk3( alb.x, alb.y, alb.z );
+    s.position         = mk3( pos.x, pos.y, pos.z );
+    s.instCustomIndex  = __float_as_uint( pos.w );
+    s.specularColor    = SpecularColor( s.albedo, mr.x );
+    s.roughness        = mr.y;
+    s.normal           = DecodeNormal( Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL )[ i ] );
+    s.toViewerDir      = mk3( -vd.x, -vd.y, -vd.z );
+    return s;
+}
+
+// hitInfoToSurface_Indirect (Surface.inl:96-108)
+__device__ Surface HitToSurface( const HitInfo& h, v3 rayDir )
+{
+    Surface s;
+    s.position        = h.position;
+    s.instCustomIndex = 1u;
+    s.normal          = h.normal;
+    s.roughness       = h.roughness;
+    s.albedo          = h.albedo;
+    s.isSky           = false;
+    s.specularColor   = SpecularColor( h.albedo, h.metallic );
+    s.toViewerDir     = -rayDir;
+    return s;
+}
+
+// rndBlueNoise8 (Random.h:229-238) over the seeded table
+__device__ __forceinline__ void BlueNoise8( const uint32_t* table, uint32_t seed, uint32_t salt, float out[ 4 ] )
+{
+    const uint32_t layer = ( ( seed >> 14 ) + salt ) % 128u;
+    const uint32_t oy    = ( seed >> 7 ) & 127u;
+    const uint32_t ox    = seed & 127u;
+    const uint32_t t     = table[ ( layer * 128u + oy ) * 128u + ox ];
+    for( int k = 0; k < 4; k++ )
+    {
+        out[ k ] = float( ( t >> ( 8 * k ) ) & 0xFFu ) / 255.0f;
+    }
+}
+
+// --------------------------------------------------------- light grid
+
+struct Grid
+{
+    v3    minExt, maxExt, delta;
+    float cellRadius;
+};
+
+// LightGrid.h:31-71
+__device__ __forceinline__ Grid MakeGrid( const RgExtFrameParams& p )
+{
+    Grid        g;
+    const float d = p.cellWorldSize;
+    g.delta       = mk3( d, d, d );
+    const v3 whole = mk3( d * GRID_X, d * GRID_Y, d * GRID_Z );
+    const v3 center = mk3( p.cameraPosition[ 0 ], p.cameraPosition[ 1 ], p.cameraPosition[ 2 ] ) + g.delta * 0.5f;
+    g.minExt       = center - whole * 0.5f;
+    g.maxExt       = center + whole * 0.5f;
+    g.cellRadius   = length( g.delta ) * 0.5f;
+    return g;
+}
+__device__ __forceinline__ bool InsideGrid( const Grid& g, v3 p )
+{
+    return p.x > g.minExt.x && p.y > g.minExt.y && p.z > g.minExt.z && p.x < g.maxExt.x && p.y < g.maxExt.y && p.z < g.maxExt.z;
+}
+__device__ __forceinline__ int CellArrayIndex( const Grid& g, v3 wp )
+{
+    int c[ 3 ];
+    const v3 r = ( wp - g.minExt );
+    c[ 0 ]     = min( max( int( r.x / g.delta.x ), 0 ), GRID_X );
+    c[ 1 ]     = min( max( int( r.y / g.delta.y ), 0 ), GRID_Y );
+    c[ 2 ]     = min( max( int( r.z / g.delta.z ), 0 ), GRID_Z );
+    return GRID_CELL * ( c[ 0 ] + c[ 1 ] * GRID_X + c[ 2 ] * GRID_X * GRID_Y );
+}
+
+__global__ __launch_bounds__( THREADS ) void k_light_grid( RgExtFrameParams p, LightView lv, uint4* gridOut )
+{
+    const int arrayIndex = blockIdx.x * THREADS + threadIdx.x;
+    if( arrayIndex >= int( GRID_ENTRIES ) )
+    {
+        return;
+    }
+    const Grid g = MakeGrid( p );
+    const int  c = arrayIndex / GRID_CELL;
+    const int  ci[ 3 ] = { c % GRID_X, ( c % ( GRID_X * GRID_Y ) ) / GRID_X, c / ( GRID_X * GRID_Y ) };
+    const v3   cellCenter = g.minExt + g.delta * mk3( ci[ 0 ] + 0.5f, ci[ 1 ] + 0.5f, ci[ 2 ] + 0.5f );
+    const uint32_t seed   = randomSeed( arrayIndex, 0, p.frameId );
+    uint32_t       salt   = SALT_LIGHT_GRID_BASE;
+
+    Reservoir regular = emptyReservoir();
+    for( int i = 0; i < 8; i++ )
+    {
+        const float    rnd = rnd16( seed, salt++ );
+        const uint32_t xi  = 1u + min( uint32_t( rnd * float( p.lightCount ) ), p.lightCount - 1u );
+        const float    tp  = lightWeight( GetLight( lv, xi ), cellCenter, g.cellRadius, p.polyLightSpotlightFactor );
+        updateReservoir( regular, xi, tp, float( p.lightCount ), rnd16( seed, salt++ ) );
+    }
+    // temporal reuse of the previous frame's cell sample (LIGHT_GRID_TEMPORAL)
+    Reservoir temporal = unpackReservoir( lv.gridPrev[ arrayIndex ] );
+    temporal.M         = 1;
+    normalizeReservoir( temporal, regular.M * 20 );
+    float tpTemporal = 0.0f;
+    if( temporal.selected != LIGHT_INDEX_NONE )
+    {
+        const uint32_t cur = temporal.selected < lv.prevCount ? lv.prevToCur[ temporal.selected ] : 0xFFFFFFFFu;
+        if( cur != 0xFFFFFFFFu && cur != LIGHT_INDEX_NONE )
+        {
+            tpTemporal        = lightWeight( GetLight( lv, cur ), cellCenter, g.cellRadius, p.polyLightSpotlightFactor );
+            temporal.selected = cur;
+        }
+    }
+    const float rndRis   = rnd16( seed, salt++ );
+    Reservoir   combined = regular;
+    updateCombinedNewSurf( combined, temporal, tpTemporal, rndRis );
+    normalizeReservoir( combined, 1 );
+    // ShLightInCell: {selected, targetPdf, weightSum} (M is 1 by construction)
+    gridOut[ arrayIndex ] = make_uint4( combined.selected, __float_as_uint( combined.targetPdf ), __float_as_uint( combined.weightSum ), 0u );
+}
+
+// calcInitialReservoir (RaygenCommon.h:396-486) up to the directional-light
+// visibility test.  Returns the regular and directional reservoirs and the
+// salt to continue with.
+__device__ void InitialCandidates( const RgExtFrameParams& p, const LightView& lv, const Surface& surf, uint32_t seed, uint32_t& salt,
+                                   float pu, float pv, bool loose, Reservoir& regular, Reservoir& dir )
+{
+    const Grid g = MakeGrid( p );
+    regular      = emptyReservoir();
+    if( InsideGrid( g, surf.position ) )
+    {
+        float r4[ 4 ];
+        rnd8_4( seed, salt++, r4 );
+        v3 jp = surf.position + ( mk3( r4[ 0 ], r4[ 1 ], r4[ 2 ] ) * 2.0f - mk3( 1, 1, 1 ) ) * g.cellRadius;
+        jp    = vmin( vmax( jp, g.minExt ), g.maxExt );
+        const int base = CellArrayIndex( g, jp );
+        for( uint32_t i = 0; i < p.restirInitialCandidates; i++ )
+        {
+            const float rnd = rnd16( seed, salt++ );
+            const int   idx = base + min( max( int( rnd * GRID_CELL ), 0 ), GRID_CELL - 1 );
+            // the index can reach one cell past the grid (worldToCell clamps to
+            // the size, LightGrid.h:80-86): such a slot reads as empty
+            const uint4 cell = idx < int( GRID_ENTRIES ) ? lv.grid[ idx ] : make_uint4( LIGHT_INDEX_NONE, 0u, 0u, 0u );
+            Reservoir   r    = Reservoir{ cell.x, __uint_as_float( cell.y ), __uint_as_float( cell.z ), 1u };
+            if( !isValid( r ) )
+            {
+                continue;
+            }
+            const float       oneOverSource = r.weightSum * safePositiveRcp( r.targetPdf );
+            const LightSample ls            = sampleLight( GetLight( lv, r.selected ), surf.position, pu, pv, p.polyLightSpotlightFactor );
+            const float       tp            = targetPdf( surf, ls, loose );
+            updateReservoir( regular, r.selected, tp, oneOverSource, rnd16( seed, salt++ ) );
+        }
+    }
+    else
+    {
+        for( uint32_t i = 0; i < p.restirInitialCandidates; i++ )
+        {
+            const float       rnd = rnd16( seed, salt++ );
+            const uint32_t    xi  = 1u + min( uint32_t( rnd * float( p.lightCount ) ), p.lightCount - 1u );
+            const LightSample ls  = sampleLight( GetLight( lv, xi ), surf.position, pu, pv, p.polyLightSpotlightFactor );
+            const float       tp  = targetPdf( surf, ls, loose );
+            updateReservoir( regular, xi, tp, float( p.lightCount ), rnd16( seed, salt++ ) );
+        }
+    }
+    normalizeReservoir( regular, 1 );
+    dir = emptyReservoir();
+    if( p.directionalLightExists )
+    {
+        const LightSample ls = sampleLight( GetLight( lv, 0 ), surf.position, pu, pv, p.polyLightSpotlightFactor );
+        const float       tp = targetPdf( surf, ls, loose );
+        updateReservoir( dir, 0, tp, 1.0f, rnd16( seed, salt++ ) );
+    }
+    normalizeReservoir( dir, 1 );
+}
+
+__device__ __forceinline__ Reservoir CombineInitial( Reservoir regular, const Reservoir& dir, uint32_t seed, uint32_t& salt )
+{
+    const float rnd = rnd16( seed, salt++ );
+    updateCombined( regular, dir, rnd );
+    normalizeReservoir( regular, 1 );
+    return regular;
+}
+
+// traceShadowRay (RaygenCommon.h:322-350): from the surface biased toward
+// the viewer to the light point, tmin 0.001, tmax = distance - 0.01
+__device__ __forceinline__ RgExtRay ShadowRay( const Surface& s, v3 lightPos )
+{
+    const v3    start = s.position + s.toViewerDir * 0.01f;
+    v3          l     = lightPos - start;
+    const float dist  = length( l );
+    return MakeRay( start, l / dist, 0.001f, dist - 0.01f );
+}
+
+__device__ __forceinline__ void LightPointRnd( uint32_t seed, float* u, float* v )
+{
+    rnd16_2( seed, SALT_LIGHT_POINT, u, v );
+    *u *= 0.99f;
+    *v *= 0.99f;
+}
+
+__device__ __forceinline__ bool DirectValid( const RgExtFrameParams& p, int bounce )
+{
+    return ( uint32_t( bounce ) < p.maxBounceShadowsLights || p.maxBounceShadowsLights == 0 ) &&
+           ( p.lightCount + p.directionalLightExists > 0 );
+}
+
+// ------------------------------------------------------------ kernels
+
+__global__ __launch_bounds__( THREADS ) void k_blue_noise( uint32_t* table )
+{
+    const uint32_t i = blockIdx.x * THREADS + threadIdx.x;
+    if( i < 128u * 128u * 128u )
+    {
+        // SplitMix64 of the texel index: a white-noise stand-in for the
+        // blue-noise KTX2 array the reference loads (BlueNoise.cpp:37-180)
+        uint64_t z = 0x9E3779B97F4A7C15ull * ( uint64_t( i ) + 1ull ) + 0xC0FFEEull;
+        z          = ( z ^ ( z >> 30 ) ) * 0xBF58476D1CE4E5B9ull;
+        z          = ( z ^ ( z >> 27 ) ) * 0x94D049BB133111EBull;
+        table[ i ] = uint32_t( z ^ ( z >> 31 ) );
+    }
+}
+
+// Primary rays over regular pixels (RaygenPrimary.inl:300-314).
+__global__ __launch_bounds__( THREADS ) void k_primary_rays( RgExtFrameParams p, RgExtRay* rays )
+{
+    const int x = blockIdx.x * BX + threadIdx.x % BX, y = blockIdx.y * BY + threadIdx.x / BX;
+    const int W = p.renderWidth, H = p.renderHeight;
+    if( x >= W || y >= H )
+    {
+        return;
+    }
+    const float u = ( float( x ) + 0.5f + p.jitterX ) / float( W );
+    const float v = ( float( y ) + 0.5f + p.jitterY ) / float( H );
+    const v3    d = GetRayDir( p, u, v );
+    rays[ size_t( y ) * W + x ] = MakeRay( mk3( p.cameraPosition[ 0 ], p.cameraPosition[ 1 ], p.cameraPosition[ 2 ] ), d, p.primaryRayMinDist, p.rayLength );
+}
+
+// Moller-Trumbore barycentrics of a ray against a triangle's plane
+// (intersectRayTriangle, HitInfo.inl:181-198)
+__device__ __forceinline__ v3 RayTriBary( v3 p0, v3 p1, v3 p2, v3 o, v3 d )
+{
+    const v3    e1  = p1 - p0, e2 = p2 - p0;
+    const v3    pv  = cross( d, e2 );
+    const float inv = 1.0f / dot( e1, pv );
+    const v3    tv  = o - p0;
+    const v3    qv  = cross( tv, e1 );
+    const float u   = dot( tv, pv ) * inv;
+    const float v   = dot( d, qv ) * inv;
+    return mk3( 1.0f - u - v, u, v );
+}
+
+// G-buffer of the primary hit (RaygenPrimary.inl:280-348, storeSky :90-139).
+__global__ __launch_bounds__( THREADS ) void k_primary_shade( RgExtFrameParams p, const RgExtHit* hits, SceneView sc, FbPtrs P )
+{
+    const int x = blockIdx.x * BX + threadIdx.x % BX, y = blockIdx.y * BY + threadIdx.x / BX;
+    const int W = p.renderWidth, H = p.renderHeight;
+    if( x >= W || y >= H )
+    {
+        return;
+    }
+    int cx, cy;
+    cbFromRegular( x, y, W, &cx, &cy );
+    const size_t   ci  = size_t( cy ) * W + cx;
+    const size_t   ri  = size_t( y ) * W + x;
+    const RgExtHit hit = hits[ ri ];
+    const float    u   = ( float( x ) + 0.5f + p.jitterX ) / float( W );
+    const float    v   = ( float( y ) + 0.5f + p.jitterY ) / float( H );
+    const v3       dir = GetRayDir( p, u, v );
+    const v3       org = mk3( p.cameraPosition[ 0 ], p.cameraPosition[ 1 ], p.cameraPosition[ 2 ] );
+
+    if( hit.primitiveId == RG_EXT_MISS )
+    {
+        // storeSky; the motion of an infinitely far point uses the
+        // checkerboard pixel, as RaygenPrimary.inl:74-76 and :118 do
+        const float cu = ( float( cx ) + 0.5f + p.jitterX ) / float( W );
+        const float cv = ( float( cy ) + 0.5f + p.jitterY ) / float( H );
+        const v3    rd = GetRayDir( p, cu, cv );
+        float       w;
+        const v3    vc = MulM4( p.view, rd, 0.0f, &w ), vp = MulM4( p.viewPrev, rd, 0.0f, &w );
+        // mat3(projection) * v: the upper-left 3x3 block, no division by w
+        const v3 cc = mk3( p.projection[ 0 ] * vc.x + p.projection[ 4 ] * vc.y + p.projection[ 8 ] * vc.z,
+                           p.projection[ 1 ] * vc.x + p.projection[ 5 ] * vc.y + p.projection[ 9 ] * vc.z, 0.0f );
+        const v3 cp = mk3( p.projectionPrev[ 0 ] * vp.x + p.projectionPrev[ 4 ] * vp.y + p.projectionPrev[ 8 ] * vp.z,
+                           p.projectionPrev[ 1 ] * vp.x + p.projectionPrev[ 5 ] * vp.y + p.projectionPrev[ 9 ] * vp.z, 0.0f );
+        const float mx = ( cp.x * 0.5f + 0.5f ) - ( cc.x * 0.5f + 0.5f );
+        const float my = ( cp.y * 0.5f + 0.5f ) - ( cc.y * 0.5f + 0.5f );
+        const v3    sky = SkyColor( p );
+        Plane< uint32_t >( P, RG_EXT_PLANE_IS_SKY )[ ci ]             = 1u;
+        Plane< float4 >( P, RG_EXT_PLANE_ALBEDO )[ ri ]               = make_float4( sky.x, sky.y, sky.z, 0.0f );
+        Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL )[ ci ]             = EncodeNormal( mk3( 0, 0, 0 ) );
+        Plane< float2 >( P, RG_EXT_PLANE_METALLIC_ROUGHNESS )[ ci ]   = make_float2( 0.f, 0.f );
+        Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD )[ ci ]           = MAX_RAY_LEN * 2.0f;
+        Plane< float4 >( P, RG_EXT_PLANE_MOTION )[ ci ]               = make_float4( mx, my, 0.f, 0.f );
+        Plane< float4 >( P, RG_EXT_PLANE_SURFACE_POSITION )[ ci ]     = make_float4( 1e7f, 1e7f, 1e7f, 1e7f );
+        Plane< float4 >( P, RG_EXT_PLANE_VISIBILITY_BUFFER )[ ci ]    = make_float4( 4294967295.0f, 4294967295.0f, 4294967295.0f, 4294967295.0f );
+        Plane< float4 >( P, RG_EXT_PLANE_VIEW_DIRECTION )[ ci ]       = make_float4( dir.x, dir.y, dir.z, 0.f );
+        Plane< float >( P, RG_EXT_PLANE_DEPTH_GRAD )[ ci ]            = 0.0f;
+        Plane< float4 >( P, RG_EXT_PLANE_THROUGHPUT )[ ci ]           = make_float4( 1.f, 1.f, 1.f, 0.f );
+        return;
+    }
+
+    const HitInfo h = GetHit( sc, p, hit.primitiveId, hit.u, hit.v, org );
+    const GeomRecord& gr = sc.geoms[ h.geom ];
+    const float*      t  = sc.tris + size_t( hit.primitiveId ) * 9;
+    const v3 p0 = mk3( t[ 0 ], t[ 1 ], t[ 2 ] ), p1 = mk3( t[ 3 ], t[ 4 ], t[ 5 ] ), p2 = mk3( t[ 6 ], t[ 7 ], t[ 8 ] );
+
+    // ray differentials on the same triangle (HitInfo.inl:243-268)
+    const v3 dAX = GetRayDir( p, u + 1.0f / float( W ), v );
+    const v3 dAY = GetRayDir( p, u, v + 1.0f / float( H ) );
+    const v3 bAX = RayTriBary( p0, p1, p2, org, dAX );
+    const v3 bAY = RayTriBary( p0, p1, p2, org, dAY );
+    const v3 prevPos = gr.isStatic ? h.position : Xform34( gr.prevFromCur, h.position );
+
+    float    wc, wp, wx, wy;
+    const v3 vCur  = MulM4( p.view, h.position, 1.0f, &wc );
+    const v3 vPrev = MulM4( p.viewPrev, prevPos, 1.0f, &wp );
+    const v3 vAX   = MulM4( p.view, p0 * bAX.x + p1 * bAX.y + p2 * bAX.z, 1.0f, &wx );
+    const v3 vAY   = MulM4( p.view, p0 * bAY.x + p1 * bAY.y + p2 * bAY.z, 1.0f, &wy );
+    float    cwc, cwp;
+    const v3 cCur  = MulM4( p.projection, vCur, wc, &cwc );
+    const v3 cPrev = MulM4( p.projectionPrev, vPrev, wp, &cwp );
+    // dot(projection[2], v): the third column of the matrix, as the reference writes it
+    const float* col2 = p.projection + 8;
+    const float  dAXc = col2[ 0 ] * vAX.x + col2[ 1 ] * vAX.y + col2[ 2 ] * vAX.z + col2[ 3 ] * wx;
+    const float  dAYc = col2[ 0 ] * vAY.x + col2[ 1 ] * vAY.y + col2[ 2 ] * vAY.z + col2[ 3 ] * wy;
+    const float  gx = dAXc - cCur.z, gy = dAYc - cCur.z;
+
+    const float sxc = ( cCur.x / cwc ) * 0.5f + 0.5f, syc = ( cCur.y / cwc ) * 0.5f + 0.5f;
+    const float sxp = ( cPrev.x / cwp ) * 0.5f + 0.5f, syp = ( cPrev.y / cwp ) * 0.5f + 0.5f;
+    const float depthLinear = length( vCur );
+    const float motionZ     = length( vPrev ) - depthLinear;
+
+    Plane< uint32_t >( P, RG_EXT_PLANE_IS_SKY )[ ci ]           = 0u;
+    Plane< float4 >( P, RG_EXT_PLANE_ALBEDO )[ ri ]             = make_float4( h.albedo.x, h.albedo.y, h.albedo.z, 0.0f );
+    Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL )[ ci ]           = EncodeNormal( h.normal );
+    Plane< float2 >( P, RG_EXT_PLANE_METALLIC_ROUGHNESS )[ ci ] = make_float2( h.metallic, h.roughness );
+    Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD )[ ci ]         = depthLinear;
+    Plane< float >( P, RG_EXT_PLANE_DEPTH_GRAD )[ ci ]          = sqrtf( gx * gx + gy * gy );
+    Plane< float4 >( P, RG_EXT_PLANE_MOTION )[ ci ]             = make_float4( sxp - sxc, syp - syc, motionZ, 0.0f );
+    Plane< float4 >( P, RG_EXT_PLANE_SURFACE_POSITION )[ ci ]   = make_float4( h.position.x, h.position.y, h.position.z, __uint_as_float( 1u ) );
+    Plane< uint4 >( P, RG_EXT_PLANE_VISIBILITY_BUFFER )[ ci ]   = make_uint4( 1u, ( h.localPrim << 12 ) | ( h.geom & 0xFFFu ), __float_as_uint( hit.u ), __float_as_uint( hit.v ) );
+    Plane< float4 >( P, RG_EXT_PLANE_VIEW_DIRECTION )[ ci ]     = make_float4( dir.x, dir.y, dir.z, 0.0f );
+    Plane< float4 >( P, RG_EXT_PLANE_THROUGHPUT )[ ci ]         = make_float4( 1.f, 1.f, 1.f, 0.f );
+}
+
+// RtInitialReservoirs.rgen, first half: candidates + directional visibility ray.
+__global__ __launch_bounds__( THREADS ) void k_initial_a( RgExtFrameParams p, FbPtrs P, LightView lv, RgExtRay* rays, uint4* pend )
+{
+    const int x = blockIdx.x * BX + threadIdx.x % BX, y = blockIdx.y * BY + threadIdx.x / BX;
+    const int W = p.renderWidth, H = p.renderHeight;
+    if( x >= W || y >= H )
+    {
+        return;
+    }
+    const size_t  i = size_t( y ) * W + x;
+    const Surface s = FetchSurface( P, W, x, y );
+    if( s.isSky )
+    {
+        Plane< uint4 >( P, RG_EXT_PLANE_RESERVOIRS_INITIAL )[ i ] = packReservoir( emptyReservoir() );
+        rays[ i ]                                                 = NullRay();
+        pend[ i ]                                                 = make_uint4( 0u, 0u, 0u, 0u );
+        return;
+    }
+    const uint32_t seed = randomSeed( x, y, p.frameId );
+    uint32_t       salt = SALT_INITIAL_RESERVOIRS;
+    float          pu, pv;
+    LightPointRnd( seed, &pu, &pv );
+    Reservoir regular, dir;
+    InitialCandidates( p, lv, s, seed, salt, pu, pv, false, regular, dir );
+    Plane< uint4 >( P, RG_EXT_PLANE_RESERVOIRS_INITIAL )[ i ] = packReservoir( regular );
+    const uint4 pd = packReservoir( dir );
+    pend[ i ]      = make_uint4( pd.x, pd.y, pd.z, salt );
+    if( isValid( dir ) )
+    {
+        const LightSample ls = sampleLight( GetLight( lv, 0 ), s.position, pu, pv, p.polyLightSpotlightFactor );
+        rays[ i ]            = ShadowRay( s, ls.position );
+    }
+    else
+    {
+        rays[ i ] = NullRay();
+    }
+}
+
+// Second half: drop an occluded directional sample and combine.
+__global__ __launch_bounds__( THREADS ) void k_initial_b( RgExtFrameParams p, FbPtrs P, const RgExtHit* hits, const uint4* pend )
+{
+    const int x = blockIdx.x * BX + threadIdx.x % BX, y = blockIdx.y * BY + threadIdx.x / BX;
+    const int W = p.renderWidth, H = p.renderHeight;
+    if( x >= W || y >= H )
+    {
+        return;
+    }
+    const size_t i = size_t( y ) * W + x;
+    if( Plane< uint32_t >( P, RG_EXT_PLANE_IS_SKY )[ i ] != 0u )
+    {
+        return;
+    }
+    const uint4 pd  = pend[ i ];
+    Reservoir   dir = unpackReservoir( make_uint4( pd.x, pd.y, pd.z, 0u ) );
+    if( isValid( dir ) && hits[ i ].primitiveId != RG_EXT_MISS )
+    {
+        dir = emptyReservoir();
+    }
+    uint32_t  salt    = pd.w;
+    Reservoir regular = unpackReservoir( Plane< uint4 >( P, RG_EXT_PLANE_RESERVOIRS_INITIAL )[ i ] );
+    const uint32_t seed = randomSeed( x, y, p.frameId );
+    Plane< uint4 >( P, RG_EXT_PLANE_RESERVOIRS_INITIAL )[ i ] = packReservoir( CombineInitial( regular, dir, seed, salt ) );
+}
+
+// testSurfaceForReuse (RaygenCommon.h:488-500)
+__device__ __forceinline__ bool ReuseOk( int lo, int hi, int H, int qx, int qy, float dCur, float dOther, v3 nCur, v3 nOther )
+{
+    return inRenderArea( qx, qy, lo, hi, H ) && ( fabsf( dCur - dOther ) / fabsf( dCur ) < 0.1f ) && ( dot( nCur, nOther ) > 0.5f );
+}
+
+template< typename T >
+__device__ __forceinline__ T LdPlane( const T* base, int W, int H, int x, int y, T zero )
+{
+    return ( x < 0 || y < 0 || x >= W || y >= H ) ? zero : base[ size_t( y ) * W + x ];
+}
+
+// RtRaygenDirect.rgen first half: spatiotemporal reuse, shading, shadow ray.
+__global__ __launch_bounds__( THREADS ) void k_direct_a( RgExtFrameParams p, FbPtrs P, LightView lv, RgExtRay* rays, float4* pendD, float4* pendS )
+{
+    const int x = blockIdx.x * BX + threadIdx.x % BX, y = blockIdx.y * BY + threadIdx.x / BX;
+    const int W = p.renderWidth, H = p.renderHeight;
+    if( x >= W || y >= H )
+    {
+        return;
+    }
+    const size_t  i = size_t( y ) * W + x;
+    const Surface s = FetchSurface( P, W, x, y );
+    rays[ i ]       = NullRay();
+    if( s.isSky )
+    {
+        Plane< uint4 >( P, RG_EXT_PLANE_RESERVOIRS )[ i ] = packReservoir( emptyReservoir() );
+        pendD[ i ] = make_float4( 0.f, 0.f, 0.f, -1.0f ); // w < 0: sky
+        pendS[ i ] = make_float4( 0.f, 0.f, 0.f, 0.f );
+        return;
+    }
+    const uint32_t seed = randomSeed( x, y, p.frameId );
+    float          dist = MAX_RAY_LEN;
+    v3             diff = mk3( 0, 0, 0 ), spec = mk3( 0, 0, 0 );
+    Reservoir      res  = emptyReservoir();
+    if( DirectValid( p, 0 ) )
+    {
+        float pu, pv;
+        LightPointRnd( seed, &pu, &pv );
+        // selectLight_Direct (RaygenCommon.h:503-599)
+        int lo, hi;
+        cbRenderArea( x, W, &lo, &hi );
+        const float4 mot      = Plane< float4 >( P, RG_EXT_PLANE_MOTION )[ i ];
+        const float  depthCur = Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD )[ i ];
+        const float  px = ( ( float( x ) + 0.5f ) * ( 1.0f / ( float( W ) / 2.0f ) ) + mot.x ) * ( float( W ) / 2.0f );
+        const float  py = ( ( float( y ) + 0.5f ) * ( 1.0f / float( H ) ) + mot.y ) * float( H );
+        uint32_t     salt = SALT_LIGHT_CHOOSE_DIRECT;
+        const Reservoir init = unpackReservoir( Plane< uint4 >( P, RG_EXT_PLANE_RESERVOIRS_INITIAL )[ i ] );
+        Reservoir       comb = init;
+        for( uint32_t k = 0; k < p.restirTemporalSamples; k++ )
+        {
+            float r4[ 4 ];
+            rnd8_4( seed, salt++, r4 );
+            const int   qx    = int( floorf( px + ( r4[ 0 ] * 2.0f - 1.0f ) * 2.0f ) );
+            const int   qy    = int( floorf( py + ( r4[ 1 ] * 2.0f - 1.0f ) * 2.0f ) );
+            const float dPrev = LdPlane( Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD, true ), W, H, qx, qy, 0.0f );
+            const v3    nPrev = DecodeNormal( LdPlane( Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL, true ), W, H, qx, qy, 0u ) );
+            if( !ReuseOk( lo, hi, H, qx, qy, depthCur, dPrev - mot.z, s.normal, nPrev ) )
+            {
+                continue;
+            }
+            Reservoir t = unpackReservoir( LdPlane( Plane< uint4 >( P, RG_EXT_PLANE_RESERVOIRS, true ), W, H, qx, qy, make_uint4( 0u, 0u, 0u, 0u ) ) );
+            normalizeReservoir( t, init.M * 20 );
+            float tpCur = 0.0f;
+            if( t.selected != LIGHT_INDEX_NONE )
+            {
+                const uint32_t cur = t.selected < lv.prevCount ? lv.prevToCur[ t.selected ] : 0xFFFFFFFFu;
+                if( cur != 0xFFFFFFFFu && cur != LIGHT_INDEX_NONE )
+                {
+                    tpCur      = targetPdf( s, sampleLight( GetLight( lv, cur ), s.position, pu, pv, p.polyLightSpotlightFactor ), false );
+                    t.selected = cur;
+                }
+            }
+            updateCombinedNewSurf( comb, t, tpCur, rnd16( seed, salt++ ) );
+        }
+        for( uint32_t k = 0; k < p.restirSpatialSamples; k++ )
+        {
+            float r4[ 4 ];
+            rnd8_4( seed, salt++, r4 );
+            const int   qx = x + int( ( r4[ 0 ] * 2.0f - 1.0f ) * 30.0f );
+            const int   qy = y + int( ( r4[ 1 ] * 2.0f - 1.0f ) * 30.0f );
+            const float dO = LdPlane( Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD ), W, H, qx, qy, 0.0f );
+            const v3    nO = DecodeNormal( LdPlane( Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL ), W, H, qx, qy, 0u ) );
+            if( !ReuseOk( lo, hi, H, qx, qy, depthCur, dO, s.normal, nO ) )
+            {
+                continue;
+            }
+            const Reservoir sp = unpackReservoir( LdPlane( Plane< uint4 >( P, RG_EXT_PLANE_RESERVOIRS_INITIAL ), W, H, qx, qy, make_uint4( 0u, 0u, 0u, 0u ) ) );
+            updateCombined( comb, sp, rnd16( seed, salt++ ) );
+        }
+        if( !( comb.weightSum <= 0.0f || comb.selected == LIGHT_INDEX_NONE ) )
+        {
+            res = comb;
+            // traceDirectIllumination (RaygenCommon.h:603-637)
+            const LightSample ls = sampleLight( GetLight( lv, res.selected ), s.position, pu, pv, p.polyLightSpotlightFactor );
+            shadeLight( s, ls, selectedSampleWeight( res ), false, &diff, &spec );
+            if( luminance( diff + spec ) <= 0.0f )
+            {
+                diff = spec = mk3( 0, 0, 0 );
+            }
+            else if( 0u < p.maxBounceShadowsLights )
+            {
+                rays[ i ] = ShadowRay( s, ls.position );
+                dist      = length( ls.position - s.position );
+            }
+        }
+    }
+    Plane< uint4 >( P, RG_EXT_PLANE_RESERVOIRS )[ i ] = packReservoir( res );
+    pendD[ i ] = make_float4( diff.x, diff.y, diff.z, dist );
+    pendS[ i ] = make_float4( spec.x, spec.y, spec.z, 0.0f );
+}
+
+// Second half: visibility, demodulation and the direct-light planes.
+__global__ __launch_bounds__( THREADS ) void k_direct_b( RgExtFrameParams p, FbPtrs P, const RgExtRay* rays, const RgExtHit* hits, const float4* pendD, const float4* pendS )
+{
+    const int x = blockIdx.x * BX + threadIdx.x % BX, y = blockIdx.y * BY + threadIdx.x / BX;
+    const int W = p.renderWidth, H = p.renderHeight;
+    if( x >= W || y >= H )
+    {
+        return;
+    }
+    const size_t i  = size_t( y ) * W + x;
+    const float4 dd = pendD[ i ];
+    if( dd.w < 0.0f )
+    {
+        Plane< float4 >( P, RG_EXT_PLANE_UNFILTERED_DIRECT )[ i ]   = make_float4( 0.f, 0.f, 0.f, 0.f );
+        Plane< float4 >( P, RG_EXT_PLANE_UNFILTERED_SPECULAR )[ i ] = make_float4( 0.f, 0.f, 0.f, 0.f );
+        Plane< float2 >( P, RG_EXT_PLANE_GRADIENT_INPUTS )[ i ]     = make_float2( 0.f, 0.f );
+        return;
+    }
+    const float4 ss   = pendS[ i ];
+    v3           diff = mk3( dd.x, dd.y, dd.z ), spec = mk3( ss.x, ss.y, ss.z );
+    const bool   shadowTested = rays[ i ].tmax >= rays[ i ].tmin;
+    if( shadowTested && hits[ i ].primitiveId != RG_EXT_MISS )
+    {
+        diff = spec = mk3( 0, 0, 0 );
+    }
+    const float2 mr     = Plane< float2 >( P, RG_EXT_PLANE_METALLIC_ROUGHNESS )[ i ];
+    int          rx, ry;
+    cbToRegular( x, y, W, &rx, &ry );
+    const float4 alb    = Plane< float4 >( P, RG_EXT_PLANE_ALBEDO )[ size_t( ry ) * W + rx ];
+    const v3     sc     = SpecularColor( mk3( alb.x, alb.y, alb.z ), mr.x );
+    const v3     demod  = mk3( spec.x / fmaxf( 0.01f, sc.x ), spec.y / fmaxf( 0.01f, sc.y ), spec.z / fmaxf( 0.01f, sc.z ) );
+    float4&      vd     = Plane< float4 >( P, RG_EXT_PLANE_VIEW_DIRECTION )[ i ];
+    Plane< float4 >( P, RG_EXT_PLANE_UNFILTERED_DIRECT )[ i ]   = make_float4( diff.x, diff.y, diff.z, 0.f );
+    Plane< float4 >( P, RG_EXT_PLANE_UNFILTERED_SPECULAR )[ i ] = make_float4( demod.x, demod.y, demod.z, 0.f );
+    vd.w                                                        = dd.w;
+    Plane< float2 >( P, RG_EXT_PLANE_GRADIENT_INPUTS )[ i ]     = make_float2( luminance( diff ), luminance( spec ) );
+}
+
+// Pending state of the indirect pass, 6 float4 per pixel:
+//  0: bounce-1 direction, 1/pdf   1: hit position, flags (1 = sky)
+//  2: hit normal, directional?    3: hit albedo, -
+//  4: unshadowed direct at the hit + emission, -   5: bounce-2 direction, 1/pdf
+struct IndirPend
+{
+    float4* a;
+    size_t  n;
+    __device__ float4& at( int k, size_t i ) const { return a[ size_t( k ) * n + i ]; }
+};
+
+// RT_RAYGEN_INDIRECT_INIT: specular (GGX-VNDF) bounce from the primary surface.
+__global__ __launch_bounds__( THREADS ) void k_indirect_a( RgExtFrameParams p, FbPtrs P, LightView lv, RgExtRay* rays, IndirPend pend )
+{
+    const int x = blockIdx.x * BX + threadIdx.x % BX, y = blockIdx.y * BY + threadIdx.x / BX;
+    const int W = p.renderWidth, H = p.renderHeight;
+    if( x >= W || y >= H )
+    {
+        return;
+    }
+    const size_t i = size_t( y ) * W + x;
+    Surface      s = FetchSurface( P, W, x, y );
+    if( s.isSky )
+    {
+        rays[ i ]        = NullRay();
+        pend.at( 0, i ) = make_float4( 0.f, 0.f, 0.f, -1.0f );
+        return;
+    }
+    s.position          = s.position + s.toViewerDir * 0.01f; // RAY_ORIGIN_LEAK_BIAS
+    const uint32_t seed = randomSeed( x, y, p.frameId );
+    float          bn[ 4 ];
+    BlueNoise8( lv.blueNoise, seed, SALT_SPEC_BOUNCE( 1 ), bn );
+    float    oneOverPdf;
+    const v3 d = sampleSmithGGX( s.normal, s.toViewerDir, s.roughness, bn[ 0 ], bn[ 1 ], &oneOverPdf, true );
+    rays[ i ]  = MakeRay( s.position + s.normal * 0.01f, d, 0.001f, p.rayLength );
+    pend.at( 0, i ) = make_float4( d.x, d.y, d.z, oneOverPdf );
+}
+
+// Direct light at a bounce hit (processDirectIllumination for
+// LIGHT_SAMPLE_METHOD_INDIR, RaygenCommon.h:701-741): RIS without
+// visibility; the shadow ray is traced if bounce < maxBounceShadows.
+__device__ void IndirectDirect( const RgExtFrameParams& p, const LightView& lv, const Surface& hs, uint32_t seed, int bounce,
+                                v3* diffOut, bool* isDirectional, bool* needShadow, v3* lightPos )
+{
+    *diffOut       = mk3( 0, 0, 0 );
+    *needShadow    = false;
+    *isDirectional = false;
+    if( !DirectValid( p, bounce ) )
+    {
+        return;
+    }
+    float pu, pv;
+    LightPointRnd( seed, &pu, &pv );
+    uint32_t  salt = SALT_LIGHT_CHOOSE_INDIRECT;
+    Reservoir regular, dir;
+    InitialCandidates( p, lv, hs, seed, salt, pu, pv, true, regular, dir );
+    const Reservoir r = CombineInitial( regular, dir, seed, salt );
+    if( !isValid( r ) )
+    {
+        return;
+    }
+    const LightSample ls = sampleLight( GetLight( lv, r.selected ), hs.position, pu, pv, p.polyLightSpotlightFactor );
+    v3                d, sp;
+    shadeLight( hs, ls, selectedSampleWeight( r ), true, &d, &sp );
+    if( luminance( d + sp ) <= 0.0f )
+    {
+        return;
+    }
+    *diffOut       = d;
+    *isDirectional = r.selected == 0u;
+    *needShadow    = uint32_t( bounce ) < p.maxBounceShadowsLights;
+    *lightPos      = ls.position;
+}
+
+// Bounce-1 hit: direct light at the hit (shadow ray) and the second bounce.
+__global__ __launch_bounds__( THREADS ) void k_indirect_b( RgExtFrameParams p, FbPtrs P, SceneView sc, LightView lv, const RgExtRay* rays1, const RgExtHit* hits1,
+                                                           RgExtRay* shadowRays, RgExtRay* bounceRays, IndirPend pend )
+{
+    const int x = blockIdx.x * BX + threadIdx.x % BX, y = blockIdx.y * BY + threadIdx.x / BX;
+    const int W = p.renderWidth, H = p.renderHeight;
+    if( x >= W || y >= H )
+    {
+        return;
+    }
+    const size_t i = size_t( y ) * W + x;
+    shadowRays[ i ] = NullRay();
+    bounceRays[ i ] = NullRay();
+    if( pend.at( 0, i ).w < 0.0f )
+    {
+        return;
+    }
+    const RgExtRay r   = rays1[ i ];
+    const RgExtHit hit = hits1[ i ];
+    const v3       dir = mk3( r.direction[ 0 ], r.direction[ 1 ], r.direction[ 2 ] );
+    const v3       org = mk3( r.origin[ 0 ], r.origin[ 1 ], r.origin[ 2 ] );
+    if( hit.primitiveId == RG_EXT_MISS )
+    {
+        pend.at( 1, i ) = make_float4( 0.f, 0.f, 0.f, 1.0f ); // sky
+        return;
+    }
+    const HitInfo  h    = GetHit( sc, p, hit.primitiveId, hit.u, hit.v, org );
+    const Surface  hs   = HitToSurface( h, dir );
+    const uint32_t seed = randomSeed( x, y, p.frameId );
+    v3             diff, lightPos = mk3( 0, 0, 0 );
+    bool           isDir, needShadow;
+    IndirectDirect( p, lv, hs, seed, 1, &diff, &isDir, &needShadow, &lightPos );
+    if( needShadow )
+    {
+        shadowRays[ i ] = ShadowRay( hs, lightPos );
+    }
+    const v3 emis = h.emission * p.emissionMapBoost;
+    // second diffuse bounce (getDiffuseBounce, RtRaygenIndirect.inl:57-62)
+    float bn[ 4 ];
+    BlueNoise8( lv.blueNoise, seed, SALT_DIFF_BOUNCE( 2 ), bn );
+    float    pdf2;
+    const v3 d2     = sampleOrientedHemisphere( hs.normal, bn[ 0 ], bn[ 1 ], &pdf2 );
+    bounceRays[ i ] = MakeRay( hs.position + hs.normal * 0.01f, d2, 0.001f, p.rayLength );
+    pend.at( 1, i ) = make_float4( h.position.x, h.position.y, h.position.z, 0.0f );
+    pend.at( 2, i ) = make_float4( h.normal.x, h.normal.y, h.normal.z, isDir ? 1.0f : 0.0f );
+    pend.at( 3, i ) = make_float4( h.albedo.x, h.albedo.y, h.albedo.z, needShadow ? 1.0f : 0.0f );
+    pend.at( 4, i ) = make_float4( diff.x, diff.y, diff.z, 0.0f );
+    pend.at( 5, i ) = make_float4( d2.x, d2.y, d2.z, pdf2 );
+    ( void )emis;
+}
+
+// Second-bounce hit and the final indirect sample; shading of the primary
+// surface with it (RtRaygenIndirect.inl:66-161, :402-433 without reuse).
+__global__ __launch_bounds__( THREADS ) void k_indirect_c( RgExtFrameParams p, FbPtrs P, SceneView sc, LightView lv, const RgExtHit* shadowHits,
+                                                           const RgExtRay* bounceRays, const RgExtHit* bounceHits, const RgExtHit* hits1, IndirPend pend )
+{
+    const int x = blockIdx.x * BX + threadIdx.x % BX, y = blockIdx.y * BY + threadIdx.x / BX;
+    const int W = p.renderWidth, H = p.renderHeight;
+    if( x >= W || y >= H )
+    {
+        return;
+    }
+    const size_t i  = size_t( y ) * W + x;
+    const float4 b1 = pend.at( 0, i );
+    if( b1.w < 0.0f )
+    {
+        Plane< float4 >( P, RG_EXT_PLANE_UNFILTERED_INDIR )[ i ] = make_float4( 0.f, 0.f, 0.f, 0.f );
+        return;
+    }
+    Surface s  = FetchSurface( P, W, x, y );
+    s.position = s.position + s.toViewerDir * 0.01f;
+    const v3 dir1 = mk3( b1.x, b1.y, b1.z );
+    v3       samplePos, sampleRad;
+    const float4 hp = pend.at( 1, i );
+    if( hp.w > 0.5f )
+    {
+        // sky sample (RtRaygenIndirect.inl:123-131)
+        samplePos = s.position + dir1 * MAX_RAY_LEN;
+        sampleRad = SkyColor( p );
+    }
+    else
+    {
+        const float4 hn = pend.at( 2, i ), ha = pend.at( 3, i ), hd = pend.at( 4, i ), b2 = pend.at( 5, i );
+        v3           diffuse = mk3( hd.x, hd.y, hd.z );
+        if( ha.w > 0.5f && shadowHits[ i ].primitiveId != RG_EXT_MISS )
+        {
+            diffuse = mk3( 0, 0, 0 );
+        }
+        if( hn.w < 0.5f )
+        {
+            // SHIPPING_HACK: non-directional light at a bounce is clamped to 20
+            diffuse = mk3( clampf( diffuse.x, 0.0f, 20.0f ), clampf( diffuse.y, 0.0f, 20.0f ), clampf( diffuse.z, 0.0f, 20.0f ) );
+        }
+        // emission of the bounce-1 hit
+        const RgExtHit h1   = hits1[ i ];
+        const HitInfo  hi1  = GetHit( sc, p, h1.primitiveId, h1.u, h1.v, s.position );
+        const v3       emis = hi1.emission * p.emissionMapBoost;
+        // processSecondDiffuseBounce (RtRaygenIndirect.inl:82-101)
+        const v3       d2 = mk3( b2.x, b2.y, b2.z );
+        const RgExtHit h2 = bounceHits[ i ];
+        v3             second;
+        if( h2.primitiveId == RG_EXT_MISS )
+        {
+            second = SkyColor( p ) * b2.w;
+        }
+        else
+        {
+            const RgExtRay br   = bounceRays[ i ];
+            const HitInfo  hi2  = GetHit( sc, p, h2.primitiveId, h2.u, h2.v, mk3( br.origin[ 0 ], br.origin[ 1 ], br.origin[ 2 ] ) );
+            const Surface  hs2  = HitToSurface( hi2, d2 );
+            const uint32_t seed = randomSeed( x, y, p.frameId );
+            v3             diff2, lp;
+            bool           isDir2, needShadow2;
+            // direct light at the second bounce, unshadowed (a shadow ray is
+            // only requested there when maxBounceShadows > 2)
+            IndirectDirect( p, lv, hs2, seed, 2, &diff2, &isDir2, &needShadow2, &lp );
+            if( !isDir2 )
+            {
+                diff2 = mk3( clampf( diff2.x, 0.0f, 20.0f ), clampf( diff2.y, 0.0f, 20.0f ), clampf( diff2.z, 0.0f, 20.0f ) );
+            }
+            second = ( hi2.emission * p.emissionMapBoost + diff2 ) * hi2.albedo * b2.w;
+        }
+        diffuse   = diffuse + second;
+        samplePos = mk3( hp.x, hp.y, hp.z );
+        sampleRad = ( emis + diffuse ) * mk3( ha.x, ha.y, ha.z );
+    }
+    // shade (RtRaygenIndirect.inl:164-182) with weight 1/pdf of the bounce
+    v3          diff = mk3( 0, 0, 0 ), spec = mk3( 0, 0, 0 );
+    const v3    l    = safeNormalize( samplePos - s.position );
+    const float nl   = dot( s.normal, l );
+    if( nl > 0.0f )
+    {
+        diff = sampleRad * ( nl * ( 1.0f / RG_PI ) * b1.w );
+        spec = ( sampleRad * nl ) * evalBRDFSmithGGX( s.normal, s.toViewerDir, l, s.roughness, s.specularColor, true ) * b1.w;
+    }
+    float4&      us     = Plane< float4 >( P, RG_EXT_PLANE_UNFILTERED_SPECULAR )[ i ];
+    const v3     direct = mk3( us.x, us.y, us.z );
+    if( luminance( direct ) < luminance( spec ) )
+    {
+        Plane< float4 >( P, RG_EXT_PLANE_VIEW_DIRECTION )[ i ].w = length( samplePos - s.position );
+    }
+    const v3 sc2 = s.specularColor;
+    us           = make_float4( direct.x + spec.x / fmaxf( 0.01f, sc2.x ), direct.y + spec.y / fmaxf( 0.01f, sc2.y ), direct.z + spec.z / fmaxf( 0.01f, sc2.z ), 0.0f );
+    Plane< float4 >( P, RG_EXT_PLANE_UNFILTERED_INDIR )[ i ] = make_float4( diff.x, diff.y, diff.z, 0.0f );
+}
+
+} // namespace
+
+void PathTracer::Init( hipStream_t s )
+{
+    blueNoise.Reserve( 128u * 128u * 128u * 4u );
+    k_blue_noise<<< DivUp( 128u * 128u * 128u, THREADS ), THREADS, 0, s >>>( blueNoise.as< uint32_t >() );
+    CheckLaunch( "k_blue_noise" );
+    for( int k = 0; k < 2; k++ )
+    {
+        grid[ k ].Reserve( size_t( GRID_ENTRIES ) * 16 );
+        // empty cells: selected = LIGHT_INDEX_NONE
+        std::vector< uint32_t > init( size_t( GRID_ENTRIES ) * 4, 0u );
+        for( size_t e = 0; e < GRID_ENTRIES; e++ )
+        {
+            init[ e * 4 ] = LIGHT_INDEX_NONE;
+        }
+        RG_HIP_CHECK( hipMemcpyAsync( grid[ k ].ptr, init.data(), init.size() * 4, hipMemcpyHostToDevice, s ) );
+        RG_HIP_CHECK( hipStreamSynchronize( s ) );
+    }
+}
+
+void PathTracer::Resize( uint32_t w, uint32_t h )
+{
+    if( w == W && h == H )
+    {
+        return;
+    }
+    W                = w;
+    H                = h;
+    const size_t n   = size_t( w ) * h;
+    rays.Reserve( n * sizeof( RgExtRay ) );
+    rays2.Reserve( n * sizeof( RgExtRay ) );
+    hits.Reserve( n * sizeof( RgExtHit ) );
+    hits2.Reserve( n * sizeof( RgExtHit ) );
+    pendA.Reserve( n * 16 * 6 );
+    pendB.Reserve( n * sizeof( RgExtRay ) );
+    pendC.Reserve( n * sizeof( RgExtHit ) );
+}
+
+void PathTracer::UploadLights( const SceneCollector& sc, hipStream_t s )
+{
+    lightArraySize = uint32_t( sc.lights.size() );
+    lightCount     = lightArraySize - 1;
+    lights.Reserve( sc.lights.size() * sizeof( LightEncoded ) );
+    prevToCur.Reserve( sc.prevToCur.size() * 4 + 4 );
+    prevToCurCount = uint32_t( sc.prevToCur.size() );
+    RG_HIP_CHECK( hipMemcpyAsync( lights.ptr, sc.lights.data(), sc.lights.size() * sizeof( LightEncoded ), hipMemcpyHostToDevice, s ) );
+    RG_HIP_CHECK( hipMemcpyAsync( prevToCur.ptr, sc.prevToCur.data(), sc.prevToCur.size() * 4, hipMemcpyHostToDevice, s ) );
+}
+
+uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, TraceContext& tc, const DeviceScene& scene,
+                            FramebufferSet& fb, uint32_t traceFlags, hipStream_t s, FrameTimings* timing )
+{
+    Resize( p.renderWidth, p.renderHeight );
+    const uint32_t n = W * H;
+    const dim3     grid2( DivUp( W, BX ), DivUp( H, BY ) );
+    const FbPtrs   P = fb.Ptrs();
+    SceneView      sv{ scene.tris.as< float >(), scene.triNormals.as< float >(), scene.triGeom.as< uint32_t >(), scene.geoms.as< GeomRecord >() };
+    LightView      lv{ lights.as< LightEncoded >(), lightArraySize, prevToCur.as< uint32_t >(), prevToCurCount,
+                  grid[ gridParity ].as< uint4 >(), grid[ gridParity ^ 1 ].as< uint4 >(), blueNoise.as< uint32_t >() };
+    RgExtRay* R  = rays.as< RgExtRay >();
+    RgExtHit* Hh = hits.as< RgExtHit >();
+    RgExtRay* R2 = rays2.as< RgExtRay >();
+    RgExtHit* H2 = hits2.as< RgExtHit >();
+    uint64_t  traced = 0;
+    const bool haveGeometry = scene.triCount > 0 && bvh.nodeCount > 0;
+
+    auto trace = [ & ]( RgExtRay* rr, RgExtHit* hh, uint32_t flags ) {
+        if( haveGeometry )
+        {
+            TraceRays( bvh, tc, rr, n, hh, flags | traceFlags, s );
+        }
+        else
+        {
+            // no geometry: every ray misses
+            RG_HIP_CHECK( hipMemsetAsync( hh, 0xFF, size_t( n ) * sizeof( RgExtHit ), s ) );
+        }
+        traced += n;
+    };
+
+    // primary
+    k_primary_rays<<< grid2, THREADS, 0, s >>>( p, R );
+    CheckLaunch( "k_primary_rays" );
+    trace( R, Hh, RG_EXT_TRACE_CLOSEST );
+    k_primary_shade<<< grid2, THREADS, 0, s >>>( p, Hh, sv, P );
+    CheckLaunch( "k_primary_shade" );
+
+    // light grid (current grid = gridParity, previous = the other one)
+    if( p.lightCount > 0 )
+    {
+        lv.grid     = grid[ gridParity ^ 1 ].as< uint4 >();
+        lv.gridPrev = grid[ gridParity ].as< uint4 >();
+        k_light_grid<<< DivUp( GRID_ENTRIES, THREADS ), THREADS, 0, s >>>( p, lv, grid[ gridParity ^ 1 ].as< uint4 >() );
+        CheckLaunch( "k_light_grid" );
+        gridParity ^= 1;
+        lv.grid     = grid[ gridParity ].as< uint4 >();
+        lv.gridPrev = grid[ gridParity ^ 1 ].as< uint4 >();
+    }
+
+    // initial reservoirs (directional visibility)
+    k_initial_a<<< grid2, THREADS, 0, s >>>( p, P, lv, R, pendA.as< uint4 >() );
+    CheckLaunch( "k_initial_a" );
+    trace( R, Hh, RG_EXT_TRACE_ANY_HIT );
+    k_initial_b<<< grid2, THREADS, 0, s >>>( p, P, Hh, pendA.as< uint4 >() );
+    CheckLaunch( "k_initial_b" );
+
+    // direct illumination
+    float4* pd = pendA.as< float4 >();
+    float4* ps = pendA.as< float4 >() + n;
+    k_direct_a<<< grid2, THREADS, 0, s >>>( p, P, lv, R, pd, ps );
+    CheckLaunch( "k_direct_a" );
+    trace( R, Hh, RG_EXT_TRACE_ANY_HIT );
+    k_direct_b<<< grid2, THREADS, 0, s >>>( p, P, R, Hh, pd, ps );
+    CheckLaunch( "k_direct_b" );
+
+    // indirect: bounce 1, then the shadow ray at its hit and bounce 2
+    IndirPend ip{ pendA.as< float4 >(), n };
+    RgExtRay* shadowR = pendB.as< RgExtRay >();
+    RgExtHit* shadowH = pendC.as< RgExtHit >();
+    k_indirect_a<<< grid2, THREADS, 0, s >>>( p, P, lv, R, ip );
+    CheckLaunch( "k_indirect_a" );
+    trace( R, Hh, RG_EXT_TRACE_CLOSEST );
+    k_indirect_b<<< grid2, THREADS, 0, s >>>( p, P, sv, lv, R, Hh, shadowR, R2, ip );
+    CheckLaunch( "k_indirect_b" );
+    trace( shadowR, shadowH, RG_EXT_TRACE_ANY_HIT );
+    trace( R2, H2, RG_EXT_TRACE_CLOSEST );
+    k_indirect_c<<< grid2, THREADS, 0, s >>>( p, P, sv, lv, shadowH, R2, H2, Hh, ip );
+    CheckLaunch( "k_indirect_c" );
+    ( void )timing;
+    return traced;
+}
+
+} // namespace rg

@@ -1,0 +1,147 @@
+// Per-frame geometry and lights of an instance, as collected by
+// rgUploadMeshPrimitive / rgUpload*Light and assembled on the device.
+#pragma once
+
+#include "rg_internal.h"
+
+#include <array>
+#include <unordered_map>
+#include <vector>
+
+namespace rg
+{
+
+// One uploaded primitive (the reference's ShGeometryInstance,
+// Source/Generated/ShaderCommonC.h:306-342, reduced to what the path reads).
+struct GeomRecord
+{
+    float    model[ 12 ];       // row-major 3x4 (RgTransform)
+    float    prevFromCur[ 12 ]; // row-major 3x4: prevModel * model^-1 (identity if new)
+    uint32_t baseVertex;
+    uint32_t baseIndex;         // 0xFFFFFFFF: non-indexed
+    uint32_t firstTri;
+    uint32_t triCount;
+    uint32_t color;             // RgColor4DPacked32, albedo layer 0 (HitInfo.inl:126-127)
+    float    emissive;          // saturate(emissive) (VertexCollector.cpp:328)
+    float    roughness;         // default 1 (VertexCollector.cpp:325)
+    float    metallic;          // default 0 (VertexCollector.cpp:326)
+    uint32_t flags;             // RgMeshPrimitiveFlags
+    uint32_t isStatic;          // prevFromCur is the identity
+    uint32_t pad[ 2 ];
+};
+static_assert( sizeof( GeomRecord ) == 144, "GeomRecord" );
+
+// ShLightEncoded (Source/Generated/ShaderCommonC.h:350-357), 64 bytes.
+struct LightEncoded
+{
+    float    color[ 3 ];
+    uint32_t lightType;
+    float    data0[ 4 ];
+    float    data1[ 4 ];
+    float    data2[ 4 ];
+};
+static_assert( sizeof( LightEncoded ) == 64, "LightEncoded" );
+
+constexpr uint32_t LIGHT_TYPE_NONE        = 0;
+constexpr uint32_t LIGHT_TYPE_DIRECTIONAL = 1;
+constexpr uint32_t LIGHT_TYPE_SPHERE      = 2;
+constexpr uint32_t LIGHT_TYPE_TRIANGLE    = 3;
+constexpr uint32_t LIGHT_TYPE_SPOT        = 4;
+constexpr uint32_t LIGHT_INDEX_NONE       = 32767;
+
+// Pinned, growable host array for staging uploads.
+template< typename T >
+struct PinnedArray
+{
+    T*     data     = nullptr;
+    size_t size     = 0;
+    size_t capacity = 0;
+
+    PinnedArray() = default;
+    PinnedArray( const PinnedArray& ) = delete;
+    PinnedArray& operator=( const PinnedArray& ) = delete;
+    ~PinnedArray()
+    {
+        if( data )
+        {
+            ( void )hipHostFree( data );
+        }
+    }
+    void Clear() { size = 0; }
+    void Reserve( size_t n )
+    {
+        if( n <= capacity )
+        {
+            return;
+        }
+        size_t cap = capacity ? capacity : 1024;
+        while( cap < n )
+        {
+            cap *= 2;
+        }
+        T* nd = nullptr;
+        RG_HIP_CHECK( hipHostMalloc( reinterpret_cast< void** >( &nd ), cap * sizeof( T ), hipHostMallocDefault ) );
+        if( data )
+        {
+            std::memcpy( nd, data, size * sizeof( T ) );
+            ( void )hipHostFree( data );
+        }
+        data     = nd;
+        capacity = cap;
+    }
+    T* Append( size_t n )
+    {
+        Reserve( size + n );
+        T* p = data + size;
+        size += n;
+        return p;
+    }
+};
+
+struct SceneCollector
+{
+    PinnedArray< RgPrimitiveVertex > vertices;
+    PinnedArray< uint32_t >          indices;
+    PinnedArray< GeomRecord >        geoms;
+    uint32_t                         triCount = 0;
+
+    // uniqueObjectID << 32 | primitiveIndexInMesh -> model of the previous frame
+    // (UniqueID::MakeForPrimitive, Source/UniqueID.h:29-33)
+    std::unordered_map< uint64_t, std::array< float, 12 > > prevModels, curModels;
+
+    // lights of this and the previous frame, matched by uniqueID
+    std::vector< LightEncoded >                 lights;      // [0] directional (or NONE), then regular
+    std::vector< uint64_t >                     lightIds;
+    bool                                        hasDirectional = false;
+    std::unordered_map< uint64_t, uint32_t >    prevLightIndex; // uniqueID -> index last frame
+    std::vector< uint32_t >                     prevToCur;      // last frame's index -> this frame's
+    uint32_t                                    prevLightCount = 0;
+
+    void BeginFrame();
+    void AddPrimitive( const RgMeshInfo& mesh, const RgMeshPrimitiveInfo& prim );
+    void AddDirectional( const RgDirectionalLightUploadInfo& info );
+    void AddSphere( const RgSphericalLightUploadInfo& info );
+    void AddSpot( const RgSpotLightUploadInfo& info );
+    void AddPolygonal( const RgPolygonalLightUploadInfo& info );
+    // Builds prevToCur; call once all lights of the frame are added.
+    void FinishLights();
+};
+
+// Device side of the assembled frame geometry.
+struct DeviceScene
+{
+    DeviceBuffer vertices, indices, geoms;
+    DeviceBuffer tris;       // n * 9 floats, world space
+    DeviceBuffer triNormals; // n * 9 floats, world-space vertex normals
+    DeviceBuffer triGeom;    // n u32
+    DeviceBuffer trisPrev;   // last built copy, to skip identical rebuilds
+    DeviceBuffer diffFlag;   // 1 u32
+    uint32_t     triCount = 0, geomCount = 0, builtTriCount = 0;
+    bool         bvhValid = false;
+};
+
+// Uploads the collected geometry and assembles world-space triangles.
+// Returns true if the triangles differ from the last BVH build.
+bool AssembleScene( SceneCollector& sc, DeviceScene& ds, hipStream_t s );
+
+} // namespace rg

@@ -81,7 +81,7 @@ class RgExtFrameParams(C.Structure):
         ("minRoughness", C.c_float), ("polyLightSpotlightFactor", C.c_float), ("atrousIterations", C.c_uint32),
         ("restirInitialCandidates", C.c_uint32),
         ("restirTemporalSamples", C.c_uint32), ("restirSpatialSamples", C.c_uint32), ("debugShowFlags", C.c_uint32),
-        ("_pad0", C.c_uint32),
+        ("emissionMapBoost", C.c_float),
     ]
 
 
@@ -388,3 +388,14 @@ class FramebufferSet:
             lib().rgExtFramebuffersDestroy(self.h)
         except Exception:
             pass
+
+
+class FramebufferView(FramebufferSet):
+    """Non-owning view of an instance's framebuffer set (rgExtGetFramebuffers)."""
+
+    def __init__(self, handle: int, width: int, height: int):
+        self.h = handle
+        self.width, self.height = width, height
+
+    def __del__(self):
+        pass
