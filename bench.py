@@ -1,0 +1,247 @@
+#!/usr/bin/env python3
+"""Frames/s and Mrays/s of the full ray-traced 1080p frame on one MI355X.
+
+Workload: 1920x1080, a 999,168-triangle scene (512x512-cell fBm heightfield
+plus 500 icospheres of subdivision 2-5), 64 sphere lights, the 60-frame
+camera orbit (radius 80, height 25, 6 degrees per frame).  One step is one
+rgDrawFrame through the RTGL1 C API: primary G-buffer, light grid, ReSTIR DI
+(32 initial candidates, 1 temporal + 1 spatial sample) with shadow rays, two
+diffuse bounces, A-SVGF with 5 a-trous iterations and eye adaptation.  The
+geometry is uploaded once before the timed region and stays resident in HBM
+(static geometry); lights and camera are passed every frame.
+
+Prints one JSON line.  `roofline` prices the dominant kernel (the traversal
+kernel k_trace) by algorithmic bytes: 32 B ray in + 16 B hit out per ray,
+256 B per BVH8 node visit, 36 B per triangle test, divided by the kernel time
+measured with HIP events recorded on the instance stream around every launch.
+`cpu_baseline` times the CPU oracle on a bounded sample of the same frame.
+
+Usage: python bench.py [--gpus 1] [--steps 60] [--warmup 5] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+if str(REPO) not in sys.path:
+    sys.path.insert(0, str(REPO))
+
+METRIC = "Mrays/s & 1080p frames/s (FP16 vs FP32); p99 frame latency at 256 sessions"
+WIDTH, HEIGHT = 1920, 1080
+ORBIT_FRAMES = 60
+BYTES_PER_RAY = 32 + 16      # RgExtRay in, RgExtHit out
+BYTES_PER_NODE = 256         # one FP32 BVH8 node
+BYTES_PER_TRI = 36           # three FP32 vertices
+HBM_PEAK_GBPS = 8000.0       # MI355X HBM3E
+TRAFFIC_FILE = REPO / "profiles" / "k_trace_traffic.json"
+DENOISE_SAMPLE = (1280, 720)
+
+
+def parse_args(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle sample")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (default: min(16, cpus))")
+    return ap.parse_args(argv)
+
+
+def bounce_rays(scene, rays, hits, seed=7):
+    """One cosine-weighted diffuse bounce ray per primary hit (numpy)."""
+    import numpy as np
+    from raytracedgl1_amd.ffi import RAY_DTYPE, RG_EXT_MISS
+    h = hits["prim"] != RG_EXT_MISS
+    tri = scene.tris.reshape(-1, 3, 3)[hits["prim"][h]].astype(np.float64)
+    n = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    n /= np.linalg.norm(n, axis=1, keepdims=True) + 1e-30
+    d_in = rays["direction"][h].astype(np.float64)
+    n *= -np.sign(np.sum(n * d_in, axis=1, keepdims=True) + 1e-30)
+    p = rays["origin"][h] + d_in * hits["t"][h, None]
+    r = np.random.default_rng(seed)
+    u1, u2 = r.random(len(n)), r.random(len(n))
+    a = np.where(np.abs(n[:, :1]) > 0.9, np.array([[0.0, 1.0, 0.0]]), np.array([[1.0, 0.0, 0.0]]))
+    t = np.cross(n, a)
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    b = np.cross(n, t)
+    phi, rr = 2 * np.pi * u1, np.sqrt(u2)
+    d = t * (rr * np.cos(phi))[:, None] + b * (rr * np.sin(phi))[:, None] + n * np.sqrt(1 - u2)[:, None]
+    out = np.zeros(len(d), RAY_DTYPE)
+    out["origin"] = p + n * 1e-3
+    out["direction"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    out["tmin"] = 0.0
+    out["tmax"] = 10000.0
+    return out
+
+
+def cpu_baseline(scene, rays_per_frame: float, threads: int) -> dict:
+    """The CPU oracle on a bounded sample of one frame: traversal of a
+    frame's primary rays plus one diffuse bounce per hit on `threads`
+    threads, and the FP64 A-SVGF chain (5 a-trous iterations, one thread) at
+    1280x720; a frame's CPU time is rays_per_frame at the sampled traversal
+    rate plus the denoiser time scaled to 1920x1080 by pixel count."""
+    sys.path.insert(0, str(REPO / "tests"))
+    import denoise_inputs
+    import oracle_lib
+    from raytracedgl1_amd import ffi, scenes
+
+    o = oracle_lib.Oracle()
+    bvh = o.bvh(scene.tris, 3)  # not timed: the GPU frames reuse a resident BVH as well
+    prim = scenes.primary_rays(scenes.orbit_camera(0), WIDTH, HEIGHT)
+    t0 = time.perf_counter()
+    hits = bvh.trace(prim, 0, threads)
+    t1 = time.perf_counter()
+    sec = bounce_rays(scene, prim, hits)
+    t2 = time.perf_counter()
+    bvh.trace(sec, 0, threads)
+    t3 = time.perf_counter()
+    nrays = len(prim) + len(sec)
+    rate = nrays / ((t1 - t0) + (t3 - t2))
+
+    dw, dh = DENOISE_SAMPLE
+    cur, prev = denoise_inputs.make_gbuffer(dw, dh, 1)
+    fb = oracle_lib.OracleFramebuffers(o, dw, dh)
+    denoise_inputs.fill_planes(cur, prev, None, fb)
+    p = ffi.frame_params(dw, dh)
+    p.atrousIterations = 5
+    t4 = time.perf_counter()
+    fb.run("", p)
+    den = time.perf_counter() - t4
+
+    frame_s = rays_per_frame / rate + den * (WIDTH * HEIGHT) / (dw * dh)
+    return {
+        "value": 1.0 / frame_s, "unit": "frames/s", "cores": threads, "kind": "port",
+        "sample": (f"oracle traversal of {len(prim):,} primary + {len(sec):,} bounce rays of orbit frame 0 on "
+                   f"{threads} threads ({rate / 1e6:.3f} Mrays/s), priced at {rays_per_frame / 1e6:.2f} M rays/frame; "
+                   f"oracle FP64 A-SVGF (5 a-trous iterations) at {dw}x{dh} on 1 thread, {den:.2f} s, "
+                   f"scaled to 1920x1080; shading and ReSTIR not included"),
+        "mrays_per_s": rate / 1e6,
+        "seconds": (t1 - t0) + (t3 - t2) + den,
+    }
+
+
+def load_traffic():
+    try:
+        d = json.loads(TRAFFIC_FILE.read_text())
+        return float(d["traffic_bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def main(argv=None) -> int:
+    a = parse_args(argv)
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print(f"bench.py runs on exactly one GPU (asked for --gpus {a.gpus}, WORLD_SIZE {world}): this build has "
+              "no multi-GPU path; frames and sessions would shard across GPUs one process each with no "
+              "exchange, so the whole-job rate is N times the one-GPU line.", file=sys.stderr)
+        return 2
+    if a.steps < 1:
+        print("--steps must be >= 1", file=sys.stderr)
+        return 2
+
+    from raytracedgl1_amd import ffi, rtgl, scenes
+
+    scene = scenes.terrain_spheres_scene()
+    up = rtgl.SceneUploader(scene)
+    r = rtgl.Renderer(WIDTH, HEIGHT)
+    r.set_static_geometry(True)
+    r.path.atrousIterations = 5
+    r.path.restirInitialCandidates = 32
+    r.path.restirTemporalSamples = 1
+    r.path.restirSpatialSamples = 1
+    r.path.collectTraversalStats = 1
+
+    frame = [0]
+
+    def step(geometry=False):
+        st = r.frame(up, scenes.orbit_camera(frame[0] % ORBIT_FRAMES), geometry=geometry)
+        frame[0] += 1
+        return st
+
+    # warm-up; the first step uploads the scene, which then stays resident
+    for i in range(max(1, a.warmup)):
+        step(geometry=(i == 0))
+        print(f"warmup {i + 1}/{max(1, a.warmup)}", file=sys.stderr, flush=True)
+
+    ffi.call("rgExtSynchronize")
+    t0 = time.perf_counter()
+    stats = [step() for _ in range(a.steps)]
+    ffi.call("rgExtSynchronize")
+    wall = time.perf_counter() - t0
+    print(f"timed {a.steps} frames in {wall:.3f} s", file=sys.stderr, flush=True)
+    assert all(st.bvhRebuilt == 0 for st in stats), "static geometry was rebuilt inside the timed region"
+
+    n = len(stats)
+    rays = sum(st.raysTraced for st in stats)
+    trav_ms = sum(st.gpuMsTraversal for st in stats)
+    launches = sum(st.traversalLaunches for st in stats)
+    nodes = sum(st.nodeVisits for st in stats)
+    tris = sum(st.triangleTests for st in stats)
+    alg_bytes = rays * BYTES_PER_RAY + nodes * BYTES_PER_NODE + tris * BYTES_PER_TRI
+    achieved = alg_bytes / (trav_ms * 1e-3) / 1e9
+    traffic = load_traffic()
+
+    def avg(field):
+        return sum(getattr(st, field) for st in stats) / n
+
+    den_ms = avg("gpuMsDenoise")
+    out = {
+        "metric": METRIC,
+        "value": n / wall,
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": max(1, a.warmup),
+        "ms_per_step": wall / n * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "1920x1080 full frame, 999,168-triangle terrain + 500 icospheres, 64 sphere lights, "
+                        "60-frame orbit, static geometry resident in HBM",
+            "width": WIDTH, "height": HEIGHT, "triangles": int(stats[0].triangleCount), "sphere_lights": 64,
+            "spp": 1, "bounces": 2, "restir_di": "32 candidates / 1 temporal / 1 spatial",
+            "atrous_iterations": 5, "parallelism": "single GPU",
+        },
+        "mrays_per_s": rays / wall / 1e6,
+        "rays_per_frame": rays / n,
+        "gpu_ms_per_frame": avg("gpuMsTotal"),
+        "stage_ms": {k: avg(f) for k, f in (("upload", "gpuMsUpload"), ("assemble", "gpuMsAssemble"),
+                                            ("bvh", "gpuMsBvh"), ("trace_and_shade", "gpuMsTrace"),
+                                            ("traversal", "gpuMsTraversal"), ("denoise", "gpuMsDenoise"),
+                                            ("tonemap", "gpuMsTonemap"))},
+        "denoised_mpix_per_s": WIDTH * HEIGHT / (den_ms * 1e-3) / 1e6 if den_ms > 0 else None,
+        "roofline": {
+            "kernel": "k_trace",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": alg_bytes / launches,
+            "avg_launch_ms": trav_ms / launches,
+            "launches": launches,
+            "node_visits_per_ray": nodes / rays,
+            "triangle_tests_per_ray": tris / rays,
+        },
+        "cpu_baseline": None,
+    }
+    r.close()
+    if not a.no_cpu_baseline:
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(scene, rays / n, threads)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -137,6 +137,7 @@ typedef struct RgExtHit
 #define RG_EXT_TRACE_CLOSEST         0u
 #define RG_EXT_TRACE_ANY_HIT         1u /* stop at the first hit (occlusion) */
 #define RG_EXT_TRACE_CULL_FRONT_FACE 2u /* gl_RayFlagsCullFrontFacingTrianglesEXT, CCW = front */
+#define RG_EXT_TRACE_COLLECT_STATS   4u /* count node visits and triangle tests */
 
 RGAPI RgResult RGCONV rgExtTraceRays( RgExtBvh        bvh,
                                       const RgExtRay* d_rays,
@@ -144,6 +145,9 @@ RGAPI RgResult RGCONV rgExtTraceRays( RgExtBvh        bvh,
                                       RgExtHit*       d_hits,
                                       uint32_t        flags,
                                       void*           stream );
+/* Node visits and triangle tests of the last rgExtTraceRays call made with
+ * RG_EXT_TRACE_COLLECT_STATS (synchronises the device). */
+RGAPI RgResult RGCONV rgExtBvhGetTraceStats( RgExtBvh bvh, uint64_t* pNodeVisits, uint64_t* pTriangleTests );
 
 /* -------------------------------------------------------------- tonemap */
 /* CmLuminanceHistogram.comp:41-86: 256-bin log2-luminance histogram of a
@@ -323,6 +327,8 @@ typedef struct RgExtDrawFramePathParams
     uint32_t        restirSpatialSamples;    /* 0xFFFFFFFF = default (8)        */
     uint32_t        renderWidth;             /* 0 = RENDER_RESOLUTION params / 1280 */
     uint32_t        renderHeight;            /* 0 = RENDER_RESOLUTION params / 720  */
+    uint32_t        collectTraversalStats;   /* count node visits / triangle tests  */
+    uint32_t        _pad0;
 } RgExtDrawFramePathParams;
 
 /* Per-frame statistics of the last rgDrawFrame; GPU times from HIP events
@@ -343,7 +349,12 @@ typedef struct RgExtFrameStats
     float    gpuMsDenoise;
     float    gpuMsTonemap;
     float    avgLuminance;
-    float    _pad0;
+    float    gpuMsUpload;       /* host-to-device copies of the uploaded geometry   */
+    float    gpuMsTraversal;    /* sum over the traversal-kernel launches           */
+    uint32_t traversalLaunches;
+    uint32_t _pad1;
+    uint64_t nodeVisits;        /* with collectTraversalStats, else 0               */
+    uint64_t triangleTests;
 } RgExtFrameStats;
 
 RGAPI RgResult RGCONV rgExtGetFrameStats( RgInstance instance, RgExtFrameStats* pOut );
@@ -356,6 +367,12 @@ RGAPI RgResult RGCONV rgExtGetFrameParams( RgInstance instance, RgExtFrameParams
 RGAPI RgResult RGCONV rgExtReadbackHistogram( RgInstance instance, uint32_t* pOut256 );
 /* The instance's stream (hipStream_t). */
 RGAPI RgResult RGCONV rgExtGetStream( RgInstance instance, void** pOut );
+/* Static geometry: when enabled, a frame that uploads no primitives keeps
+ * the previous frame's geometry resident in HBM (no copy, no assembly, no
+ * BVH build), like the reference's static scene (ASManager::
+ * SubmitStaticGeometry, Source/ASManager.cpp:696-764); uploading primitives
+ * replaces it. */
+RGAPI RgResult RGCONV rgExtSetStaticGeometry( RgInstance instance, RgBool32 enable );
 
 #ifdef __cplusplus
 }

@@ -765,10 +765,12 @@ extern "C" void or_trace_rays( const OrBvh* b, const RgExtRay* rays, uint32_t co
                         h++;
                     }
                 }
-                // visit near to far: push far first
+                // visit near to far (ties: smaller child reference first): push far first
                 std::vector< int > ord( h );
                 for( int k = 0; k < h; k++ ) ord[ k ] = k;
-                std::stable_sort( ord.begin(), ord.end(), [ & ]( int x, int y ) { return dist[ x ] < dist[ y ]; } );
+                std::sort( ord.begin(), ord.end(), [ & ]( int x, int y ) {
+                    return dist[ x ] < dist[ y ] || ( dist[ x ] == dist[ y ] && cr[ x ] < cr[ y ] );
+                } );
                 for( int k = h - 1; k >= 0; k-- )
                 {
                     stack.push_back( cr[ ord[ k ] ] );

@@ -219,8 +219,22 @@ RgResult rgExtTraceRays( RgExtBvh bvh, const RgExtRay* d_rays, uint32_t rayCount
 {
     return Guard( [ & ] {
         RG_REQUIRE( bvh && ( ( d_rays && d_hits ) || rayCount == 0 ), RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
-        RG_REQUIRE( ( flags & ~3u ) == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "unknown trace flags" );
+        RG_REQUIRE( ( flags & ~7u ) == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "unknown trace flags" );
         TraceRays( bvh->bvh, bvh->trace, d_rays, rayCount, d_hits, flags, AsStream( stream ) );
+    } );
+}
+
+RgResult rgExtBvhGetTraceStats( RgExtBvh bvh, uint64_t* pNodeVisits, uint64_t* pTriangleTests )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( bvh && pNodeVisits && pTriangleTests, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        uint64_t v[ 2 ] = { 0, 0 };
+        if( bvh->trace.stats.ptr )
+        {
+            RG_HIP_CHECK( hipMemcpy( v, bvh->trace.stats.ptr, sizeof( v ), hipMemcpyDeviceToHost ) );
+        }
+        *pNodeVisits    = v[ 0 ];
+        *pTriangleTests = v[ 1 ];
     } );
 }
 

@@ -90,9 +90,12 @@ struct TraceContext
 {
     DeviceBuffer counter;
     DeviceBuffer overflow;
+    DeviceBuffer stats; // 2 x u64: node visits, triangle tests of the last collecting launch
     uint32_t     gridBlocks = 0;
 };
 
-void TraceRays( const BvhObject& bvh, TraceContext& ctx, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags, hipStream_t stream );
+// evBefore / evAfter (optional) are recorded right around the kernel launch.
+void TraceRays( const BvhObject& bvh, TraceContext& ctx, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags, hipStream_t stream,
+                hipEvent_t evBefore = nullptr, hipEvent_t evAfter = nullptr );
 
 } // namespace rg

@@ -82,6 +82,7 @@ const T* FindParams( const RgDrawFrameInfo& info, RgStructureType type )
 enum Stage
 {
     ST_BEGIN,
+    ST_UPLOADED,
     ST_ASSEMBLED,
     ST_BVH,
     ST_TRACED,
@@ -106,6 +107,7 @@ struct Instance
     RgExtFrameParams                            params{};
     RgExtFrameStats                             stats{};
     bool                                        frameStarted = false;
+    bool                                        staticGeometry = false;
     bool                                        firstFrame   = true;
     uint32_t                                    frameId      = 0;
     double                                      prevTime     = 0.0;
@@ -282,9 +284,18 @@ struct Instance
         }
 
         hipStream_t s = stream;
+        const auto* ext = FindParams< RgExtDrawFramePathParams >( di, RG_EXT_STRUCTURE_TYPE_PATH_PARAMS );
+        pt.collectStats = ext && ext->collectTraversalStats;
         RG_HIP_CHECK( hipEventRecord( ev[ ST_BEGIN ], s ) );
         pt.UploadLights( scene, s );
-        const bool changed = AssembleScene( scene, dscene, s );
+        // static geometry: a frame without primitive uploads keeps the resident scene
+        const bool keep = staticGeometry && scene.triCount == 0 && dscene.triCount > 0;
+        if( !keep )
+        {
+            UploadScene( scene, dscene, s );
+        }
+        RG_HIP_CHECK( hipEventRecord( ev[ ST_UPLOADED ], s ) );
+        const bool changed = keep ? false : AssembleScene( dscene, s );
         RG_HIP_CHECK( hipEventRecord( ev[ ST_ASSEMBLED ], s ) );
         stats.bvhRebuilt = 0;
         if( dscene.triCount > 0 && ( changed || !dscene.bvhValid ) )
@@ -321,7 +332,11 @@ struct Instance
         stats.triangleCount  = dscene.triCount;
         stats.bvhNodeCount   = bvh.nodeCount;
         stats.raysTraced     = rays;
-        stats.gpuMsAssemble  = ms( ST_BEGIN, ST_ASSEMBLED );
+        stats.gpuMsUpload    = ms( ST_BEGIN, ST_UPLOADED );
+        stats.gpuMsAssemble  = ms( ST_UPLOADED, ST_ASSEMBLED );
+        stats.gpuMsTraversal = pt.TraversalMs();
+        stats.traversalLaunches = pt.traceLaunches;
+        pt.TraversalCounts( &stats.nodeVisits, &stats.triangleTests );
         stats.gpuMsBvh       = ms( ST_ASSEMBLED, ST_BVH );
         stats.gpuMsTrace     = ms( ST_BVH, ST_TRACED );
         stats.gpuMsDenoise   = ms( ST_TRACED, ST_DENOISED );
@@ -580,6 +595,11 @@ RgResult rgExtReadbackHistogram( RgInstance instance, uint32_t* pOut256 )
         Instance& in = Get( instance );
         RG_HIP_CHECK( hipMemcpy( pOut256, in.histCopy.ptr, 256 * 4, hipMemcpyDeviceToHost ) );
     } );
+}
+
+RgResult rgExtSetStaticGeometry( RgInstance instance, RgBool32 enable )
+{
+    return Guard( [ & ] { Get( instance ).staticGeometry = enable != 0; } );
 }
 
 RgResult rgExtGetStream( RgInstance instance, void** pOut )

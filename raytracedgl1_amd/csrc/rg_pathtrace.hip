@@ -982,8 +982,54 @@ __global__ __launch_bounds__( THREADS ) void k_indirect_c( RgExtFrameParams p, F
 
 } // namespace
 
+PathTracer::~PathTracer()
+{
+    for( auto& e : traceEv )
+    {
+        if( e )
+        {
+            ( void )hipEventDestroy( e );
+        }
+    }
+}
+
+float PathTracer::TraversalMs() const
+{
+    float total = 0.0f;
+    for( uint32_t k = 0; k < traceLaunches; k++ )
+    {
+        float ms = 0.0f;
+        if( hipEventElapsedTime( &ms, traceEv[ 2 * k ], traceEv[ 2 * k + 1 ] ) == hipSuccess )
+        {
+            total += ms;
+        }
+    }
+    return total;
+}
+
+void PathTracer::TraversalCounts( uint64_t* nodes, uint64_t* tris ) const
+{
+    *nodes = *tris = 0;
+    if( !collectStats || traceLaunches == 0 )
+    {
+        return;
+    }
+    uint64_t v[ 32 ] = {};
+    RG_HIP_CHECK( hipMemcpy( v, traceStats.ptr, size_t( traceLaunches ) * 16, hipMemcpyDeviceToHost ) );
+    for( uint32_t k = 0; k < traceLaunches; k++ )
+    {
+        *nodes += v[ 2 * k ];
+        *tris += v[ 2 * k + 1 ];
+    }
+}
+
 void PathTracer::Init( hipStream_t s )
 {
+    for( auto& e : traceEv )
+    {
+        RG_HIP_CHECK( hipEventCreate( &e ) );
+    }
+    traceStats.Reserve( 16 * 16 );
     blueNoise.Reserve( 128u * 128u * 128u * 4u );
     k_blue_noise<<< DivUp( 128u * 128u * 128u, THREADS ), THREADS, 0, s >>>( blueNoise.as< uint32_t >() );
     CheckLaunch( "k_blue_noise" );
@@ -1047,10 +1093,18 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
     uint64_t  traced = 0;
     const bool haveGeometry = scene.triCount > 0 && bvh.nodeCount > 0;
 
+    traceLaunches = 0;
     auto trace = [ & ]( RgExtRay* rr, RgExtHit* hh, uint32_t flags ) {
         if( haveGeometry )
         {
-            TraceRays( bvh, tc, rr, n, hh, flags | traceFlags, s );
+            const uint32_t k = traceLaunches < 16 ? traceLaunches : 15;
+            TraceRays( bvh, tc, rr, n, hh, flags | traceFlags | ( collectStats ? RG_EXT_TRACE_COLLECT_STATS : 0u ), s,
+                       traceEv[ 2 * k ], traceEv[ 2 * k + 1 ] );
+            if( collectStats )
+            {
+                RG_HIP_CHECK( hipMemcpyAsync( traceStats.as< uint8_t >() + size_t( k ) * 16, tc.stats.ptr, 16, hipMemcpyDeviceToDevice, s ) );
+            }
+            traceLaunches = k + 1;
         }
         else
         {

@@ -140,8 +140,10 @@ struct DeviceScene
     bool         bvhValid = false;
 };
 
-// Uploads the collected geometry and assembles world-space triangles.
-// Returns true if the triangles differ from the last BVH build.
-bool AssembleScene( SceneCollector& sc, DeviceScene& ds, hipStream_t s );
+// Copies the collected geometry to the device (asynchronous).
+void UploadScene( SceneCollector& sc, DeviceScene& ds, hipStream_t s );
+// Assembles world-space triangles from the uploaded geometry.  Returns true
+// if they differ from the triangles of the last BVH build.
+bool AssembleScene( DeviceScene& ds, hipStream_t s );
 
 } // namespace rg

@@ -373,15 +373,14 @@ __global__ __launch_bounds__( THREADS ) void k_compare( const uint32_t* a, const
 }
 } // namespace
 
-bool AssembleScene( SceneCollector& sc, DeviceScene& ds, hipStream_t s )
+void UploadScene( SceneCollector& sc, DeviceScene& ds, hipStream_t s )
 {
     const uint32_t n = sc.triCount;
     ds.triCount      = n;
     ds.geomCount     = uint32_t( sc.geoms.size );
     if( n == 0 )
     {
-        ds.bvhValid = false;
-        return false;
+        return;
     }
     ds.vertices.Reserve( sc.vertices.size * sizeof( RgPrimitiveVertex ) );
     ds.indices.Reserve( ( sc.indices.size + 1 ) * 4 );
@@ -392,7 +391,16 @@ bool AssembleScene( SceneCollector& sc, DeviceScene& ds, hipStream_t s )
         RG_HIP_CHECK( hipMemcpyAsync( ds.indices.ptr, sc.indices.data, sc.indices.size * 4, hipMemcpyHostToDevice, s ) );
     }
     RG_HIP_CHECK( hipMemcpyAsync( ds.geoms.ptr, sc.geoms.data, sc.geoms.size * sizeof( GeomRecord ), hipMemcpyHostToDevice, s ) );
+}
 
+bool AssembleScene( DeviceScene& ds, hipStream_t s )
+{
+    const uint32_t n = ds.triCount;
+    if( n == 0 )
+    {
+        ds.bvhValid = false;
+        return false;
+    }
     // Keep the previous build's triangles to detect an unchanged scene.
     const bool canCompare = ds.bvhValid && ds.builtTriCount == n;
     if( canCompare )

@@ -38,11 +38,14 @@ struct TraceArgs
     uint32_t*       rayCounter;
     uint32_t*       overflowStack;
     uint32_t*       errorFlag;
+    uint64_t*       stats; // [0] node visits, [1] triangle tests (RG_EXT_TRACE_COLLECT_STATS)
 };
 
+// Orders by (distance, child reference): a total order, so the visit order
+// (and the node / triangle counts) does not depend on the sorting network.
 __device__ __forceinline__ void CmpSwap( float& da, uint32_t& ra, float& db, uint32_t& rb )
 {
-    if( db < da )
+    if( db < da || ( db == da && rb < ra ) )
     {
         float    td = da;
         uint32_t tr = ra;
@@ -86,6 +89,7 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
     uint32_t node     = 0;
     int      sp       = 0;
     uint32_t steps    = 0;
+    uint32_t nodeVisits = 0, triTests = 0;
 
     while( true )
     {
@@ -128,6 +132,20 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
         }
         if( __ballot( active ) == 0ull )
         {
+            if( a.stats )
+            {
+                uint64_t nv = nodeVisits, tt = triTests;
+                for( int off = 32; off > 0; off >>= 1 )
+                {
+                    nv += __shfl_xor( nv, off, 64 );
+                    tt += __shfl_xor( tt, off, 64 );
+                }
+                if( lane == 0 )
+                {
+                    atomicAdd( reinterpret_cast< unsigned long long* >( &a.stats[ 0 ] ), ( unsigned long long )nv );
+                    atomicAdd( reinterpret_cast< unsigned long long* >( &a.stats[ 1 ] ), ( unsigned long long )tt );
+                }
+            }
             break;
         }
         if( !active )
@@ -146,6 +164,7 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
         else if( ( node & RG_EXT_BVH_CHILD_LEAF_BIT ) == 0u )
         {
             // Interior node: slab-test the 8 children, visit near to far.
+            nodeVisits++;
             const BvhNode8& nd = a.nodes[ node ];
             float           dist[ 8 ];
             uint32_t        ref[ 8 ];
@@ -199,6 +218,7 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
             const uint32_t cnt   = ( node & 7u ) + 1u;
             for( uint32_t k = 0; k < cnt; k++ )
             {
+                triTests++;
                 const float4 q0 = a.tris[ size_t( first + k ) * 3 + 0 ];
                 const float4 q1 = a.tris[ size_t( first + k ) * 3 + 1 ];
                 const float4 q2 = a.tris[ size_t( first + k ) * 3 + 2 ];
@@ -251,7 +271,8 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
 
 } // namespace
 
-void TraceRays( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags, hipStream_t stream )
+void TraceRays( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags, hipStream_t stream,
+                hipEvent_t evBefore, hipEvent_t evAfter )
 {
     RG_REQUIRE( bvh.nodeCount > 0, RG_RESULT_WRONG_FUNCTION_CALL, "BVH is not built" );
     RG_REQUIRE( bvh.nodeFormat == RG_EXT_BVH_NODE_FP32, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "unsupported node format" );
@@ -269,6 +290,7 @@ void TraceRays( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, u
     }
     const uint32_t blocks = DivUp( count, TR_THREADS ) < s.gridBlocks ? DivUp( count, TR_THREADS ) : s.gridBlocks;
     s.counter.Reserve( 2 * sizeof( uint32_t ) );
+    s.stats.Reserve( 2 * sizeof( uint64_t ) );
     s.overflow.Reserve( size_t( s.gridBlocks ) * TR_THREADS * STACK_GLOBAL * sizeof( uint32_t ) );
     RG_HIP_CHECK( hipMemsetAsync( s.counter.ptr, 0, 2 * sizeof( uint32_t ), stream ) );
 
@@ -282,8 +304,22 @@ void TraceRays( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, u
     a.rayCounter    = s.counter.as< uint32_t >();
     a.errorFlag     = s.counter.as< uint32_t >() + 1;
     a.overflowStack = s.overflow.as< uint32_t >();
+    a.stats         = nullptr;
+    if( flags & RG_EXT_TRACE_COLLECT_STATS )
+    {
+        RG_HIP_CHECK( hipMemsetAsync( s.stats.ptr, 0, 2 * sizeof( uint64_t ), stream ) );
+        a.stats = s.stats.as< uint64_t >();
+    }
+    if( evBefore )
+    {
+        RG_HIP_CHECK( hipEventRecord( evBefore, stream ) );
+    }
     k_trace<<< blocks, TR_THREADS, 0, stream >>>( a );
     CheckLaunch( "k_trace" );
+    if( evAfter )
+    {
+        RG_HIP_CHECK( hipEventRecord( evAfter, stream ) );
+    }
 }
 
 } // namespace rg

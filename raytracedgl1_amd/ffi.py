@@ -30,6 +30,7 @@ RG_EXT_MISS = 0xFFFFFFFF
 RG_EXT_TRACE_CLOSEST = 0
 RG_EXT_TRACE_ANY_HIT = 1
 RG_EXT_TRACE_CULL_FRONT_FACE = 2
+RG_EXT_TRACE_COLLECT_STATS = 4
 RG_EXT_BVH_NODE_FP32 = 0
 RG_EXT_BVH_NODE_FP16 = 1
 RG_EXT_BVH_CHILD_EMPTY = 0xFFFFFFFF
@@ -118,6 +119,7 @@ _SIGS = {
     "rgExtBvhBuild": (_RES, [_P, _P, _U32, C.POINTER(RgExtBvhBuildParams), _P]),
     "rgExtBvhGetView": (_RES, [_P, C.POINTER(RgExtBvhView)]),
     "rgExtTraceRays": (_RES, [_P, _P, _U32, _P, _U32, _P]),
+    "rgExtBvhGetTraceStats": (_RES, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "rgExtLuminanceHistogram": (_RES, [_P, _U32, _U32, _F, _F, _P, _P]),
     "rgExtLuminanceAverage": (_RES, [_P, _U32, _F, _F, _F, _U32, _P, _P]),
     "rgExtFramebuffersCreate": (_RES, [_U32, _U32, C.POINTER(_P)]),
@@ -264,6 +266,12 @@ class Bvh:
 
     def trace(self, d_rays: DeviceArray, count: int, d_hits: DeviceArray, flags: int = 0, stream=None) -> None:
         call("rgExtTraceRays", self.h, d_rays.ptr, count, d_hits.ptr, flags, stream)
+
+    def trace_stats(self) -> tuple:
+        """(node visits, triangle tests) of the last trace with RG_EXT_TRACE_COLLECT_STATS."""
+        nodes, tris = C.c_uint64(), C.c_uint64()
+        call("rgExtBvhGetTraceStats", self.h, C.byref(nodes), C.byref(tris))
+        return int(nodes.value), int(tris.value)
 
     def download(self) -> dict:
         """Copies every build output to host numpy arrays."""

@@ -83,7 +83,7 @@ class RgDrawFrameSkyParams(C.Structure):
 class RgExtDrawFramePathParams(C.Structure):
     _fields_ = [("sType", C.c_int), ("pNext", _P), ("atrousIterations", _U32), ("restirInitialCandidates", _U32),
                 ("restirTemporalSamples", _U32), ("restirSpatialSamples", _U32),
-                ("renderWidth", _U32), ("renderHeight", _U32)]
+                ("renderWidth", _U32), ("renderHeight", _U32), ("collectTraversalStats", _U32), ("_pad0", _U32)]
 
 
 class RgDrawFrameInfo(C.Structure):
@@ -97,12 +97,15 @@ class RgExtFrameStats(C.Structure):
     _fields_ = [("frameId", _U32), ("renderWidth", _U32), ("renderHeight", _U32), ("triangleCount", _U32),
                 ("bvhRebuilt", _U32), ("bvhNodeCount", _U32), ("raysTraced", C.c_uint64),
                 ("gpuMsTotal", _F), ("gpuMsAssemble", _F), ("gpuMsBvh", _F), ("gpuMsTrace", _F),
-                ("gpuMsDenoise", _F), ("gpuMsTonemap", _F), ("avgLuminance", _F), ("_pad0", _F)]
+                ("gpuMsDenoise", _F), ("gpuMsTonemap", _F), ("avgLuminance", _F), ("gpuMsUpload", _F),
+                ("gpuMsTraversal", _F), ("traversalLaunches", _U32), ("_pad1", _U32), ("nodeVisits", C.c_uint64),
+                ("triangleTests", C.c_uint64)]
 
 
 assert C.sizeof(RgInstanceCreateInfo) == 184 and C.sizeof(RgMeshInfo) == 88
 assert C.sizeof(RgMeshPrimitiveInfo) == 80 and C.sizeof(RgDrawFrameInfo) == 120
 assert C.sizeof(RgDirectionalLightUploadInfo) == 48 and C.sizeof(RgSphericalLightUploadInfo) == 48
+assert C.sizeof(RgExtDrawFramePathParams) == 48 and C.sizeof(RgExtFrameStats) == 96
 
 RG_STRUCTURE_TYPE_SKY = 7
 RG_EXT_STRUCTURE_TYPE_PATH_PARAMS = 0x52470001
@@ -127,6 +130,8 @@ _API = {
     "rgExtGetFramebuffers": (C.c_int, [_P, C.POINTER(_P)]),
     "rgExtGetFrameParams": (C.c_int, [_P, C.POINTER(RgExtFrameParams)]),
     "rgExtReadbackHistogram": (C.c_int, [_P, _P]),
+    "rgExtGetStream": (C.c_int, [_P, C.POINTER(_P)]),
+    "rgExtSetStaticGeometry": (C.c_int, [_P, _U32]),
 }
 _bound = False
 
@@ -168,9 +173,10 @@ class SceneUploader:
         self.mesh.transform = identity_transform()
         self.prim = RgMeshPrimitiveInfo()
 
-    def upload(self, inst) -> None:
+    def upload(self, inst, geometry: bool = True) -> None:
+        """Uploads the primitives (unless geometry is False) and the lights."""
         l = api()
-        for oid, v, color in self.objects:
+        for oid, v, color in (self.objects if geometry else []):
             self.mesh.uniqueObjectID = oid
             self.prim.primitiveIndexInMesh = 0
             self.prim.pVertices = v.ctypes.data
@@ -225,11 +231,15 @@ class Renderer:
         self.sky.pNext = C.cast(C.pointer(self.path), _P)
         self.time = 0.0
 
-    def frame(self, uploader: SceneUploader, cam, dt: float = 1.0 / 60.0) -> RgExtFrameStats:
+    def set_static_geometry(self, enable: bool) -> None:
+        ffi.check("rgExtSetStaticGeometry", api().rgExtSetStaticGeometry(self.h, 1 if enable else 0))
+
+    def frame(self, uploader: SceneUploader, cam, dt: float = 1.0 / 60.0, geometry: bool = True) -> RgExtFrameStats:
+        """rgStartFrame, uploads, rgDrawFrame; geometry=False uploads lights only."""
         l = api()
         sfi = RgStartFrameInfo()
         ffi.check("rgStartFrame", l.rgStartFrame(self.h, C.byref(sfi)))
-        uploader.upload(self.h)
+        uploader.upload(self.h, geometry)
         di = RgDrawFrameInfo()
         di.view[:] = cam.view().tolist()
         di.fovYRadians = cam.fovy

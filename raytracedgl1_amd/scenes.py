@@ -1,4 +1,4 @@
-"""Seeded synthetic scenes and cameras of the benchmark configurations.
+"""Seeded synthetic scenes and cameras of the benchmark workloads.
 
 All randomness comes from a counter-based SplitMix64 stream (Steele, Lea,
 Flood, "Fast splittable pseudorandom number generators", OOPSLA 2014), so a
@@ -16,10 +16,10 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-SEED_CONFIG1 = 0xC0FFEE01
-SEED_CONFIG2 = 0xC0FFEE02
-SEED_CONFIG4 = 0xC0FFEE04
-SEED_CONFIG5 = 0xC0FFEE05
+SEED_TERRAIN_BOXES = 0xC0FFEE01
+SEED_TERRAIN_SPHERES = 0xC0FFEE02
+SEED_SCENE_BATCH = 0xC0FFEE04
+SEED_SERVING = 0xC0FFEE05
 
 _GOLDEN = np.uint64(0x9E3779B97F4A7C15)
 _M1 = np.uint64(0xBF58476D1CE4E5B9)
@@ -192,7 +192,7 @@ def _finish(parts, objects, lights=None) -> Scene:
     return Scene(tris, nrms, cols, objects, lights or {})
 
 
-def config1_scene(seed: int = SEED_CONFIG1) -> Scene:
+def terrain_boxes_scene(seed: int = SEED_TERRAIN_BOXES) -> Scene:
     """64x64-cell heightfield (8,192 tris, fBm in [0,4]) + 200 boxes (2,400 tris)."""
     size = 64.0
     h, ht, hn = heightfield(seed, 64, size, 4.0)
@@ -235,8 +235,8 @@ def _sphere_levels(seed: int, count: int, budget: int) -> np.ndarray:
     return levels
 
 
-def config2_scene(seed: int = SEED_CONFIG2, cells: int = 512, spheres: int = 500,
-                  target_tris: int = 1_000_000, lights: int = 64) -> Scene:
+def terrain_spheres_scene(seed: int = SEED_TERRAIN_SPHERES, cells: int = 512, spheres: int = 500,
+                          target_tris: int = 1_000_000, lights: int = 64) -> Scene:
     """512x512 heightfield (524,288 tris) + 500 icospheres (levels 2-5, radii
     U[0.5,3]) totalling 1.0M +- 0.5% triangles, and 64 sphere lights."""
     size = 256.0
@@ -315,7 +315,7 @@ class Camera:
         return projection(width / height, self.fovy, self.znear, self.zfar)
 
 
-def config1_camera(seed: int = SEED_CONFIG1) -> Camera:
+def terrain_boxes_camera(seed: int = SEED_TERRAIN_BOXES) -> Camera:
     """Seeded pose over the terrain, height 6-10, looking down 10-30 degrees."""
     u = uniform(seed, 20, 5)
     x, z = -20 + 40 * u[0], -20 + 40 * u[1]
@@ -327,7 +327,7 @@ def config1_camera(seed: int = SEED_CONFIG1) -> Camera:
     return Camera(eye, eye + d)
 
 
-def config2_camera(frame: int) -> Camera:
+def orbit_camera(frame: int) -> Camera:
     """Orbit of radius 80 at height 25, 6 degrees per frame, looking at the centre."""
     a = math.radians(6.0 * frame)
     eye = np.array([80 * math.cos(a), 25.0, 80 * math.sin(a)])

@@ -277,8 +277,8 @@ def test_lbvh_invariants_duplicates(oracle):
     _check_bvh_invariants(tris, out, 3)
 
 
-def test_lbvh_invariants_config1(oracle):
-    s = scenes.config1_scene()
+def test_lbvh_invariants_terrain_boxes(oracle):
+    s = scenes.terrain_boxes_scene()
     _check_bvh_invariants(s.tris, oracle.bvh(s.tris, 3).download(), 3)
 
 
@@ -314,9 +314,9 @@ def test_traversal_matches_bruteforce_small(oracle, n):
         assert _compare_with_bruteforce(hits, t, p) <= 2
 
 
-def test_traversal_matches_bruteforce_config1(oracle):
-    s = scenes.config1_scene()
-    rays = scenes.primary_rays(scenes.config1_camera(), 640, 360)
+def test_traversal_matches_bruteforce_terrain_boxes(oracle):
+    s = scenes.terrain_boxes_scene()
+    rays = scenes.primary_rays(scenes.terrain_boxes_camera(), 640, 360)
     sub = rays[:: 23]  # 10k rays, spread over the image
     bvh = oracle.bvh(s.tris, 3)
     hits = bvh.trace(sub, 0)
@@ -331,7 +331,7 @@ def test_traversal_matches_bruteforce_config1(oracle):
 
 
 def test_shadow_rays_any_hit(oracle):
-    s = scenes.config1_scene()
+    s = scenes.terrain_boxes_scene()
     r = np.random.default_rng(7)
     rays = np.zeros(5000, RAY_DTYPE)
     rays["origin"] = np.stack([r.uniform(-30, 30, 5000), r.uniform(0.5, 6, 5000), r.uniform(-30, 30, 5000)], 1)
@@ -389,8 +389,8 @@ def test_cull_front_face_convention(oracle):
 
 
 def test_traversal_barycentrics_reconstruct_hit_point(oracle):
-    s = scenes.config1_scene()
-    rays = scenes.primary_rays(scenes.config1_camera(), 160, 90)
+    s = scenes.terrain_boxes_scene()
+    rays = scenes.primary_rays(scenes.terrain_boxes_camera(), 160, 90)
     hits = oracle.bvh(s.tris, 3).trace(rays, 0)
     h = hits["prim"] != RG_EXT_MISS
     t = s.tris.reshape(-1, 3, 3)[hits["prim"][h]].astype(np.float64)
