@@ -49,6 +49,9 @@ def parse_args(argv=None):
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (default: min(16, cpus))")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="instead of the frame benchmark, run 5 plain device copies of MIB MiB (k_copy16): the "
+                         "known-size reference for scaling HBM counter readings")
     return ap.parse_args(argv)
 
 
@@ -125,6 +128,25 @@ def cpu_baseline(scene, rays_per_frame: float, threads: int) -> dict:
     }
 
 
+def calibrate_copy(mib: int) -> int:
+    import ctypes as C
+    from raytracedgl1_amd import ffi
+    nbytes = mib << 20
+    src, dst = ffi.DeviceArray(nbytes, "u1"), ffi.DeviceArray(nbytes, "u1")
+    ffi.call("rgExtMemsetDevice", src.ptr, 1, nbytes)
+    t = ffi.Timer()
+    times = []
+    for _ in range(5):
+        t.start()
+        ffi.call("rgExtCopyDeviceToDevice", dst.ptr, src.ptr, nbytes, None)
+        t.stop()
+        times.append(t.elapsed_ms())
+    ms = sorted(times)[2]
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "dispatches": 5,
+                      "median_ms": ms, "gbps_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+    return 0
+
+
 def load_traffic():
     try:
         d = json.loads(TRAFFIC_FILE.read_text())
@@ -144,6 +166,8 @@ def main(argv=None) -> int:
     if a.steps < 1:
         print("--steps must be >= 1", file=sys.stderr)
         return 2
+    if a.calibrate_copy:
+        return calibrate_copy(a.calibrate_copy)
 
     from raytracedgl1_amd import ffi, rtgl, scenes
 

@@ -32,6 +32,9 @@ RGAPI RgResult RGCONV rgExtDeviceFree( void* d_ptr );
 RGAPI RgResult RGCONV rgExtCopyHostToDevice( void* d_dst, const void* src, size_t bytes );
 RGAPI RgResult RGCONV rgExtCopyDeviceToHost( void* dst, const void* d_src, size_t bytes );
 RGAPI RgResult RGCONV rgExtMemsetDevice( void* d_dst, int value, size_t bytes );
+/* Device-to-device copy by a plain kernel (k_copy16) on `stream`: every byte
+ * read once and written once.  Pointers and size 16-byte aligned. */
+RGAPI RgResult RGCONV rgExtCopyDeviceToDevice( void* d_dst, const void* d_src, size_t bytes, void* stream );
 RGAPI RgResult RGCONV rgExtSynchronize( void );
 /* Number of visible HIP devices (0 without a GPU). */
 RGAPI int RGCONV rgExtGetDeviceCount( void );

@@ -35,6 +35,19 @@ struct RgExtBvh_T
     TraceContext trace;
 };
 
+namespace
+{
+// Plain 16-byte-per-lane copy: reads and writes each byte once.
+__global__ __launch_bounds__( 256 ) void k_copy16( uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n )
+{
+    const size_t stride = size_t( gridDim.x ) * blockDim.x;
+    for( size_t i = size_t( blockIdx.x ) * blockDim.x + threadIdx.x; i < n; i += stride )
+    {
+        dst[ i ] = src[ i ];
+    }
+}
+} // namespace
+
 extern "C" {
 
 // ------------------------------------------------------------ memory
@@ -77,6 +90,24 @@ RgResult rgExtCopyDeviceToHost( void* dst, const void* d_src, size_t bytes )
         {
             RG_HIP_CHECK( hipMemcpy( dst, d_src, bytes, hipMemcpyDeviceToHost ) );
         }
+    } );
+}
+
+RgResult rgExtCopyDeviceToDevice( void* d_dst, const void* d_src, size_t bytes, void* stream )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( ( d_dst && d_src ) || bytes == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null pointer" );
+        RG_REQUIRE( bytes % 16 == 0 && ( reinterpret_cast< uintptr_t >( d_dst ) % 16 ) == 0 &&
+                        ( reinterpret_cast< uintptr_t >( d_src ) % 16 ) == 0,
+                    RG_RESULT_WRONG_FUNCTION_ARGUMENT, "pointers and size must be 16-byte aligned" );
+        if( bytes == 0 )
+        {
+            return;
+        }
+        const size_t n      = bytes / 16;
+        const size_t blocks = ( n + 255 ) / 256 < 8192 ? ( n + 255 ) / 256 : 8192;
+        k_copy16<<< uint32_t( blocks ), 256, 0, AsStream( stream ) >>>( static_cast< uint4* >( d_dst ), static_cast< const uint4* >( d_src ), n );
+        CheckLaunch( "k_copy16" );
     } );
 }
 

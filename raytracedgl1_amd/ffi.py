@@ -106,6 +106,7 @@ _SIGS = {
     "rgExtCopyHostToDevice": (_RES, [_P, _P, C.c_size_t]),
     "rgExtCopyDeviceToHost": (_RES, [_P, _P, C.c_size_t]),
     "rgExtMemsetDevice": (_RES, [_P, C.c_int, C.c_size_t]),
+    "rgExtCopyDeviceToDevice": (_RES, [_P, _P, C.c_size_t, _P]),
     "rgExtSynchronize": (_RES, []),
     "rgExtGetDeviceCount": (C.c_int, []),
     "rgExtGetLastErrorMessage": (C.c_char_p, []),

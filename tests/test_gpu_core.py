@@ -209,3 +209,15 @@ def test_histogram_bit_exact(gpu, oracle):
     oa = oracle.lum_average(o, w * h, float(np.float32(min_log)), float(np.float32(max_log)), 0.016, False, 0.5)
     assert ga == np.float32(oa)
     assert np.all(dh.download() == 0)
+
+
+def test_device_copy_kernel(gpu):
+    a = np.random.default_rng(2).integers(0, 2**32, 4096 * 4 + 8, dtype=np.uint32)
+    src = DeviceArray.from_numpy(a)
+    dst = DeviceArray(a.shape, a.dtype, zero=True)
+    ffi.call("rgExtCopyDeviceToDevice", dst.ptr, src.ptr, a.nbytes, None)
+    assert np.array_equal(dst.download(), a)
+    ffi.call("rgExtCopyDeviceToDevice", dst.ptr, src.ptr, 0, None)
+    with pytest.raises(ffi.RgError):
+        ffi.call("rgExtCopyDeviceToDevice", dst.ptr, src.ptr, 20, None)
+
