@@ -4,9 +4,11 @@
 // hit; :322-359 shadow rays with gl_RayFlagsSkipClosestHitShaderEXT).
 //
 // Execution model: a persistent grid of 256-thread blocks.  Each lane owns
-// one ray at a time; whenever lanes of a wave go idle, the wave takes that
-// many new rays from a global counter with a single atomic, so long rays do
-// not hold the wave's SIMD lanes empty.  The traversal stack is a short LDS
+// one ray at a time; whenever lanes of a wave go idle they take the next rays
+// of the wave's current chunk, so long rays do not hold the wave's SIMD lanes
+// empty.  Chunks of 256 rays come from one global counter, one atomic per
+// chunk: a refill atomic per wave iteration on a single address serialises
+// at the memory side and was the kernel's limiter (4.5x slower at 1080p).  The traversal stack is a short LDS
 // stack (24 entries per lane, interleaved by lane to avoid bank conflicts)
 // backed by a per-lane overflow area in global memory.
 //
@@ -26,6 +28,7 @@ constexpr int      STACK_LDS     = 24;
 constexpr int      STACK_GLOBAL  = 104;
 constexpr int      STACK_TOTAL   = STACK_LDS + STACK_GLOBAL;
 constexpr uint32_t MAX_STEPS_PER_RAY = 1u << 22;
+constexpr uint32_t RAY_CHUNK         = 256; // rays per global-counter atomic
 
 struct TraceArgs
 {
@@ -90,25 +93,45 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
     int      sp       = 0;
     uint32_t steps    = 0;
     uint32_t nodeVisits = 0, triTests = 0;
+    uint32_t chunkNext = 0, chunkEnd = 0; // this wave's rays [chunkNext, chunkEnd), wave-uniform
+    bool     drained   = false;           // the global counter has passed the last ray
 
     while( true )
     {
-        // Refill idle lanes with new rays, one atomic per wave.
+        // Refill idle lanes.  Each wave owns a chunk of RAY_CHUNK consecutive
+        // rays taken with one global atomic; idle lanes take rays from the
+        // chunk (wave-uniform bookkeeping, no atomic), so the global counter
+        // sees one atomic per RAY_CHUNK rays instead of one per refill.
         const bool     need = !active && !exhausted;
         const uint64_t mask = __ballot( need );
         if( mask != 0ull )
         {
             const uint32_t cnt    = __popcll( mask );
+            const uint32_t rank   = __popcll( mask & ltMask );
+            const uint32_t left   = chunkEnd - chunkNext;
             const int      leader = __ffsll( ( long long )mask ) - 1;
-            uint32_t       base   = 0;
-            if( lane == leader )
+            uint32_t       newBase = 0;
+            const bool     grab    = left < cnt && !drained;
+            if( grab )
             {
-                base = atomicAdd( a.rayCounter, cnt );
+                uint32_t b = 0;
+                if( lane == leader )
+                {
+                    b = atomicAdd( a.rayCounter, RAY_CHUNK );
+                }
+                newBase = __shfl( b, leader, 64 );
             }
-            base = __shfl( base, leader, 64 );
             if( need )
             {
-                const uint32_t my = base + __popcll( mask & ltMask );
+                uint32_t my = a.count;
+                if( rank < left )
+                {
+                    my = chunkNext + rank;
+                }
+                else if( grab && newBase < a.count )
+                {
+                    my = newBase + ( rank - left );
+                }
                 if( my < a.count )
                 {
                     const RgExtRay r = a.rays[ my ];
@@ -128,6 +151,20 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
                 {
                     exhausted = true;
                 }
+            }
+            if( !grab )
+            {
+                chunkNext += cnt < left ? cnt : left;
+            }
+            else if( newBase < a.count )
+            {
+                chunkEnd  = a.count - newBase > RAY_CHUNK ? newBase + RAY_CHUNK : a.count;
+                chunkNext = newBase + ( cnt - left ) < chunkEnd ? newBase + ( cnt - left ) : chunkEnd;
+            }
+            else
+            {
+                drained   = true;
+                chunkNext = chunkEnd;
             }
         }
         if( __ballot( active ) == 0ull )
@@ -164,18 +201,26 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
         else if( ( node & RG_EXT_BVH_CHILD_LEAF_BIT ) == 0u )
         {
             // Interior node: slab-test the 8 children, visit near to far.
+            // The node is fetched as 14 16-byte loads (boxes + child refs).
             nodeVisits++;
-            const BvhNode8& nd = a.nodes[ node ];
-            float           dist[ 8 ];
-            uint32_t        ref[ 8 ];
-            int             hitCount = 0;
+            const float4* np = reinterpret_cast< const float4* >( a.nodes + node );
+            float4        q[ 14 ];
+#pragma unroll
+            for( int i = 0; i < 14; i++ )
+            {
+                q[ i ] = np[ i ];
+            }
+            const float* f = reinterpret_cast< const float* >( q ); // lo_x hi_x lo_y hi_y lo_z hi_z child
+            float        dist[ 8 ];
+            uint32_t     ref[ 8 ];
+            int          hitCount = 0;
 #pragma unroll
             for( int k = 0; k < 8; k++ )
             {
-                const uint32_t c = nd.child[ k ];
+                const uint32_t c = __float_as_uint( f[ 48 + k ] );
                 float          tn;
-                const bool     h = c != RG_EXT_BVH_CHILD_EMPTY &&
-                               SlabTest( ray, nd.lo_x[ k ], nd.lo_y[ k ], nd.lo_z[ k ], nd.hi_x[ k ], nd.hi_y[ k ], nd.hi_z[ k ], tmin, bestT, &tn );
+                const bool     inBox = SlabTest( ray, f[ k ], f[ 16 + k ], f[ 32 + k ], f[ 8 + k ], f[ 24 + k ], f[ 40 + k ], tmin, bestT, &tn );
+                const bool     h     = inBox && c != RG_EXT_BVH_CHILD_EMPTY;
                 dist[ k ] = h ? tn : INFINITY;
                 ref[ k ]  = h ? c : RG_EXT_BVH_CHILD_EMPTY;
                 hitCount += h ? 1 : 0;
@@ -213,7 +258,6 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
         }
         else
         {
-            // Leaf: test its triangles.
             const uint32_t first = ( node >> 3 ) & 0x0FFFFFFFu;
             const uint32_t cnt   = ( node & 7u ) + 1u;
             for( uint32_t k = 0; k < cnt; k++ )
@@ -225,7 +269,6 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
                 float        t, u, v;
                 if( IntersectTri( ray, mk3( q0.x, q0.y, q0.z ), mk3( q1.x, q1.y, q1.z ), mk3( q2.x, q2.y, q2.z ), tmin, bestT, cullFront, &t, &u, &v ) )
                 {
-                    // IntersectTri guarantees t <= bestT; on equal t the smaller id wins.
                     const uint32_t prim = __float_as_uint( q0.w );
                     if( t < bestT || prim < bestPrim )
                     {
@@ -253,7 +296,17 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
             else
             {
                 sp--;
-                node = sp < STACK_LDS ? s_stack[ sp * TR_THREADS + tid ] : ovf[ sp - STACK_LDS ];
+                if( sp < STACK_LDS )
+                {
+                    node = s_stack[ sp * TR_THREADS + tid ];
+                }
+                else
+                {
+                    // Keeps the rare overflow read a separate global load
+                    // (no generic-address load on the common LDS path).
+                    asm volatile( "" ::: "memory" );
+                    node = ovf[ sp - STACK_LDS ];
+                }
             }
         }
         if( finished )
@@ -276,6 +329,8 @@ void TraceRays( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, u
 {
     RG_REQUIRE( bvh.nodeCount > 0, RG_RESULT_WRONG_FUNCTION_CALL, "BVH is not built" );
     RG_REQUIRE( bvh.nodeFormat == RG_EXT_BVH_NODE_FP32, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "unsupported node format" );
+    // the chunk counter runs up to count + (waves + 1) * RAY_CHUNK without wrapping
+    RG_REQUIRE( count <= 0xF0000000u, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "too many rays in one launch" );
     if( count == 0 )
     {
         return;
