@@ -151,6 +151,11 @@ RGAPI RgResult RGCONV rgExtTraceRays( RgExtBvh        bvh,
 /* Node visits and triangle tests of the last rgExtTraceRays call made with
  * RG_EXT_TRACE_COLLECT_STATS (synchronises the device). */
 RGAPI RgResult RGCONV rgExtBvhGetTraceStats( RgExtBvh bvh, uint64_t* pNodeVisits, uint64_t* pTriangleTests );
+/* Error bits of the last rgExtTraceRays call (synchronises the device); 0 when
+ * every ray was traced completely.  A set bit means some hits may be wrong. */
+#define RG_EXT_TRACE_ERROR_STACK_OVERFLOW 1u /* a traversal stack entry was dropped */
+#define RG_EXT_TRACE_ERROR_STEP_LIMIT     2u /* a ray exceeded the step limit (malformed tree) */
+RGAPI RgResult RGCONV rgExtBvhGetTraceErrors( RgExtBvh bvh, uint32_t* pErrors );
 
 /* -------------------------------------------------------------- tonemap */
 /* CmLuminanceHistogram.comp:41-86: 256-bin log2-luminance histogram of a
@@ -355,7 +360,7 @@ typedef struct RgExtFrameStats
     float    gpuMsUpload;       /* host-to-device copies of the uploaded geometry   */
     float    gpuMsTraversal;    /* sum over the traversal-kernel launches           */
     uint32_t traversalLaunches;
-    uint32_t _pad1;
+    uint32_t traversalErrors;   /* OR of RG_EXT_TRACE_ERROR_* over the frame's launches */
     uint64_t nodeVisits;        /* with collectTraversalStats, else 0               */
     uint64_t triangleTests;
 } RgExtFrameStats;

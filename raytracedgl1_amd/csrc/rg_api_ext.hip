@@ -259,13 +259,15 @@ RgResult rgExtBvhGetTraceStats( RgExtBvh bvh, uint64_t* pNodeVisits, uint64_t* p
 {
     return Guard( [ & ] {
         RG_REQUIRE( bvh && pNodeVisits && pTriangleTests, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
-        uint64_t v[ 2 ] = { 0, 0 };
-        if( bvh->trace.stats.ptr )
-        {
-            RG_HIP_CHECK( hipMemcpy( v, bvh->trace.stats.ptr, sizeof( v ), hipMemcpyDeviceToHost ) );
-        }
-        *pNodeVisits    = v[ 0 ];
-        *pTriangleTests = v[ 1 ];
+        ReadTraceStats( bvh->trace, pNodeVisits, pTriangleTests );
+    } );
+}
+
+RgResult rgExtBvhGetTraceErrors( RgExtBvh bvh, uint32_t* pErrors )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( bvh && pErrors, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        *pErrors = ReadTraceErrors( bvh->trace );
     } );
 }
 

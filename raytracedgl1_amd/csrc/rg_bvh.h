@@ -94,8 +94,20 @@ struct TraceContext
     uint32_t     gridBlocks = 0;
 };
 
+// Device accumulators a caller clears once and lets several launches add to.
+struct TraceAccum
+{
+    uint64_t* stats  = nullptr; // node visits, triangle tests (with RG_EXT_TRACE_COLLECT_STATS)
+    uint32_t* errors = nullptr; // OR of RG_EXT_TRACE_ERROR_* bits
+};
+
 // evBefore / evAfter (optional) are recorded right around the kernel launch.
+// Without accum, the statistics and error bits of this launch alone are kept
+// in ctx (read back by ReadTraceStats / ReadTraceErrors).
 void TraceRays( const BvhObject& bvh, TraceContext& ctx, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags, hipStream_t stream,
-                hipEvent_t evBefore = nullptr, hipEvent_t evAfter = nullptr );
+                hipEvent_t evBefore = nullptr, hipEvent_t evAfter = nullptr, const TraceAccum* accum = nullptr );
+// Synchronous reads of the last un-accumulated launch's results.
+void     ReadTraceStats( const TraceContext& ctx, uint64_t* nodeVisits, uint64_t* triTests );
+uint32_t ReadTraceErrors( const TraceContext& ctx );
 
 } // namespace rg

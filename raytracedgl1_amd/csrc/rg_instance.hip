@@ -336,7 +336,7 @@ struct Instance
         stats.gpuMsAssemble  = ms( ST_UPLOADED, ST_ASSEMBLED );
         stats.gpuMsTraversal = pt.TraversalMs();
         stats.traversalLaunches = pt.traceLaunches;
-        pt.TraversalCounts( &stats.nodeVisits, &stats.triangleTests );
+        pt.TraversalCounts( &stats.nodeVisits, &stats.triangleTests, &stats.traversalErrors );
         stats.gpuMsBvh       = ms( ST_ASSEMBLED, ST_BVH );
         stats.gpuMsTrace     = ms( ST_BVH, ST_TRACED );
         stats.gpuMsDenoise   = ms( ST_TRACED, ST_DENOISED );

@@ -23,7 +23,7 @@ struct PathTracer
     DeviceBuffer lights, prevToCur;
     DeviceBuffer grid[ 2 ];                // 16^3 * 128 light-grid reservoirs, ping-pong
     DeviceBuffer blueNoise;                // 128 layers of 128x128 RGBA8
-    DeviceBuffer traceStats;               // per-launch node/triangle counts (16 launches x 2 u64)
+    DeviceBuffer frameAccum;               // node visits, triangle tests (u64), error bits (u32) of the frame
     hipEvent_t   traceEv[ 32 ] = {};
     uint32_t     traceLaunches = 0;
     bool         collectStats  = false;
@@ -36,7 +36,7 @@ struct PathTracer
     // After the stream has finished the frame: traversal time summed over the
     // frame's launches, and the node / triangle counts when collected.
     float    TraversalMs() const;
-    void     TraversalCounts( uint64_t* nodes, uint64_t* tris ) const;
+    void     TraversalCounts( uint64_t* nodes, uint64_t* tris, uint32_t* errors ) const;
     void Resize( uint32_t w, uint32_t h );
     void UploadLights( const SceneCollector& sc, hipStream_t s );
 

@@ -1007,20 +1007,16 @@ float PathTracer::TraversalMs() const
     return total;
 }
 
-void PathTracer::TraversalCounts( uint64_t* nodes, uint64_t* tris ) const
+void PathTracer::TraversalCounts( uint64_t* nodes, uint64_t* tris, uint32_t* errors ) const
 {
-    *nodes = *tris = 0;
-    if( !collectStats || traceLaunches == 0 )
+    uint64_t v[ 3 ] = {};
+    if( traceLaunches > 0 )
     {
-        return;
+        RG_HIP_CHECK( hipMemcpy( v, frameAccum.ptr, sizeof( v ), hipMemcpyDeviceToHost ) );
     }
-    uint64_t v[ 32 ] = {};
-    RG_HIP_CHECK( hipMemcpy( v, traceStats.ptr, size_t( traceLaunches ) * 16, hipMemcpyDeviceToHost ) );
-    for( uint32_t k = 0; k < traceLaunches; k++ )
-    {
-        *nodes += v[ 2 * k ];
-        *tris += v[ 2 * k + 1 ];
-    }
+    *nodes  = collectStats ? v[ 0 ] : 0;
+    *tris   = collectStats ? v[ 1 ] : 0;
+    *errors = uint32_t( v[ 2 ] );
 }
 
 void PathTracer::Init( hipStream_t s )
@@ -1029,7 +1025,7 @@ void PathTracer::Init( hipStream_t s )
     {
         RG_HIP_CHECK( hipEventCreate( &e ) );
     }
-    traceStats.Reserve( 16 * 16 );
+    frameAccum.Reserve( 3 * sizeof( uint64_t ) );
     blueNoise.Reserve( 128u * 128u * 128u * 4u );
     k_blue_noise<<< DivUp( 128u * 128u * 128u, THREADS ), THREADS, 0, s >>>( blueNoise.as< uint32_t >() );
     CheckLaunch( "k_blue_noise" );
@@ -1094,16 +1090,16 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
     const bool haveGeometry = scene.triCount > 0 && bvh.nodeCount > 0;
 
     traceLaunches = 0;
+    RG_HIP_CHECK( hipMemsetAsync( frameAccum.ptr, 0, 3 * sizeof( uint64_t ), s ) );
+    TraceAccum accum;
+    accum.stats  = frameAccum.as< uint64_t >();
+    accum.errors = reinterpret_cast< uint32_t* >( frameAccum.as< uint64_t >() + 2 );
     auto trace = [ & ]( RgExtRay* rr, RgExtHit* hh, uint32_t flags ) {
         if( haveGeometry )
         {
             const uint32_t k = traceLaunches < 16 ? traceLaunches : 15;
             TraceRays( bvh, tc, rr, n, hh, flags | traceFlags | ( collectStats ? RG_EXT_TRACE_COLLECT_STATS : 0u ), s,
-                       traceEv[ 2 * k ], traceEv[ 2 * k + 1 ] );
-            if( collectStats )
-            {
-                RG_HIP_CHECK( hipMemcpyAsync( traceStats.as< uint8_t >() + size_t( k ) * 16, tc.stats.ptr, 16, hipMemcpyDeviceToDevice, s ) );
-            }
+                       traceEv[ 2 * k ], traceEv[ 2 * k + 1 ], &accum );
             traceLaunches = k + 1;
         }
         else

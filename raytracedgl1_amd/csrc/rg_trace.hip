@@ -6,9 +6,10 @@
 // Execution model: a persistent grid of 256-thread blocks.  Each lane owns
 // one ray at a time; whenever lanes of a wave go idle they take the next rays
 // of the wave's current chunk, so long rays do not hold the wave's SIMD lanes
-// empty.  Chunks of 256 rays come from one global counter, one atomic per
-// chunk: a refill atomic per wave iteration on a single address serialises
-// at the memory side and was the kernel's limiter (4.5x slower at 1080p).  The traversal stack is a short LDS
+// empty.  Chunks of 64 rays come from 32 counters 256 bytes apart (block b
+// takes chunks j * 32 + b % 32), one atomic per chunk: a refill atomic per
+// wave iteration on a single address serialises at the memory side and made
+// the kernel 5.5x slower at 1080p.  The traversal stack is a short LDS
 // stack (24 entries per lane, interleaved by lane to avoid bank conflicts)
 // backed by a per-lane overflow area in global memory.
 //
@@ -28,7 +29,9 @@ constexpr int      STACK_LDS     = 24;
 constexpr int      STACK_GLOBAL  = 104;
 constexpr int      STACK_TOTAL   = STACK_LDS + STACK_GLOBAL;
 constexpr uint32_t MAX_STEPS_PER_RAY = 1u << 22;
-constexpr uint32_t RAY_CHUNK         = 256; // rays per global-counter atomic
+constexpr uint32_t RAY_CHUNK         = 64; // rays per counter atomic
+constexpr uint32_t RAY_PARTITIONS    = 32;  // ray counters
+constexpr uint32_t COUNTER_STRIDE    = 64;  // u32 between counters (256 B)
 
 struct TraceArgs
 {
@@ -38,6 +41,7 @@ struct TraceArgs
     RgExtHit*       hits;
     uint32_t        count;
     uint32_t        flags;
+    uint32_t        partitions;
     uint32_t*       rayCounter;
     uint32_t*       overflowStack;
     uint32_t*       errorFlag;
@@ -93,15 +97,15 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
     int      sp       = 0;
     uint32_t steps    = 0;
     uint32_t nodeVisits = 0, triTests = 0;
+    const uint32_t part = blockIdx.x % a.partitions; // chunks j * partitions + part
     uint32_t chunkNext = 0, chunkEnd = 0; // this wave's rays [chunkNext, chunkEnd), wave-uniform
     bool     drained   = false;           // the global counter has passed the last ray
 
     while( true )
     {
         // Refill idle lanes.  Each wave owns a chunk of RAY_CHUNK consecutive
-        // rays taken with one global atomic; idle lanes take rays from the
-        // chunk (wave-uniform bookkeeping, no atomic), so the global counter
-        // sees one atomic per RAY_CHUNK rays instead of one per refill.
+        // rays taken with one atomic on its partition's counter; idle lanes
+        // take rays from the chunk (wave-uniform bookkeeping, no atomic).
         const bool     need = !active && !exhausted;
         const uint64_t mask = __ballot( need );
         if( mask != 0ull )
@@ -117,7 +121,9 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
                 uint32_t b = 0;
                 if( lane == leader )
                 {
-                    b = atomicAdd( a.rayCounter, RAY_CHUNK );
+                    const uint32_t j = atomicAdd( a.rayCounter + part * COUNTER_STRIDE, 1u );
+                    const uint64_t g = uint64_t( j ) * a.partitions + part;
+                    b                = g * RAY_CHUNK < a.count ? uint32_t( g * RAY_CHUNK ) : a.count;
                 }
                 newBase = __shfl( b, leader, 64 );
             }
@@ -194,7 +200,7 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
         if( ++steps > MAX_STEPS_PER_RAY )
         {
             // Only a malformed tree can get here; end the ray instead of spinning.
-            atomicOr( a.errorFlag, 2u );
+            atomicOr( a.errorFlag, RG_EXT_TRACE_ERROR_STEP_LIMIT );
             node = RG_EXT_BVH_CHILD_EMPTY;
             sp   = 0;
         }
@@ -243,7 +249,7 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
                         }
                         else
                         {
-                            atomicOr( a.errorFlag, 1u ); // dropped: stack overflow
+                            atomicOr( a.errorFlag, RG_EXT_TRACE_ERROR_STACK_OVERFLOW ); // dropped entry
                             continue;
                         }
                         sp++;
@@ -324,8 +330,29 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
 
 } // namespace
 
+void ReadTraceStats( const TraceContext& s, uint64_t* nodeVisits, uint64_t* triTests )
+{
+    uint64_t v[ 2 ] = { 0, 0 };
+    if( s.stats.ptr )
+    {
+        RG_HIP_CHECK( hipMemcpy( v, s.stats.ptr, sizeof( v ), hipMemcpyDeviceToHost ) );
+    }
+    *nodeVisits = v[ 0 ];
+    *triTests   = v[ 1 ];
+}
+
+uint32_t ReadTraceErrors( const TraceContext& s )
+{
+    uint32_t e = 0;
+    if( s.counter.ptr )
+    {
+        RG_HIP_CHECK( hipMemcpy( &e, s.counter.as< uint32_t >() + RAY_PARTITIONS * COUNTER_STRIDE, sizeof( e ), hipMemcpyDeviceToHost ) );
+    }
+    return e;
+}
+
 void TraceRays( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags, hipStream_t stream,
-                hipEvent_t evBefore, hipEvent_t evAfter )
+                hipEvent_t evBefore, hipEvent_t evAfter, const TraceAccum* accum )
 {
     RG_REQUIRE( bvh.nodeCount > 0, RG_RESULT_WRONG_FUNCTION_CALL, "BVH is not built" );
     RG_REQUIRE( bvh.nodeFormat == RG_EXT_BVH_NODE_FP32, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "unsupported node format" );
@@ -344,10 +371,10 @@ void TraceRays( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, u
         s.gridBlocks = uint32_t( cus * ( perCu > 0 ? perCu : 1 ) );
     }
     const uint32_t blocks = DivUp( count, TR_THREADS ) < s.gridBlocks ? DivUp( count, TR_THREADS ) : s.gridBlocks;
-    s.counter.Reserve( 2 * sizeof( uint32_t ) );
+    s.counter.Reserve( ( RAY_PARTITIONS * COUNTER_STRIDE + 1 ) * sizeof( uint32_t ) );
     s.stats.Reserve( 2 * sizeof( uint64_t ) );
     s.overflow.Reserve( size_t( s.gridBlocks ) * TR_THREADS * STACK_GLOBAL * sizeof( uint32_t ) );
-    RG_HIP_CHECK( hipMemsetAsync( s.counter.ptr, 0, 2 * sizeof( uint32_t ), stream ) );
+    RG_HIP_CHECK( hipMemsetAsync( s.counter.ptr, 0, ( RAY_PARTITIONS * COUNTER_STRIDE + 1 ) * sizeof( uint32_t ), stream ) );
 
     TraceArgs a;
     a.nodes         = reinterpret_cast< const BvhNode8* >( bvh.nodes.ptr );
@@ -357,10 +384,17 @@ void TraceRays( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, u
     a.count         = count;
     a.flags         = flags;
     a.rayCounter    = s.counter.as< uint32_t >();
-    a.errorFlag     = s.counter.as< uint32_t >() + 1;
+    a.errorFlag     = s.counter.as< uint32_t >() + RAY_PARTITIONS * COUNTER_STRIDE;
+    a.partitions    = blocks < RAY_PARTITIONS ? blocks : RAY_PARTITIONS;
     a.overflowStack = s.overflow.as< uint32_t >();
     a.stats         = nullptr;
-    if( flags & RG_EXT_TRACE_COLLECT_STATS )
+    if( accum )
+    {
+        RG_REQUIRE( accum->errors, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "accumulator without an error word" );
+        a.errorFlag = accum->errors;
+        a.stats     = ( flags & RG_EXT_TRACE_COLLECT_STATS ) ? accum->stats : nullptr;
+    }
+    else if( flags & RG_EXT_TRACE_COLLECT_STATS )
     {
         RG_HIP_CHECK( hipMemsetAsync( s.stats.ptr, 0, 2 * sizeof( uint64_t ), stream ) );
         a.stats = s.stats.as< uint64_t >();

@@ -121,6 +121,7 @@ _SIGS = {
     "rgExtBvhGetView": (_RES, [_P, C.POINTER(RgExtBvhView)]),
     "rgExtTraceRays": (_RES, [_P, _P, _U32, _P, _U32, _P]),
     "rgExtBvhGetTraceStats": (_RES, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "rgExtBvhGetTraceErrors": (_RES, [_P, C.POINTER(C.c_uint32)]),
     "rgExtLuminanceHistogram": (_RES, [_P, _U32, _U32, _F, _F, _P, _P]),
     "rgExtLuminanceAverage": (_RES, [_P, _U32, _F, _F, _F, _U32, _P, _P]),
     "rgExtFramebuffersCreate": (_RES, [_U32, _U32, C.POINTER(_P)]),
@@ -273,6 +274,12 @@ class Bvh:
         nodes, tris = C.c_uint64(), C.c_uint64()
         call("rgExtBvhGetTraceStats", self.h, C.byref(nodes), C.byref(tris))
         return int(nodes.value), int(tris.value)
+
+    def trace_errors(self) -> int:
+        """RG_EXT_TRACE_ERROR_* bits of the last trace (0: every ray complete)."""
+        e = C.c_uint32()
+        call("rgExtBvhGetTraceErrors", self.h, C.byref(e))
+        return int(e.value)
 
     def download(self) -> dict:
         """Copies every build output to host numpy arrays."""

@@ -98,7 +98,7 @@ class RgExtFrameStats(C.Structure):
                 ("bvhRebuilt", _U32), ("bvhNodeCount", _U32), ("raysTraced", C.c_uint64),
                 ("gpuMsTotal", _F), ("gpuMsAssemble", _F), ("gpuMsBvh", _F), ("gpuMsTrace", _F),
                 ("gpuMsDenoise", _F), ("gpuMsTonemap", _F), ("avgLuminance", _F), ("gpuMsUpload", _F),
-                ("gpuMsTraversal", _F), ("traversalLaunches", _U32), ("_pad1", _U32), ("nodeVisits", C.c_uint64),
+                ("gpuMsTraversal", _F), ("traversalLaunches", _U32), ("traversalErrors", _U32), ("nodeVisits", C.c_uint64),
                 ("triangleTests", C.c_uint64)]
 
 

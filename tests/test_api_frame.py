@@ -119,7 +119,7 @@ def test_frame_statistics_and_outputs_are_sane(boxes):
         assert st.triangleCount == len(s.tris)
         assert st.bvhRebuilt == (1 if i == 0 else 0)
         assert st.raysTraced >= 5 * W1 * H1
-        assert st.traversalLaunches == 6
+        assert st.traversalLaunches == 6 and st.traversalErrors == 0
         assert 0.0 < st.gpuMsTraversal <= st.gpuMsTrace <= st.gpuMsTotal
         assert st.nodeVisits >= st.raysTraced // 2 and st.triangleTests > 0
         assert st.avgLuminance > 0.0

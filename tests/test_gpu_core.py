@@ -88,7 +88,9 @@ def _trace_gpu(b, rays, flags):
     dr = DeviceArray.from_numpy(rays)
     dh = DeviceArray(len(rays), HIT_DTYPE)
     b.trace(dr, len(rays), dh, flags)
-    return dh.download()
+    out = dh.download()
+    assert b.trace_errors() == 0, "traversal reported a dropped stack entry or a step-limit stop"
+    return out
 
 
 def _assert_same_hits(g, o, any_hit=False):
