@@ -1132,7 +1132,11 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
     // initial reservoirs (directional visibility)
     k_initial_a<<< grid2, THREADS, 0, s >>>( p, P, lv, R, pendA.as< uint4 >() );
     CheckLaunch( "k_initial_a" );
-    trace( R, Hh, RG_EXT_TRACE_ANY_HIT );
+    if( p.directionalLightExists )
+    {
+        // without a directional light every ray is null and k_initial_b reads no hit
+        trace( R, Hh, RG_EXT_TRACE_ANY_HIT );
+    }
     k_initial_b<<< grid2, THREADS, 0, s >>>( p, P, Hh, pendA.as< uint4 >() );
     CheckLaunch( "k_initial_b" );
 
