@@ -765,12 +765,19 @@ extern "C" void or_trace_rays( const OrBvh* b, const RgExtRay* rays, uint32_t co
                         h++;
                     }
                 }
-                // visit near to far (ties: smaller child reference first): push far first
+                // visit near to far: order by the 64-bit key (distance as an
+                // order-preserving unsigned integer, +0 for -0; then the child
+                // reference), push far first
+                uint64_t key[ 8 ];
+                for( int k = 0; k < h; k++ )
+                {
+                    const uint32_t u = bitcast< uint32_t >( dist[ k ] + 0.0f );
+                    const uint32_t m = ( u & 0x80000000u ) ? ~u : ( u | 0x80000000u );
+                    key[ k ]         = ( uint64_t( m ) << 32 ) | cr[ k ];
+                }
                 std::vector< int > ord( h );
                 for( int k = 0; k < h; k++ ) ord[ k ] = k;
-                std::sort( ord.begin(), ord.end(), [ & ]( int x, int y ) {
-                    return dist[ x ] < dist[ y ] || ( dist[ x ] == dist[ y ] && cr[ x ] < cr[ y ] );
-                } );
+                std::sort( ord.begin(), ord.end(), [ & ]( int x, int y ) { return key[ x ] < key[ y ]; } );
                 for( int k = h - 1; k >= 0; k-- )
                 {
                     stack.push_back( cr[ ord[ k ] ] );

@@ -300,7 +300,9 @@ struct Instance
         stats.bvhRebuilt = 0;
         if( dscene.triCount > 0 && ( changed || !dscene.bvhValid ) )
         {
-            RgExtBvhBuildParams bp = { 3, RG_EXT_BVH_NODE_FP32 };
+            // one triangle per leaf: measured fastest at 1M triangles (more
+            // nodes, fewer triangle tests per leaf visit)
+            RgExtBvhBuildParams bp = { 1, RG_EXT_BVH_NODE_FP32 };
             BuildBvh( bvh, dscene.tris.as< float >(), dscene.triCount, bp, s );
             dscene.bvhValid      = true;
             dscene.builtTriCount = dscene.triCount;

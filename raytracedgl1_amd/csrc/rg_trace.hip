@@ -48,25 +48,29 @@ struct TraceArgs
     uint64_t*       stats; // [0] node visits, [1] triangle tests (RG_EXT_TRACE_COLLECT_STATS)
 };
 
-// Orders by (distance, child reference): a total order, so the visit order
-// (and the node / triangle counts) does not depend on the sorting network.
-__device__ __forceinline__ void CmpSwap( float& da, uint32_t& ra, float& db, uint32_t& rb )
+// Children are ordered by a 64-bit key: the distance (+0 for -0) mapped to
+// an unsigned integer with the same order, then the child reference.  A
+// total order, so the visit order (and the node / triangle counts) does not
+// depend on the sorting network.
+__device__ __forceinline__ uint64_t SortKey( float d, uint32_t ref )
 {
-    if( db < da || ( db == da && rb < ra ) )
-    {
-        float    td = da;
-        uint32_t tr = ra;
-        da          = db;
-        ra          = rb;
-        db          = td;
-        rb          = tr;
-    }
+    const uint32_t u = __float_as_uint( d + 0.0f );
+    const uint32_t s = ( u & 0x80000000u ) ? ~u : ( u | 0x80000000u );
+    return ( uint64_t( s ) << 32 ) | ref;
+}
+
+__device__ __forceinline__ void CmpSwap( uint64_t& a, uint64_t& b )
+{
+    const uint64_t lo = a < b ? a : b;
+    const uint64_t hi = a < b ? b : a;
+    a                 = lo;
+    b                 = hi;
 }
 
 // Batcher odd-even merge sort network for 8 keys (19 compare-exchanges).
-__device__ __forceinline__ void Sort8( float d[ 8 ], uint32_t r[ 8 ] )
+__device__ __forceinline__ void Sort8( uint64_t k[ 8 ] )
 {
-#define CS( a, b ) CmpSwap( d[ a ], r[ a ], d[ b ], r[ b ] )
+#define CS( a, b ) CmpSwap( k[ a ], k[ b ] )
     CS( 0, 1 ); CS( 2, 3 ); CS( 4, 5 ); CS( 6, 7 );
     CS( 0, 2 ); CS( 1, 3 ); CS( 4, 6 ); CS( 5, 7 );
     CS( 1, 2 ); CS( 5, 6 );
@@ -217,8 +221,7 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
                 q[ i ] = np[ i ];
             }
             const float* f = reinterpret_cast< const float* >( q ); // lo_x hi_x lo_y hi_y lo_z hi_z child
-            float        dist[ 8 ];
-            uint32_t     ref[ 8 ];
+            uint64_t     key[ 8 ];
             int          hitCount = 0;
 #pragma unroll
             for( int k = 0; k < 8; k++ )
@@ -227,13 +230,12 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
                 float          tn;
                 const bool     inBox = SlabTest( ray, f[ k ], f[ 16 + k ], f[ 32 + k ], f[ 8 + k ], f[ 24 + k ], f[ 40 + k ], tmin, bestT, &tn );
                 const bool     h     = inBox && c != RG_EXT_BVH_CHILD_EMPTY;
-                dist[ k ] = h ? tn : INFINITY;
-                ref[ k ]  = h ? c : RG_EXT_BVH_CHILD_EMPTY;
+                key[ k ] = h ? SortKey( tn, c ) : ~0ull;
                 hitCount += h ? 1 : 0;
             }
             if( hitCount > 0 )
             {
-                Sort8( dist, ref );
+                Sort8( key );
 #pragma unroll
                 for( int k = 7; k >= 1; k-- )
                 {
@@ -241,11 +243,11 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
                     {
                         if( sp < STACK_LDS )
                         {
-                            s_stack[ sp * TR_THREADS + tid ] = ref[ k ];
+                            s_stack[ sp * TR_THREADS + tid ] = uint32_t( key[ k ] );
                         }
                         else if( sp < STACK_TOTAL )
                         {
-                            ovf[ sp - STACK_LDS ] = ref[ k ];
+                            ovf[ sp - STACK_LDS ] = uint32_t( key[ k ] );
                         }
                         else
                         {
@@ -255,7 +257,7 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
                         sp++;
                     }
                 }
-                node = ref[ 0 ];
+                node = uint32_t( key[ 0 ] );
             }
             else
             {
