@@ -226,6 +226,12 @@ typedef enum RgExtPlane
     RG_EXT_PLANE_DIS_PONG_GRADIENT,      /* G f32x4                           */
     RG_EXT_PLANE_DIS_GRADIENT_HISTORY,   /* G f32x4                           */
     RG_EXT_PLANE_GRADIENT_PREV_PIX,      /* G u32                             */
+    /* ReSTIR GI (RestirBuffers.cpp:117-133; ReservoirIndirect.h:160-258),
+     * the reference's 6- and 8-word records split into 16-byte-aligned planes */
+    RG_EXT_PLANE_INDIR_INITIAL_SAMPLE,   /* C u32x4 (position bits xyz, encoded normal)      */
+    RG_EXT_PLANE_INDIR_INITIAL_RADIANCE, /* C u32x2 (E5B9G9R9 radiance, 1/source pdf bits)   */
+    RG_EXT_PLANE_INDIR_RESERVOIR_SAMPLE, /* C u32x4 (position bits xyz, encoded normal), PREV */
+    RG_EXT_PLANE_INDIR_RESERVOIR_WEIGHT, /* C u32x4 (E5 radiance, target pdf, weight sum, M), PREV */
     RG_EXT_PLANE_COUNT
 } RgExtPlane;
 

@@ -200,7 +200,8 @@ extern "C" int or_plane_components( int plane )
         case RG_EXT_PLANE_GRADIENT_PREV_PIX: return 1;
         case RG_EXT_PLANE_METALLIC_ROUGHNESS:
         case RG_EXT_PLANE_DIFF_ACCUM_MOMENTS:
-        case RG_EXT_PLANE_GRADIENT_INPUTS: return 2;
+        case RG_EXT_PLANE_GRADIENT_INPUTS:
+        case RG_EXT_PLANE_INDIR_INITIAL_RADIANCE: return 2;
         default: return 4;
     }
 }
@@ -235,7 +236,9 @@ extern "C" int or_plane_has_prev( int plane )
         case RG_EXT_PLANE_SPEC_ACCUM_COLOR:
         case RG_EXT_PLANE_INDIR_ACCUM:
         case RG_EXT_PLANE_RESERVOIRS:
-        case RG_EXT_PLANE_GRADIENT_INPUTS: return 1;
+        case RG_EXT_PLANE_GRADIENT_INPUTS:
+        case RG_EXT_PLANE_INDIR_RESERVOIR_SAMPLE:
+        case RG_EXT_PLANE_INDIR_RESERVOIR_WEIGHT: return 1;
         default: return 0;
     }
 }

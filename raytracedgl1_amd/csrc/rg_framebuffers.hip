@@ -44,6 +44,10 @@ const PlaneDesc g_planes[ RG_EXT_PLANE_COUNT ] = {
     { LAYOUT_STRATA, 4, false },       // DIS_PONG_GRADIENT
     { LAYOUT_STRATA, 4, false },       // DIS_GRADIENT_HISTORY
     { LAYOUT_STRATA, 1, false },       // GRADIENT_PREV_PIX
+    { LAYOUT_CHECKERBOARD, 4, false }, // INDIR_INITIAL_SAMPLE
+    { LAYOUT_CHECKERBOARD, 2, false }, // INDIR_INITIAL_RADIANCE
+    { LAYOUT_CHECKERBOARD, 4, true },  // INDIR_RESERVOIR_SAMPLE
+    { LAYOUT_CHECKERBOARD, 4, true },  // INDIR_RESERVOIR_WEIGHT
 };
 } // namespace
 

@@ -294,6 +294,15 @@ struct Instance
         {
             UploadScene( scene, dscene, s );
         }
+        else
+        {
+            scene.geomUids = scene.prevGeomUids; // the resident geometry is last frame's
+        }
+        {
+            std::vector< uint32_t > geomMap;
+            scene.GeomPrevToCur( geomMap );
+            pt.UploadGeomMap( geomMap, s );
+        }
         RG_HIP_CHECK( hipEventRecord( ev[ ST_UPLOADED ], s ) );
         const bool changed = keep ? false : AssembleScene( dscene, s );
         RG_HIP_CHECK( hipEventRecord( ev[ ST_ASSEMBLED ], s ) );

@@ -21,6 +21,9 @@ struct PathTracer
     DeviceBuffer rays, hits, rays2, hits2; // W*H wavefront buffers
     DeviceBuffer pendA, pendB, pendC;      // per-pixel state between wavefronts (W*H float4 / uint4)
     DeviceBuffer lights, prevToCur;
+    DeviceBuffer geomMap;                  // last frame's geometry index -> this frame's
+    std::vector< uint32_t > geomMapHost;
+    uint32_t     geomMapCount = 0;
     DeviceBuffer grid[ 2 ];                // 16^3 * 128 light-grid reservoirs, ping-pong
     DeviceBuffer blueNoise;                // 128 layers of 128x128 RGBA8
     DeviceBuffer frameAccum;               // node visits, triangle tests (u64), error bits (u32) of the frame
@@ -39,6 +42,7 @@ struct PathTracer
     void     TraversalCounts( uint64_t* nodes, uint64_t* tris, uint32_t* errors ) const;
     void Resize( uint32_t w, uint32_t h );
     void UploadLights( const SceneCollector& sc, hipStream_t s );
+    void UploadGeomMap( const std::vector< uint32_t >& map, hipStream_t s );
 
     // One frame of ray tracing into fb (G-buffer, unfiltered direct /
     // specular / indirect, reservoirs).  Returns the number of rays traced.

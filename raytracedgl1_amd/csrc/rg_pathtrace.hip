@@ -84,6 +84,33 @@ __device__ __forceinline__ uint32_t EncodeNormal( v3 n )
     return ( j << 16 ) | k;
 }
 
+// encodeE5B9G9R9 / decodeE5B9G9R9 (Utils.h:150-207, EXT_texture_shared_exponent):
+// 9-bit mantissas, shared 5-bit exponent with bias 15; floor(log2) and the
+// powers of two are taken from the exponent bits, so both are exact.
+__device__ __forceinline__ uint32_t EncodeE5B9G9R9( v3 c )
+{
+    const float r = fminf( fmaxf( c.x, 0.0f ), 65408.0f );
+    const float g = fminf( fmaxf( c.y, 0.0f ), 65408.0f );
+    const float b = fminf( fmaxf( c.z, 0.0f ), 65408.0f );
+    const float m = fmaxf( r, fmaxf( g, b ) );
+    if( !( m > 0.0f ) )
+    {
+        return 0u;
+    }
+    const int   e    = int( ( __float_as_uint( m ) >> 23 ) & 255u ) - 127;
+    const int   expP = ( e > -16 ? e : -16 ) + 16;
+    const int   maxS = int( rintf( m * ldexpf( 1.0f, 24 - expP ) ) );
+    const int   ex   = maxS != 512 ? expP : expP + 1;
+    const float sc   = ldexpf( 1.0f, 24 - ex );
+    return ( uint32_t( ex ) << 27 ) | ( uint32_t( rintf( b * sc ) ) << 18 ) | ( uint32_t( rintf( g * sc ) ) << 9 ) | uint32_t( rintf( r * sc ) );
+}
+
+__device__ __forceinline__ v3 DecodeE5B9G9R9( uint32_t p )
+{
+    const float sc = ldexpf( 1.0f, int( p >> 27 ) - 24 );
+    return mk3( sc * float( p & 511u ), sc * float( ( p >> 9 ) & 511u ), sc * float( ( p >> 18 ) & 511u ) );
+}
+
 __device__ __forceinline__ v3 MulM4( const float* m, v3 v, float w, float* wOut )
 {
     const v3 r = mk3( m[ 0 ] * v.x + m[ 4 ] * v.y + m[ 8 ] * v.z + m[ 12 ] * w,
@@ -900,18 +927,22 @@ __global__ __launch_bounds__( THREADS ) void k_indirect_c( RgExtFrameParams p, F
     const float4 b1 = pend.at( 0, i );
     if( b1.w < 0.0f )
     {
-        Plane< float4 >( P, RG_EXT_PLANE_UNFILTERED_INDIR )[ i ] = make_float4( 0.f, 0.f, 0.f, 0.f );
+        // sky pixel: emptySampleIndirect with 1/pdf 0 (RtRaygenIndirect.inl:225-229)
+        Plane< float4 >( P, RG_EXT_PLANE_UNFILTERED_INDIR )[ i ]        = make_float4( 0.f, 0.f, 0.f, 0.f );
+        Plane< uint4 >( P, RG_EXT_PLANE_INDIR_INITIAL_SAMPLE )[ i ]     = make_uint4( 0u, 0u, 0u, 0u );
+        Plane< uint2 >( P, RG_EXT_PLANE_INDIR_INITIAL_RADIANCE )[ i ] = make_uint2( 0u, 0u );
         return;
     }
     Surface s  = FetchSurface( P, W, x, y );
     s.position = s.position + s.toViewerDir * 0.01f;
     const v3 dir1 = mk3( b1.x, b1.y, b1.z );
-    v3       samplePos, sampleRad;
+    v3       samplePos, sampleRad, sampleNrm;
     const float4 hp = pend.at( 1, i );
     if( hp.w > 0.5f )
     {
         // sky sample (RtRaygenIndirect.inl:123-131)
         samplePos = s.position + dir1 * MAX_RAY_LEN;
+        sampleNrm = -dir1;
         sampleRad = SkyColor( p );
     }
     else
@@ -958,26 +989,465 @@ __global__ __launch_bounds__( THREADS ) void k_indirect_c( RgExtFrameParams p, F
         }
         diffuse   = diffuse + second;
         samplePos = mk3( hp.x, hp.y, hp.z );
+        sampleNrm = mk3( hn.x, hn.y, hn.z );
         sampleRad = ( emis + diffuse ) * mk3( ha.x, ha.y, ha.z );
     }
-    // shade (RtRaygenIndirect.inl:164-182) with weight 1/pdf of the bounce
+    // restirIndirect_StoreInitialSample (ReservoirIndirect.h:160-177)
+    Plane< uint4 >( P, RG_EXT_PLANE_INDIR_INITIAL_SAMPLE )[ i ] =
+        make_uint4( __float_as_uint( samplePos.x ), __float_as_uint( samplePos.y ), __float_as_uint( samplePos.z ), EncodeNormal( sampleNrm ) );
+    Plane< uint2 >( P, RG_EXT_PLANE_INDIR_INITIAL_RADIANCE )[ i ] = make_uint2( EncodeE5B9G9R9( sampleRad ), __float_as_uint( b1.w ) );
+}
+
+// ------------------------------------------------------------ ReSTIR GI
+
+// SampleIndirect / ReservoirIndirect (ReservoirIndirect.h:23-36)
+struct SampleIndir
+{
+    v3 position, normal, radiance;
+};
+
+struct ReservoirIndir
+{
+    SampleIndir sel;
+    float       selTargetPdf;
+    float       weightSum;
+    uint32_t    M;
+};
+
+__device__ __forceinline__ ReservoirIndir EmptyReservoirIndir()
+{
+    ReservoirIndir r;
+    r.sel.position = r.sel.normal = r.sel.radiance = mk3( 0, 0, 0 );
+    r.selTargetPdf = r.weightSum = 0.0f;
+    r.M                          = 0u;
+    return r;
+}
+
+// calcSelectedSampleWeightIndirect (:57-60)
+__device__ __forceinline__ float SelectedWeightIndir( const ReservoirIndir& r )
+{
+    return safePositiveRcp( r.selTargetPdf ) * ( r.weightSum / float( r.M > 1u ? r.M : 1u ) );
+}
+
+// normalizeReservoirIndirect (:62-68)
+__device__ __forceinline__ void NormalizeIndir( ReservoirIndir& r, uint32_t maxM )
+{
+    r.weightSum /= float( r.M > 1u ? r.M : 1u );
+    r.M = r.M < maxM ? r.M : maxM;
+    r.weightSum *= float( r.M );
+}
+
+// updateCombinedReservoirIndirect (:90-104)
+__device__ __forceinline__ void CombineIndir( ReservoirIndir& c, const ReservoirIndir& b, float rnd )
+{
+    const float w = b.weightSum;
+    c.weightSum += w;
+    c.M += b.M;
+    if( rnd * c.weightSum < w )
+    {
+        c.sel          = b.sel;
+        c.selTargetPdf = b.selTargetPdf;
+    }
+}
+
+// updateCombinedReservoirIndirect_newSurf (:106-119)
+__device__ __forceinline__ void CombineIndirNewSurf( ReservoirIndir& c, const ReservoirIndir& b, float tpB, float rnd )
+{
+    const float w = tpB * safePositiveRcp( b.selTargetPdf ) * b.weightSum;
+    c.weightSum += w;
+    c.M += b.M;
+    if( rnd * c.weightSum < w )
+    {
+        c.sel          = b.sel;
+        c.selTargetPdf = tpB;
+    }
+}
+
+// loadInitialSampleAsReservoir (RtRaygenIndirect.inl:246-255):
+// updateReservoirIndirect (ReservoirIndirect.h:70-84) with rnd = 0.5
+__device__ __forceinline__ ReservoirIndir InitialAsReservoir( const FbPtrs& P, size_t i )
+{
+    const uint4    a = Plane< uint4 >( P, RG_EXT_PLANE_INDIR_INITIAL_SAMPLE )[ i ];
+    const uint2    b = Plane< uint2 >( P, RG_EXT_PLANE_INDIR_INITIAL_RADIANCE )[ i ];
+    SampleIndir    s{ mk3( __uint_as_float( a.x ), __uint_as_float( a.y ), __uint_as_float( a.z ) ), DecodeNormal( a.w ), DecodeE5B9G9R9( b.x ) };
+    const float    oneOverSourcePdf = __uint_as_float( b.y );
+    const float    tp               = luminance( s.radiance ); // targetPdfForIndirectSample
+    ReservoirIndir r                = EmptyReservoirIndir();
+    const float    w                = tp * oneOverSourcePdf;
+    r.weightSum += w;
+    r.M += 1u;
+    if( 0.5f * r.weightSum < w )
+    {
+        r.sel          = s;
+        r.selTargetPdf = tp;
+    }
+    return r;
+}
+
+__device__ __forceinline__ ReservoirIndir LoadReservoirIndir( const FbPtrs& P, size_t i, bool prev )
+{
+    const uint4    a = Plane< uint4 >( P, RG_EXT_PLANE_INDIR_RESERVOIR_SAMPLE, prev )[ i ];
+    const uint4    b = Plane< uint4 >( P, RG_EXT_PLANE_INDIR_RESERVOIR_WEIGHT, prev )[ i ];
+    ReservoirIndir r;
+    r.sel.position = mk3( __uint_as_float( a.x ), __uint_as_float( a.y ), __uint_as_float( a.z ) );
+    r.sel.normal   = DecodeNormal( a.w );
+    r.sel.radiance = DecodeE5B9G9R9( b.x );
+    r.selTargetPdf = __uint_as_float( b.y );
+    r.weightSum    = __uint_as_float( b.z );
+    r.M            = b.w;
+    return r;
+}
+
+// restirIndirect_StoreReservoir (:179-212): non-finite or negative weights store zeros
+__device__ __forceinline__ void StoreReservoirIndir( const FbPtrs& P, size_t i, const ReservoirIndir& r )
+{
+    uint4 a = make_uint4( 0u, 0u, 0u, 0u ), b = make_uint4( 0u, 0u, 0u, 0u );
+    if( isfinite( r.weightSum ) && r.weightSum >= 0.0f )
+    {
+        a = make_uint4( __float_as_uint( r.sel.position.x ), __float_as_uint( r.sel.position.y ), __float_as_uint( r.sel.position.z ), EncodeNormal( r.sel.normal ) );
+        b = make_uint4( EncodeE5B9G9R9( r.sel.radiance ), __float_as_uint( r.selTargetPdf ), __float_as_uint( r.weightSum ), r.M );
+    }
+    Plane< uint4 >( P, RG_EXT_PLANE_INDIR_RESERVOIR_SAMPLE )[ i ] = a;
+    Plane< uint4 >( P, RG_EXT_PLANE_INDIR_RESERVOIR_WEIGHT )[ i ] = b;
+}
+
+// getDiffuseWeight (RtRaygenIndirect.inl:207-212): smoothstep(MIN_GGX_ROUGHNESS, 0.5 + 0.25, roughness)
+__device__ __forceinline__ float DiffuseWeight( float roughness )
+{
+    const float t = clampf( ( roughness - 0.005f ) / ( 0.75f - 0.005f ), 0.0f, 1.0f );
+    return t * t * ( 3.0f - 2.0f * t );
+}
+
+// testSurfaceForReuseIndirect (RtRaygenIndirect.inl:189-203)
+__device__ __forceinline__ bool ReuseIndirOk( int lo, int hi, int H, int qx, int qy, float dCur, float dOther, v3 nCur, v3 nOther )
+{
+    return inRenderArea( qx, qy, lo, hi, H ) && ( fabsf( dCur - dOther ) / fabsf( dCur ) < 0.05f ) && ( dot( nCur, nOther ) > 0.0f );
+}
+
+// getGeometryFactorClamped (Light.h:135-138) with calcDirectionAndLengthSafe (Utils.h:93-101)
+__device__ __forceinline__ float GeometryFactorSafe( v3 n, v3 from, v3 to )
+{
+    v3          d   = to - from;
+    const float len = fmaxf( length( d ), 0.001f );
+    d               = d / len;
+    return fmaxf( 0.0f, dot( n, d ) ) * safePositiveRcp( square( len ) );
+}
+
+// RT_RAYGEN_INDIRECT_FINAL (RtRaygenIndirect.inl:257-434): temporal and
+// spatial reuse of the indirect samples, then shading of the primary
+// surface with the selected sample.
+__global__ __launch_bounds__( THREADS ) void k_indirect_final( RgExtFrameParams p, FbPtrs P )
+{
+    const int x = blockIdx.x * BX + threadIdx.x % BX, y = blockIdx.y * BY + threadIdx.x / BX;
+    const int W = p.renderWidth, H = p.renderHeight;
+    if( x >= W || y >= H )
+    {
+        return;
+    }
+    const size_t i = size_t( y ) * W + x;
+    Surface      s = FetchSurface( P, W, x, y );
+    if( s.isSky )
+    {
+        StoreReservoirIndir( P, i, EmptyReservoirIndir() );
+        return;
+    }
+    s.position          = s.position + s.toViewerDir * 0.01f; // RAY_ORIGIN_LEAK_BIAS
+    const uint32_t seed = randomSeed( x, y, p.frameId );
+    uint32_t       salt = SALT_RESAMPLE_INDIRECT_BASE;
+
+    ReservoirIndir comb = InitialAsReservoir( P, i );
+
+    int lo, hi;
+    cbRenderArea( x, W, &lo, &hi );
+    const float4 mot      = Plane< float4 >( P, RG_EXT_PLANE_MOTION )[ i ];
+    const float  depthCur = Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD )[ i ];
+    // getPrevScreenPos (ShaderCommonGLSLFunc.h:271-277) over the checkerboard half width
+    const float  px       = ( ( float( x ) + 0.5f ) * ( 1.0f / ( float( W ) / 2.0f ) ) + mot.x ) * ( float( W ) / 2.0f );
+    const float  py       = ( ( float( y ) + 0.5f ) * ( 1.0f / float( H ) ) + mot.y ) * float( H );
+    const float  dw       = DiffuseWeight( s.roughness );
+    int          spatialCount = int( 6.0f * dw ); // SPATIAL_SAMPLES_INDIR
+
+    // TEMPORAL_SAMPLES_INDIR = 1, TEMPORAL_RADIUS_INDIR_MAX = 2
+    {
+        float r4[ 4 ];
+        rnd8_4( seed, salt++, r4 );
+        const float ox = ( r4[ 0 ] * 2.0f - 1.0f ) * square( dw );
+        const float oy = ( r4[ 1 ] * 2.0f - 1.0f ) * square( dw );
+        const int   qx = int( floorf( px + ox * 2.0f ) );
+        const int   qy = int( floorf( py + oy * 2.0f ) );
+        bool        use = LdPlane( Plane< uint32_t >( P, RG_EXT_PLANE_IS_SKY ), W, H, qx, qy, 0u ) == 0u;
+        if( use )
+        {
+            const float dPrev = LdPlane( Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD, true ), W, H, qx, qy, 0.0f );
+            const v3    nPrev = DecodeNormal( LdPlane( Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL, true ), W, H, qx, qy, 0u ) );
+            use               = ReuseIndirOk( lo, hi, H, qx, qy, depthCur, dPrev - mot.z, s.normal, nPrev );
+        }
+        if( use )
+        {
+            // antilag of the indirect signal in the gradient history (strata pixels)
+            const int    GW    = ( W + 1 ) / 3 > 1 ? ( W + 1 ) / 3 : 1;
+            const int    GH    = ( H + 1 ) / 3 > 1 ? ( H + 1 ) / 3 : 1;
+            const float4 gh    = LdPlane( Plane< float4 >( P, RG_EXT_PLANE_DIS_GRADIENT_HISTORY ), GW, GH, qx / 3, qy / 3, make_float4( 0.f, 0.f, 0.f, 0.f ) );
+            if( gh.y > 0.25f )
+            {
+                spatialCount++;
+                use = false;
+            }
+        }
+        if( use )
+        {
+            ReservoirIndir t = LoadReservoirIndir( P, size_t( qy ) * W + qx, true );
+            NormalizeIndir( t, 20u );
+            CombineIndir( comb, t, rnd16( seed, salt++ ) );
+        }
+    }
+
+    // spatial reuse with the Jacobian of the sample's solid angle
+    {
+        const float radius  = mixf( 8.0f, 32.0f, clampf( float( H ) / 1080.0f, 0.0f, 1.0f ) );
+        uint32_t    nobiasM = comb.M;
+        for( int k = 0; k < spatialCount; k++ )
+        {
+            float r4[ 4 ];
+            rnd8_4( seed, salt++, r4 );
+            const int qx = x + int( ( r4[ 0 ] * 2.0f - 1.0f ) * radius );
+            const int qy = y + int( ( r4[ 1 ] * 2.0f - 1.0f ) * radius );
+            if( LdPlane( Plane< uint32_t >( P, RG_EXT_PLANE_IS_SKY ), W, H, qx, qy, 0u ) != 0u )
+            {
+                continue;
+            }
+            const float dO = LdPlane( Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD ), W, H, qx, qy, 0.0f );
+            const v3    nO = DecodeNormal( LdPlane( Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL ), W, H, qx, qy, 0u ) );
+            if( !ReuseIndirOk( lo, hi, H, qx, qy, depthCur, dO, s.normal, nO ) )
+            {
+                continue;
+            }
+            const size_t         qi  = size_t( qy ) * W + qx;
+            const ReservoirIndir rq  = InitialAsReservoir( P, qi );
+            const float4         x1q = Plane< float4 >( P, RG_EXT_PLANE_SURFACE_POSITION )[ qi ];
+            float oneOverJacobian    = safePositiveRcp( GeometryFactorSafe( rq.sel.normal, rq.sel.position, s.position ) ) *
+                                    GeometryFactorSafe( rq.sel.normal, rq.sel.position, mk3( x1q.x, x1q.y, x1q.z ) );
+            oneOverJacobian          = clampf( oneOverJacobian, 0.0f, 1.0f ); // SHIPPING_HACK
+            const float tpCur        = luminance( rq.sel.radiance ) * oneOverJacobian;
+            CombineIndirNewSurf( comb, rq, tpCur, rnd16( seed, salt++ ) );
+            if( tpCur > 0.0f )
+            {
+                nobiasM += rq.M;
+            }
+        }
+        comb.M = nobiasM;
+    }
+    StoreReservoirIndir( P, i, comb );
+
+    // shade (RtRaygenIndirect.inl:164-182)
+    const float oneOverPdf = SelectedWeightIndir( comb );
     v3          diff = mk3( 0, 0, 0 ), spec = mk3( 0, 0, 0 );
-    const v3    l    = safeNormalize( samplePos - s.position );
+    const v3    l    = safeNormalize( comb.sel.position - s.position );
     const float nl   = dot( s.normal, l );
     if( nl > 0.0f )
     {
-        diff = sampleRad * ( nl * ( 1.0f / RG_PI ) * b1.w );
-        spec = ( sampleRad * nl ) * evalBRDFSmithGGX( s.normal, s.toViewerDir, l, s.roughness, s.specularColor, true ) * b1.w;
+        diff = comb.sel.radiance * ( nl * ( 1.0f / RG_PI ) ) * oneOverPdf;
+        spec = ( comb.sel.radiance * nl ) * evalBRDFSmithGGX( s.normal, s.toViewerDir, l, s.roughness, s.specularColor, true ) * oneOverPdf;
     }
-    float4&      us     = Plane< float4 >( P, RG_EXT_PLANE_UNFILTERED_SPECULAR )[ i ];
-    const v3     direct = mk3( us.x, us.y, us.z );
+    float4&  us     = Plane< float4 >( P, RG_EXT_PLANE_UNFILTERED_SPECULAR )[ i ];
+    const v3 direct = mk3( us.x, us.y, us.z );
     if( luminance( direct ) < luminance( spec ) )
     {
-        Plane< float4 >( P, RG_EXT_PLANE_VIEW_DIRECTION )[ i ].w = length( samplePos - s.position );
+        Plane< float4 >( P, RG_EXT_PLANE_VIEW_DIRECTION )[ i ].w = length( comb.sel.position - s.position );
     }
     const v3 sc2 = s.specularColor;
     us           = make_float4( direct.x + spec.x / fmaxf( 0.01f, sc2.x ), direct.y + spec.y / fmaxf( 0.01f, sc2.y ), direct.z + spec.z / fmaxf( 0.01f, sc2.z ), 0.0f );
     Plane< float4 >( P, RG_EXT_PLANE_UNFILTERED_INDIR )[ i ] = make_float4( diff.x, diff.y, diff.z, 0.0f );
+}
+
+
+// ------------------------------------------------------------ gradient samples
+
+// getAntilagAlpha (ShaderCommonGLSLFunc.h:383-390)
+__device__ __forceinline__ float AntilagAlpha( float gradSample, float normFactor )
+{
+    const float lambda = normFactor > 0.0001f ? clampf( fabsf( gradSample ) / normFactor, 0.0f, 1.0f ) : 0.0f;
+    return clampf( lambda, 0.0f, 1.0f );
+}
+
+struct GradPend
+{
+    float4* lum; // forward-projected diffuse / specular luminance before visibility, previous luminance
+    uint4*  pix; // prev shading pixel x, y, strata position, state (0 rejected, 1 no shadow ray, 2 shadow ray)
+};
+
+__device__ __forceinline__ void StoreGradient( const FbPtrs& P, int GW, int gx, int gy, float4 g, uint32_t strataPos )
+{
+    Plane< float4 >( P, RG_EXT_PLANE_DIS_PING_GRADIENT )[ size_t( gy ) * GW + gx ] = g;
+    Plane< uint32_t >( P, RG_EXT_PLANE_GRADIENT_PREV_PIX )[ size_t( gy ) * GW + gx ] = strataPos;
+}
+
+// RtGradients.rgen:44-131: per 3x3 stratum, the brightest pixel of the
+// previous frame (not the one used last frame) is re-shaded with its old
+// reservoir, light point and seed on the surface as it is now.
+__global__ __launch_bounds__( THREADS ) void k_grad_a( RgExtFrameParams p, FbPtrs P, SceneView sc, LightView lv, const uint32_t* geomPrevToCur,
+                                                       uint32_t geomPrevCount, uint32_t geomCurCount, RgExtRay* rays, GradPend pend )
+{
+    const int W = p.renderWidth, H = p.renderHeight;
+    const int GW = ( W + 1 ) / 3 > 1 ? ( W + 1 ) / 3 : 1, GH = ( H + 1 ) / 3 > 1 ? ( H + 1 ) / 3 : 1;
+    const int gx = blockIdx.x * BX + threadIdx.x % BX, gy = blockIdx.y * BY + threadIdx.x / BX;
+    if( gx >= GW || gy >= GH )
+    {
+        return;
+    }
+    const size_t gi = size_t( gy ) * GW + gx;
+    rays[ gi ]      = NullRay();
+    pend.pix[ gi ]  = make_uint4( 0u, 0u, 10u, 0u );
+    const float4 zero = make_float4( 0.f, 0.f, 0.f, 0.f );
+    if( gx * 3 >= W || gy * 3 >= H )
+    {
+        return;
+    }
+    if( p.gradientMultDiffuse < 0.001f && p.gradientMultIndirect < 0.001f && p.gradientMultSpecular < 0.001f )
+    {
+        StoreGradient( P, GW, gx, gy, zero, 10u );
+        return;
+    }
+    const uint32_t prevStrataPos = Plane< uint32_t >( P, RG_EXT_PLANE_GRADIENT_PREV_PIX )[ gi ];
+    int            psx = 0, psy = 0, csx = 0, csy = 0;
+    uint32_t       curStrataPos = 0;
+    float2         prevLum      = make_float2( 0.f, 0.f );
+    for( int yy = 0; yy < 3; yy++ )
+    {
+        for( int xx = 0; xx < 3; xx++ )
+        {
+            if( uint32_t( yy * 3 + xx ) == prevStrataPos )
+            {
+                continue;
+            }
+            const int    cx  = gx * 3 + xx, cy = gy * 3 + yy;
+            const float4 mot = LdPlane( Plane< float4 >( P, RG_EXT_PLANE_MOTION ), W, H, cx, cy, zero );
+            const int    qx  = int( floorf( ( ( float( cx ) + 0.5f ) * ( 1.0f / ( float( W ) / 2.0f ) ) + mot.x ) * ( float( W ) / 2.0f ) ) );
+            const int    qy  = int( floorf( ( ( float( cy ) + 0.5f ) * ( 1.0f / float( H ) ) + mot.y ) * float( H ) ) );
+            const float2 l   = LdPlane( Plane< float2 >( P, RG_EXT_PLANE_GRADIENT_INPUTS, true ), W, H, qx, qy, make_float2( 0.f, 0.f ) );
+            if( fmaxf( l.x, l.y ) > fmaxf( prevLum.x, prevLum.y ) )
+            {
+                psx = qx, psy = qy, csx = cx, csy = cy;
+                curStrataPos = uint32_t( yy * 3 + xx );
+                prevLum      = l;
+            }
+        }
+    }
+    int lo, hi;
+    cbRenderArea( csx, W, &lo, &hi );
+    if( !inRenderArea( psx, psy, lo, hi, H ) || LdPlane( Plane< uint32_t >( P, RG_EXT_PLANE_IS_SKY ), W, H, csx, csy, 0u ) != 0u ||
+        ( prevLum.x <= 0.0f && prevLum.y <= 0.0f ) )
+    {
+        StoreGradient( P, GW, gx, gy, zero, 10u );
+        return;
+    }
+    const size_t   pi      = size_t( psy ) * W + psx;
+    const uint32_t oldSeed = randomSeed( psx, psy, p.frameId - 1u );
+
+    // fetchGbufferSurface_NoAlbedoViewDir_Prev (Surface.inl:72-90)
+    Surface ps;
+    ps.isSky           = false;
+    ps.albedo          = mk3( 1, 1, 1 );
+    const float4 pos   = Plane< float4 >( P, RG_EXT_PLANE_SURFACE_POSITION, true )[ pi ];
+    const float2 mr    = Plane< float2 >( P, RG_EXT_PLANE_METALLIC_ROUGHNESS, true )[ pi ];
+    ps.position        = mk3( pos.x, pos.y, pos.z );
+    ps.instCustomIndex = __float_as_uint( pos.w );
+    ps.specularColor   = SpecularColor( ps.albedo, mr.x );
+    ps.roughness       = mr.y;
+    ps.normal          = DecodeNormal( Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL, true )[ pi ] );
+
+    // unpackPrevVisibilityBuffer (VertexData.inl:636-658): the previous hit's
+    // triangle as it is in this frame
+    const uint4 vb      = Plane< uint4 >( P, RG_EXT_PLANE_VISIBILITY_BUFFER, true )[ pi ];
+    bool        matched = false;
+    if( vb.x == 1u )
+    {
+        const uint32_t pg    = vb.y & 0xFFFu;
+        const uint32_t local = vb.y >> 12;
+        const uint32_t cg    = pg < geomPrevCount ? geomPrevToCur[ pg ] : 0xFFFFFFFFu;
+        if( cg < geomCurCount && local < sc.geoms[ cg ].triCount )
+        {
+            const float* t  = sc.tris + size_t( sc.geoms[ cg ].firstTri + local ) * 9;
+            const float  u  = __uint_as_float( vb.z ), v = __uint_as_float( vb.w );
+            const float  b0 = 1.0f - u - v;
+            ps.position     = mk3( t[ 0 ], t[ 1 ], t[ 2 ] ) * b0 + mk3( t[ 3 ], t[ 4 ], t[ 5 ] ) * u + mk3( t[ 6 ], t[ 7 ], t[ 8 ] ) * v;
+            matched         = true;
+        }
+    }
+    ps.toViewerDir = normalize( mk3( p.cameraPositionPrev[ 0 ], p.cameraPositionPrev[ 1 ], p.cameraPositionPrev[ 2 ] ) - ps.position );
+
+    float lumD = 0.0f, lumS = 0.0f;
+    uint32_t state = 1u;
+    if( matched )
+    {
+        Reservoir r = unpackReservoir( Plane< uint4 >( P, RG_EXT_PLANE_RESERVOIRS, true )[ pi ] );
+        if( isValid( r ) )
+        {
+            const uint32_t cur = r.selected < lv.prevCount ? lv.prevToCur[ r.selected ] : 0xFFFFFFFFu;
+            if( cur != 0xFFFFFFFFu && cur != LIGHT_INDEX_NONE && DirectValid( p, 0 ) )
+            {
+                r.selected = cur;
+                // processDirectIllumination for LIGHT_SAMPLE_METHOD_GRADIENTS
+                // (RaygenCommon.h:742-760) -> traceDirectIllumination (:639-674)
+                float pu, pv;
+                LightPointRnd( oldSeed, &pu, &pv );
+                const LightSample ls = sampleLight( GetLight( lv, cur ), ps.position, pu, pv, p.polyLightSpotlightFactor );
+                v3                d, sp;
+                shadeLight( ps, ls, selectedSampleWeight( r ), false, &d, &sp );
+                if( luminance( d + sp ) > 0.0f )
+                {
+                    lumD = luminance( d );
+                    lumS = luminance( sp );
+                    if( 0u < p.maxBounceShadowsLights )
+                    {
+                        rays[ gi ] = ShadowRay( ps, ls.position );
+                        state      = 2u;
+                    }
+                }
+            }
+        }
+    }
+    pend.lum[ gi ] = make_float4( lumD, lumS, prevLum.x, prevLum.y );
+    pend.pix[ gi ] = make_uint4( uint32_t( psx ), uint32_t( psy ), curStrataPos, state );
+}
+
+// RtGradients.rgen:133-181: gradients of the three signals, temporally smoothed.
+__global__ __launch_bounds__( THREADS ) void k_grad_b( RgExtFrameParams p, FbPtrs P, const RgExtHit* hits, GradPend pend )
+{
+    const int W = p.renderWidth, H = p.renderHeight;
+    const int GW = ( W + 1 ) / 3 > 1 ? ( W + 1 ) / 3 : 1, GH = ( H + 1 ) / 3 > 1 ? ( H + 1 ) / 3 : 1;
+    const int gx = blockIdx.x * BX + threadIdx.x % BX, gy = blockIdx.y * BY + threadIdx.x / BX;
+    if( gx >= GW || gy >= GH )
+    {
+        return;
+    }
+    const size_t gi = size_t( gy ) * GW + gx;
+    const uint4  px = pend.pix[ gi ];
+    if( px.w == 0u )
+    {
+        return; // rejected: k_grad_a stored the zero gradient
+    }
+    const float4 l    = pend.lum[ gi ];
+    const float  vis  = ( px.w == 2u && hits[ gi ].primitiveId != RG_EXT_MISS ) ? 0.0f : 1.0f;
+    const float  fwdD = l.x * vis, fwdS = l.y * vis;
+    float        g[ 3 ];
+    g[ 0 ] = AntilagAlpha( fwdD - l.z, fmaxf( fwdD, l.z ) );
+    {
+        // indirect: only a sudden darkening of the direct diffuse, unless bright
+        float       gs         = fmaxf( 0.0f, l.z - fwdD );
+        const float4 ia        = Plane< float4 >( P, RG_EXT_PLANE_INDIR_ACCUM, true )[ size_t( px.y ) * W + px.x ];
+        const float brightness = luminance( mk3( ia.x, ia.y, ia.z ) );
+        const float t          = clampf( ( brightness - 1.0f ) / ( 10.0f - 1.0f ), 0.0f, 1.0f );
+        gs *= 1.0f - t * t * ( 3.0f - 2.0f * t );
+        g[ 1 ] = AntilagAlpha( gs, fmaxf( fwdD, l.z ) );
+    }
+    g[ 2 ] = AntilagAlpha( fwdS - l.w, fmaxf( fwdS, l.w ) );
+    g[ 0 ] *= p.gradientMultDiffuse;
+    g[ 1 ] *= p.gradientMultIndirect;
+    g[ 2 ] *= p.gradientMultSpecular;
+    const float4 hp = LdPlane( Plane< float4 >( P, RG_EXT_PLANE_DIS_GRADIENT_HISTORY ), GW, GH, int( px.x ) / 3, int( px.y ) / 3, make_float4( 0.f, 0.f, 0.f, 0.f ) );
+    const float4 out = make_float4( mixf( g[ 0 ], hp.x, 0.5f ), mixf( g[ 1 ], hp.y, 0.5f ), mixf( g[ 2 ], hp.z, 0.5f ), 0.0f );
+    StoreGradient( P, GW, gx, gy, out, px.z );
 }
 
 } // namespace
@@ -1061,6 +1531,14 @@ void PathTracer::Resize( uint32_t w, uint32_t h )
     pendC.Reserve( n * sizeof( RgExtHit ) );
 }
 
+void PathTracer::UploadGeomMap( const std::vector< uint32_t >& map, hipStream_t s )
+{
+    geomMapHost = map; // stays alive until the frame's stream is synchronised
+    geomMap.Reserve( geomMapHost.size() * 4 + 4 );
+    geomMapCount = uint32_t( geomMapHost.size() );
+    RG_HIP_CHECK( hipMemcpyAsync( geomMap.ptr, geomMapHost.data(), geomMapHost.size() * 4, hipMemcpyHostToDevice, s ) );
+}
+
 void PathTracer::UploadLights( const SceneCollector& sc, hipStream_t s )
 {
     lightArraySize = uint32_t( sc.lights.size() );
@@ -1094,26 +1572,26 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
     TraceAccum accum;
     accum.stats  = frameAccum.as< uint64_t >();
     accum.errors = reinterpret_cast< uint32_t* >( frameAccum.as< uint64_t >() + 2 );
-    auto trace = [ & ]( RgExtRay* rr, RgExtHit* hh, uint32_t flags ) {
+    auto trace = [ & ]( RgExtRay* rr, RgExtHit* hh, uint32_t flags, uint32_t count ) {
         if( haveGeometry )
         {
             const uint32_t k = traceLaunches < 16 ? traceLaunches : 15;
-            TraceRays( bvh, tc, rr, n, hh, flags | traceFlags | ( collectStats ? RG_EXT_TRACE_COLLECT_STATS : 0u ), s,
+            TraceRays( bvh, tc, rr, count, hh, flags | traceFlags | ( collectStats ? RG_EXT_TRACE_COLLECT_STATS : 0u ), s,
                        traceEv[ 2 * k ], traceEv[ 2 * k + 1 ], &accum );
             traceLaunches = k + 1;
         }
         else
         {
             // no geometry: every ray misses
-            RG_HIP_CHECK( hipMemsetAsync( hh, 0xFF, size_t( n ) * sizeof( RgExtHit ), s ) );
+            RG_HIP_CHECK( hipMemsetAsync( hh, 0xFF, size_t( count ) * sizeof( RgExtHit ), s ) );
         }
-        traced += n;
+        traced += count;
     };
 
     // primary
     k_primary_rays<<< grid2, THREADS, 0, s >>>( p, R );
     CheckLaunch( "k_primary_rays" );
-    trace( R, Hh, RG_EXT_TRACE_CLOSEST );
+    trace( R, Hh, RG_EXT_TRACE_CLOSEST, n );
     k_primary_shade<<< grid2, THREADS, 0, s >>>( p, Hh, sv, P );
     CheckLaunch( "k_primary_shade" );
 
@@ -1135,7 +1613,7 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
     if( p.directionalLightExists )
     {
         // without a directional light every ray is null and k_initial_b reads no hit
-        trace( R, Hh, RG_EXT_TRACE_ANY_HIT );
+        trace( R, Hh, RG_EXT_TRACE_ANY_HIT, n );
     }
     k_initial_b<<< grid2, THREADS, 0, s >>>( p, P, Hh, pendA.as< uint4 >() );
     CheckLaunch( "k_initial_b" );
@@ -1145,7 +1623,7 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
     float4* ps = pendA.as< float4 >() + n;
     k_direct_a<<< grid2, THREADS, 0, s >>>( p, P, lv, R, pd, ps );
     CheckLaunch( "k_direct_a" );
-    trace( R, Hh, RG_EXT_TRACE_ANY_HIT );
+    trace( R, Hh, RG_EXT_TRACE_ANY_HIT, n );
     k_direct_b<<< grid2, THREADS, 0, s >>>( p, P, R, Hh, pd, ps );
     CheckLaunch( "k_direct_b" );
 
@@ -1155,13 +1633,27 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
     RgExtHit* shadowH = pendC.as< RgExtHit >();
     k_indirect_a<<< grid2, THREADS, 0, s >>>( p, P, lv, R, ip );
     CheckLaunch( "k_indirect_a" );
-    trace( R, Hh, RG_EXT_TRACE_CLOSEST );
+    trace( R, Hh, RG_EXT_TRACE_CLOSEST, n );
     k_indirect_b<<< grid2, THREADS, 0, s >>>( p, P, sv, lv, R, Hh, shadowR, R2, ip );
     CheckLaunch( "k_indirect_b" );
-    trace( shadowR, shadowH, RG_EXT_TRACE_ANY_HIT );
-    trace( R2, H2, RG_EXT_TRACE_CLOSEST );
+    trace( shadowR, shadowH, RG_EXT_TRACE_ANY_HIT, n );
+    trace( R2, H2, RG_EXT_TRACE_CLOSEST, n );
     k_indirect_c<<< grid2, THREADS, 0, s >>>( p, P, sv, lv, shadowH, R2, H2, Hh, ip );
     CheckLaunch( "k_indirect_c" );
+    k_indirect_final<<< grid2, THREADS, 0, s >>>( p, P );
+    CheckLaunch( "k_indirect_final" );
+
+    // gradient samples (CalculateGradientsSamples, PathTracer.cpp:196-219)
+    {
+        const uint32_t GW = ( W + 1 ) / 3 > 1 ? ( W + 1 ) / 3 : 1, GH = ( H + 1 ) / 3 > 1 ? ( H + 1 ) / 3 : 1;
+        const dim3     gridG( DivUp( GW, BX ), DivUp( GH, BY ) );
+        GradPend       gp{ pendA.as< float4 >(), reinterpret_cast< uint4* >( pendA.as< float4 >() + size_t( GW ) * GH ) };
+        k_grad_a<<< gridG, THREADS, 0, s >>>( p, P, sv, lv, geomMap.as< uint32_t >(), geomMapCount, scene.geomCount, R, gp );
+        CheckLaunch( "k_grad_a" );
+        trace( R, Hh, RG_EXT_TRACE_ANY_HIT, GW * GH );
+        k_grad_b<<< gridG, THREADS, 0, s >>>( p, P, Hh, gp );
+        CheckLaunch( "k_grad_b" );
+    }
     ( void )timing;
     return traced;
 }

@@ -108,6 +108,8 @@ struct SceneCollector
     // uniqueObjectID << 32 | primitiveIndexInMesh -> model of the previous frame
     // (UniqueID::MakeForPrimitive, Source/UniqueID.h:29-33)
     std::unordered_map< uint64_t, std::array< float, 12 > > prevModels, curModels;
+    // the same id per geometry index, this frame and the previous one
+    std::vector< uint64_t > geomUids, prevGeomUids;
 
     // lights of this and the previous frame, matched by uniqueID
     std::vector< LightEncoded >                 lights;      // [0] directional (or NONE), then regular
@@ -125,6 +127,9 @@ struct SceneCollector
     void AddPolygonal( const RgPolygonalLightUploadInfo& info );
     // Builds prevToCur; call once all lights of the frame are added.
     void FinishLights();
+    // Last frame's geometry index -> this frame's, 0xFFFFFFFF when gone
+    // (getCurrentGeometryIndexByPrev, Shaders/VertexData.inl:636-658).
+    void GeomPrevToCur( std::vector< uint32_t >& out ) const;
 };
 
 // Device side of the assembled frame geometry.

@@ -71,6 +71,8 @@ void SceneCollector::BeginFrame()
     triCount = 0;
     prevModels.swap( curModels );
     curModels.clear();
+    prevGeomUids.swap( geomUids );
+    geomUids.clear();
 
     prevLightIndex.clear();
     for( uint32_t i = 0; i < lights.size(); i++ )
@@ -162,6 +164,7 @@ void SceneCollector::AddPrimitive( const RgMeshInfo& mesh, const RgMeshPrimitive
         std::memcpy( indices.Append( prim.indexCount ), prim.pIndices, size_t( prim.indexCount ) * 4 );
     }
     *geoms.Append( 1 ) = g;
+    geomUids.push_back( uid );
     triCount += tris;
 }
 
@@ -288,6 +291,25 @@ void SceneCollector::FinishLights()
         if( it != prevLightIndex.end() && it->second < prevToCur.size() )
         {
             prevToCur[ it->second ] = i;
+        }
+    }
+}
+
+void SceneCollector::GeomPrevToCur( std::vector< uint32_t >& out ) const
+{
+    std::unordered_map< uint64_t, uint32_t > cur;
+    cur.reserve( geomUids.size() );
+    for( uint32_t i = 0; i < geomUids.size(); i++ )
+    {
+        cur.emplace( geomUids[ i ], i ); // keeps the first of duplicate ids
+    }
+    out.assign( prevGeomUids.size() > 0 ? prevGeomUids.size() : 1, 0xFFFFFFFFu );
+    for( uint32_t i = 0; i < prevGeomUids.size(); i++ )
+    {
+        auto it = cur.find( prevGeomUids[ i ] );
+        if( it != cur.end() )
+        {
+            out[ i ] = it->second;
         }
     }
 }

@@ -23,6 +23,7 @@ constexpr uint32_t SALT_LIGHT_GRID_BASE       = 24;
 constexpr uint32_t SALT_INITIAL_RESERVOIRS    = 48;
 constexpr uint32_t SALT_LIGHT_CHOOSE_DIRECT   = 72;
 constexpr uint32_t SALT_LIGHT_CHOOSE_INDIRECT = 96;
+constexpr uint32_t SALT_RESAMPLE_INDIRECT_BASE = 132;
 
 struct Surface
 {

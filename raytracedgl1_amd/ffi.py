@@ -351,6 +351,10 @@ PLANES = [
     ("DIS_PONG_GRADIENT", 2, 4, np.float32, False),
     ("DIS_GRADIENT_HISTORY", 2, 4, np.float32, False),
     ("GRADIENT_PREV_PIX", 2, 1, np.uint32, False),
+    ("INDIR_INITIAL_SAMPLE", 0, 4, np.uint32, False),
+    ("INDIR_INITIAL_RADIANCE", 0, 2, np.uint32, False),
+    ("INDIR_RESERVOIR_SAMPLE", 0, 4, np.uint32, True),
+    ("INDIR_RESERVOIR_WEIGHT", 0, 4, np.uint32, True),
 ]
 PLANE = {name: i for i, (name, *_rest) in enumerate(PLANES)}
 

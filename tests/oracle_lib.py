@@ -180,7 +180,7 @@ class OracleBvh:
         return hits
 
 
-PLANE_COUNT = 37
+PLANE_COUNT = 41
 
 
 class OrFramebuffersC(C.Structure):

@@ -119,7 +119,7 @@ def test_frame_statistics_and_outputs_are_sane(boxes):
         assert st.triangleCount == len(s.tris)
         assert st.bvhRebuilt == (1 if i == 0 else 0)
         assert st.raysTraced >= 5 * W1 * H1
-        assert st.traversalLaunches == 6 and st.traversalErrors == 0
+        assert st.traversalLaunches == 7 and st.traversalErrors == 0  # 6 ray sets + gradient samples
         assert 0.0 < st.gpuMsTraversal <= st.gpuMsTrace <= st.gpuMsTotal
         assert st.nodeVisits >= st.raysTraced // 2 and st.triangleTests > 0
         assert st.avgLuminance > 0.0
@@ -203,3 +203,61 @@ def test_protocol_errors(boxes):
     assert l.rgStartFrame(C.c_void_p(h.value), None) == RG_RESULT_WRONG_INSTANCE
     r2 = rtgl.Renderer(64, 32)
     r2.close()
+
+
+def _render_seq(uploader_seq, cams, width=W1, height=H1):
+    """Frames with a different uploader per frame (geometry or lights change)."""
+    r = rtgl.Renderer(width, height)
+    try:
+        stats = [r.frame(up, c) for up, c in zip(uploader_seq, cams)]
+        fb = r.framebuffers()
+        out = {name: fb.download(PLANE[name]) for name in ("DIS_GRADIENT_HISTORY", "IS_SKY", "INDIR_INITIAL_RADIANCE",
+                                                             "INDIR_RESERVOIR_WEIGHT", "INDIR_RESERVOIR_SAMPLE",
+                                                             "UNFILTERED_INDIR", "GRADIENT_PREV_PIX")}
+        return stats, out
+    finally:
+        r.close()
+
+
+def test_restir_gi_reservoirs_are_well_formed(boxes, oracle):
+    print("api: ReSTIR GI reservoirs terrain-boxes", flush=True)
+    _, up = boxes
+    _, out = _render_seq([up] * 4, [scenes.terrain_boxes_camera()] * 4)
+    sky = out["IS_SKY"][..., 0] != 0
+    w = out["INDIR_RESERVOIR_WEIGHT"]
+    M = w[..., 3]
+    wsum = w[..., 2].view(np.float32)
+    tp = w[..., 1].view(np.float32)
+    # sky pixels store the empty reservoir; others carry at least their own sample
+    assert (M[sky] == 0).all() and (wsum[sky] == 0).all()
+    assert (M[~sky] >= 1).all()
+    # temporal history is renormalised to 20 before adding 1 initial + at most 7 spatial
+    assert M.max() <= 20 + 1 + 7 * 21
+    assert np.isfinite(wsum).all() and (wsum >= 0).all() and (tp >= 0).all()
+    ind = out["UNFILTERED_INDIR"][..., :3]
+    assert np.isfinite(ind).all() and (ind >= 0).all() and ind[~sky].max() > 0
+    # the kernels' E5B9G9R9 encodings are canonical: re-encoding their decoded
+    # value with the oracle's EXT_texture_shared_exponent encoder reproduces them
+    codes = np.unique(np.concatenate([out["INDIR_INITIAL_RADIANCE"][..., 0].ravel(), w[..., 0].ravel()]))
+    codes = codes[:: max(1, len(codes) // 4000)]
+    for c in codes:
+        assert oracle.encode_e5(oracle.decode_e5(int(c))) == int(c), hex(int(c))
+
+
+def test_gradients_stay_low_when_static_and_rise_when_light_goes(boxes):
+    print("api: A-SVGF gradient samples terrain-boxes", flush=True)
+    s, up = boxes
+    dark = rtgl.SceneUploader(scenes.Scene(s.tris, s.normals, s.colors, s.objects, {}))
+    cam = scenes.terrain_boxes_camera()
+    _, still = _render_seq([up] * 5, [cam] * 5)
+    _, lost = _render_seq([up] * 4 + [dark], [cam] * 5)
+    g_still = still["DIS_GRADIENT_HISTORY"][..., 0]
+    g_lost = lost["DIS_GRADIENT_HISTORY"][..., 0]
+    assert np.isfinite(g_still).all() and (g_still >= 0).all() and (g_still <= 1).all()
+    # a static frame re-shades each stratum's sample to (almost) its old value
+    assert np.quantile(g_still, 0.99) < 0.05
+    # removing the sun makes the forward-projected direct light vanish
+    assert g_lost.mean() > 5 * max(g_still.mean(), 1e-4) and g_lost.max() > 0.2
+    # stratum positions are 0..8 or 10 (rejected)
+    pp = still["GRADIENT_PREV_PIX"][..., 0]
+    assert np.isin(pp, list(range(9)) + [10]).all()
