@@ -1,0 +1,55 @@
+/* CPU ORACLE - TEST INFRASTRUCTURE ONLY (see oracle.h): the ray-traced
+ * passes of a frame in FP64 (oracle_pathtrace.cpp). */
+#ifndef RG_ORACLE_PATHTRACE_H_
+#define RG_ORACLE_PATHTRACE_H_
+
+#include "oracle.h"
+#include "oracle_denoise.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One geometry (an uploaded primitive) as the frame sees it. */
+typedef struct OrGeom
+{
+    float    prevFromCur[ 12 ]; /* row-major 3x4: previous model * model^-1 */
+    uint32_t isStatic;          /* prevFromCur is the identity              */
+    uint32_t color;             /* RgColor4DPacked32                        */
+    float    emissive;
+    float    roughness;
+    float    metallic;
+    uint32_t flags;             /* RgMeshPrimitiveFlags                     */
+    uint32_t firstTri;
+    uint32_t triCount;
+} OrGeom;
+
+/* World-space triangles with their geometries and the frame's lights. */
+typedef struct OrPtScene
+{
+    const float*                        tris;     /* triCount * 9 */
+    const float*                        normals;  /* triCount * 9 */
+    const uint32_t*                     triGeom;  /* triCount     */
+    uint32_t                            triCount;
+    const OrGeom*                       geoms;
+    const uint64_t*                     geomIds;  /* uniqueObjectID << 32 | primitive index */
+    uint32_t                            geomCount;
+    const RgDirectionalLightUploadInfo* dirLight; /* may be null */
+    const RgSphericalLightUploadInfo*   spheres;
+    uint32_t                            sphereCount;
+} OrPtScene;
+
+typedef struct OrPathTracer OrPathTracer;
+OrPathTracer* or_pt_create( int width, int height );
+void          or_pt_destroy( OrPathTracer* pt );
+/* The traced passes of one frame into fb's current planes (previous-frame
+ * planes are read from fb's prev planes); light grid and light / geometry
+ * matches persist in pt.  p supplies the camera and settings; its light
+ * counts are replaced by the scene's. */
+void or_pt_frame( OrPathTracer* pt, const OrPtScene* scene, const OrBvh* bvh, const RgExtFrameParams* p, const OrFramebuffers* fb, int threads );
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -58,6 +58,9 @@ class Oracle:
             "or_plane_layout": (C.c_int, [C.c_int]),
             "or_plane_has_prev": (C.c_int, [C.c_int]),
             "or_denoise_gradient_atrous": (None, [_P, _P, _U32]),
+            "or_pt_create": (_P, [C.c_int, C.c_int]),
+            "or_pt_destroy": (None, [_P]),
+            "or_pt_frame": (None, [_P, _P, _P, _P, _P, C.c_int]),
             "or_denoise_temporal": (None, [_P, _P]),
             "or_denoise_antifirefly": (None, [_P, _P]),
             "or_denoise_variance": (None, [_P, _P]),
@@ -214,3 +217,87 @@ class OracleFramebuffers:
     def run(self, name: str, params, *args):
         fn = getattr(self.o.l, "or_denoise" + ("_" + name if name else ""))
         fn(C.byref(self.c), C.byref(params), *args)
+
+    def swap(self):
+        """Start of a frame: previous-frame planes become the current frame's prev."""
+        for p in list(self.prev):
+            self.cur[p], self.prev[p] = self.prev[p], self.cur[p]
+            self.c.cur[p] = self.cur[p].ctypes.data
+            self.c.prev[p] = self.prev[p].ctypes.data
+
+
+class OrGeomC(C.Structure):
+    _fields_ = [("prevFromCur", C.c_float * 12), ("isStatic", _U32), ("color", _U32), ("emissive", _F),
+                ("roughness", _F), ("metallic", _F), ("flags", _U32), ("firstTri", _U32), ("triCount", _U32)]
+
+
+class OrPtSceneC(C.Structure):
+    _fields_ = [("tris", _P), ("normals", _P), ("triGeom", _P), ("triCount", _U32), ("geoms", _P), ("geomIds", _P),
+                ("geomCount", _U32), ("dirLight", _P), ("spheres", _P), ("sphereCount", _U32)]
+
+
+class OraclePathTracer:
+    """The FP64 oracle of a frame's traced passes (or_pt_frame) over a
+    scenes.Scene uploaded the way rtgl.SceneUploader uploads it: one
+    non-indexed primitive per object, identity transform, opaque colour,
+    roughness 1, metallic 0."""
+
+    def __init__(self, o: Oracle, scene, width: int, height: int, leaf: int = 3):
+        self.o, self.scene = o, scene
+        self.h = o.l.or_pt_create(width, height)
+        self.bvh = OracleBvh(o, scene.tris, leaf)
+        n = len(scene.tris)
+        self.tris = np.ascontiguousarray(scene.tris, np.float32)
+        self.normals = np.ascontiguousarray(scene.normals, np.float32)
+        self.tri_geom = np.zeros(n, np.uint32)
+        self.geoms = (OrGeomC * max(1, len(scene.objects)))()
+        self.ids = np.zeros(max(1, len(scene.objects)), np.uint64)
+        for g, (_, first, count) in enumerate(scene.objects):
+            self.tri_geom[first:first + count] = g
+            gr = self.geoms[g]
+            for k in range(12):
+                gr.prevFromCur[k] = 1.0 if k % 5 == 0 else 0.0
+            gr.isStatic, gr.color = 1, int(scene.colors[first]) | 0xFF000000
+            gr.emissive, gr.roughness, gr.metallic = 0.0, 1.0, 0.0
+            gr.flags, gr.firstTri, gr.triCount = 0, first, count
+            self.ids[g] = g << 32
+        self.c = OrPtSceneC(self.tris.ctypes.data, self.normals.ctypes.data, self.tri_geom.ctypes.data, n,
+                            C.cast(self.geoms, _P), self.ids.ctypes.data, len(scene.objects), None, None, 0)
+        self.scene_lights(scene)
+
+    def scene_lights(self, scene):
+        """Uses scene.lights for the next frames (same uniqueIDs as SceneUploader)."""
+        from raytracedgl1_amd import rtgl
+        lights = scene.lights or {}
+        l = rtgl.api()
+        self.sun = None
+        if lights.get("sun"):
+            sun = lights["sun"]
+            d = rtgl.RgDirectionalLightUploadInfo()
+            d.uniqueID = 1
+            d.color = l.rgUtilPackColorFloat4D(*sun["color"], 1.0)
+            d.intensity = sun["intensity"]
+            d.direction.data[:] = sun["direction"]
+            d.angularDiameterDegrees = sun["angularDiameterDegrees"]
+            self.sun = d
+        sph = lights.get("spheres", [])
+        self.spheres = (rtgl.RgSphericalLightUploadInfo * max(1, len(sph)))()
+        for i, sp in enumerate(sph):
+            s = self.spheres[i]
+            s.uniqueID = 100 + i
+            s.color = l.rgUtilPackColorFloat4D(*sp["color"], 1.0)
+            s.intensity = sp["intensity"]
+            s.position.data[:] = sp["position"]
+            s.radius = sp["radius"]
+        self.c.dirLight = C.cast(C.pointer(self.sun), _P) if self.sun else None
+        self.c.spheres = C.cast(self.spheres, _P)
+        self.c.sphereCount = len(sph)
+
+    def frame(self, params, fb: "OracleFramebuffers", threads: int = 8):
+        self.o.l.or_pt_frame(self.h, C.byref(self.c), self.bvh.h, C.byref(params), C.byref(fb.c), threads)
+
+    def __del__(self):
+        try:
+            self.o.l.or_pt_destroy(self.h)
+        except Exception:
+            pass

@@ -1,0 +1,100 @@
+"""Whole frames through the C API against the FP64 oracle of the traced passes
+(oracle/oracle_pathtrace.cpp) followed by the FP64 A-SVGF oracle.
+
+The scene is the 10,592-triangle terrain with boxes, its sun and eight sphere
+lights, at 160x90 over four frames of a slowly moving camera; the last frame
+drops the sun, so the gradient samples see a change.  Each frame the oracle
+runs with the frame constants the instance used (rgExtGetFrameParams) on its
+own previous-frame planes.
+
+Tolerances (per frame, over non-sky pixels):
+- G-buffer: sky mask, primitive ids and barycentrics bit-exact (both trace
+  the same FP32 primary rays); depth within 1e-5 relative; motion 1e-5.
+- ReSTIR DI reservoirs (selected light and M) equal on >= 99.9% of pixels.
+- Unfiltered direct light within 1e-3 relative on >= 99.9% of pixels.
+- Unfiltered specular and indirect within 1e-3 on >= 97%: bounce directions
+  are FP32 on the GPU and FP64 in the oracle, so a bounce grazing an edge can
+  hit a different triangle.
+- Gradient stratum choice equal on >= 99.9%; filtered gradients within 0.02.
+- Denoised radiance (PRE_FINAL) within 1e-2 relative on >= 80% of pixels and
+  its image mean within 1%: the a-trous filter spreads the few differing
+  indirect samples over their neighbourhood.
+"""
+import numpy as np
+import pytest
+
+from raytracedgl1_amd import ffi, rtgl, scenes
+from oracle_lib import OracleFramebuffers, OraclePathTracer
+
+pytestmark = pytest.mark.gpu
+
+W, H, FRAMES = 160, 90, 4
+P = ffi.PLANE
+
+
+def _scene_with_spheres():
+    s = scenes.terrain_boxes_scene()
+    r = np.random.default_rng(3)
+    lights = dict(s.lights)
+    lights["spheres"] = [{"color": tuple(float(c) for c in r.uniform(0.6, 1.0, 3)), "intensity": float(r.uniform(50, 300)),
+                          "position": (float(r.uniform(-20, 20)), float(r.uniform(5, 9)), float(r.uniform(-20, 20))),
+                          "radius": float(r.uniform(0.2, 1.0))} for _ in range(8)]
+    return scenes.Scene(s.tris, s.normals, s.colors, s.objects, lights)
+
+
+def _rel(g, e, scale):
+    return np.abs(g - e) / (np.abs(e) + scale)
+
+
+def test_frames_match_the_fp64_oracle(oracle):
+    print("parity: traced passes + A-SVGF vs FP64 oracle, terrain-boxes 160x90", flush=True)
+    s = _scene_with_spheres()
+    no_sun = scenes.Scene(s.tris, s.normals, s.colors, s.objects, {"spheres": s.lights["spheres"]})
+    scenes_per_frame = [s, s, s, no_sun]
+    cam0 = scenes.terrain_boxes_camera()
+    cams = [scenes.Camera(cam0.eye + np.array([0.05 * f, 0, 0.03 * f]), cam0.target + np.array([0.05 * f, 0, 0.03 * f]))
+            for f in range(FRAMES)]
+    ofb = OracleFramebuffers(oracle, W, H)
+    opt = OraclePathTracer(oracle, s, W, H)
+    r = rtgl.Renderer(W, H)
+    try:
+        for f in range(FRAMES):
+            sc = scenes_per_frame[f]
+            r.frame(rtgl.SceneUploader(sc), cams[f])
+            prm = r.frame_params()
+            fbv = r.framebuffers()
+            g = {k: fbv.download(P[k]) for k in ("IS_SKY", "VISIBILITY_BUFFER", "DEPTH_WORLD", "MOTION", "RESERVOIRS_INITIAL",
+                                                   "RESERVOIRS", "UNFILTERED_DIRECT", "UNFILTERED_SPECULAR", "UNFILTERED_INDIR",
+                                                   "GRADIENT_PREV_PIX", "DIS_GRADIENT_HISTORY", "PRE_FINAL")}
+            if f:
+                ofb.swap()
+            opt.scene_lights(sc)
+            opt.frame(prm, ofb, threads=8)
+            ofb.run("", prm)
+            oc = ofb.cur
+            sky = g["IS_SKY"][..., 0] != 0
+            hit = ~sky
+            assert np.array_equal(sky, oc[P["IS_SKY"]][..., 0] != 0), f
+            gv, ov = g["VISIBILITY_BUFFER"], oc[P["VISIBILITY_BUFFER"]]
+            assert np.array_equal(gv[hit, 1], ov[hit, 1].astype(np.uint32)), f
+            assert np.array_equal(gv[hit, 2:4].view(np.float32), ov[hit, 2:4].astype(np.float32)), f
+            assert _rel(g["DEPTH_WORLD"][hit, 0], oc[P["DEPTH_WORLD"]][hit, 0], 0).max() < 1e-5
+            assert np.abs(g["MOTION"][hit, :2] - oc[P["MOTION"]][hit, :2]).max() < 1e-5
+            for k in ("RESERVOIRS_INITIAL", "RESERVOIRS"):
+                assert np.mean(g[k][hit, 0] == oc[P[k]][hit, 0].astype(np.uint32)) >= 0.999, (f, k)
+            for k, need in (("UNFILTERED_DIRECT", 0.999), ("UNFILTERED_SPECULAR", 0.97), ("UNFILTERED_INDIR", 0.97)):
+                e, gg = oc[P[k]][hit, :3], g[k][hit, :3]
+                rr = _rel(gg, e, 1e-3 * np.abs(e).mean() + 1e-12).max(axis=1)
+                assert np.mean(rr < 1e-3) >= need, (f, k, np.mean(rr < 1e-3))
+                assert abs(gg.mean() / max(e.mean(), 1e-30) - 1.0) < 0.03, (f, k)
+            assert np.mean(g["GRADIENT_PREV_PIX"][..., 0] == oc[P["GRADIENT_PREV_PIX"]][..., 0]) >= 0.999, f
+            gh, oh = g["DIS_GRADIENT_HISTORY"][..., :3], oc[P["DIS_GRADIENT_HISTORY"]][..., :3]
+            assert np.abs(gh - oh).max() < 0.02, (f, np.abs(gh - oh).max())
+            e, gg = oc[P["PRE_FINAL"]][..., :3], g["PRE_FINAL"][..., :3]
+            rr = _rel(gg, e, 1e-3 * np.abs(e).mean()).max(axis=-1)
+            assert np.mean(rr < 1e-2) >= 0.8, (f, np.mean(rr < 1e-2))
+            assert abs(gg.mean() / e.mean() - 1.0) < 0.01, f
+        # the sun's removal in the last frame shows in the gradients
+        assert oc[P["DIS_GRADIENT_HISTORY"]][..., 0].max() > 0.1
+    finally:
+        r.close()
