@@ -232,6 +232,7 @@ typedef enum RgExtPlane
     RG_EXT_PLANE_INDIR_INITIAL_RADIANCE, /* C u32x2 (E5B9G9R9 radiance, 1/source pdf bits)   */
     RG_EXT_PLANE_INDIR_RESERVOIR_SAMPLE, /* C u32x4 (position bits xyz, encoded normal), PREV */
     RG_EXT_PLANE_INDIR_RESERVOIR_WEIGHT, /* C u32x4 (E5 radiance, target pdf, weight sum, M), PREV */
+    RG_EXT_PLANE_FINAL,                  /* R f32x4 tone-mapped LDR colour in [0,1]            */
     RG_EXT_PLANE_COUNT
 } RgExtPlane;
 
@@ -324,6 +325,14 @@ RGAPI RgResult RGCONV rgExtDenoiseVariance( RgExtFramebuffers fb, const RgExtFra
 RGAPI RgResult RGCONV rgExtDenoiseAtrous( RgExtFramebuffers fb, const RgExtFrameParams* p, uint32_t iteration, void* stream );
 /* The whole chain as Denoiser::Denoise runs it. */
 RGAPI RgResult RGCONV rgExtDenoise( RgExtFramebuffers fb, const RgExtFrameParams* p, void* stream );
+/* Final compose: CmCheckerboard.comp:49-76 and CmPrepareFinal.comp:148-176
+ * (ImageComposition.cpp:87-156). PRE_FINAL -> FINAL: checkerboard resolve of
+ * split pixels, auto exposure from *d_avgLuminance (Exposure.h:29-56) and
+ * the LPM 709->709 tone map (Shaders/LPM/ffx_lpm.h, set up as
+ * ImageComposition.cpp:377-394). tonemapping may be null (defaults). */
+RGAPI RgResult RGCONV rgExtComposeFinal( RgExtFramebuffers framebuffers, const RgDrawFrameTonemappingParams* tonemapping,
+                                         const float* d_avgLuminance, void* stream );
+
 
 /* ------------------------------------------------------ instance extras */
 /* Build-only knobs, passed in the RgDrawFrameInfo::pParams chain with this

@@ -281,6 +281,20 @@ RgResult rgExtLuminanceHistogram( const float* d_luminance, uint32_t width, uint
     } );
 }
 
+RgResult rgExtComposeFinal( RgExtFramebuffers fbs, const RgDrawFrameTonemappingParams* tonemapping, const float* d_avgLuminance, void* stream )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( fbs && d_avgLuminance, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        float sat[ 3 ] = { 0.0f, 0.0f, 0.0f }, ct[ 3 ] = { 1.0f, 1.0f, 1.0f };
+        if( tonemapping )
+        {
+            std::memcpy( sat, tonemapping->saturation.data, sizeof( sat ) );
+            std::memcpy( ct, tonemapping->crosstalk.data, sizeof( ct ) );
+        }
+        ComposeFinal( fbs->fb, MakeLpmConsts( sat, ct ), d_avgLuminance, AsStream( stream ) );
+    } );
+}
+
 RgResult rgExtLuminanceAverage( uint32_t* d_histogram, uint32_t pixelCount, float minLogLuminance, float maxLogLuminance, float timeDelta, RgBool32 stopEyeAdaptation, float* d_avgLuminance, void* stream )
 {
     return Guard( [ & ] {

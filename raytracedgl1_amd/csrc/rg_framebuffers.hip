@@ -48,6 +48,7 @@ const PlaneDesc g_planes[ RG_EXT_PLANE_COUNT ] = {
     { LAYOUT_CHECKERBOARD, 2, false }, // INDIR_INITIAL_RADIANCE
     { LAYOUT_CHECKERBOARD, 4, true },  // INDIR_RESERVOIR_SAMPLE
     { LAYOUT_CHECKERBOARD, 4, true },  // INDIR_RESERVOIR_WEIGHT
+    { LAYOUT_REGULAR, 4, false },      // FINAL
 };
 } // namespace
 

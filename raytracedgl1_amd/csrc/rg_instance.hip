@@ -328,6 +328,17 @@ struct Instance
         RG_HIP_CHECK( hipMemcpyAsync( histCopy.ptr, hist.ptr, 256 * 4, hipMemcpyDeviceToDevice, s ) );
         LuminanceAverage( hist.as< uint32_t >(), w * h, params.minLogLuminance, params.maxLogLuminance, params.timeDelta,
                           params.stopEyeAdaptation != 0, avgLum.as< float >(), s );
+        {
+            // ImageComposition::Finalize (Source/ImageComposition.cpp:94-156)
+            const auto* tm     = FindParams< RgDrawFrameTonemappingParams >( di, RG_STRUCTURE_TYPE_TONEMAPPING );
+            float       sat[ 3 ] = { 0.0f, 0.0f, 0.0f }, ct[ 3 ] = { 1.0f, 1.0f, 1.0f }; // DrawFrameInfo.h:104-105
+            if( tm )
+            {
+                std::memcpy( sat, tm->saturation.data, sizeof( sat ) );
+                std::memcpy( ct, tm->crosstalk.data, sizeof( ct ) );
+            }
+            ComposeFinal( fb->fb, MakeLpmConsts( sat, ct ), avgLum.as< float >(), s );
+        }
         RG_HIP_CHECK( hipEventRecord( ev[ ST_END ], s ) );
         // The staging arrays are reused by the next frame's uploads.
         RG_HIP_CHECK( hipStreamSynchronize( s ) );

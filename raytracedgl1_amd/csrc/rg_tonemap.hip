@@ -6,8 +6,11 @@
 // folds many pixels into its LDS bins before one global atomic per bin.
 // CmLuminanceAvg.comp:39-90 is a single 256-thread group.
 #include "rg_device_math.h"
+#include "rg_framebuffers.h"
 #include "rg_internal.h"
 #include "rg_kernels.h"
+
+#include <cmath>
 
 namespace rg
 {
@@ -99,7 +102,148 @@ int HistogramGrid()
     }
     return blocks;
 }
+// ------------------------------------------------------------ final compose
+
+constexpr int CBX = 32, CBY = 8;
+
+// LPM 709 -> 709 filter (LpmMap, Shaders/LPM/ffx_lpm.h:726-828, with
+// shoulder, con, soft and con2 off: LPM_CONFIG_709_709).
+__device__ __forceinline__ v3 LpmMap709( v3 c, const LpmConsts& k )
+{
+    const float mx = fmaxf( c.x, fmaxf( c.y, c.z ) );
+    if( !( mx > 0.0f ) )
+    {
+        return mk3( 0, 0, 0 );
+    }
+    const float rcpMax = 1.0f / mx;
+    v3          ratio  = mk3( powf( c.x * rcpMax, k.saturation[ 0 ] ), powf( c.y * rcpMax, k.saturation[ 1 ] ), powf( c.z * rcpMax, k.saturation[ 2 ] ) );
+    float       luma   = c.y * k.lumaT[ 1 ] + ( c.x * k.lumaT[ 0 ] + ( c.z * k.lumaT[ 2 ] ) );
+    luma               = powf( luma, k.contrast );
+    luma               = luma / ( luma * k.toneScaleBias[ 0 ] + k.toneScaleBias[ 1 ] );
+    const float lumaRatio  = ratio.x * k.lumaT[ 0 ] + ratio.y * k.lumaT[ 1 ] + ratio.z * k.lumaT[ 2 ];
+    const float ratioScale = saturate( luma / lumaRatio );
+    v3          col        = mk3( saturate( ratio.x * ratioScale ), saturate( ratio.y * ratioScale ), saturate( ratio.z * ratioScale ) );
+    // crosstalk: push the unclipped channels up to make up for clipping
+    const v3    cap     = mk3( k.crosstalk[ 0 ] - k.crosstalk[ 0 ] * col.x, k.crosstalk[ 1 ] - k.crosstalk[ 1 ] * col.y, k.crosstalk[ 2 ] - k.crosstalk[ 2 ] * col.z );
+    float       lumaAdd = saturate( luma - ( col.y * k.lumaT[ 1 ] + ( col.x * k.lumaT[ 0 ] + col.z * k.lumaT[ 2 ] ) ) );
+    const float t       = lumaAdd / ( cap.y * k.lumaT[ 1 ] + ( cap.x * k.lumaT[ 0 ] + cap.z * k.lumaT[ 2 ] ) );
+    col                 = mk3( saturate( t * cap.x + col.x ), saturate( t * cap.y + col.y ), saturate( t * cap.z + col.z ) );
+    lumaAdd             = saturate( luma - ( col.z * k.lumaT[ 2 ] + ( col.x * k.lumaT[ 0 ] + col.y * k.lumaT[ 1 ] ) ) );
+    return mk3( saturate( lumaAdd * k.rcpLumaT[ 0 ] + col.x ), saturate( lumaAdd * k.rcpLumaT[ 1 ] + col.y ), saturate( lumaAdd * k.rcpLumaT[ 2 ] + col.z ) );
+}
+
+__device__ __forceinline__ float4 LdReg( const float4* p, int W, int H, int x, int y )
+{
+    return ( x < 0 || y < 0 || x >= W || y >= H ) ? make_float4( 0.f, 0.f, 0.f, 0.f ) : p[ size_t( y ) * W + x ];
+}
+
+// CmCheckerboard.comp:49-76 then CmPrepareFinal.comp:148-176 (no volumetrics,
+// screen emission or acid fog; classic shading is off at the default
+// lightmapScreenCoverage of 0)
+__global__ __launch_bounds__( CBX * CBY ) void k_compose( FbPtrs P, int W, int H, LpmConsts k, const float* avgLum )
+{
+    const int x = blockIdx.x * CBX + threadIdx.x % CBX, y = blockIdx.y * CBY + threadIdx.x / CBX;
+    if( x >= W || y >= H )
+    {
+        return;
+    }
+    const float4* pre = reinterpret_cast< const float4* >( P.cur[ RG_EXT_PLANE_PRE_FINAL ] );
+    float4        c4  = pre[ size_t( y ) * W + x ];
+    int           cx, cy;
+    cbFromRegular( x, y, W, &cx, &cy );
+    const float4 thr = reinterpret_cast< const float4* >( P.cur[ RG_EXT_PLANE_THROUGHPUT ] )[ size_t( cy ) * W + cx ];
+    v3           hdr = mk3( c4.x, c4.y, c4.z );
+    if( thr.w > 0.5f ) // wasSplit: a refracted / reflected pixel is resolved with its cross
+    {
+        const int   sep  = cbSeparator( W );
+        const float l    = ( cx == 0 || cx == sep ) ? 0.0f : 1.0f;
+        const float r    = ( cx == sep || cx == W ) ? 0.0f : 1.0f;
+        const float4 a = LdReg( pre, W, H, x - 1, y ), b = LdReg( pre, W, H, x + 1, y ), u = LdReg( pre, W, H, x, y + 1 ), d = LdReg( pre, W, H, x, y - 1 );
+        const v3    cross = mk3( a.x, a.y, a.z ) * ( 0.25f * l ) + mk3( b.x, b.y, b.z ) * ( 0.25f * r ) + mk3( u.x, u.y, u.z ) * 0.25f + mk3( d.x, d.y, d.z ) * 0.25f;
+        hdr               = mix3( hdr, cross, 0.5f );
+    }
+    // ev100ToLuminousExposure(getAutoEV100()) (Exposure.h:29-56)
+    const float lum      = fmaxf( 0.0f, *avgLum );
+    const float ev100    = log2f( lum * 100.0f / 12.5f );
+    const float maxLum   = 1.2f * exp2f( ev100 );
+    const float exposure = maxLum > 0.0f ? 1.0f / maxLum : 0.0f;
+    const v3    out      = LpmMap709( hdr * exposure, k );
+    reinterpret_cast< float4* >( P.cur[ RG_EXT_PLANE_FINAL ] )[ size_t( y ) * W + x ] = make_float4( out.x, out.y, out.z, 0.0f );
+}
+
 } // namespace
+
+// LpmSetup (Shaders/LPM/LpmSetupCustom.inl, ffx_lpm.h:300-420) for the
+// reference's call (ImageComposition.cpp:377-394): LPM_CONFIG_709_709,
+// LPM_COLORS_709_709, softGap 0, hdrMax 256, exposure 8, contrast 0.1,
+// shoulder contrast 1; computed in FP64 and stored as FP32.
+LpmConsts MakeLpmConsts( const float saturation[ 3 ], const float crosstalk[ 3 ] )
+{
+    const double hdrMax = 256.0, exposure = 8.0, shoulderContrast = 1.0;
+    const double contrast = 0.1 + 1.0;
+    const double midIn = hdrMax * 0.18 * std::exp2( -exposure ), midOut = 0.18;
+    const double cs = contrast * shoulderContrast;
+    const double z0 = -std::pow( midIn, contrast );
+    const double z1 = std::pow( hdrMax, cs ) * std::pow( midIn, contrast );
+    const double z2 = std::pow( hdrMax, contrast ) * std::pow( midIn, cs ) * midOut;
+    const double z3 = std::pow( hdrMax, cs ) * midOut;
+    const double z4 = std::pow( midIn, cs ) * midOut;
+    const double tsb0 = -( ( z0 + ( midOut * ( z1 - z2 ) ) / ( z3 - z4 ) ) / z4 );
+    const double w0 = std::pow( hdrMax, cs ) * std::pow( midIn, contrast );
+    const double w1 = std::pow( hdrMax, contrast ) * std::pow( midIn, cs ) * midOut;
+    const double w2 = std::pow( hdrMax, cs ) * midOut;
+    const double w3 = std::pow( midIn, cs ) * midOut;
+    const double tsb1 = ( w0 - w1 ) / ( w2 - w3 );
+    // Y row of the Rec.709 / D65 RGB -> XYZ matrix (LpmColRgbToXyz), normalised
+    const double xy[ 4 ][ 2 ] = { { 0.64, 0.33 }, { 0.30, 0.60 }, { 0.15, 0.06 }, { 0.3127, 0.3290 } };
+    double       m[ 3 ][ 3 ]; // columns: primaries' xyz
+    for( int c = 0; c < 3; c++ )
+    {
+        m[ 0 ][ c ] = xy[ c ][ 0 ];
+        m[ 1 ][ c ] = xy[ c ][ 1 ];
+        m[ 2 ][ c ] = 1.0 - xy[ c ][ 0 ] - xy[ c ][ 1 ];
+    }
+    const double wX = xy[ 3 ][ 0 ] / xy[ 3 ][ 1 ], wY = 1.0, wZ = ( 1.0 - xy[ 3 ][ 0 ] - xy[ 3 ][ 1 ] ) / xy[ 3 ][ 1 ];
+    const double det = m[ 0 ][ 0 ] * ( m[ 1 ][ 1 ] * m[ 2 ][ 2 ] - m[ 1 ][ 2 ] * m[ 2 ][ 1 ] ) - m[ 0 ][ 1 ] * ( m[ 1 ][ 0 ] * m[ 2 ][ 2 ] - m[ 1 ][ 2 ] * m[ 2 ][ 0 ] ) +
+                       m[ 0 ][ 2 ] * ( m[ 1 ][ 0 ] * m[ 2 ][ 1 ] - m[ 1 ][ 1 ] * m[ 2 ][ 0 ] );
+    double inv[ 3 ][ 3 ];
+    inv[ 0 ][ 0 ] = ( m[ 1 ][ 1 ] * m[ 2 ][ 2 ] - m[ 1 ][ 2 ] * m[ 2 ][ 1 ] ) / det;
+    inv[ 0 ][ 1 ] = ( m[ 0 ][ 2 ] * m[ 2 ][ 1 ] - m[ 0 ][ 1 ] * m[ 2 ][ 2 ] ) / det;
+    inv[ 0 ][ 2 ] = ( m[ 0 ][ 1 ] * m[ 1 ][ 2 ] - m[ 0 ][ 2 ] * m[ 1 ][ 1 ] ) / det;
+    inv[ 1 ][ 0 ] = ( m[ 1 ][ 2 ] * m[ 2 ][ 0 ] - m[ 1 ][ 0 ] * m[ 2 ][ 2 ] ) / det;
+    inv[ 1 ][ 1 ] = ( m[ 0 ][ 0 ] * m[ 2 ][ 2 ] - m[ 0 ][ 2 ] * m[ 2 ][ 0 ] ) / det;
+    inv[ 1 ][ 2 ] = ( m[ 0 ][ 2 ] * m[ 1 ][ 0 ] - m[ 0 ][ 0 ] * m[ 1 ][ 2 ] ) / det;
+    inv[ 2 ][ 0 ] = ( m[ 1 ][ 0 ] * m[ 2 ][ 1 ] - m[ 1 ][ 1 ] * m[ 2 ][ 0 ] ) / det;
+    inv[ 2 ][ 1 ] = ( m[ 0 ][ 1 ] * m[ 2 ][ 0 ] - m[ 0 ][ 0 ] * m[ 2 ][ 1 ] ) / det;
+    inv[ 2 ][ 2 ] = ( m[ 0 ][ 0 ] * m[ 1 ][ 1 ] - m[ 0 ][ 1 ] * m[ 1 ][ 0 ] ) / det;
+    double scale[ 3 ], lumaT[ 3 ];
+    for( int c = 0; c < 3; c++ )
+    {
+        scale[ c ] = inv[ c ][ 0 ] * wX + inv[ c ][ 1 ] * wY + inv[ c ][ 2 ] * wZ;
+        lumaT[ c ] = m[ 1 ][ c ] * scale[ c ];
+    }
+    const double sum = lumaT[ 0 ] + lumaT[ 1 ] + lumaT[ 2 ];
+    LpmConsts    k;
+    for( int c = 0; c < 3; c++ )
+    {
+        lumaT[ c ] /= sum;
+        k.lumaT[ c ]      = float( lumaT[ c ] );
+        k.rcpLumaT[ c ]   = float( 1.0 / lumaT[ c ] );
+        k.saturation[ c ] = float( double( saturation[ c ] ) + contrast );
+        k.crosstalk[ c ]  = crosstalk[ c ];
+    }
+    k.contrast          = float( contrast );
+    k.toneScaleBias[ 0 ] = float( tsb0 );
+    k.toneScaleBias[ 1 ] = float( tsb1 );
+    return k;
+}
+
+void ComposeFinal( FramebufferSet& fb, const LpmConsts& k, const float* d_avgLum, hipStream_t stream )
+{
+    const dim3 grid( DivUp( fb.width, CBX ), DivUp( fb.height, CBY ) );
+    k_compose<<< grid, CBX * CBY, 0, stream >>>( fb.Ptrs(), int( fb.width ), int( fb.height ), k, d_avgLum );
+    CheckLaunch( "k_compose" );
+}
 
 void LuminanceHistogram( const float* d_lum, uint32_t count, float minLog, float maxLog, uint32_t* d_hist, hipStream_t stream )
 {

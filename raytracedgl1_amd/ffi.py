@@ -124,6 +124,7 @@ _SIGS = {
     "rgExtBvhGetTraceErrors": (_RES, [_P, C.POINTER(C.c_uint32)]),
     "rgExtLuminanceHistogram": (_RES, [_P, _U32, _U32, _F, _F, _P, _P]),
     "rgExtLuminanceAverage": (_RES, [_P, _U32, _F, _F, _F, _U32, _P, _P]),
+    "rgExtComposeFinal": (_RES, [_P, _P, _P, _P]),
     "rgExtFramebuffersCreate": (_RES, [_U32, _U32, C.POINTER(_P)]),
     "rgExtFramebuffersDestroy": (_RES, [_P]),
     "rgExtFramebuffersGetPlane": (_RES, [_P, C.c_int, _U32, C.POINTER(_P), C.POINTER(C.c_size_t)]),
@@ -355,6 +356,7 @@ PLANES = [
     ("INDIR_INITIAL_RADIANCE", 0, 2, np.uint32, False),
     ("INDIR_RESERVOIR_SAMPLE", 0, 4, np.uint32, True),
     ("INDIR_RESERVOIR_WEIGHT", 0, 4, np.uint32, True),
+    ("FINAL", 1, 4, np.float32, False),
 ]
 PLANE = {name: i for i, (name, *_rest) in enumerate(PLANES)}
 

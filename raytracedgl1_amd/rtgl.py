@@ -80,6 +80,11 @@ class RgDrawFrameSkyParams(C.Structure):
                 ("pSkyCubemapTextureName", C.c_char_p), ("skyCubemapRotationTransform", C.c_float * 9)]
 
 
+class RgDrawFrameTonemappingParams(C.Structure):
+    _fields_ = [("sType", C.c_int), ("pNext", _P), ("disableEyeAdaptation", _U32), ("ev100Min", _F), ("ev100Max", _F),
+                ("luminanceWhitePoint", _F), ("saturation", RgFloat3D), ("crosstalk", RgFloat3D)]
+
+
 class RgExtDrawFramePathParams(C.Structure):
     _fields_ = [("sType", C.c_int), ("pNext", _P), ("atrousIterations", _U32), ("restirInitialCandidates", _U32),
                 ("restirTemporalSamples", _U32), ("restirSpatialSamples", _U32),
@@ -105,8 +110,10 @@ class RgExtFrameStats(C.Structure):
 assert C.sizeof(RgInstanceCreateInfo) == 184 and C.sizeof(RgMeshInfo) == 88
 assert C.sizeof(RgMeshPrimitiveInfo) == 80 and C.sizeof(RgDrawFrameInfo) == 120
 assert C.sizeof(RgDirectionalLightUploadInfo) == 48 and C.sizeof(RgSphericalLightUploadInfo) == 48
+assert C.sizeof(RgDrawFrameTonemappingParams) == 56
 assert C.sizeof(RgExtDrawFramePathParams) == 48 and C.sizeof(RgExtFrameStats) == 96
 
+RG_STRUCTURE_TYPE_TONEMAPPING = 4
 RG_STRUCTURE_TYPE_SKY = 7
 RG_EXT_STRUCTURE_TYPE_PATH_PARAMS = 0x52470001
 RG_DRAW_FRAME_RAY_CULL_WORLD_0_BIT = 1

@@ -58,6 +58,9 @@ class Oracle:
             "or_plane_layout": (C.c_int, [C.c_int]),
             "or_plane_has_prev": (C.c_int, [C.c_int]),
             "or_denoise_gradient_atrous": (None, [_P, _P, _U32]),
+            "or_lpm_luma_coefficients": (None, [_P]),
+            "or_lpm_map": (None, [_P, _P, _P, _P]),
+            "or_compose_final": (None, [_P, _P, _P, C.c_double]),
             "or_pt_create": (_P, [C.c_int, C.c_int]),
             "or_pt_destroy": (None, [_P]),
             "or_pt_frame": (None, [_P, _P, _P, _P, _P, C.c_int]),
@@ -183,7 +186,7 @@ class OracleBvh:
         return hits
 
 
-PLANE_COUNT = 41
+PLANE_COUNT = 42
 
 
 class OrFramebuffersC(C.Structure):

@@ -1,0 +1,214 @@
+"""Final compose (checkerboard resolve, auto exposure, LPM 709->709 tone map):
+the FP64 oracle (oracle/oracle_compose.cpp) pinned by what the mathematics
+fixes (CPU), and rgExtComposeFinal against the oracle (GPU).
+
+LPM facts used:
+- its luma coefficients are the Y row of the Rec.709 / D65 RGB->XYZ matrix:
+  the textbook (0.2126, 0.7152, 0.0722);
+- on grey, the map is the tone curve f(x) = x^c / (x^c b0 + b1) with
+  c = 1 + contrast; LpmSetup chooses b0, b1 so that f(midIn) = midOut and
+  f(hdrMax) = 1, which for the reference's setup (hdrMax 256, exposure 8)
+  is f(0.18) = 0.18 and f(256) = 1;
+- for any colour whose output does not clip all three channels, the output
+  luma is f(input luma) (the map is luma preserving), and below clipping the
+  output chromaticity is (in / max(in)) ^ (1 + contrast + saturation).
+
+GPU tolerance: FINAL within 1e-4 absolute of the FP64 oracle (outputs lie
+in [0, 1]).
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from raytracedgl1_amd import ffi, rtgl
+from raytracedgl1_amd.ffi import PLANE, DeviceArray, FramebufferSet
+from oracle_lib import OracleFramebuffers
+
+ZERO3 = (C.c_float * 3)(0, 0, 0)
+ONE3 = (C.c_float * 3)(1, 1, 1)
+CONTRAST = 1.1
+
+
+def lpm(o, rgb, sat=ZERO3, ct=ONE3):
+    i = (C.c_double * 3)(*rgb)
+    out = (C.c_double * 3)()
+    o.l.or_lpm_map(sat, ct, i, out)
+    return np.array(out[:])
+
+
+def luma_coefficients(o):
+    out = (C.c_double * 3)()
+    o.l.or_lpm_luma_coefficients(out)
+    return np.array(out[:])
+
+
+def test_luma_is_rec709(oracle):
+    assert np.abs(luma_coefficients(oracle) - [0.2126, 0.7152, 0.0722]).max() < 1e-4
+    assert math.isclose(luma_coefficients(oracle).sum(), 1.0, rel_tol=1e-14)
+
+
+def test_grey_tone_curve(oracle):
+    assert np.allclose(lpm(oracle, (0.18,) * 3), 0.18, rtol=1e-12)
+    assert np.allclose(lpm(oracle, (256.0,) * 3), 1.0, rtol=1e-12)
+    xs = np.geomspace(1e-4, 200.0, 60)
+    ys = np.array([lpm(oracle, (x,) * 3) for x in xs])
+    assert (np.abs(ys - ys[:, :1]) < 1e-14).all()  # grey stays grey
+    assert (np.diff(ys[:, 0]) > 0).all()  # monotone
+    # f(x) = x^c / (x^c b0 + b1): x^c / f(x) is affine in x^c
+    xc = xs ** CONTRAST
+    q = xc / ys[:, 0]
+    b0, b1 = np.polyfit(xc, q, 1)
+    assert np.abs(q - (b0 * xc + b1)).max() < 1e-9 * q.max()
+    # grey above hdrMax clips to white (all three channels clip at once)
+    assert np.array_equal(lpm(oracle, (1000.0,) * 3), [1.0, 1.0, 1.0])
+    assert np.array_equal(lpm(oracle, (0.0,) * 3), [0.0, 0.0, 0.0])
+
+
+def test_luma_preserved_and_hue_powered(oracle):
+    lt = luma_coefficients(oracle)
+    f = lambda y: lpm(oracle, (y,) * 3)[0]
+    r = np.random.default_rng(11)
+    checked = 0
+    for _ in range(300):
+        c = r.uniform(0, 1, 3) * 10.0 ** r.uniform(-3, 2.5)
+        out = lpm(oracle, c)
+        if out.min() >= 1.0:
+            continue
+        assert math.isclose(out @ lt, f(c @ lt), rel_tol=1e-9, abs_tol=1e-12)
+        if out.max() < 1.0 - 1e-9:
+            assert np.allclose(out / out.max(), (c / c.max()) ** CONTRAST, rtol=1e-9, atol=1e-12)
+            checked += 1
+    assert checked > 100
+    # saturation shifts the power: -0.1 cancels the contrast, leaving the hue as is
+    c = np.array([0.3, 0.1, 0.02])
+    out = lpm(oracle, c, sat=(C.c_float * 3)(-0.1, -0.1, -0.1))
+    assert np.allclose(out / out.max(), c / c.max(), rtol=1e-6)
+
+
+def test_crosstalk_moves_clipped_energy(oracle):
+    # a bright saturated red clips R; crosstalk gives the missing luma to G and B
+    lt = luma_coefficients(oracle)
+    c = np.array([40.0, 1.0, 0.5])
+    out = lpm(oracle, c)
+    f = lpm(oracle, (c @ lt,) * 3)[0]
+    assert out[0] == 1.0 and out[1] > (1.0 / 40.0) ** CONTRAST
+    assert math.isclose(out @ lt, f, rel_tol=1e-9)
+    # no crosstalk on blue: green takes the clipped luma, blue stays lower
+    out0 = lpm(oracle, c, ct=(C.c_float * 3)(1, 1, 0))
+    assert math.isclose(out0 @ lt, f, rel_tol=1e-9)
+    assert out0[2] < out[2] and out0[1] > out[1]
+
+
+def _compose_oracle(o, pre, thr, avg, sat=ZERO3, ct=ONE3):
+    H, W = pre.shape[:2]
+    fb = OracleFramebuffers(o, W, H)
+    fb.set(PLANE["PRE_FINAL"], pre)
+    fb.set(PLANE["THROUGHPUT"], thr)
+    o.l.or_compose_final(C.byref(fb.c), sat, ct, float(avg))
+    return fb.cur[PLANE["FINAL"]]
+
+
+def test_compose_exposure(oracle):
+    W, H = 8, 6
+    pre = np.zeros((H, W, 4)); pre[..., :3] = 2.0
+    thr = np.zeros((H, W, 4))
+    out = _compose_oracle(oracle, pre, thr, 0.5)
+    # exposure = 1 / (1.2 * 2^log2(avg * 100 / 12.5)) = 1 / (9.6 avg)
+    assert np.allclose(out[..., :3], lpm(oracle, (2.0 / 4.8,) * 3), rtol=1e-12)
+    assert (out[..., 3] == 0).all()
+    assert (_compose_oracle(oracle, pre, thr, 0.0)[..., :3] == 0).all()
+
+
+def test_compose_checkerboard_resolve(oracle):
+    W, H = 12, 6
+    ys, xs = np.mgrid[0:H, 0:W].astype(np.float64)
+    v = 1.0 + 0.1 * xs + 0.05 * ys
+    pre = np.zeros((H, W, 4)); pre[..., :3] = v[..., None]
+    thr = np.zeros((H, W, 4)); thr[..., 3] = 1.0  # every pixel was split
+    avg = 1.0 / 9.6  # exposure 1
+    out = _compose_oracle(oracle, pre, thr, avg)[..., 0]
+    f = lambda y: lpm(oracle, (y,) * 3)[0]
+    # the cross average of an affine image is its centre value: interior
+    # pixels away from the checkerboard bounds are unchanged
+    for y in range(1, H - 1):
+        for x in range(2, W - 1):
+            assert math.isclose(out[y, x], f(v[y, x]), rel_tol=1e-12), (x, y)
+    # (0, 2) is checkerboard x 0: its left neighbour is dropped
+    e = 0.5 * v[2, 0] + 0.5 * 0.25 * (v[2, 1] + v[3, 0] + v[1, 0])
+    assert math.isclose(out[2, 0], f(e), rel_tol=1e-12)
+    # (1, 2) is checkerboard x = W/2: both horizontal neighbours are dropped
+    e = 0.5 * v[2, 1] + 0.5 * 0.25 * (v[3, 1] + v[1, 1])
+    assert math.isclose(out[2, 1], f(e), rel_tol=1e-12)
+    # top row: the neighbour outside the image reads as zero
+    e = 0.5 * v[0, 5] + 0.5 * 0.25 * (v[0, 4] + v[0, 6] + v[1, 5])
+    assert math.isclose(out[0, 5], f(e), rel_tol=1e-12)
+
+
+# ------------------------------------------------------------ GPU parity
+
+def _inputs(W, H, seed):
+    r = np.random.default_rng(seed)
+    pre = np.zeros((H, W, 4), np.float32)
+    pre[..., :3] = r.uniform(0, 1, (H, W, 3)) * 10.0 ** r.uniform(-4, 3, (H, W, 1))
+    grey = r.uniform(0, 1, (H, W)) < 0.1
+    pre[grey, 1] = pre[grey, 0]
+    pre[grey, 2] = pre[grey, 0]
+    pre[r.uniform(0, 1, (H, W)) < 0.02, :3] = 0.0
+    thr = np.zeros((H, W, 4), np.float32)
+    thr[..., :3] = 1.0
+    thr[..., 3] = (r.uniform(0, 1, (H, W)) < 0.3).astype(np.float32)
+    return pre, thr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("W,H,custom", [(64, 32, False), (640, 360, True), (1920, 1080, False)])
+def test_gpu_compose_final(gpu, oracle, W, H, custom):
+    print(f"compose {W}x{H} custom={custom}", flush=True)
+    pre, thr = _inputs(W, H, seed=W + H)
+    avg = np.array([0.37], np.float32)
+    g = FramebufferSet(W, H)
+    g.upload(PLANE["PRE_FINAL"], pre)
+    g.upload(PLANE["THROUGHPUT"], thr)
+    d_avg = DeviceArray.from_numpy(avg)
+    sat, ct = ZERO3, ONE3
+    tm = None
+    if custom:
+        tm = rtgl.RgDrawFrameTonemappingParams()
+        tm.sType = rtgl.RG_STRUCTURE_TYPE_TONEMAPPING
+        tm.saturation.data[:] = (-0.1, 0.2, 0.05)
+        tm.crosstalk.data[:] = (1.0, 0.5, 0.25)
+        sat, ct = (C.c_float * 3)(*tm.saturation.data), (C.c_float * 3)(*tm.crosstalk.data)
+        tm = C.byref(tm)
+    ffi.call("rgExtComposeFinal", g.h, tm, d_avg.ptr, None)
+    got = g.download(PLANE["FINAL"]).astype(np.float64)
+    ref = _compose_oracle(oracle, pre.astype(np.float64), thr.astype(np.float64), float(avg[0]), sat, ct)
+    assert np.isfinite(got).all()
+    err = np.abs(got - ref).max()
+    assert err < 1e-4, err
+    assert got[..., :3].min() >= 0.0 and got[..., :3].max() <= 1.0
+
+
+@pytest.mark.gpu
+def test_gpu_frame_final_plane(gpu, oracle):
+    """rgDrawFrame writes FINAL from its own PRE_FINAL, THROUGHPUT and
+    average luminance, as the oracle composes them."""
+    from raytracedgl1_amd import scenes
+    print("frame FINAL plane 160x90", flush=True)
+    W, H = 160, 90
+    s = scenes.terrain_boxes_scene()
+    r = rtgl.Renderer(W, H)
+    try:
+        for f in range(2):
+            st = r.frame(rtgl.SceneUploader(s), scenes.terrain_boxes_camera())
+        fbv = r.framebuffers()
+        pre = fbv.download(PLANE["PRE_FINAL"]).astype(np.float64)
+        thr = fbv.download(PLANE["THROUGHPUT"]).astype(np.float64)
+        got = fbv.download(PLANE["FINAL"]).astype(np.float64)
+    finally:
+        r.close()
+    assert st.avgLuminance > 0
+    ref = _compose_oracle(oracle, pre, thr, st.avgLuminance)
+    assert np.abs(got - ref).max() < 1e-4
+    assert 0.05 < got[..., :3].mean() < 0.95  # exposed, not black or blown out
