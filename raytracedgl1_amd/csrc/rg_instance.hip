@@ -271,6 +271,18 @@ struct Instance
         RenderSize( di, &w, &h );
         scene.FinishLights();
         FillParams( di, w, h );
+        {
+            // styles for the lights of the next frame's uploads
+            const auto* il = FindParams< RgDrawFrameIlluminationParams >( di, RG_STRUCTURE_TYPE_ILLUMINATION );
+            if( il && il->pLightstyleValues && il->lightstyleValuesCount > 0 )
+            {
+                scene.lightstyles.assign( il->pLightstyleValues, il->pLightstyleValues + il->lightstyleValuesCount );
+            }
+            else
+            {
+                scene.lightstyles.clear();
+            }
+        }
 
         if( !fb || fb->fb.width != w || fb->fb.height != h )
         {

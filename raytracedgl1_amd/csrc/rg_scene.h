@@ -118,6 +118,11 @@ struct SceneCollector
     std::unordered_map< uint64_t, uint32_t >    prevLightIndex; // uniqueID -> index last frame
     std::vector< uint32_t >                     prevToCur;      // last frame's index -> this frame's
     uint32_t                                    prevLightCount = 0;
+    // RgDrawFrameIlluminationParams::pLightstyleValues of the last rgDrawFrame.
+    // Lights are encoded at upload, before the frame's draw call sets new
+    // values, so a frame's lights use the previous draw call's styles
+    // (LightManager::SetLightstyles, Source/VulkanDevice.cpp:504).
+    std::vector< float >                        lightstyles;
 
     void BeginFrame();
     void AddPrimitive( const RgMeshInfo& mesh, const RgMeshPrimitiveInfo& prim );
@@ -125,6 +130,8 @@ struct SceneCollector
     void AddSphere( const RgSphericalLightUploadInfo& info );
     void AddSpot( const RgSpotLightUploadInfo& info );
     void AddPolygonal( const RgPolygonalLightUploadInfo& info );
+    // Intensity multiplier of a light (CalculateLightStyle, Source/LightManager.cpp:366-383)
+    float LightStyle( const RgLightExtraInfo& extra ) const;
     // Builds prevToCur; call once all lights of the frame are added.
     void FinishLights();
     // Last frame's geometry index -> this frame's, 0xFFFFFFFF when gone

@@ -169,6 +169,16 @@ void SceneCollector::AddPrimitive( const RgMeshInfo& mesh, const RgMeshPrimitive
 }
 
 // Light encoders: LightManager.cpp:99-234.
+float SceneCollector::LightStyle( const RgLightExtraInfo& extra ) const
+{
+    // an index outside the table counts as no style
+    if( extra.exists && extra.lightstyle >= 0 && size_t( extra.lightstyle ) < lightstyles.size() )
+    {
+        return lightstyles[ size_t( extra.lightstyle ) ];
+    }
+    return 1.0f;
+}
+
 void SceneCollector::AddDirectional( const RgDirectionalLightUploadInfo& info )
 {
     if( hasDirectional )
@@ -181,11 +191,12 @@ void SceneCollector::AddDirectional( const RgDirectionalLightUploadInfo& info )
     }
     float       d[ 3 ] = { info.direction.data[ 0 ], info.direction.data[ 1 ], info.direction.data[ 2 ] };
     const float len    = std::sqrt( d[ 0 ] * d[ 0 ] + d[ 1 ] * d[ 1 ] + d[ 2 ] * d[ 2 ] );
+    const float  mult = LightStyle( info.extra );
     LightEncoded lt{};
     lt.lightType = LIGHT_TYPE_DIRECTIONAL;
     for( int i = 0; i < 3; i++ )
     {
-        lt.color[ i ] = UnpackChannel( info.color, i ) * info.intensity;
+        lt.color[ i ] = UnpackChannel( info.color, i ) * info.intensity * mult;
         lt.data0[ i ] = len > 0.0f ? d[ i ] / len : 0.0f;
     }
     lt.data0[ 3 ]  = 0.5f * info.angularDiameterDegrees * float( M_PI ) / 180.0f;
@@ -202,11 +213,12 @@ void SceneCollector::AddSphere( const RgSphericalLightUploadInfo& info )
     }
     const float  radius = info.radius > 0.005f ? info.radius : 0.005f; // MIN_SPHERE_RADIUS
     const float  area   = float( M_PI ) * radius * radius;
+    const float  mult = LightStyle( info.extra );
     LightEncoded lt{};
     lt.lightType = LIGHT_TYPE_SPHERE;
     for( int i = 0; i < 3; i++ )
     {
-        lt.color[ i ] = UnpackChannel( info.color, i ) * info.intensity / area;
+        lt.color[ i ] = UnpackChannel( info.color, i ) * info.intensity / area * mult;
         lt.data0[ i ] = info.position.data[ i ];
     }
     lt.data0[ 3 ] = radius;
@@ -227,11 +239,12 @@ void SceneCollector::AddSpot( const RgSpotLightUploadInfo& info )
     const float deg    = float( M_PI ) / 180.0f;
     auto        clampc = [ & ]( float a ) { return a < 0.0f ? 0.0f : ( a > 89.0f * deg ? 89.0f * deg : a ); };
     const float inner  = std::min( info.angleInner, info.angleOuter - 1.0f * deg );
+    const float  mult = LightStyle( info.extra );
     LightEncoded lt{};
     lt.lightType = LIGHT_TYPE_SPOT;
     for( int i = 0; i < 3; i++ )
     {
-        lt.color[ i ] = UnpackChannel( info.color, i ) * info.intensity / area;
+        lt.color[ i ] = UnpackChannel( info.color, i ) * info.intensity / area * mult;
         lt.data0[ i ] = info.position.data[ i ];
         lt.data1[ i ] = len > 0.0f ? d[ i ] / len : 0.0f;
     }
@@ -260,11 +273,12 @@ void SceneCollector::AddPolygonal( const RgPolygonalLightUploadInfo& info )
         return;
     }
     const float  area = len * 0.5f;
+    const float  mult = LightStyle( info.extra );
     LightEncoded lt{};
     lt.lightType = LIGHT_TYPE_TRIANGLE;
     for( int i = 0; i < 3; i++ )
     {
-        lt.color[ i ] = UnpackChannel( info.color, i ) * info.intensity / area;
+        lt.color[ i ] = UnpackChannel( info.color, i ) * info.intensity / area * mult;
         lt.data0[ i ] = p0[ i ];
         lt.data1[ i ] = p1[ i ];
         lt.data2[ i ] = p2[ i ];

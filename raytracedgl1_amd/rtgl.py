@@ -85,6 +85,14 @@ class RgDrawFrameTonemappingParams(C.Structure):
                 ("luminanceWhitePoint", _F), ("saturation", RgFloat3D), ("crosstalk", RgFloat3D)]
 
 
+class RgDrawFrameIlluminationParams(C.Structure):
+    _fields_ = [("sType", C.c_int), ("pNext", _P), ("maxBounceShadows", _U32), ("enableSecondBounceForIndirect", _U32),
+                ("cellWorldSize", _F), ("directDiffuseSensitivityToChange", _F), ("indirectDiffuseSensitivityToChange", _F),
+                ("specularSensitivityToChange", _F), ("polygonalLightSpotlightFactor", _F),
+                ("lightUniqueIdIgnoreFirstPersonViewerShadows", _P), ("lightstyleValuesCount", _U32),
+                ("pLightstyleValues", _P)]
+
+
 class RgExtDrawFramePathParams(C.Structure):
     _fields_ = [("sType", C.c_int), ("pNext", _P), ("atrousIterations", _U32), ("restirInitialCandidates", _U32),
                 ("restirTemporalSamples", _U32), ("restirSpatialSamples", _U32),
@@ -110,9 +118,10 @@ class RgExtFrameStats(C.Structure):
 assert C.sizeof(RgInstanceCreateInfo) == 184 and C.sizeof(RgMeshInfo) == 88
 assert C.sizeof(RgMeshPrimitiveInfo) == 80 and C.sizeof(RgDrawFrameInfo) == 120
 assert C.sizeof(RgDirectionalLightUploadInfo) == 48 and C.sizeof(RgSphericalLightUploadInfo) == 48
-assert C.sizeof(RgDrawFrameTonemappingParams) == 56
+assert C.sizeof(RgDrawFrameTonemappingParams) == 56 and C.sizeof(RgDrawFrameIlluminationParams) == 72
 assert C.sizeof(RgExtDrawFramePathParams) == 48 and C.sizeof(RgExtFrameStats) == 96
 
+RG_STRUCTURE_TYPE_ILLUMINATION = 2
 RG_STRUCTURE_TYPE_TONEMAPPING = 4
 RG_STRUCTURE_TYPE_SKY = 7
 RG_EXT_STRUCTURE_TYPE_PATH_PARAMS = 0x52470001
@@ -163,6 +172,13 @@ def identity_transform() -> RgTransform:
     return t
 
 
+def _style(info, light: dict) -> None:
+    """A light dict's optional "lightstyle" index -> RgLightExtraInfo."""
+    if light.get("lightstyle") is not None:
+        info.extra.exists = 1
+        info.extra.lightstyle = int(light["lightstyle"])
+
+
 class SceneUploader:
     """Prebuilt per-object vertex arrays of a scenes.Scene, uploaded each frame
     with rgUploadMeshPrimitive (non-indexed triangle lists, identity transform)."""
@@ -202,6 +218,7 @@ class SceneUploader:
             d.intensity = sun["intensity"]
             d.direction.data[:] = sun["direction"]
             d.angularDiameterDegrees = sun["angularDiameterDegrees"]
+            _style(d, sun)
             ffi.check("rgUploadDirectionalLight", l.rgUploadDirectionalLight(inst, C.byref(d)))
         for i, sp in enumerate(lights.get("spheres", [])):
             s = RgSphericalLightUploadInfo()
@@ -210,6 +227,7 @@ class SceneUploader:
             s.intensity = sp["intensity"]
             s.position.data[:] = sp["position"]
             s.radius = sp["radius"]
+            _style(s, sp)
             ffi.check("rgUploadSphericalLight", l.rgUploadSphericalLight(inst, C.byref(s)))
 
 
@@ -236,7 +254,30 @@ class Renderer:
         self.sky.skyColorMultiplier = 1000.0
         self.sky.skyColorSaturation = 1.0
         self.sky.pNext = C.cast(C.pointer(self.path), _P)
+        self.illum = None
         self.time = 0.0
+
+    def set_lightstyles(self, values) -> None:
+        """Chains RgDrawFrameIlluminationParams (defaults of DrawFrameInfo.h)
+        with these light style values into every following rgDrawFrame."""
+        if values is None:
+            self.path.pNext = None
+            self.illum = None
+            return
+        self.styles = (C.c_float * max(1, len(values)))(*values)
+        il = RgDrawFrameIlluminationParams()
+        il.sType = RG_STRUCTURE_TYPE_ILLUMINATION
+        il.maxBounceShadows = 2
+        il.enableSecondBounceForIndirect = 1
+        il.cellWorldSize = 1.0
+        il.directDiffuseSensitivityToChange = 0.5
+        il.indirectDiffuseSensitivityToChange = 0.2
+        il.specularSensitivityToChange = 0.5
+        il.polygonalLightSpotlightFactor = 2.0
+        il.lightstyleValuesCount = len(values)
+        il.pLightstyleValues = C.cast(self.styles, _P)
+        self.illum = il
+        self.path.pNext = C.cast(C.pointer(il), _P)
 
     def set_static_geometry(self, enable: bool) -> None:
         ffi.check("rgExtSetStaticGeometry", api().rgExtSetStaticGeometry(self.h, 1 if enable else 0))

@@ -261,3 +261,42 @@ def test_gradients_stay_low_when_static_and_rise_when_light_goes(boxes):
     # stratum positions are 0..8 or 10 (rejected)
     pp = still["GRADIENT_PREV_PIX"][..., 0]
     assert np.isin(pp, list(range(9)) + [10]).all()
+
+
+def test_lightstyles_scale_lights_one_frame_late(boxes):
+    """RgLightExtraInfo::lightstyle picks a multiplier from the last
+    rgDrawFrame's RgDrawFrameIlluminationParams::pLightstyleValues
+    (CalculateLightStyle, Source/LightManager.cpp:366-383; the values are set
+    in DrawFrame after the frame's lights were encoded, VulkanDevice.cpp:504).
+    A styled sun at 0.5 renders exactly as an unstyled sun of half the
+    intensity, starting with the frame after the values were first given."""
+    print("api: light styles terrain-boxes 160x90", flush=True)
+    s, _ = boxes
+    sun = dict(s.lights["sun"])
+    cam = scenes.terrain_boxes_camera()
+    planes = ("UNFILTERED_DIRECT", "PRE_FINAL", "FINAL")
+
+    def run(suns, styles):
+        r = rtgl.Renderer(160, 90)
+        outs = []
+        try:
+            if styles is not None:
+                r.set_lightstyles(styles)
+            for sn in suns:
+                r.frame(rtgl.SceneUploader(scenes.Scene(s.tris, s.normals, s.colors, s.objects, {"sun": sn})), cam)
+                fb = r.framebuffers()
+                outs.append({k: fb.download(PLANE[k]).copy() for k in planes})
+        finally:
+            r.close()
+        return outs
+
+    styled = run([dict(sun, lightstyle=0)] * 3, [0.5, 3.0])
+    half = dict(sun, intensity=sun["intensity"] * 0.5)
+    plain = run([sun, half, half], None)
+    for f in range(3):
+        for k in planes:
+            assert np.array_equal(styled[f][k], plain[f][k]), (f, k)
+    # an index outside the table is no style
+    out_of_range = run([dict(sun, lightstyle=7)] * 2, [0.5])
+    full = run([sun] * 2, None)
+    assert np.array_equal(out_of_range[1]["UNFILTERED_DIRECT"], full[1]["UNFILTERED_DIRECT"])
