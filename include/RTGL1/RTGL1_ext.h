@@ -238,7 +238,19 @@ typedef enum RgExtPlane
 
 typedef struct RgExtFramebuffers_T* RgExtFramebuffers;
 
+/* Storage of the temporal and a-trous planes (ACCUM_HISTORY_LENGTH,
+ * DIFF_TEMPORARY .. INDIR_PONG, ATROUS_FILTERED_VARIANCE).  FP16 stores
+ * them as IEEE half (2 bytes per component, same component count), like the
+ * reference's RGBA16F / E5B9G9R9 images of these buffers
+ * (GenerateShaderCommon.py:830-849); values are rounded to nearest and
+ * clamped to +-65504, and all arithmetic stays FP32.  Every other plane is
+ * FP32 in both modes. */
+#define RG_EXT_STORAGE_FP32 0u
+#define RG_EXT_STORAGE_FP16 1u
+
 RGAPI RgResult RGCONV rgExtFramebuffersCreate( uint32_t width, uint32_t height, RgExtFramebuffers* pOut );
+/* As rgExtFramebuffersCreate, with a storage mode (RG_EXT_STORAGE_*). */
+RGAPI RgResult RGCONV rgExtFramebuffersCreateWithStorage( uint32_t width, uint32_t height, uint32_t storage, RgExtFramebuffers* pOut );
 RGAPI RgResult RGCONV rgExtFramebuffersDestroy( RgExtFramebuffers fb );
 /* Current (prev = 0) or previous-frame (prev = 1) copy of a plane.  Planes
  * without PREV return the same pointer for both. */
@@ -351,8 +363,14 @@ typedef struct RgExtDrawFramePathParams
     uint32_t        renderWidth;             /* 0 = RENDER_RESOLUTION params / 1280 */
     uint32_t        renderHeight;            /* 0 = RENDER_RESOLUTION params / 720  */
     uint32_t        collectTraversalStats;   /* count node visits / triangle tests  */
-    uint32_t        _pad0;
+    uint32_t        precisionFlags;          /* RG_EXT_PRECISION_* bits; 0 = FP32  */
 } RgExtDrawFramePathParams;
+
+/* FP16 precision form: FP16 storage of the denoiser's temporal and a-trous
+ * planes (RG_EXT_STORAGE_FP16) and FP16 BVH nodes (RG_EXT_BVH_NODE_FP16);
+ * camera matrices stay FP64 on the host and ray-triangle tests FP32. */
+#define RG_EXT_PRECISION_FP16_STORAGE 1u
+#define RG_EXT_PRECISION_FP16_NODES   2u
 
 /* Per-frame statistics of the last rgDrawFrame; GPU times from HIP events
  * recorded on the instance stream around each stage. */

@@ -315,6 +315,16 @@ RgResult rgExtFramebuffersCreate( uint32_t width, uint32_t height, RgExtFramebuf
     } );
 }
 
+RgResult rgExtFramebuffersCreateWithStorage( uint32_t width, uint32_t height, uint32_t storage, RgExtFramebuffers* pOut )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( pOut, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "pOut is null" );
+        auto f = std::make_unique< RgExtFramebuffers_T >();
+        f->fb.Create( width, height, storage );
+        *pOut = f.release();
+    } );
+}
+
 RgResult rgExtFramebuffersDestroy( RgExtFramebuffers fb )
 {
     return Guard( [ & ] { delete fb; } );

@@ -73,6 +73,106 @@ __device__ __forceinline__ void St1( void* base, int w, int x, int y, float v )
 {
     reinterpret_cast< float* >( base )[ size_t( y ) * w + x ] = v;
 }
+// Loads and stores of the planes that RG_EXT_STORAGE_FP16 keeps as half
+// (IsHalfPlane): F16 selects the 2-byte form.  Stores round to nearest and
+// clamp to the largest finite half, so an overflow cannot become inf.
+typedef _Float16 half4v __attribute__( ( ext_vector_type( 4 ) ) );
+typedef _Float16 half2v __attribute__( ( ext_vector_type( 2 ) ) );
+constexpr float HALF_MAX = 65504.0f;
+__device__ __forceinline__ _Float16 ToHalf( float v )
+{
+    return _Float16( fminf( fmaxf( v, -HALF_MAX ), HALF_MAX ) );
+}
+template< bool F16 >
+__device__ __forceinline__ float4 LdP4( const void* base, int w, int h, int x, int y )
+{
+    if constexpr( !F16 )
+    {
+        return Ld4( base, w, h, x, y );
+    }
+    else
+    {
+        if( x < 0 || y < 0 || x >= w || y >= h )
+        {
+            return make_float4( 0.f, 0.f, 0.f, 0.f );
+        }
+        const half4v v = reinterpret_cast< const half4v* >( base )[ size_t( y ) * w + x ];
+        return make_float4( float( v.x ), float( v.y ), float( v.z ), float( v.w ) );
+    }
+}
+template< bool F16 >
+__device__ __forceinline__ float2 LdP2( const void* base, int w, int h, int x, int y )
+{
+    if constexpr( !F16 )
+    {
+        return Ld2( base, w, h, x, y );
+    }
+    else
+    {
+        if( x < 0 || y < 0 || x >= w || y >= h )
+        {
+            return make_float2( 0.f, 0.f );
+        }
+        const half2v v = reinterpret_cast< const half2v* >( base )[ size_t( y ) * w + x ];
+        return make_float2( float( v.x ), float( v.y ) );
+    }
+}
+template< bool F16 >
+__device__ __forceinline__ float LdP1( const void* base, int w, int h, int x, int y )
+{
+    if constexpr( !F16 )
+    {
+        return Ld1( base, w, h, x, y );
+    }
+    else
+    {
+        if( x < 0 || y < 0 || x >= w || y >= h )
+        {
+            return 0.f;
+        }
+        return float( reinterpret_cast< const _Float16* >( base )[ size_t( y ) * w + x ] );
+    }
+}
+template< bool F16 >
+__device__ __forceinline__ void StP4( void* base, int w, int x, int y, float4 v )
+{
+    if constexpr( !F16 )
+    {
+        St4( base, w, x, y, v );
+    }
+    else
+    {
+        half4v h;
+        h.x = ToHalf( v.x ), h.y = ToHalf( v.y ), h.z = ToHalf( v.z ), h.w = ToHalf( v.w );
+        reinterpret_cast< half4v* >( base )[ size_t( y ) * w + x ] = h;
+    }
+}
+template< bool F16 >
+__device__ __forceinline__ void StP2( void* base, int w, int x, int y, float2 v )
+{
+    if constexpr( !F16 )
+    {
+        St2( base, w, x, y, v );
+    }
+    else
+    {
+        half2v h;
+        h.x = ToHalf( v.x ), h.y = ToHalf( v.y );
+        reinterpret_cast< half2v* >( base )[ size_t( y ) * w + x ] = h;
+    }
+}
+template< bool F16 >
+__device__ __forceinline__ void StP1( void* base, int w, int x, int y, float v )
+{
+    if constexpr( !F16 )
+    {
+        St1( base, w, x, y, v );
+    }
+    else
+    {
+        reinterpret_cast< _Float16* >( base )[ size_t( y ) * w + x ] = ToHalf( v );
+    }
+}
 __device__ __forceinline__ v3 xyz( float4 a )
 {
     return mk3( a.x, a.y, a.z );
@@ -193,6 +293,7 @@ __global__ __launch_bounds__( THREADS ) void k_grad_atrous( FbPtrs P, Dims d, in
 
 // ---------------------------------------------------------- temporal
 
+template< bool F16 >
 __global__ __launch_bounds__( THREADS ) void k_temporal( FbPtrs P, Dims d, RgExtFrameParams g )
 {
     const int x = blockIdx.x * BX + threadIdx.x % BX;
@@ -237,17 +338,17 @@ __global__ __launch_bounds__( THREADS ) void k_temporal( FbPtrs P, Dims d, RgExt
             if( inRenderArea( qx, qy, alo, ahi, H ) && TestDepth( depth, dPrev, motion.z ) && TestNormal( normal, nPrev ) )
             {
                 const float  w  = bw[ yy ][ xx ];
-                const float2 mp = Ld2( P.prev[ RG_EXT_PLANE_DIFF_ACCUM_MOMENTS ], W, H, qx, qy );
-                const float4 hp = Ld4( P.prev[ RG_EXT_PLANE_ACCUM_HISTORY_LENGTH ], W, H, qx, qy );
-                diffPrev        = diffPrev + xyz( Ld4( P.cur[ RG_EXT_PLANE_DIFF_COLOR_HISTORY ], W, H, qx, qy ) ) * w;
+                const float2 mp = LdP2< F16 >( P.prev[ RG_EXT_PLANE_DIFF_ACCUM_MOMENTS ], W, H, qx, qy );
+                const float4 hp = LdP4< F16 >( P.prev[ RG_EXT_PLANE_ACCUM_HISTORY_LENGTH ], W, H, qx, qy );
+                diffPrev        = diffPrev + xyz( LdP4< F16 >( P.cur[ RG_EXT_PLANE_DIFF_COLOR_HISTORY ], W, H, qx, qy ) ) * w;
                 momPrev.x += w * mp.x;
                 momPrev.y += w * mp.y;
                 hist.x += w * hp.x;
                 hist.y += w * hp.y;
                 hist.z += w * hp.z;
                 hist.w += w * hp.w;
-                specPrev  = specPrev + xyz( Ld4( P.prev[ RG_EXT_PLANE_SPEC_ACCUM_COLOR ], W, H, qx, qy ) ) * w;
-                indirPrev = indirPrev + xyz( Ld4( P.prev[ RG_EXT_PLANE_INDIR_ACCUM ], W, H, qx, qy ) ) * w;
+                specPrev  = specPrev + xyz( LdP4< F16 >( P.prev[ RG_EXT_PLANE_SPEC_ACCUM_COLOR ], W, H, qx, qy ) ) * w;
+                indirPrev = indirPrev + xyz( LdP4< F16 >( P.prev[ RG_EXT_PLANE_INDIR_ACCUM ], W, H, qx, qy ) ) * w;
                 weightSum += w;
             }
         }
@@ -368,15 +469,16 @@ __global__ __launch_bounds__( THREADS ) void k_temporal( FbPtrs P, Dims d, RgExt
         indirAccum = unfIndir;
         indirHist  = 1.0f;
     }
-    St4( P.cur[ RG_EXT_PLANE_DIFF_TEMPORARY ], W, x, y, f4( diffAccum, 0.0f ) );
-    St2( P.cur[ RG_EXT_PLANE_DIFF_ACCUM_MOMENTS ], W, x, y, diffMom );
-    St4( P.cur[ RG_EXT_PLANE_ACCUM_HISTORY_LENGTH ], W, x, y, make_float4( diffHist, indirHist, specHistAccum, specHitAccum ) );
-    St4( P.cur[ RG_EXT_PLANE_SPEC_ACCUM_COLOR ], W, x, y, f4( specAccum, 0.0f ) );
-    St4( P.cur[ RG_EXT_PLANE_INDIR_ACCUM ], W, x, y, f4( indirAccum, 0.0f ) );
+    StP4< F16 >( P.cur[ RG_EXT_PLANE_DIFF_TEMPORARY ], W, x, y, f4( diffAccum, 0.0f ) );
+    StP2< F16 >( P.cur[ RG_EXT_PLANE_DIFF_ACCUM_MOMENTS ], W, x, y, diffMom );
+    StP4< F16 >( P.cur[ RG_EXT_PLANE_ACCUM_HISTORY_LENGTH ], W, x, y, make_float4( diffHist, indirHist, specHistAccum, specHitAccum ) );
+    StP4< F16 >( P.cur[ RG_EXT_PLANE_SPEC_ACCUM_COLOR ], W, x, y, f4( specAccum, 0.0f ) );
+    StP4< F16 >( P.cur[ RG_EXT_PLANE_INDIR_ACCUM ], W, x, y, f4( indirAccum, 0.0f ) );
 }
 
 // --------------------------------------------------------- anti-firefly
 
+template< bool F16 >
 __global__ __launch_bounds__( THREADS ) void k_antifirefly( FbPtrs P, Dims d, uint32_t enabled )
 {
     constexpr int R = 2, TW = BX + 2 * R, TH = BY + 2 * R;
@@ -390,7 +492,7 @@ __global__ __launch_bounds__( THREADS ) void k_antifirefly( FbPtrs P, Dims d, ui
 #pragma unroll
         for( int s = 0; s < 3; s++ )
         {
-            s_c[ s ][ ty ][ tx ] = xyz( Ld4( src[ s ], W, H, bx0 + tx, by0 + ty ) );
+            s_c[ s ][ ty ][ tx ] = xyz( LdP4< F16 >( src[ s ], W, H, bx0 + tx, by0 + ty ) );
         }
     }
     __syncthreads();
@@ -423,12 +525,13 @@ __global__ __launch_bounds__( THREADS ) void k_antifirefly( FbPtrs P, Dims d, ui
             }
             c = mk3( clampf( c.x, mn.x, mx.x ), clampf( c.y, mn.y, mx.y ), clampf( c.z, mn.z, mx.z ) );
         }
-        St4( dst[ s ], W, x, y, f4( c, 0.0f ) );
+        StP4< F16 >( dst[ s ], W, x, y, f4( c, 0.0f ) );
     }
 }
 
 // ------------------------------------------------------------ variance
 
+template< bool F16 >
 __global__ __launch_bounds__( THREADS ) void k_variance( FbPtrs P, Dims d )
 {
     constexpr int R = 1, TW = BX + 2 * R, TH = BY + 2 * R;
@@ -440,7 +543,7 @@ __global__ __launch_bounds__( THREADS ) void k_variance( FbPtrs P, Dims d )
     for( int i = threadIdx.x; i < TW * TH; i += THREADS )
     {
         const int tx = i % TW, ty = i / TW;
-        s_col[ ty ][ tx ] = xyz( Ld4( P.cur[ RG_EXT_PLANE_DIFF_ACCUM_COLOR ], W, H, bx0 + tx, by0 + ty ) );
+        s_col[ ty ][ tx ] = xyz( LdP4< F16 >( P.cur[ RG_EXT_PLANE_DIFF_ACCUM_COLOR ], W, H, bx0 + tx, by0 + ty ) );
         s_nrm[ ty ][ tx ] = DecodeNormal( LdU( P.cur[ RG_EXT_PLANE_NORMAL ], W, H, bx0 + tx, by0 + ty ) );
         s_dep[ ty ][ tx ] = Ld1( P.cur[ RG_EXT_PLANE_DEPTH_WORLD ], W, H, bx0 + tx, by0 + ty );
     }
@@ -456,17 +559,17 @@ __global__ __launch_bounds__( THREADS ) void k_variance( FbPtrs P, Dims d )
     void*       out   = P.cur[ RG_EXT_PLANE_DIFF_PING ];
     if( depth < 0.0f || depth > MAX_RAY_LENGTH )
     {
-        St4( out, W, x, y, f4( color, 0.0f ) );
+        StP4< F16 >( out, W, x, y, f4( color, 0.0f ) );
         return;
     }
-    const float  histLen = Ld4( P.cur[ RG_EXT_PLANE_ACCUM_HISTORY_LENGTH ], W, H, x, y ).x;
-    const float2 mom     = Ld2( P.cur[ RG_EXT_PLANE_DIFF_ACCUM_MOMENTS ], W, H, x, y );
+    const float  histLen = LdP4< F16 >( P.cur[ RG_EXT_PLANE_ACCUM_HISTORY_LENGTH ], W, H, x, y ).x;
+    const float2 mom     = LdP2< F16 >( P.cur[ RG_EXT_PLANE_DIFF_ACCUM_MOMENTS ], W, H, x, y );
     constexpr float HT   = 4.0f;
     if( histLen > HT )
     {
         float var = fmaxf( 0.0f, mom.y - mom.x * mom.x );
         var /= fmaxf( 1.0f, histLen - HT );
-        St4( out, W, x, y, f4( color, var ) );
+        StP4< F16 >( out, W, x, y, f4( color, var ) );
         return;
     }
     const float l    = luminance( color );
@@ -503,7 +606,7 @@ __global__ __launch_bounds__( THREADS ) void k_variance( FbPtrs P, Dims d )
     s1 *= inv;
     color           = color * inv;
     const float var = fmaxf( 0.0f, s1 - s0 * s0 ) * ( 1.0f + 2.0f * ( 1.0f - histLen / HT ) );
-    St4( out, W, x, y, f4( color, var ) );
+    StP4< F16 >( out, W, x, y, f4( color, var ) );
 }
 
 // ------------------------------------------------------------- a-trous
@@ -588,7 +691,7 @@ __device__ void Compose( const FbPtrs& P, const Dims& d, int x, int y, v3 diff, 
 }
 
 // One edge-avoiding a-trous step; tap data comes from `Tap`.
-template< typename TapFn >
+template< bool F16, typename TapFn >
 __device__ void AtrousPixel( const FbPtrs& P, const Dims& d, int x, int y, int k, int step, const AtrousPlanes& ap,
                              v3 c_n, float c_depth, float c_grad, float c_rough, float4 c_diff, v3 c_spec, v3 c_indir,
                              float pvar, TapFn&& tap )
@@ -603,7 +706,7 @@ __device__ void AtrousPixel( const FbPtrs& P, const Dims& d, int x, int y, int k
     const float l      = luminance( outD );
     const float wLum   = 1.0f / ( 4.0f * pvar + 0.00001f );
     const float wRough = clampf( c_rough * 30.0f - float( k ), 0.0f, 1.0f );
-    const float hSpec  = Ld4( P.cur[ RG_EXT_PLANE_ACCUM_HISTORY_LENGTH ], W, H, x, y ).z;
+    const float hSpec  = LdP4< F16 >( P.cur[ RG_EXT_PLANE_ACCUM_HISTORY_LENGTH ], W, H, x, y ).z;
     float       nws    = clampf( RoughnessSquaredToSpecPower( c_rough * c_rough ), 8.0f, 1024.0f );
     nws *= clampf( hSpec / 8.0f, 0.0f, 1.0f );
 
@@ -653,9 +756,9 @@ __device__ void AtrousPixel( const FbPtrs& P, const Dims& d, int x, int y, int k
     outI = outI * ( 1.0f / wsi );
     if( ap.dstD >= 0 )
     {
-        St4( P.cur[ ap.dstD ], W, x, y, f4( outD, outV ) );
-        St4( P.cur[ ap.dstS ], W, x, y, f4( outS, 0.0f ) );
-        St4( P.cur[ ap.dstI ], W, x, y, f4( outI, 0.0f ) );
+        StP4< F16 >( P.cur[ ap.dstD ], W, x, y, f4( outD, outV ) );
+        StP4< F16 >( P.cur[ ap.dstS ], W, x, y, f4( outS, 0.0f ) );
+        StP4< F16 >( P.cur[ ap.dstI ], W, x, y, f4( outI, 0.0f ) );
     }
     else
     {
@@ -665,6 +768,7 @@ __device__ void AtrousPixel( const FbPtrs& P, const Dims& d, int x, int y, int k
 
 // Iteration 0: the 3x3 footprint (plus the 3x3 Gaussian of the variance) is
 // staged in LDS (CmSVGFAtrous_Iter0.comp:96-163).
+template< bool F16 >
 __global__ __launch_bounds__( THREADS ) void k_atrous0( FbPtrs P, Dims d, AtrousPlanes ap )
 {
     constexpr int R = 1, TW = BX + 2 * R, TH = BY + 2 * R;
@@ -679,9 +783,9 @@ __global__ __launch_bounds__( THREADS ) void k_atrous0( FbPtrs P, Dims d, Atrous
     for( int i = threadIdx.x; i < TW * TH; i += THREADS )
     {
         const int tx = i % TW, ty = i / TW, gx = bx0 + tx, gy = by0 + ty;
-        s_diff[ ty ][ tx ] = Ld4( P.cur[ ap.srcD ], W, H, gx, gy );
-        s_spec[ ty ][ tx ] = xyz( Ld4( P.cur[ ap.srcS ], W, H, gx, gy ) );
-        s_ind[ ty ][ tx ]  = xyz( Ld4( P.cur[ ap.srcI ], W, H, gx, gy ) );
+        s_diff[ ty ][ tx ] = LdP4< F16 >( P.cur[ ap.srcD ], W, H, gx, gy );
+        s_spec[ ty ][ tx ] = xyz( LdP4< F16 >( P.cur[ ap.srcS ], W, H, gx, gy ) );
+        s_ind[ ty ][ tx ]  = xyz( LdP4< F16 >( P.cur[ ap.srcI ], W, H, gx, gy ) );
         s_nrm[ ty ][ tx ]  = DecodeNormal( LdU( P.cur[ RG_EXT_PLANE_NORMAL ], W, H, gx, gy ) );
         s_dep[ ty ][ tx ]  = Ld1( P.cur[ RG_EXT_PLANE_DEPTH_WORLD ], W, H, gx, gy );
         s_rgh[ ty ][ tx ]  = Ld2( P.cur[ RG_EXT_PLANE_METALLIC_ROUGHNESS ], W, H, gx, gy ).y;
@@ -699,9 +803,9 @@ __global__ __launch_bounds__( THREADS ) void k_atrous0( FbPtrs P, Dims d, Atrous
         const float4 z = make_float4( 0.f, 0.f, 0.f, 0.f );
         if( ap.dstD >= 0 )
         {
-            St4( P.cur[ ap.dstD ], W, x, y, z );
-            St4( P.cur[ ap.dstS ], W, x, y, z );
-            St4( P.cur[ ap.dstI ], W, x, y, z );
+            StP4< F16 >( P.cur[ ap.dstD ], W, x, y, z );
+            StP4< F16 >( P.cur[ ap.dstS ], W, x, y, z );
+            StP4< F16 >( P.cur[ ap.dstI ], W, x, y, z );
         }
         else
         {
@@ -720,8 +824,8 @@ __global__ __launch_bounds__( THREADS ) void k_atrous0( FbPtrs P, Dims d, Atrous
         }
     }
     const float pvar = sqrtf( fmaxf( r, 0.0f ) );
-    St1( P.cur[ RG_EXT_PLANE_ATROUS_FILTERED_VARIANCE ], W, x, y, pvar );
-    AtrousPixel( P, d, x, y, 0, 1, ap, s_nrm[ ly ][ lx ], depth, Ld1( P.cur[ RG_EXT_PLANE_DEPTH_GRAD ], W, H, x, y ),
+    StP1< F16 >( P.cur[ RG_EXT_PLANE_ATROUS_FILTERED_VARIANCE ], W, x, y, pvar );
+    AtrousPixel< F16 >( P, d, x, y, 0, 1, ap, s_nrm[ ly ][ lx ], depth, Ld1( P.cur[ RG_EXT_PLANE_DEPTH_GRAD ], W, H, x, y ),
                  s_rgh[ ly ][ lx ], s_diff[ ly ][ lx ], s_spec[ ly ][ lx ], s_ind[ ly ][ lx ], pvar,
                  [ & ]( int xx, int yy, int, int, v3& nq, float& dq, float& rq, float4& cq, v3& sq, v3& iq ) {
                      nq = s_nrm[ ly + yy ][ lx + xx ];
@@ -734,6 +838,7 @@ __global__ __launch_bounds__( THREADS ) void k_atrous0( FbPtrs P, Dims d, Atrous
 }
 
 // Iterations >= 1 (steps 2, 4, 8, 16): taps gathered from global memory.
+template< bool F16 >
 __global__ __launch_bounds__( THREADS ) void k_atrousN( FbPtrs P, Dims d, AtrousPlanes ap, int k )
 {
     const int x = blockIdx.x * BX + threadIdx.x % BX;
@@ -743,17 +848,17 @@ __global__ __launch_bounds__( THREADS ) void k_atrousN( FbPtrs P, Dims d, Atrous
     {
         return;
     }
-    const float4 cd    = Ld4( P.cur[ ap.srcD ], W, H, x, y );
-    const v3     cs    = xyz( Ld4( P.cur[ ap.srcS ], W, H, x, y ) );
-    const v3     ci    = xyz( Ld4( P.cur[ ap.srcI ], W, H, x, y ) );
+    const float4 cd    = LdP4< F16 >( P.cur[ ap.srcD ], W, H, x, y );
+    const v3     cs    = xyz( LdP4< F16 >( P.cur[ ap.srcS ], W, H, x, y ) );
+    const v3     ci    = xyz( LdP4< F16 >( P.cur[ ap.srcI ], W, H, x, y ) );
     const float  depth = Ld1( P.cur[ RG_EXT_PLANE_DEPTH_WORLD ], W, H, x, y );
     if( depth < 0.0f || depth > MAX_RAY_LENGTH )
     {
         if( ap.dstD >= 0 )
         {
-            St4( P.cur[ ap.dstD ], W, x, y, cd );
-            St4( P.cur[ ap.dstS ], W, x, y, f4( cs, 0.0f ) );
-            St4( P.cur[ ap.dstI ], W, x, y, f4( ci, 0.0f ) );
+            StP4< F16 >( P.cur[ ap.dstD ], W, x, y, cd );
+            StP4< F16 >( P.cur[ ap.dstS ], W, x, y, f4( cs, 0.0f ) );
+            StP4< F16 >( P.cur[ ap.dstI ], W, x, y, f4( ci, 0.0f ) );
         }
         else
         {
@@ -763,15 +868,15 @@ __global__ __launch_bounds__( THREADS ) void k_atrousN( FbPtrs P, Dims d, Atrous
     }
     const v3    n     = DecodeNormal( LdU( P.cur[ RG_EXT_PLANE_NORMAL ], W, H, x, y ) );
     const float rough = Ld2( P.cur[ RG_EXT_PLANE_METALLIC_ROUGHNESS ], W, H, x, y ).y;
-    const float pvar  = Ld1( P.cur[ RG_EXT_PLANE_ATROUS_FILTERED_VARIANCE ], W, H, x, y );
-    AtrousPixel( P, d, x, y, k, 1 << k, ap, n, depth, Ld1( P.cur[ RG_EXT_PLANE_DEPTH_GRAD ], W, H, x, y ), rough, cd, cs, ci, pvar,
+    const float pvar  = LdP1< F16 >( P.cur[ RG_EXT_PLANE_ATROUS_FILTERED_VARIANCE ], W, H, x, y );
+    AtrousPixel< F16 >( P, d, x, y, k, 1 << k, ap, n, depth, Ld1( P.cur[ RG_EXT_PLANE_DEPTH_GRAD ], W, H, x, y ), rough, cd, cs, ci, pvar,
                  [ & ]( int, int, int qx, int qy, v3& nq, float& dq, float& rq, float4& cq, v3& sq, v3& iq ) {
                      nq = DecodeNormal( LdU( P.cur[ RG_EXT_PLANE_NORMAL ], W, H, qx, qy ) );
                      dq = Ld1( P.cur[ RG_EXT_PLANE_DEPTH_WORLD ], W, H, qx, qy );
                      rq = Ld2( P.cur[ RG_EXT_PLANE_METALLIC_ROUGHNESS ], W, H, qx, qy ).y;
-                     cq = Ld4( P.cur[ ap.srcD ], W, H, qx, qy );
-                     sq = xyz( Ld4( P.cur[ ap.srcS ], W, H, qx, qy ) );
-                     iq = xyz( Ld4( P.cur[ ap.srcI ], W, H, qx, qy ) );
+                     cq = LdP4< F16 >( P.cur[ ap.srcD ], W, H, qx, qy );
+                     sq = xyz( LdP4< F16 >( P.cur[ ap.srcS ], W, H, qx, qy ) );
+                     iq = xyz( LdP4< F16 >( P.cur[ ap.srcI ], W, H, qx, qy ) );
                  } );
 }
 
@@ -802,7 +907,15 @@ void DenoiseTemporal( FramebufferSet& fb, const RgExtFrameParams& p, hipStream_t
 {
     CheckParams( fb, p );
     const Dims d = GetDims( fb );
-    k_temporal<<< dim3( DivUp( d.W, BX ), DivUp( d.H, BY ) ), THREADS, 0, s >>>( fb.Ptrs(), d, p );
+    const dim3 grid( DivUp( d.W, BX ), DivUp( d.H, BY ) );
+    if( fb.Half() )
+    {
+        k_temporal< true ><<< grid, THREADS, 0, s >>>( fb.Ptrs(), d, p );
+    }
+    else
+    {
+        k_temporal< false ><<< grid, THREADS, 0, s >>>( fb.Ptrs(), d, p );
+    }
     CheckLaunch( "k_temporal" );
 }
 
@@ -810,7 +923,15 @@ void DenoiseAntiFirefly( FramebufferSet& fb, const RgExtFrameParams& p, hipStrea
 {
     CheckParams( fb, p );
     const Dims d = GetDims( fb );
-    k_antifirefly<<< dim3( DivUp( d.W, BX ), DivUp( d.H, BY ) ), THREADS, 0, s >>>( fb.Ptrs(), d, p.antiFireflyEnabled );
+    const dim3 grid( DivUp( d.W, BX ), DivUp( d.H, BY ) );
+    if( fb.Half() )
+    {
+        k_antifirefly< true ><<< grid, THREADS, 0, s >>>( fb.Ptrs(), d, p.antiFireflyEnabled );
+    }
+    else
+    {
+        k_antifirefly< false ><<< grid, THREADS, 0, s >>>( fb.Ptrs(), d, p.antiFireflyEnabled );
+    }
     CheckLaunch( "k_antifirefly" );
 }
 
@@ -818,7 +939,15 @@ void DenoiseVariance( FramebufferSet& fb, const RgExtFrameParams& p, hipStream_t
 {
     CheckParams( fb, p );
     const Dims d = GetDims( fb );
-    k_variance<<< dim3( DivUp( d.W, BX ), DivUp( d.H, BY ) ), THREADS, 0, s >>>( fb.Ptrs(), d );
+    const dim3 grid( DivUp( d.W, BX ), DivUp( d.H, BY ) );
+    if( fb.Half() )
+    {
+        k_variance< true ><<< grid, THREADS, 0, s >>>( fb.Ptrs(), d );
+    }
+    else
+    {
+        k_variance< false ><<< grid, THREADS, 0, s >>>( fb.Ptrs(), d );
+    }
     CheckLaunch( "k_variance" );
 }
 
@@ -831,12 +960,26 @@ void DenoiseAtrous( FramebufferSet& fb, const RgExtFrameParams& p, uint32_t k, h
     const dim3         grid( DivUp( d.W, BX ), DivUp( d.H, BY ) );
     if( k == 0 )
     {
-        k_atrous0<<< grid, THREADS, 0, s >>>( fb.Ptrs(), d, ap );
+        if( fb.Half() )
+        {
+            k_atrous0< true ><<< grid, THREADS, 0, s >>>( fb.Ptrs(), d, ap );
+        }
+        else
+        {
+            k_atrous0< false ><<< grid, THREADS, 0, s >>>( fb.Ptrs(), d, ap );
+        }
         CheckLaunch( "k_atrous0" );
     }
     else
     {
-        k_atrousN<<< grid, THREADS, 0, s >>>( fb.Ptrs(), d, ap, int( k ) );
+        if( fb.Half() )
+        {
+            k_atrousN< true ><<< grid, THREADS, 0, s >>>( fb.Ptrs(), d, ap, int( k ) );
+        }
+        else
+        {
+            k_atrousN< false ><<< grid, THREADS, 0, s >>>( fb.Ptrs(), d, ap, int( k ) );
+        }
         CheckLaunch( "k_atrousN" );
     }
 }

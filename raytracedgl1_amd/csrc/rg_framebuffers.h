@@ -27,20 +27,32 @@ struct PlaneDesc
 
 const PlaneDesc& GetPlaneDesc( int plane );
 
+// Planes stored as IEEE half in RG_EXT_STORAGE_FP16 mode: the temporal and
+// a-trous buffers, ACCUM_HISTORY_LENGTH .. INDIR_PONG plus
+// ATROUS_FILTERED_VARIANCE.
+__host__ __device__ constexpr bool IsHalfPlane( int plane )
+{
+    return ( plane >= RG_EXT_PLANE_ACCUM_HISTORY_LENGTH && plane <= RG_EXT_PLANE_INDIR_PONG ) ||
+           plane == RG_EXT_PLANE_ATROUS_FILTERED_VARIANCE;
+}
+
 // Device pointers of every plane, passed to kernels by value.
 struct FbPtrs
 {
-    void* cur[ RG_EXT_PLANE_COUNT ];
-    void* prev[ RG_EXT_PLANE_COUNT ];
+    void*    cur[ RG_EXT_PLANE_COUNT ];
+    void*    prev[ RG_EXT_PLANE_COUNT ];
+    uint32_t half; // RG_EXT_STORAGE_FP16: IsHalfPlane planes hold IEEE half
 };
 
 struct FramebufferSet
 {
     uint32_t     width = 0, height = 0, gw = 0, gh = 0;
+    uint32_t     storage = RG_EXT_STORAGE_FP32;
     DeviceBuffer planes[ RG_EXT_PLANE_COUNT ][ 2 ];
     int          parity = 0;
 
-    void   Create( uint32_t w, uint32_t h );
+    void   Create( uint32_t w, uint32_t h, uint32_t storageMode = RG_EXT_STORAGE_FP32 );
+    bool   Half() const { return storage == RG_EXT_STORAGE_FP16; }
     size_t PlaneBytes( int plane ) const;
     void*  Cur( int plane );
     void*  Prev( int plane );

@@ -61,11 +61,14 @@ size_t FramebufferSet::PlaneBytes( int plane ) const
 {
     const PlaneDesc& d = g_planes[ plane ];
     const size_t     px = d.layout == LAYOUT_STRATA ? size_t( gw ) * gh : size_t( width ) * height;
-    return px * d.comps * 4;
+    return px * d.comps * ( Half() && IsHalfPlane( plane ) ? 2 : 4 );
 }
 
-void FramebufferSet::Create( uint32_t w, uint32_t h )
+void FramebufferSet::Create( uint32_t w, uint32_t h, uint32_t storageMode )
 {
+    RG_REQUIRE( storageMode == RG_EXT_STORAGE_FP32 || storageMode == RG_EXT_STORAGE_FP16, RG_RESULT_WRONG_FUNCTION_ARGUMENT,
+                "unknown storage mode" );
+    storage = storageMode;
     RG_REQUIRE( w >= 2 && h >= 1 && ( w % 2 ) == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "render width must be even and >= 2" );
     RG_REQUIRE( uint64_t( w ) * h <= ( 1ull << 26 ), RG_RESULT_WRONG_FUNCTION_ARGUMENT, "render size too large" );
     width  = w;
@@ -103,6 +106,7 @@ FbPtrs FramebufferSet::Ptrs()
         p.cur[ i ]  = Cur( i );
         p.prev[ i ] = Prev( i );
     }
+    p.half = Half() ? 1u : 0u;
     return p;
 }
 

@@ -284,10 +284,16 @@ struct Instance
             }
         }
 
-        if( !fb || fb->fb.width != w || fb->fb.height != h )
+        const auto*    pathExt  = FindParams< RgExtDrawFramePathParams >( di, RG_EXT_STRUCTURE_TYPE_PATH_PARAMS );
+        const uint32_t prec     = pathExt ? pathExt->precisionFlags : 0u;
+        RG_REQUIRE( ( prec & ~( RG_EXT_PRECISION_FP16_STORAGE | RG_EXT_PRECISION_FP16_NODES ) ) == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT,
+                    "unknown precisionFlags bits" );
+        const uint32_t storage  = ( prec & RG_EXT_PRECISION_FP16_STORAGE ) ? RG_EXT_STORAGE_FP16 : RG_EXT_STORAGE_FP32;
+        const uint32_t nodeFmt  = ( prec & RG_EXT_PRECISION_FP16_NODES ) ? RG_EXT_BVH_NODE_FP16 : RG_EXT_BVH_NODE_FP32;
+        if( !fb || fb->fb.width != w || fb->fb.height != h || fb->fb.storage != storage )
         {
             fb = std::make_unique< RgExtFramebuffers_T >();
-            fb->fb.Create( w, h );
+            fb->fb.Create( w, h, storage );
             firstFrame = true;
         }
         else
@@ -319,11 +325,11 @@ struct Instance
         const bool changed = keep ? false : AssembleScene( dscene, s );
         RG_HIP_CHECK( hipEventRecord( ev[ ST_ASSEMBLED ], s ) );
         stats.bvhRebuilt = 0;
-        if( dscene.triCount > 0 && ( changed || !dscene.bvhValid ) )
+        if( dscene.triCount > 0 && ( changed || !dscene.bvhValid || bvh.nodeFormat != nodeFmt ) )
         {
             // one triangle per leaf: measured fastest at 1M triangles (more
             // nodes, fewer triangle tests per leaf visit)
-            RgExtBvhBuildParams bp = { 1, RG_EXT_BVH_NODE_FP32 };
+            RgExtBvhBuildParams bp = { 1, nodeFmt };
             BuildBvh( bvh, dscene.tris.as< float >(), dscene.triCount, bp, s );
             dscene.bvhValid      = true;
             dscene.builtTriCount = dscene.triCount;

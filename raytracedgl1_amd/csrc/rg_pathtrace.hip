@@ -1435,7 +1435,17 @@ __global__ __launch_bounds__( THREADS ) void k_grad_b( RgExtFrameParams p, FbPtr
     {
         // indirect: only a sudden darkening of the direct diffuse, unless bright
         float       gs         = fmaxf( 0.0f, l.z - fwdD );
-        const float4 ia        = Plane< float4 >( P, RG_EXT_PLANE_INDIR_ACCUM, true )[ size_t( px.y ) * W + px.x ];
+        const size_t q         = size_t( px.y ) * W + px.x;
+        float4       ia;
+        if( P.half )
+        {
+            const _Float16* h = Plane< _Float16 >( P, RG_EXT_PLANE_INDIR_ACCUM, true ) + q * 4;
+            ia                = make_float4( float( h[ 0 ] ), float( h[ 1 ] ), float( h[ 2 ] ), 0.0f );
+        }
+        else
+        {
+            ia = Plane< float4 >( P, RG_EXT_PLANE_INDIR_ACCUM, true )[ q ];
+        }
         const float brightness = luminance( mk3( ia.x, ia.y, ia.z ) );
         const float t          = clampf( ( brightness - 1.0f ) / ( 10.0f - 1.0f ), 0.0f, 1.0f );
         gs *= 1.0f - t * t * ( 3.0f - 2.0f * t );

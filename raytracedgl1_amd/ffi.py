@@ -126,6 +126,7 @@ _SIGS = {
     "rgExtLuminanceAverage": (_RES, [_P, _U32, _F, _F, _F, _U32, _P, _P]),
     "rgExtComposeFinal": (_RES, [_P, _P, _P, _P]),
     "rgExtFramebuffersCreate": (_RES, [_U32, _U32, C.POINTER(_P)]),
+    "rgExtFramebuffersCreateWithStorage": (_RES, [_U32, _U32, _U32, C.POINTER(_P)]),
     "rgExtFramebuffersDestroy": (_RES, [_P]),
     "rgExtFramebuffersGetPlane": (_RES, [_P, C.c_int, _U32, C.POINTER(_P), C.POINTER(C.c_size_t)]),
     "rgExtFramebuffersSwap": (_RES, [_P]),
@@ -361,6 +362,15 @@ PLANES = [
 PLANE = {name: i for i, (name, *_rest) in enumerate(PLANES)}
 
 
+RG_EXT_STORAGE_FP32, RG_EXT_STORAGE_FP16 = 0, 1
+RG_EXT_PRECISION_FP16_STORAGE, RG_EXT_PRECISION_FP16_NODES = 1, 2
+
+
+def is_half_plane(plane: int) -> bool:
+    """Planes stored as IEEE half by RG_EXT_STORAGE_FP16."""
+    return PLANE["ACCUM_HISTORY_LENGTH"] <= plane <= PLANE["INDIR_PONG"] or plane == PLANE["ATROUS_FILTERED_VARIANCE"]
+
+
 def plane_shape(plane: int, width: int, height: int) -> tuple:
     _, layout, comps, _, _ = PLANES[plane]
     if layout == 2:
@@ -379,11 +389,19 @@ def frame_params(width: int, height: int) -> RgExtFrameParams:
 class FramebufferSet:
     """rgExtFramebuffers*: the device planes of one render size."""
 
-    def __init__(self, width: int, height: int):
+    def __init__(self, width: int, height: int, storage: int = 0):
         p = _P()
-        call("rgExtFramebuffersCreate", width, height, C.byref(p))
+        call("rgExtFramebuffersCreateWithStorage", width, height, storage, C.byref(p))
         self.h = p.value
         self.width, self.height = width, height
+
+    def dtype(self, plane: int, nbytes: int):
+        """Element type of a plane: float16 for the planes the FP16 storage
+        mode halves (their byte size tells), else the FP32-mode type."""
+        full = int(np.prod(plane_shape(plane, self.width, self.height))) * 4
+        if is_half_plane(plane) and nbytes * 2 == full:
+            return np.float16
+        return PLANES[plane][3]
 
     def plane_ptr(self, plane: int, prev: bool = False) -> tuple:
         d = _P()
@@ -393,14 +411,14 @@ class FramebufferSet:
 
     def upload(self, plane: int, arr: np.ndarray, prev: bool = False) -> None:
         ptr, nbytes = self.plane_ptr(plane, prev)
-        a = np.ascontiguousarray(arr, dtype=PLANES[plane][3])
+        a = np.ascontiguousarray(arr, dtype=self.dtype(plane, nbytes))
         assert a.nbytes == nbytes, (PLANES[plane][0], a.nbytes, nbytes)
         call("rgExtCopyHostToDevice", ptr, a.ctypes.data, nbytes)
 
     def download(self, plane: int, prev: bool = False) -> np.ndarray:
         ptr, nbytes = self.plane_ptr(plane, prev)
         shape = plane_shape(plane, self.width, self.height)
-        return download_raw(ptr, nbytes, PLANES[plane][3], shape)
+        return download_raw(ptr, nbytes, self.dtype(plane, nbytes), shape)
 
     def swap(self) -> None:
         call("rgExtFramebuffersSwap", self.h)

@@ -96,7 +96,7 @@ class RgDrawFrameIlluminationParams(C.Structure):
 class RgExtDrawFramePathParams(C.Structure):
     _fields_ = [("sType", C.c_int), ("pNext", _P), ("atrousIterations", _U32), ("restirInitialCandidates", _U32),
                 ("restirTemporalSamples", _U32), ("restirSpatialSamples", _U32),
-                ("renderWidth", _U32), ("renderHeight", _U32), ("collectTraversalStats", _U32), ("_pad0", _U32)]
+                ("renderWidth", _U32), ("renderHeight", _U32), ("collectTraversalStats", _U32), ("precisionFlags", _U32)]
 
 
 class RgDrawFrameInfo(C.Structure):

@@ -113,13 +113,21 @@ def make_gbuffer(width: int, height: int, seed: int, sky_fraction: float = 0.05,
 
 def fill_planes(cur, prev, gpu_fb=None, oracle_fb=None):
     """Uploads the same values to a GPU FramebufferSet and an oracle set.
-    Planes not given are zero on both sides."""
+    Planes not given are zero on both sides.  A plane the GPU set stores as
+    half is rounded to half for the oracle too, so both start equal."""
+    def stored(p, a):
+        if gpu_fb is None:
+            return a
+        return np.asarray(a).astype(gpu_fb.dtype(p, gpu_fb.plane_ptr(p)[1]))
+
     for p, a in cur.items():
+        a = stored(p, a)
         if gpu_fb is not None:
             gpu_fb.upload(p, a)
         if oracle_fb is not None:
             oracle_fb.set(p, a)
     for p, a in prev.items():
+        a = stored(p, a)
         if gpu_fb is not None:
             gpu_fb.upload(p, a, prev=True)
         if oracle_fb is not None:

@@ -200,8 +200,10 @@ def test_gpu_frame_final_plane(gpu, oracle):
     s = scenes.terrain_boxes_scene()
     r = rtgl.Renderer(W, H)
     try:
-        for f in range(2):
-            st = r.frame(rtgl.SceneUploader(s), scenes.terrain_boxes_camera())
+        # eye adaptation starts from zero and moves by 1 - exp(-1.1 dt) per
+        # frame (CmLuminanceAvg.comp): long time steps let it settle
+        for f in range(3):
+            st = r.frame(rtgl.SceneUploader(s), scenes.terrain_boxes_camera(), dt=10.0)
         fbv = r.framebuffers()
         pre = fbv.download(PLANE["PRE_FINAL"]).astype(np.float64)
         thr = fbv.download(PLANE["THROUGHPUT"]).astype(np.float64)
