@@ -415,6 +415,107 @@ RGAPI RgResult RGCONV rgExtGetStream( RgInstance instance, void** pOut );
  * replaces it. */
 RGAPI RgResult RGCONV rgExtSetStaticGeometry( RgInstance instance, RgBool32 enable );
 
+/* ----------------------------------------------------------------- serving */
+/* A render server: many camera sessions over shared static scenes, their
+ * frames in flight together on a few HIP streams.  The reference renders one
+ * camera per instance (VulkanDevice); a session is that instance's per-camera
+ * state (A-SVGF history and moments, ReSTIR DI/GI reservoirs, light grid,
+ * eye adaptation) kept resident in HBM, and a frame is rgDrawFrame's work
+ * from the traced passes to the FINAL plane.
+ *
+ * Memory: a session owns only the planes a frame reads back from the frame
+ * before (RG_EXT_PLANE_* with PREV, DIFF_COLOR_HISTORY, DIS_GRADIENT_HISTORY,
+ * GRADIENT_PREV_PIX) and FINAL; the within-frame planes and wavefront
+ * buffers belong to a stream and serve every session that stream runs.
+ * Sessions are admitted while their bytes fit memoryBudgetBytes.
+ *
+ * Scheduling: each session runs on one stream (round robin at creation), so
+ * its frames stay in order.  A dispatcher thread launches the next waiting
+ * frame of any session whose stream has fewer than framesInFlightPerStream
+ * frames queued, and retires frames as their completion events fire.  A
+ * request that is still waiting when its session submits again is replaced
+ * by the newer one and reported as dropped (the newest camera wins).
+ *
+ * A session renders the same pixels alone or among others: its frames
+ * depend only on its own requests and the scene. */
+typedef struct RgExtServer_T*  RgExtServer;
+typedef struct RgExtScene_T*   RgExtScene;
+typedef struct RgExtSession_T* RgExtSession;
+
+typedef struct RgExtServerCreateInfo
+{
+    uint32_t streamCount;             /* 1..16; 0 = 4                                  */
+    uint32_t framesInFlightPerStream; /* 1..8; 0 = 2                                   */
+    uint32_t maxRenderWidth;          /* capacity of the per-stream within-frame planes */
+    uint32_t maxRenderHeight;
+    uint32_t precisionFlags;          /* RG_EXT_PRECISION_* for every scene and session */
+    uint32_t _reserved;
+    uint64_t memoryBudgetBytes;       /* session bytes admitted; 0 = no limit           */
+} RgExtServerCreateInfo;
+
+/* rgExtSessionCreate: the session's bytes do not fit the budget. */
+#define RG_EXT_RESULT_OVER_BUDGET ( ( RgResult )0x52470101 )
+
+#define RG_EXT_FRAME_DONE    0u
+#define RG_EXT_FRAME_DROPPED 1u /* replaced by a newer request before it started */
+#define RG_EXT_FRAME_FAILED  2u
+
+typedef struct RgExtFrameResult
+{
+    RgExtSession session;
+    uint64_t     tag;        /* the caller's tag from rgExtSessionSubmit            */
+    uint32_t     status;     /* RG_EXT_FRAME_*                                      */
+    uint32_t     frameId;    /* the session's frame index (DONE)                    */
+    double       submitTime; /* seconds on the server's steady clock                */
+    double       startTime;  /* the frame's work was enqueued (DONE)                */
+    double       endTime;    /* completion seen (DONE) or drop time                  */
+} RgExtFrameResult;
+
+typedef struct RgExtServerStats
+{
+    uint32_t sessions;
+    uint32_t streams;
+    uint64_t sessionBytes;   /* admitted against the budget                        */
+    uint64_t sharedBytes;    /* per-stream within-frame planes, wavefronts, traversal stacks */
+    uint64_t sceneBytes;
+    uint64_t framesDone;
+    uint64_t framesDropped;
+    uint64_t framesFailed;
+    double   now;            /* the server's steady clock, seconds                 */
+} RgExtServerStats;
+
+RGAPI RgResult RGCONV rgExtServerCreate( const RgExtServerCreateInfo* pInfo, RgExtServer* pOut );
+/* Waits for every frame, then frees sessions, scenes and the server. */
+RGAPI RgResult RGCONV rgExtServerDestroy( RgExtServer server );
+/* A static scene: primitives and lights are collected like rgUploadMeshPrimitive
+ * and rgUpload*Light, then rgExtSceneBuild uploads them and builds the BVH
+ * once (synchronously). */
+RGAPI RgResult RGCONV rgExtSceneCreate( RgExtServer server, RgExtScene* pOut );
+RGAPI RgResult RGCONV rgExtSceneUploadMeshPrimitive( RgExtScene scene, const RgMeshInfo* pMesh, const RgMeshPrimitiveInfo* pPrimitive );
+RGAPI RgResult RGCONV rgExtSceneUploadDirectionalLight( RgExtScene scene, const RgDirectionalLightUploadInfo* pInfo );
+RGAPI RgResult RGCONV rgExtSceneUploadSphericalLight( RgExtScene scene, const RgSphericalLightUploadInfo* pInfo );
+RGAPI RgResult RGCONV rgExtSceneBuild( RgExtScene scene );
+/* Fails with RG_RESULT_WRONG_FUNCTION_CALL while sessions use the scene. */
+RGAPI RgResult RGCONV rgExtSceneDestroy( RgExtScene scene );
+/* A camera session of a fixed render size over a built scene; returns
+ * RG_EXT_RESULT_OVER_BUDGET when its bytes do not fit. */
+RGAPI RgResult RGCONV rgExtSessionCreate( RgExtScene scene, uint32_t width, uint32_t height, RgExtSession* pOut );
+/* Drops its waiting request, waits for its frames in flight, frees it. */
+RGAPI RgResult RGCONV rgExtSessionDestroy( RgExtSession session );
+/* Requests a frame: camera, time and pParams as for rgDrawFrame (the render
+ * size is the session's; RENDER_RESOLUTION is ignored).  The parameters are
+ * copied; the call returns at once. */
+RGAPI RgResult RGCONV rgExtSessionSubmit( RgExtSession session, const RgDrawFrameInfo* pInfo, uint64_t tag );
+/* Results of finished and dropped requests since the last call, oldest first. */
+RGAPI RgResult RGCONV rgExtServerPoll( RgExtServer server, RgExtFrameResult* pResults, uint32_t maxCount, uint32_t* pCount );
+/* Blocks until no request is waiting or in flight. */
+RGAPI RgResult RGCONV rgExtServerWaitIdle( RgExtServer server );
+RGAPI RgResult RGCONV rgExtServerGetStats( RgExtServer server, RgExtServerStats* pOut );
+/* The session's framebuffer set: its own persistent planes (FINAL holds the
+ * last finished frame) and views of its stream's within-frame planes, which
+ * the stream's next frame overwrites.  Owned by the session. */
+RGAPI RgResult RGCONV rgExtSessionGetFramebuffers( RgExtSession session, RgExtFramebuffers* pOut );
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,68 @@
+// Frame orchestration shared by the RTGL1 instance and the render server:
+// frame constants from an RgDrawFrameInfo (FillUniform,
+// Source/VulkanDevice.cpp:142-482) and the GPU passes of one frame after the
+// scene is resident (VulkanDevice::Render, :540-588, as rebuilt here).
+#pragma once
+
+#include "rg_bvh.h"
+#include "rg_framebuffers.h"
+#include "rg_pathtrace.h"
+#include "rg_scene.h"
+
+namespace rg
+{
+
+// Finds a structure of the RgDrawFrameInfo::pParams chain by its sType.
+template< typename T >
+const T* FindParams( const RgDrawFrameInfo& info, RgStructureType type )
+{
+    for( auto* n = static_cast< const RgDrawFrameTonemappingParams* >( info.pParams ); n;
+         n       = static_cast< const RgDrawFrameTonemappingParams* >( n->pNext ) )
+    {
+        if( n->sType == type )
+        {
+            return reinterpret_cast< const T* >( n );
+        }
+    }
+    return nullptr;
+}
+
+// Render size of a frame: RENDER_RESOLUTION custom size or the path
+// params' size, else 1280 x 720; the width is made even.
+void RenderSizeOf( const RgDrawFrameInfo& di, uint32_t* w, uint32_t* h );
+
+// What a camera carries between frames on the host.
+struct CameraState
+{
+    RgExtFrameParams params{};
+    uint32_t         frameId    = 0;
+    double           prevTime   = 0.0;
+    bool             firstFrame = true;
+
+    CameraState();
+};
+
+// FillUniform's hot-path fields for this frame; the previous frame's camera
+// becomes the *Prev fields.
+void FillFrameParams( CameraState& cs, const RgDrawFrameInfo& di, uint32_t w, uint32_t h, uint32_t lightArraySize, uint32_t prevLightArraySize,
+                      bool hasDirectional, bool cullBackFaces );
+
+// Device state of one camera's frames besides its framebuffers.
+struct FrameGpu
+{
+    PathTracer   pt;
+    DeviceBuffer hist, histCopy, avgLum; // 256-bin histogram, its last copy, adapted luminance
+
+    void Init( hipStream_t s );
+    static size_t OwnBytes() { return PathTracer::OwnBytes() + 2 * 256 * 4 + 4; }
+};
+
+// Ray-traced passes, A-SVGF, eye adaptation and final compose of one frame,
+// enqueued on s without host synchronisation.  evTraced / evDenoised
+// (optional) are recorded after the traced passes and after the denoiser.
+// Returns the number of rays traced.
+uint64_t EncodeFrame( FrameGpu& g, TraceContext& tc, const RgExtFrameParams& p, const BvhObject& bvh, const DeviceScene& scene,
+                      FramebufferSet& fb, const RgDrawFrameInfo& di, hipStream_t s, hipEvent_t evTraced = nullptr,
+                      hipEvent_t evDenoised = nullptr );
+
+} // namespace rg

@@ -36,6 +36,13 @@ __host__ __device__ constexpr bool IsHalfPlane( int plane )
            plane == RG_EXT_PLANE_ATROUS_FILTERED_VARIANCE;
 }
 
+// Planes a frame reads back from the frame before: the double-buffered ones,
+// DIFF_COLOR_HISTORY (the a-trous history read by the next temporal pass),
+// DIS_GRADIENT_HISTORY and GRADIENT_PREV_PIX (read by the next frame's
+// gradient samples and GI reuse), plus the FINAL output.  All other planes
+// are written and read within one frame.
+bool IsPersistentPlane( int plane );
+
 // Device pointers of every plane, passed to kernels by value.
 struct FbPtrs
 {
@@ -44,14 +51,28 @@ struct FbPtrs
     uint32_t half; // RG_EXT_STORAGE_FP16: IsHalfPlane planes hold IEEE half
 };
 
+// The within-frame planes at one capacity, shared by framebuffer sets whose
+// frames run one after another on one stream (render sessions of a server).
+struct TransientPlanes
+{
+    uint32_t     capW = 0, capH = 0, storage = RG_EXT_STORAGE_FP32;
+    DeviceBuffer planes[ RG_EXT_PLANE_COUNT ];
+
+    void Create( uint32_t maxW, uint32_t maxH, uint32_t storageMode );
+};
+
 struct FramebufferSet
 {
-    uint32_t     width = 0, height = 0, gw = 0, gh = 0;
-    uint32_t     storage = RG_EXT_STORAGE_FP32;
-    DeviceBuffer planes[ RG_EXT_PLANE_COUNT ][ 2 ];
-    int          parity = 0;
+    uint32_t         width = 0, height = 0, gw = 0, gh = 0;
+    uint32_t         storage = RG_EXT_STORAGE_FP32;
+    DeviceBuffer     planes[ RG_EXT_PLANE_COUNT ][ 2 ];
+    int              parity = 0;
+    TransientPlanes* shared = nullptr; // within-frame planes live there when set
 
-    void   Create( uint32_t w, uint32_t h, uint32_t storageMode = RG_EXT_STORAGE_FP32 );
+    // With `transient`, only the persistent planes are allocated here.
+    void   Create( uint32_t w, uint32_t h, uint32_t storageMode = RG_EXT_STORAGE_FP32, TransientPlanes* transient = nullptr );
+    // Device bytes Create allocates for a w x h set.
+    static size_t BytesFor( uint32_t w, uint32_t h, uint32_t storageMode, bool persistentOnly );
     bool   Half() const { return storage == RG_EXT_STORAGE_FP16; }
     size_t PlaneBytes( int plane ) const;
     void*  Cur( int plane );

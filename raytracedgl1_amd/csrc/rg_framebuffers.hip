@@ -57,6 +57,61 @@ const PlaneDesc& GetPlaneDesc( int plane )
     return g_planes[ plane ];
 }
 
+bool IsPersistentPlane( int plane )
+{
+    return g_planes[ plane ].prev || plane == RG_EXT_PLANE_DIFF_COLOR_HISTORY || plane == RG_EXT_PLANE_DIS_GRADIENT_HISTORY ||
+           plane == RG_EXT_PLANE_GRADIENT_PREV_PIX || plane == RG_EXT_PLANE_FINAL;
+}
+
+namespace
+{
+size_t PlaneBytesFor( int plane, uint32_t w, uint32_t h, uint32_t storage )
+{
+    const PlaneDesc& d  = g_planes[ plane ];
+    const size_t     gw = ( w + 1 ) / 3 > 1 ? ( w + 1 ) / 3 : 1, gh = ( h + 1 ) / 3 > 1 ? ( h + 1 ) / 3 : 1;
+    const size_t     px = d.layout == LAYOUT_STRATA ? gw * gh : size_t( w ) * h;
+    return px * d.comps * ( storage == RG_EXT_STORAGE_FP16 && IsHalfPlane( plane ) ? 2 : 4 );
+}
+void CheckSize( uint32_t w, uint32_t h, uint32_t storageMode )
+{
+    RG_REQUIRE( storageMode == RG_EXT_STORAGE_FP32 || storageMode == RG_EXT_STORAGE_FP16, RG_RESULT_WRONG_FUNCTION_ARGUMENT,
+                "unknown storage mode" );
+    RG_REQUIRE( w >= 2 && h >= 1 && ( w % 2 ) == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "render width must be even and >= 2" );
+    RG_REQUIRE( uint64_t( w ) * h <= ( 1ull << 26 ), RG_RESULT_WRONG_FUNCTION_ARGUMENT, "render size too large" );
+}
+} // namespace
+
+size_t FramebufferSet::BytesFor( uint32_t w, uint32_t h, uint32_t storageMode, bool persistentOnly )
+{
+    size_t t = 0;
+    for( int p = 0; p < RG_EXT_PLANE_COUNT; p++ )
+    {
+        if( !persistentOnly || IsPersistentPlane( p ) )
+        {
+            t += PlaneBytesFor( p, w, h, storageMode ) * ( g_planes[ p ].prev ? 2 : 1 );
+        }
+    }
+    return t;
+}
+
+void TransientPlanes::Create( uint32_t maxW, uint32_t maxH, uint32_t storageMode )
+{
+    CheckSize( maxW, maxH, storageMode );
+    capW    = maxW;
+    capH    = maxH;
+    storage = storageMode;
+    for( int p = 0; p < RG_EXT_PLANE_COUNT; p++ )
+    {
+        planes[ p ].Release();
+        if( !IsPersistentPlane( p ) )
+        {
+            const size_t bytes = PlaneBytesFor( p, maxW, maxH, storageMode );
+            planes[ p ].Reserve( bytes );
+            RG_HIP_CHECK( hipMemset( planes[ p ].ptr, 0, bytes ) );
+        }
+    }
+}
+
 size_t FramebufferSet::PlaneBytes( int plane ) const
 {
     const PlaneDesc& d = g_planes[ plane ];
@@ -64,13 +119,21 @@ size_t FramebufferSet::PlaneBytes( int plane ) const
     return px * d.comps * ( Half() && IsHalfPlane( plane ) ? 2 : 4 );
 }
 
-void FramebufferSet::Create( uint32_t w, uint32_t h, uint32_t storageMode )
+
+void FramebufferSet::Create( uint32_t w, uint32_t h, uint32_t storageMode, TransientPlanes* transient )
 {
-    RG_REQUIRE( storageMode == RG_EXT_STORAGE_FP32 || storageMode == RG_EXT_STORAGE_FP16, RG_RESULT_WRONG_FUNCTION_ARGUMENT,
-                "unknown storage mode" );
+    CheckSize( w, h, storageMode );
+    if( transient )
+    {
+        RG_REQUIRE( transient->storage == storageMode, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "shared planes have another storage mode" );
+        for( int p = 0; p < RG_EXT_PLANE_COUNT; p++ )
+        {
+            RG_REQUIRE( IsPersistentPlane( p ) || PlaneBytesFor( p, w, h, storageMode ) <= transient->planes[ p ].bytes,
+                        RG_RESULT_WRONG_FUNCTION_ARGUMENT, "render size exceeds the shared planes' capacity" );
+        }
+    }
     storage = storageMode;
-    RG_REQUIRE( w >= 2 && h >= 1 && ( w % 2 ) == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "render width must be even and >= 2" );
-    RG_REQUIRE( uint64_t( w ) * h <= ( 1ull << 26 ), RG_RESULT_WRONG_FUNCTION_ARGUMENT, "render size too large" );
+    shared  = transient;
     width  = w;
     height = h;
     gw     = ( w + 1 ) / 3 > 1 ? ( w + 1 ) / 3 : 1;
@@ -82,6 +145,10 @@ void FramebufferSet::Create( uint32_t w, uint32_t h, uint32_t storageMode )
         for( int k = 0; k < ( g_planes[ p ].prev ? 2 : 1 ); k++ )
         {
             planes[ p ][ k ].Release();
+            if( shared && !IsPersistentPlane( p ) )
+            {
+                continue;
+            }
             planes[ p ][ k ].Reserve( bytes );
             RG_HIP_CHECK( hipMemset( planes[ p ][ k ].ptr, 0, bytes ) );
         }
@@ -90,11 +157,19 @@ void FramebufferSet::Create( uint32_t w, uint32_t h, uint32_t storageMode )
 
 void* FramebufferSet::Cur( int plane )
 {
+    if( shared && !IsPersistentPlane( plane ) )
+    {
+        return shared->planes[ plane ].ptr;
+    }
     return planes[ plane ][ g_planes[ plane ].prev ? parity : 0 ].ptr;
 }
 
 void* FramebufferSet::Prev( int plane )
 {
+    if( shared && !IsPersistentPlane( plane ) )
+    {
+        return shared->planes[ plane ].ptr;
+    }
     return planes[ plane ][ g_planes[ plane ].prev ? ( parity ^ 1 ) : 0 ].ptr;
 }
 
@@ -112,12 +187,7 @@ FbPtrs FramebufferSet::Ptrs()
 
 size_t FramebufferSet::TotalBytes() const
 {
-    size_t t = 0;
-    for( int p = 0; p < RG_EXT_PLANE_COUNT; p++ )
-    {
-        t += PlaneBytes( p ) * ( g_planes[ p ].prev ? 2 : 1 );
-    }
-    return t;
+    return BytesFor( width, height, storage, shared != nullptr );
 }
 
 } // namespace rg

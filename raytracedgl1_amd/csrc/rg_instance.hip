@@ -7,6 +7,7 @@
 // assembly, LBVH build, ray-traced passes, A-SVGF, eye adaptation.
 #include "rg_bvh.h"
 #include "rg_denoise.h"
+#include "rg_frame.h"
 #include "rg_framebuffers.h"
 #include "rg_internal.h"
 #include "rg_kernels.h"
@@ -20,64 +21,6 @@ namespace rg
 {
 namespace
 {
-
-// Column-major 4x4 inverse in FP64 (cofactor expansion).
-void Inverse4( const float* m, float* out )
-{
-    double a[ 16 ], inv[ 16 ];
-    for( int i = 0; i < 16; i++ )
-    {
-        a[ i ] = m[ i ];
-    }
-    inv[ 0 ]  = a[ 5 ] * a[ 10 ] * a[ 15 ] - a[ 5 ] * a[ 11 ] * a[ 14 ] - a[ 9 ] * a[ 6 ] * a[ 15 ] + a[ 9 ] * a[ 7 ] * a[ 14 ] + a[ 13 ] * a[ 6 ] * a[ 11 ] - a[ 13 ] * a[ 7 ] * a[ 10 ];
-    inv[ 4 ]  = -a[ 4 ] * a[ 10 ] * a[ 15 ] + a[ 4 ] * a[ 11 ] * a[ 14 ] + a[ 8 ] * a[ 6 ] * a[ 15 ] - a[ 8 ] * a[ 7 ] * a[ 14 ] - a[ 12 ] * a[ 6 ] * a[ 11 ] + a[ 12 ] * a[ 7 ] * a[ 10 ];
-    inv[ 8 ]  = a[ 4 ] * a[ 9 ] * a[ 15 ] - a[ 4 ] * a[ 11 ] * a[ 13 ] - a[ 8 ] * a[ 5 ] * a[ 15 ] + a[ 8 ] * a[ 7 ] * a[ 13 ] + a[ 12 ] * a[ 5 ] * a[ 11 ] - a[ 12 ] * a[ 7 ] * a[ 9 ];
-    inv[ 12 ] = -a[ 4 ] * a[ 9 ] * a[ 14 ] + a[ 4 ] * a[ 10 ] * a[ 13 ] + a[ 8 ] * a[ 5 ] * a[ 14 ] - a[ 8 ] * a[ 6 ] * a[ 13 ] - a[ 12 ] * a[ 5 ] * a[ 10 ] + a[ 12 ] * a[ 6 ] * a[ 9 ];
-    inv[ 1 ]  = -a[ 1 ] * a[ 10 ] * a[ 15 ] + a[ 1 ] * a[ 11 ] * a[ 14 ] + a[ 9 ] * a[ 2 ] * a[ 15 ] - a[ 9 ] * a[ 3 ] * a[ 14 ] - a[ 13 ] * a[ 2 ] * a[ 11 ] + a[ 13 ] * a[ 3 ] * a[ 10 ];
-    inv[ 5 ]  = a[ 0 ] * a[ 10 ] * a[ 15 ] - a[ 0 ] * a[ 11 ] * a[ 14 ] - a[ 8 ] * a[ 2 ] * a[ 15 ] + a[ 8 ] * a[ 3 ] * a[ 14 ] + a[ 12 ] * a[ 2 ] * a[ 11 ] - a[ 12 ] * a[ 3 ] * a[ 10 ];
-    inv[ 9 ]  = -a[ 0 ] * a[ 9 ] * a[ 15 ] + a[ 0 ] * a[ 11 ] * a[ 13 ] + a[ 8 ] * a[ 1 ] * a[ 15 ] - a[ 8 ] * a[ 3 ] * a[ 13 ] - a[ 12 ] * a[ 1 ] * a[ 11 ] + a[ 12 ] * a[ 3 ] * a[ 9 ];
-    inv[ 13 ] = a[ 0 ] * a[ 9 ] * a[ 14 ] - a[ 0 ] * a[ 10 ] * a[ 13 ] - a[ 8 ] * a[ 1 ] * a[ 14 ] + a[ 8 ] * a[ 2 ] * a[ 13 ] + a[ 12 ] * a[ 1 ] * a[ 10 ] - a[ 12 ] * a[ 2 ] * a[ 9 ];
-    inv[ 2 ]  = a[ 1 ] * a[ 6 ] * a[ 15 ] - a[ 1 ] * a[ 7 ] * a[ 14 ] - a[ 5 ] * a[ 2 ] * a[ 15 ] + a[ 5 ] * a[ 3 ] * a[ 14 ] + a[ 13 ] * a[ 2 ] * a[ 7 ] - a[ 13 ] * a[ 3 ] * a[ 6 ];
-    inv[ 6 ]  = -a[ 0 ] * a[ 6 ] * a[ 15 ] + a[ 0 ] * a[ 7 ] * a[ 14 ] + a[ 4 ] * a[ 2 ] * a[ 15 ] - a[ 4 ] * a[ 3 ] * a[ 14 ] - a[ 12 ] * a[ 2 ] * a[ 7 ] + a[ 12 ] * a[ 3 ] * a[ 6 ];
-    inv[ 10 ] = a[ 0 ] * a[ 5 ] * a[ 15 ] - a[ 0 ] * a[ 7 ] * a[ 13 ] - a[ 4 ] * a[ 1 ] * a[ 15 ] + a[ 4 ] * a[ 3 ] * a[ 13 ] + a[ 12 ] * a[ 1 ] * a[ 7 ] - a[ 12 ] * a[ 3 ] * a[ 5 ];
-    inv[ 14 ] = -a[ 0 ] * a[ 5 ] * a[ 14 ] + a[ 0 ] * a[ 6 ] * a[ 13 ] + a[ 4 ] * a[ 1 ] * a[ 14 ] - a[ 4 ] * a[ 2 ] * a[ 13 ] - a[ 12 ] * a[ 1 ] * a[ 6 ] + a[ 12 ] * a[ 2 ] * a[ 5 ];
-    inv[ 3 ]  = -a[ 1 ] * a[ 6 ] * a[ 11 ] + a[ 1 ] * a[ 7 ] * a[ 10 ] + a[ 5 ] * a[ 2 ] * a[ 11 ] - a[ 5 ] * a[ 3 ] * a[ 10 ] - a[ 9 ] * a[ 2 ] * a[ 7 ] + a[ 9 ] * a[ 3 ] * a[ 6 ];
-    inv[ 7 ]  = a[ 0 ] * a[ 6 ] * a[ 11 ] - a[ 0 ] * a[ 7 ] * a[ 10 ] - a[ 4 ] * a[ 2 ] * a[ 11 ] + a[ 4 ] * a[ 3 ] * a[ 10 ] + a[ 8 ] * a[ 2 ] * a[ 7 ] - a[ 8 ] * a[ 3 ] * a[ 6 ];
-    inv[ 11 ] = -a[ 0 ] * a[ 5 ] * a[ 11 ] + a[ 0 ] * a[ 7 ] * a[ 9 ] + a[ 4 ] * a[ 1 ] * a[ 11 ] - a[ 4 ] * a[ 3 ] * a[ 9 ] - a[ 8 ] * a[ 1 ] * a[ 7 ] + a[ 8 ] * a[ 3 ] * a[ 5 ];
-    inv[ 15 ] = a[ 0 ] * a[ 5 ] * a[ 10 ] - a[ 0 ] * a[ 6 ] * a[ 9 ] - a[ 4 ] * a[ 1 ] * a[ 10 ] + a[ 4 ] * a[ 2 ] * a[ 9 ] + a[ 8 ] * a[ 1 ] * a[ 6 ] - a[ 8 ] * a[ 2 ] * a[ 5 ];
-    double det = a[ 0 ] * inv[ 0 ] + a[ 1 ] * inv[ 4 ] + a[ 2 ] * inv[ 8 ] + a[ 3 ] * inv[ 12 ];
-    det        = det != 0.0 ? 1.0 / det : 0.0;
-    for( int i = 0; i < 16; i++ )
-    {
-        out[ i ] = float( inv[ i ] * det );
-    }
-}
-
-// Matrix::MakeProjectionMatrix (Source/Matrix.cpp:380-395)
-void MakeProjection( float* m, float aspect, float fovY, float zn, float zf )
-{
-    const float t = tanf( fovY * 0.5f );
-    std::memset( m, 0, 16 * sizeof( float ) );
-    m[ 0 ]  = 1.0f / ( aspect * t );
-    m[ 5 ]  = -1.0f / t;
-    m[ 10 ] = zf / ( zn - zf );
-    m[ 11 ] = -1.0f;
-    m[ 14 ] = ( zf * zn ) / ( zn - zf );
-}
-
-template< typename T >
-const T* FindParams( const RgDrawFrameInfo& info, RgStructureType type )
-{
-    for( auto* n = static_cast< const RgDrawFrameTonemappingParams* >( info.pParams ); n;
-         n       = static_cast< const RgDrawFrameTonemappingParams* >( n->pNext ) )
-    {
-        if( n->sType == type )
-        {
-            return reinterpret_cast< const T* >( n );
-        }
-    }
-    return nullptr;
-}
 
 enum Stage
 {
@@ -101,16 +44,12 @@ struct Instance
     DeviceScene                                 dscene;
     BvhObject                                   bvh;
     TraceContext                                tc;
-    PathTracer                                  pt;
+    FrameGpu                                    g;
+    CameraState                                 cam;
     std::unique_ptr< RgExtFramebuffers_T >      fb;
-    DeviceBuffer                                hist, histCopy, avgLum;
-    RgExtFrameParams                            params{};
     RgExtFrameStats                             stats{};
     bool                                        frameStarted = false;
     bool                                        staticGeometry = false;
-    bool                                        firstFrame   = true;
-    uint32_t                                    frameId      = 0;
-    double                                      prevTime     = 0.0;
     hipEvent_t                                  ev[ ST_COUNT ] = {};
 
     Instance( const RgInstanceCreateInfo& ci )
@@ -126,15 +65,7 @@ struct Instance
         {
             RG_HIP_CHECK( hipEventCreate( &e ) );
         }
-        hist.Reserve( 256 * 4 );
-        histCopy.Reserve( 256 * 4 );
-        avgLum.Reserve( 4 );
-        RG_HIP_CHECK( hipMemsetAsync( hist.ptr, 0, 256 * 4, stream ) );
-        RG_HIP_CHECK( hipMemsetAsync( avgLum.ptr, 0, 4, stream ) );
-        pt.Init( stream );
-        RgExtFrameParams d;
-        rgExtFrameParamsDefaults( 1280, 720, &d );
-        params = d;
+        g.Init( stream );
         RG_HIP_CHECK( hipStreamSynchronize( stream ) );
     }
     ~Instance()
@@ -167,99 +98,6 @@ struct Instance
         scene.BeginFrame();
     }
 
-    // FillUniform (Source/VulkanDevice.cpp:142-482), hot-path fields only.
-    void FillParams( const RgDrawFrameInfo& di, uint32_t w, uint32_t h )
-    {
-        RgExtFrameParams& p = params;
-        std::memcpy( p.viewPrev, p.view, sizeof( p.view ) );
-        std::memcpy( p.projectionPrev, p.projection, sizeof( p.projection ) );
-        std::memcpy( p.cameraPositionPrev, p.cameraPosition, sizeof( p.cameraPosition ) );
-        std::memcpy( p.view, di.view, sizeof( p.view ) );
-        MakeProjection( p.projection, float( w ) / float( h ), di.fovYRadians, di.cameraNear, di.cameraFar );
-        Inverse4( p.view, p.invView );
-        Inverse4( p.projection, p.invProjection );
-        p.cameraPosition[ 0 ] = p.invView[ 12 ];
-        p.cameraPosition[ 1 ] = p.invView[ 13 ];
-        p.cameraPosition[ 2 ] = p.invView[ 14 ];
-        p.cameraPosition[ 3 ] = 0.0f;
-        if( firstFrame )
-        {
-            std::memcpy( p.viewPrev, p.view, sizeof( p.view ) );
-            std::memcpy( p.projectionPrev, p.projection, sizeof( p.projection ) );
-            std::memcpy( p.cameraPositionPrev, p.cameraPosition, sizeof( p.cameraPosition ) );
-        }
-        p.renderWidth  = w;
-        p.renderHeight = h;
-        p.frameId      = frameId;
-        p.timeDelta    = float( std::max( di.currentTime - prevTime, 0.001 ) );
-        prevTime       = di.currentTime;
-        p.jitterX = p.jitterY = 0.0f;
-
-        const auto* tm      = FindParams< RgDrawFrameTonemappingParams >( di, RG_STRUCTURE_TYPE_TONEMAPPING );
-        const float ev100Min = tm ? tm->ev100Min : 0.0f, ev100Max = tm ? tm->ev100Max : 10.0f;
-        p.stopEyeAdaptation  = tm ? tm->disableEyeAdaptation : 0;
-        p.minLogLuminance    = std::log2( std::exp2( ev100Min ) * 12.5f / 100.0f );
-        p.maxLogLuminance    = std::log2( std::exp2( ev100Max ) * 12.5f / 100.0f );
-
-        p.lightCount             = uint32_t( scene.lights.size() ) - 1u;
-        p.lightCountPrev         = scene.prevLightCount > 0 ? scene.prevLightCount - 1u : 0u;
-        p.directionalLightExists = scene.hasDirectional ? 1u : 0u;
-
-        const auto* sky = FindParams< RgDrawFrameSkyParams >( di, RG_STRUCTURE_TYPE_SKY );
-        if( sky )
-        {
-            for( int i = 0; i < 3; i++ )
-            {
-                p.skyColorDefault[ i ] = sky->skyColorDefault.data[ i ];
-            }
-            p.skyColorMultiplier = std::max( 0.0f, sky->skyColorMultiplier );
-            p.skyColorSaturation = std::max( 0.0f, sky->skyColorSaturation );
-        }
-        const auto* il = FindParams< RgDrawFrameIlluminationParams >( di, RG_STRUCTURE_TYPE_ILLUMINATION );
-        p.maxBounceShadowsLights   = il ? il->maxBounceShadows : 2u;
-        p.polyLightSpotlightFactor = il ? std::max( 0.0f, il->polygonalLightSpotlightFactor ) : 2.0f;
-        p.indirSecondBounce        = il ? !!il->enableSecondBounceForIndirect : 1u;
-        p.cellWorldSize            = il ? std::max( il->cellWorldSize, 0.001f ) : 1.0f;
-        p.gradientMultDiffuse      = il ? std::clamp( il->directDiffuseSensitivityToChange, 0.0f, 1.0f ) : 0.5f;
-        p.gradientMultIndirect     = il ? std::clamp( il->indirectDiffuseSensitivityToChange, 0.0f, 1.0f ) : 0.2f;
-        p.gradientMultSpecular     = il ? std::clamp( il->specularSensitivityToChange, 0.0f, 1.0f ) : 0.5f;
-        const auto* tx             = FindParams< RgDrawFrameTexturesParams >( di, RG_STRUCTURE_TYPE_TEXTURES );
-        p.minRoughness             = tx ? std::clamp( tx->minRoughness, 0.0f, 1.0f ) : 0.0f;
-        p.emissionMapBoost         = tx ? std::max( tx->emissionMapBoost, 0.0f ) : 100.0f;
-
-        p.rayCullBackFaces  = info.rayCullBackFacingTriangles ? 1u : 0u;
-        p.rayLength         = std::clamp( di.rayLength, 0.1f, 10000.0f );
-        p.primaryRayMinDist = std::clamp( di.cameraNear, 0.001f, p.rayLength );
-        p.antiFireflyEnabled = 1;
-
-        const auto* ext = FindParams< RgExtDrawFramePathParams >( di, RG_EXT_STRUCTURE_TYPE_PATH_PARAMS );
-        p.atrousIterations        = ext && ext->atrousIterations ? ext->atrousIterations : 4u;
-        p.restirInitialCandidates = ext && ext->restirInitialCandidates ? ext->restirInitialCandidates : 8u;
-        p.restirTemporalSamples   = ext && ext->restirTemporalSamples != 0xFFFFFFFFu ? ext->restirTemporalSamples : 1u;
-        p.restirSpatialSamples    = ext && ext->restirSpatialSamples != 0xFFFFFFFFu ? ext->restirSpatialSamples : 8u;
-        RG_REQUIRE( p.atrousIterations == 4 || p.atrousIterations == 5, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "atrousIterations must be 4 or 5" );
-    }
-
-    void RenderSize( const RgDrawFrameInfo& di, uint32_t* w, uint32_t* h )
-    {
-        *w      = 1280;
-        *h      = 720;
-        auto rr = FindParams< RgDrawFrameRenderResolutionParams >( di, RG_STRUCTURE_TYPE_RENDER_RESOLUTION );
-        if( rr && rr->resolutionMode == RG_RENDER_RESOLUTION_MODE_CUSTOM && rr->customRenderSize.width && rr->customRenderSize.height )
-        {
-            *w = rr->customRenderSize.width;
-            *h = rr->customRenderSize.height;
-        }
-        auto ext = FindParams< RgExtDrawFramePathParams >( di, RG_EXT_STRUCTURE_TYPE_PATH_PARAMS );
-        if( ext && ext->renderWidth && ext->renderHeight )
-        {
-            *w = ext->renderWidth;
-            *h = ext->renderHeight;
-        }
-        *w &= ~1u; // render width must be even (VulkanDevice.cpp:186-187)
-        RG_REQUIRE( *w >= 2 && *h >= 1, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "render size too small" );
-    }
-
     void DrawFrame( const RgDrawFrameInfo& di )
     {
         if( !frameStarted )
@@ -268,9 +106,10 @@ struct Instance
         }
         frameStarted = false;
         uint32_t w, h;
-        RenderSize( di, &w, &h );
+        RenderSizeOf( di, &w, &h );
         scene.FinishLights();
-        FillParams( di, w, h );
+        FillFrameParams( cam, di, w, h, uint32_t( scene.lights.size() ), scene.prevLightCount, scene.hasDirectional,
+                         info.rayCullBackFacingTriangles != 0 );
         {
             // styles for the lights of the next frame's uploads
             const auto* il = FindParams< RgDrawFrameIlluminationParams >( di, RG_STRUCTURE_TYPE_ILLUMINATION );
@@ -294,7 +133,7 @@ struct Instance
         {
             fb = std::make_unique< RgExtFramebuffers_T >();
             fb->fb.Create( w, h, storage );
-            firstFrame = true;
+            cam.firstFrame = true;
         }
         else
         {
@@ -303,9 +142,9 @@ struct Instance
 
         hipStream_t s = stream;
         const auto* ext = FindParams< RgExtDrawFramePathParams >( di, RG_EXT_STRUCTURE_TYPE_PATH_PARAMS );
-        pt.collectStats = ext && ext->collectTraversalStats;
+        g.pt.collectStats = ext && ext->collectTraversalStats;
         RG_HIP_CHECK( hipEventRecord( ev[ ST_BEGIN ], s ) );
-        pt.UploadLights( scene, s );
+        g.pt.UploadLights( scene, s );
         // static geometry: a frame without primitive uploads keeps the resident scene
         const bool keep = staticGeometry && scene.triCount == 0 && dscene.triCount > 0;
         if( !keep )
@@ -319,7 +158,7 @@ struct Instance
         {
             std::vector< uint32_t > geomMap;
             scene.GeomPrevToCur( geomMap );
-            pt.UploadGeomMap( geomMap, s );
+            g.pt.UploadGeomMap( geomMap, s );
         }
         RG_HIP_CHECK( hipEventRecord( ev[ ST_UPLOADED ], s ) );
         const bool changed = keep ? false : AssembleScene( dscene, s );
@@ -336,27 +175,7 @@ struct Instance
             stats.bvhRebuilt     = 1;
         }
         RG_HIP_CHECK( hipEventRecord( ev[ ST_BVH ], s ) );
-        const uint32_t flags = params.rayCullBackFaces ? RG_EXT_TRACE_CULL_FRONT_FACE : 0u;
-        const uint64_t rays  = pt.Trace( params, bvh, tc, dscene, fb->fb, flags, s, nullptr );
-        RG_HIP_CHECK( hipEventRecord( ev[ ST_TRACED ], s ) );
-        Denoise( fb->fb, params, s );
-        RG_HIP_CHECK( hipEventRecord( ev[ ST_DENOISED ], s ) );
-        LuminanceHistogram( reinterpret_cast< const float* >( fb->fb.Cur( RG_EXT_PLANE_HISTOGRAM_INPUT ) ), w * h, params.minLogLuminance,
-                            params.maxLogLuminance, hist.as< uint32_t >(), s );
-        RG_HIP_CHECK( hipMemcpyAsync( histCopy.ptr, hist.ptr, 256 * 4, hipMemcpyDeviceToDevice, s ) );
-        LuminanceAverage( hist.as< uint32_t >(), w * h, params.minLogLuminance, params.maxLogLuminance, params.timeDelta,
-                          params.stopEyeAdaptation != 0, avgLum.as< float >(), s );
-        {
-            // ImageComposition::Finalize (Source/ImageComposition.cpp:94-156)
-            const auto* tm     = FindParams< RgDrawFrameTonemappingParams >( di, RG_STRUCTURE_TYPE_TONEMAPPING );
-            float       sat[ 3 ] = { 0.0f, 0.0f, 0.0f }, ct[ 3 ] = { 1.0f, 1.0f, 1.0f }; // DrawFrameInfo.h:104-105
-            if( tm )
-            {
-                std::memcpy( sat, tm->saturation.data, sizeof( sat ) );
-                std::memcpy( ct, tm->crosstalk.data, sizeof( ct ) );
-            }
-            ComposeFinal( fb->fb, MakeLpmConsts( sat, ct ), avgLum.as< float >(), s );
-        }
+        const uint64_t rays = EncodeFrame( g, tc, cam.params, bvh, dscene, fb->fb, di, s, ev[ ST_TRACED ], ev[ ST_DENOISED ] );
         RG_HIP_CHECK( hipEventRecord( ev[ ST_END ], s ) );
         // The staging arrays are reused by the next frame's uploads.
         RG_HIP_CHECK( hipStreamSynchronize( s ) );
@@ -366,7 +185,7 @@ struct Instance
             ( void )hipEventElapsedTime( &v, ev[ a ], ev[ b ] );
             return v;
         };
-        stats.frameId        = frameId;
+        stats.frameId        = cam.frameId;
         stats.renderWidth    = w;
         stats.renderHeight   = h;
         stats.triangleCount  = dscene.triCount;
@@ -374,17 +193,17 @@ struct Instance
         stats.raysTraced     = rays;
         stats.gpuMsUpload    = ms( ST_BEGIN, ST_UPLOADED );
         stats.gpuMsAssemble  = ms( ST_UPLOADED, ST_ASSEMBLED );
-        stats.gpuMsTraversal = pt.TraversalMs();
-        stats.traversalLaunches = pt.traceLaunches;
-        pt.TraversalCounts( &stats.nodeVisits, &stats.triangleTests, &stats.traversalErrors );
+        stats.gpuMsTraversal = g.pt.TraversalMs();
+        stats.traversalLaunches = g.pt.traceLaunches;
+        g.pt.TraversalCounts( &stats.nodeVisits, &stats.triangleTests, &stats.traversalErrors );
         stats.gpuMsBvh       = ms( ST_ASSEMBLED, ST_BVH );
         stats.gpuMsTrace     = ms( ST_BVH, ST_TRACED );
         stats.gpuMsDenoise   = ms( ST_TRACED, ST_DENOISED );
         stats.gpuMsTonemap   = ms( ST_DENOISED, ST_END );
         stats.gpuMsTotal     = ms( ST_BEGIN, ST_END );
-        RG_HIP_CHECK( hipMemcpy( &stats.avgLuminance, avgLum.ptr, 4, hipMemcpyDeviceToHost ) );
-        frameId++;
-        firstFrame = false;
+        RG_HIP_CHECK( hipMemcpy( &stats.avgLuminance, g.avgLum.ptr, 4, hipMemcpyDeviceToHost ) );
+        cam.frameId++;
+        cam.firstFrame = false;
     }
 };
 
@@ -624,7 +443,7 @@ RgResult rgExtGetFrameParams( RgInstance instance, RgExtFrameParams* pOut )
 {
     return Guard( [ & ] {
         RG_REQUIRE( pOut, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
-        *pOut = Get( instance ).params;
+        *pOut = Get( instance ).cam.params;
     } );
 }
 
@@ -633,7 +452,7 @@ RgResult rgExtReadbackHistogram( RgInstance instance, uint32_t* pOut256 )
     return Guard( [ & ] {
         RG_REQUIRE( pOut256, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
         Instance& in = Get( instance );
-        RG_HIP_CHECK( hipMemcpy( pOut256, in.histCopy.ptr, 256 * 4, hipMemcpyDeviceToHost ) );
+        RG_HIP_CHECK( hipMemcpy( pOut256, in.g.histCopy.ptr, 256 * 4, hipMemcpyDeviceToHost ) );
     } );
 }
 

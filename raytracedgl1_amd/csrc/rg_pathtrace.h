@@ -16,16 +16,29 @@ struct FrameTimings
     uint64_t rays = 0;
 };
 
+// Ray, hit and per-pixel state buffers between the wavefronts of one frame.
+// Nothing in them survives a frame, so path tracers whose frames run one after
+// another on one stream can share one set.
+struct Wavefront
+{
+    DeviceBuffer rays, hits, rays2, hits2; // W*H
+    DeviceBuffer pendA, pendB, pendC;      // W*H x (6 float4 / ray / hit)
+    size_t       capPx = 0;
+
+    void          Reserve( size_t pixels );
+    static size_t BytesFor( size_t pixels );
+};
+
 struct PathTracer
 {
-    DeviceBuffer rays, hits, rays2, hits2; // W*H wavefront buffers
-    DeviceBuffer pendA, pendB, pendC;      // per-pixel state between wavefronts (W*H float4 / uint4)
+    Wavefront    own;
+    Wavefront*   wf = &own;                // set to a shared set before the first frame
     DeviceBuffer lights, prevToCur;
     DeviceBuffer geomMap;                  // last frame's geometry index -> this frame's
     std::vector< uint32_t > geomMapHost;
     uint32_t     geomMapCount = 0;
     DeviceBuffer grid[ 2 ];                // 16^3 * 128 light-grid reservoirs, ping-pong
-    DeviceBuffer blueNoise;                // 128 layers of 128x128 RGBA8
+    const uint32_t* blueNoise = nullptr;   // 128 layers of 128x128 RGBA8, one table per process
     DeviceBuffer frameAccum;               // node visits, triangle tests (u64), error bits (u32) of the frame
     hipEvent_t   traceEv[ 32 ] = {};
     uint32_t     traceLaunches = 0;
@@ -35,6 +48,8 @@ struct PathTracer
     uint32_t     W = 0, H = 0;
 
     void Init( hipStream_t s );
+    // Device bytes a path tracer owns apart from its wavefront buffers.
+    static size_t OwnBytes();
     ~PathTracer();
     // After the stream has finished the frame: traversal time summed over the
     // frame's launches, and the node / triangle counts when collected.

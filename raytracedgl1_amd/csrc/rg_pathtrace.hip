@@ -19,6 +19,8 @@
 #include "rg_pathtrace.h"
 #include "rg_shading.h"
 
+#include <mutex>
+
 namespace rg
 {
 namespace
@@ -1499,6 +1501,55 @@ void PathTracer::TraversalCounts( uint64_t* nodes, uint64_t* tris, uint32_t* err
     *errors = uint32_t( v[ 2 ] );
 }
 
+namespace
+{
+// The seeded noise table is constant: one copy serves every path tracer of
+// the process.  It is never freed (8 MB), so no path tracer outlives it.
+std::mutex g_noiseMutex;
+uint32_t*  g_noise = nullptr;
+
+const uint32_t* SharedBlueNoise( hipStream_t s )
+{
+    std::lock_guard< std::mutex > lock( g_noiseMutex );
+    if( !g_noise )
+    {
+        void* p = nullptr;
+        RG_HIP_CHECK( hipMalloc( &p, size_t( 128u ) * 128u * 128u * 4u ) );
+        k_blue_noise<<< DivUp( 128u * 128u * 128u, THREADS ), THREADS, 0, s >>>( static_cast< uint32_t* >( p ) );
+        CheckLaunch( "k_blue_noise" );
+        RG_HIP_CHECK( hipStreamSynchronize( s ) );
+        g_noise = static_cast< uint32_t* >( p );
+    }
+    return g_noise;
+}
+} // namespace
+
+size_t Wavefront::BytesFor( size_t n )
+{
+    return n * ( 2 * sizeof( RgExtRay ) + 2 * sizeof( RgExtHit ) + 16 * 6 + sizeof( RgExtRay ) + sizeof( RgExtHit ) );
+}
+
+void Wavefront::Reserve( size_t n )
+{
+    if( n <= capPx )
+    {
+        return;
+    }
+    rays.Reserve( n * sizeof( RgExtRay ) );
+    rays2.Reserve( n * sizeof( RgExtRay ) );
+    hits.Reserve( n * sizeof( RgExtHit ) );
+    hits2.Reserve( n * sizeof( RgExtHit ) );
+    pendA.Reserve( n * 16 * 6 );
+    pendB.Reserve( n * sizeof( RgExtRay ) );
+    pendC.Reserve( n * sizeof( RgExtHit ) );
+    capPx = n;
+}
+
+size_t PathTracer::OwnBytes()
+{
+    return 2 * size_t( GRID_ENTRIES ) * 16 + 3 * sizeof( uint64_t );
+}
+
 void PathTracer::Init( hipStream_t s )
 {
     for( auto& e : traceEv )
@@ -1506,9 +1557,7 @@ void PathTracer::Init( hipStream_t s )
         RG_HIP_CHECK( hipEventCreate( &e ) );
     }
     frameAccum.Reserve( 3 * sizeof( uint64_t ) );
-    blueNoise.Reserve( 128u * 128u * 128u * 4u );
-    k_blue_noise<<< DivUp( 128u * 128u * 128u, THREADS ), THREADS, 0, s >>>( blueNoise.as< uint32_t >() );
-    CheckLaunch( "k_blue_noise" );
+    blueNoise = SharedBlueNoise( s );
     for( int k = 0; k < 2; k++ )
     {
         grid[ k ].Reserve( size_t( GRID_ENTRIES ) * 16 );
@@ -1529,16 +1578,14 @@ void PathTracer::Resize( uint32_t w, uint32_t h )
     {
         return;
     }
-    W                = w;
-    H                = h;
-    const size_t n   = size_t( w ) * h;
-    rays.Reserve( n * sizeof( RgExtRay ) );
-    rays2.Reserve( n * sizeof( RgExtRay ) );
-    hits.Reserve( n * sizeof( RgExtHit ) );
-    hits2.Reserve( n * sizeof( RgExtHit ) );
-    pendA.Reserve( n * 16 * 6 );
-    pendB.Reserve( n * sizeof( RgExtRay ) );
-    pendC.Reserve( n * sizeof( RgExtHit ) );
+    W              = w;
+    H              = h;
+    const size_t n = size_t( w ) * h;
+    if( wf == &own )
+    {
+        own.Reserve( n );
+    }
+    RG_REQUIRE( wf->capPx >= n, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "render size exceeds the shared wavefront buffers" );
 }
 
 void PathTracer::UploadGeomMap( const std::vector< uint32_t >& map, hipStream_t s )
@@ -1569,11 +1616,14 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
     const FbPtrs   P = fb.Ptrs();
     SceneView      sv{ scene.tris.as< float >(), scene.triNormals.as< float >(), scene.triGeom.as< uint32_t >(), scene.geoms.as< GeomRecord >() };
     LightView      lv{ lights.as< LightEncoded >(), lightArraySize, prevToCur.as< uint32_t >(), prevToCurCount,
-                  grid[ gridParity ].as< uint4 >(), grid[ gridParity ^ 1 ].as< uint4 >(), blueNoise.as< uint32_t >() };
-    RgExtRay* R  = rays.as< RgExtRay >();
-    RgExtHit* Hh = hits.as< RgExtHit >();
-    RgExtRay* R2 = rays2.as< RgExtRay >();
-    RgExtHit* H2 = hits2.as< RgExtHit >();
+                  grid[ gridParity ].as< uint4 >(), grid[ gridParity ^ 1 ].as< uint4 >(), blueNoise };
+    RgExtRay* R  = wf->rays.as< RgExtRay >();
+    RgExtHit* Hh = wf->hits.as< RgExtHit >();
+    RgExtRay* R2 = wf->rays2.as< RgExtRay >();
+    RgExtHit* H2 = wf->hits2.as< RgExtHit >();
+    DeviceBuffer& pendA = wf->pendA;
+    DeviceBuffer& pendB = wf->pendB;
+    DeviceBuffer& pendC = wf->pendC;
     uint64_t  traced = 0;
     const bool haveGeometry = scene.triCount > 0 && bvh.nodeCount > 0;
 

@@ -68,6 +68,17 @@ struct PinnedArray
         }
     }
     void Clear() { size = 0; }
+    // Returns the pinned memory to the system.
+    void Free()
+    {
+        if( data )
+        {
+            ( void )hipHostFree( data );
+        }
+        data     = nullptr;
+        size     = 0;
+        capacity = 0;
+    }
     void Reserve( size_t n )
     {
         if( n <= capacity )

@@ -1,0 +1,669 @@
+// Render server: camera sessions over shared static scenes, with the frames
+// of many sessions in flight together on a few HIP streams.
+//
+// A session is the per-camera state of the reference's VulkanDevice (the
+// A-SVGF history and moments, ReSTIR DI/GI reservoirs, the light grid and the
+// adapted luminance), kept resident in HBM; a frame is the part of
+// VulkanDevice::Render (Source/VulkanDevice.cpp:540-588) from the traced
+// passes to the final compose, as EncodeFrame runs it.  Within-frame planes
+// and wavefront buffers belong to a stream, so a session costs only what it
+// carries from frame to frame.
+#include "rg_frame.h"
+#include "rg_internal.h"
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <list>
+#include <memory>
+#include <mutex>
+#include <optional>
+#include <thread>
+#include <vector>
+
+namespace rg
+{
+namespace
+{
+
+double NowSeconds()
+{
+    return std::chrono::duration< double >( std::chrono::steady_clock::now().time_since_epoch() ).count();
+}
+
+// The parameter structures a frame reads, copied at submit so that the
+// caller's chain may go away before the frame runs.
+struct FrameRequest
+{
+    RgDrawFrameInfo               di{};
+    RgDrawFrameTonemappingParams  tm{};
+    RgDrawFrameSkyParams          sky{};
+    RgDrawFrameIlluminationParams il{};
+    RgDrawFrameTexturesParams     tx{};
+    RgExtDrawFramePathParams      path{};
+    bool                          hasTm = false, hasSky = false, hasIl = false, hasTx = false, hasPath = false;
+    uint64_t                      tag    = 0;
+    double                        submit = 0.0;
+
+    void Capture( const RgDrawFrameInfo& in )
+    {
+        di = in;
+        if( const auto* p = FindParams< RgDrawFrameTonemappingParams >( in, RG_STRUCTURE_TYPE_TONEMAPPING ) )
+        {
+            tm = *p, hasTm = true;
+        }
+        if( const auto* p = FindParams< RgDrawFrameSkyParams >( in, RG_STRUCTURE_TYPE_SKY ) )
+        {
+            sky = *p, hasSky = true;
+            sky.pSkyCubemapTextureName = nullptr;
+        }
+        if( const auto* p = FindParams< RgDrawFrameIlluminationParams >( in, RG_STRUCTURE_TYPE_ILLUMINATION ) )
+        {
+            il = *p, hasIl = true;
+            // the scene's lights are encoded once, so light styles do not apply
+            il.lightUniqueIdIgnoreFirstPersonViewerShadows = nullptr;
+            il.lightstyleValuesCount                       = 0;
+            il.pLightstyleValues                           = nullptr;
+        }
+        if( const auto* p = FindParams< RgDrawFrameTexturesParams >( in, RG_STRUCTURE_TYPE_TEXTURES ) )
+        {
+            tx = *p, hasTx = true;
+        }
+        if( const auto* p = FindParams< RgExtDrawFramePathParams >( in, RG_EXT_STRUCTURE_TYPE_PATH_PARAMS ) )
+        {
+            path = *p, hasPath = true;
+        }
+        Relink();
+    }
+    // Rebuilds di.pParams over the copies (after the request has moved).
+    void Relink()
+    {
+        void* head = nullptr;
+        auto  push = [ & ]( bool has, auto& s ) {
+            if( has )
+            {
+                s.pNext = head;
+                head    = &s;
+            }
+        };
+        push( hasTm, tm );
+        push( hasSky, sky );
+        push( hasIl, il );
+        push( hasTx, tx );
+        push( hasPath, path );
+        di.pParams = head;
+    }
+};
+
+} // namespace
+} // namespace rg
+
+using namespace rg;
+
+struct RgExtServer_T;
+
+struct RgExtScene_T
+{
+    RgExtServer_T*             server = nullptr;
+    SceneCollector          sc;
+    DeviceScene             ds;
+    BvhObject               bvh;
+    std::vector< uint32_t > geomMap;
+    bool                    built    = false;
+    uint32_t                sessions = 0;
+    size_t                  bytes    = 0;
+};
+
+struct RgExtSession_T
+{
+    RgExtServer_T*                   server = nullptr;
+    RgExtScene_T*                    scene  = nullptr;
+    uint32_t                      width = 0, height = 0, stream = 0;
+    RgExtFramebuffers_T           fbs;
+    FrameGpu                      g;
+    CameraState                   cam;
+    std::optional< FrameRequest > pending;
+    bool                          queued   = false;
+    bool                          dying    = false;
+    uint32_t                      inflight = 0;
+    size_t                        bytes    = 0;
+};
+
+namespace rg
+{
+namespace
+{
+struct StreamCtx
+{
+    struct InFlight
+    {
+        RgExtSession_T* s;
+        uint64_t     tag;
+        uint32_t     frameId;
+        double       submit, start;
+        hipEvent_t   ev;
+    };
+    hipStream_t               stream = nullptr;
+    TransientPlanes           planes;
+    Wavefront                 wf;
+    TraceContext              tc;
+    std::vector< hipEvent_t > events;
+    uint32_t                  nextEvent = 0;
+    std::deque< InFlight >    inflight;
+
+    ~StreamCtx()
+    {
+        for( hipEvent_t e : events )
+        {
+            ( void )hipEventDestroy( e );
+        }
+        if( stream )
+        {
+            ( void )hipStreamDestroy( stream );
+        }
+    }
+};
+} // namespace
+} // namespace rg
+
+struct RgExtServer_T
+{
+    RgExtServerCreateInfo                       info{};
+    uint32_t                                    nodeFormat = RG_EXT_BVH_NODE_FP32;
+    uint32_t                                    storage    = RG_EXT_STORAGE_FP32;
+    hipStream_t                                 setup      = nullptr; // scene builds and session set-up
+    std::vector< std::unique_ptr< StreamCtx > > streams;
+    std::list< std::unique_ptr< RgExtScene_T > >   scenes;
+    std::list< std::unique_ptr< RgExtSession_T > > sessions;
+    std::deque< RgExtSession_T* >                  ready; // sessions with a waiting request
+    std::deque< RgExtFrameResult >              results;
+    std::mutex                                  m;
+    std::condition_variable                     cvWork, cvIdle;
+    std::thread                                 worker;
+    bool                                        stop = false;
+    uint32_t                                    roundRobin = 0;
+    uint64_t                                    sessionBytes = 0, sharedBytes = 0, sceneBytes = 0;
+    uint64_t                                    done = 0, dropped = 0, failed = 0;
+
+    void Result( RgExtSession_T* s, uint64_t tag, uint32_t status, uint32_t frameId, double submit, double start, double end )
+    {
+        RgExtFrameResult r;
+        r.session    = reinterpret_cast< RgExtSession >( s );
+        r.tag        = tag;
+        r.status     = status;
+        r.frameId    = frameId;
+        r.submitTime = submit;
+        r.startTime  = start;
+        r.endTime    = end;
+        results.push_back( r );
+        ( status == RG_EXT_FRAME_DONE ? done : status == RG_EXT_FRAME_DROPPED ? dropped : failed )++;
+    }
+
+    bool Busy() const
+    {
+        if( !ready.empty() )
+        {
+            return true;
+        }
+        for( const auto& st : streams )
+        {
+            if( !st->inflight.empty() )
+            {
+                return true;
+            }
+        }
+        return false;
+    }
+
+    // Retires the frames whose completion events have fired.
+    void Retire()
+    {
+        for( auto& st : streams )
+        {
+            while( !st->inflight.empty() )
+            {
+                const auto&      f = st->inflight.front();
+                const hipError_t q = hipEventQuery( f.ev );
+                if( q == hipErrorNotReady )
+                {
+                    break;
+                }
+                Result( f.s, f.tag, q == hipSuccess ? RG_EXT_FRAME_DONE : RG_EXT_FRAME_FAILED, f.frameId, f.submit, f.start, NowSeconds() );
+                f.s->inflight--;
+                st->inflight.pop_front();
+            }
+        }
+    }
+
+    void Launch( RgExtSession_T& s, StreamCtx& st )
+    {
+        FrameRequest req = std::move( *s.pending );
+        s.pending.reset();
+        req.Relink();
+        const double start = NowSeconds();
+        try
+        {
+            RgExtScene_T& sc = *s.scene;
+            FillFrameParams( s.cam, req.di, s.width, s.height, uint32_t( sc.sc.lights.size() ), uint32_t( sc.sc.lights.size() ),
+                             sc.sc.hasDirectional, false );
+            if( !s.cam.firstFrame )
+            {
+                s.fbs.fb.Swap();
+            }
+            EncodeFrame( s.g, st.tc, s.cam.params, sc.bvh, sc.ds, s.fbs.fb, req.di, st.stream );
+            hipEvent_t ev = st.events[ st.nextEvent++ % st.events.size() ];
+            RG_HIP_CHECK( hipEventRecord( ev, st.stream ) );
+            st.inflight.push_back( { &s, req.tag, s.cam.frameId, req.submit, start, ev } );
+            s.inflight++;
+            s.cam.frameId++;
+            s.cam.firstFrame = false;
+        }
+        catch( const Error& e )
+        {
+            SetLastError( e.what() );
+            Result( &s, req.tag, RG_EXT_FRAME_FAILED, s.cam.frameId, req.submit, start, NowSeconds() );
+        }
+    }
+
+    void Run()
+    {
+        std::unique_lock< std::mutex > lk( m );
+        for( ;; )
+        {
+            Retire();
+            bool launched = false;
+            for( auto it = ready.begin(); it != ready.end(); )
+            {
+                RgExtSession_T* s  = *it;
+                StreamCtx&   st = *streams[ s->stream ];
+                if( st.inflight.size() < info.framesInFlightPerStream )
+                {
+                    it        = ready.erase( it );
+                    s->queued = false;
+                    Launch( *s, st );
+                    launched = true;
+                }
+                else
+                {
+                    ++it;
+                }
+            }
+            cvIdle.notify_all();
+            if( stop && !Busy() )
+            {
+                return;
+            }
+            if( !launched )
+            {
+                bool inflight = false;
+                for( const auto& st : streams )
+                {
+                    inflight = inflight || !st->inflight.empty();
+                }
+                if( inflight )
+                {
+                    cvWork.wait_for( lk, std::chrono::microseconds( 50 ) );
+                }
+                else
+                {
+                    cvWork.wait( lk, [ & ] { return stop || !ready.empty(); } );
+                }
+            }
+        }
+    }
+
+    void WaitIdle()
+    {
+        std::unique_lock< std::mutex > lk( m );
+        cvIdle.wait( lk, [ & ] { return !Busy(); } );
+    }
+
+    ~RgExtServer_T()
+    {
+        if( worker.joinable() )
+        {
+            {
+                std::lock_guard< std::mutex > lk( m );
+                stop = true;
+            }
+            cvWork.notify_all();
+            worker.join();
+        }
+        sessions.clear();
+        scenes.clear();
+        streams.clear();
+        if( setup )
+        {
+            ( void )hipStreamDestroy( setup );
+        }
+    }
+};
+
+namespace
+{
+RgExtServer_T& Srv( RgExtServer s )
+{
+    RG_REQUIRE( s, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "server is null" );
+    return *s;
+}
+} // namespace
+
+extern "C" {
+
+RgResult rgExtServerCreate( const RgExtServerCreateInfo* pInfo, RgExtServer* pOut )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( pInfo && pOut, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        RgExtServerCreateInfo ci = *pInfo;
+        ci.streamCount             = ci.streamCount ? ci.streamCount : 4u;
+        ci.framesInFlightPerStream = ci.framesInFlightPerStream ? ci.framesInFlightPerStream : 2u;
+        RG_REQUIRE( ci.streamCount <= 16 && ci.framesInFlightPerStream <= 8, RG_RESULT_WRONG_FUNCTION_ARGUMENT,
+                    "streamCount must be <= 16 and framesInFlightPerStream <= 8" );
+        RG_REQUIRE( ( ci.precisionFlags & ~( RG_EXT_PRECISION_FP16_STORAGE | RG_EXT_PRECISION_FP16_NODES ) ) == 0,
+                    RG_RESULT_WRONG_FUNCTION_ARGUMENT, "unknown precisionFlags bits" );
+        auto srv        = std::make_unique< RgExtServer_T >();
+        srv->info       = ci;
+        srv->storage    = ( ci.precisionFlags & RG_EXT_PRECISION_FP16_STORAGE ) ? RG_EXT_STORAGE_FP16 : RG_EXT_STORAGE_FP32;
+        srv->nodeFormat = ( ci.precisionFlags & RG_EXT_PRECISION_FP16_NODES ) ? RG_EXT_BVH_NODE_FP16 : RG_EXT_BVH_NODE_FP32;
+        RG_HIP_CHECK( hipStreamCreateWithFlags( &srv->setup, hipStreamNonBlocking ) );
+        const size_t capPx = size_t( ci.maxRenderWidth ) * ci.maxRenderHeight;
+        for( uint32_t i = 0; i < ci.streamCount; i++ )
+        {
+            auto st = std::make_unique< StreamCtx >();
+            RG_HIP_CHECK( hipStreamCreateWithFlags( &st->stream, hipStreamNonBlocking ) );
+            st->planes.Create( ci.maxRenderWidth, ci.maxRenderHeight, srv->storage );
+            st->wf.Reserve( capPx );
+            st->events.resize( ci.framesInFlightPerStream );
+            for( auto& e : st->events )
+            {
+                RG_HIP_CHECK( hipEventCreateWithFlags( &e, hipEventDisableTiming ) );
+            }
+            srv->sharedBytes += FramebufferSet::BytesFor( ci.maxRenderWidth, ci.maxRenderHeight, srv->storage, false ) -
+                                FramebufferSet::BytesFor( ci.maxRenderWidth, ci.maxRenderHeight, srv->storage, true ) +
+                                Wavefront::BytesFor( capPx );
+            srv->streams.push_back( std::move( st ) );
+        }
+        RgExtServer_T* raw = srv.get();
+        srv->worker     = std::thread( [ raw ] { raw->Run(); } );
+        *pOut           = srv.release();
+    } );
+}
+
+RgResult rgExtServerDestroy( RgExtServer server )
+{
+    return Guard( [ & ] {
+        Srv( server ).WaitIdle();
+        delete server;
+    } );
+}
+
+RgResult rgExtSceneCreate( RgExtServer server, RgExtScene* pOut )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( pOut, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "pOut is null" );
+        RgExtServer_T& s  = Srv( server );
+        auto        sc = std::make_unique< RgExtScene_T >();
+        sc->server     = &s;
+        sc->sc.BeginFrame();
+        RgExtScene out = sc.get();
+        std::lock_guard< std::mutex > lk( s.m );
+        s.scenes.push_back( std::move( sc ) );
+        *pOut = out;
+    } );
+}
+
+RgResult rgExtSceneUploadMeshPrimitive( RgExtScene scene, const RgMeshInfo* pMesh, const RgMeshPrimitiveInfo* pPrimitive )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( scene && pMesh && pPrimitive, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        RG_REQUIRE( !scene->built, RG_RESULT_WRONG_FUNCTION_CALL, "the scene is already built" );
+        scene->sc.AddPrimitive( *pMesh, *pPrimitive );
+    } );
+}
+
+RgResult rgExtSceneUploadDirectionalLight( RgExtScene scene, const RgDirectionalLightUploadInfo* pInfo )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( scene && pInfo, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        RG_REQUIRE( !scene->built, RG_RESULT_WRONG_FUNCTION_CALL, "the scene is already built" );
+        scene->sc.AddDirectional( *pInfo );
+    } );
+}
+
+RgResult rgExtSceneUploadSphericalLight( RgExtScene scene, const RgSphericalLightUploadInfo* pInfo )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( scene && pInfo, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        RG_REQUIRE( !scene->built, RG_RESULT_WRONG_FUNCTION_CALL, "the scene is already built" );
+        scene->sc.AddSphere( *pInfo );
+    } );
+}
+
+RgResult rgExtSceneBuild( RgExtScene scene )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( scene, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "scene is null" );
+        RG_REQUIRE( !scene->built, RG_RESULT_WRONG_FUNCTION_CALL, "the scene is already built" );
+        RgExtServer_T&     s  = *scene->server;
+        SceneCollector& sc = scene->sc;
+        // A static scene is its own previous frame: every light and geometry
+        // maps to itself (the steady state of an instance that re-uploads the
+        // same scene each frame).
+        sc.prevLightIndex.clear();
+        for( uint32_t i = 0; i < sc.lights.size(); i++ )
+        {
+            if( sc.lights[ i ].lightType != LIGHT_TYPE_NONE )
+            {
+                sc.prevLightIndex[ sc.lightIds[ i ] ] = i;
+            }
+        }
+        sc.prevLightCount = uint32_t( sc.lights.size() );
+        sc.FinishLights();
+        sc.prevGeomUids = sc.geomUids;
+        sc.GeomPrevToCur( scene->geomMap );
+        UploadScene( sc, scene->ds, s.setup );
+        AssembleScene( scene->ds, s.setup );
+        if( scene->ds.triCount > 0 )
+        {
+            RgExtBvhBuildParams bp = { 1, s.nodeFormat };
+            BuildBvh( scene->bvh, scene->ds.tris.as< float >(), scene->ds.triCount, bp, s.setup );
+            scene->ds.bvhValid      = true;
+            scene->ds.builtTriCount = scene->ds.triCount;
+        }
+        RG_HIP_CHECK( hipStreamSynchronize( s.setup ) );
+        sc.vertices.Free();
+        sc.indices.Free();
+        const size_t n = scene->ds.triCount;
+        scene->bytes   = n * ( 9 + 9 + 1 + 9 ) * 4 + size_t( scene->bvh.nodeCount ) * scene->bvh.NodeStride() + n * 48;
+        scene->built   = true;
+        std::lock_guard< std::mutex > lk( s.m );
+        s.sceneBytes += scene->bytes;
+    } );
+}
+
+RgResult rgExtSceneDestroy( RgExtScene scene )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( scene, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "scene is null" );
+        RgExtServer_T&                   s = *scene->server;
+        std::unique_ptr< RgExtScene_T >  keep;
+        {
+            std::lock_guard< std::mutex > lk( s.m );
+            RG_REQUIRE( scene->sessions == 0, RG_RESULT_WRONG_FUNCTION_CALL, "sessions still use the scene" );
+            for( auto it = s.scenes.begin(); it != s.scenes.end(); ++it )
+            {
+                if( it->get() == scene )
+                {
+                    keep = std::move( *it );
+                    s.scenes.erase( it );
+                    break;
+                }
+            }
+            s.sceneBytes -= scene->bytes;
+        }
+    } );
+}
+
+RgResult rgExtSessionCreate( RgExtScene scene, uint32_t width, uint32_t height, RgExtSession* pOut )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( scene && pOut, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        RG_REQUIRE( scene->built, RG_RESULT_WRONG_FUNCTION_CALL, "the scene is not built" );
+        RgExtServer_T& s = *scene->server;
+        RG_REQUIRE( width <= s.info.maxRenderWidth && height <= s.info.maxRenderHeight, RG_RESULT_WRONG_FUNCTION_ARGUMENT,
+                    "render size exceeds the server's maximum" );
+        const size_t bytes = FramebufferSet::BytesFor( width, height, s.storage, true ) + FrameGpu::OwnBytes();
+        uint32_t     streamIdx;
+        {
+            std::lock_guard< std::mutex > lk( s.m );
+            if( s.info.memoryBudgetBytes && s.sessionBytes + bytes > s.info.memoryBudgetBytes )
+            {
+                throw Error( RG_EXT_RESULT_OVER_BUDGET, "the session does not fit the memory budget" );
+            }
+            s.sessionBytes += bytes; // reserved while the session is set up
+            scene->sessions++;
+            streamIdx = s.roundRobin++ % uint32_t( s.streams.size() );
+        }
+        try
+        {
+            auto ss    = std::make_unique< RgExtSession_T >();
+            ss->server = &s;
+            ss->scene  = scene;
+            ss->width  = width;
+            ss->height = height;
+            ss->stream = streamIdx;
+            ss->bytes  = bytes;
+            StreamCtx& st = *s.streams[ streamIdx ];
+            ss->fbs.fb.Create( width, height, s.storage, &st.planes );
+            ss->g.pt.wf = &st.wf;
+            ss->g.Init( s.setup );
+            ss->g.pt.UploadLights( scene->sc, s.setup );
+            ss->g.pt.UploadGeomMap( scene->geomMap, s.setup );
+            RG_HIP_CHECK( hipStreamSynchronize( s.setup ) );
+            RgExtSession out = ss.get();
+            std::lock_guard< std::mutex > lk( s.m );
+            s.sessions.push_back( std::move( ss ) );
+            *pOut = out;
+        }
+        catch( ... )
+        {
+            std::lock_guard< std::mutex > lk( s.m );
+            s.sessionBytes -= bytes;
+            scene->sessions--;
+            throw;
+        }
+    } );
+}
+
+RgResult rgExtSessionDestroy( RgExtSession session )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( session, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "session is null" );
+        RgExtServer_T&                    s = *session->server;
+        std::unique_ptr< RgExtSession_T > keep;
+        {
+            std::unique_lock< std::mutex > lk( s.m );
+            session->dying = true;
+            if( session->pending )
+            {
+                s.Result( session, session->pending->tag, RG_EXT_FRAME_DROPPED, 0, session->pending->submit, 0.0, NowSeconds() );
+                session->pending.reset();
+            }
+            for( auto it = s.ready.begin(); it != s.ready.end(); ++it )
+            {
+                if( *it == session )
+                {
+                    s.ready.erase( it );
+                    break;
+                }
+            }
+            s.cvIdle.wait( lk, [ & ] { return session->inflight == 0; } );
+            for( auto it = s.sessions.begin(); it != s.sessions.end(); ++it )
+            {
+                if( it->get() == session )
+                {
+                    keep = std::move( *it );
+                    s.sessions.erase( it );
+                    break;
+                }
+            }
+            s.sessionBytes -= session->bytes;
+            session->scene->sessions--;
+        }
+        // results still naming the session stay valid as opaque handles
+    } );
+}
+
+RgResult rgExtSessionSubmit( RgExtSession session, const RgDrawFrameInfo* pInfo, uint64_t tag )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( session && pInfo, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        FrameRequest r;
+        r.Capture( *pInfo );
+        r.tag                            = tag;
+        r.submit                         = NowSeconds();
+        RgExtServer_T&                   s  = *session->server;
+        std::lock_guard< std::mutex > lk( s.m );
+        RG_REQUIRE( !session->dying, RG_RESULT_WRONG_FUNCTION_CALL, "the session is being destroyed" );
+        if( session->pending )
+        {
+            s.Result( session, session->pending->tag, RG_EXT_FRAME_DROPPED, 0, session->pending->submit, 0.0, r.submit );
+        }
+        session->pending = std::move( r );
+        if( !session->queued )
+        {
+            s.ready.push_back( session );
+            session->queued = true;
+        }
+        s.cvWork.notify_one();
+    } );
+}
+
+RgResult rgExtServerPoll( RgExtServer server, RgExtFrameResult* pResults, uint32_t maxCount, uint32_t* pCount )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( pCount && ( pResults || maxCount == 0 ), RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        RgExtServer_T&                   s = Srv( server );
+        std::lock_guard< std::mutex > lk( s.m );
+        uint32_t                      n = 0;
+        while( n < maxCount && !s.results.empty() )
+        {
+            pResults[ n++ ] = s.results.front();
+            s.results.pop_front();
+        }
+        *pCount = n;
+    } );
+}
+
+RgResult rgExtServerWaitIdle( RgExtServer server )
+{
+    return Guard( [ & ] { Srv( server ).WaitIdle(); } );
+}
+
+RgResult rgExtServerGetStats( RgExtServer server, RgExtServerStats* pOut )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( pOut, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "pOut is null" );
+        RgExtServer_T&                   s = Srv( server );
+        std::lock_guard< std::mutex > lk( s.m );
+        pOut->sessions      = uint32_t( s.sessions.size() );
+        pOut->streams       = uint32_t( s.streams.size() );
+        pOut->sessionBytes  = s.sessionBytes;
+        pOut->sharedBytes   = s.sharedBytes;
+        pOut->sceneBytes    = s.sceneBytes;
+        pOut->framesDone    = s.done;
+        pOut->framesDropped = s.dropped;
+        pOut->framesFailed  = s.failed;
+        pOut->now           = NowSeconds();
+    } );
+}
+
+RgResult rgExtSessionGetFramebuffers( RgExtSession session, RgExtFramebuffers* pOut )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( session && pOut, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        *pOut = &session->fbs;
+    } );
+}
+
+} // extern "C"

@@ -148,6 +148,9 @@ _API = {
     "rgExtReadbackHistogram": (C.c_int, [_P, _P]),
     "rgExtGetStream": (C.c_int, [_P, C.POINTER(_P)]),
     "rgExtSetStaticGeometry": (C.c_int, [_P, _U32]),
+    "rgExtSceneUploadMeshPrimitive": (C.c_int, [_P, C.POINTER(RgMeshInfo), C.POINTER(RgMeshPrimitiveInfo)]),
+    "rgExtSceneUploadDirectionalLight": (C.c_int, [_P, C.POINTER(RgDirectionalLightUploadInfo)]),
+    "rgExtSceneUploadSphericalLight": (C.c_int, [_P, C.POINTER(RgSphericalLightUploadInfo)]),
 }
 _bound = False
 
@@ -179,6 +182,20 @@ def _style(info, light: dict) -> None:
         info.extra.lightstyle = int(light["lightstyle"])
 
 
+def draw_info(cam, current_time: float, params_ptr) -> RgDrawFrameInfo:
+    """RgDrawFrameInfo for a camera (world culling mask 0, ray length 10000)."""
+    di = RgDrawFrameInfo()
+    di.view[:] = cam.view().tolist()
+    di.fovYRadians = cam.fovy
+    di.cameraNear = cam.znear
+    di.cameraFar = cam.zfar
+    di.rayLength = 10000.0
+    di.rayCullMaskWorld = RG_DRAW_FRAME_RAY_CULL_WORLD_0_BIT
+    di.currentTime = current_time
+    di.pParams = params_ptr
+    return di
+
+
 class SceneUploader:
     """Prebuilt per-object vertex arrays of a scenes.Scene, uploaded each frame
     with rgUploadMeshPrimitive (non-indexed triangle lists, identity transform)."""
@@ -196,9 +213,14 @@ class SceneUploader:
         self.mesh.transform = identity_transform()
         self.prim = RgMeshPrimitiveInfo()
 
-    def upload(self, inst, geometry: bool = True) -> None:
-        """Uploads the primitives (unless geometry is False) and the lights."""
+    def upload(self, inst, geometry: bool = True, prefix: str = "rg") -> None:
+        """Uploads the primitives (unless geometry is False) and the lights
+        to an instance (prefix "rg": rgUploadMeshPrimitive, ...) or to a
+        server scene (prefix "rgExtScene": rgExtSceneUploadMeshPrimitive, ...)."""
         l = api()
+        mesh_fn = getattr(l, prefix + "UploadMeshPrimitive")
+        dir_fn = getattr(l, prefix + "UploadDirectionalLight")
+        sph_fn = getattr(l, prefix + "UploadSphericalLight")
         for oid, v, color in (self.objects if geometry else []):
             self.mesh.uniqueObjectID = oid
             self.prim.primitiveIndexInMesh = 0
@@ -208,7 +230,7 @@ class SceneUploader:
             self.prim.indexCount = 0
             self.prim.color = color | 0xFF000000
             self.prim.emissive = 0.0
-            ffi.check("rgUploadMeshPrimitive", l.rgUploadMeshPrimitive(inst, C.byref(self.mesh), C.byref(self.prim)))
+            ffi.check(prefix + "UploadMeshPrimitive", mesh_fn(inst, C.byref(self.mesh), C.byref(self.prim)))
         lights = self.scene.lights or {}
         sun = lights.get("sun")
         if sun:
@@ -219,7 +241,7 @@ class SceneUploader:
             d.direction.data[:] = sun["direction"]
             d.angularDiameterDegrees = sun["angularDiameterDegrees"]
             _style(d, sun)
-            ffi.check("rgUploadDirectionalLight", l.rgUploadDirectionalLight(inst, C.byref(d)))
+            ffi.check(prefix + "UploadDirectionalLight", dir_fn(inst, C.byref(d)))
         for i, sp in enumerate(lights.get("spheres", [])):
             s = RgSphericalLightUploadInfo()
             s.uniqueID = 100 + i
@@ -228,7 +250,7 @@ class SceneUploader:
             s.position.data[:] = sp["position"]
             s.radius = sp["radius"]
             _style(s, sp)
-            ffi.check("rgUploadSphericalLight", l.rgUploadSphericalLight(inst, C.byref(s)))
+            ffi.check(prefix + "UploadSphericalLight", sph_fn(inst, C.byref(s)))
 
 
 class Renderer:
@@ -288,16 +310,8 @@ class Renderer:
         sfi = RgStartFrameInfo()
         ffi.check("rgStartFrame", l.rgStartFrame(self.h, C.byref(sfi)))
         uploader.upload(self.h, geometry)
-        di = RgDrawFrameInfo()
-        di.view[:] = cam.view().tolist()
-        di.fovYRadians = cam.fovy
-        di.cameraNear = cam.znear
-        di.cameraFar = cam.zfar
-        di.rayLength = 10000.0
-        di.rayCullMaskWorld = RG_DRAW_FRAME_RAY_CULL_WORLD_0_BIT
         self.time += dt
-        di.currentTime = self.time
-        di.pParams = C.cast(C.pointer(self.sky), _P)
+        di = draw_info(cam, self.time, C.cast(C.pointer(self.sky), _P))
         ffi.check("rgDrawFrame", l.rgDrawFrame(self.h, C.byref(di)))
         return self.stats()
 
