@@ -70,7 +70,8 @@ RGAPI RgResult RGCONV rgExtTimerElapsedMs( RgExtTimer timer, float* pMs );
 typedef struct RgExtBvh_T* RgExtBvh;
 
 #define RG_EXT_BVH_NODE_FP32 0u /* 256-byte nodes with FP32 child boxes     */
-#define RG_EXT_BVH_NODE_FP16 1u /* 128-byte nodes, boxes rounded outward to FP16 */
+#define RG_EXT_BVH_NODE_FP16 1u /* 144-byte nodes: FP32 origin, child boxes as FP16
+                                   offsets from it rounded outward */
 
 typedef struct RgExtBvhBuildParams
 {
@@ -89,7 +90,7 @@ typedef struct RgExtBvhView
     uint32_t           triangleCount;
     uint32_t           nodeCount;        /* 8-wide nodes; node 0 is the root  */
     uint32_t           nodeFormat;
-    uint32_t           nodeStrideBytes;  /* 256 (FP32) or 128 (FP16)          */
+    uint32_t           nodeStrideBytes;  /* 256 (FP32) or 144 (FP16)          */
     const void*        d_nodes;
     /* Sorted triangles, 3 float4 each: (v0, primId bits) (v1, 0) (v2, 0). */
     const float*       d_sortedTriangles;

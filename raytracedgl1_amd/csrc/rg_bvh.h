@@ -21,9 +21,16 @@ struct alignas( 256 ) BvhNode8
 };
 static_assert( sizeof( BvhNode8 ) == 256, "node size" );
 
-// 8-wide node with FP16 child boxes rounded outward, 128 bytes.
-struct alignas( 128 ) BvhNode8H
+// 8-wide node with FP16 child boxes, 144 bytes: an FP32 origin (the min
+// corner of the node's own box) and each child box as IEEE-half offsets from
+// it, rounded outward so that origin + float(offset), evaluated in FP32 as
+// the traversal does, contains the FP32 box.  Offsets keep 11 significant
+// bits relative to the node's extent, where absolute FP16 coordinates would
+// lose all precision far from the scene origin.
+struct alignas( 16 ) BvhNode8H
 {
+    float    origin[ 3 ];
+    uint32_t pad;
     uint16_t lo_x[ 8 ];
     uint16_t hi_x[ 8 ];
     uint16_t lo_y[ 8 ];
@@ -32,7 +39,7 @@ struct alignas( 128 ) BvhNode8H
     uint16_t hi_z[ 8 ];
     uint32_t child[ 8 ];
 };
-static_assert( sizeof( BvhNode8H ) == 128, "node size" );
+static_assert( sizeof( BvhNode8H ) == 144, "node size" );
 
 constexpr uint32_t BVH_MAX_LEVELS = 128;
 
@@ -64,6 +71,7 @@ struct BvhObject
     DeviceBuffer frontier[ 2 ];
     DeviceBuffer levelCounts;  // BVH_MAX_LEVELS + 2 u32 (last two: node count, error flag)
     DeviceBuffer nodes;        // nodeCount * stride
+    DeviceBuffer nodes32;      // FP32 nodes an FP16 build converts from
     uint32_t*    hostCounts = nullptr; // pinned
 
     // Index of the sorted-order permutation after the sort (keysA/valsA or B).
@@ -78,7 +86,7 @@ struct BvhObject
         }
     }
 
-    uint32_t NodeStride() const { return nodeFormat == RG_EXT_BVH_NODE_FP16 ? 128u : 256u; }
+    uint32_t NodeStride() const { return nodeFormat == RG_EXT_BVH_NODE_FP16 ? uint32_t( sizeof( BvhNode8H ) ) : uint32_t( sizeof( BvhNode8 ) ); }
 };
 
 void BuildBvh( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtBvhBuildParams& params, hipStream_t stream );
@@ -91,7 +99,8 @@ struct TraceContext
     DeviceBuffer counter;
     DeviceBuffer overflow;
     DeviceBuffer stats; // 2 x u64: node visits, triangle tests of the last collecting launch
-    uint32_t     gridBlocks = 0;
+    uint32_t     gridBlocks = 0;     // resident blocks of the FP32-node kernel
+    uint32_t     gridBlocksHalf = 0; // ... of the FP16-node kernel
 };
 
 // Device accumulators a caller clears once and lets several launches add to.

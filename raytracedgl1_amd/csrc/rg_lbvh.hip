@@ -516,6 +516,92 @@ __global__ __launch_bounds__( THREADS ) void k_write_nodes( uint32_t        slot
     }
 }
 
+// Outward rounding to half: the half nearest to v - origin, stepped down
+// (lo) or up (hi) until origin + float(h), in FP32, is on the outside of v.
+__device__ __forceinline__ uint16_t HalfBitsOf( _Float16 h )
+{
+    return __builtin_bit_cast( uint16_t, h );
+}
+__device__ __forceinline__ _Float16 HalfOfBits( uint16_t b )
+{
+    return __builtin_bit_cast( _Float16, b );
+}
+__device__ _Float16 HalfStep( _Float16 h, bool up )
+{
+    const uint16_t b = HalfBitsOf( h );
+    if( ( b & 0x7C00u ) == 0x7C00u )
+    {
+        return h; // inf (or NaN) stays
+    }
+    if( ( b & 0x7FFFu ) == 0u )
+    {
+        return HalfOfBits( up ? 0x0001u : 0x8001u ); // +-0 -> smallest subnormal
+    }
+    const bool neg = ( b & 0x8000u ) != 0u;
+    return HalfOfBits( uint16_t( ( neg != up ) ? b + 1u : b - 1u ) );
+}
+__device__ uint16_t EncodeOffset( float v, float origin, bool up )
+{
+    _Float16 h = _Float16( v - origin );
+    if( !up && HalfBitsOf( h ) == 0x7C00u )
+    {
+        h = HalfOfBits( 0x7BFFu ); // beyond the half range: the largest finite offset is still below v
+    }
+    for( int i = 0; i < 64; i++ )
+    {
+        const float d = origin + float( h );
+        if( up ? !( d < v ) : !( d > v ) )
+        {
+            break;
+        }
+        h = HalfStep( h, up );
+    }
+    return HalfBitsOf( h );
+}
+
+__global__ __launch_bounds__( THREADS ) void k_half_nodes( const BvhNode8* in, uint32_t count, BvhNode8H* out )
+{
+    const uint32_t i = blockIdx.x * THREADS + threadIdx.x;
+    if( i >= count )
+    {
+        return;
+    }
+    const BvhNode8& a = in[ i ];
+    BvhNode8H       h;
+    float           o[ 3 ] = { INFINITY, INFINITY, INFINITY };
+    for( int k = 0; k < 8; k++ )
+    {
+        if( a.child[ k ] != RG_EXT_BVH_CHILD_EMPTY )
+        {
+            o[ 0 ] = fminf( o[ 0 ], a.lo_x[ k ] );
+            o[ 1 ] = fminf( o[ 1 ], a.lo_y[ k ] );
+            o[ 2 ] = fminf( o[ 2 ], a.lo_z[ k ] );
+        }
+    }
+    for( int c = 0; c < 3; c++ )
+    {
+        h.origin[ c ] = o[ c ] < INFINITY ? o[ c ] : 0.0f;
+    }
+    h.pad = 0;
+    for( int k = 0; k < 8; k++ )
+    {
+        h.child[ k ] = a.child[ k ];
+        if( a.child[ k ] == RG_EXT_BVH_CHILD_EMPTY )
+        {
+            h.lo_x[ k ] = h.lo_y[ k ] = h.lo_z[ k ] = 0x7C00u; // +inf
+            h.hi_x[ k ] = h.hi_y[ k ] = h.hi_z[ k ] = 0xFC00u; // -inf
+            continue;
+        }
+        h.lo_x[ k ] = EncodeOffset( a.lo_x[ k ], h.origin[ 0 ], false );
+        h.lo_y[ k ] = EncodeOffset( a.lo_y[ k ], h.origin[ 1 ], false );
+        h.lo_z[ k ] = EncodeOffset( a.lo_z[ k ], h.origin[ 2 ], false );
+        h.hi_x[ k ] = EncodeOffset( a.hi_x[ k ], h.origin[ 0 ], true );
+        h.hi_y[ k ] = EncodeOffset( a.hi_y[ k ], h.origin[ 1 ], true );
+        h.hi_z[ k ] = EncodeOffset( a.hi_z[ k ], h.origin[ 2 ], true );
+    }
+    out[ i ] = h;
+}
+
 __global__ __launch_bounds__( THREADS ) void k_sorted_tris( const float* tris, const uint32_t* sortedIdx, uint32_t n, float4* out )
 {
     const uint32_t k = blockIdx.x * THREADS + threadIdx.x;
@@ -544,7 +630,8 @@ void BuildBvh( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtBvhBu
     RG_REQUIRE( n > 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "triangle count must be > 0" );
     RG_REQUIRE( n < ( 1u << 28 ), RG_RESULT_WRONG_FUNCTION_ARGUMENT, "triangle count must be < 2^28" );
     RG_REQUIRE( params.maxLeafSize >= 1 && params.maxLeafSize <= 8, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "maxLeafSize must be in [1, 8]" );
-    RG_REQUIRE( params.nodeFormat == RG_EXT_BVH_NODE_FP32, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "only FP32 nodes are built by this version" );
+    RG_REQUIRE( params.nodeFormat == RG_EXT_BVH_NODE_FP32 || params.nodeFormat == RG_EXT_BVH_NODE_FP16, RG_RESULT_WRONG_FUNCTION_ARGUMENT,
+                "unknown node format" );
 
     bvh.n          = n;
     bvh.leafMax    = params.maxLeafSize;
@@ -675,9 +762,17 @@ void BuildBvh( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtBvhBu
     bvh.nodeCount = bvh.hostCounts[ 0 ];
     RG_REQUIRE( bvh.nodeCount >= 1, RG_RESULT_INTERNAL_ERROR, "BVH collapse produced no nodes" );
 
-    bvh.nodes.Reserve( size_t( bvh.nodeCount ) * sizeof( BvhNode8 ) );
-    k_write_nodes<<< DivUp( slots, THREADS ), THREADS, 0, stream >>>( slots, n, bvh.leafMax, bvh.isNode8.as< uint32_t >(), bvh.node8Index.as< uint32_t >(), bvh.expansion.as< uint32_t >(), rf, rl, bounds, bvh.nodes.as< BvhNode8 >() );
+    const bool    half    = params.nodeFormat == RG_EXT_BVH_NODE_FP16;
+    DeviceBuffer& fp32Out = half ? bvh.nodes32 : bvh.nodes;
+    fp32Out.Reserve( size_t( bvh.nodeCount ) * sizeof( BvhNode8 ) );
+    k_write_nodes<<< DivUp( slots, THREADS ), THREADS, 0, stream >>>( slots, n, bvh.leafMax, bvh.isNode8.as< uint32_t >(), bvh.node8Index.as< uint32_t >(), bvh.expansion.as< uint32_t >(), rf, rl, bounds, fp32Out.as< BvhNode8 >() );
     CheckLaunch( "k_write_nodes" );
+    if( half )
+    {
+        bvh.nodes.Reserve( size_t( bvh.nodeCount ) * sizeof( BvhNode8H ) );
+        k_half_nodes<<< DivUp( bvh.nodeCount, THREADS ), THREADS, 0, stream >>>( bvh.nodes32.as< BvhNode8 >(), bvh.nodeCount, bvh.nodes.as< BvhNode8H >() );
+        CheckLaunch( "k_half_nodes" );
+    }
     k_sorted_tris<<< DivUp( n, THREADS ), THREADS, 0, stream >>>( d_tris, vin, n, bvh.sortedTris.as< float4 >() );
     CheckLaunch( "k_sorted_tris" );
 

@@ -35,7 +35,7 @@ constexpr uint32_t COUNTER_STRIDE    = 64;  // u32 between counters (256 B)
 
 struct TraceArgs
 {
-    const BvhNode8* nodes;
+    const void*     nodes; // BvhNode8 or BvhNode8H
     const float4*   tris;
     const RgExtRay* rays;
     RgExtHit*       hits;
@@ -80,6 +80,41 @@ __device__ __forceinline__ void Sort8( uint64_t k[ 8 ] )
 #undef CS
 }
 
+// Child boxes of an FP16 node as FP32 (rows lo x, y, z, hi x, y, z) plus
+// the child references: origin + float(offset), the sum its build rounded
+// outward against.  9 16-byte loads.
+__device__ __forceinline__ void LoadHalfNode( const void* nodes, uint32_t node, float box[ 6 ][ 8 ], uint32_t child[ 8 ] )
+{
+    typedef _Float16 half8 __attribute__( ( ext_vector_type( 8 ) ) );
+    const float4* np = reinterpret_cast< const float4* >( reinterpret_cast< const BvhNode8H* >( nodes ) + node );
+    float4        q[ 9 ];
+#pragma unroll
+    for( int i = 0; i < 9; i++ )
+    {
+        q[ i ] = np[ i ];
+    }
+    const float o[ 3 ] = { q[ 0 ].x, q[ 0 ].y, q[ 0 ].z };
+    // q[1..6]: lo_x hi_x lo_y hi_y lo_z hi_z, 8 halves each
+    const int src[ 6 ] = { 1, 3, 5, 2, 4, 6 };
+#pragma unroll
+    for( int r = 0; r < 6; r++ )
+    {
+        const half8 hv = __builtin_bit_cast( half8, q[ src[ r ] ] );
+#pragma unroll
+        for( int k = 0; k < 8; k++ )
+        {
+            box[ r ][ k ] = o[ r % 3 ] + float( hv[ k ] );
+        }
+    }
+    const uint32_t* c = reinterpret_cast< const uint32_t* >( &q[ 7 ] );
+#pragma unroll
+    for( int k = 0; k < 8; k++ )
+    {
+        child[ k ] = c[ k ];
+    }
+}
+
+template< bool H >
 __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
 {
     __shared__ uint32_t s_stack[ STACK_LDS * TR_THREADS ];
@@ -211,27 +246,47 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
         else if( ( node & RG_EXT_BVH_CHILD_LEAF_BIT ) == 0u )
         {
             // Interior node: slab-test the 8 children, visit near to far.
-            // The node is fetched as 14 16-byte loads (boxes + child refs).
+            // An FP32 node is 14 16-byte loads (boxes + child refs), an FP16
+            // node 9.
             nodeVisits++;
-            const float4* np = reinterpret_cast< const float4* >( a.nodes + node );
-            float4        q[ 14 ];
-#pragma unroll
-            for( int i = 0; i < 14; i++ )
+            uint64_t key[ 8 ];
+            int      hitCount = 0;
+            if constexpr( !H )
             {
-                q[ i ] = np[ i ];
+                const float4* np = reinterpret_cast< const float4* >( reinterpret_cast< const BvhNode8* >( a.nodes ) + node );
+                float4        q[ 14 ];
+#pragma unroll
+                for( int i = 0; i < 14; i++ )
+                {
+                    q[ i ] = np[ i ];
+                }
+                const float* f = reinterpret_cast< const float* >( q ); // lo_x hi_x lo_y hi_y lo_z hi_z child
+#pragma unroll
+                for( int k = 0; k < 8; k++ )
+                {
+                    const uint32_t c = __float_as_uint( f[ 48 + k ] );
+                    float          tn;
+                    const bool     inBox = SlabTest( ray, f[ k ], f[ 16 + k ], f[ 32 + k ], f[ 8 + k ], f[ 24 + k ], f[ 40 + k ], tmin, bestT, &tn );
+                    const bool     h     = inBox && c != RG_EXT_BVH_CHILD_EMPTY;
+                    key[ k ] = h ? SortKey( tn, c ) : ~0ull;
+                    hitCount += h ? 1 : 0;
+                }
             }
-            const float* f = reinterpret_cast< const float* >( q ); // lo_x hi_x lo_y hi_y lo_z hi_z child
-            uint64_t     key[ 8 ];
-            int          hitCount = 0;
-#pragma unroll
-            for( int k = 0; k < 8; k++ )
+            else
             {
-                const uint32_t c = __float_as_uint( f[ 48 + k ] );
-                float          tn;
-                const bool     inBox = SlabTest( ray, f[ k ], f[ 16 + k ], f[ 32 + k ], f[ 8 + k ], f[ 24 + k ], f[ 40 + k ], tmin, bestT, &tn );
-                const bool     h     = inBox && c != RG_EXT_BVH_CHILD_EMPTY;
-                key[ k ] = h ? SortKey( tn, c ) : ~0ull;
-                hitCount += h ? 1 : 0;
+                float    box[ 6 ][ 8 ];
+                uint32_t child[ 8 ];
+                LoadHalfNode( a.nodes, node, box, child );
+#pragma unroll
+                for( int k = 0; k < 8; k++ )
+                {
+                    const uint32_t c = child[ k ];
+                    float          tn;
+                    const bool     inBox = SlabTest( ray, box[ 0 ][ k ], box[ 1 ][ k ], box[ 2 ][ k ], box[ 3 ][ k ], box[ 4 ][ k ], box[ 5 ][ k ], tmin, bestT, &tn );
+                    const bool     h     = inBox && c != RG_EXT_BVH_CHILD_EMPTY;
+                    key[ k ] = h ? SortKey( tn, c ) : ~0ull;
+                    hitCount += h ? 1 : 0;
+                }
             }
             if( hitCount > 0 )
             {
@@ -357,7 +412,8 @@ void TraceRays( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, u
                 hipEvent_t evBefore, hipEvent_t evAfter, const TraceAccum* accum )
 {
     RG_REQUIRE( bvh.nodeCount > 0, RG_RESULT_WRONG_FUNCTION_CALL, "BVH is not built" );
-    RG_REQUIRE( bvh.nodeFormat == RG_EXT_BVH_NODE_FP32, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "unsupported node format" );
+    RG_REQUIRE( bvh.nodeFormat == RG_EXT_BVH_NODE_FP32 || bvh.nodeFormat == RG_EXT_BVH_NODE_FP16, RG_RESULT_WRONG_FUNCTION_ARGUMENT,
+                "unsupported node format" );
     // the chunk counter runs up to count + (waves + 1) * RAY_CHUNK without wrapping
     RG_REQUIRE( count <= 0xF0000000u, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "too many rays in one launch" );
     if( count == 0 )
@@ -366,20 +422,26 @@ void TraceRays( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, u
     }
     if( s.gridBlocks == 0 )
     {
-        int dev = 0, cus = 0, perCu = 0;
+        int dev = 0, cus = 0, perCu = 0, perCuHalf = 0;
         RG_HIP_CHECK( hipGetDevice( &dev ) );
         RG_HIP_CHECK( hipDeviceGetAttribute( &cus, hipDeviceAttributeMultiprocessorCount, dev ) );
-        RG_HIP_CHECK( hipOccupancyMaxActiveBlocksPerMultiprocessor( &perCu, k_trace, TR_THREADS, 0 ) );
-        s.gridBlocks = uint32_t( cus * ( perCu > 0 ? perCu : 1 ) );
+        RG_HIP_CHECK( hipOccupancyMaxActiveBlocksPerMultiprocessor( &perCu, k_trace< false >, TR_THREADS, 0 ) );
+        RG_HIP_CHECK( hipOccupancyMaxActiveBlocksPerMultiprocessor( &perCuHalf, k_trace< true >, TR_THREADS, 0 ) );
+        s.gridBlocks     = uint32_t( cus * ( perCu > 0 ? perCu : 1 ) );
+        s.gridBlocksHalf = uint32_t( cus * ( perCuHalf > 0 ? perCuHalf : 1 ) );
     }
-    const uint32_t blocks = DivUp( count, TR_THREADS ) < s.gridBlocks ? DivUp( count, TR_THREADS ) : s.gridBlocks;
+    const bool     half   = bvh.nodeFormat == RG_EXT_BVH_NODE_FP16;
+    const uint32_t grid   = half ? s.gridBlocksHalf : s.gridBlocks;
+    const uint32_t blocks = DivUp( count, TR_THREADS ) < grid ? DivUp( count, TR_THREADS ) : grid;
     s.counter.Reserve( ( RAY_PARTITIONS * COUNTER_STRIDE + 1 ) * sizeof( uint32_t ) );
     s.stats.Reserve( 2 * sizeof( uint64_t ) );
-    s.overflow.Reserve( size_t( s.gridBlocks ) * TR_THREADS * STACK_GLOBAL * sizeof( uint32_t ) );
+    // one overflow stack per thread of the larger of the two grids
+    const uint32_t maxGrid = s.gridBlocks > s.gridBlocksHalf ? s.gridBlocks : s.gridBlocksHalf;
+    s.overflow.Reserve( size_t( maxGrid ) * TR_THREADS * STACK_GLOBAL * sizeof( uint32_t ) );
     RG_HIP_CHECK( hipMemsetAsync( s.counter.ptr, 0, ( RAY_PARTITIONS * COUNTER_STRIDE + 1 ) * sizeof( uint32_t ), stream ) );
 
     TraceArgs a;
-    a.nodes         = reinterpret_cast< const BvhNode8* >( bvh.nodes.ptr );
+    a.nodes         = bvh.nodes.ptr;
     a.tris          = reinterpret_cast< const float4* >( bvh.sortedTris.ptr );
     a.rays          = d_rays;
     a.hits          = d_hits;
@@ -405,7 +467,14 @@ void TraceRays( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, u
     {
         RG_HIP_CHECK( hipEventRecord( evBefore, stream ) );
     }
-    k_trace<<< blocks, TR_THREADS, 0, stream >>>( a );
+    if( half )
+    {
+        k_trace< true ><<< blocks, TR_THREADS, 0, stream >>>( a );
+    }
+    else
+    {
+        k_trace< false ><<< blocks, TR_THREADS, 0, stream >>>( a );
+    }
     CheckLaunch( "k_trace" );
     if( evAfter )
     {

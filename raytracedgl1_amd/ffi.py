@@ -40,7 +40,11 @@ RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("direction", "<f4"
 HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("prim", "<u4")])
 NODE8_DTYPE = np.dtype([("lo_x", "<f4", 8), ("hi_x", "<f4", 8), ("lo_y", "<f4", 8), ("hi_y", "<f4", 8),
                         ("lo_z", "<f4", 8), ("hi_z", "<f4", 8), ("child", "<u4", 8), ("pad", "<u4", 8)])
+NODE8H_DTYPE = np.dtype([("origin", "<f4", 3), ("pad", "<u4"), ("lo_x", "<u2", 8), ("hi_x", "<u2", 8),
+                         ("lo_y", "<u2", 8), ("hi_y", "<u2", 8), ("lo_z", "<u2", 8), ("hi_z", "<u2", 8),
+                         ("child", "<u4", 8)])
 assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16 and NODE8_DTYPE.itemsize == 256
+assert NODE8H_DTYPE.itemsize == 144
 
 
 class RgExtBvhBuildParams(C.Structure):
@@ -290,7 +294,8 @@ class Bvh:
         out = {
             "n": n,
             "node_count": m,
-            "nodes": download_raw(v.d_nodes, m * 256, NODE8_DTYPE),
+            "nodes": download_raw(v.d_nodes, m * v.nodeStrideBytes,
+                                  NODE8H_DTYPE if v.nodeFormat == RG_EXT_BVH_NODE_FP16 else NODE8_DTYPE),
             "sorted_tris": download_raw(v.d_sortedTriangles, n * 48, np.float32, (n, 12)),
             "codes": download_raw(v.d_mortonCodes, n * 8, np.uint64),
             "sorted_idx": download_raw(v.d_sortedIndices, n * 4, np.uint32),

@@ -33,12 +33,13 @@ def boxes():
     return s, rtgl.SceneUploader(s)
 
 
-def _render(uploader, cams, width=W1, height=H1, static=False, collect=False):
+def _render(uploader, cams, width=W1, height=H1, static=False, collect=False, precision=0):
     r = rtgl.Renderer(width, height)
     try:
         if static:
             r.set_static_geometry(True)
         r.path.collectTraversalStats = 1 if collect else 0
+        r.path.precisionFlags = precision
         stats = [r.frame(uploader, c, geometry=(not static) or i == 0) for i, c in enumerate(cams)]
         fb = r.framebuffers()
         out = {name: fb.download(PLANE[name]) for name in ("PRE_FINAL", "VISIBILITY_BUFFER", "UNFILTERED_DIRECT",
@@ -300,3 +301,25 @@ def test_lightstyles_scale_lights_one_frame_late(boxes):
     out_of_range = run([dict(sun, lightstyle=7)] * 2, [0.5])
     full = run([sun] * 2, None)
     assert np.array_equal(out_of_range[1]["UNFILTERED_DIRECT"], full[1]["UNFILTERED_DIRECT"])
+
+
+def test_fp16_precision_form(boxes):
+    """FP16 nodes change only the order boxes are visited in, so frames are
+    bit-identical to the FP32 path's; FP16 storage of the denoiser planes
+    rounds the temporal and a-trous buffers, so the denoised image stays
+    within a few 1e-3."""
+    print("api: FP16 nodes / FP16 storage vs FP32, terrain-boxes", flush=True)
+    _, up = boxes
+    cams = [scenes.terrain_boxes_camera()] * 3
+    _, ref = _render(up, cams)
+    _, nodes16 = _render(up, cams, precision=ffi.RG_EXT_PRECISION_FP16_NODES)
+    for k in ("PRE_FINAL", "VISIBILITY_BUFFER", "UNFILTERED_DIRECT", "UNFILTERED_INDIR", "DEPTH_WORLD"):
+        assert np.array_equal(nodes16[k], ref[k]), k
+    _, both = _render(up, cams, precision=ffi.RG_EXT_PRECISION_FP16_NODES | ffi.RG_EXT_PRECISION_FP16_STORAGE)
+    assert np.array_equal(both["VISIBILITY_BUFFER"], ref["VISIBILITY_BUFFER"])
+    g, e = both["PRE_FINAL"][..., :3].astype(np.float64), ref["PRE_FINAL"][..., :3].astype(np.float64)
+    rel = (np.abs(g - e) / (np.abs(e) + 1e-3 * np.abs(e).mean())).max(axis=-1)
+    print(f"FP16 storage frame vs FP32: PRE_FINAL rel p50 {np.median(rel):.2e} p99 {np.quantile(rel, 0.99):.2e} "
+          f"max {rel.max():.2e}", flush=True)
+    assert np.quantile(rel, 0.99) < 1e-2
+    assert abs(g.mean() / e.mean() - 1.0) < 1e-3

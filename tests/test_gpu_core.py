@@ -223,3 +223,63 @@ def test_device_copy_kernel(gpu):
     with pytest.raises(ffi.RgError):
         ffi.call("rgExtCopyDeviceToDevice", dst.ptr, src.ptr, 20, None)
 
+
+
+def _decode_half_nodes(nodes):
+    """Child boxes of FP16 nodes as the traversal decodes them: origin +
+    float(offset), rounded to FP32."""
+    o = nodes["origin"].astype(np.float32)
+    out = {}
+    for f, axis in (("lo_x", 0), ("hi_x", 0), ("lo_y", 1), ("hi_y", 1), ("lo_z", 2), ("hi_z", 2)):
+        out[f] = o[:, axis:axis + 1] + nodes[f].view(np.float16).astype(np.float32)
+    return out
+
+
+@pytest.mark.parametrize("case", ["terrain_boxes", "far_soup"])
+def test_fp16_nodes_contain_fp32_boxes_and_hit_the_same(gpu, case):
+    """FP16 nodes (FP32 origin + half offsets rounded outward) have the FP32
+    build's topology and boxes that contain its boxes, so closest-hit traversal
+    finds the same hits bit for bit (the hit rule is order-independent)."""
+    print(f"fp16 nodes {case}", flush=True)
+    if case == "terrain_boxes":
+        tris = scenes.terrain_boxes_scene().tris
+        rays = np.concatenate([scenes.primary_rays(scenes.terrain_boxes_camera(), 640, 360)])
+    else:
+        # 200k small triangles 20,000 units from the origin: absolute FP16
+        # coordinates there are 16 units apart
+        tris = _random_tris(200_000, 9, spread=500.0, size=0.3) + np.float32(20000.0)
+        r = np.random.default_rng(4)
+        rays = np.zeros(100_000, RAY_DTYPE)
+        rays["origin"] = r.uniform(-600, 600, (100_000, 3)) + 20000.0
+        d = r.normal(size=(100_000, 3))
+        rays["direction"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+        rays["tmax"] = 2000.0
+    d = DeviceArray.from_numpy(tris)
+    b32, b16 = Bvh(), Bvh()
+    b32.build(d, len(tris), 1, ffi.RG_EXT_BVH_NODE_FP32)
+    b16.build(d, len(tris), 1, ffi.RG_EXT_BVH_NODE_FP16)
+    g32, g16 = b32.download(), b16.download()
+    assert b16.view().nodeStrideBytes == 144
+    assert g16["node_count"] == g32["node_count"]
+    assert np.array_equal(g16["nodes"]["child"], g32["nodes"]["child"])
+    assert np.array_equal(g16["sorted_tris"], g32["sorted_tris"])
+    dec = _decode_half_nodes(g16["nodes"])
+    full = g32["nodes"]["child"] != 0xFFFFFFFF
+    for f in ("lo_x", "lo_y", "lo_z"):
+        assert (dec[f][full] <= g32["nodes"][f][full]).all(), f
+    for f in ("hi_x", "hi_y", "hi_z"):
+        assert (dec[f][full] >= g32["nodes"][f][full]).all(), f
+    # outward rounding costs ~2^-11 of the node's extent, not of the coordinates
+    ext32 = g32["nodes"]["hi_x"][full].astype(np.float64) - g32["nodes"]["lo_x"][full]
+    ext16 = dec["hi_x"][full].astype(np.float64) - dec["lo_x"][full]
+    grow = (ext16 - ext32) / np.maximum(ext32, 1e-6)
+    print(f"fp16 box growth (x): median {np.median(grow):.2e}, p99 {np.quantile(grow, 0.99):.2e}", flush=True)
+    assert np.median(grow) < 2e-2
+    for flags in (0, RG_EXT_TRACE_ANY_HIT):
+        h32 = _trace_gpu(b32, rays, flags | RG_EXT_TRACE_COLLECT_STATS)
+        n32 = b32.trace_stats()
+        h16 = _trace_gpu(b16, rays, flags | RG_EXT_TRACE_COLLECT_STATS)
+        n16 = b16.trace_stats()
+        print(f"flags {flags}: node visits fp32 {n32[0]} fp16 {n16[0]}, triangle tests {n32[1]} {n16[1]}", flush=True)
+        assert b16.trace_errors() == 0
+        _assert_same_hits(h16, h32, any_hit=bool(flags & RG_EXT_TRACE_ANY_HIT))
