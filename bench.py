@@ -16,7 +16,16 @@ kernel k_trace) by algorithmic bytes: 32 B ray in + 16 B hit out per ray,
 measured with HIP events recorded on the instance stream around every launch.
 `cpu_baseline` times the CPU oracle on a bounded sample of the same frame.
 
-Usage: python bench.py [--gpus 1] [--steps 60] [--warmup 5] [--no-cpu-baseline]
+Beside the FP32 line (`value`), the same JSON object carries
+- `fp16`: the same frames with FP16 BVH nodes and FP16 storage of the
+  denoiser's temporal and a-trous planes (frames/s, Mrays/s, and the last
+  frame's denoised-radiance difference from the FP32 run);
+- `serving`: 256 camera sessions (960x540 .. 2560x1440) over 16 scenes of
+  200k-2M triangles, admitted against a 200 GiB budget, with Poisson frame
+  requests at 30 Hz per session for a few seconds: p50 / p99 frame latency
+  and frames/s at that load, then at a sustainable number of sessions.
+
+Usage: python bench.py [--gpus 1] [--steps 60] [--warmup 5] [--no-cpu-baseline] [--no-fp16] [--no-serving]
 """
 from __future__ import annotations
 
@@ -51,6 +60,11 @@ def parse_args(argv=None):
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (default: min(16, cpus))")
     ap.add_argument("--dynamic-upload", action="store_true",
                     help="re-upload every primitive each frame (the host-to-device-inclusive rate; not the headline)")
+    ap.add_argument("--no-fp16", action="store_true", help="skip the FP16 precision-form run")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving run")
+    ap.add_argument("--serving-sessions", type=int, default=256)
+    ap.add_argument("--serving-scenes", type=int, default=16)
+    ap.add_argument("--serving-seconds", type=float, default=5.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="instead of the frame benchmark, run 5 plain device copies of MIB MiB (k_copy16): the "
                          "known-size reference for scaling HBM counter readings")
@@ -149,6 +163,101 @@ def calibrate_copy(mib: int) -> int:
     return 0
 
 
+def run_frames(a, scene, precision: int):
+    """Warm-up, then a.steps timed rgDrawFrame calls of the orbit; returns the
+    per-frame stats, the wall time of the timed frames and the last frame's
+    PRE_FINAL plane."""
+    from raytracedgl1_amd import ffi, rtgl, scenes
+
+    up = rtgl.SceneUploader(scene)
+    r = rtgl.Renderer(WIDTH, HEIGHT)
+    try:
+        r.set_static_geometry(not a.dynamic_upload)
+        r.path.atrousIterations = 5
+        r.path.restirInitialCandidates = 32
+        r.path.restirTemporalSamples = 1
+        r.path.restirSpatialSamples = 1
+        r.path.collectTraversalStats = 1
+        r.path.precisionFlags = precision
+        frame = [0]
+
+        def step(geometry=False):
+            geometry = geometry or a.dynamic_upload
+            st = r.frame(up, scenes.orbit_camera(frame[0] % ORBIT_FRAMES), geometry=geometry)
+            frame[0] += 1
+            return st
+
+        # warm-up; the first step uploads the scene, which then stays resident
+        for i in range(max(1, a.warmup)):
+            step(geometry=(i == 0))
+            print(f"warmup {i + 1}/{max(1, a.warmup)} (precision flags {precision})", file=sys.stderr, flush=True)
+        ffi.call("rgExtSynchronize")
+        t0 = time.perf_counter()
+        stats = [step() for _ in range(a.steps)]
+        ffi.call("rgExtSynchronize")
+        wall = time.perf_counter() - t0
+        print(f"timed {a.steps} frames in {wall:.3f} s (precision flags {precision})", file=sys.stderr, flush=True)
+        assert all(st.bvhRebuilt == 0 for st in stats), "static geometry was rebuilt inside the timed region"
+        final = r.framebuffers().download(ffi.PLANE["PRE_FINAL"]).copy()
+        return stats, wall, final
+    finally:
+        r.close()
+
+
+def serving_run(a) -> dict:
+    """The serving form: sessions over shared scenes, Poisson requests at
+    30 Hz per session, FP16 storage and nodes; then a sustainable load of
+    sessions sized from the measured throughput."""
+    import numpy as np
+
+    from raytracedgl1_amd import ffi, scenes, serving
+
+    rng = np.random.default_rng(scenes.SEED_SERVING)
+    counts = np.exp(rng.uniform(np.log(200_000), np.log(2_000_000), a.serving_scenes)).astype(int)
+    res = [serving.SERVING_RESOLUTIONS[i] for i in rng.integers(0, 4, a.serving_sessions)]
+    precision = ffi.RG_EXT_PRECISION_FP16_STORAGE | ffi.RG_EXT_PRECISION_FP16_NODES
+    srv = serving.Server(2560, 1440, streams=4, frames_in_flight=2, precision=precision, budget_bytes=200 << 30,
+                         atrous=5, candidates=32, temporal=1, spatial=1)
+    try:
+        t = time.perf_counter()
+        handles = []
+        for i, c in enumerate(counts):
+            handles.append(srv.scene(serving.serving_scene(i, int(c))))
+            print(f"serving scene {i + 1}/{len(counts)}: {int(c)} triangles", file=sys.stderr, flush=True)
+        sessions, refused = [], 0
+        for k, (w, h) in enumerate(res):
+            try:
+                sessions.append(srv.session(handles[k % len(handles)], w, h))
+            except serving.OverBudget:
+                refused += 1
+        st = srv.stats()
+        print(f"serving: {len(sessions)} sessions admitted, {refused} refused, set-up {time.perf_counter() - t:.1f} s",
+              file=sys.stderr, flush=True)
+        # warm every session once (first frames allocate the traversal stacks)
+        for k, s in enumerate(sessions):
+            srv.submit(s, serving.session_camera(k, 0.0), 0.0, 0)
+        srv.wait_idle()
+        srv.poll()
+        full = serving.run_poisson(srv, sessions, 30.0, a.serving_seconds, seed=1, t_offset=1.0)
+        print(f"serving at {len(sessions)} sessions: {full}", file=sys.stderr, flush=True)
+        # sustainable load: the first sessions, as many as 70% of the measured
+        # frame rate can serve at 30 Hz
+        m = max(1, min(len(sessions), int(0.7 * full["frames_per_s"] / 30.0)))
+        light = serving.run_poisson(srv, sessions[:m], 30.0, min(3.0, a.serving_seconds), seed=2,
+                                    t_offset=100.0)
+        print(f"serving at {m} sessions: {light}", file=sys.stderr, flush=True)
+        mpx = float(np.mean([s.width * s.height for s in sessions])) / 1e6 if sessions else 0.0
+        return {
+            "sessions_requested": a.serving_sessions, "sessions_admitted": len(sessions), "sessions_refused": refused,
+            "scenes": len(handles), "scene_triangles": [int(c) for c in counts], "mean_mpix_per_session": mpx,
+            "memory_budget_gib": 200, "session_gib": st.sessionBytes / 2**30, "shared_gib": st.sharedBytes / 2**30,
+            "scene_gib": st.sceneBytes / 2**30, "streams": st.streams, "precision": "fp16 storage + fp16 nodes",
+            "request_rate_hz_per_session": 30.0, "full_load": full, "sustainable_load": light,
+        }
+    finally:
+        srv.close()
+
+
 def load_traffic():
     try:
         d = json.loads(TRAFFIC_FILE.read_text())
@@ -171,39 +280,10 @@ def main(argv=None) -> int:
     if a.calibrate_copy:
         return calibrate_copy(a.calibrate_copy)
 
-    from raytracedgl1_amd import ffi, rtgl, scenes
+    from raytracedgl1_amd import scenes
 
     scene = scenes.terrain_spheres_scene()
-    up = rtgl.SceneUploader(scene)
-    r = rtgl.Renderer(WIDTH, HEIGHT)
-    r.set_static_geometry(not a.dynamic_upload)
-    r.path.atrousIterations = 5
-    r.path.restirInitialCandidates = 32
-    r.path.restirTemporalSamples = 1
-    r.path.restirSpatialSamples = 1
-    r.path.collectTraversalStats = 1
-
-    frame = [0]
-
-    def step(geometry=False):
-        geometry = geometry or a.dynamic_upload
-        st = r.frame(up, scenes.orbit_camera(frame[0] % ORBIT_FRAMES), geometry=geometry)
-        frame[0] += 1
-        return st
-
-    # warm-up; the first step uploads the scene, which then stays resident
-    for i in range(max(1, a.warmup)):
-        step(geometry=(i == 0))
-        print(f"warmup {i + 1}/{max(1, a.warmup)}", file=sys.stderr, flush=True)
-
-    ffi.call("rgExtSynchronize")
-    t0 = time.perf_counter()
-    stats = [step() for _ in range(a.steps)]
-    ffi.call("rgExtSynchronize")
-    wall = time.perf_counter() - t0
-    print(f"timed {a.steps} frames in {wall:.3f} s", file=sys.stderr, flush=True)
-    assert all(st.bvhRebuilt == 0 for st in stats), "static geometry was rebuilt inside the timed region"
-
+    stats, wall, final32 = run_frames(a, scene, precision=0)
     n = len(stats)
     rays = sum(st.raysTraced for st in stats)
     trav_ms = sum(st.gpuMsTraversal for st in stats)
@@ -264,7 +344,29 @@ def main(argv=None) -> int:
         },
         "cpu_baseline": None,
     }
-    r.close()
+    if not a.no_fp16:
+        from raytracedgl1_amd import ffi
+        import numpy as np
+
+        st16, wall16, final16 = run_frames(a, scene, ffi.RG_EXT_PRECISION_FP16_NODES | ffi.RG_EXT_PRECISION_FP16_STORAGE)
+        n16 = len(st16)
+        rays16 = sum(s_.raysTraced for s_ in st16)
+        g, e = final16[..., :3].astype(np.float64), final32[..., :3].astype(np.float64)
+        rel = (np.abs(g - e) / (np.abs(e) + 1e-3 * np.abs(e).mean())).max(axis=-1)
+        out["fp16"] = {
+            "precision": "FP16 BVH nodes (144 B) + FP16 temporal/a-trous storage; FP32 arithmetic",
+            "frames_per_s": n16 / wall16,
+            "mrays_per_s": rays16 / wall16 / 1e6,
+            "speedup_vs_fp32": (n16 / wall16) / (n / wall),
+            "gpu_ms_per_frame": sum(s_.gpuMsTotal for s_ in st16) / n16,
+            "traversal_ms_per_frame": sum(s_.gpuMsTraversal for s_ in st16) / n16,
+            "denoise_ms_per_frame": sum(s_.gpuMsDenoise for s_ in st16) / n16,
+            "node_visits_per_ray": sum(s_.nodeVisits for s_ in st16) / rays16,
+            "pre_final_rel_err_vs_fp32": {"p50": float(np.median(rel)), "p99": float(np.quantile(rel, 0.99)),
+                                          "max": float(rel.max())},
+        }
+    if not a.no_serving:
+        out["serving"] = serving_run(a)
     if not a.no_cpu_baseline:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(scene, rays / n, threads)
