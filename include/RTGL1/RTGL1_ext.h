@@ -149,6 +149,58 @@ RGAPI RgResult RGCONV rgExtTraceRays( RgExtBvh        bvh,
                                       RgExtHit*       d_hits,
                                       uint32_t        flags,
                                       void*           stream );
+/* ------------------------------------------------------ batched scenes */
+/* Several scenes in one BVH object, built by one pass sequence whose every
+ * kernel covers all scenes: the triangles of scene s are
+ * [sceneTriangleOffsets[s], sceneTriangleOffsets[s+1]) of d_triangles (host
+ * array, sceneCount + 1 entries, each scene >= 2 triangles, 1..128 scenes).
+ * Each scene is quantised in its own centroid bounds; the sort key holds the
+ * scene in its top 7 bits above 19 Morton bits per axis, so the Karras
+ * hierarchy splits at scene boundaries first and every scene has its own
+ * root.  No counterpart in the reference, which builds one BLAS per
+ * geometry bucket through the driver (ASBuilder::BuildBottomLevel,
+ * Source/ASBuilder.cpp:95-155). */
+RGAPI RgResult RGCONV rgExtBvhBuildBatch( RgExtBvh                   bvh,
+                                          const float*               d_triangles,
+                                          const uint32_t*            sceneTriangleOffsets,
+                                          uint32_t                   sceneCount,
+                                          const RgExtBvhBuildParams* pParams,
+                                          void*                      stream );
+/* Root node index of every scene of a batched BVH (sceneCount entries). */
+RGAPI RgResult RGCONV rgExtBvhGetSceneRoots( RgExtBvh bvh, uint32_t* pRoots, uint32_t capacity, uint32_t* pSceneCount );
+/* One persistent launch over the rays of all scenes of a batched BVH: rays
+ * of scene s are [rayOffsets[s], rayOffsets[s+1]) of d_rays (host array,
+ * sceneCount + 1 entries); hits report primitive ids within the scene. */
+RGAPI RgResult RGCONV rgExtTraceRaysBatch( RgExtBvh        bvh,
+                                           const RgExtRay* d_rays,
+                                           const uint32_t* rayOffsets,
+                                           RgExtHit*       d_hits,
+                                           uint32_t        flags,
+                                           void*           stream );
+
+/* Traversal tuning chosen by rgExtTraceAutotune. */
+typedef struct RgExtTraceTuning
+{
+    uint32_t stackLdsEntries; /* LDS part of the 128-entry per-lane stack: 16, 24, 32 */
+    uint32_t rayChunk;        /* rays a wave takes per counter atomic: 32, 64, 128   */
+    float    msPerLaunch;     /* best time on the tuning rays                         */
+    float    msDefault;       /* time with the defaults (24, 64)                      */
+} RgExtTraceTuning;
+
+/* Times every stack depth x ray chunk on the given rays (rayOffsets for a
+ * batched BVH, else rayCount) and keeps the fastest for this BVH object's
+ * later traces; results do not depend on the choice.  Synchronises the
+ * stream.  No counterpart in the reference (the driver schedules its own
+ * traversal). */
+RGAPI RgResult RGCONV rgExtTraceAutotune( RgExtBvh          bvh,
+                                          const RgExtRay*   d_rays,
+                                          uint32_t          rayCount,
+                                          const uint32_t*   rayOffsets,
+                                          RgExtHit*         d_hits,
+                                          uint32_t          flags,
+                                          void*             stream,
+                                          RgExtTraceTuning* pOut );
+
 /* Node visits and triangle tests of the last rgExtTraceRays call made with
  * RG_EXT_TRACE_COLLECT_STATS (synchronises the device). */
 RGAPI RgResult RGCONV rgExtBvhGetTraceStats( RgExtBvh bvh, uint64_t* pNodeVisits, uint64_t* pTriangleTests );

@@ -255,6 +255,53 @@ RgResult rgExtTraceRays( RgExtBvh bvh, const RgExtRay* d_rays, uint32_t rayCount
     } );
 }
 
+RgResult rgExtBvhBuildBatch( RgExtBvh bvh, const float* d_triangles, const uint32_t* sceneTriangleOffsets, uint32_t sceneCount,
+                             const RgExtBvhBuildParams* pParams, void* stream )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( bvh && d_triangles && sceneTriangleOffsets, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        RgExtBvhBuildParams p = { 3, RG_EXT_BVH_NODE_FP32 };
+        if( pParams )
+        {
+            p = *pParams;
+        }
+        BuildBvhBatch( bvh->bvh, d_triangles, sceneTriangleOffsets, sceneCount, p, AsStream( stream ) );
+    } );
+}
+
+RgResult rgExtBvhGetSceneRoots( RgExtBvh bvh, uint32_t* pRoots, uint32_t capacity, uint32_t* pSceneCount )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( bvh && pSceneCount && ( pRoots || capacity == 0 ), RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        const auto& r = bvh->bvh.hostSceneRoots;
+        *pSceneCount  = bvh->bvh.sceneCount;
+        for( uint32_t i = 0; i < capacity && i < bvh->bvh.sceneCount; i++ )
+        {
+            pRoots[ i ] = r[ i ];
+        }
+    } );
+}
+
+RgResult rgExtTraceRaysBatch( RgExtBvh bvh, const RgExtRay* d_rays, const uint32_t* rayOffsets, RgExtHit* d_hits, uint32_t flags, void* stream )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( bvh && rayOffsets, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        RG_REQUIRE( ( flags & ~7u ) == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "unknown trace flags" );
+        RG_REQUIRE( ( d_rays && d_hits ) || rayOffsets[ bvh->bvh.sceneCount ] == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        TraceRaysBatch( bvh->bvh, bvh->trace, d_rays, rayOffsets, d_hits, flags, AsStream( stream ) );
+    } );
+}
+
+RgResult rgExtTraceAutotune( RgExtBvh bvh, const RgExtRay* d_rays, uint32_t rayCount, const uint32_t* rayOffsets, RgExtHit* d_hits, uint32_t flags,
+                             void* stream, RgExtTraceTuning* pOut )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( bvh && d_rays && d_hits, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        RG_REQUIRE( ( flags & ~7u ) == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "unknown trace flags" );
+        AutotuneTrace( bvh->bvh, bvh->trace, d_rays, rayCount, rayOffsets, d_hits, flags, AsStream( stream ), pOut );
+    } );
+}
+
 RgResult rgExtBvhGetTraceStats( RgExtBvh bvh, uint64_t* pNodeVisits, uint64_t* pTriangleTests )
 {
     return Guard( [ & ] {

@@ -3,6 +3,8 @@
 
 #include "rg_internal.h"
 
+#include <vector>
+
 namespace rg
 {
 
@@ -73,6 +75,10 @@ struct BvhObject
     DeviceBuffer nodes;        // nodeCount * stride
     DeviceBuffer nodes32;      // FP32 nodes an FP16 build converts from
     uint32_t*    hostCounts = nullptr; // pinned
+    // batched build: per-scene offsets, binary and 8-wide roots, bounds
+    uint32_t                sceneCount = 0;
+    DeviceBuffer            sceneOffsets, sceneRoots, sceneRoots8, sceneBounds, scenePartials;
+    std::vector< uint32_t > hostSceneOffsets, hostSceneRoots;
 
     // Index of the sorted-order permutation after the sort (keysA/valsA or B).
     const uint32_t* sortedIdx = nullptr;
@@ -90,6 +96,13 @@ struct BvhObject
 };
 
 void BuildBvh( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtBvhBuildParams& params, hipStream_t stream );
+// Several scenes in one BVH object: triangles of scene s are
+// [offsets[s], offsets[s+1]) of d_tris (host offsets); each scene gets its
+// own root (hostSceneRoots) and never shares a node with another scene.
+void BuildBvhBatch( BvhObject& bvh, const float* d_tris, const uint32_t* offsets, uint32_t sceneCount, const RgExtBvhBuildParams& params,
+                    hipStream_t stream );
+
+constexpr int TRACE_STACK_VARIANTS = 3;
 
 // Per-caller traversal scratch: the ray counter, an overflow-flag word and
 // the global overflow stacks of the persistent grid.  Launches that share a
@@ -99,8 +112,12 @@ struct TraceContext
     DeviceBuffer counter;
     DeviceBuffer overflow;
     DeviceBuffer stats; // 2 x u64: node visits, triangle tests of the last collecting launch
-    uint32_t     gridBlocks = 0;     // resident blocks of the FP32-node kernel
-    uint32_t     gridBlocksHalf = 0; // ... of the FP16-node kernel
+    DeviceBuffer            segRays; // ray offsets of a batched launch
+    std::vector< uint32_t > segHost;
+    uint32_t     gridBlocks = 0;                        // the largest grid (sizes the overflow stacks)
+    uint32_t     grid[ 2 ][ TRACE_STACK_VARIANTS ] = {}; // resident blocks per node format and stack variant
+    uint32_t     stackVariant = 1;                      // LDS stack depth 16, 24 (default) or 32
+    uint32_t     rayChunk     = 64;                     // rays per counter atomic
 };
 
 // Device accumulators a caller clears once and lets several launches add to.
@@ -115,6 +132,15 @@ struct TraceAccum
 // in ctx (read back by ReadTraceStats / ReadTraceErrors).
 void TraceRays( const BvhObject& bvh, TraceContext& ctx, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags, hipStream_t stream,
                 hipEvent_t evBefore = nullptr, hipEvent_t evAfter = nullptr, const TraceAccum* accum = nullptr );
+// One persistent launch over the rays of every scene of a batched BVH: rays
+// of scene s are [rayOffsets[s], rayOffsets[s+1]) (host array, sceneCount + 1
+// entries); primitive ids in the hits are relative to the scene.
+void TraceRaysBatch( const BvhObject& bvh, TraceContext& ctx, const RgExtRay* d_rays, const uint32_t* rayOffsets, RgExtHit* d_hits, uint32_t flags,
+                     hipStream_t stream, hipEvent_t evBefore = nullptr, hipEvent_t evAfter = nullptr );
+// Times every LDS stack depth x ray chunk on these rays (a batched BVH takes
+// rayOffsets, else count) and keeps the fastest on ctx for later launches.
+void AutotuneTrace( const BvhObject& bvh, TraceContext& ctx, const RgExtRay* d_rays, uint32_t count, const uint32_t* rayOffsets, RgExtHit* d_hits,
+                    uint32_t flags, hipStream_t stream, RgExtTraceTuning* out );
 // Synchronous reads of the last un-accumulated launch's results.
 void     ReadTraceStats( const TraceContext& ctx, uint64_t* nodeVisits, uint64_t* triTests );
 uint32_t ReadTraceErrors( const TraceContext& ctx );

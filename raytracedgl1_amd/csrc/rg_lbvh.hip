@@ -143,6 +143,149 @@ __global__ __launch_bounds__( THREADS ) void k_morton( const float* tris, uint32
     vals[ i ]     = i;
 }
 
+// ------------------------------------------------------ batched scenes
+// S scenes concatenated: scene s owns triangles [offsets[s], offsets[s+1]).
+
+__device__ __forceinline__ uint32_t SceneOf( const uint32_t* offsets, uint32_t S, uint32_t i )
+{
+    uint32_t lo = 0, hi = S; // last scene with offsets[s] <= i
+    while( hi - lo > 1 )
+    {
+        const uint32_t mid = ( lo + hi ) / 2;
+        if( offsets[ mid ] <= i )
+        {
+            lo = mid;
+        }
+        else
+        {
+            hi = mid;
+        }
+    }
+    return lo;
+}
+
+// Block b of scene s (blockIdx.x = s * B + b) reduces part of that scene.
+__global__ __launch_bounds__( THREADS ) void k_centroid_bounds_seg( const float* tris, const uint32_t* offsets, uint32_t B, double* partials )
+{
+    __shared__ double s[ 6 ][ THREADS ];
+    const uint32_t    scene = blockIdx.x / B, b = blockIdx.x % B;
+    const uint32_t    first = offsets[ scene ], last = offsets[ scene + 1 ];
+    double            lo[ 3 ] = { DBL_MAX, DBL_MAX, DBL_MAX };
+    double            hi[ 3 ] = { -DBL_MAX, -DBL_MAX, -DBL_MAX };
+    for( uint32_t i = first + b * THREADS + threadIdx.x; i < last; i += B * THREADS )
+    {
+        double c[ 3 ];
+        Centroid( tris, i, c );
+        for( int k = 0; k < 3; k++ )
+        {
+            lo[ k ] = fmin( lo[ k ], c[ k ] );
+            hi[ k ] = fmax( hi[ k ], c[ k ] );
+        }
+    }
+    for( int k = 0; k < 3; k++ )
+    {
+        s[ k ][ threadIdx.x ]     = lo[ k ];
+        s[ 3 + k ][ threadIdx.x ] = hi[ k ];
+    }
+    __syncthreads();
+    for( int stride = THREADS / 2; stride > 0; stride >>= 1 )
+    {
+        if( int( threadIdx.x ) < stride )
+        {
+            for( int k = 0; k < 3; k++ )
+            {
+                s[ k ][ threadIdx.x ]     = fmin( s[ k ][ threadIdx.x ], s[ k ][ threadIdx.x + stride ] );
+                s[ 3 + k ][ threadIdx.x ] = fmax( s[ 3 + k ][ threadIdx.x ], s[ 3 + k ][ threadIdx.x + stride ] );
+            }
+        }
+        __syncthreads();
+    }
+    if( threadIdx.x < 6 )
+    {
+        partials[ size_t( blockIdx.x ) * 6 + threadIdx.x ] = s[ threadIdx.x ][ 0 ];
+    }
+}
+
+// Block s combines scene s's B partials.
+__global__ void k_bounds_final_seg( const double* partials, uint32_t B, double* out )
+{
+    if( threadIdx.x < 6 )
+    {
+        const bool isMin = threadIdx.x < 3;
+        double     v     = isMin ? DBL_MAX : -DBL_MAX;
+        for( uint32_t b = 0; b < B; b++ )
+        {
+            const double p = partials[ ( size_t( blockIdx.x ) * B + b ) * 6 + threadIdx.x ];
+            v              = isMin ? fmin( v, p ) : fmax( v, p );
+        }
+        out[ size_t( blockIdx.x ) * 6 + threadIdx.x ] = v;
+    }
+}
+
+// Morton code in the scene's own bounds; the sort key carries the scene in
+// its top 7 bits above the code's top 57 bits (19 per axis), so one sort
+// groups the scenes and orders each by its Morton curve, and the Karras
+// hierarchy over the keys splits at scene boundaries first.
+__global__ __launch_bounds__( THREADS ) void k_morton_seg( const float* tris, uint32_t n, const uint32_t* offsets, uint32_t S,
+                                                           const double* bounds, uint64_t* codes, uint64_t* keys, uint32_t* vals )
+{
+    uint32_t i = blockIdx.x * THREADS + threadIdx.x;
+    if( i >= n )
+    {
+        return;
+    }
+    const uint32_t scene = SceneOf( offsets, S, i );
+    const double*  bb    = bounds + size_t( scene ) * 6;
+    double         c[ 3 ];
+    Centroid( tris, i, c );
+    uint32_t q[ 3 ];
+    for( int k = 0; k < 3; k++ )
+    {
+        q[ k ] = Quantize21( c[ k ], bb[ k ], bb[ 3 + k ] );
+    }
+    const uint64_t code = ( ExpandBits21( q[ 0 ] ) << 2 ) | ( ExpandBits21( q[ 1 ] ) << 1 ) | ExpandBits21( q[ 2 ] );
+    codes[ i ]          = code;
+    keys[ i ]           = ( uint64_t( scene ) << 57 ) | ( code >> 6 );
+    vals[ i ]           = i;
+}
+
+// The internal node whose range is exactly scene s (each scene has >= 2
+// triangles, and its keys differ from its neighbours' in the scene bits).
+__global__ __launch_bounds__( THREADS ) void k_scene_roots( uint32_t internal, const uint32_t* rf, const uint32_t* rl, const uint32_t* offsets,
+                                                            uint32_t S, uint32_t* roots )
+{
+    const uint32_t i = blockIdx.x * THREADS + threadIdx.x;
+    if( i >= internal )
+    {
+        return;
+    }
+    const uint32_t scene = SceneOf( offsets, S, rf[ i ] );
+    if( rf[ i ] == offsets[ scene ] && rl[ i ] + 1u == offsets[ scene + 1 ] )
+    {
+        roots[ scene ] = i;
+    }
+}
+
+__global__ void k_seed_frontier_seg( const uint32_t* roots, uint32_t S, uint32_t* frontier, uint32_t* counts )
+{
+    for( uint32_t s = threadIdx.x; s < S; s += blockDim.x )
+    {
+        frontier[ s ] = roots[ s ];
+    }
+    if( threadIdx.x == 0 )
+    {
+        counts[ 0 ] = S;
+    }
+}
+
+__global__ void k_gather_roots8( const uint32_t* roots, uint32_t S, const uint32_t* node8Index, uint32_t* roots8 )
+{
+    for( uint32_t s = threadIdx.x; s < S; s += blockDim.x )
+    {
+        roots8[ s ] = node8Index[ roots[ s ] ];
+    }
+}
+
 // --------------------------------------------------------- radix sort
 
 __global__ __launch_bounds__( THREADS ) void k_radix_hist( const uint64_t* keys, uint32_t n, int shift, uint32_t* hist, uint32_t numBlocks )
@@ -625,7 +768,13 @@ __global__ void k_seed_frontier( uint32_t* frontier, uint32_t* counts )
 
 } // namespace
 
-void BuildBvh( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtBvhBuildParams& params, hipStream_t stream )
+namespace
+{
+// Shared by the single-scene and the batched build; `batch` (host offsets,
+// S + 1 of them) switches steps 1-2 to per-scene bounds and keys and seeds
+// the collapse with every scene's root.
+void BuildBvhImpl( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtBvhBuildParams& params, const uint32_t* batchOffsets,
+                   uint32_t sceneCount, hipStream_t stream )
 {
     RG_REQUIRE( n > 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "triangle count must be > 0" );
     RG_REQUIRE( n < ( 1u << 28 ), RG_RESULT_WRONG_FUNCTION_ARGUMENT, "triangle count must be < 2^28" );
@@ -672,16 +821,47 @@ void BuildBvh( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtBvhBu
 
     double* partials    = bvh.boundsPartials.as< double >();
     double* sceneBounds = partials + size_t( bndBlocks ) * 6;
+    const bool batched  = batchOffsets != nullptr;
+    bvh.sceneCount      = batched ? sceneCount : 0u;
 
-    // 1. FP64 centroid bounds.
-    k_centroid_bounds<<< bndBlocks, THREADS, 0, stream >>>( d_tris, n, partials );
-    CheckLaunch( "k_centroid_bounds" );
-    k_bounds_final<<< 1, 64, 0, stream >>>( partials, bndBlocks, sceneBounds );
-    CheckLaunch( "k_bounds_final" );
+    if( !batched )
+    {
+        // 1. FP64 centroid bounds.
+        k_centroid_bounds<<< bndBlocks, THREADS, 0, stream >>>( d_tris, n, partials );
+        CheckLaunch( "k_centroid_bounds" );
+        k_bounds_final<<< 1, 64, 0, stream >>>( partials, bndBlocks, sceneBounds );
+        CheckLaunch( "k_bounds_final" );
 
-    // 2. Morton codes (input order) and the identity permutation.
-    k_morton<<< DivUp( n, THREADS ), THREADS, 0, stream >>>( d_tris, n, sceneBounds, bvh.codes.as< uint64_t >(), bvh.keysA.as< uint64_t >(), bvh.valsA.as< uint32_t >() );
-    CheckLaunch( "k_morton" );
+        // 2. Morton codes (input order) and the identity permutation.
+        k_morton<<< DivUp( n, THREADS ), THREADS, 0, stream >>>( d_tris, n, sceneBounds, bvh.codes.as< uint64_t >(), bvh.keysA.as< uint64_t >(), bvh.valsA.as< uint32_t >() );
+        CheckLaunch( "k_morton" );
+    }
+    else
+    {
+        // 1-2 per scene: bounds of each scene's centroids, then keys with the
+        // scene in the top bits
+        const uint32_t B = 16;
+        bvh.sceneOffsets.Reserve( size_t( sceneCount + 1 ) * 4 );
+        bvh.sceneRoots.Reserve( size_t( sceneCount ) * 4 );
+        bvh.sceneRoots8.Reserve( size_t( sceneCount ) * 4 );
+        bvh.sceneBounds.Reserve( size_t( sceneCount ) * 6 * 8 );
+        bvh.scenePartials.Reserve( size_t( sceneCount ) * B * 6 * 8 );
+        bvh.hostSceneOffsets.assign( batchOffsets, batchOffsets + sceneCount + 1 );
+        RG_HIP_CHECK( hipMemcpyAsync( bvh.sceneOffsets.ptr, bvh.hostSceneOffsets.data(), size_t( sceneCount + 1 ) * 4, hipMemcpyHostToDevice, stream ) );
+        const uint32_t* offs = bvh.sceneOffsets.as< uint32_t >();
+        k_centroid_bounds_seg<<< sceneCount * B, THREADS, 0, stream >>>( d_tris, offs, B, bvh.scenePartials.as< double >() );
+        CheckLaunch( "k_centroid_bounds_seg" );
+        k_bounds_final_seg<<< sceneCount, 64, 0, stream >>>( bvh.scenePartials.as< double >(), B, bvh.sceneBounds.as< double >() );
+        CheckLaunch( "k_bounds_final_seg" );
+        k_morton_seg<<< DivUp( n, THREADS ), THREADS, 0, stream >>>( d_tris, n, offs, sceneCount, bvh.sceneBounds.as< double >(),
+                                                                     bvh.codes.as< uint64_t >(), bvh.keysA.as< uint64_t >(), bvh.valsA.as< uint32_t >() );
+        CheckLaunch( "k_morton_seg" );
+        // the whole-batch bounds are not used; keep the view's fields defined
+        k_centroid_bounds<<< bndBlocks, THREADS, 0, stream >>>( d_tris, n, partials );
+        CheckLaunch( "k_centroid_bounds" );
+        k_bounds_final<<< 1, 64, 0, stream >>>( partials, bndBlocks, sceneBounds );
+        CheckLaunch( "k_bounds_final" );
+    }
 
     // 3. Stable LSD radix sort, 8 passes of 8 bits.
     uint64_t* kin  = bvh.keysA.as< uint64_t >();
@@ -723,8 +903,20 @@ void BuildBvh( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtBvhBu
     uint32_t* counts = bvh.levelCounts.as< uint32_t >();
     RG_HIP_CHECK( hipMemsetAsync( counts, 0, ( BVH_MAX_LEVELS + 2 ) * 4, stream ) );
     RG_HIP_CHECK( hipMemsetAsync( bvh.isNode8.ptr, 0, size_t( slots ) * 4, stream ) );
-    k_seed_frontier<<< 1, 1, 0, stream >>>( bvh.frontier[ 0 ].as< uint32_t >(), counts );
-    CheckLaunch( "k_seed_frontier" );
+    if( !batched )
+    {
+        k_seed_frontier<<< 1, 1, 0, stream >>>( bvh.frontier[ 0 ].as< uint32_t >(), counts );
+        CheckLaunch( "k_seed_frontier" );
+    }
+    else
+    {
+        RG_HIP_CHECK( hipMemsetAsync( bvh.sceneRoots.ptr, 0xFF, size_t( sceneCount ) * 4, stream ) );
+        k_scene_roots<<< DivUp( internal, THREADS ), THREADS, 0, stream >>>( internal, rf, rl, bvh.sceneOffsets.as< uint32_t >(), sceneCount,
+                                                                           bvh.sceneRoots.as< uint32_t >() );
+        CheckLaunch( "k_scene_roots" );
+        k_seed_frontier_seg<<< 1, 256, 0, stream >>>( bvh.sceneRoots.as< uint32_t >(), sceneCount, bvh.frontier[ 0 ].as< uint32_t >(), counts );
+        CheckLaunch( "k_seed_frontier_seg" );
+    }
 
     const uint32_t maxFrontier   = n / ( bvh.leafMax + 1 ) + 1;
     const uint32_t collapseGrid  = DivUp( maxFrontier, THREADS ) < 4096 ? DivUp( maxFrontier, THREADS ) : 4096;
@@ -775,10 +967,40 @@ void BuildBvh( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtBvhBu
     }
     k_sorted_tris<<< DivUp( n, THREADS ), THREADS, 0, stream >>>( d_tris, vin, n, bvh.sortedTris.as< float4 >() );
     CheckLaunch( "k_sorted_tris" );
+    if( batched )
+    {
+        k_gather_roots8<<< 1, 256, 0, stream >>>( bvh.sceneRoots.as< uint32_t >(), sceneCount, bvh.node8Index.as< uint32_t >(),
+                                                  bvh.sceneRoots8.as< uint32_t >() );
+        CheckLaunch( "k_gather_roots8" );
+        bvh.hostSceneRoots.resize( sceneCount );
+        RG_HIP_CHECK( hipMemcpyAsync( bvh.hostSceneRoots.data(), bvh.sceneRoots8.ptr, size_t( sceneCount ) * 4, hipMemcpyDeviceToHost, stream ) );
+    }
 
     RG_HIP_CHECK( hipMemcpyAsync( bvh.sceneMin, sceneBounds, 3 * sizeof( double ), hipMemcpyDeviceToHost, stream ) );
     RG_HIP_CHECK( hipMemcpyAsync( bvh.sceneMax, sceneBounds + 3, 3 * sizeof( double ), hipMemcpyDeviceToHost, stream ) );
     RG_HIP_CHECK( hipStreamSynchronize( stream ) );
+}
+} // namespace
+
+void BuildBvh( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtBvhBuildParams& params, hipStream_t stream )
+{
+    BuildBvhImpl( bvh, d_tris, n, params, nullptr, 0, stream );
+}
+
+void BuildBvhBatch( BvhObject& bvh, const float* d_tris, const uint32_t* offsets, uint32_t sceneCount, const RgExtBvhBuildParams& params,
+                    hipStream_t stream )
+{
+    RG_REQUIRE( offsets && sceneCount >= 1 && sceneCount <= 128, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "1..128 scenes" );
+    RG_REQUIRE( offsets[ 0 ] == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "scene offsets must start at 0" );
+    for( uint32_t s = 0; s < sceneCount; s++ )
+    {
+        RG_REQUIRE( offsets[ s + 1 ] >= offsets[ s ] + 2, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "every scene needs at least 2 triangles" );
+    }
+    BuildBvhImpl( bvh, d_tris, offsets[ sceneCount ], params, offsets, sceneCount, stream );
+    for( uint32_t s = 0; s < sceneCount; s++ )
+    {
+        RG_REQUIRE( bvh.hostSceneRoots[ s ] < bvh.nodeCount, RG_RESULT_INTERNAL_ERROR, "a scene has no root node" );
+    }
 }
 
 } // namespace rg

@@ -25,11 +25,10 @@ namespace
 {
 
 constexpr int      TR_THREADS    = 256;
-constexpr int      STACK_LDS     = 24;
-constexpr int      STACK_GLOBAL  = 104;
-constexpr int      STACK_TOTAL   = STACK_LDS + STACK_GLOBAL;
+constexpr int      STACK_TOTAL   = 128; // entries per lane: LDS part + global overflow part
+constexpr int      STACK_OVF_STRIDE = 112; // global entries per lane, enough for the smallest LDS part (16)
 constexpr uint32_t MAX_STEPS_PER_RAY = 1u << 22;
-constexpr uint32_t RAY_CHUNK         = 64; // rays per counter atomic
+constexpr uint32_t RAY_CHUNK_DEFAULT = 64; // rays per counter atomic
 constexpr uint32_t RAY_PARTITIONS    = 32;  // ray counters
 constexpr uint32_t COUNTER_STRIDE    = 64;  // u32 between counters (256 B)
 
@@ -46,7 +45,33 @@ struct TraceArgs
     uint32_t*       overflowStack;
     uint32_t*       errorFlag;
     uint64_t*       stats; // [0] node visits, [1] triangle tests (RG_EXT_TRACE_COLLECT_STATS)
+    // rays of several scenes of a batched BVH (segCount > 0): rays of scene s
+    // are [segRays[s], segRays[s+1]), traversed from segRoots[s]; hits report
+    // primitive ids relative to segTris[s]
+    const uint32_t* segRays;
+    const uint32_t* segRoots;
+    const uint32_t* segTris;
+    uint32_t        segCount;
+    uint32_t        chunk; // rays per counter atomic (tuned)
 };
+
+__device__ __forceinline__ uint32_t SegmentOf( const uint32_t* offsets, uint32_t S, uint32_t i )
+{
+    uint32_t lo = 0, hi = S; // last segment with offsets[s] <= i
+    while( hi - lo > 1 )
+    {
+        const uint32_t mid = ( lo + hi ) / 2;
+        if( offsets[ mid ] <= i )
+        {
+            lo = mid;
+        }
+        else
+        {
+            hi = mid;
+        }
+    }
+    return lo;
+}
 
 // Children are ordered by a 64-bit key: the distance (+0 for -0) mapped to
 // an unsigned integer with the same order, then the child reference.  A
@@ -114,7 +139,7 @@ __device__ __forceinline__ void LoadHalfNode( const void* nodes, uint32_t node, 
     }
 }
 
-template< bool H >
+template< bool H, int STACK_LDS >
 __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
 {
     __shared__ uint32_t s_stack[ STACK_LDS * TR_THREADS ];
@@ -123,7 +148,7 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
     const int      lane      = tid & 63;
     const uint64_t ltMask    = lane == 0 ? 0ull : ( ~0ull >> ( 64 - lane ) );
     const uint32_t gthread   = blockIdx.x * TR_THREADS + tid;
-    uint32_t*      ovf       = a.overflowStack + size_t( gthread ) * STACK_GLOBAL;
+    uint32_t*      ovf       = a.overflowStack + size_t( gthread ) * STACK_OVF_STRIDE;
     const bool     anyHit    = ( a.flags & RG_EXT_TRACE_ANY_HIT ) != 0;
     const bool     cullFront = ( a.flags & RG_EXT_TRACE_CULL_FRONT_FACE ) != 0;
 
@@ -133,6 +158,7 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
     float    tmin = 0.0f, bestT = 0.0f, bestU = 0.0f, bestV = 0.0f;
     uint32_t bestPrim = RG_EXT_MISS;
     uint32_t node     = 0;
+    uint32_t triBase  = 0; // first triangle of the ray's scene (batched BVHs)
     int      sp       = 0;
     uint32_t steps    = 0;
     uint32_t nodeVisits = 0, triTests = 0;
@@ -142,7 +168,7 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
 
     while( true )
     {
-        // Refill idle lanes.  Each wave owns a chunk of RAY_CHUNK consecutive
+        // Refill idle lanes.  Each wave owns a chunk of a.chunk consecutive
         // rays taken with one atomic on its partition's counter; idle lanes
         // take rays from the chunk (wave-uniform bookkeeping, no atomic).
         const bool     need = !active && !exhausted;
@@ -162,7 +188,7 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
                 {
                     const uint32_t j = atomicAdd( a.rayCounter + part * COUNTER_STRIDE, 1u );
                     const uint64_t g = uint64_t( j ) * a.partitions + part;
-                    b                = g * RAY_CHUNK < a.count ? uint32_t( g * RAY_CHUNK ) : a.count;
+                    b                = g * a.chunk < a.count ? uint32_t( g * a.chunk ) : a.count;
                 }
                 newBase = __shfl( b, leader, 64 );
             }
@@ -188,6 +214,13 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
                     bestU = bestV = 0.0f;
                     bestPrim      = RG_EXT_MISS;
                     node          = 0; // root
+                    triBase       = 0;
+                    if( a.segCount )
+                    {
+                        const uint32_t seg = SegmentOf( a.segRays, a.segCount, my );
+                        node               = a.segRoots[ seg ];
+                        triBase            = a.segTris[ seg ];
+                    }
                     sp            = 0;
                     steps         = 0;
                     active        = true;
@@ -203,7 +236,7 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
             }
             else if( newBase < a.count )
             {
-                chunkEnd  = a.count - newBase > RAY_CHUNK ? newBase + RAY_CHUNK : a.count;
+                chunkEnd  = a.count - newBase > a.chunk ? newBase + a.chunk : a.count;
                 chunkNext = newBase + ( cnt - left ) < chunkEnd ? newBase + ( cnt - left ) : chunkEnd;
             }
             else
@@ -378,7 +411,7 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
             h.t                = bestT;
             h.u                = bestU;
             h.v                = bestV;
-            h.primitiveId      = bestPrim;
+            h.primitiveId      = bestPrim == RG_EXT_MISS ? RG_EXT_MISS : bestPrim - triBase;
             a.hits[ rayId ]    = h;
             active             = false;
         }
@@ -408,36 +441,68 @@ uint32_t ReadTraceErrors( const TraceContext& s )
     return e;
 }
 
-void TraceRays( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags, hipStream_t stream,
-                hipEvent_t evBefore, hipEvent_t evAfter, const TraceAccum* accum )
+namespace
+{
+// LDS stack depths the autotuner chooses from (entries per lane; the rest of
+// the 128 entries are in global memory).
+constexpr int TRACE_STACK_DEPTH[ TRACE_STACK_VARIANTS ] = { 16, 24, 32 };
+
+using TraceKernelFn = void ( * )( TraceArgs );
+
+TraceKernelFn TraceKernel( int half, int v )
+{
+    static const TraceKernelFn k[ 2 ][ TRACE_STACK_VARIANTS ] = {
+        { k_trace< false, 16 >, k_trace< false, 24 >, k_trace< false, 32 > },
+        { k_trace< true, 16 >, k_trace< true, 24 >, k_trace< true, 32 > },
+    };
+    return k[ half ][ v ];
+}
+
+// Resident blocks of every variant; the overflow stacks are sized for the
+// largest grid, so any variant may run on a context.
+void InitGrids( TraceContext& s )
+{
+    if( s.gridBlocks != 0 )
+    {
+        return;
+    }
+    int dev = 0, cus = 0;
+    RG_HIP_CHECK( hipGetDevice( &dev ) );
+    RG_HIP_CHECK( hipDeviceGetAttribute( &cus, hipDeviceAttributeMultiprocessorCount, dev ) );
+    uint32_t maxGrid = 0;
+    for( int h = 0; h < 2; h++ )
+    {
+        for( int v = 0; v < TRACE_STACK_VARIANTS; v++ )
+        {
+            int perCu = 0;
+            RG_HIP_CHECK( hipOccupancyMaxActiveBlocksPerMultiprocessor( &perCu, TraceKernel( h, v ), TR_THREADS, 0 ) );
+            s.grid[ h ][ v ] = uint32_t( cus * ( perCu > 0 ? perCu : 1 ) );
+            maxGrid          = s.grid[ h ][ v ] > maxGrid ? s.grid[ h ][ v ] : maxGrid;
+        }
+    }
+    s.overflow.Reserve( size_t( maxGrid ) * TR_THREADS * STACK_OVF_STRIDE * sizeof( uint32_t ) );
+    s.gridBlocks = maxGrid;
+}
+
+void TraceRaysImpl( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags,
+                    hipStream_t stream, hipEvent_t evBefore, hipEvent_t evAfter, const TraceAccum* accum, const uint32_t* d_segRays )
 {
     RG_REQUIRE( bvh.nodeCount > 0, RG_RESULT_WRONG_FUNCTION_CALL, "BVH is not built" );
     RG_REQUIRE( bvh.nodeFormat == RG_EXT_BVH_NODE_FP32 || bvh.nodeFormat == RG_EXT_BVH_NODE_FP16, RG_RESULT_WRONG_FUNCTION_ARGUMENT,
                 "unsupported node format" );
-    // the chunk counter runs up to count + (waves + 1) * RAY_CHUNK without wrapping
+    // the chunk counter runs up to count + (waves + 1) * chunk without wrapping
     RG_REQUIRE( count <= 0xF0000000u, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "too many rays in one launch" );
     if( count == 0 )
     {
         return;
     }
-    if( s.gridBlocks == 0 )
-    {
-        int dev = 0, cus = 0, perCu = 0, perCuHalf = 0;
-        RG_HIP_CHECK( hipGetDevice( &dev ) );
-        RG_HIP_CHECK( hipDeviceGetAttribute( &cus, hipDeviceAttributeMultiprocessorCount, dev ) );
-        RG_HIP_CHECK( hipOccupancyMaxActiveBlocksPerMultiprocessor( &perCu, k_trace< false >, TR_THREADS, 0 ) );
-        RG_HIP_CHECK( hipOccupancyMaxActiveBlocksPerMultiprocessor( &perCuHalf, k_trace< true >, TR_THREADS, 0 ) );
-        s.gridBlocks     = uint32_t( cus * ( perCu > 0 ? perCu : 1 ) );
-        s.gridBlocksHalf = uint32_t( cus * ( perCuHalf > 0 ? perCuHalf : 1 ) );
-    }
-    const bool     half   = bvh.nodeFormat == RG_EXT_BVH_NODE_FP16;
-    const uint32_t grid   = half ? s.gridBlocksHalf : s.gridBlocks;
+    InitGrids( s );
+    const int      h      = bvh.nodeFormat == RG_EXT_BVH_NODE_FP16 ? 1 : 0;
+    const int      v      = int( s.stackVariant < TRACE_STACK_VARIANTS ? s.stackVariant : 1u );
+    const uint32_t grid   = s.grid[ h ][ v ];
     const uint32_t blocks = DivUp( count, TR_THREADS ) < grid ? DivUp( count, TR_THREADS ) : grid;
     s.counter.Reserve( ( RAY_PARTITIONS * COUNTER_STRIDE + 1 ) * sizeof( uint32_t ) );
     s.stats.Reserve( 2 * sizeof( uint64_t ) );
-    // one overflow stack per thread of the larger of the two grids
-    const uint32_t maxGrid = s.gridBlocks > s.gridBlocksHalf ? s.gridBlocks : s.gridBlocksHalf;
-    s.overflow.Reserve( size_t( maxGrid ) * TR_THREADS * STACK_GLOBAL * sizeof( uint32_t ) );
     RG_HIP_CHECK( hipMemsetAsync( s.counter.ptr, 0, ( RAY_PARTITIONS * COUNTER_STRIDE + 1 ) * sizeof( uint32_t ), stream ) );
 
     TraceArgs a;
@@ -452,6 +517,11 @@ void TraceRays( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, u
     a.partitions    = blocks < RAY_PARTITIONS ? blocks : RAY_PARTITIONS;
     a.overflowStack = s.overflow.as< uint32_t >();
     a.stats         = nullptr;
+    a.segRays       = d_segRays;
+    a.segRoots      = d_segRays ? bvh.sceneRoots8.as< uint32_t >() : nullptr;
+    a.segTris       = d_segRays ? bvh.sceneOffsets.as< uint32_t >() : nullptr;
+    a.segCount      = d_segRays ? bvh.sceneCount : 0u;
+    a.chunk         = s.rayChunk >= 16 && s.rayChunk <= 1024 ? s.rayChunk : RAY_CHUNK_DEFAULT;
     if( accum )
     {
         RG_REQUIRE( accum->errors, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "accumulator without an error word" );
@@ -467,18 +537,97 @@ void TraceRays( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, u
     {
         RG_HIP_CHECK( hipEventRecord( evBefore, stream ) );
     }
-    if( half )
-    {
-        k_trace< true ><<< blocks, TR_THREADS, 0, stream >>>( a );
-    }
-    else
-    {
-        k_trace< false ><<< blocks, TR_THREADS, 0, stream >>>( a );
-    }
+    hipLaunchKernelGGL( TraceKernel( h, v ), dim3( blocks ), dim3( TR_THREADS ), 0, stream, a );
     CheckLaunch( "k_trace" );
     if( evAfter )
     {
         RG_HIP_CHECK( hipEventRecord( evAfter, stream ) );
+    }
+}
+} // namespace
+
+void TraceRays( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags, hipStream_t stream,
+                hipEvent_t evBefore, hipEvent_t evAfter, const TraceAccum* accum )
+{
+    RG_REQUIRE( bvh.sceneCount == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "a batched BVH is traced with TraceRaysBatch" );
+    TraceRaysImpl( bvh, s, d_rays, count, d_hits, flags, stream, evBefore, evAfter, accum, nullptr );
+}
+
+void TraceRaysBatch( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, const uint32_t* rayOffsets, RgExtHit* d_hits, uint32_t flags,
+                     hipStream_t stream, hipEvent_t evBefore, hipEvent_t evAfter )
+{
+    const uint32_t S = bvh.sceneCount;
+    RG_REQUIRE( S > 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "the BVH was not built as a batch" );
+    RG_REQUIRE( rayOffsets && rayOffsets[ 0 ] == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "ray offsets must start at 0" );
+    for( uint32_t k = 0; k < S; k++ )
+    {
+        RG_REQUIRE( rayOffsets[ k + 1 ] >= rayOffsets[ k ], RG_RESULT_WRONG_FUNCTION_ARGUMENT, "ray offsets must not decrease" );
+    }
+    s.segHost.assign( rayOffsets, rayOffsets + S + 1 ); // kept alive until the copy has run
+    s.segRays.Reserve( size_t( S + 1 ) * 4 );
+    RG_HIP_CHECK( hipMemcpyAsync( s.segRays.ptr, s.segHost.data(), size_t( S + 1 ) * 4, hipMemcpyHostToDevice, stream ) );
+    TraceRaysImpl( bvh, s, d_rays, rayOffsets[ S ], d_hits, flags, stream, evBefore, evAfter, nullptr, s.segRays.as< uint32_t >() );
+}
+
+void AutotuneTrace( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, uint32_t count, const uint32_t* rayOffsets, RgExtHit* d_hits,
+                    uint32_t flags, hipStream_t stream, RgExtTraceTuning* out )
+{
+    RG_REQUIRE( count > 0 || rayOffsets, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "no rays to tune on" );
+    static const uint32_t chunks[ 3 ] = { 32, 64, 128 };
+    hipEvent_t            e0, e1;
+    RG_HIP_CHECK( hipEventCreate( &e0 ) );
+    RG_HIP_CHECK( hipEventCreate( &e1 ) );
+    auto run = [ & ]() {
+        if( rayOffsets )
+        {
+            TraceRaysBatch( bvh, s, d_rays, rayOffsets, d_hits, flags, stream, e0, e1 );
+        }
+        else
+        {
+            TraceRays( bvh, s, d_rays, count, d_hits, flags, stream, e0, e1 );
+        }
+        RG_HIP_CHECK( hipEventSynchronize( e1 ) );
+        float ms = 0.0f;
+        RG_HIP_CHECK( hipEventElapsedTime( &ms, e0, e1 ) );
+        return ms;
+    };
+    float    best = 1e30f, def = 0.0f;
+    uint32_t bestV = 1, bestC = 64;
+    for( uint32_t v = 0; v < TRACE_STACK_VARIANTS; v++ )
+    {
+        for( uint32_t c : chunks )
+        {
+            s.stackVariant = v;
+            s.rayChunk     = c;
+            run(); // warm
+            float t = 1e30f;
+            for( int r = 0; r < 3; r++ )
+            {
+                const float ms = run();
+                t              = ms < t ? ms : t;
+            }
+            if( v == 1 && c == 64 )
+            {
+                def = t;
+            }
+            if( t < best )
+            {
+                best  = t;
+                bestV = v;
+                bestC = c;
+            }
+        }
+    }
+    ( void )hipEventDestroy( e0 );
+    ( void )hipEventDestroy( e1 );
+    s.stackVariant = bestV;
+    s.rayChunk     = bestC;
+    if( out )
+    {
+        out->stackLdsEntries = uint32_t( TRACE_STACK_DEPTH[ bestV ] );
+        out->rayChunk        = bestC;
+        out->msPerLaunch     = best;
+        out->msDefault       = def;
     }
 }
 

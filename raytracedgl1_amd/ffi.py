@@ -47,6 +47,11 @@ assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16 and NODE8_DTYPE.ite
 assert NODE8H_DTYPE.itemsize == 144
 
 
+class RgExtTraceTuning(C.Structure):
+    _fields_ = [("stackLdsEntries", C.c_uint32), ("rayChunk", C.c_uint32), ("msPerLaunch", C.c_float),
+                ("msDefault", C.c_float)]
+
+
 class RgExtBvhBuildParams(C.Structure):
     _fields_ = [("maxLeafSize", C.c_uint32), ("nodeFormat", C.c_uint32)]
 
@@ -124,6 +129,10 @@ _SIGS = {
     "rgExtBvhBuild": (_RES, [_P, _P, _U32, C.POINTER(RgExtBvhBuildParams), _P]),
     "rgExtBvhGetView": (_RES, [_P, C.POINTER(RgExtBvhView)]),
     "rgExtTraceRays": (_RES, [_P, _P, _U32, _P, _U32, _P]),
+    "rgExtBvhBuildBatch": (_RES, [_P, _P, _P, _U32, C.POINTER(RgExtBvhBuildParams), _P]),
+    "rgExtBvhGetSceneRoots": (_RES, [_P, _P, _U32, C.POINTER(_U32)]),
+    "rgExtTraceRaysBatch": (_RES, [_P, _P, _P, _P, _U32, _P]),
+    "rgExtTraceAutotune": (_RES, [_P, _P, _U32, _P, _P, _U32, _P, C.POINTER(RgExtTraceTuning)]),
     "rgExtBvhGetTraceStats": (_RES, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "rgExtBvhGetTraceErrors": (_RES, [_P, C.POINTER(C.c_uint32)]),
     "rgExtLuminanceHistogram": (_RES, [_P, _U32, _U32, _F, _F, _P, _P]),
@@ -274,6 +283,32 @@ class Bvh:
 
     def trace(self, d_rays: DeviceArray, count: int, d_hits: DeviceArray, flags: int = 0, stream=None) -> None:
         call("rgExtTraceRays", self.h, d_rays.ptr, count, d_hits.ptr, flags, stream)
+
+    def build_batch(self, d_tris: DeviceArray, offsets, leaf_size: int = 1, node_format: int = RG_EXT_BVH_NODE_FP32,
+                    stream=None) -> None:
+        """Scenes concatenated in d_tris; scene s is triangles [offsets[s], offsets[s+1])."""
+        offs = np.ascontiguousarray(offsets, np.uint32)
+        params = RgExtBvhBuildParams(leaf_size, node_format)
+        call("rgExtBvhBuildBatch", self.h, d_tris.ptr, offs.ctypes.data, len(offs) - 1, C.byref(params), stream)
+
+    def scene_roots(self) -> np.ndarray:
+        n = C.c_uint32()
+        call("rgExtBvhGetSceneRoots", self.h, None, 0, C.byref(n))
+        out = np.zeros(max(1, n.value), np.uint32)
+        call("rgExtBvhGetSceneRoots", self.h, out.ctypes.data, n.value, C.byref(n))
+        return out[:n.value]
+
+    def trace_batch(self, d_rays: DeviceArray, ray_offsets, d_hits: DeviceArray, flags: int = 0, stream=None) -> None:
+        offs = np.ascontiguousarray(ray_offsets, np.uint32)
+        call("rgExtTraceRaysBatch", self.h, d_rays.ptr, offs.ctypes.data, d_hits.ptr, flags, stream)
+
+    def autotune(self, d_rays: DeviceArray, count: int, d_hits: DeviceArray, flags: int = 0, ray_offsets=None,
+                 stream=None) -> RgExtTraceTuning:
+        t = RgExtTraceTuning()
+        offs = None if ray_offsets is None else np.ascontiguousarray(ray_offsets, np.uint32)
+        call("rgExtTraceAutotune", self.h, d_rays.ptr, count, None if offs is None else offs.ctypes.data, d_hits.ptr, flags,
+             stream, C.byref(t))
+        return t
 
     def trace_stats(self) -> tuple:
         """(node visits, triangle tests) of the last trace with RG_EXT_TRACE_COLLECT_STATS."""
