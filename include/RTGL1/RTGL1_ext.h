@@ -178,6 +178,12 @@ RGAPI RgResult RGCONV rgExtTraceRaysBatch( RgExtBvh        bvh,
                                            uint32_t        flags,
                                            void*           stream );
 
+/* Primary rays of a camera into d_rays (renderWidth x renderHeight rays,
+ * row-major): getRayDir over pixel centres plus jitter from invView /
+ * invProjection, origin cameraPosition, [primaryRayMinDist, rayLength]
+ * (RaygenPrimary.inl:300-314, ShaderCommonGLSLFunc.h:460-478). */
+RGAPI RgResult RGCONV rgExtPrimaryRays( const RgExtFrameParams* pParams, RgExtRay* d_rays, void* stream );
+
 /* Traversal tuning chosen by rgExtTraceAutotune. */
 typedef struct RgExtTraceTuning
 {

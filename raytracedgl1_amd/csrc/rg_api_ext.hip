@@ -4,6 +4,7 @@
 #include "rg_denoise.h"
 #include "rg_internal.h"
 #include "rg_kernels.h"
+#include "rg_pathtrace.h"
 
 #include <cmath>
 #include <memory>
@@ -289,6 +290,14 @@ RgResult rgExtTraceRaysBatch( RgExtBvh bvh, const RgExtRay* d_rays, const uint32
         RG_REQUIRE( ( flags & ~7u ) == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "unknown trace flags" );
         RG_REQUIRE( ( d_rays && d_hits ) || rayOffsets[ bvh->bvh.sceneCount ] == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
         TraceRaysBatch( bvh->bvh, bvh->trace, d_rays, rayOffsets, d_hits, flags, AsStream( stream ) );
+    } );
+}
+
+RgResult rgExtPrimaryRays( const RgExtFrameParams* pParams, RgExtRay* d_rays, void* stream )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( pParams && d_rays, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        PrimaryRays( *pParams, d_rays, AsStream( stream ) );
     } );
 }
 

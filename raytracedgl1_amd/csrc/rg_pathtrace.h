@@ -16,6 +16,10 @@ struct FrameTimings
     uint64_t rays = 0;
 };
 
+// Primary rays of a camera (p's inverse view / projection, render size,
+// jitter, primaryRayMinDist, rayLength), row-major over regular pixels.
+void PrimaryRays( const RgExtFrameParams& p, RgExtRay* rays, hipStream_t s );
+
 // Ray, hit and per-pixel state buffers between the wavefronts of one frame.
 // Nothing in them survives a frame, so path tracers whose frames run one after
 // another on one stream can share one set.

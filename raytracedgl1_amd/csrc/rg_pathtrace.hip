@@ -1524,6 +1524,14 @@ const uint32_t* SharedBlueNoise( hipStream_t s )
 }
 } // namespace
 
+void PrimaryRays( const RgExtFrameParams& p, RgExtRay* rays, hipStream_t s )
+{
+    RG_REQUIRE( p.renderWidth > 0 && p.renderHeight > 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "render size must be > 0" );
+    const dim3 grid( DivUp( p.renderWidth, BX ), DivUp( p.renderHeight, BY ) );
+    k_primary_rays<<< grid, THREADS, 0, s >>>( p, rays );
+    CheckLaunch( "k_primary_rays" );
+}
+
 size_t Wavefront::BytesFor( size_t n )
 {
     return n * ( 2 * sizeof( RgExtRay ) + 2 * sizeof( RgExtHit ) + 16 * 6 + sizeof( RgExtRay ) + sizeof( RgExtHit ) );

@@ -132,6 +132,7 @@ _SIGS = {
     "rgExtBvhBuildBatch": (_RES, [_P, _P, _P, _U32, C.POINTER(RgExtBvhBuildParams), _P]),
     "rgExtBvhGetSceneRoots": (_RES, [_P, _P, _U32, C.POINTER(_U32)]),
     "rgExtTraceRaysBatch": (_RES, [_P, _P, _P, _P, _U32, _P]),
+    "rgExtPrimaryRays": (_RES, [C.POINTER(RgExtFrameParams), _P, _P]),
     "rgExtTraceAutotune": (_RES, [_P, _P, _U32, _P, _P, _U32, _P, C.POINTER(RgExtTraceTuning)]),
     "rgExtBvhGetTraceStats": (_RES, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "rgExtBvhGetTraceErrors": (_RES, [_P, C.POINTER(C.c_uint32)]),
