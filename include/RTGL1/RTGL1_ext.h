@@ -178,12 +178,6 @@ RGAPI RgResult RGCONV rgExtTraceRaysBatch( RgExtBvh        bvh,
                                            uint32_t        flags,
                                            void*           stream );
 
-/* Primary rays of a camera into d_rays (renderWidth x renderHeight rays,
- * row-major): getRayDir over pixel centres plus jitter from invView /
- * invProjection, origin cameraPosition, [primaryRayMinDist, rayLength]
- * (RaygenPrimary.inl:300-314, ShaderCommonGLSLFunc.h:460-478). */
-RGAPI RgResult RGCONV rgExtPrimaryRays( const RgExtFrameParams* pParams, RgExtRay* d_rays, void* stream );
-
 /* Traversal tuning chosen by rgExtTraceAutotune. */
 typedef struct RgExtTraceTuning
 {
@@ -379,6 +373,13 @@ typedef struct RgExtFrameParams
 /* Fills an RgExtFrameParams with the reference defaults
  * (Source/DrawFrameInfo.h:34-214) for a width x height render. */
 RGAPI RgResult RGCONV rgExtFrameParamsDefaults( uint32_t width, uint32_t height, RgExtFrameParams* pOut );
+
+/* Primary rays of a camera into d_rays (renderWidth x renderHeight rays,
+ * row-major): getRayDir over pixel centres plus jitter from invView /
+ * invProjection, origin cameraPosition, [primaryRayMinDist, rayLength]
+ * (RaygenPrimary.inl:300-314, ShaderCommonGLSLFunc.h:460-478). */
+RGAPI RgResult RGCONV rgExtPrimaryRays( const RgExtFrameParams* pParams, RgExtRay* d_rays, void* stream );
+
 
 /* A-SVGF passes, in the order of Denoiser::Denoise (Source/Denoiser.cpp:64-255). */
 /* CmASVGFGradientAtrous.comp:42-105, iteration 0..3 (steps 1,2,4,8). */
