@@ -24,8 +24,11 @@ Beside the FP32 line (`value`), the same JSON object carries
   200k-2M triangles, admitted against a 200 GiB budget, with Poisson frame
   requests at 30 Hz per session for a few seconds: p50 / p99 frame latency
   and frames/s at that load, then at a sustainable number of sessions.
+- `batch`: 64 scenes of 10k-2M triangles built as one batch and as 64
+  builds, and their primary rays (one camera per scene, 256x256 .. 1080p)
+  traced in one launch and as 64, before and after the traversal autotuner.
 
-Usage: python bench.py [--gpus 1] [--steps 60] [--warmup 5] [--no-cpu-baseline] [--no-fp16] [--no-serving]
+Usage: python bench.py [--gpus 1] [--steps 60] [--warmup 5] [--no-cpu-baseline] [--no-fp16] [--no-serving] [--no-batch]
 """
 from __future__ import annotations
 
@@ -65,6 +68,8 @@ def parse_args(argv=None):
     ap.add_argument("--serving-sessions", type=int, default=256)
     ap.add_argument("--serving-scenes", type=int, default=16)
     ap.add_argument("--serving-seconds", type=float, default=5.0)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes run")
+    ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="instead of the frame benchmark, run 5 plain device copies of MIB MiB (k_copy16): the "
                          "known-size reference for scaling HBM counter readings")
@@ -258,6 +263,108 @@ def serving_run(a) -> dict:
         srv.close()
 
 
+def batch_run(a) -> dict:
+    """Batched scenes: 64 soups of 10k-2M triangles (log-uniform), one camera
+    each at a resolution from {256x256, 640x360, 1280x720, 1920x1080}.  The
+    scenes are built as one batch (one pass sequence over every triangle) and
+    as 64 separate builds; their primary rays are traced in one persistent
+    launch over all scenes and as 64 launches; the traversal autotuner then
+    picks the batch's stack depth and ray chunk."""
+    import ctypes as C
+
+    import numpy as np
+
+    from raytracedgl1_amd import ffi, scenes
+
+    rng = np.random.default_rng(scenes.SEED_SCENE_BATCH)
+    S = a.batch_scenes
+    counts = np.exp(rng.uniform(np.log(1e4), np.log(2e6), S)).astype(np.int64)
+    resl = ((256, 256), (640, 360), (1280, 720), (1920, 1080))
+    res = [resl[k] for k in rng.integers(0, 4, S)]
+    t = time.perf_counter()
+    tris = np.concatenate([scenes.soup_tris(scenes.SEED_SCENE_BATCH + 1 + i, int(c)) for i, c in enumerate(counts)])
+    offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint32)
+    print(f"batch: {S} scenes, {len(tris)} triangles generated in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+    d_tris = ffi.DeviceArray.from_numpy(tris)
+    del tris
+    params = ffi.RgExtBvhBuildParams(1, ffi.RG_EXT_BVH_NODE_FP32)
+
+    bb = ffi.Bvh()
+    bb.build_batch(d_tris, offs, 1)  # warm (allocates)
+    ffi.call("rgExtSynchronize")
+    t0 = time.perf_counter()
+    bb.build_batch(d_tris, offs, 1)
+    ffi.call("rgExtSynchronize")
+    t_batch_build = time.perf_counter() - t0
+
+    singles = [ffi.Bvh() for _ in range(S)]
+    for k in range(S):  # warm
+        ffi.call("rgExtBvhBuild", singles[k].h, d_tris.ptr + int(offs[k]) * 36, int(counts[k]), C.byref(params), None)
+    ffi.call("rgExtSynchronize")
+    t0 = time.perf_counter()
+    for k in range(S):
+        ffi.call("rgExtBvhBuild", singles[k].h, d_tris.ptr + int(offs[k]) * 36, int(counts[k]), C.byref(params), None)
+    ffi.call("rgExtSynchronize")
+    t_single_build = time.perf_counter() - t0
+    print(f"batch build {t_batch_build * 1e3:.1f} ms, 64 single builds {t_single_build * 1e3:.1f} ms", file=sys.stderr, flush=True)
+
+    roffs = np.concatenate([[0], np.cumsum([w * h for w, h in res])]).astype(np.uint32)
+    n_rays = int(roffs[-1])
+    d_rays = ffi.DeviceArray(n_rays, ffi.RAY_DTYPE)
+    d_hits = ffi.DeviceArray(n_rays, ffi.HIT_DTYPE)
+    for k, (w, h) in enumerate(res):
+        ang = rng.uniform(0, 2 * np.pi)
+        eye = np.array([180 * np.cos(ang), rng.uniform(20, 80), 180 * np.sin(ang)])
+        cam = scenes.Camera(eye, np.zeros(3))
+        p = ffi.frame_params(w, h)
+        view = cam.view().astype(np.float32)
+        proj = cam.proj(w, h)
+        iv = np.linalg.inv(scenes.mat_cm(view)).T.reshape(16).astype(np.float32)
+        ip = np.linalg.inv(scenes.mat_cm(proj)).T.reshape(16).astype(np.float32)
+        p.invView[:] = iv.tolist()
+        p.invProjection[:] = ip.tolist()
+        p.cameraPosition[:] = [float(iv[12]), float(iv[13]), float(iv[14]), 0.0]
+        p.rayLength = 10000.0
+        p.primaryRayMinDist = 0.1
+        ffi.call("rgExtPrimaryRays", C.byref(p), d_rays.ptr + int(roffs[k]) * 32, None)
+
+    def timed(fn, reps=3):
+        fn()
+        ffi.call("rgExtSynchronize")
+        best = 1e30
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ffi.call("rgExtSynchronize")
+            best = min(best, time.perf_counter() - t0)
+        return best
+
+    t_batch_trace = timed(lambda: bb.trace_batch(d_rays, roffs, d_hits, 0))
+
+    def singles_trace():
+        for k in range(S):
+            ffi.call("rgExtTraceRays", singles[k].h, d_rays.ptr + int(roffs[k]) * 32, int(roffs[k + 1] - roffs[k]),
+                     d_hits.ptr + int(roffs[k]) * 16, 0, None)
+
+    t_single_trace = timed(singles_trace)
+    tune = bb.autotune(d_rays, n_rays, d_hits, 0, roffs)
+    t_tuned_trace = timed(lambda: bb.trace_batch(d_rays, roffs, d_hits, 0))
+    hits = np.frombuffer(ffi.download_raw(d_hits.ptr, n_rays * 16, np.uint32).tobytes(), np.uint32).reshape(-1, 4)
+    hit_frac = float((hits[:, 3] != ffi.RG_EXT_MISS).mean())
+    print(f"batch trace {t_batch_trace * 1e3:.1f} ms (tuned {t_tuned_trace * 1e3:.1f}), 64 launches {t_single_trace * 1e3:.1f} ms",
+          file=sys.stderr, flush=True)
+    return {
+        "scenes": S, "triangles": int(offs[-1]), "rays": n_rays, "hit_fraction": hit_frac,
+        "resolutions": [f"{w}x{h}" for w, h in res],
+        "build_ms": {"batched": t_batch_build * 1e3, "separate": t_single_build * 1e3},
+        "build_mtris_per_s": {"batched": offs[-1] / t_batch_build / 1e6, "separate": offs[-1] / t_single_build / 1e6},
+        "trace_ms": {"batched": t_batch_trace * 1e3, "batched_tuned": t_tuned_trace * 1e3, "separate": t_single_trace * 1e3},
+        "trace_mrays_per_s": {"batched": n_rays / t_batch_trace / 1e6, "batched_tuned": n_rays / t_tuned_trace / 1e6,
+                              "separate": n_rays / t_single_trace / 1e6},
+        "autotune": {"stack_lds_entries": tune.stackLdsEntries, "ray_chunk": tune.rayChunk},
+    }
+
+
 def load_traffic():
     try:
         d = json.loads(TRAFFIC_FILE.read_text())
@@ -367,6 +474,8 @@ def main(argv=None) -> int:
         }
     if not a.no_serving:
         out["serving"] = serving_run(a)
+    if not a.no_batch:
+        out["batch"] = batch_run(a)
     if not a.no_cpu_baseline:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(scene, rays / n, threads)

@@ -185,6 +185,31 @@ def icosphere(level: int) -> np.ndarray:
     return _ICO_CACHE[level]
 
 
+def soup_tris(seed: int, n: int, extent: float = 100.0) -> np.ndarray:
+    """n triangles of a random soup in a 2*extent cube: icospheres of
+    subdivision 2 (radii U[0.5,3]) for about half, boxes (sizes U[0.2,2])
+    for the rest (the batched-scenes workload)."""
+    r = np.random.default_rng(seed)
+    unit = icosphere(2).astype(np.float32)  # (320, 3, 3)
+    ns = (n // 2) // len(unit)
+    nb = (n - ns * len(unit)) // 12
+    parts = []
+    if ns:
+        c = r.uniform(-extent, extent, (ns, 1, 1, 3)).astype(np.float32)
+        rad = r.uniform(0.5, 3.0, (ns, 1, 1, 1)).astype(np.float32)
+        parts.append((unit[None] * rad + c).reshape(-1, 9))
+    if nb:
+        ub = boxes(np.zeros((1, 3)), np.ones((1, 3)))[0].reshape(12, 3, 3)
+        c = r.uniform(-extent, extent, (nb, 1, 1, 3)).astype(np.float32)
+        sz = r.uniform(0.2, 2.0, (nb, 1, 1, 3)).astype(np.float32)
+        parts.append((ub[None] * sz + c).reshape(-1, 9))
+    left = n - sum(len(p) for p in parts)
+    if left:
+        c = r.uniform(-extent, extent, (left, 1, 3))
+        parts.append((c + r.uniform(-1.0, 1.0, (left, 3, 3))).reshape(-1, 9))
+    return np.concatenate(parts).astype(np.float32)
+
+
 def _finish(parts, objects, lights=None) -> Scene:
     tris = np.concatenate([p[0] for p in parts]).astype(np.float32)
     nrms = np.concatenate([p[1] for p in parts]).astype(np.float32)
