@@ -195,20 +195,25 @@ def session_camera(k: int, t: float) -> scenes.Camera:
 
 
 def run_poisson(server: Server, sessions: list, rate_hz: float, duration_s: float, seed: int = 0,
-                t_offset: float = 0.0) -> dict:
-    """Submits frame requests with Poisson arrivals at rate_hz per session for
-    duration_s seconds of wall time, then waits for the server to drain.
-    Latency of a finished frame = its completion - its request's submit."""
+                t_offset: float = 0.0, tail_s: float = 1.5) -> dict:
+    """Submits frame requests with Poisson arrivals at rate_hz per session.
+    Requests submitted in [0, duration_s) are measured; arrivals go on for
+    tail_s more seconds so that measured requests still meet newer ones (a
+    request left waiting at the end of the load would otherwise count the
+    drain as latency), then the server drains.  Latency of a finished frame
+    = its completion - its request's submit."""
     rng = np.random.default_rng(seed)
     n = len(sessions)
-    expected = int(rate_hz * duration_s * n * 1.2) + 16
-    gaps = rng.exponential(1.0 / rate_hz, size=(n, max(4, expected // max(1, n))))
+    span = duration_s + tail_s
+    expected = int(rate_hz * span * 1.3) + 16
+    gaps = rng.exponential(1.0 / rate_hz, size=(n, expected))
     times = np.cumsum(gaps, axis=1)
-    events = sorted((float(tt), k) for k in range(n) for tt in times[k] if tt < duration_s)
+    events = sorted((float(tt), k) for k in range(n) for tt in times[k] if tt < span)
     server.poll()
     t0 = time.perf_counter()
     results = []
     tag = 0
+    measured_tags = 0
     i = 0
     next_poll = 0.0
     while i < len(events):
@@ -216,6 +221,8 @@ def run_poisson(server: Server, sessions: list, rate_hz: float, duration_s: floa
         while i < len(events) and events[i][0] <= now:
             tt, k = events[i]
             server.submit(sessions[k], session_camera(k, tt + t_offset), tt + t_offset, tag)
+            if tt < duration_s:
+                measured_tags = tag + 1
             tag += 1
             i += 1
         if i < len(events):
@@ -227,6 +234,7 @@ def run_poisson(server: Server, sessions: list, rate_hz: float, duration_s: floa
             next_poll += 0.01
     server.wait_idle()
     results.extend(server.poll())
+    results = [r for r in results if r[1] < measured_tags]  # requests of the measured window
     done = [r for r in results if r[2] == RG_EXT_FRAME_DONE]
     lat = np.array([r[6] - r[4] for r in done]) * 1000.0
     first = min((r[4] for r in done), default=0.0)
@@ -236,7 +244,7 @@ def run_poisson(server: Server, sessions: list, rate_hz: float, duration_s: floa
         "sessions": n,
         "rate_hz_per_session": rate_hz,
         "offered_fps": rate_hz * n,
-        "requests": tag,
+        "requests": measured_tags,
         "frames_done": len(done),
         "frames_dropped": sum(1 for r in results if r[2] == RG_EXT_FRAME_DROPPED),
         "frames_failed": sum(1 for r in results if r[2] == RG_EXT_FRAME_FAILED),
