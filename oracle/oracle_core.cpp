@@ -765,9 +765,9 @@ extern "C" void or_trace_rays( const OrBvh* b, const RgExtRay* rays, uint32_t co
                         h++;
                     }
                 }
-                // visit near to far: order by the 64-bit key (distance as an
-                // order-preserving unsigned integer, +0 for -0; then the child
-                // reference), push far first
+                // closest hit visits near to far: order by the 64-bit key
+                // (distance as an order-preserving unsigned integer, +0 for
+                // -0; then the child reference), push far first
                 uint64_t key[ 8 ];
                 for( int k = 0; k < h; k++ )
                 {
@@ -777,7 +777,12 @@ extern "C" void or_trace_rays( const OrBvh* b, const RgExtRay* rays, uint32_t co
                 }
                 std::vector< int > ord( h );
                 for( int k = 0; k < h; k++ ) ord[ k ] = k;
-                std::sort( ord.begin(), ord.end(), [ & ]( int x, int y ) { return key[ x ] < key[ y ]; } );
+                // any hit: slot order, unsorted (the kernel's choice: the
+                // interval does not shrink before the ray ends)
+                if( !anyHit )
+                {
+                    std::sort( ord.begin(), ord.end(), [ & ]( int x, int y ) { return key[ x ] < key[ y ]; } );
+                }
                 for( int k = h - 1; k >= 0; k-- )
                 {
                     stack.push_back( cr[ ord[ k ] ] );

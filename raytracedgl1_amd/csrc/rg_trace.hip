@@ -287,22 +287,29 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
             if constexpr( !H )
             {
                 const float4* np = reinterpret_cast< const float4* >( reinterpret_cast< const BvhNode8* >( a.nodes ) + node );
-                float4        q[ 14 ];
+                // two halves of four children (7 loads each) keep fewer
+                // registers live than the whole node
 #pragma unroll
-                for( int i = 0; i < 14; i++ )
+                for( int half = 0; half < 2; half++ )
                 {
-                    q[ i ] = np[ i ];
-                }
-                const float* f = reinterpret_cast< const float* >( q ); // lo_x hi_x lo_y hi_y lo_z hi_z child
+                    float4 q[ 7 ];
 #pragma unroll
-                for( int k = 0; k < 8; k++ )
-                {
-                    const uint32_t c = __float_as_uint( f[ 48 + k ] );
-                    float          tn;
-                    const bool     inBox = SlabTest( ray, f[ k ], f[ 16 + k ], f[ 32 + k ], f[ 8 + k ], f[ 24 + k ], f[ 40 + k ], tmin, bestT, &tn );
-                    const bool     h     = inBox && c != RG_EXT_BVH_CHILD_EMPTY;
-                    key[ k ] = h ? SortKey( tn, c ) : ~0ull;
-                    hitCount += h ? 1 : 0;
+                    for( int i = 0; i < 7; i++ )
+                    {
+                        q[ i ] = np[ 2 * i + half ];
+                    }
+                    const float* f = reinterpret_cast< const float* >( q ); // lo_x hi_x lo_y hi_y lo_z hi_z child, 4 each
+#pragma unroll
+                    for( int j = 0; j < 4; j++ )
+                    {
+                        const int      k = half * 4 + j;
+                        const uint32_t c = __float_as_uint( f[ 24 + j ] );
+                        float          tn;
+                        const bool     inBox = SlabTest( ray, f[ j ], f[ 8 + j ], f[ 16 + j ], f[ 4 + j ], f[ 12 + j ], f[ 20 + j ], tmin, bestT, &tn );
+                        const bool     h     = inBox && c != RG_EXT_BVH_CHILD_EMPTY;
+                        key[ k ] = h ? SortKey( tn, c ) : ~0ull;
+                        hitCount += h ? 1 : 0;
+                    }
                 }
             }
             else
@@ -321,35 +328,59 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
                     hitCount += h ? 1 : 0;
                 }
             }
-            if( hitCount > 0 )
+            auto push = [ & ]( uint32_t v ) {
+                if( sp < STACK_LDS )
+                {
+                    s_stack[ sp * TR_THREADS + tid ] = v;
+                }
+                else if( sp < STACK_TOTAL )
+                {
+                    ovf[ sp - STACK_LDS ] = v;
+                }
+                else
+                {
+                    atomicOr( a.errorFlag, RG_EXT_TRACE_ERROR_STACK_OVERFLOW ); // dropped entry
+                    return;
+                }
+                sp++;
+            };
+            if( hitCount == 0 )
             {
+                node = RG_EXT_BVH_CHILD_EMPTY;
+            }
+            else if( !anyHit )
+            {
+                // closest hit: near to far, so nearer hits shrink the interval early
                 Sort8( key );
 #pragma unroll
                 for( int k = 7; k >= 1; k-- )
                 {
                     if( k < hitCount )
                     {
-                        if( sp < STACK_LDS )
-                        {
-                            s_stack[ sp * TR_THREADS + tid ] = uint32_t( key[ k ] );
-                        }
-                        else if( sp < STACK_TOTAL )
-                        {
-                            ovf[ sp - STACK_LDS ] = uint32_t( key[ k ] );
-                        }
-                        else
-                        {
-                            atomicOr( a.errorFlag, RG_EXT_TRACE_ERROR_STACK_OVERFLOW ); // dropped entry
-                            continue;
-                        }
-                        sp++;
+                        push( uint32_t( key[ k ] ) );
                     }
                 }
                 node = uint32_t( key[ 0 ] );
             }
             else
             {
-                node = RG_EXT_BVH_CHILD_EMPTY;
+                // any hit: the interval never shrinks before the ray ends, so
+                // the order does not change which nodes an unoccluded ray
+                // visits; children go in slot order, unsorted
+                uint32_t next = RG_EXT_BVH_CHILD_EMPTY;
+#pragma unroll
+                for( int k = 7; k >= 0; k-- )
+                {
+                    if( key[ k ] != ~0ull )
+                    {
+                        if( next != RG_EXT_BVH_CHILD_EMPTY )
+                        {
+                            push( next );
+                        }
+                        next = uint32_t( key[ k ] );
+                    }
+                }
+                node = next;
             }
         }
         else
