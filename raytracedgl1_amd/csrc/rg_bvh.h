@@ -66,7 +66,9 @@ struct BvhObject
     DeviceBuffer rangeFirst;   // (n-1) u32
     DeviceBuffer rangeLast;    // (n-1) u32
     DeviceBuffer nodeBounds;   // (2n-1) * 6 f32
-    DeviceBuffer refitCounter; // (n-1) u32
+    DeviceBuffer depthA, depthB, jumpA, jumpB; // (2n-1) u32: node depths by pointer jumping
+    DeviceBuffer levelHist;    // per-depth counts, then offsets and cursors
+    DeviceBuffer levelOrder;   // (n-1) internal nodes grouped by depth
     DeviceBuffer expansion;    // max(1,n-1) * 8 u32
     DeviceBuffer isNode8;      // max(1,n-1) u32
     DeviceBuffer node8Index;   // max(1,n-1) u32
@@ -75,6 +77,7 @@ struct BvhObject
     DeviceBuffer nodes;        // nodeCount * stride
     DeviceBuffer nodes32;      // FP32 nodes an FP16 build converts from
     uint32_t*    hostCounts = nullptr; // pinned
+    uint32_t*    hostLevels = nullptr; // pinned per-depth node counts
     // batched build: per-scene offsets, binary and 8-wide roots, bounds
     uint32_t                sceneCount = 0;
     DeviceBuffer            sceneOffsets, sceneRoots, sceneRoots8, sceneBounds, scenePartials;
@@ -89,6 +92,10 @@ struct BvhObject
         if( hostCounts )
         {
             ( void )hipHostFree( hostCounts );
+        }
+        if( hostLevels )
+        {
+            ( void )hipHostFree( hostLevels );
         }
     }
 

@@ -84,18 +84,39 @@ __global__ __launch_bounds__( THREADS ) void k_centroid_bounds( const float* tri
     }
 }
 
-__global__ void k_bounds_final( const double* partials, uint32_t blocks, double* out )
+// One block of THREADS: strided partial reductions, then an LDS tree.
+__global__ __launch_bounds__( THREADS ) void k_bounds_final( const double* partials, uint32_t blocks, double* out )
 {
+    __shared__ double s[ 6 ][ THREADS ];
+    double            v[ 6 ] = { DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX };
+    for( uint32_t b = threadIdx.x; b < blocks; b += THREADS )
+    {
+        for( int k = 0; k < 3; k++ )
+        {
+            v[ k ]     = fmin( v[ k ], partials[ b * 6 + k ] );
+            v[ 3 + k ] = fmax( v[ 3 + k ], partials[ b * 6 + 3 + k ] );
+        }
+    }
+    for( int k = 0; k < 6; k++ )
+    {
+        s[ k ][ threadIdx.x ] = v[ k ];
+    }
+    __syncthreads();
+    for( int stride = THREADS / 2; stride > 0; stride >>= 1 )
+    {
+        if( int( threadIdx.x ) < stride )
+        {
+            for( int k = 0; k < 3; k++ )
+            {
+                s[ k ][ threadIdx.x ]     = fmin( s[ k ][ threadIdx.x ], s[ k ][ threadIdx.x + stride ] );
+                s[ 3 + k ][ threadIdx.x ] = fmax( s[ 3 + k ][ threadIdx.x ], s[ 3 + k ][ threadIdx.x + stride ] );
+            }
+        }
+        __syncthreads();
+    }
     if( threadIdx.x < 6 )
     {
-        const bool isMin = threadIdx.x < 3;
-        double     v     = isMin ? DBL_MAX : -DBL_MAX;
-        for( uint32_t b = 0; b < blocks; b++ )
-        {
-            double p = partials[ b * 6 + threadIdx.x ];
-            v        = isMin ? fmin( v, p ) : fmax( v, p );
-        }
-        out[ threadIdx.x ] = v;
+        out[ threadIdx.x ] = s[ threadIdx.x ][ 0 ];
     }
 }
 
@@ -467,13 +488,15 @@ __device__ __forceinline__ void StoreBox( float* b, v3 lo, v3 hi )
     b[ 5 ] = hi.z;
 }
 
-__global__ __launch_bounds__( THREADS ) void k_refit( const float*    tris,
-                                                      const uint32_t* sortedIdx,
-                                                      uint32_t        n,
-                                                      const uint32_t* children,
-                                                      const uint32_t* parents,
-                                                      float*          bounds,
-                                                      uint32_t*       counters )
+// Level-synchronous refit: the depth of every binary node by pointer
+// jumping over the parent links (7 rounds reach depth 128), a counting sort
+// of the internal nodes by depth, then one launch per level from the deepest
+// up, each node the union of its two children.  No atomics on the boxes and
+// no cross-wave ordering: min/max unions give the same boxes in any order.
+constexpr int      REFIT_JUMP_ROUNDS = 7;
+constexpr uint32_t REFIT_MAX_DEPTH   = 128;
+
+__global__ __launch_bounds__( THREADS ) void k_leaf_boxes( const float* tris, const uint32_t* sortedIdx, uint32_t n, float* bounds )
 {
     const uint32_t k = blockIdx.x * THREADS + threadIdx.x;
     if( k >= n )
@@ -482,32 +505,80 @@ __global__ __launch_bounds__( THREADS ) void k_refit( const float*    tris,
     }
     v3 a, b, c;
     LoadTri( tris, sortedIdx[ k ], a, b, c );
-    const uint32_t leafId = ( n - 1 ) + k;
-    StoreBox( bounds + size_t( leafId ) * 6, vmin( vmin( a, b ), c ), vmax( vmax( a, b ), c ) );
-    if( n == 1 )
+    StoreBox( bounds + size_t( ( n - 1 ) + k ) * 6, vmin( vmin( a, b ), c ), vmax( vmax( a, b ), c ) );
+}
+
+// dist[i] = 1 and jump[i] = parent for every node but the root (0, itself)
+__global__ __launch_bounds__( THREADS ) void k_depth_init( const uint32_t* parents, uint32_t count, uint32_t* dist, uint32_t* jump )
+{
+    const uint32_t i = blockIdx.x * THREADS + threadIdx.x;
+    if( i >= count )
     {
         return;
     }
+    const uint32_t p    = parents[ i ];
+    const bool     root = i == 0 || p == 0xFFFFFFFFu;
+    dist[ i ]           = root ? 0u : 1u;
+    jump[ i ]           = root ? 0u : p;
+}
 
-    uint32_t node = parents[ leafId ];
-    // A Karras tree is at most 96 levels deep (63-bit codes + 32-bit index
-    // tie-break); the cap only guarantees termination.
-    for( int guard = 0; node != 0xFFFFFFFFu && guard < 128; guard++ )
+__global__ __launch_bounds__( THREADS ) void k_depth_jump( uint32_t count, const uint32_t* distIn, const uint32_t* jumpIn, uint32_t* distOut,
+                                                           uint32_t* jumpOut )
+{
+    const uint32_t i = blockIdx.x * THREADS + threadIdx.x;
+    if( i >= count )
     {
-        // The first child to arrive stops; the second one (which sees the
-        // first one's box through the acquire) builds the union.
-        const uint32_t prev = __hip_atomic_fetch_add( &counters[ node ], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT );
-        if( prev == 0 )
-        {
-            return;
-        }
-        const float* l = bounds + size_t( children[ node * 2 + 0 ] ) * 6;
-        const float* r = bounds + size_t( children[ node * 2 + 1 ] ) * 6;
-        v3           lo = vmin( mk3( l[ 0 ], l[ 1 ], l[ 2 ] ), mk3( r[ 0 ], r[ 1 ], r[ 2 ] ) );
-        v3           hi = vmax( mk3( l[ 3 ], l[ 4 ], l[ 5 ] ), mk3( r[ 3 ], r[ 4 ], r[ 5 ] ) );
-        StoreBox( bounds + size_t( node ) * 6, lo, hi );
-        node = parents[ node ];
+        return;
     }
+    const uint32_t j = jumpIn[ i ];
+    distOut[ i ]     = distIn[ i ] + distIn[ j ];
+    jumpOut[ i ]     = jumpIn[ j ];
+}
+
+// Histogram of internal-node depths, privatised per block in LDS (one
+// global atomic per bin per block); counts[REFIT_MAX_DEPTH] flags a deeper
+// tree.  Also writes the (depth, node) pairs one radix pass groups by depth.
+__global__ __launch_bounds__( THREADS ) void k_depth_keys( uint32_t internal, const uint32_t* depth, uint32_t* counts, uint64_t* keys,
+                                                           uint32_t* vals )
+{
+    __shared__ uint32_t s[ REFIT_MAX_DEPTH + 1 ];
+    for( uint32_t b = threadIdx.x; b <= REFIT_MAX_DEPTH; b += THREADS )
+    {
+        s[ b ] = 0;
+    }
+    __syncthreads();
+    const uint32_t i = blockIdx.x * THREADS + threadIdx.x;
+    if( i < internal )
+    {
+        const uint32_t d = depth[ i ] < REFIT_MAX_DEPTH ? depth[ i ] : REFIT_MAX_DEPTH;
+        atomicAdd( &s[ d ], 1u );
+        keys[ i ] = d;
+        vals[ i ] = i;
+    }
+    __syncthreads();
+    for( uint32_t b = threadIdx.x; b <= REFIT_MAX_DEPTH; b += THREADS )
+    {
+        if( s[ b ] )
+        {
+            atomicAdd( &counts[ b ], s[ b ] );
+        }
+    }
+}
+
+__global__ __launch_bounds__( THREADS ) void k_refit_level( const uint32_t* order, uint32_t first, uint32_t count, const uint32_t* children,
+                                                            float* bounds )
+{
+    const uint32_t t = blockIdx.x * THREADS + threadIdx.x;
+    if( t >= count )
+    {
+        return;
+    }
+    const uint32_t node = order[ first + t ];
+    const float*   l    = bounds + size_t( children[ node * 2 + 0 ] ) * 6;
+    const float*   r    = bounds + size_t( children[ node * 2 + 1 ] ) * 6;
+    const v3       lo   = vmin( mk3( l[ 0 ], l[ 1 ], l[ 2 ] ), mk3( r[ 0 ], r[ 1 ], r[ 2 ] ) );
+    const v3       hi   = vmax( mk3( l[ 3 ], l[ 4 ], l[ 5 ] ), mk3( r[ 3 ], r[ 4 ], r[ 5 ] ) );
+    StoreBox( bounds + size_t( node ) * 6, lo, hi );
 }
 
 // ------------------------------------------------------------ collapse
@@ -619,13 +690,16 @@ __global__ __launch_bounds__( THREADS ) void k_write_nodes( uint32_t        slot
                                                             const float*    bounds,
                                                             BvhNode8*       nodes )
 {
-    const uint32_t b = blockIdx.x * THREADS + threadIdx.x;
+    // one thread per (slot, child): the 8 threads of a node write its SoA
+    // arrays as contiguous 32-byte runs
+    const uint64_t t = uint64_t( blockIdx.x ) * THREADS + threadIdx.x;
+    const uint32_t b = uint32_t( t >> 3 );
+    const int      k = int( t & 7u );
     if( b >= slots || !isNode8[ b ] )
     {
         return;
     }
     BvhNode8& node = nodes[ node8Index[ b ] ];
-    for( int k = 0; k < 8; k++ )
     {
         const uint32_t c = expansion[ size_t( b ) * 8 + k ];
         if( c == 0xFFFFFFFFu )
@@ -634,7 +708,7 @@ __global__ __launch_bounds__( THREADS ) void k_write_nodes( uint32_t        slot
             node.hi_x[ k ] = node.hi_y[ k ] = node.hi_z[ k ] = -INFINITY;
             node.child[ k ] = RG_EXT_BVH_CHILD_EMPTY;
             node.pad[ k ]   = 0;
-            continue;
+            return;
         }
         const float* bb = bounds + size_t( c ) * 6;
         node.lo_x[ k ]  = bb[ 0 ];
@@ -807,7 +881,16 @@ void BuildBvhImpl( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtB
     bvh.rangeFirst.Reserve( size_t( slots ) * 4 );
     bvh.rangeLast.Reserve( size_t( slots ) * 4 );
     bvh.nodeBounds.Reserve( size_t( 2 * n - 1 ) * 24 );
-    bvh.refitCounter.Reserve( size_t( slots ) * 4 );
+    bvh.depthA.Reserve( size_t( 2 * n - 1 ) * 4 );
+    bvh.depthB.Reserve( size_t( 2 * n - 1 ) * 4 );
+    bvh.jumpA.Reserve( size_t( 2 * n - 1 ) * 4 );
+    bvh.jumpB.Reserve( size_t( 2 * n - 1 ) * 4 );
+    bvh.levelHist.Reserve( size_t( REFIT_MAX_DEPTH + 1 ) * 4 );
+    bvh.levelOrder.Reserve( size_t( slots ) * 4 );
+    if( !bvh.hostLevels )
+    {
+        RG_HIP_CHECK( hipHostMalloc( reinterpret_cast< void** >( &bvh.hostLevels ), ( REFIT_MAX_DEPTH + 1 ) * 4, hipHostMallocDefault ) );
+    }
     bvh.expansion.Reserve( size_t( slots ) * 32 );
     bvh.isNode8.Reserve( size_t( slots ) * 4 );
     bvh.node8Index.Reserve( size_t( slots ) * 4 );
@@ -829,7 +912,7 @@ void BuildBvhImpl( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtB
         // 1. FP64 centroid bounds.
         k_centroid_bounds<<< bndBlocks, THREADS, 0, stream >>>( d_tris, n, partials );
         CheckLaunch( "k_centroid_bounds" );
-        k_bounds_final<<< 1, 64, 0, stream >>>( partials, bndBlocks, sceneBounds );
+        k_bounds_final<<< 1, THREADS, 0, stream >>>( partials, bndBlocks, sceneBounds );
         CheckLaunch( "k_bounds_final" );
 
         // 2. Morton codes (input order) and the identity permutation.
@@ -859,7 +942,7 @@ void BuildBvhImpl( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtB
         // the whole-batch bounds are not used; keep the view's fields defined
         k_centroid_bounds<<< bndBlocks, THREADS, 0, stream >>>( d_tris, n, partials );
         CheckLaunch( "k_centroid_bounds" );
-        k_bounds_final<<< 1, 64, 0, stream >>>( partials, bndBlocks, sceneBounds );
+        k_bounds_final<<< 1, THREADS, 0, stream >>>( partials, bndBlocks, sceneBounds );
         CheckLaunch( "k_bounds_final" );
     }
 
@@ -895,9 +978,59 @@ void BuildBvhImpl( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtB
 
     // 5. Refit.
     float* bounds = bvh.nodeBounds.as< float >();
-    RG_HIP_CHECK( hipMemsetAsync( bvh.refitCounter.ptr, 0, size_t( slots ) * 4, stream ) );
-    k_refit<<< DivUp( n, THREADS ), THREADS, 0, stream >>>( d_tris, vin, n, children, parents, bounds, bvh.refitCounter.as< uint32_t >() );
-    CheckLaunch( "k_refit" );
+    k_leaf_boxes<<< DivUp( n, THREADS ), THREADS, 0, stream >>>( d_tris, vin, n, bounds );
+    CheckLaunch( "k_leaf_boxes" );
+    if( internal > 0 )
+    {
+        const uint32_t all  = 2 * n - 1;
+        uint32_t*      dA   = bvh.depthA.as< uint32_t >();
+        uint32_t*      dB   = bvh.depthB.as< uint32_t >();
+        uint32_t*      jA   = bvh.jumpA.as< uint32_t >();
+        uint32_t*      jB   = bvh.jumpB.as< uint32_t >();
+        k_depth_init<<< DivUp( all, THREADS ), THREADS, 0, stream >>>( parents, all, dA, jA );
+        CheckLaunch( "k_depth_init" );
+        for( int r = 0; r < REFIT_JUMP_ROUNDS; r++ )
+        {
+            k_depth_jump<<< DivUp( all, THREADS ), THREADS, 0, stream >>>( all, dA, jA, dB, jB );
+            CheckLaunch( "k_depth_jump" );
+            std::swap( dA, dB );
+            std::swap( jA, jB );
+        }
+        uint32_t* hist = bvh.levelHist.as< uint32_t >();
+        RG_HIP_CHECK( hipMemsetAsync( hist, 0, ( REFIT_MAX_DEPTH + 1 ) * 4, stream ) );
+        // (depth, node) pairs into the free key buffer and dB, then one
+        // stable 8-bit radix pass groups the nodes by depth into levelOrder
+        k_depth_keys<<< DivUp( internal, THREADS ), THREADS, 0, stream >>>( internal, dA, hist, kout, dB );
+        CheckLaunch( "k_depth_keys" );
+        RG_HIP_CHECK( hipMemcpyAsync( bvh.hostLevels, hist, ( REFIT_MAX_DEPTH + 1 ) * 4, hipMemcpyDeviceToHost, stream ) );
+        const uint32_t lvBlocks = DivUp( internal, RS_TILE );
+        k_radix_hist<<< lvBlocks, THREADS, 0, stream >>>( kout, internal, 0, bvh.hist.as< uint32_t >(), lvBlocks );
+        CheckLaunch( "k_radix_hist" );
+        ExclusiveScan( bvh.hist.as< uint32_t >(), 256 * lvBlocks, bvh.hist.as< uint32_t >(), nullptr, bvh.scanScratch.as< uint32_t >(), stream );
+        k_radix_scatter<<< lvBlocks, THREADS, 0, stream >>>( kout, dB, kin, bvh.levelOrder.as< uint32_t >(), internal, 0, bvh.hist.as< uint32_t >(),
+                                                             lvBlocks );
+        CheckLaunch( "k_radix_scatter" );
+        bvh.sortedKeys = nullptr; // the sorted Morton keys were overwritten
+        RG_HIP_CHECK( hipStreamSynchronize( stream ) );
+        RG_REQUIRE( bvh.hostLevels[ REFIT_MAX_DEPTH ] == 0, RG_RESULT_INTERNAL_ERROR, "BVH deeper than the refit level limit" );
+        uint32_t first[ REFIT_MAX_DEPTH ], sum = 0;
+        int      deepest = 0;
+        for( uint32_t d = 0; d < REFIT_MAX_DEPTH; d++ )
+        {
+            first[ d ] = sum;
+            sum += bvh.hostLevels[ d ];
+            deepest = bvh.hostLevels[ d ] ? int( d ) : deepest;
+        }
+        for( int d = deepest; d >= 0; d-- )
+        {
+            const uint32_t c = bvh.hostLevels[ d ];
+            if( c )
+            {
+                k_refit_level<<< DivUp( c, THREADS ), THREADS, 0, stream >>>( bvh.levelOrder.as< uint32_t >(), first[ d ], c, children, bounds );
+                CheckLaunch( "k_refit_level" );
+            }
+        }
+    }
 
     // 6. Collapse to 8-wide nodes, level by level.
     uint32_t* counts = bvh.levelCounts.as< uint32_t >();
@@ -957,7 +1090,7 @@ void BuildBvhImpl( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtB
     const bool    half    = params.nodeFormat == RG_EXT_BVH_NODE_FP16;
     DeviceBuffer& fp32Out = half ? bvh.nodes32 : bvh.nodes;
     fp32Out.Reserve( size_t( bvh.nodeCount ) * sizeof( BvhNode8 ) );
-    k_write_nodes<<< DivUp( slots, THREADS ), THREADS, 0, stream >>>( slots, n, bvh.leafMax, bvh.isNode8.as< uint32_t >(), bvh.node8Index.as< uint32_t >(), bvh.expansion.as< uint32_t >(), rf, rl, bounds, fp32Out.as< BvhNode8 >() );
+    k_write_nodes<<< DivUp( slots * 8u, THREADS ), THREADS, 0, stream >>>( slots, n, bvh.leafMax, bvh.isNode8.as< uint32_t >(), bvh.node8Index.as< uint32_t >(), bvh.expansion.as< uint32_t >(), rf, rl, bounds, fp32Out.as< BvhNode8 >() );
     CheckLaunch( "k_write_nodes" );
     if( half )
     {

@@ -9,6 +9,10 @@
 #include "rg_device_math.h"
 #include "rg_scene.h"
 
+#include <algorithm>
+#include <thread>
+#include <vector>
+
 #include <cmath>
 
 namespace rg
@@ -88,6 +92,41 @@ void SceneCollector::BeginFrame()
     hasDirectional = false;
 }
 
+namespace
+{
+// Copies into pinned staging: a large primitive (a 524k-triangle terrain is
+// 100 MB of vertices) is split over a few threads, since one thread's
+// memcpy runs at a fraction of the host's memory bandwidth.
+void StagingCopy( void* dst, const void* src, size_t bytes )
+{
+    constexpr size_t SPLIT = size_t( 8 ) << 20;
+    const unsigned   hw    = std::thread::hardware_concurrency();
+    const unsigned   T     = bytes >= SPLIT ? std::min( 8u, hw > 1 ? hw : 1u ) : 1u;
+    if( T <= 1 )
+    {
+        std::memcpy( dst, src, bytes );
+        return;
+    }
+    const size_t             chunk = ( bytes / T + 63 ) & ~size_t( 63 );
+    std::vector< std::thread > pool;
+    for( unsigned t = 1; t < T; t++ )
+    {
+        const size_t off = chunk * t;
+        if( off >= bytes )
+        {
+            break;
+        }
+        const size_t len = std::min( chunk, bytes - off );
+        pool.emplace_back( [ = ] { std::memcpy( static_cast< char* >( dst ) + off, static_cast< const char* >( src ) + off, len ); } );
+    }
+    std::memcpy( dst, src, std::min( chunk, bytes ) );
+    for( auto& th : pool )
+    {
+        th.join();
+    }
+}
+} // namespace
+
 void SceneCollector::AddPrimitive( const RgMeshInfo& mesh, const RgMeshPrimitiveInfo& prim )
 {
     if( prim.vertexCount == 0 || prim.pVertices == nullptr )
@@ -157,11 +196,11 @@ void SceneCollector::AddPrimitive( const RgMeshInfo& mesh, const RgMeshPrimitive
         g.isStatic = 1;
     }
 
-    std::memcpy( vertices.Append( prim.vertexCount ), prim.pVertices, size_t( prim.vertexCount ) * sizeof( RgPrimitiveVertex ) );
+    StagingCopy( vertices.Append( prim.vertexCount ), prim.pVertices, size_t( prim.vertexCount ) * sizeof( RgPrimitiveVertex ) );
     if( prim.pIndices && prim.indexCount > 0 )
     {
         g.baseIndex = uint32_t( indices.size );
-        std::memcpy( indices.Append( prim.indexCount ), prim.pIndices, size_t( prim.indexCount ) * 4 );
+        StagingCopy( indices.Append( prim.indexCount ), prim.pIndices, size_t( prim.indexCount ) * 4 );
     }
     *geoms.Append( 1 ) = g;
     geomUids.push_back( uid );
