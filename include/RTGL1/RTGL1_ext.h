@@ -489,10 +489,12 @@ RGAPI RgResult RGCONV rgExtSetStaticGeometry( RgInstance instance, RgBool32 enab
  * buffers belong to a stream and serve every session that stream runs.
  * Sessions are admitted while their bytes fit memoryBudgetBytes.
  *
- * Scheduling: each session runs on one stream (round robin at creation), so
- * its frames stay in order.  A dispatcher thread launches the next waiting
- * frame of any session whose stream has fewer than framesInFlightPerStream
- * frames queued, and retires frames as their completion events fire.  A
+ * Scheduling: a session with a frame in flight stays on that frame's
+ * stream, so its frames stay in order; an idle session's next frame goes to
+ * the stream with the fewest frames queued.  A dispatcher thread launches
+ * the oldest waiting request whose stream has fewer than
+ * framesInFlightPerStream frames queued, and retires frames as their
+ * completion events fire.  A
  * request that is still waiting when its session submits again is replaced
  * by the newer one and reported as dropped (the newest camera wins).
  *

@@ -274,10 +274,31 @@ struct RgExtServer_T
             bool launched = false;
             for( auto it = ready.begin(); it != ready.end(); )
             {
-                RgExtSession_T* s  = *it;
-                StreamCtx&   st = *streams[ s->stream ];
+                RgExtSession_T* s = *it;
+                // A session with a frame in flight stays on its stream (its
+                // frames must stay ordered); an idle one moves to the stream
+                // with the fewest frames queued: its persistent planes are its
+                // own, the within-frame planes are taken from that stream.
+                uint32_t target = s->stream;
+                if( s->inflight == 0 )
+                {
+                    for( uint32_t k = 0; k < streams.size(); k++ )
+                    {
+                        if( streams[ k ]->inflight.size() < streams[ target ]->inflight.size() )
+                        {
+                            target = k;
+                        }
+                    }
+                }
+                StreamCtx& st = *streams[ target ];
                 if( st.inflight.size() < info.framesInFlightPerStream )
                 {
+                    if( target != s->stream )
+                    {
+                        s->stream         = target;
+                        s->fbs.fb.shared  = &st.planes;
+                        s->g.pt.wf        = &st.wf;
+                    }
                     it        = ready.erase( it );
                     s->queued = false;
                     Launch( *s, st );

@@ -213,16 +213,27 @@ class SceneUploader:
         self.mesh.transform = identity_transform()
         self.prim = RgMeshPrimitiveInfo()
 
-    def upload(self, inst, geometry: bool = True, prefix: str = "rg") -> None:
+    def upload(self, inst, geometry: bool = True, prefix: str = "rg", transforms: dict | None = None,
+               order=None) -> None:
         """Uploads the primitives (unless geometry is False) and the lights
         to an instance (prefix "rg": rgUploadMeshPrimitive, ...) or to a
-        server scene (prefix "rgExtScene": rgExtSceneUploadMeshPrimitive, ...)."""
+        server scene (prefix "rgExtScene": rgExtSceneUploadMeshPrimitive, ...).
+        transforms maps an object index to its 3x4 row-major model matrix
+        (identity otherwise); order lists the object indices in upload order."""
         l = api()
         mesh_fn = getattr(l, prefix + "UploadMeshPrimitive")
         dir_fn = getattr(l, prefix + "UploadDirectionalLight")
         sph_fn = getattr(l, prefix + "UploadSphericalLight")
-        for oid, v, color in (self.objects if geometry else []):
+        objs = [self.objects[i] for i in order] if order is not None else self.objects
+        for oid, v, color in (objs if geometry else []):
             self.mesh.uniqueObjectID = oid
+            m = (transforms or {}).get(oid)
+            if m is None:
+                self.mesh.transform = identity_transform()
+            else:
+                for r_ in range(3):
+                    for c_ in range(4):
+                        self.mesh.transform.matrix[r_][c_] = float(m[r_][c_])
             self.prim.primitiveIndexInMesh = 0
             self.prim.pVertices = v.ctypes.data
             self.prim.vertexCount = len(v)

@@ -323,3 +323,86 @@ def test_fp16_precision_form(boxes):
           f"max {rel.max():.2e}", flush=True)
     assert np.quantile(rel, 0.99) < 1e-2
     assert abs(g.mean() / e.mean() - 1.0) < 1e-3
+
+
+def test_moving_object_motion_vectors(boxes):
+    """A primitive whose transform changes between frames (matched by
+    uniqueObjectID and primitive index, UniqueID::MakeForPrimitive) gets
+    motion vectors of its own movement: the pixel's current world position
+    moved back by the previous transform, projected with the previous camera
+    (HitInfo.inl:303-313); static geometry under a static camera has none."""
+    print("api: motion vectors of a moving object", flush=True)
+    s, _ = boxes
+    up = rtgl.SceneUploader(s)
+    cam = scenes.terrain_boxes_camera()
+    W, H = 320, 180
+    r = rtgl.Renderer(W, H)
+    try:
+        r.frame(up, cam)
+        # move the box that covers the most pixels
+        fb0 = r.framebuffers()
+        # visibility buffer .y: local primitive << 12 | geometry index, and
+        # the uploader makes object k geometry k
+        v0 = fb0.download(PLANE["VISIBILITY_BUFFER"])[..., 1]
+        hit0 = fb0.download(PLANE["IS_SKY"])[..., 0] == 0
+        owner = (v0[hit0] & 0xFFF).astype(np.int64)
+        cnt = np.bincount(owner, minlength=len(s.objects))
+        cnt[0] = 0  # not the terrain
+        moved = int(np.argmax(cnt))
+        shift = [[1, 0, 0, 0.5], [0, 1, 0, 0.0], [0, 0, 1, 0.25]]
+        l = rtgl.api()
+        ffi.check("rgStartFrame", l.rgStartFrame(r.h, C.byref(rtgl.RgStartFrameInfo())))
+        up.upload(r.h, True, transforms={moved: shift})
+        di = rtgl.draw_info(cam, r.time + 1.0 / 60.0, C.cast(C.pointer(r.sky), C.c_void_p))
+        ffi.check("rgDrawFrame", l.rgDrawFrame(r.h, C.byref(di)))
+        p = r.frame_params()
+        fb = r.framebuffers()
+        vis, mot, pos = (fb.download(PLANE[k]) for k in ("VISIBILITY_BUFFER", "MOTION", "SURFACE_POSITION"))
+        sky = fb.download(PLANE["IS_SKY"])[..., 0] != 0
+    finally:
+        r.close()
+    on = (~sky) & ((vis[..., 1] & 0xFFF) == moved)
+    off = (~sky) & ~on
+    assert on.sum() > 20
+    assert np.abs(mot[off][:, :2]).max() < 1e-6
+    view, proj = scenes.mat_cm(p.view), scenes.mat_cm(p.projection)
+    vprev, pprev = scenes.mat_cm(p.viewPrev), scenes.mat_cm(p.projectionPrev)
+    P = pos[on][:, :3].astype(np.float64)
+    prev = P - np.array([0.5, 0.0, 0.25])
+
+    def screen(m_v, m_p, x):
+        h = np.concatenate([x, np.ones((len(x), 1))], 1) @ m_v.T @ m_p.T
+        return h[:, :2] / h[:, 3:4] * 0.5 + 0.5
+
+    expect = screen(vprev, pprev, prev) - screen(view, proj, P)
+    assert np.abs(expect).max() > 1e-3  # the object did move on screen
+    assert np.abs(mot[on][:, :2] - expect).max() < 2e-5
+
+
+def test_upload_order_does_not_change_the_image(boxes):
+    """The same primitives uploaded in reverse order give the same frame:
+    only triangle and geometry numbering change, and closest hits break
+    exact ties by primitive id, so at most a few edge pixels may differ."""
+    print("api: upload order invariance", flush=True)
+    s, up = boxes
+    cam = scenes.terrain_boxes_camera()
+    out = []
+    for order in (None, list(range(len(s.objects)))[::-1]):
+        r = rtgl.Renderer(320, 180)
+        try:
+            for f in range(2):
+                l = rtgl.api()
+                ffi.check("rgStartFrame", l.rgStartFrame(r.h, C.byref(rtgl.RgStartFrameInfo())))
+                up.upload(r.h, True, order=order)
+                r.time += 1.0 / 60.0
+                di = rtgl.draw_info(cam, r.time, C.cast(C.pointer(r.sky), C.c_void_p))
+                ffi.check("rgDrawFrame", l.rgDrawFrame(r.h, C.byref(di)))
+            fb = r.framebuffers()
+            out.append({k: fb.download(PLANE[k]).copy() for k in ("PRE_FINAL", "DEPTH_WORLD", "UNFILTERED_DIRECT")})
+        finally:
+            r.close()
+    a, b = out
+    for k in ("PRE_FINAL", "DEPTH_WORLD", "UNFILTERED_DIRECT"):
+        same = np.all(a[k] == b[k], axis=-1) if a[k].ndim == 3 else a[k] == b[k]
+        assert same.mean() >= 0.999, (k, same.mean())
+    assert abs(a["PRE_FINAL"][..., :3].mean() / b["PRE_FINAL"][..., :3].mean() - 1.0) < 1e-4
