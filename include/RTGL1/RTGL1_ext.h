@@ -368,6 +368,18 @@ typedef struct RgExtFrameParams
     uint32_t restirSpatialSamples;    /* 8 as the reference                   */
     uint32_t debugShowFlags;
     float    emissionMapBoost;    /* 100 as the reference                   */
+
+    /* Reflection / refraction chain (RgDrawFrameReflectRefractParams, filled
+     * as Source/VulkanDevice.cpp:306-339 does; defaults DrawFrameInfo.h:125-146). */
+    uint32_t reflectRefractMaxDepth;  /* 2; at most 16                        */
+    uint32_t cameraMediaType;         /* RgMediaType around the camera       */
+    float    indexOfRefractionGlass;  /* 1.52                                */
+    float    indexOfRefractionWater;  /* 1.33 (also acid)                    */
+    float    thinMediaWidth;          /* 0.1                                 */
+    float    pad0[ 3 ];
+    float    waterColorAndDensity[ 4 ]; /* (0.3, 0.73, 0.63, 0)             */
+    float    acidColorAndDensity[ 4 ];  /* (0.0, 0.66, 0.55, 10)            */
+    float    worldUp[ 4 ];              /* normalised RgInstanceCreateInfo::worldUp */
 } RgExtFrameParams;
 
 /* Fills an RgExtFrameParams with the reference defaults

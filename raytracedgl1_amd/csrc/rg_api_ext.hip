@@ -444,6 +444,16 @@ RgResult rgExtFrameParamsDefaults( uint32_t width, uint32_t height, RgExtFramePa
         p->restirTemporalSamples    = 1;
         p->restirSpatialSamples     = 8;
         p->emissionMapBoost         = 100.0f;
+        // RgDrawFrameReflectRefractParams defaults (DrawFrameInfo.h:125-146)
+        p->reflectRefractMaxDepth   = 2;
+        p->cameraMediaType          = RG_MEDIA_TYPE_VACUUM;
+        p->indexOfRefractionGlass   = 1.52f;
+        p->indexOfRefractionWater   = 1.33f;
+        p->thinMediaWidth           = 0.1f;
+        const float water[ 4 ] = { 0.3f, 0.73f, 0.63f, 0.0f }, acid[ 4 ] = { 0.0f, 0.66f, 0.55f, 10.0f };
+        std::memcpy( p->waterColorAndDensity, water, sizeof( water ) );
+        std::memcpy( p->acidColorAndDensity, acid, sizeof( acid ) );
+        p->worldUp[ 1 ] = 1.0f;
     } );
 }
 

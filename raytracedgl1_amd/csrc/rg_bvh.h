@@ -100,6 +100,21 @@ struct BvhObject
     }
 
     uint32_t NodeStride() const { return nodeFormat == RG_EXT_BVH_NODE_FP16 ? uint32_t( sizeof( BvhNode8H ) ) : uint32_t( sizeof( BvhNode8 ) ); }
+
+    // Frame BVH with a refracting range (BuildFrameBvh): device words
+    // { top node, opaque root, 0 }.  FRAME_SINGLE: one tree, every ray type
+    // sees every triangle; FRAME_SPLIT: primary and reflection/refraction
+    // rays start at the top node over both subtrees, shadow and bounce rays
+    // at the opaque root; FRAME_REFRACT_ONLY: no opaque triangles, so
+    // shadow and bounce rays miss.
+    enum FrameMode : uint32_t
+    {
+        FRAME_SINGLE       = 0,
+        FRAME_SPLIT        = 1,
+        FRAME_REFRACT_ONLY = 2,
+    };
+    uint32_t     frameMode = FRAME_SINGLE;
+    DeviceBuffer frameRoots;
 };
 
 void BuildBvh( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtBvhBuildParams& params, hipStream_t stream );
@@ -108,6 +123,14 @@ void BuildBvh( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtBvhBu
 // own root (hostSceneRoots) and never shares a node with another scene.
 void BuildBvhBatch( BvhObject& bvh, const float* d_tris, const uint32_t* offsets, uint32_t sceneCount, const RgExtBvhBuildParams& params,
                     hipStream_t stream );
+
+// The frame's BVH over n triangles whose last n - opaqueN refract (water,
+// glass, acid).  With at least two triangles on each side the two ranges
+// become separate subtrees of one batched build plus a top node over both
+// (FRAME_SPLIT), so rays that must not see refracting geometry start below
+// it (the reference's INSTANCE_MASK_REFRACT, RaygenCommon.h:103-170).
+// Otherwise a single tree (FRAME_SINGLE; FRAME_REFRACT_ONLY when opaqueN is 0).
+void BuildFrameBvh( BvhObject& bvh, const float* d_tris, uint32_t n, uint32_t opaqueN, const RgExtBvhBuildParams& params, hipStream_t stream );
 
 constexpr int TRACE_STACK_VARIANTS = 3;
 
@@ -139,6 +162,10 @@ struct TraceAccum
 // in ctx (read back by ReadTraceStats / ReadTraceErrors).
 void TraceRays( const BvhObject& bvh, TraceContext& ctx, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags, hipStream_t stream,
                 hipEvent_t evBefore = nullptr, hipEvent_t evAfter = nullptr, const TraceAccum* accum = nullptr );
+// TraceRays over a frame BVH (BuildFrameBvh); opaqueOnly rays skip the
+// refracting range.
+void TraceRaysFrame( const BvhObject& bvh, TraceContext& ctx, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags, bool opaqueOnly,
+                     hipStream_t stream, hipEvent_t evBefore, hipEvent_t evAfter, const TraceAccum* accum );
 // One persistent launch over the rays of every scene of a batched BVH: rays
 // of scene s are [rayOffsets[s], rayOffsets[s+1]) (host array, sceneCount + 1
 // entries); primitive ids in the hits are relative to the scene.

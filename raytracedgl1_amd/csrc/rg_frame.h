@@ -38,6 +38,7 @@ struct CameraState
     uint32_t         frameId    = 0;
     double           prevTime   = 0.0;
     bool             firstFrame = true;
+    float            worldUp[ 3 ] = { 0.0f, 1.0f, 0.0f }; // RgInstanceCreateInfo::worldUp
 
     CameraState();
 };

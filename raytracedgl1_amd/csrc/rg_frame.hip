@@ -134,6 +134,30 @@ void FillFrameParams( CameraState& cs, const RgDrawFrameInfo& di, uint32_t w, ui
     p.restirTemporalSamples   = ext && ext->restirTemporalSamples != 0xFFFFFFFFu ? ext->restirTemporalSamples : 1u;
     p.restirSpatialSamples    = ext && ext->restirSpatialSamples != 0xFFFFFFFFu ? ext->restirSpatialSamples : 8u;
     RG_REQUIRE( p.atrousIterations == 4 || p.atrousIterations == 5, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "atrousIterations must be 4 or 5" );
+
+    // Source/VulkanDevice.cpp:306-339
+    const auto* rr           = FindParams< RgDrawFrameReflectRefractParams >( di, RG_STRUCTURE_TYPE_REFLECTREFRACT );
+    const uint32_t media     = rr ? uint32_t( rr->typeOfMediaAroundCamera ) : uint32_t( RG_MEDIA_TYPE_VACUUM );
+    p.cameraMediaType        = media <= uint32_t( RG_MEDIA_TYPE_ACID ) ? media : uint32_t( RG_MEDIA_TYPE_VACUUM );
+    p.reflectRefractMaxDepth = std::min( 16u, rr ? rr->maxReflectRefractDepth : 2u );
+    p.indexOfRefractionGlass = std::max( 0.0f, rr ? rr->indexOfRefractionGlass : 1.52f );
+    p.indexOfRefractionWater = std::max( 0.0f, rr ? rr->indexOfRefractionWater : 1.33f );
+    p.thinMediaWidth         = std::max( 0.0f, rr ? rr->thinMediaWidth : 0.1f );
+    const float water[ 3 ] = { 0.3f, 0.73f, 0.63f }, acid[ 3 ] = { 0.0f, 0.66f, 0.55f };
+    for( int i = 0; i < 3; i++ )
+    {
+        p.waterColorAndDensity[ i ] = rr ? rr->waterColor.data[ i ] : water[ i ];
+        p.acidColorAndDensity[ i ]  = rr ? rr->acidColor.data[ i ] : acid[ i ];
+    }
+    p.waterColorAndDensity[ 3 ] = 0.0f;
+    p.acidColorAndDensity[ 3 ]  = std::max( 0.0f, rr ? rr->acidDensity : 10.0f );
+    // SceneImportExport's worldUp: SafeNormalize(worldUp, {0,1,0}) (Source/Scene.cpp:385)
+    const double ux = cs.worldUp[ 0 ], uy = cs.worldUp[ 1 ], uz = cs.worldUp[ 2 ];
+    const double ul = std::sqrt( ux * ux + uy * uy + uz * uz );
+    p.worldUp[ 0 ] = ul > 1e-12 ? float( ux / ul ) : 0.0f;
+    p.worldUp[ 1 ] = ul > 1e-12 ? float( uy / ul ) : 1.0f;
+    p.worldUp[ 2 ] = ul > 1e-12 ? float( uz / ul ) : 0.0f;
+    p.worldUp[ 3 ] = 0.0f;
 }
 
 void RenderSizeOf( const RgDrawFrameInfo& di, uint32_t* w, uint32_t* h )

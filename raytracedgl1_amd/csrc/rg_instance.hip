@@ -55,6 +55,10 @@ struct Instance
     Instance( const RgInstanceCreateInfo& ci )
     {
         info = ci;
+        for( int i = 0; i < 3; i++ )
+        {
+            cam.worldUp[ i ] = ci.worldUp.data[ i ];
+        }
         int n = 0;
         if( hipGetDeviceCount( &n ) != hipSuccess || n == 0 )
         {
@@ -164,14 +168,16 @@ struct Instance
         const bool changed = keep ? false : AssembleScene( dscene, s );
         RG_HIP_CHECK( hipEventRecord( ev[ ST_ASSEMBLED ], s ) );
         stats.bvhRebuilt = 0;
-        if( dscene.triCount > 0 && ( changed || !dscene.bvhValid || bvh.nodeFormat != nodeFmt ) )
+        if( dscene.triCount > 0 &&
+            ( changed || !dscene.bvhValid || bvh.nodeFormat != nodeFmt || dscene.builtOpaqueTriCount != dscene.opaqueTriCount ) )
         {
             // one triangle per leaf: measured fastest at 1M triangles (more
             // nodes, fewer triangle tests per leaf visit)
             RgExtBvhBuildParams bp = { 1, nodeFmt };
-            BuildBvh( bvh, dscene.tris.as< float >(), dscene.triCount, bp, s );
-            dscene.bvhValid      = true;
-            dscene.builtTriCount = dscene.triCount;
+            BuildFrameBvh( bvh, dscene.tris.as< float >(), dscene.triCount, dscene.opaqueTriCount, bp, s );
+            dscene.bvhValid            = true;
+            dscene.builtTriCount       = dscene.triCount;
+            dscene.builtOpaqueTriCount = dscene.opaqueTriCount;
             stats.bvhRebuilt     = 1;
         }
         RG_HIP_CHECK( hipEventRecord( ev[ ST_BVH ], s ) );

@@ -299,6 +299,39 @@ __global__ void k_seed_frontier_seg( const uint32_t* roots, uint32_t S, uint32_t
     }
 }
 
+// Node `top` over the 8-wide roots of S <= 8 scenes: each child box is the
+// union of that root's child boxes.
+__global__ void k_top_node( BvhNode8* nodes, const uint32_t* roots8, uint32_t S, uint32_t top )
+{
+    // roots8 entries index nodes below `top`; anything else stays an empty slot
+    const int k = threadIdx.x;
+    if( k >= 8 )
+    {
+        return;
+    }
+    BvhNode8& t = nodes[ top ];
+    float     lo[ 3 ] = { INFINITY, INFINITY, INFINITY }, hi[ 3 ] = { -INFINITY, -INFINITY, -INFINITY };
+    uint32_t  child   = RG_EXT_BVH_CHILD_EMPTY;
+    if( uint32_t( k ) < S && roots8[ k ] < top )
+    {
+        const BvhNode8& r = nodes[ roots8[ k ] ];
+        child             = roots8[ k ];
+        for( int j = 0; j < 8; j++ )
+        {
+            if( r.child[ j ] != RG_EXT_BVH_CHILD_EMPTY )
+            {
+                lo[ 0 ] = fminf( lo[ 0 ], r.lo_x[ j ] ), hi[ 0 ] = fmaxf( hi[ 0 ], r.hi_x[ j ] );
+                lo[ 1 ] = fminf( lo[ 1 ], r.lo_y[ j ] ), hi[ 1 ] = fmaxf( hi[ 1 ], r.hi_y[ j ] );
+                lo[ 2 ] = fminf( lo[ 2 ], r.lo_z[ j ] ), hi[ 2 ] = fmaxf( hi[ 2 ], r.hi_z[ j ] );
+            }
+        }
+    }
+    t.lo_x[ k ] = lo[ 0 ], t.lo_y[ k ] = lo[ 1 ], t.lo_z[ k ] = lo[ 2 ];
+    t.hi_x[ k ] = hi[ 0 ], t.hi_y[ k ] = hi[ 1 ], t.hi_z[ k ] = hi[ 2 ];
+    t.child[ k ] = child;
+    t.pad[ k ]   = 0;
+}
+
 __global__ void k_gather_roots8( const uint32_t* roots, uint32_t S, const uint32_t* node8Index, uint32_t* roots8 )
 {
     for( uint32_t s = threadIdx.x; s < S; s += blockDim.x )
@@ -848,7 +881,7 @@ namespace
 // S + 1 of them) switches steps 1-2 to per-scene bounds and keys and seeds
 // the collapse with every scene's root.
 void BuildBvhImpl( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtBvhBuildParams& params, const uint32_t* batchOffsets,
-                   uint32_t sceneCount, hipStream_t stream )
+                   uint32_t sceneCount, hipStream_t stream, bool topNode = false )
 {
     RG_REQUIRE( n > 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "triangle count must be > 0" );
     RG_REQUIRE( n < ( 1u << 28 ), RG_RESULT_WRONG_FUNCTION_ARGUMENT, "triangle count must be < 2^28" );
@@ -1089,17 +1122,11 @@ void BuildBvhImpl( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtB
 
     const bool    half    = params.nodeFormat == RG_EXT_BVH_NODE_FP16;
     DeviceBuffer& fp32Out = half ? bvh.nodes32 : bvh.nodes;
-    fp32Out.Reserve( size_t( bvh.nodeCount ) * sizeof( BvhNode8 ) );
+    const bool    withTop = batched && topNode;
+    RG_REQUIRE( !withTop || sceneCount <= 8, RG_RESULT_INTERNAL_ERROR, "a top node holds at most 8 scenes" );
+    fp32Out.Reserve( size_t( bvh.nodeCount + ( withTop ? 1u : 0u ) ) * sizeof( BvhNode8 ) );
     k_write_nodes<<< DivUp( slots * 8u, THREADS ), THREADS, 0, stream >>>( slots, n, bvh.leafMax, bvh.isNode8.as< uint32_t >(), bvh.node8Index.as< uint32_t >(), bvh.expansion.as< uint32_t >(), rf, rl, bounds, fp32Out.as< BvhNode8 >() );
     CheckLaunch( "k_write_nodes" );
-    if( half )
-    {
-        bvh.nodes.Reserve( size_t( bvh.nodeCount ) * sizeof( BvhNode8H ) );
-        k_half_nodes<<< DivUp( bvh.nodeCount, THREADS ), THREADS, 0, stream >>>( bvh.nodes32.as< BvhNode8 >(), bvh.nodeCount, bvh.nodes.as< BvhNode8H >() );
-        CheckLaunch( "k_half_nodes" );
-    }
-    k_sorted_tris<<< DivUp( n, THREADS ), THREADS, 0, stream >>>( d_tris, vin, n, bvh.sortedTris.as< float4 >() );
-    CheckLaunch( "k_sorted_tris" );
     if( batched )
     {
         k_gather_roots8<<< 1, 256, 0, stream >>>( bvh.sceneRoots.as< uint32_t >(), sceneCount, bvh.node8Index.as< uint32_t >(),
@@ -1108,6 +1135,20 @@ void BuildBvhImpl( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtB
         bvh.hostSceneRoots.resize( sceneCount );
         RG_HIP_CHECK( hipMemcpyAsync( bvh.hostSceneRoots.data(), bvh.sceneRoots8.ptr, size_t( sceneCount ) * 4, hipMemcpyDeviceToHost, stream ) );
     }
+    if( withTop )
+    {
+        k_top_node<<< 1, 64, 0, stream >>>( fp32Out.as< BvhNode8 >(), bvh.sceneRoots8.as< uint32_t >(), sceneCount, bvh.nodeCount );
+        CheckLaunch( "k_top_node" );
+        bvh.nodeCount += 1;
+    }
+    if( half )
+    {
+        bvh.nodes.Reserve( size_t( bvh.nodeCount ) * sizeof( BvhNode8H ) );
+        k_half_nodes<<< DivUp( bvh.nodeCount, THREADS ), THREADS, 0, stream >>>( bvh.nodes32.as< BvhNode8 >(), bvh.nodeCount, bvh.nodes.as< BvhNode8H >() );
+        CheckLaunch( "k_half_nodes" );
+    }
+    k_sorted_tris<<< DivUp( n, THREADS ), THREADS, 0, stream >>>( d_tris, vin, n, bvh.sortedTris.as< float4 >() );
+    CheckLaunch( "k_sorted_tris" );
 
     RG_HIP_CHECK( hipMemcpyAsync( bvh.sceneMin, sceneBounds, 3 * sizeof( double ), hipMemcpyDeviceToHost, stream ) );
     RG_HIP_CHECK( hipMemcpyAsync( bvh.sceneMax, sceneBounds + 3, 3 * sizeof( double ), hipMemcpyDeviceToHost, stream ) );
@@ -1118,6 +1159,7 @@ void BuildBvhImpl( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtB
 void BuildBvh( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtBvhBuildParams& params, hipStream_t stream )
 {
     BuildBvhImpl( bvh, d_tris, n, params, nullptr, 0, stream );
+    bvh.frameMode = BvhObject::FRAME_SINGLE;
 }
 
 void BuildBvhBatch( BvhObject& bvh, const float* d_tris, const uint32_t* offsets, uint32_t sceneCount, const RgExtBvhBuildParams& params,
@@ -1130,10 +1172,37 @@ void BuildBvhBatch( BvhObject& bvh, const float* d_tris, const uint32_t* offsets
         RG_REQUIRE( offsets[ s + 1 ] >= offsets[ s ] + 2, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "every scene needs at least 2 triangles" );
     }
     BuildBvhImpl( bvh, d_tris, offsets[ sceneCount ], params, offsets, sceneCount, stream );
+    bvh.frameMode = BvhObject::FRAME_SINGLE;
     for( uint32_t s = 0; s < sceneCount; s++ )
     {
         RG_REQUIRE( bvh.hostSceneRoots[ s ] < bvh.nodeCount, RG_RESULT_INTERNAL_ERROR, "a scene has no root node" );
     }
+}
+
+void BuildFrameBvh( BvhObject& bvh, const float* d_tris, uint32_t n, uint32_t opaqueN, const RgExtBvhBuildParams& params, hipStream_t stream )
+{
+    RG_REQUIRE( opaqueN <= n, RG_RESULT_INTERNAL_ERROR, "opaque range exceeds the triangle count" );
+    uint32_t roots[ 3 ] = { 0, 0, 0 };
+    if( opaqueN >= 2 && n - opaqueN >= 2 )
+    {
+        const uint32_t offsets[ 3 ] = { 0, opaqueN, n };
+        BuildBvhImpl( bvh, d_tris, n, params, offsets, 2, stream, true );
+        RG_REQUIRE( bvh.hostSceneRoots[ 0 ] < bvh.nodeCount && bvh.hostSceneRoots[ 1 ] < bvh.nodeCount, RG_RESULT_INTERNAL_ERROR,
+                    "a triangle range has no root node" );
+        bvh.frameMode = BvhObject::FRAME_SPLIT;
+        roots[ 0 ]    = bvh.nodeCount - 1; // the top node
+        roots[ 1 ]    = bvh.hostSceneRoots[ 0 ];
+    }
+    else
+    {
+        // A range of one triangle has no subtree of its own: a single tree,
+        // seen by every ray type unless nothing in it is opaque.
+        BuildBvhImpl( bvh, d_tris, n, params, nullptr, 0, stream );
+        bvh.frameMode = opaqueN == 0 && n > 0 ? BvhObject::FRAME_REFRACT_ONLY : BvhObject::FRAME_SINGLE;
+    }
+    bvh.frameRoots.Reserve( sizeof( roots ) );
+    RG_HIP_CHECK( hipMemcpyAsync( bvh.frameRoots.ptr, roots, sizeof( roots ), hipMemcpyHostToDevice, stream ) );
+    RG_HIP_CHECK( hipStreamSynchronize( stream ) );
 }
 
 } // namespace rg

@@ -44,7 +44,8 @@ struct PathTracer
     DeviceBuffer grid[ 2 ];                // 16^3 * 128 light-grid reservoirs, ping-pong
     const uint32_t* blueNoise = nullptr;   // 128 layers of 128x128 RGBA8, one table per process
     DeviceBuffer frameAccum;               // node visits, triangle tests (u64), error bits (u32) of the frame
-    hipEvent_t   traceEv[ 32 ] = {};
+    static constexpr uint32_t TRACE_EVENT_PAIRS = 32; // traversal launches timed per frame
+    hipEvent_t   traceEv[ 2 * TRACE_EVENT_PAIRS ] = {};
     uint32_t     traceLaunches = 0;
     bool         collectStats  = false;
     int          gridParity = 0;

@@ -481,6 +481,41 @@ __device__ __forceinline__ v3 RayTriBary( v3 p0, v3 p1, v3 p2, v3 o, v3 d )
     return mk3( 1.0f - u - v, u, v );
 }
 
+// storeSky (RaygenPrimary.inl:90-139) for a colour sky; the motion of an
+// infinitely far point uses the checkerboard pixel, as :74-76 and :118 do.
+// The primary pass also clears DEPTH_GRAD; reflections keep it.
+__device__ void StoreSky( const RgExtFrameParams& p, const FbPtrs& P, size_t ci, size_t ri, int cx, int cy, v3 dir, v3 thr, float thrW, bool primary )
+{
+    const int   W  = p.renderWidth, H = p.renderHeight;
+    const float cu = ( float( cx ) + 0.5f + p.jitterX ) / float( W );
+    const float cv = ( float( cy ) + 0.5f + p.jitterY ) / float( H );
+    const v3    rd = GetRayDir( p, cu, cv );
+    float       w;
+    const v3    vc = MulM4( p.view, rd, 0.0f, &w ), vp = MulM4( p.viewPrev, rd, 0.0f, &w );
+    // mat3(projection) * v: the upper-left 3x3 block, no division by w
+    const v3 cc = mk3( p.projection[ 0 ] * vc.x + p.projection[ 4 ] * vc.y + p.projection[ 8 ] * vc.z,
+                       p.projection[ 1 ] * vc.x + p.projection[ 5 ] * vc.y + p.projection[ 9 ] * vc.z, 0.0f );
+    const v3 cp = mk3( p.projectionPrev[ 0 ] * vp.x + p.projectionPrev[ 4 ] * vp.y + p.projectionPrev[ 8 ] * vp.z,
+                       p.projectionPrev[ 1 ] * vp.x + p.projectionPrev[ 5 ] * vp.y + p.projectionPrev[ 9 ] * vp.z, 0.0f );
+    const float mx  = ( cp.x * 0.5f + 0.5f ) - ( cc.x * 0.5f + 0.5f );
+    const float my  = ( cp.y * 0.5f + 0.5f ) - ( cc.y * 0.5f + 0.5f );
+    const v3    sky = SkyColor( p );
+    Plane< uint32_t >( P, RG_EXT_PLANE_IS_SKY )[ ci ]           = 1u;
+    Plane< float4 >( P, RG_EXT_PLANE_ALBEDO )[ ri ]             = make_float4( sky.x, sky.y, sky.z, 0.0f );
+    Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL )[ ci ]           = EncodeNormal( mk3( 0, 0, 0 ) );
+    Plane< float2 >( P, RG_EXT_PLANE_METALLIC_ROUGHNESS )[ ci ] = make_float2( 0.f, 0.f );
+    Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD )[ ci ]         = MAX_RAY_LEN * 2.0f;
+    Plane< float4 >( P, RG_EXT_PLANE_MOTION )[ ci ]             = make_float4( mx, my, 0.f, 0.f );
+    Plane< float4 >( P, RG_EXT_PLANE_SURFACE_POSITION )[ ci ]   = make_float4( 1e7f, 1e7f, 1e7f, 1e7f );
+    Plane< float4 >( P, RG_EXT_PLANE_VISIBILITY_BUFFER )[ ci ]  = make_float4( 4294967295.0f, 4294967295.0f, 4294967295.0f, 4294967295.0f );
+    Plane< float4 >( P, RG_EXT_PLANE_VIEW_DIRECTION )[ ci ]     = make_float4( dir.x, dir.y, dir.z, 0.f );
+    Plane< float4 >( P, RG_EXT_PLANE_THROUGHPUT )[ ci ]         = make_float4( thr.x, thr.y, thr.z, thrW );
+    if( primary )
+    {
+        Plane< float >( P, RG_EXT_PLANE_DEPTH_GRAD )[ ci ] = 0.0f;
+    }
+}
+
 // G-buffer of the primary hit (RaygenPrimary.inl:280-348, storeSky :90-139).
 __global__ __launch_bounds__( THREADS ) void k_primary_shade( RgExtFrameParams p, const RgExtHit* hits, SceneView sc, FbPtrs P )
 {
@@ -502,32 +537,7 @@ __global__ __launch_bounds__( THREADS ) void k_primary_shade( RgExtFrameParams p
 
     if( hit.primitiveId == RG_EXT_MISS )
     {
-        // storeSky; the motion of an infinitely far point uses the
-        // checkerboard pixel, as RaygenPrimary.inl:74-76 and :118 do
-        const float cu = ( float( cx ) + 0.5f + p.jitterX ) / float( W );
-        const float cv = ( float( cy ) + 0.5f + p.jitterY ) / float( H );
-        const v3    rd = GetRayDir( p, cu, cv );
-        float       w;
-        const v3    vc = MulM4( p.view, rd, 0.0f, &w ), vp = MulM4( p.viewPrev, rd, 0.0f, &w );
-        // mat3(projection) * v: the upper-left 3x3 block, no division by w
-        const v3 cc = mk3( p.projection[ 0 ] * vc.x + p.projection[ 4 ] * vc.y + p.projection[ 8 ] * vc.z,
-                           p.projection[ 1 ] * vc.x + p.projection[ 5 ] * vc.y + p.projection[ 9 ] * vc.z, 0.0f );
-        const v3 cp = mk3( p.projectionPrev[ 0 ] * vp.x + p.projectionPrev[ 4 ] * vp.y + p.projectionPrev[ 8 ] * vp.z,
-                           p.projectionPrev[ 1 ] * vp.x + p.projectionPrev[ 5 ] * vp.y + p.projectionPrev[ 9 ] * vp.z, 0.0f );
-        const float mx = ( cp.x * 0.5f + 0.5f ) - ( cc.x * 0.5f + 0.5f );
-        const float my = ( cp.y * 0.5f + 0.5f ) - ( cc.y * 0.5f + 0.5f );
-        const v3    sky = SkyColor( p );
-        Plane< uint32_t >( P, RG_EXT_PLANE_IS_SKY )[ ci ]             = 1u;
-        Plane< float4 >( P, RG_EXT_PLANE_ALBEDO )[ ri ]               = make_float4( sky.x, sky.y, sky.z, 0.0f );
-        Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL )[ ci ]             = EncodeNormal( mk3( 0, 0, 0 ) );
-        Plane< float2 >( P, RG_EXT_PLANE_METALLIC_ROUGHNESS )[ ci ]   = make_float2( 0.f, 0.f );
-        Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD )[ ci ]           = MAX_RAY_LEN * 2.0f;
-        Plane< float4 >( P, RG_EXT_PLANE_MOTION )[ ci ]               = make_float4( mx, my, 0.f, 0.f );
-        Plane< float4 >( P, RG_EXT_PLANE_SURFACE_POSITION )[ ci ]     = make_float4( 1e7f, 1e7f, 1e7f, 1e7f );
-        Plane< float4 >( P, RG_EXT_PLANE_VISIBILITY_BUFFER )[ ci ]    = make_float4( 4294967295.0f, 4294967295.0f, 4294967295.0f, 4294967295.0f );
-        Plane< float4 >( P, RG_EXT_PLANE_VIEW_DIRECTION )[ ci ]       = make_float4( dir.x, dir.y, dir.z, 0.f );
-        Plane< float >( P, RG_EXT_PLANE_DEPTH_GRAD )[ ci ]            = 0.0f;
-        Plane< float4 >( P, RG_EXT_PLANE_THROUGHPUT )[ ci ]           = make_float4( 1.f, 1.f, 1.f, 0.f );
+        StoreSky( p, P, ci, ri, cx, cy, dir, mk3( 1.f, 1.f, 1.f ), 0.0f, true );
         return;
     }
 
@@ -573,6 +583,345 @@ __global__ __launch_bounds__( THREADS ) void k_primary_shade( RgExtFrameParams p
     Plane< uint4 >( P, RG_EXT_PLANE_VISIBILITY_BUFFER )[ ci ]   = make_uint4( 1u, ( h.localPrim << 12 ) | ( h.geom & 0xFFFu ), __float_as_uint( hit.u ), __float_as_uint( hit.v ) );
     Plane< float4 >( P, RG_EXT_PLANE_VIEW_DIRECTION )[ ci ]     = make_float4( dir.x, dir.y, dir.z, 0.0f );
     Plane< float4 >( P, RG_EXT_PLANE_THROUGHPUT )[ ci ]         = make_float4( 1.f, 1.f, 1.f, 0.f );
+}
+
+// ------------------------------------------- reflection / refraction chain
+// RAYGEN_REFL_REFR_SHADER (RaygenPrimary.inl:352-599) as a wavefront: each
+// depth is k_reflrefr_a (decide, build the ray), one closest-hit launch, then
+// k_reflrefr_b (store the hit or the sky into the G-buffer).  The loop's
+// local state lives in three float4 per pixel:
+//   S[0] = throughput.xyz, bits (1: still tracing, 2: wasSplit, media << 2)
+//   S[1] = ray direction.xyz, full path length
+//   S[2] = virtual position for motion.xyz
+// Portals, animated water normals, screen emission and acid fog are not
+// part of this path (they need textures or portal instances).
+constexpr uint32_t RR_ACTIVE = 1u, RR_SPLIT = 2u;
+constexpr uint32_t MEDIA_VACUUM = 0u, MEDIA_WATER = 1u, MEDIA_GLASS = 2u, MEDIA_ACID = 3u;
+
+// getMediaTypeFromFlags / isRefractFromFlags / isReflectFromFlags
+// (Media.h:83-124) over RgMeshPrimitiveFlags as GeomInfoManager.cpp:131-186
+// maps them to the instance flags.
+__device__ __forceinline__ uint32_t MediaFromFlags( uint32_t f, float roughness )
+{
+    if( f & RG_MESH_PRIMITIVE_WATER )
+    {
+        return MEDIA_WATER;
+    }
+    if( ( f & RG_MESH_PRIMITIVE_GLASS ) || ( ( f & RG_MESH_PRIMITIVE_GLASS_IF_SMOOTH ) && roughness < 0.1f ) )
+    {
+        return MEDIA_GLASS;
+    }
+    if( f & RG_MESH_PRIMITIVE_ACID )
+    {
+        return MEDIA_ACID;
+    }
+    return MEDIA_VACUUM;
+}
+__device__ __forceinline__ bool IsRefractFlags( uint32_t f, float roughness )
+{
+    return ( f & ( RG_MESH_PRIMITIVE_WATER | RG_MESH_PRIMITIVE_ACID | RG_MESH_PRIMITIVE_GLASS ) ) ||
+           ( ( f & RG_MESH_PRIMITIVE_GLASS_IF_SMOOTH ) && roughness < 0.1f );
+}
+__device__ __forceinline__ bool IsReflectFlags( uint32_t f, float roughness )
+{
+    return ( f & ( RG_MESH_PRIMITIVE_MIRROR | RG_MESH_PRIMITIVE_WATER | RG_MESH_PRIMITIVE_ACID | RG_MESH_PRIMITIVE_GLASS ) ) ||
+           ( ( f & RG_MESH_PRIMITIVE_MIRROR_IF_SMOOTH ) && roughness < 0.1f );
+}
+
+// getIndexOfRefraction (Media.h:28-41)
+__device__ __forceinline__ float IndexOfRefraction( const RgExtFrameParams& p, uint32_t media )
+{
+    return media == MEDIA_WATER || media == MEDIA_ACID ? p.indexOfRefractionWater : ( media == MEDIA_GLASS ? p.indexOfRefractionGlass : 1.0f );
+}
+
+// getMediaTransmittance (Media.h:43-58): exp(-distance * -log(colour)),
+// acid extinction scaled by max(1, sqrt(density)).  Vacuum and glass do not
+// attenuate (also for an infinite distance).
+__device__ v3 MediaTransmittance( const RgExtFrameParams& p, uint32_t media, float distance )
+{
+    if( media != MEDIA_WATER && media != MEDIA_ACID )
+    {
+        return mk3( 1.0f, 1.0f, 1.0f );
+    }
+    const float* c = media == MEDIA_WATER ? p.waterColorAndDensity : p.acidColorAndDensity;
+    v3           e = mk3( -logf( c[ 0 ] ), -logf( c[ 1 ] ), -logf( c[ 2 ] ) );
+    if( media == MEDIA_ACID )
+    {
+        e = e * fmaxf( 1.0f, sqrtf( c[ 3 ] ) );
+    }
+    return mk3( expf( -distance * e.x ), expf( -distance * e.y ), expf( -distance * e.z ) );
+}
+
+// calcRefractionDirection, Ray Tracing Gems II ch. 8 (Media.h:62-77):
+// false on total internal reflection.
+__device__ __forceinline__ bool RefractDir( float n1, float n2, v3 I, v3 N, v3* T )
+{
+    const float eta = n1 / n2;
+    const float c1  = -dot( I, N );
+    const float w   = eta * c1;
+    const float c2m = ( w - eta ) * ( w + eta );
+    if( c2m < -1.0f )
+    {
+        return false;
+    }
+    *T = I * eta + N * ( w - sqrtf( 1.0f + c2m ) );
+    return true;
+}
+
+__device__ __forceinline__ float Pow5( float x )
+{
+    const float x2 = x * x;
+    return x2 * x2 * x;
+}
+
+// getFresnelSchlick(n1, n2, V, N) (BRDF.h:78-84)
+__device__ __forceinline__ float FresnelIor( float n1, float n2, v3 V, v3 N )
+{
+    float r0 = ( n1 - n2 ) / ( n1 + n2 );
+    r0 *= r0;
+    const float t = Pow5( 1.0f - fabsf( dot( N, V ) ) );
+    return r0 * ( 1.0f - t ) + t;
+}
+
+// getFresnelSchlick(nl, specularColor) (BRDF.h:73-76)
+__device__ __forceinline__ v3 FresnelSpec( float nl, v3 f0 )
+{
+    const float t = Pow5( 1.0f - fmaxf( nl, 0.0f ) );
+    return f0 + ( mk3( 1.0f, 1.0f, 1.0f ) - f0 ) * t;
+}
+
+struct RrState
+{
+    float4* s; // 3 x W*H float4
+    size_t  n;
+};
+
+// One depth of the loop up to traceReflectionRefractionRay.  Depth 0 reads
+// the primary surface from the G-buffer (as the reference restores it) and
+// its flags from the primary hit; later depths recompute the last hit.
+__global__ __launch_bounds__( THREADS ) void k_reflrefr_a( RgExtFrameParams p, FbPtrs P, SceneView sc, RgExtRay* rays, const RgExtHit* hits,
+                                                           RrState st, uint32_t depth )
+{
+    const int x = blockIdx.x * BX + threadIdx.x % BX, y = blockIdx.y * BY + threadIdx.x / BX;
+    const int W = p.renderWidth, H = p.renderHeight;
+    if( x >= W || y >= H )
+    {
+        return;
+    }
+    int cx, cy;
+    cbFromRegular( x, y, W, &cx, &cy );
+    const size_t ci = size_t( cy ) * W + cx;
+    const size_t ri = size_t( y ) * W + x;
+
+    v3       pos, normal, albedo, thr, dir;
+    float    roughness, metallic, fpl;
+    uint32_t flags, bits;
+    if( depth == 0 )
+    {
+        if( Plane< uint32_t >( P, RG_EXT_PLANE_IS_SKY )[ ci ] != 0u )
+        {
+            st.s[ ri ]  = make_float4( 0.f, 0.f, 0.f, __uint_as_float( 0u ) );
+            rays[ ri ]  = NullRay();
+            return;
+        }
+        const float4 alb = Plane< float4 >( P, RG_EXT_PLANE_ALBEDO )[ ri ];
+        const float4 sp  = Plane< float4 >( P, RG_EXT_PLANE_SURFACE_POSITION )[ ci ];
+        const float2 mr  = Plane< float2 >( P, RG_EXT_PLANE_METALLIC_ROUGHNESS )[ ci ];
+        const float4 th  = Plane< float4 >( P, RG_EXT_PLANE_THROUGHPUT )[ ci ];
+        albedo           = mk3( alb.x, alb.y, alb.z );
+        pos              = mk3( sp.x, sp.y, sp.z );
+        normal           = DecodeNormal( Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL )[ ci ] );
+        metallic         = mr.x;
+        roughness        = mr.y;
+        thr              = mk3( th.x, th.y, th.z );
+        const float u    = ( float( x ) + 0.5f + p.jitterX ) / float( W );
+        const float v    = ( float( y ) + 0.5f + p.jitterY ) / float( H );
+        dir              = GetRayDir( p, u, v );
+        fpl              = Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD )[ ci ];
+        flags            = sc.geoms[ sc.triGeom[ hits[ ri ].primitiveId ] ].flags;
+        bits             = RR_ACTIVE | ( ( p.cameraMediaType & 3u ) << 2 );
+        st.s[ 2 * st.n + ri ] = make_float4( pos.x, pos.y, pos.z, 0.0f );
+    }
+    else
+    {
+        const float4 s0 = st.s[ ri ];
+        bits            = __float_as_uint( s0.w );
+        if( ( bits & RR_ACTIVE ) == 0u )
+        {
+            rays[ ri ] = NullRay();
+            return;
+        }
+        const float4   s1  = st.s[ st.n + ri ];
+        const RgExtHit hit = hits[ ri ];
+        const RgExtRay r   = rays[ ri ];
+        const HitInfo  h   = GetHit( sc, p, hit.primitiveId, hit.u, hit.v, mk3( r.origin[ 0 ], r.origin[ 1 ], r.origin[ 2 ] ) );
+        pos                = h.position;
+        normal             = h.normal;
+        albedo             = h.albedo;
+        metallic           = h.metallic;
+        roughness          = h.roughness;
+        flags              = sc.geoms[ h.geom ].flags;
+        thr                = mk3( s0.x, s0.y, s0.z );
+        dir                = mk3( s1.x, s1.y, s1.z );
+        fpl                = s1.w;
+    }
+
+    // getNewRayMedia (RaygenPrimary.inl:141-150)
+    const uint32_t curMedia = ( bits >> 2 ) & 3u;
+    uint32_t       newMedia = ( depth == 0 && p.cameraMediaType != MEDIA_VACUUM ) ? MEDIA_VACUUM : MediaFromFlags( flags, roughness );
+    const bool     toRefract = IsRefractFlags( flags, roughness );
+    const bool     toReflect = IsReflectFlags( flags, roughness );
+    if( !toReflect && !toRefract )
+    {
+        st.s[ ri ] = make_float4( thr.x, thr.y, thr.z, __uint_as_float( bits & ~RR_ACTIVE ) );
+        rays[ ri ] = NullRay();
+        return;
+    }
+    const float curIor  = IndexOfRefraction( p, curMedia );
+    const float newIor  = IndexOfRefraction( p, newMedia );
+    const bool  isWater = newMedia == MEDIA_WATER || curMedia == MEDIA_WATER || newMedia == MEDIA_ACID || curMedia == MEDIA_ACID;
+    // getNormal (RaygenPrimary.inl:246-270): face the ray; the animated
+    // water normal needs the water texture and is not applied
+    if( dot( normal, -dir ) < 0.0f )
+    {
+        normal = -normal;
+    }
+    normal = normalize( normal );
+
+    v3       origin  = pos;
+    bool     wasSplit = ( bits & RR_SPLIT ) != 0u;
+    bool     doSplit  = !wasSplit;
+    bool     doRefr   = false;
+    float    F        = 1.0f;
+    v3       refr;
+    uint32_t media = curMedia;
+    if( toRefract && RefractDir( curIor, newIor, dir, normal, &refr ) )
+    {
+        doRefr = cx >= cbSeparator( W ); // isCheckerboardPixOdd
+        F      = FresnelIor( curIor, newIor, -dir, normal );
+    }
+    else
+    {
+        doSplit = false; // total internal reflection
+    }
+    if( doRefr )
+    {
+        dir = refr;
+        thr = thr * ( 1.0f - F );
+        if( flags & RG_MESH_PRIMITIVE_THIN_MEDIA )
+        {
+            // a slab of thinMediaWidth along the normal, then back out
+            const float len = p.thinMediaWidth / fmaxf( 0.001f, -dot( normal, dir ) );
+            origin          = origin + dir * len;
+            thr             = thr * MediaTransmittance( p, newMedia, len );
+            fpl += len;
+            if( RefractDir( newIor, curIor, dir, normal, &refr ) )
+            {
+                dir = refr;
+            }
+            newMedia = curMedia;
+        }
+        media = newMedia;
+    }
+    else
+    {
+        // reflect(normalize(I), N)
+        const v3 i = normalize( dir );
+        dir        = i - normal * ( 2.0f * dot( normal, i ) );
+        if( !isWater )
+        {
+            thr = thr * FresnelSpec( fmaxf( 0.0f, dot( normal, dir ) ), SpecularColor( albedo, metallic ) );
+        }
+        else
+        {
+            thr    = thr * F;
+            origin = origin + dir * 0.005f; // SHIPPING_HACK offset of the reference
+        }
+    }
+    if( doSplit )
+    {
+        thr      = thr * 2.0f;
+        wasSplit = true;
+    }
+    bits = RR_ACTIVE | ( wasSplit ? RR_SPLIT : 0u ) | ( ( media & 3u ) << 2 );
+    st.s[ ri ]        = make_float4( thr.x, thr.y, thr.z, __uint_as_float( bits ) );
+    st.s[ st.n + ri ] = make_float4( dir.x, dir.y, dir.z, fpl );
+    // traceReflectionRefractionRay (RaygenCommon.h:235-251)
+    rays[ ri ] = MakeRay( origin, dir, 0.001f, p.rayLength );
+}
+
+// After the trace: the sky (RaygenPrimary.inl:550-556) or the new surface
+// (getHitInfoWithRayCone_ReflectionRefraction, HitInfo.inl:213-330, and the
+// G-buffer stores of :584-597), with its virtual-position motion.
+__global__ __launch_bounds__( THREADS ) void k_reflrefr_b( RgExtFrameParams p, FbPtrs P, SceneView sc, const RgExtRay* rays, const RgExtHit* hits,
+                                                           RrState st )
+{
+    const int x = blockIdx.x * BX + threadIdx.x % BX, y = blockIdx.y * BY + threadIdx.x / BX;
+    const int W = p.renderWidth, H = p.renderHeight;
+    if( x >= W || y >= H )
+    {
+        return;
+    }
+    const size_t ri = size_t( y ) * W + x;
+    const float4 s0 = st.s[ ri ];
+    uint32_t     bits = __float_as_uint( s0.w );
+    if( ( bits & RR_ACTIVE ) == 0u )
+    {
+        return;
+    }
+    int cx, cy;
+    cbFromRegular( x, y, W, &cx, &cy );
+    const size_t   ci    = size_t( cy ) * W + cx;
+    const float4   s1    = st.s[ st.n + ri ];
+    const v3       dir   = mk3( s1.x, s1.y, s1.z );
+    const uint32_t media = ( bits >> 2 ) & 3u;
+    const float    thrW  = ( bits & RR_SPLIT ) ? 1.0f : -1.0f;
+    v3             thr   = mk3( s0.x, s0.y, s0.z );
+    const RgExtHit hit   = hits[ ri ];
+    if( hit.primitiveId == RG_EXT_MISS )
+    {
+        const float up = fabsf( dot( dir, mk3( p.worldUp[ 0 ], p.worldUp[ 1 ], p.worldUp[ 2 ] ) ) );
+        thr            = thr * MediaTransmittance( p, media, 1.0f / ( up * up * up ) );
+        StoreSky( p, P, ci, ri, cx, cy, dir, thr, thrW, false );
+        st.s[ ri ] = make_float4( thr.x, thr.y, thr.z, __uint_as_float( bits & ~RR_ACTIVE ) );
+        return;
+    }
+    const RgExtRay r      = rays[ ri ];
+    const v3       origin = mk3( r.origin[ 0 ], r.origin[ 1 ], r.origin[ 2 ] );
+    const HitInfo  h      = GetHit( sc, p, hit.primitiveId, hit.u, hit.v, origin );
+    const float    rayLen = length( h.position - origin );
+
+    // virtualPosForMotion += viewDir * rayLen, seen by both cameras
+    const float u   = ( float( x ) + 0.5f + p.jitterX ) / float( W );
+    const float v   = ( float( y ) + 0.5f + p.jitterY ) / float( H );
+    const v3    cam = GetRayDir( p, u, v );
+    const float4 s2 = st.s[ 2 * st.n + ri ];
+    const v3    vp  = mk3( s2.x, s2.y, s2.z ) + cam * rayLen;
+    float       wc, wp, cwc, cwp;
+    const v3    vCur  = MulM4( p.view, vp, 1.0f, &wc );
+    const v3    vPrev = MulM4( p.viewPrev, vp, 1.0f, &wp );
+    const v3    cCur  = MulM4( p.projection, vCur, wc, &cwc );
+    const v3    cPrev = MulM4( p.projectionPrev, vPrev, wp, &cwp );
+    const float sxc = ( cCur.x / cwc ) * 0.5f + 0.5f, syc = ( cCur.y / cwc ) * 0.5f + 0.5f;
+    const float sxp = ( cPrev.x / cwp ) * 0.5f + 0.5f, syp = ( cPrev.y / cwp ) * 0.5f + 0.5f;
+    const float motionZ = length( vPrev ) - length( vCur );
+
+    thr             = thr * MediaTransmittance( p, media, rayLen );
+    const float fpl = s1.w + rayLen;
+
+    Plane< uint32_t >( P, RG_EXT_PLANE_IS_SKY )[ ci ]           = 0u;
+    Plane< float4 >( P, RG_EXT_PLANE_ALBEDO )[ ri ]             = make_float4( h.albedo.x, h.albedo.y, h.albedo.z, 0.0f );
+    Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL )[ ci ]           = EncodeNormal( h.normal );
+    Plane< float2 >( P, RG_EXT_PLANE_METALLIC_ROUGHNESS )[ ci ] = make_float2( h.metallic, h.roughness );
+    Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD )[ ci ]         = fpl;
+    Plane< float4 >( P, RG_EXT_PLANE_MOTION )[ ci ]             = make_float4( sxp - sxc, syp - syc, motionZ, 0.0f );
+    Plane< float4 >( P, RG_EXT_PLANE_SURFACE_POSITION )[ ci ]   = make_float4( h.position.x, h.position.y, h.position.z, __uint_as_float( 1u ) );
+    Plane< uint4 >( P, RG_EXT_PLANE_VISIBILITY_BUFFER )[ ci ]   = make_uint4( 1u, ( h.localPrim << 12 ) | ( h.geom & 0xFFFu ), __float_as_uint( hit.u ), __float_as_uint( hit.v ) );
+    Plane< float4 >( P, RG_EXT_PLANE_VIEW_DIRECTION )[ ci ]     = make_float4( dir.x, dir.y, dir.z, 0.0f );
+    Plane< float4 >( P, RG_EXT_PLANE_THROUGHPUT )[ ci ]         = make_float4( thr.x, thr.y, thr.z, thrW );
+    st.s[ ri ]                 = make_float4( thr.x, thr.y, thr.z, __uint_as_float( bits ) );
+    st.s[ st.n + ri ]          = make_float4( dir.x, dir.y, dir.z, fpl );
+    st.s[ 2 * st.n + ri ]      = make_float4( vp.x, vp.y, vp.z, 0.0f );
 }
 
 // RtInitialReservoirs.rgen, first half: candidates + directional visibility ray.
@@ -1640,12 +1989,15 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
     TraceAccum accum;
     accum.stats  = frameAccum.as< uint64_t >();
     accum.errors = reinterpret_cast< uint32_t* >( frameAccum.as< uint64_t >() + 2 );
-    auto trace = [ & ]( RgExtRay* rr, RgExtHit* hh, uint32_t flags, uint32_t count ) {
+    // Primary and reflection / refraction rays see every triangle; shadow,
+    // bounce and gradient rays skip the refracting range (the reference's
+    // cull masks, RaygenCommon.h:103-170).
+    auto trace = [ & ]( RgExtRay* rr, RgExtHit* hh, uint32_t flags, uint32_t count, bool allGeometry = false ) {
         if( haveGeometry )
         {
-            const uint32_t k = traceLaunches < 16 ? traceLaunches : 15;
-            TraceRays( bvh, tc, rr, count, hh, flags | traceFlags | ( collectStats ? RG_EXT_TRACE_COLLECT_STATS : 0u ), s,
-                       traceEv[ 2 * k ], traceEv[ 2 * k + 1 ], &accum );
+            const uint32_t k = traceLaunches < TRACE_EVENT_PAIRS ? traceLaunches : TRACE_EVENT_PAIRS - 1;
+            TraceRaysFrame( bvh, tc, rr, count, hh, flags | traceFlags | ( collectStats ? RG_EXT_TRACE_COLLECT_STATS : 0u ), !allGeometry, s,
+                            traceEv[ 2 * k ], traceEv[ 2 * k + 1 ], &accum );
             traceLaunches = k + 1;
         }
         else
@@ -1659,9 +2011,24 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
     // primary
     k_primary_rays<<< grid2, THREADS, 0, s >>>( p, R );
     CheckLaunch( "k_primary_rays" );
-    trace( R, Hh, RG_EXT_TRACE_CLOSEST, n );
+    trace( R, Hh, RG_EXT_TRACE_CLOSEST, n, true );
     k_primary_shade<<< grid2, THREADS, 0, s >>>( p, Hh, sv, P );
     CheckLaunch( "k_primary_shade" );
+
+    // reflections and refractions replace the G-buffer surface; without
+    // such geometry the reference's loop ends at depth 0 with no writes
+    if( p.reflectRefractMaxDepth > 0 && haveGeometry && scene.reflRefrGeometry )
+    {
+        RrState st{ pendA.as< float4 >(), n };
+        for( uint32_t d = 0; d < p.reflectRefractMaxDepth && d < 16u; d++ )
+        {
+            k_reflrefr_a<<< grid2, THREADS, 0, s >>>( p, P, sv, R, Hh, st, d );
+            CheckLaunch( "k_reflrefr_a" );
+            trace( R, Hh, RG_EXT_TRACE_CLOSEST, n, true );
+            k_reflrefr_b<<< grid2, THREADS, 0, s >>>( p, P, sv, R, Hh, st );
+            CheckLaunch( "k_reflrefr_b" );
+        }
+    }
 
     // light grid (current grid = gridParity, previous = the other one)
     if( p.lightCount > 0 )

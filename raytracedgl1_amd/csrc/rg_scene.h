@@ -115,6 +115,14 @@ struct SceneCollector
     PinnedArray< uint32_t >          indices;
     PinnedArray< GeomRecord >        geoms;
     uint32_t                         triCount = 0;
+    // Water, glass and acid primitives (the reference's PT_REFRACT BLAS,
+    // VertexCollectorFilterType.cpp:215-229) collect here and follow all other
+    // geometry once Seal() runs, so their triangles form one trailing range
+    // that shadow and bounce rays skip.
+    PinnedArray< GeomRecord >        refrGeoms;
+    std::vector< uint64_t >          refrUids;
+    uint32_t                         refrTriCount = 0;
+    bool                             sealed       = false;
 
     // uniqueObjectID << 32 | primitiveIndexInMesh -> model of the previous frame
     // (UniqueID::MakeForPrimitive, Source/UniqueID.h:29-33)
@@ -137,6 +145,10 @@ struct SceneCollector
 
     void BeginFrame();
     void AddPrimitive( const RgMeshInfo& mesh, const RgMeshPrimitiveInfo& prim );
+    // Appends the refracting geometry after the rest (idempotent until the
+    // next BeginFrame); UploadScene and GeomPrevToCur call it.
+    void Seal();
+    uint32_t OpaqueTriCount() const { return triCount - refrTriCount; }
     void AddDirectional( const RgDirectionalLightUploadInfo& info );
     void AddSphere( const RgSphericalLightUploadInfo& info );
     void AddSpot( const RgSpotLightUploadInfo& info );
@@ -147,7 +159,7 @@ struct SceneCollector
     void FinishLights();
     // Last frame's geometry index -> this frame's, 0xFFFFFFFF when gone
     // (getCurrentGeometryIndexByPrev, Shaders/VertexData.inl:636-658).
-    void GeomPrevToCur( std::vector< uint32_t >& out ) const;
+    void GeomPrevToCur( std::vector< uint32_t >& out );
 };
 
 // Device side of the assembled frame geometry.
@@ -160,8 +172,19 @@ struct DeviceScene
     DeviceBuffer trisPrev;   // last built copy, to skip identical rebuilds
     DeviceBuffer diffFlag;   // 1 u32
     uint32_t     triCount = 0, geomCount = 0, builtTriCount = 0;
+    uint32_t     opaqueTriCount = 0; // triangles [opaqueTriCount, triCount) refract
+    uint32_t     builtOpaqueTriCount = 0;
+    bool         reflRefrGeometry = false; // some geometry reflects or refracts
     bool         bvhValid = false;
 };
+
+// Refracting primitives: water, glass, glass-if-smooth and acid that are not
+// alpha tested (VertexCollectorFilterType.cpp:215-229).
+inline bool IsRefractPrimitive( uint32_t flags )
+{
+    return ( flags & RG_MESH_PRIMITIVE_ALPHA_TESTED ) == 0u &&
+           ( flags & ( RG_MESH_PRIMITIVE_WATER | RG_MESH_PRIMITIVE_GLASS | RG_MESH_PRIMITIVE_GLASS_IF_SMOOTH | RG_MESH_PRIMITIVE_ACID ) ) != 0u;
+}
 
 // Copies the collected geometry to the device (asynchronous).
 void UploadScene( SceneCollector& sc, DeviceScene& ds, hipStream_t s );

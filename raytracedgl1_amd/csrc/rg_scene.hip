@@ -72,7 +72,11 @@ void SceneCollector::BeginFrame()
     vertices.Clear();
     indices.Clear();
     geoms.Clear();
-    triCount = 0;
+    refrGeoms.Clear();
+    refrUids.clear();
+    triCount     = 0;
+    refrTriCount = 0;
+    sealed       = false;
     prevModels.swap( curModels );
     curModels.clear();
     prevGeomUids.swap( geomUids );
@@ -161,11 +165,14 @@ void SceneCollector::AddPrimitive( const RgMeshInfo& mesh, const RgMeshPrimitive
     }
     RG_REQUIRE( uint64_t( triCount ) + tris < ( 1ull << 28 ), RG_RESULT_WRONG_FUNCTION_ARGUMENT, "too many triangles in one frame" );
 
+    RG_REQUIRE( !sealed, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "primitive uploaded after the frame's geometry was sealed" );
+    const bool refract = IsRefractPrimitive( prim.flags );
+
     GeomRecord g{};
     std::memcpy( g.model, &mesh.transform.matrix[ 0 ][ 0 ], sizeof( g.model ) );
     g.baseVertex = uint32_t( vertices.size );
     g.baseIndex  = 0xFFFFFFFFu;
-    g.firstTri   = triCount;
+    g.firstTri   = refract ? refrTriCount : triCount - refrTriCount; // within its own list until Seal()
     g.triCount   = tris;
     g.color      = prim.color;
     g.emissive   = Saturate( prim.emissive );
@@ -202,9 +209,37 @@ void SceneCollector::AddPrimitive( const RgMeshInfo& mesh, const RgMeshPrimitive
         g.baseIndex = uint32_t( indices.size );
         StagingCopy( indices.Append( prim.indexCount ), prim.pIndices, size_t( prim.indexCount ) * 4 );
     }
-    *geoms.Append( 1 ) = g;
-    geomUids.push_back( uid );
+    if( refract )
+    {
+        *refrGeoms.Append( 1 ) = g;
+        refrUids.push_back( uid );
+        refrTriCount += tris;
+    }
+    else
+    {
+        *geoms.Append( 1 ) = g;
+        geomUids.push_back( uid );
+    }
     triCount += tris;
+}
+
+void SceneCollector::Seal()
+{
+    if( sealed )
+    {
+        return;
+    }
+    sealed              = true;
+    const uint32_t base = triCount - refrTriCount;
+    for( size_t i = 0; i < refrGeoms.size; i++ )
+    {
+        GeomRecord g = refrGeoms.data[ i ];
+        g.firstTri += base;
+        *geoms.Append( 1 ) = g;
+    }
+    geomUids.insert( geomUids.end(), refrUids.begin(), refrUids.end() );
+    refrGeoms.Clear();
+    refrUids.clear();
 }
 
 // Light encoders: LightManager.cpp:99-234.
@@ -348,8 +383,9 @@ void SceneCollector::FinishLights()
     }
 }
 
-void SceneCollector::GeomPrevToCur( std::vector< uint32_t >& out ) const
+void SceneCollector::GeomPrevToCur( std::vector< uint32_t >& out )
 {
+    Seal();
     std::unordered_map< uint64_t, uint32_t > cur;
     cur.reserve( geomUids.size() );
     for( uint32_t i = 0; i < geomUids.size(); i++ )
@@ -450,8 +486,17 @@ __global__ __launch_bounds__( THREADS ) void k_compare( const uint32_t* a, const
 
 void UploadScene( SceneCollector& sc, DeviceScene& ds, hipStream_t s )
 {
+    sc.Seal();
     const uint32_t n = sc.triCount;
-    ds.triCount      = n;
+    ds.triCount       = n;
+    ds.opaqueTriCount = sc.OpaqueTriCount();
+    ds.reflRefrGeometry = false;
+    for( size_t i = 0; i < sc.geoms.size; i++ )
+    {
+        constexpr uint32_t RR = RG_MESH_PRIMITIVE_MIRROR | RG_MESH_PRIMITIVE_WATER | RG_MESH_PRIMITIVE_GLASS | RG_MESH_PRIMITIVE_ACID |
+                                RG_MESH_PRIMITIVE_GLASS_IF_SMOOTH | RG_MESH_PRIMITIVE_MIRROR_IF_SMOOTH;
+        ds.reflRefrGeometry = ds.reflRefrGeometry || ( sc.geoms.data[ i ].flags & RR ) != 0u;
+    }
     ds.geomCount     = uint32_t( sc.geoms.size );
     if( n == 0 )
     {

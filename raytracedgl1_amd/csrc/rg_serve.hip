@@ -41,7 +41,8 @@ struct FrameRequest
     RgDrawFrameIlluminationParams il{};
     RgDrawFrameTexturesParams     tx{};
     RgExtDrawFramePathParams      path{};
-    bool                          hasTm = false, hasSky = false, hasIl = false, hasTx = false, hasPath = false;
+    RgDrawFrameReflectRefractParams rr{};
+    bool                          hasTm = false, hasSky = false, hasIl = false, hasTx = false, hasPath = false, hasRr = false;
     uint64_t                      tag    = 0;
     double                        submit = 0.0;
 
@@ -73,6 +74,10 @@ struct FrameRequest
         {
             path = *p, hasPath = true;
         }
+        if( const auto* p = FindParams< RgDrawFrameReflectRefractParams >( in, RG_STRUCTURE_TYPE_REFLECTREFRACT ) )
+        {
+            rr = *p, hasRr = true;
+        }
         Relink();
     }
     // Rebuilds di.pParams over the copies (after the request has moved).
@@ -91,6 +96,7 @@ struct FrameRequest
         push( hasIl, il );
         push( hasTx, tx );
         push( hasPath, path );
+        push( hasRr, rr );
         di.pParams = head;
     }
 };
@@ -480,6 +486,7 @@ RgResult rgExtSceneBuild( RgExtScene scene )
         }
         sc.prevLightCount = uint32_t( sc.lights.size() );
         sc.FinishLights();
+        sc.Seal();
         sc.prevGeomUids = sc.geomUids;
         sc.GeomPrevToCur( scene->geomMap );
         UploadScene( sc, scene->ds, s.setup );
@@ -487,7 +494,7 @@ RgResult rgExtSceneBuild( RgExtScene scene )
         if( scene->ds.triCount > 0 )
         {
             RgExtBvhBuildParams bp = { 1, s.nodeFormat };
-            BuildBvh( scene->bvh, scene->ds.tris.as< float >(), scene->ds.triCount, bp, s.setup );
+            BuildFrameBvh( scene->bvh, scene->ds.tris.as< float >(), scene->ds.triCount, scene->ds.opaqueTriCount, bp, s.setup );
             scene->ds.bvhValid      = true;
             scene->ds.builtTriCount = scene->ds.triCount;
         }

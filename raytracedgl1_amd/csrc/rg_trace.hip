@@ -516,7 +516,8 @@ void InitGrids( TraceContext& s )
 }
 
 void TraceRaysImpl( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags,
-                    hipStream_t stream, hipEvent_t evBefore, hipEvent_t evAfter, const TraceAccum* accum, const uint32_t* d_segRays )
+                    hipStream_t stream, hipEvent_t evBefore, hipEvent_t evAfter, const TraceAccum* accum, const uint32_t* d_segRays,
+                    const uint32_t* d_frameRoot = nullptr )
 {
     RG_REQUIRE( bvh.nodeCount > 0, RG_RESULT_WRONG_FUNCTION_CALL, "BVH is not built" );
     RG_REQUIRE( bvh.nodeFormat == RG_EXT_BVH_NODE_FP32 || bvh.nodeFormat == RG_EXT_BVH_NODE_FP16, RG_RESULT_WRONG_FUNCTION_ARGUMENT,
@@ -552,6 +553,16 @@ void TraceRaysImpl( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_ray
     a.segRoots      = d_segRays ? bvh.sceneRoots8.as< uint32_t >() : nullptr;
     a.segTris       = d_segRays ? bvh.sceneOffsets.as< uint32_t >() : nullptr;
     a.segCount      = d_segRays ? bvh.sceneCount : 0u;
+    if( d_frameRoot )
+    {
+        // one segment: every ray starts at *d_frameRoot; the third word of
+        // the frame roots is 0, so primitive ids stay global (a single
+        // segment never reads segRays)
+        a.segRays  = d_frameRoot;
+        a.segRoots = d_frameRoot;
+        a.segTris  = bvh.frameRoots.as< uint32_t >() + 2;
+        a.segCount = 1;
+    }
     a.chunk         = s.rayChunk >= 16 && s.rayChunk <= 1024 ? s.rayChunk : RAY_CHUNK_DEFAULT;
     if( accum )
     {
@@ -581,6 +592,33 @@ void TraceRays( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, u
                 hipEvent_t evBefore, hipEvent_t evAfter, const TraceAccum* accum )
 {
     RG_REQUIRE( bvh.sceneCount == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "a batched BVH is traced with TraceRaysBatch" );
+    TraceRaysImpl( bvh, s, d_rays, count, d_hits, flags, stream, evBefore, evAfter, accum, nullptr );
+}
+
+void TraceRaysFrame( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags, bool opaqueOnly,
+                     hipStream_t stream, hipEvent_t evBefore, hipEvent_t evAfter, const TraceAccum* accum )
+{
+    if( bvh.frameMode == BvhObject::FRAME_SPLIT )
+    {
+        TraceRaysImpl( bvh, s, d_rays, count, d_hits, flags, stream, evBefore, evAfter, accum, nullptr,
+                       bvh.frameRoots.as< uint32_t >() + ( opaqueOnly ? 1 : 0 ) );
+        return;
+    }
+    RG_REQUIRE( bvh.sceneCount == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "a batched BVH is traced with TraceRaysBatch" );
+    if( opaqueOnly && bvh.frameMode == BvhObject::FRAME_REFRACT_ONLY )
+    {
+        // nothing opaque to hit
+        if( evBefore )
+        {
+            RG_HIP_CHECK( hipEventRecord( evBefore, stream ) );
+        }
+        RG_HIP_CHECK( hipMemsetAsync( d_hits, 0xFF, size_t( count ) * sizeof( RgExtHit ), stream ) );
+        if( evAfter )
+        {
+            RG_HIP_CHECK( hipEventRecord( evAfter, stream ) );
+        }
+        return;
+    }
     TraceRaysImpl( bvh, s, d_rays, count, d_hits, flags, stream, evBefore, evAfter, accum, nullptr );
 }
 

@@ -92,6 +92,10 @@ class RgExtFrameParams(C.Structure):
         ("restirInitialCandidates", C.c_uint32),
         ("restirTemporalSamples", C.c_uint32), ("restirSpatialSamples", C.c_uint32), ("debugShowFlags", C.c_uint32),
         ("emissionMapBoost", C.c_float),
+        ("reflectRefractMaxDepth", C.c_uint32), ("cameraMediaType", C.c_uint32),
+        ("indexOfRefractionGlass", C.c_float), ("indexOfRefractionWater", C.c_float), ("thinMediaWidth", C.c_float),
+        ("pad0", C.c_float * 3), ("waterColorAndDensity", C.c_float * 4), ("acidColorAndDensity", C.c_float * 4),
+        ("worldUp", C.c_float * 4),
     ]
 
 

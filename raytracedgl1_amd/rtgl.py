@@ -93,6 +93,39 @@ class RgDrawFrameIlluminationParams(C.Structure):
                 ("pLightstyleValues", _P)]
 
 
+class RgDrawFrameReflectRefractParams(C.Structure):
+    _fields_ = [("sType", C.c_int), ("pNext", _P), ("maxReflectRefractDepth", _U32), ("typeOfMediaAroundCamera", C.c_int),
+                ("indexOfRefractionGlass", _F), ("indexOfRefractionWater", _F), ("thinMediaWidth", _F),
+                ("waterWaveSpeed", _F), ("waterWaveNormalStrength", _F), ("waterColor", RgFloat3D), ("acidColor", RgFloat3D),
+                ("acidDensity", _F), ("waterWaveTextureDerivativesMultiplier", _F), ("waterTextureAreaScale", _F),
+                ("portalNormalTwirl", _U32)]
+
+
+def reflect_refract_params(**kw) -> RgDrawFrameReflectRefractParams:
+    """RgDrawFrameReflectRefractParams with the reference defaults
+    (Source/DrawFrameInfo.h:125-146), overridden by keyword."""
+    r = RgDrawFrameReflectRefractParams()
+    r.sType = RG_STRUCTURE_TYPE_REFLECTREFRACT
+    r.maxReflectRefractDepth = 2
+    r.typeOfMediaAroundCamera = RG_MEDIA_TYPE_VACUUM
+    r.indexOfRefractionGlass = 1.52
+    r.indexOfRefractionWater = 1.33
+    r.thinMediaWidth = 0.1
+    r.waterWaveSpeed = 1.0
+    r.waterWaveNormalStrength = 1.0
+    r.waterColor.data[:] = (0.3, 0.73, 0.63)
+    r.acidColor.data[:] = (0.0, 0.66, 0.55)
+    r.acidDensity = 10.0
+    r.waterWaveTextureDerivativesMultiplier = 1.0
+    r.waterTextureAreaScale = 1.0
+    for k, v in kw.items():
+        if k in ("waterColor", "acidColor"):
+            getattr(r, k).data[:] = v
+        else:
+            setattr(r, k, v)
+    return r
+
+
 class RgExtDrawFramePathParams(C.Structure):
     _fields_ = [("sType", C.c_int), ("pNext", _P), ("atrousIterations", _U32), ("restirInitialCandidates", _U32),
                 ("restirTemporalSamples", _U32), ("restirSpatialSamples", _U32),
@@ -124,6 +157,18 @@ assert C.sizeof(RgExtDrawFramePathParams) == 48 and C.sizeof(RgExtFrameStats) ==
 RG_STRUCTURE_TYPE_ILLUMINATION = 2
 RG_STRUCTURE_TYPE_TONEMAPPING = 4
 RG_STRUCTURE_TYPE_SKY = 7
+RG_STRUCTURE_TYPE_REFLECTREFRACT = 6
+# RgMeshPrimitiveFlags (RTGL1.h) used by the reflection / refraction chain
+RG_MESH_PRIMITIVE_MIRROR = 32
+RG_MESH_PRIMITIVE_GLASS = 64
+RG_MESH_PRIMITIVE_WATER = 128
+RG_MESH_PRIMITIVE_FORCE_EXACT_NORMALS = 512
+RG_MESH_PRIMITIVE_MIRROR_IF_SMOOTH = 1024
+RG_MESH_PRIMITIVE_GLASS_IF_SMOOTH = 2048
+RG_MESH_PRIMITIVE_ACID = 8192
+RG_MESH_PRIMITIVE_THIN_MEDIA = 16384
+# RgMediaType
+RG_MEDIA_TYPE_VACUUM, RG_MEDIA_TYPE_WATER, RG_MEDIA_TYPE_GLASS, RG_MEDIA_TYPE_ACID = 0, 1, 2, 3
 RG_EXT_STRUCTURE_TYPE_PATH_PARAMS = 0x52470001
 RG_DRAW_FRAME_RAY_CULL_WORLD_0_BIT = 1
 
@@ -200,8 +245,9 @@ class SceneUploader:
     """Prebuilt per-object vertex arrays of a scenes.Scene, uploaded each frame
     with rgUploadMeshPrimitive (non-indexed triangle lists, identity transform)."""
 
-    def __init__(self, scene):
+    def __init__(self, scene, flags: dict | None = None):
         self.scene = scene
+        self.flags = dict(flags or {})  # object index -> RgMeshPrimitiveFlags
         self.objects = []
         for oid, (name, first, count) in enumerate(scene.objects):
             v = np.zeros(count * 3, VERTEX_DTYPE)
@@ -241,6 +287,7 @@ class SceneUploader:
             self.prim.indexCount = 0
             self.prim.color = color | 0xFF000000
             self.prim.emissive = 0.0
+            self.prim.flags = self.flags.get(oid, 0)
             ffi.check(prefix + "UploadMeshPrimitive", mesh_fn(inst, C.byref(self.mesh), C.byref(self.prim)))
         lights = self.scene.lights or {}
         sun = lights.get("sun")
@@ -288,14 +335,25 @@ class Renderer:
         self.sky.skyColorSaturation = 1.0
         self.sky.pNext = C.cast(C.pointer(self.path), _P)
         self.illum = None
+        self.rr = None
         self.time = 0.0
+
+    def _link(self) -> None:
+        chain = [self.sky, self.path] + [x for x in (self.illum, self.rr) if x is not None]
+        for a, b in zip(chain, chain[1:] + [None]):
+            a.pNext = C.cast(C.pointer(b), _P) if b is not None else None
+
+    def set_reflect_refract(self, params: RgDrawFrameReflectRefractParams | None) -> None:
+        """Chains RgDrawFrameReflectRefractParams (None: the defaults apply)."""
+        self.rr = params
+        self._link()
 
     def set_lightstyles(self, values) -> None:
         """Chains RgDrawFrameIlluminationParams (defaults of DrawFrameInfo.h)
         with these light style values into every following rgDrawFrame."""
         if values is None:
-            self.path.pNext = None
             self.illum = None
+            self._link()
             return
         self.styles = (C.c_float * max(1, len(values)))(*values)
         il = RgDrawFrameIlluminationParams()
@@ -310,7 +368,7 @@ class Renderer:
         il.lightstyleValuesCount = len(values)
         il.pLightstyleValues = C.cast(self.styles, _P)
         self.illum = il
-        self.path.pNext = C.cast(C.pointer(il), _P)
+        self._link()
 
     def set_static_geometry(self, enable: bool) -> None:
         ffi.check("rgExtSetStaticGeometry", api().rgExtSetStaticGeometry(self.h, 1 if enable else 0))
