@@ -496,10 +496,17 @@ const Light& LightAt( const Ctx& c, uint32_t i )
     return i < c.lights.size() ? c.lights[ i ] : none;
 }
 
-void Trace( const Ctx& c, std::vector< RgExtRay >& rays, std::vector< RgExtHit >& hits, uint32_t flags )
+// allGeometry: primary and reflection / refraction rays; the others skip the
+// refracting triangles (INSTANCE_MASK_REFRACT, RaygenCommon.h:103-170)
+void Trace( const Ctx& c, std::vector< RgExtRay >& rays, std::vector< RgExtHit >& hits, uint32_t flags, bool allGeometry = false )
 {
     hits.resize( rays.size() );
-    if( c.sc->triCount == 0 )
+    if( !allGeometry && c.sc->bvhOpaque )
+    {
+        or_trace_rays( c.sc->bvhOpaque, rays.data(), uint32_t( rays.size() ), hits.data(), flags | c.traceFlags, c.threads, nullptr, nullptr );
+        return;
+    }
+    if( c.sc->triCount == 0 || ( !allGeometry && c.sc->opaqueNone ) )
     {
         for( auto& h : hits )
         {
@@ -661,6 +668,252 @@ Surface FetchSurface( const Ctx& c, int x, int y )
     s.normal    = DecodeNormal( c.fb.get( RG_EXT_PLANE_NORMAL, x, y, 0 ) );
     s.toViewer  = -mk( c.fb.get( RG_EXT_PLANE_VIEW_DIRECTION, x, y, 0 ), c.fb.get( RG_EXT_PLANE_VIEW_DIRECTION, x, y, 1 ), c.fb.get( RG_EXT_PLANE_VIEW_DIRECTION, x, y, 2 ) );
     return s;
+}
+
+// ---------------------------------------------------------------- reflection / refraction
+// RAYGEN_REFL_REFR_SHADER (RaygenPrimary.inl:352-599) in FP64 over FP32
+// rays; portals, animated water normals, screen emission and acid fog are
+// not part of the path.  Media.h:28-124 and BRDF.h:73-84 give the media,
+// Fresnel and refraction terms; GeomInfoManager.cpp:131-186 maps the
+// primitive flags.
+const uint32_t MEDIA_VACUUM = 0, MEDIA_WATER = 1, MEDIA_GLASS = 2, MEDIA_ACID = 3;
+
+uint32_t MediaOf( uint32_t f, double rough )
+{
+    if( f & RG_MESH_PRIMITIVE_WATER ) return MEDIA_WATER;
+    if( ( f & RG_MESH_PRIMITIVE_GLASS ) || ( ( f & RG_MESH_PRIMITIVE_GLASS_IF_SMOOTH ) && rough < 0.1 ) ) return MEDIA_GLASS;
+    if( f & RG_MESH_PRIMITIVE_ACID ) return MEDIA_ACID;
+    return MEDIA_VACUUM;
+}
+bool Refracts( uint32_t f, double rough )
+{
+    return ( f & ( RG_MESH_PRIMITIVE_WATER | RG_MESH_PRIMITIVE_ACID | RG_MESH_PRIMITIVE_GLASS ) ) ||
+           ( ( f & RG_MESH_PRIMITIVE_GLASS_IF_SMOOTH ) && rough < 0.1 );
+}
+bool Reflects( uint32_t f, double rough )
+{
+    return ( f & ( RG_MESH_PRIMITIVE_MIRROR | RG_MESH_PRIMITIVE_WATER | RG_MESH_PRIMITIVE_ACID | RG_MESH_PRIMITIVE_GLASS ) ) ||
+           ( ( f & RG_MESH_PRIMITIVE_MIRROR_IF_SMOOTH ) && rough < 0.1 );
+}
+double Ior( const RgExtFrameParams& p, uint32_t m )
+{
+    return m == MEDIA_WATER || m == MEDIA_ACID ? p.indexOfRefractionWater : ( m == MEDIA_GLASS ? p.indexOfRefractionGlass : 1.0 );
+}
+D3 Transmittance( const RgExtFrameParams& p, uint32_t m, double dist )
+{
+    if( m != MEDIA_WATER && m != MEDIA_ACID ) return mk( 1, 1, 1 );
+    const float* col = m == MEDIA_WATER ? p.waterColorAndDensity : p.acidColorAndDensity;
+    const double k   = m == MEDIA_ACID ? std::max( 1.0, std::sqrt( double( col[ 3 ] ) ) ) : 1.0;
+    return mk( std::exp( dist * std::log( double( col[ 0 ] ) ) * k ), std::exp( dist * std::log( double( col[ 1 ] ) ) * k ),
+               std::exp( dist * std::log( double( col[ 2 ] ) ) * k ) );
+}
+// Snell's law in vector form; false on total internal reflection
+bool Refract( double n1, double n2, D3 I, D3 N, D3* T )
+{
+    const double eta = n1 / n2, cosI = -dot( I, N );
+    const double k   = 1.0 - eta * eta * ( 1.0 - cosI * cosI );
+    if( k < 0.0 ) return false;
+    *T = I * eta + N * ( eta * cosI - std::sqrt( k ) );
+    return true;
+}
+
+struct RrPix
+{
+    D3       thr, dir, vpos;
+    double   fpl;
+    uint32_t media;
+    bool     active, split;
+};
+
+void ReflectRefract( Ctx& c, std::vector< RgExtRay >& rays, std::vector< RgExtHit >& hits )
+{
+    const RgExtFrameParams& p = c.p;
+    const Fb&               fb = c.fb;
+    const int               W = fb.W, H = fb.H;
+    bool                    any = false;
+    for( uint32_t g = 0; g < c.sc->geomCount; g++ )
+    {
+        any = any || ( c.sc->geoms[ g ].flags & ( RG_MESH_PRIMITIVE_MIRROR | RG_MESH_PRIMITIVE_WATER | RG_MESH_PRIMITIVE_GLASS | RG_MESH_PRIMITIVE_ACID |
+                                                  RG_MESH_PRIMITIVE_GLASS_IF_SMOOTH | RG_MESH_PRIMITIVE_MIRROR_IF_SMOOTH ) ) != 0;
+    }
+    if( p.reflectRefractMaxDepth == 0 || !any || c.sc->triCount == 0 ) return;
+    std::vector< RrPix > st( size_t( W ) * H );
+    const D3             up = mk( p.worldUp[ 0 ], p.worldUp[ 1 ], p.worldUp[ 2 ] );
+    for( uint32_t depth = 0; depth < std::min( p.reflectRefractMaxDepth, 16u ); depth++ )
+    {
+        ParallelFor( H, c.threads, [ & ]( int y ) {
+            for( int x = 0; x < W; x++ )
+            {
+                const size_t i = size_t( y ) * W + x;
+                int          cx, cy;
+                or_checkerboard_from_regular( x, y, W, &cx, &cy );
+                RrPix&   s = st[ i ];
+                D3       pos, nrm, alb;
+                double   rough, metal;
+                uint32_t flags;
+                if( depth == 0 )
+                {
+                    s.active = false;
+                    rays[ i ] = NullRay();
+                    if( fb.get( RG_EXT_PLANE_IS_SKY, cx, cy, 0 ) != 0.0 ) continue;
+                    // the primary surface as the G-buffer holds it
+                    alb   = mk( fb.get( RG_EXT_PLANE_ALBEDO, x, y, 0 ), fb.get( RG_EXT_PLANE_ALBEDO, x, y, 1 ), fb.get( RG_EXT_PLANE_ALBEDO, x, y, 2 ) );
+                    pos   = mk( fb.get( RG_EXT_PLANE_SURFACE_POSITION, cx, cy, 0 ), fb.get( RG_EXT_PLANE_SURFACE_POSITION, cx, cy, 1 ),
+                                fb.get( RG_EXT_PLANE_SURFACE_POSITION, cx, cy, 2 ) );
+                    nrm   = DecodeNormal( fb.get( RG_EXT_PLANE_NORMAL, cx, cy, 0 ) );
+                    metal = fb.get( RG_EXT_PLANE_METALLIC_ROUGHNESS, cx, cy, 0 );
+                    rough = fb.get( RG_EXT_PLANE_METALLIC_ROUGHNESS, cx, cy, 1 );
+                    flags = c.sc->geoms[ c.sc->triGeom[ hits[ i ].primitiveId ] ].flags;
+                    s.thr   = mk( fb.get( RG_EXT_PLANE_THROUGHPUT, cx, cy, 0 ), fb.get( RG_EXT_PLANE_THROUGHPUT, cx, cy, 1 ),
+                                  fb.get( RG_EXT_PLANE_THROUGHPUT, cx, cy, 2 ) );
+                    s.dir   = RayDir( p, ( double( x ) + 0.5 + p.jitterX ) / W, ( double( y ) + 0.5 + p.jitterY ) / H );
+                    s.fpl   = fb.get( RG_EXT_PLANE_DEPTH_WORLD, cx, cy, 0 );
+                    s.vpos  = pos;
+                    s.media = p.cameraMediaType & 3u;
+                    s.split = false;
+                }
+                else
+                {
+                    if( !s.active )
+                    {
+                        rays[ i ] = NullRay();
+                        continue;
+                    }
+                    const RgExtRay& r = rays[ i ];
+                    const Hit       h = GetHit( c, hits[ i ].primitiveId, hits[ i ].u, hits[ i ].v, mk( r.origin[ 0 ], r.origin[ 1 ], r.origin[ 2 ] ) );
+                    pos = h.position, nrm = h.normal, alb = h.albedo, rough = h.roughness, metal = h.metallic;
+                    flags = c.sc->geoms[ h.geom ].flags;
+                }
+                // getNewRayMedia (RaygenPrimary.inl:141-150)
+                uint32_t   newMedia = ( depth == 0 && p.cameraMediaType != MEDIA_VACUUM ) ? MEDIA_VACUUM : MediaOf( flags, rough );
+                const bool toRefract = Refracts( flags, rough ), toReflect = Reflects( flags, rough );
+                if( !toReflect && !toRefract )
+                {
+                    s.active  = false;
+                    rays[ i ] = NullRay();
+                    continue;
+                }
+                const double n1 = Ior( p, s.media ), n2 = Ior( p, newMedia );
+                const bool   water = newMedia == MEDIA_WATER || s.media == MEDIA_WATER || newMedia == MEDIA_ACID || s.media == MEDIA_ACID;
+                if( dot( nrm, -s.dir ) < 0.0 ) nrm = -nrm;
+                nrm = normalize( nrm );
+                D3     origin = pos, t;
+                bool   doSplit = !s.split, doRefr = false;
+                double F       = 1.0;
+                if( toRefract && Refract( n1, n2, s.dir, nrm, &t ) )
+                {
+                    doRefr         = cx >= W / 2; // odd checkerboard pixels refract
+                    const double r0 = square( ( n1 - n2 ) / ( n1 + n2 ) );
+                    F               = r0 + ( 1.0 - r0 ) * std::pow( 1.0 - std::fabs( dot( nrm, -s.dir ) ), 5.0 );
+                }
+                else
+                {
+                    doSplit = false; // total internal reflection
+                }
+                if( doRefr )
+                {
+                    s.dir = t;
+                    s.thr = s.thr * ( 1.0 - F );
+                    if( flags & RG_MESH_PRIMITIVE_THIN_MEDIA )
+                    {
+                        const double len = p.thinMediaWidth / std::max( 0.001, -dot( nrm, s.dir ) );
+                        origin           = origin + s.dir * len;
+                        s.thr            = s.thr * Transmittance( p, newMedia, len );
+                        s.fpl += len;
+                        if( Refract( n2, n1, s.dir, nrm, &t ) ) s.dir = t;
+                        newMedia = s.media;
+                    }
+                    s.media = newMedia;
+                }
+                else
+                {
+                    const D3 in = normalize( s.dir );
+                    s.dir       = in - nrm * ( 2.0 * dot( nrm, in ) );
+                    if( !water )
+                    {
+                        const D3     f0 = SpecColor( alb, metal );
+                        const double k  = std::pow( 1.0 - std::max( dot( nrm, s.dir ), 0.0 ), 5.0 );
+                        s.thr           = s.thr * ( f0 + ( mk( 1, 1, 1 ) - f0 ) * k );
+                    }
+                    else
+                    {
+                        s.thr  = s.thr * F;
+                        origin = origin + s.dir * 0.005; // the reference's SHIPPING_HACK offset
+                    }
+                }
+                if( doSplit )
+                {
+                    s.thr   = s.thr * 2.0;
+                    s.split = true;
+                }
+                s.active  = true;
+                rays[ i ] = MakeRay( origin, s.dir, 0.001, p.rayLength );
+            }
+        } );
+        Trace( c, rays, hits, RG_EXT_TRACE_CLOSEST, true );
+        ParallelFor( H, c.threads, [ & ]( int y ) {
+            for( int x = 0; x < W; x++ )
+            {
+                const size_t i = size_t( y ) * W + x;
+                RrPix&       s = st[ i ];
+                if( !s.active ) continue;
+                int cx, cy;
+                or_checkerboard_from_regular( x, y, W, &cx, &cy );
+                const double thrW = s.split ? 1.0 : -1.0;
+                if( hits[ i ].primitiveId == RG_EXT_MISS )
+                {
+                    const double a = std::fabs( dot( s.dir, up ) );
+                    s.thr          = s.thr * Transmittance( p, s.media, 1.0 / ( a * a * a ) );
+                    // storeSky (RaygenPrimary.inl:90-139)
+                    const double cu = ( double( cx ) + 0.5 + p.jitterX ) / W, cv = ( double( cy ) + 0.5 + p.jitterY ) / H;
+                    const D3     rd = RayDir( p, cu, cv );
+                    const D3     vc = MulCM( p.view, rd, 0.0, nullptr ), vp = MulCM( p.viewPrev, rd, 0.0, nullptr );
+                    const double ccx = p.projection[ 0 ] * vc.x + p.projection[ 4 ] * vc.y + p.projection[ 8 ] * vc.z;
+                    const double ccy = p.projection[ 1 ] * vc.x + p.projection[ 5 ] * vc.y + p.projection[ 9 ] * vc.z;
+                    const double cpx = p.projectionPrev[ 0 ] * vp.x + p.projectionPrev[ 4 ] * vp.y + p.projectionPrev[ 8 ] * vp.z;
+                    const double cpy = p.projectionPrev[ 1 ] * vp.x + p.projectionPrev[ 5 ] * vp.y + p.projectionPrev[ 9 ] * vp.z;
+                    const D3     sky = SkyColor( p );
+                    fb.set4( RG_EXT_PLANE_IS_SKY, cx, cy, 1 );
+                    fb.set4( RG_EXT_PLANE_ALBEDO, x, y, sky.x, sky.y, sky.z, 0 );
+                    fb.set4( RG_EXT_PLANE_NORMAL, cx, cy, EncodeNormal( mk( 0, 0, 0 ) ) );
+                    fb.set4( RG_EXT_PLANE_METALLIC_ROUGHNESS, cx, cy, 0, 0 );
+                    fb.set4( RG_EXT_PLANE_DEPTH_WORLD, cx, cy, MAX_RAY_LEN * 2.0 );
+                    fb.set4( RG_EXT_PLANE_MOTION, cx, cy, ( cpx * 0.5 + 0.5 ) - ( ccx * 0.5 + 0.5 ), ( cpy * 0.5 + 0.5 ) - ( ccy * 0.5 + 0.5 ), 0, 0 );
+                    fb.set4( RG_EXT_PLANE_SURFACE_POSITION, cx, cy, 1e7, 1e7, 1e7, 1e7 );
+                    fb.set4( RG_EXT_PLANE_VISIBILITY_BUFFER, cx, cy, 4294967295.0, 4294967295.0, 4294967295.0, 4294967295.0 );
+                    fb.set4( RG_EXT_PLANE_VIEW_DIRECTION, cx, cy, s.dir.x, s.dir.y, s.dir.z, 0 );
+                    fb.set4( RG_EXT_PLANE_THROUGHPUT, cx, cy, s.thr.x, s.thr.y, s.thr.z, thrW );
+                    s.active = false;
+                    continue;
+                }
+                const RgExtRay& r      = rays[ i ];
+                const D3        origin = mk( r.origin[ 0 ], r.origin[ 1 ], r.origin[ 2 ] );
+                const Hit       h      = GetHit( c, hits[ i ].primitiveId, hits[ i ].u, hits[ i ].v, origin );
+                const double    len    = length( h.position - origin );
+                // virtual position for motion: along the camera ray by the path length
+                const D3 cam = RayDir( p, ( double( x ) + 0.5 + p.jitterX ) / W, ( double( y ) + 0.5 + p.jitterY ) / H );
+                s.vpos       = s.vpos + cam * len;
+                double   wc, wp, cwc, cwp;
+                const D3 vCur  = MulCM( p.view, s.vpos, 1.0, &wc );
+                const D3 vPrev = MulCM( p.viewPrev, s.vpos, 1.0, &wp );
+                const D3 cCur  = MulCM( p.projection, vCur, wc, &cwc );
+                const D3 cPrev = MulCM( p.projectionPrev, vPrev, wp, &cwp );
+                s.thr          = s.thr * Transmittance( p, s.media, len );
+                s.fpl += len;
+                fb.set4( RG_EXT_PLANE_IS_SKY, cx, cy, 0 );
+                fb.set4( RG_EXT_PLANE_ALBEDO, x, y, h.albedo.x, h.albedo.y, h.albedo.z, 0 );
+                fb.set4( RG_EXT_PLANE_NORMAL, cx, cy, EncodeNormal( h.normal ) );
+                fb.set4( RG_EXT_PLANE_METALLIC_ROUGHNESS, cx, cy, h.metallic, h.roughness );
+                fb.set4( RG_EXT_PLANE_DEPTH_WORLD, cx, cy, s.fpl );
+                fb.set4( RG_EXT_PLANE_MOTION, cx, cy, ( cPrev.x / cwp ) * 0.5 - ( cCur.x / cwc ) * 0.5, ( cPrev.y / cwp ) * 0.5 - ( cCur.y / cwc ) * 0.5,
+                         length( vPrev ) - length( vCur ), 0 );
+                fb.set4( RG_EXT_PLANE_SURFACE_POSITION, cx, cy, h.position.x, h.position.y, h.position.z, 0 );
+                fb.set4( RG_EXT_PLANE_VISIBILITY_BUFFER, cx, cy, 1, double( ( h.localPrim << 12 ) | ( h.geom & 0xFFFu ) ), hits[ i ].u, hits[ i ].v );
+                fb.set4( RG_EXT_PLANE_VIEW_DIRECTION, cx, cy, s.dir.x, s.dir.y, s.dir.z, 0 );
+                fb.set4( RG_EXT_PLANE_THROUGHPUT, cx, cy, s.thr.x, s.thr.y, s.thr.z, thrW );
+            }
+        } );
+    }
 }
 
 // ---------------------------------------------------------------- light grid (LightGrid.h:31-71)
@@ -963,7 +1216,7 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
             r.tmin = p.primaryRayMinDist, r.tmax = p.rayLength;
         }
     }
-    Trace( c, rays, hits, RG_EXT_TRACE_CLOSEST );
+    Trace( c, rays, hits, RG_EXT_TRACE_CLOSEST, true );
     ParallelFor( H, threads, [ & ]( int y ) {
         for( int x = 0; x < W; x++ )
         {
@@ -1045,6 +1298,9 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
             fb.set4( RG_EXT_PLANE_THROUGHPUT, cx, cy, 1, 1, 1, 0 );
         }
     } );
+
+    // ---- reflection / refraction chain
+    ReflectRefract( c, rays, hits );
 
     // ---- light grid
     LightGrid( c );

@@ -37,6 +37,11 @@ typedef struct OrPtScene
     const RgDirectionalLightUploadInfo* dirLight; /* may be null */
     const RgSphericalLightUploadInfo*   spheres;
     uint32_t                            sphereCount;
+    /* Refracting geometry (water, glass, acid) occupies the last triangles;
+     * shadow, bounce and gradient rays trace bvhOpaque, a tree over the
+     * others (null: they trace bvh), or miss everything with opaqueNone. */
+    const struct OrBvh*                 bvhOpaque;
+    uint32_t                            opaqueNone;
 } OrPtScene;
 
 typedef struct OrPathTracer OrPathTracer;

@@ -236,7 +236,17 @@ class OrGeomC(C.Structure):
 
 class OrPtSceneC(C.Structure):
     _fields_ = [("tris", _P), ("normals", _P), ("triGeom", _P), ("triCount", _U32), ("geoms", _P), ("geomIds", _P),
-                ("geomCount", _U32), ("dirLight", _P), ("spheres", _P), ("sphereCount", _U32)]
+                ("geomCount", _U32), ("dirLight", _P), ("spheres", _P), ("sphereCount", _U32),
+                ("bvhOpaque", _P), ("opaqueNone", _U32)]
+
+# RgMeshPrimitiveFlags of the refracting primitives that are not alpha tested
+# (VertexCollectorFilterType.cpp:215-229): water 128, glass 64,
+# glass-if-smooth 2048, acid 8192; alpha tested 1.
+_REFRACT_FLAGS, _ALPHA_TESTED = 128 | 64 | 2048 | 8192, 1
+
+
+def refracts(flags: int) -> bool:
+    return (flags & _ALPHA_TESTED) == 0 and (flags & _REFRACT_FLAGS) != 0
 
 
 class OraclePathTracer:
@@ -245,27 +255,40 @@ class OraclePathTracer:
     non-indexed primitive per object, identity transform, opaque colour,
     roughness 1, metallic 0."""
 
-    def __init__(self, o: Oracle, scene, width: int, height: int, leaf: int = 3):
+    def __init__(self, o: Oracle, scene, width: int, height: int, leaf: int = 3, flags: dict | None = None):
+        """flags: object index -> RgMeshPrimitiveFlags, as rtgl.SceneUploader
+        takes them.  Refracting objects follow the others, in upload order,
+        as the frame assembles them."""
         self.o, self.scene = o, scene
+        flags = dict(flags or {})
         self.h = o.l.or_pt_create(width, height)
-        self.bvh = OracleBvh(o, scene.tris, leaf)
-        n = len(scene.tris)
-        self.tris = np.ascontiguousarray(scene.tris, np.float32)
-        self.normals = np.ascontiguousarray(scene.normals, np.float32)
+        objs = list(enumerate(scene.objects))
+        objs = [ob for ob in objs if not refracts(flags.get(ob[0], 0))] + [ob for ob in objs if refracts(flags.get(ob[0], 0))]
+        order = np.concatenate([np.arange(first, first + count) for _, (_, first, count) in objs]) if objs else np.zeros(0, np.int64)
+        n = len(order)
+        self.tris = np.ascontiguousarray(scene.tris[order], np.float32)
+        self.normals = np.ascontiguousarray(scene.normals[order], np.float32)
+        self.bvh = OracleBvh(o, self.tris, leaf)
         self.tri_geom = np.zeros(n, np.uint32)
         self.geoms = (OrGeomC * max(1, len(scene.objects)))()
         self.ids = np.zeros(max(1, len(scene.objects)), np.uint64)
-        for g, (_, first, count) in enumerate(scene.objects):
-            self.tri_geom[first:first + count] = g
+        pos, opaque = 0, 0
+        for g, (oid, (_, first, count)) in enumerate(objs):
+            self.tri_geom[pos:pos + count] = g
             gr = self.geoms[g]
             for k in range(12):
                 gr.prevFromCur[k] = 1.0 if k % 5 == 0 else 0.0
             gr.isStatic, gr.color = 1, int(scene.colors[first]) | 0xFF000000
             gr.emissive, gr.roughness, gr.metallic = 0.0, 1.0, 0.0
-            gr.flags, gr.firstTri, gr.triCount = 0, first, count
-            self.ids[g] = g << 32
+            gr.flags, gr.firstTri, gr.triCount = flags.get(oid, 0), pos, count
+            self.ids[g] = oid << 32
+            opaque += 0 if refracts(gr.flags) else count
+            pos += count
+        # the frame's split: a subtree per side when both have >= 2 triangles
+        self.bvh_opaque = OracleBvh(o, self.tris[:opaque], leaf) if opaque >= 2 and n - opaque >= 2 else None
         self.c = OrPtSceneC(self.tris.ctypes.data, self.normals.ctypes.data, self.tri_geom.ctypes.data, n,
-                            C.cast(self.geoms, _P), self.ids.ctypes.data, len(scene.objects), None, None, 0)
+                            C.cast(self.geoms, _P), self.ids.ctypes.data, len(scene.objects), None, None, 0,
+                            self.bvh_opaque.h if self.bvh_opaque else None, 1 if (opaque == 0 and n > 0) else 0)
         self.scene_lights(scene)
 
     def scene_lights(self, scene):
