@@ -20,6 +20,8 @@ Beside the FP32 line (`value`), the same JSON object carries
 - `fp16`: the same frames with FP16 BVH nodes and FP16 storage of the
   denoiser's temporal and a-trous planes (frames/s, Mrays/s, and the last
   frame's denoised-radiance difference from the FP32 run);
+- `reflrefr`: the same frames with every third icosphere a mirror, glass or
+  water (reflection / refraction depth 2);
 - `serving`: 256 camera sessions (960x540 .. 2560x1440) over 16 scenes of
   200k-2M triangles, admitted against a 200 GiB budget, with Poisson frame
   requests at 30 Hz per session for a few seconds: p50 / p99 frame latency
@@ -28,7 +30,8 @@ Beside the FP32 line (`value`), the same JSON object carries
   builds, and their primary rays (one camera per scene, 256x256 .. 1080p)
   traced in one launch and as 64, before and after the traversal autotuner.
 
-Usage: python bench.py [--gpus 1] [--steps 60] [--warmup 5] [--no-cpu-baseline] [--no-fp16] [--no-serving] [--no-batch]
+Usage: python bench.py [--gpus 1] [--steps 60] [--warmup 5] [--no-cpu-baseline] [--no-fp16] [--no-reflrefr]
+                       [--no-serving] [--no-batch]
 """
 from __future__ import annotations
 
@@ -64,6 +67,7 @@ def parse_args(argv=None):
     ap.add_argument("--dynamic-upload", action="store_true",
                     help="re-upload every primitive each frame (the host-to-device-inclusive rate; not the headline)")
     ap.add_argument("--no-fp16", action="store_true", help="skip the FP16 precision-form run")
+    ap.add_argument("--no-reflrefr", action="store_true", help="skip the reflection / refraction run")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving run")
     ap.add_argument("--serving-sessions", type=int, default=256)
     ap.add_argument("--serving-scenes", type=int, default=16)
@@ -168,13 +172,14 @@ def calibrate_copy(mib: int) -> int:
     return 0
 
 
-def run_frames(a, scene, precision: int):
+def run_frames(a, scene, precision: int, flags=None, planes=("PRE_FINAL",)):
     """Warm-up, then a.steps timed rgDrawFrame calls of the orbit; returns the
     per-frame stats, the wall time of the timed frames and the last frame's
-    PRE_FINAL plane."""
+    PRE_FINAL plane (with `planes` beyond it: a dict of them).  flags maps
+    object indices to RgMeshPrimitiveFlags."""
     from raytracedgl1_amd import ffi, rtgl, scenes
 
-    up = rtgl.SceneUploader(scene)
+    up = rtgl.SceneUploader(scene, flags)
     r = rtgl.Renderer(WIDTH, HEIGHT)
     try:
         r.set_static_geometry(not a.dynamic_upload)
@@ -203,8 +208,10 @@ def run_frames(a, scene, precision: int):
         wall = time.perf_counter() - t0
         print(f"timed {a.steps} frames in {wall:.3f} s (precision flags {precision})", file=sys.stderr, flush=True)
         assert all(st.bvhRebuilt == 0 for st in stats), "static geometry was rebuilt inside the timed region"
-        final = r.framebuffers().download(ffi.PLANE["PRE_FINAL"]).copy()
-        return stats, wall, final
+        fb = r.framebuffers()
+        if planes == ("PRE_FINAL",):
+            return stats, wall, fb.download(ffi.PLANE["PRE_FINAL"]).copy()
+        return stats, wall, {k: fb.download(ffi.PLANE[k]).copy() for k in planes}
     finally:
         r.close()
 
@@ -471,6 +478,29 @@ def main(argv=None) -> int:
             "node_visits_per_ray": sum(s_.nodeVisits for s_ in st16) / rays16,
             "pre_final_rel_err_vs_fp32": {"p50": float(np.median(rel)), "p99": float(np.quantile(rel, 0.99)),
                                           "max": float(rel.max())},
+        }
+    if not a.no_reflrefr:
+        from raytracedgl1_amd import rtgl
+        import numpy as np
+
+        # every third icosphere a mirror, glass or water (depth 2, the default)
+        kinds = (rtgl.RG_MESH_PRIMITIVE_MIRROR, rtgl.RG_MESH_PRIMITIVE_GLASS, rtgl.RG_MESH_PRIMITIVE_WATER)
+        flags = {i: kinds[i % 3] for i in range(1, len(scene.objects))}
+        st_r, wall_r, pl = run_frames(a, scene, 0, flags, planes=("THROUGHPUT", "IS_SKY"))
+        n_r = len(st_r)
+        rays_r = sum(s_.raysTraced for s_ in st_r)
+        chain = np.abs(pl["THROUGHPUT"][..., 3]) == 1.0
+        out["reflrefr"] = {
+            "workload": "the same frames with every third icosphere a mirror, glass or water; reflection / refraction "
+                        "depth 2; glass and water invisible to shadow and bounce rays",
+            "frames_per_s": n_r / wall_r,
+            # every chain launch spans all pixels; finished pixels pass null rays
+            "ray_slots_per_frame": rays_r / n_r,
+            "vs_plain_frames": (n_r / wall_r) / (n / wall),
+            "gpu_ms_per_frame": sum(s_.gpuMsTotal for s_ in st_r) / n_r,
+            "traversal_ms_per_frame": sum(s_.gpuMsTraversal for s_ in st_r) / n_r,
+            "traversal_launches_per_frame": sum(s_.traversalLaunches for s_ in st_r) / n_r,
+            "pixels_with_chain_surface": float(chain.mean()),
         }
     if not a.no_serving:
         out["serving"] = serving_run(a)
