@@ -164,8 +164,11 @@ void TraceRays( const BvhObject& bvh, TraceContext& ctx, const RgExtRay* d_rays,
                 hipEvent_t evBefore = nullptr, hipEvent_t evAfter = nullptr, const TraceAccum* accum = nullptr );
 // TraceRays over a frame BVH (BuildFrameBvh); opaqueOnly rays skip the
 // refracting range.
+// alpha (with tex): per-triangle alpha-test records; the hits of
+// alpha-tested triangles whose texture is transparent there are ignored.
 void TraceRaysFrame( const BvhObject& bvh, TraceContext& ctx, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags, bool opaqueOnly,
-                     hipStream_t stream, hipEvent_t evBefore, hipEvent_t evAfter, const TraceAccum* accum );
+                     hipStream_t stream, hipEvent_t evBefore, hipEvent_t evAfter, const TraceAccum* accum,
+                     const struct AlphaRec* alpha = nullptr, const struct TexView* tex = nullptr );
 // One persistent launch over the rays of every scene of a batched BVH: rays
 // of scene s are [rayOffsets[s], rayOffsets[s+1]) (host array, sceneCount + 1
 // entries); primitive ids in the hits are relative to the scene.

@@ -42,6 +42,7 @@ struct Instance
     RgInstanceCreateInfo                        info{};
     SceneCollector                              scene;
     DeviceScene                                 dscene;
+    TextureManager                              textures;
     BvhObject                                   bvh;
     TraceContext                                tc;
     FrameGpu                                    g;
@@ -59,6 +60,8 @@ struct Instance
         {
             cam.worldUp[ i ] = ci.worldUp.data[ i ];
         }
+        scene.textures          = &textures;
+        textures.forceMinLinear = ci.textureSamplerForceMinificationFilterLinear != 0;
         int n = 0;
         if( hipGetDeviceCount( &n ) != hipSuccess || n == 0 )
         {
@@ -181,6 +184,7 @@ struct Instance
             stats.bvhRebuilt     = 1;
         }
         RG_HIP_CHECK( hipEventRecord( ev[ ST_BVH ], s ) );
+        dscene.tex          = textures.View( s );
         const uint64_t rays = EncodeFrame( g, tc, cam.params, bvh, dscene, fb->fb, di, s, ev[ ST_TRACED ], ev[ ST_DENOISED ] );
         RG_HIP_CHECK( hipEventRecord( ev[ ST_END ], s ) );
         // The staging arrays are reused by the next frame's uploads.
@@ -320,9 +324,13 @@ RgResult rgUploadPolygonalLight( RgInstance instance, const RgPolygonalLightUplo
     } );
 }
 
-RgResult rgProvideOriginalTexture( RgInstance instance, const RgOriginalTextureInfo* )
+RgResult rgProvideOriginalTexture( RgInstance instance, const RgOriginalTextureInfo* pInfo )
 {
-    return Guard( [ & ] { Get( instance ); } );
+    return Guard( [ & ] {
+        Instance& in = Get( instance );
+        RG_REQUIRE( pInfo, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "Argument is null" );
+        in.textures.Provide( *pInfo, in.stream );
+    } );
 }
 
 RgResult rgProvideOriginalCubemapTexture( RgInstance instance, const RgOriginalCubemapInfo* )
@@ -330,9 +338,12 @@ RgResult rgProvideOriginalCubemapTexture( RgInstance instance, const RgOriginalC
     return Guard( [ & ] { Get( instance ); } );
 }
 
-RgResult rgMarkOriginalTextureAsDeleted( RgInstance instance, const char* )
+RgResult rgMarkOriginalTextureAsDeleted( RgInstance instance, const char* pTextureName )
 {
-    return Guard( [ & ] { Get( instance ); } );
+    return Guard( [ & ] {
+        Instance& in = Get( instance );
+        in.textures.MarkDeleted( pTextureName );
+    } );
 }
 
 RgResult rgStartFrame( RgInstance instance, const RgStartFrameInfo* )

@@ -18,6 +18,7 @@
 #include "rg_denoise.h"
 #include "rg_pathtrace.h"
 #include "rg_shading.h"
+#include "rg_textures.h"
 
 #include <mutex>
 
@@ -32,10 +33,30 @@ constexpr uint32_t GRID_ENTRIES = GRID_X * GRID_Y * GRID_Z * GRID_CELL;
 
 struct SceneView
 {
-    const float*      tris;
-    const float*      normals;
-    const uint32_t*   triGeom;
-    const GeomRecord* geoms;
+    const float*             tris;
+    const float*             normals;
+    const uint32_t*          triGeom;
+    const GeomRecord*        geoms;
+    const RgPrimitiveVertex* verts;
+    const uint32_t*          indices;
+    TexView                  tex;
+};
+
+// How a hit samples its albedo texture: the three texture LOD sources of
+// HitInfo.inl (primary rays: gradients from ray differentials; reflection /
+// refraction rays: ray cones; bounces: getBounceLOD).
+enum TexMode : uint32_t
+{
+    TQ_LOD    = 0, // a = lod
+    TQ_GRAD   = 1, // dbx, dby: barycentric deltas of the neighbouring pixels' rays
+    TQ_CONE   = 2, // a = cone width, dir = ray direction
+    TQ_BOUNCE = 3, // a = origin roughness, b = mip bias
+};
+struct TexQuery
+{
+    uint32_t mode = TQ_LOD;
+    float    a = 0.0f, b = 0.0f;
+    v3       dbx, dby, dir;
 };
 
 struct LightView
@@ -183,9 +204,69 @@ struct HitInfo
     uint32_t localPrim;
 };
 
-// getHitInfo* (HitInfo.inl:204-466) for untextured geometry: albedo is the
-// primitive colour (layer 0), normals interpolated or exact.
-__device__ HitInfo GetHit( const SceneView& sc, const RgExtFrameParams& p, uint32_t prim, float u, float v, v3 rayOrigin )
+// getBounceLOD (HitInfo.inl:160-175), "Ray Traced Reflections in
+// 'Wolfenstein: Youngblood'"
+__device__ float BounceLod( float roughness, float viewDist, float hitDist, float screenWidth, float bias )
+{
+    const float k     = ( 1.0f - roughness ) * 0.9f + 0.1f;
+    const float range = 300.0f * ( k * k * k * k );
+    const float fx    = sqrtf( clampf( viewDist / range, 0.0f, 1.0f ) );
+    const float fy    = sqrtf( clampf( hitDist / range, 0.0f, 1.0f ) );
+    return fmaxf( log2f( 3840.0f / screenWidth ), 0.0f ) + fx * 10.0f + fy * 10.0f + bias;
+}
+
+// Layer-0 albedo with its texture (processAlbedo*, HitInfo.inl:28-128, for
+// one opaque layer): colour * texture, saturated.
+__device__ v3 TexturedAlbedo( const SceneView& sc, const RgExtFrameParams& p, const GeomRecord& gr, uint32_t localPrim, float u, float v,
+                              v3 p0, v3 p1, v3 p2, v3 position, v3 normal, v3 rayOrigin, const TexQuery& q )
+{
+    float2 t[ 3 ];
+#pragma unroll
+    for( int k = 0; k < 3; k++ )
+    {
+        const uint32_t idx = gr.baseIndex == 0xFFFFFFFFu ? 3 * localPrim + k : sc.indices[ gr.baseIndex + 3 * localPrim + k ];
+        const RgPrimitiveVertex& vx = sc.verts[ gr.baseVertex + idx ];
+        t[ k ] = make_float2( vx.texCoord[ 0 ], vx.texCoord[ 1 ] );
+    }
+    const float b0 = 1.0f - u - v;
+    const float tu = t[ 0 ].x * b0 + t[ 1 ].x * u + t[ 2 ].x * v;
+    const float tv = t[ 0 ].y * b0 + t[ 1 ].y * u + t[ 2 ].y * v;
+    float4      c;
+    if( q.mode == TQ_GRAD )
+    {
+        const float dux = t[ 0 ].x * q.dbx.x + t[ 1 ].x * q.dbx.y + t[ 2 ].x * q.dbx.z;
+        const float dvx = t[ 0 ].y * q.dbx.x + t[ 1 ].y * q.dbx.y + t[ 2 ].y * q.dbx.z;
+        const float duy = t[ 0 ].x * q.dby.x + t[ 1 ].x * q.dby.y + t[ 2 ].x * q.dby.z;
+        const float dvy = t[ 0 ].y * q.dby.x + t[ 1 ].y * q.dby.y + t[ 2 ].y * q.dby.z;
+        c               = SampleGrad( sc.tex, gr.texture, tu, tv, dux, dvx, duy, dvy );
+    }
+    else if( q.mode == TQ_CONE )
+    {
+        // getTriangleUVDerivativesFromRayCone (RayCone.h:86-124)
+        const float quadArea = length( cross( p1 - p0, p2 - p0 ) );
+        const float pcw      = q.a / fabsf( dot( q.dir, normal ) );
+        const float ratio    = pcw * pcw / quadArea;
+        const float uvArea   = fabsf( ( t[ 1 ].x - t[ 0 ].x ) * ( t[ 2 ].y - t[ 0 ].y ) - ( t[ 2 ].x - t[ 0 ].x ) * ( t[ 1 ].y - t[ 0 ].y ) );
+        const float U        = sqrtf( uvArea * ratio );
+        c                    = SampleGrad( sc.tex, gr.texture, tu, tv, U, 0.0f, 0.0f, U );
+    }
+    else if( q.mode == TQ_BOUNCE )
+    {
+        const v3 cam = mk3( p.cameraPosition[ 0 ], p.cameraPosition[ 1 ], p.cameraPosition[ 2 ] );
+        c = SampleLod( sc.tex, gr.texture, tu, tv, BounceLod( q.a, length( position - cam ), length( position - rayOrigin ), float( p.renderWidth ), q.b ) );
+    }
+    else
+    {
+        c = SampleLod( sc.tex, gr.texture, tu, tv, q.a );
+    }
+    const v3 col = UnpackColor( gr.color );
+    return mk3( saturate( col.x * c.x ), saturate( col.y * c.y ), saturate( col.z * c.z ) );
+}
+
+// getHitInfo* (HitInfo.inl:204-466): albedo is the primitive colour (layer
+// 0) times its albedo texture if it has one, normals interpolated or exact.
+__device__ HitInfo GetHit( const SceneView& sc, const RgExtFrameParams& p, uint32_t prim, float u, float v, v3 rayOrigin,
+                           const TexQuery& q = TexQuery{} )
 {
     HitInfo           h;
     const uint32_t    g  = sc.triGeom[ prim ];
@@ -207,8 +288,15 @@ __device__ HitInfo GetHit( const SceneView& sc, const RgExtFrameParams& p, uint3
             h.normal = -h.normal;
         }
     }
-    const v3 c  = UnpackColor( gr.color );
-    h.albedo    = mk3( saturate( c.x ), saturate( c.y ), saturate( c.z ) );
+    if( !TexValid( sc.tex, gr.texture ) )
+    {
+        const v3 c = UnpackColor( gr.color );
+        h.albedo   = mk3( saturate( c.x ), saturate( c.y ), saturate( c.z ) );
+    }
+    else
+    {
+        h.albedo = TexturedAlbedo( sc, p, gr, prim - gr.firstTri, u, v, p0, p1, p2, h.position, h.normal, rayOrigin, q );
+    }
     h.roughness = fmaxf( gr.roughness, fmaxf( p.minRoughness, MIN_GGX_ROUGH ) );
     h.metallic  = gr.metallic;
     h.emission  = h.albedo * gr.emissive;
@@ -541,16 +629,21 @@ __global__ __launch_bounds__( THREADS ) void k_primary_shade( RgExtFrameParams p
         return;
     }
 
-    const HitInfo h = GetHit( sc, p, hit.primitiveId, hit.u, hit.v, org );
-    const GeomRecord& gr = sc.geoms[ h.geom ];
     const float*      t  = sc.tris + size_t( hit.primitiveId ) * 9;
     const v3 p0 = mk3( t[ 0 ], t[ 1 ], t[ 2 ] ), p1 = mk3( t[ 3 ], t[ 4 ], t[ 5 ] ), p2 = mk3( t[ 6 ], t[ 7 ], t[ 8 ] );
 
-    // ray differentials on the same triangle (HitInfo.inl:243-268)
+    // ray differentials on the same triangle (HitInfo.inl:243-268); their
+    // barycentric deltas give the texture gradients
     const v3 dAX = GetRayDir( p, u + 1.0f / float( W ), v );
     const v3 dAY = GetRayDir( p, u, v + 1.0f / float( H ) );
     const v3 bAX = RayTriBary( p0, p1, p2, org, dAX );
     const v3 bAY = RayTriBary( p0, p1, p2, org, dAY );
+    TexQuery tq;
+    tq.mode = TQ_GRAD;
+    tq.dbx  = bAX - mk3( 1.0f - hit.u - hit.v, hit.u, hit.v );
+    tq.dby  = bAY - mk3( 1.0f - hit.u - hit.v, hit.u, hit.v );
+    const HitInfo     h  = GetHit( sc, p, hit.primitiveId, hit.u, hit.v, org, tq );
+    const GeomRecord& gr = sc.geoms[ h.geom ];
     const v3 prevPos = gr.isStatic ? h.position : Xform34( gr.prevFromCur, h.position );
 
     float    wc, wp, wx, wy;
@@ -593,8 +686,9 @@ __global__ __launch_bounds__( THREADS ) void k_primary_shade( RgExtFrameParams p
 //   S[0] = throughput.xyz, bits (1: still tracing, 2: wasSplit, media << 2)
 //   S[1] = ray direction.xyz, full path length
 //   S[2] = virtual position for motion.xyz
+//   S[3] = .x: the ray-cone width the last hit was shaded with
 // Portals, animated water normals, screen emission and acid fog are not
-// part of this path (they need textures or portal instances).
+// part of this path (they need the water texture or portal instances).
 constexpr uint32_t RR_ACTIVE = 1u, RR_SPLIT = 2u;
 constexpr uint32_t MEDIA_VACUUM = 0u, MEDIA_WATER = 1u, MEDIA_GLASS = 2u, MEDIA_ACID = 3u;
 
@@ -740,7 +834,9 @@ __global__ __launch_bounds__( THREADS ) void k_reflrefr_a( RgExtFrameParams p, F
         fpl              = Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD )[ ci ];
         flags            = sc.geoms[ sc.triGeom[ hits[ ri ].primitiveId ] ].flags;
         bits             = RR_ACTIVE | ( ( p.cameraMediaType & 3u ) << 2 );
-        st.s[ 2 * st.n + ri ] = make_float4( pos.x, pos.y, pos.z, 0.0f );
+        // ray cone (RaygenPrimary.inl:396-406): width 0, then propagated
+        // over the primary hit's distance
+        st.s[ 2 * st.n + ri ] = make_float4( pos.x, pos.y, pos.z, p.cameraRayConeSpreadAngle * fpl );
     }
     else
     {
@@ -754,7 +850,11 @@ __global__ __launch_bounds__( THREADS ) void k_reflrefr_a( RgExtFrameParams p, F
         const float4   s1  = st.s[ st.n + ri ];
         const RgExtHit hit = hits[ ri ];
         const RgExtRay r   = rays[ ri ];
-        const HitInfo  h   = GetHit( sc, p, hit.primitiveId, hit.u, hit.v, mk3( r.origin[ 0 ], r.origin[ 1 ], r.origin[ 2 ] ) );
+        TexQuery       cq;
+        cq.mode            = TQ_CONE;
+        cq.a               = st.s[ 3 * st.n + ri ].x; // the cone the last hit was shaded with
+        cq.dir             = mk3( s1.x, s1.y, s1.z );
+        const HitInfo  h   = GetHit( sc, p, hit.primitiveId, hit.u, hit.v, mk3( r.origin[ 0 ], r.origin[ 1 ], r.origin[ 2 ] ), cq );
         pos                = h.position;
         normal             = h.normal;
         albedo             = h.albedo;
@@ -854,7 +954,7 @@ __global__ __launch_bounds__( THREADS ) void k_reflrefr_a( RgExtFrameParams p, F
 // (getHitInfoWithRayCone_ReflectionRefraction, HitInfo.inl:213-330, and the
 // G-buffer stores of :584-597), with its virtual-position motion.
 __global__ __launch_bounds__( THREADS ) void k_reflrefr_b( RgExtFrameParams p, FbPtrs P, SceneView sc, const RgExtRay* rays, const RgExtHit* hits,
-                                                           RrState st )
+                                                           RrState st, uint32_t depth )
 {
     const int x = blockIdx.x * BX + threadIdx.x % BX, y = blockIdx.y * BY + threadIdx.x / BX;
     const int W = p.renderWidth, H = p.renderHeight;
@@ -888,14 +988,21 @@ __global__ __launch_bounds__( THREADS ) void k_reflrefr_b( RgExtFrameParams p, F
     }
     const RgExtRay r      = rays[ ri ];
     const v3       origin = mk3( r.origin[ 0 ], r.origin[ 1 ], r.origin[ 2 ] );
-    const HitInfo  h      = GetHit( sc, p, hit.primitiveId, hit.u, hit.v, origin );
+    const float4   s2     = st.s[ 2 * st.n + ri ];
+    TexQuery       cq;
+    cq.mode               = TQ_CONE;
+    cq.a                  = s2.w;
+    cq.dir                = dir;
+    const HitInfo  h      = GetHit( sc, p, hit.primitiveId, hit.u, hit.v, origin, cq );
     const float    rayLen = length( h.position - origin );
+    // propagateRayCone (RayCone.h:30-36): the spread angle doubles at every
+    // propagation, so at this depth it is the camera's times 2^(depth + 1)
+    const float spread    = p.cameraRayConeSpreadAngle * exp2f( float( depth + 1 ) );
 
     // virtualPosForMotion += viewDir * rayLen, seen by both cameras
     const float u   = ( float( x ) + 0.5f + p.jitterX ) / float( W );
     const float v   = ( float( y ) + 0.5f + p.jitterY ) / float( H );
     const v3    cam = GetRayDir( p, u, v );
-    const float4 s2 = st.s[ 2 * st.n + ri ];
     const v3    vp  = mk3( s2.x, s2.y, s2.z ) + cam * rayLen;
     float       wc, wp, cwc, cwp;
     const v3    vCur  = MulM4( p.view, vp, 1.0f, &wc );
@@ -921,7 +1028,8 @@ __global__ __launch_bounds__( THREADS ) void k_reflrefr_b( RgExtFrameParams p, F
     Plane< float4 >( P, RG_EXT_PLANE_THROUGHPUT )[ ci ]         = make_float4( thr.x, thr.y, thr.z, thrW );
     st.s[ ri ]                 = make_float4( thr.x, thr.y, thr.z, __uint_as_float( bits ) );
     st.s[ st.n + ri ]          = make_float4( dir.x, dir.y, dir.z, fpl );
-    st.s[ 2 * st.n + ri ]      = make_float4( vp.x, vp.y, vp.z, 0.0f );
+    st.s[ 2 * st.n + ri ]      = make_float4( vp.x, vp.y, vp.z, s2.w + spread * rayLen );
+    st.s[ 3 * st.n + ri ]      = make_float4( s2.w, 0.0f, 0.0f, 0.0f );
 }
 
 // RtInitialReservoirs.rgen, first half: candidates + directional visibility ray.
@@ -1238,7 +1346,10 @@ __global__ __launch_bounds__( THREADS ) void k_indirect_b( RgExtFrameParams p, F
         pend.at( 1, i ) = make_float4( 0.f, 0.f, 0.f, 1.0f ); // sky
         return;
     }
-    const HitInfo  h    = GetHit( sc, p, hit.primitiveId, hit.u, hit.v, org );
+    TexQuery bq;
+    bq.mode             = TQ_BOUNCE; // FIRST_BOUNCE_MIP_BIAS 0 (RtRaygenIndirect.inl:63)
+    bq.a                = Plane< float2 >( P, RG_EXT_PLANE_METALLIC_ROUGHNESS )[ i ].y;
+    const HitInfo  h    = GetHit( sc, p, hit.primitiveId, hit.u, hit.v, org, bq );
     const Surface  hs   = HitToSurface( h, dir );
     const uint32_t seed = randomSeed( x, y, p.frameId );
     v3             diff, lightPos = mk3( 0, 0, 0 );
@@ -1311,7 +1422,10 @@ __global__ __launch_bounds__( THREADS ) void k_indirect_c( RgExtFrameParams p, F
         }
         // emission of the bounce-1 hit
         const RgExtHit h1   = hits1[ i ];
-        const HitInfo  hi1  = GetHit( sc, p, h1.primitiveId, h1.u, h1.v, s.position );
+        TexQuery       q1;
+        q1.mode             = TQ_BOUNCE;
+        q1.a                = s.roughness;
+        const HitInfo  hi1  = GetHit( sc, p, h1.primitiveId, h1.u, h1.v, s.position + s.normal * 0.01f, q1 );
         const v3       emis = hi1.emission * p.emissionMapBoost;
         // processSecondDiffuseBounce (RtRaygenIndirect.inl:82-101)
         const v3       d2 = mk3( b2.x, b2.y, b2.z );
@@ -1324,7 +1438,11 @@ __global__ __launch_bounds__( THREADS ) void k_indirect_c( RgExtFrameParams p, F
         else
         {
             const RgExtRay br   = bounceRays[ i ];
-            const HitInfo  hi2  = GetHit( sc, p, h2.primitiveId, h2.u, h2.v, mk3( br.origin[ 0 ], br.origin[ 1 ], br.origin[ 2 ] ) );
+            TexQuery       q2;
+            q2.mode             = TQ_BOUNCE;
+            q2.a                = hi1.roughness;
+            q2.b                = 32.0f; // SECOND_BOUNCE_MIP_BIAS (RtRaygenIndirect.inl:64)
+            const HitInfo  hi2  = GetHit( sc, p, h2.primitiveId, h2.u, h2.v, mk3( br.origin[ 0 ], br.origin[ 1 ], br.origin[ 2 ] ), q2 );
             const Surface  hs2  = HitToSurface( hi2, d2 );
             const uint32_t seed = randomSeed( x, y, p.frameId );
             v3             diff2, lp;
@@ -1969,9 +2087,12 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
 {
     Resize( p.renderWidth, p.renderHeight );
     const uint32_t n = W * H;
+    const TexView  tex   = scene.tex;
+    const AlphaRec* alpha = scene.alphaTested ? scene.alphaRecs.as< AlphaRec >() : nullptr;
     const dim3     grid2( DivUp( W, BX ), DivUp( H, BY ) );
     const FbPtrs   P = fb.Ptrs();
-    SceneView      sv{ scene.tris.as< float >(), scene.triNormals.as< float >(), scene.triGeom.as< uint32_t >(), scene.geoms.as< GeomRecord >() };
+    SceneView      sv{ scene.tris.as< float >(), scene.triNormals.as< float >(), scene.triGeom.as< uint32_t >(), scene.geoms.as< GeomRecord >(),
+                       scene.vertices.as< RgPrimitiveVertex >(), scene.indices.as< uint32_t >(), tex };
     LightView      lv{ lights.as< LightEncoded >(), lightArraySize, prevToCur.as< uint32_t >(), prevToCurCount,
                   grid[ gridParity ].as< uint4 >(), grid[ gridParity ^ 1 ].as< uint4 >(), blueNoise };
     RgExtRay* R  = wf->rays.as< RgExtRay >();
@@ -1997,7 +2118,7 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
         {
             const uint32_t k = traceLaunches < TRACE_EVENT_PAIRS ? traceLaunches : TRACE_EVENT_PAIRS - 1;
             TraceRaysFrame( bvh, tc, rr, count, hh, flags | traceFlags | ( collectStats ? RG_EXT_TRACE_COLLECT_STATS : 0u ), !allGeometry, s,
-                            traceEv[ 2 * k ], traceEv[ 2 * k + 1 ], &accum );
+                            traceEv[ 2 * k ], traceEv[ 2 * k + 1 ], &accum, alpha, &tex );
             traceLaunches = k + 1;
         }
         else
@@ -2025,7 +2146,7 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
             k_reflrefr_a<<< grid2, THREADS, 0, s >>>( p, P, sv, R, Hh, st, d );
             CheckLaunch( "k_reflrefr_a" );
             trace( R, Hh, RG_EXT_TRACE_CLOSEST, n, true );
-            k_reflrefr_b<<< grid2, THREADS, 0, s >>>( p, P, sv, R, Hh, st );
+            k_reflrefr_b<<< grid2, THREADS, 0, s >>>( p, P, sv, R, Hh, st, d );
             CheckLaunch( "k_reflrefr_b" );
         }
     }

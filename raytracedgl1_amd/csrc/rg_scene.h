@@ -3,6 +3,7 @@
 #pragma once
 
 #include "rg_internal.h"
+#include "rg_textures.h"
 
 #include <array>
 #include <unordered_map>
@@ -27,7 +28,8 @@ struct GeomRecord
     float    metallic;          // default 0 (VertexCollector.cpp:326)
     uint32_t flags;             // RgMeshPrimitiveFlags
     uint32_t isStatic;          // prevFromCur is the identity
-    uint32_t pad[ 2 ];
+    uint32_t texture;           // albedo texture id (pTextureName), TEX_NONE if none
+    uint32_t pad;
 };
 static_assert( sizeof( GeomRecord ) == 144, "GeomRecord" );
 
@@ -123,6 +125,7 @@ struct SceneCollector
     std::vector< uint64_t >          refrUids;
     uint32_t                         refrTriCount = 0;
     bool                             sealed       = false;
+    const struct TextureManager*     textures     = nullptr; // resolves pTextureName at upload
 
     // uniqueObjectID << 32 | primitiveIndexInMesh -> model of the previous frame
     // (UniqueID::MakeForPrimitive, Source/UniqueID.h:29-33)
@@ -175,6 +178,9 @@ struct DeviceScene
     uint32_t     opaqueTriCount = 0; // triangles [opaqueTriCount, triCount) refract
     uint32_t     builtOpaqueTriCount = 0;
     bool         reflRefrGeometry = false; // some geometry reflects or refracts
+    bool         alphaTested      = false; // some geometry is alpha tested with a texture
+    DeviceBuffer alphaRecs;                // n AlphaRec when alphaTested
+    TexView      tex;                      // the instance's textures for this frame
     bool         bvhValid = false;
 };
 

@@ -8,6 +8,7 @@
 // Shaders/VertexData.inl:317-493).
 #include "rg_device_math.h"
 #include "rg_scene.h"
+#include "rg_textures.h"
 
 #include <algorithm>
 #include <thread>
@@ -180,6 +181,7 @@ void SceneCollector::AddPrimitive( const RgMeshInfo& mesh, const RgMeshPrimitive
     g.roughness    = pbr ? Saturate( prim.pEditorInfo->pbrInfo.roughnessDefault ) : 1.0f;
     g.metallic     = pbr ? Saturate( prim.pEditorInfo->pbrInfo.metallicDefault ) : 0.0f;
     g.flags        = prim.flags;
+    g.texture      = textures ? textures->Find( prim.pTextureName ) : TEX_NONE;
 
     // Previous transform of the same (object, primitive): UniqueID::MakeForPrimitive.
     const uint64_t uid = ( uint64_t( mesh.uniqueObjectID ) << 32 ) | prim.primitiveIndexInMesh;
@@ -445,7 +447,8 @@ __global__ __launch_bounds__( THREADS ) void k_assemble( const RgPrimitiveVertex
                                                          const uint32_t*          triGeom,
                                                          uint32_t                 triCount,
                                                          float*                   tris,
-                                                         float*                   normals )
+                                                         float*                   normals,
+                                                         AlphaRec*                alpha )
 {
     const uint32_t t = blockIdx.x * THREADS + threadIdx.x;
     if( t >= triCount )
@@ -467,6 +470,17 @@ __global__ __launch_bounds__( THREADS ) void k_assemble( const RgPrimitiveVertex
         normals[ size_t( t ) * 9 + 3 * k + 0 ] = n.x;
         normals[ size_t( t ) * 9 + 3 * k + 1 ] = n.y;
         normals[ size_t( t ) * 9 + 3 * k + 2 ] = n.z;
+        if( alpha )
+        {
+            alpha[ t ].uv[ 2 * k + 0 ] = v.texCoord[ 0 ];
+            alpha[ t ].uv[ 2 * k + 1 ] = v.texCoord[ 1 ];
+        }
+    }
+    if( alpha )
+    {
+        // only alpha-tested, textured primitives are tested (RtAlphaTest.rahit)
+        alpha[ t ].tex   = ( g.flags & RG_MESH_PRIMITIVE_ALPHA_TESTED ) ? g.texture : TEX_NONE;
+        alpha[ t ].color = g.color;
     }
 }
 
@@ -491,8 +505,10 @@ void UploadScene( SceneCollector& sc, DeviceScene& ds, hipStream_t s )
     ds.triCount       = n;
     ds.opaqueTriCount = sc.OpaqueTriCount();
     ds.reflRefrGeometry = false;
+    ds.alphaTested      = false;
     for( size_t i = 0; i < sc.geoms.size; i++ )
     {
+        ds.alphaTested = ds.alphaTested || ( ( sc.geoms.data[ i ].flags & RG_MESH_PRIMITIVE_ALPHA_TESTED ) && sc.geoms.data[ i ].texture != TEX_NONE );
         constexpr uint32_t RR = RG_MESH_PRIMITIVE_MIRROR | RG_MESH_PRIMITIVE_WATER | RG_MESH_PRIMITIVE_GLASS | RG_MESH_PRIMITIVE_ACID |
                                 RG_MESH_PRIMITIVE_GLASS_IF_SMOOTH | RG_MESH_PRIMITIVE_MIRROR_IF_SMOOTH;
         ds.reflRefrGeometry = ds.reflRefrGeometry || ( sc.geoms.data[ i ].flags & RR ) != 0u;
@@ -535,8 +551,13 @@ bool AssembleScene( DeviceScene& ds, hipStream_t s )
 
     k_tri_geom<<< DivUp( n, THREADS ), THREADS, 0, s >>>( ds.geoms.as< GeomRecord >(), ds.geomCount, n, ds.triGeom.as< uint32_t >() );
     CheckLaunch( "k_tri_geom" );
+    if( ds.alphaTested )
+    {
+        ds.alphaRecs.Reserve( size_t( n ) * sizeof( AlphaRec ) );
+    }
     k_assemble<<< DivUp( n, THREADS ), THREADS, 0, s >>>( ds.vertices.as< RgPrimitiveVertex >(), ds.indices.as< uint32_t >(), ds.geoms.as< GeomRecord >(),
-                                                          ds.triGeom.as< uint32_t >(), n, ds.tris.as< float >(), ds.triNormals.as< float >() );
+                                                          ds.triGeom.as< uint32_t >(), n, ds.tris.as< float >(), ds.triNormals.as< float >(),
+                                                          ds.alphaTested ? ds.alphaRecs.as< AlphaRec >() : nullptr );
     CheckLaunch( "k_assemble" );
 
     if( !canCompare )

@@ -18,6 +18,7 @@
 // brute-force loop over all triangles with the same intersection routine.
 #include "rg_bvh.h"
 #include "rg_trace_common.h"
+#include "rg_textures.h"
 
 namespace rg
 {
@@ -53,7 +54,27 @@ struct TraceArgs
     const uint32_t* segTris;
     uint32_t        segCount;
     uint32_t        chunk; // rays per counter atomic (tuned)
+    // alpha test (the ALPHA kernels): per-triangle records by global id and
+    // the texture table
+    const AlphaRec* alpha;
+    TexView         tex;
 };
+
+// RtAlphaTest.rahit: the layer-0 colour times the texture at the base level
+// (an any-hit stage has no derivatives); the hit is ignored when
+// (r+g+b)/3 * a + a < 0.5.
+__device__ __noinline__ bool AlphaPasses( const TexView& tv, const AlphaRec& ar, float u, float v )
+{
+    const float  b0 = 1.0f - u - v;
+    const float  tu = ar.uv[ 0 ] * b0 + ar.uv[ 2 ] * u + ar.uv[ 4 ] * v;
+    const float  tv_ = ar.uv[ 1 ] * b0 + ar.uv[ 3 ] * u + ar.uv[ 5 ] * v;
+    const float4 t  = SampleLod( tv, ar.tex, tu, tv_, 0.0f );
+    const float  r  = t.x * float( ar.color & 0xFFu ) * ( 1.0f / 255.0f );
+    const float  g  = t.y * float( ( ar.color >> 8 ) & 0xFFu ) * ( 1.0f / 255.0f );
+    const float  b  = t.z * float( ( ar.color >> 16 ) & 0xFFu ) * ( 1.0f / 255.0f );
+    const float  al = t.w * float( ( ar.color >> 24 ) & 0xFFu ) * ( 1.0f / 255.0f );
+    return ( r + g + b ) / 3.0f * al + al >= 0.5f;
+}
 
 __device__ __forceinline__ uint32_t SegmentOf( const uint32_t* offsets, uint32_t S, uint32_t i )
 {
@@ -139,7 +160,7 @@ __device__ __forceinline__ void LoadHalfNode( const void* nodes, uint32_t node, 
     }
 }
 
-template< bool H, int STACK_LDS >
+template< bool H, int STACK_LDS, bool ALPHA >
 __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
 {
     __shared__ uint32_t s_stack[ STACK_LDS * TR_THREADS ];
@@ -397,6 +418,14 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
                 if( IntersectTri( ray, mk3( q0.x, q0.y, q0.z ), mk3( q1.x, q1.y, q1.z ), mk3( q2.x, q2.y, q2.z ), tmin, bestT, cullFront, &t, &u, &v ) )
                 {
                     const uint32_t prim = __float_as_uint( q0.w );
+                    if constexpr( ALPHA )
+                    {
+                        const AlphaRec& ar = a.alpha[ prim ];
+                        if( TexValid( a.tex, ar.tex ) && !AlphaPasses( a.tex, ar, u, v ) )
+                        {
+                            continue; // ignoreIntersectionEXT
+                        }
+                    }
                     if( t < bestT || prim < bestPrim )
                     {
                         bestT    = t;
@@ -480,11 +509,19 @@ constexpr int TRACE_STACK_DEPTH[ TRACE_STACK_VARIANTS ] = { 16, 24, 32 };
 
 using TraceKernelFn = void ( * )( TraceArgs );
 
-TraceKernelFn TraceKernel( int half, int v )
+TraceKernelFn TraceKernel( int half, int v, bool alpha = false )
 {
+    if( alpha )
+    {
+        static const TraceKernelFn ka[ 2 ][ TRACE_STACK_VARIANTS ] = {
+            { k_trace< false, 16, true >, k_trace< false, 24, true >, k_trace< false, 32, true > },
+            { k_trace< true, 16, true >, k_trace< true, 24, true >, k_trace< true, 32, true > },
+        };
+        return ka[ half ][ v ];
+    }
     static const TraceKernelFn k[ 2 ][ TRACE_STACK_VARIANTS ] = {
-        { k_trace< false, 16 >, k_trace< false, 24 >, k_trace< false, 32 > },
-        { k_trace< true, 16 >, k_trace< true, 24 >, k_trace< true, 32 > },
+        { k_trace< false, 16, false >, k_trace< false, 24, false >, k_trace< false, 32, false > },
+        { k_trace< true, 16, false >, k_trace< true, 24, false >, k_trace< true, 32, false > },
     };
     return k[ half ][ v ];
 }
@@ -517,7 +554,7 @@ void InitGrids( TraceContext& s )
 
 void TraceRaysImpl( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags,
                     hipStream_t stream, hipEvent_t evBefore, hipEvent_t evAfter, const TraceAccum* accum, const uint32_t* d_segRays,
-                    const uint32_t* d_frameRoot = nullptr )
+                    const uint32_t* d_frameRoot = nullptr, const AlphaRec* alpha = nullptr, const TexView* tex = nullptr )
 {
     RG_REQUIRE( bvh.nodeCount > 0, RG_RESULT_WRONG_FUNCTION_CALL, "BVH is not built" );
     RG_REQUIRE( bvh.nodeFormat == RG_EXT_BVH_NODE_FP32 || bvh.nodeFormat == RG_EXT_BVH_NODE_FP16, RG_RESULT_WRONG_FUNCTION_ARGUMENT,
@@ -564,6 +601,8 @@ void TraceRaysImpl( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_ray
         a.segCount = 1;
     }
     a.chunk         = s.rayChunk >= 16 && s.rayChunk <= 1024 ? s.rayChunk : RAY_CHUNK_DEFAULT;
+    a.alpha         = alpha && tex ? alpha : nullptr;
+    a.tex           = tex ? *tex : TexView{};
     if( accum )
     {
         RG_REQUIRE( accum->errors, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "accumulator without an error word" );
@@ -579,7 +618,7 @@ void TraceRaysImpl( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_ray
     {
         RG_HIP_CHECK( hipEventRecord( evBefore, stream ) );
     }
-    hipLaunchKernelGGL( TraceKernel( h, v ), dim3( blocks ), dim3( TR_THREADS ), 0, stream, a );
+    hipLaunchKernelGGL( TraceKernel( h, v, a.alpha != nullptr ), dim3( blocks ), dim3( TR_THREADS ), 0, stream, a );
     CheckLaunch( "k_trace" );
     if( evAfter )
     {
@@ -596,12 +635,13 @@ void TraceRays( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, u
 }
 
 void TraceRaysFrame( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags, bool opaqueOnly,
-                     hipStream_t stream, hipEvent_t evBefore, hipEvent_t evAfter, const TraceAccum* accum )
+                     hipStream_t stream, hipEvent_t evBefore, hipEvent_t evAfter, const TraceAccum* accum, const AlphaRec* alpha,
+                     const TexView* tex )
 {
     if( bvh.frameMode == BvhObject::FRAME_SPLIT )
     {
         TraceRaysImpl( bvh, s, d_rays, count, d_hits, flags, stream, evBefore, evAfter, accum, nullptr,
-                       bvh.frameRoots.as< uint32_t >() + ( opaqueOnly ? 1 : 0 ) );
+                       bvh.frameRoots.as< uint32_t >() + ( opaqueOnly ? 1 : 0 ), alpha, tex );
         return;
     }
     RG_REQUIRE( bvh.sceneCount == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "a batched BVH is traced with TraceRaysBatch" );
@@ -619,7 +659,7 @@ void TraceRaysFrame( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_ra
         }
         return;
     }
-    TraceRaysImpl( bvh, s, d_rays, count, d_hits, flags, stream, evBefore, evAfter, accum, nullptr );
+    TraceRaysImpl( bvh, s, d_rays, count, d_hits, flags, stream, evBefore, evAfter, accum, nullptr, nullptr, alpha, tex );
 }
 
 void TraceRaysBatch( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, const uint32_t* rayOffsets, RgExtHit* d_hits, uint32_t flags,

@@ -93,6 +93,21 @@ class RgDrawFrameIlluminationParams(C.Structure):
                 ("pLightstyleValues", _P)]
 
 
+class RgExtent2D(C.Structure):
+    _fields_ = [("width", _U32), ("height", _U32)]
+
+
+class RgOriginalTextureInfo(C.Structure):
+    _fields_ = [("pTextureName", C.c_char_p), ("pPixels", _P), ("size", RgExtent2D), ("filter", C.c_int),
+                ("addressModeU", C.c_int), ("addressModeV", C.c_int)]
+
+
+# RgSamplerFilter / RgSamplerAddressMode
+RG_SAMPLER_FILTER_AUTO, RG_SAMPLER_FILTER_LINEAR, RG_SAMPLER_FILTER_NEAREST = 0, 1, 2
+RG_SAMPLER_ADDRESS_MODE_REPEAT, RG_SAMPLER_ADDRESS_MODE_CLAMP = 0, 1
+RG_MESH_PRIMITIVE_ALPHA_TESTED = 1
+
+
 class RgDrawFrameReflectRefractParams(C.Structure):
     _fields_ = [("sType", C.c_int), ("pNext", _P), ("maxReflectRefractDepth", _U32), ("typeOfMediaAroundCamera", C.c_int),
                 ("indexOfRefractionGlass", _F), ("indexOfRefractionWater", _F), ("thinMediaWidth", _F),
@@ -177,6 +192,8 @@ VERTEX_DTYPE = np.dtype([("position", "<f4", 3), ("_p0", "<u4"), ("normal", "<f4
 assert VERTEX_DTYPE.itemsize == 64
 
 _API = {
+    "rgProvideOriginalTexture": (C.c_int, [_P, C.POINTER(RgOriginalTextureInfo)]),
+    "rgMarkOriginalTextureAsDeleted": (C.c_int, [_P, C.c_char_p]),
     "rgCreateInstance": (C.c_int, [C.POINTER(RgInstanceCreateInfo), C.POINTER(_P)]),
     "rgDestroyInstance": (C.c_int, [_P]),
     "rgUploadMeshPrimitive": (C.c_int, [_P, C.POINTER(RgMeshInfo), C.POINTER(RgMeshPrimitiveInfo)]),
@@ -245,14 +262,20 @@ class SceneUploader:
     """Prebuilt per-object vertex arrays of a scenes.Scene, uploaded each frame
     with rgUploadMeshPrimitive (non-indexed triangle lists, identity transform)."""
 
-    def __init__(self, scene, flags: dict | None = None):
+    def __init__(self, scene, flags: dict | None = None, textures: dict | None = None, texcoords: dict | None = None):
+        """flags: object index -> RgMeshPrimitiveFlags; textures: object
+        index -> pTextureName; texcoords: object index -> (tris * 3, 2)
+        float32 vertex texture coordinates."""
         self.scene = scene
         self.flags = dict(flags or {})  # object index -> RgMeshPrimitiveFlags
+        self.textures = {k: v.encode() for k, v in (textures or {}).items()}
         self.objects = []
         for oid, (name, first, count) in enumerate(scene.objects):
             v = np.zeros(count * 3, VERTEX_DTYPE)
             v["position"] = scene.tris[first:first + count].reshape(-1, 3)
             v["normal"] = scene.normals[first:first + count].reshape(-1, 3)
+            if texcoords and oid in texcoords:
+                v["texCoord"] = np.asarray(texcoords[oid], np.float32).reshape(-1, 2)
             color = int(scene.colors[first])
             self.objects.append((oid, v, color))
         self.mesh = RgMeshInfo()
@@ -288,6 +311,7 @@ class SceneUploader:
             self.prim.color = color | 0xFF000000
             self.prim.emissive = 0.0
             self.prim.flags = self.flags.get(oid, 0)
+            self.prim.pTextureName = self.textures.get(oid)
             ffi.check(prefix + "UploadMeshPrimitive", mesh_fn(inst, C.byref(self.mesh), C.byref(self.prim)))
         lights = self.scene.lights or {}
         sun = lights.get("sun")
@@ -369,6 +393,20 @@ class Renderer:
         il.pLightstyleValues = C.cast(self.styles, _P)
         self.illum = il
         self._link()
+
+    def provide_texture(self, name: str, rgba8: np.ndarray, filter: int = RG_SAMPLER_FILTER_AUTO,
+                        address_u: int = RG_SAMPLER_ADDRESS_MODE_REPEAT, address_v: int = RG_SAMPLER_ADDRESS_MODE_REPEAT) -> None:
+        """rgProvideOriginalTexture with (height, width, 4) uint8 sRGB pixels."""
+        px = np.ascontiguousarray(rgba8, np.uint8)
+        info = RgOriginalTextureInfo()
+        info.pTextureName = name.encode()
+        info.pPixels = px.ctypes.data
+        info.size.width, info.size.height = px.shape[1], px.shape[0]
+        info.filter, info.addressModeU, info.addressModeV = filter, address_u, address_v
+        ffi.check("rgProvideOriginalTexture", api().rgProvideOriginalTexture(self.h, C.byref(info)))
+
+    def delete_texture(self, name: str) -> None:
+        ffi.check("rgMarkOriginalTextureAsDeleted", api().rgMarkOriginalTextureAsDeleted(self.h, name.encode()))
 
     def set_static_geometry(self, enable: bool) -> None:
         ffi.check("rgExtSetStaticGeometry", api().rgExtSetStaticGeometry(self.h, 1 if enable else 0))
