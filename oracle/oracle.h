@@ -55,6 +55,11 @@ void     or_bvh_copy_nodes( const OrBvh* b, void* out );               /* nodeCo
  * over rays.  Optional statistics: total node visits and triangle tests. */
 void or_trace_rays( const OrBvh* b, const RgExtRay* rays, uint32_t count, RgExtHit* hits, uint32_t flags,
                     int threads, uint64_t* nodeVisits, uint64_t* triTests );
+/* With an any-hit filter: candidate hits for which accept returns 0 are
+ * ignored (the alpha test). */
+typedef int ( *OrHitFilter )( const void* ctx, uint32_t prim, float u, float v );
+void or_trace_rays_filtered( const OrBvh* b, const RgExtRay* rays, uint32_t count, RgExtHit* hits, uint32_t flags,
+                             int threads, uint64_t* nodeVisits, uint64_t* triTests, OrHitFilter accept, const void* ctx );
 /* Independent FP64 Moller-Trumbore over every triangle (closest hit, ties to
  * the smaller id).  t, u, v in double. */
 void or_trace_bruteforce_f64( const float* tris, uint32_t ntri, const RgExtRay* rays, uint32_t count,

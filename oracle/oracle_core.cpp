@@ -729,6 +729,14 @@ bool Tri( const Pre& r, F3 p0, F3 p1, F3 p2, float tmin, float tmax, bool cullFr
 extern "C" void or_trace_rays( const OrBvh* b, const RgExtRay* rays, uint32_t count, RgExtHit* hits, uint32_t flags,
                                int threads, uint64_t* nodeVisits, uint64_t* triTests )
 {
+    or_trace_rays_filtered( b, rays, count, hits, flags, threads, nodeVisits, triTests, nullptr, nullptr );
+}
+
+// The same traversal with an any-hit filter: a candidate hit the filter
+// rejects is ignored (ignoreIntersectionEXT of RtAlphaTest.rahit).
+extern "C" void or_trace_rays_filtered( const OrBvh* b, const RgExtRay* rays, uint32_t count, RgExtHit* hits, uint32_t flags,
+                                        int threads, uint64_t* nodeVisits, uint64_t* triTests, OrHitFilter accept, const void* ctx )
+{
     const Node8* nodes     = reinterpret_cast< const Node8* >( b->nodes.data() );
     const float* tris      = b->sortedTris.data();
     const bool   anyHit    = ( flags & RG_EXT_TRACE_ANY_HIT ) != 0;
@@ -799,6 +807,10 @@ extern "C" void or_trace_rays( const OrBvh* b, const RgExtRay* rays, uint32_t co
                     if( Tri( r, { q[ 0 ], q[ 1 ], q[ 2 ] }, { q[ 4 ], q[ 5 ], q[ 6 ] }, { q[ 8 ], q[ 9 ], q[ 10 ] }, ry.tmin, bestT, cullFront, &t, &u, &v ) )
                     {
                         const uint32_t prim = bitcast< uint32_t >( q[ 3 ] );
+                        if( accept && !accept( ctx, prim, u, v ) )
+                        {
+                            continue;
+                        }
                         if( t < bestT || prim < bestP )
                         {
                             bestT = t; bestU = u; bestV = v; bestP = prim;

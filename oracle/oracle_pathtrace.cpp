@@ -498,12 +498,31 @@ const Light& LightAt( const Ctx& c, uint32_t i )
 
 // allGeometry: primary and reflection / refraction rays; the others skip the
 // refracting triangles (INSTANCE_MASK_REFRACT, RaygenCommon.h:103-170)
+// RtAlphaTest.rahit: alpha-tested primitives with a texture ignore hits
+// where the layer-0 colour times the base-level texel has
+// (r+g+b)/3 * a + a < 0.5.
+int AlphaAccept( const void* ctx, uint32_t prim, float u, float v )
+{
+    const OrPtScene* sc = static_cast< const OrPtScene* >( ctx );
+    const OrGeom&    g  = sc->geoms[ sc->triGeom[ prim ] ];
+    if( !( g.flags & RG_MESH_PRIMITIVE_ALPHA_TESTED ) || g.texture == 0 || g.texture > sc->textureCount ) return 1;
+    const float* t  = sc->texcoords + size_t( prim ) * 6;
+    const double b0 = 1.0 - double( u ) - double( v );
+    double       c[ 4 ];
+    or_texture_sample_lod( sc->textures[ g.texture - 1 ], t[ 0 ] * b0 + t[ 2 ] * u + t[ 4 ] * v, t[ 1 ] * b0 + t[ 3 ] * u + t[ 5 ] * v, 0.0, c );
+    const double r = c[ 0 ] * ( g.color & 0xFFu ) / 255.0, gg = c[ 1 ] * ( ( g.color >> 8 ) & 0xFFu ) / 255.0;
+    const double b = c[ 2 ] * ( ( g.color >> 16 ) & 0xFFu ) / 255.0, a = c[ 3 ] * ( ( g.color >> 24 ) & 0xFFu ) / 255.0;
+    return ( r + gg + b ) / 3.0 * a + a >= 0.5 ? 1 : 0;
+}
+
 void Trace( const Ctx& c, std::vector< RgExtRay >& rays, std::vector< RgExtHit >& hits, uint32_t flags, bool allGeometry = false )
 {
     hits.resize( rays.size() );
+    const OrHitFilter filter = c.sc->textures ? AlphaAccept : nullptr;
     if( !allGeometry && c.sc->bvhOpaque )
     {
-        or_trace_rays( c.sc->bvhOpaque, rays.data(), uint32_t( rays.size() ), hits.data(), flags | c.traceFlags, c.threads, nullptr, nullptr );
+        or_trace_rays_filtered( c.sc->bvhOpaque, rays.data(), uint32_t( rays.size() ), hits.data(), flags | c.traceFlags, c.threads, nullptr,
+                                nullptr, filter, c.sc );
         return;
     }
     if( c.sc->triCount == 0 || ( !allGeometry && c.sc->opaqueNone ) )
@@ -514,7 +533,8 @@ void Trace( const Ctx& c, std::vector< RgExtRay >& rays, std::vector< RgExtHit >
         }
         return;
     }
-    or_trace_rays( c.bvh, rays.data(), uint32_t( rays.size() ), hits.data(), flags | c.traceFlags, c.threads, nullptr, nullptr );
+    or_trace_rays_filtered( c.bvh, rays.data(), uint32_t( rays.size() ), hits.data(), flags | c.traceFlags, c.threads, nullptr, nullptr, filter,
+                            c.sc );
 }
 
 RgExtRay MakeRay( D3 o, D3 d, double tmin, double tmax )
@@ -611,8 +631,27 @@ struct Hit
     uint32_t geom, localPrim;
 };
 
-// getHitInfo (HitInfo.inl:204-466), untextured
-Hit GetHit( const Ctx& c, uint32_t prim, double u, double v, D3 rayOrigin )
+// The texture LOD source of a hit (HitInfo.inl): primary rays sample with
+// gradients from the neighbouring pixels' rays (barycentric deltas dbx,
+// dby), reflection / refraction rays with a ray cone (RayCone.h:86-130),
+// bounces at getBounceLOD (HitInfo.inl:160-175).
+enum TexMode { TQ_LOD = 0, TQ_GRAD = 1, TQ_CONE = 2, TQ_BOUNCE = 3 };
+struct TexQuery
+{
+    int    mode = TQ_LOD;
+    double a = 0, b = 0;
+    D3     dbx{}, dby{}, dir{};
+};
+
+double BounceLod( double roughness, double viewDist, double hitDist, double screenWidth, double bias )
+{
+    const double range = 300.0 * std::pow( ( 1.0 - roughness ) * 0.9 + 0.1, 4.0 );
+    const double fx = std::sqrt( clampd( viewDist / range, 0, 1 ) ), fy = std::sqrt( clampd( hitDist / range, 0, 1 ) );
+    return std::max( std::log2( 3840.0 / screenWidth ), 0.0 ) + fx * 10.0 + fy * 10.0 + bias;
+}
+
+// getHitInfo (HitInfo.inl:204-466): layer-0 colour times its albedo texture
+Hit GetHit( const Ctx& c, uint32_t prim, double u, double v, D3 rayOrigin, const TexQuery& q = TexQuery{} )
 {
     Hit           h;
     const uint32_t g = c.sc->triGeom[ prim ];
@@ -633,6 +672,37 @@ Hit GetHit( const Ctx& c, uint32_t prim, double u, double v, D3 rayOrigin )
     }
     const D3 col = UnpackColor( gr.color );
     h.albedo     = mk( saturate( col.x ), saturate( col.y ), saturate( col.z ) );
+    if( gr.texture != 0 && gr.texture <= c.sc->textureCount )
+    {
+        const OrTexture* tex = c.sc->textures[ gr.texture - 1 ];
+        const float*     tc  = c.sc->texcoords + size_t( prim ) * 6;
+        const double     tu  = tc[ 0 ] * b0 + tc[ 2 ] * u + tc[ 4 ] * v, tv = tc[ 1 ] * b0 + tc[ 3 ] * u + tc[ 5 ] * v;
+        double           s[ 4 ];
+        if( q.mode == TQ_GRAD )
+        {
+            or_texture_sample_grad( tex, tu, tv, tc[ 0 ] * q.dbx.x + tc[ 2 ] * q.dbx.y + tc[ 4 ] * q.dbx.z,
+                                    tc[ 1 ] * q.dbx.x + tc[ 3 ] * q.dbx.y + tc[ 5 ] * q.dbx.z, tc[ 0 ] * q.dby.x + tc[ 2 ] * q.dby.y + tc[ 4 ] * q.dby.z,
+                                    tc[ 1 ] * q.dby.x + tc[ 3 ] * q.dby.y + tc[ 5 ] * q.dby.z, s );
+        }
+        else if( q.mode == TQ_CONE )
+        {
+            const double area   = length( cross( p1 - p0, p2 - p0 ) );
+            const double pcw    = q.a / std::fabs( dot( q.dir, h.normal ) );
+            const double uvArea = std::fabs( ( tc[ 2 ] - tc[ 0 ] ) * ( tc[ 5 ] - tc[ 1 ] ) - ( tc[ 4 ] - tc[ 0 ] ) * ( tc[ 3 ] - tc[ 1 ] ) );
+            const double U      = std::sqrt( uvArea * pcw * pcw / area );
+            or_texture_sample_grad( tex, tu, tv, U, 0, 0, U, s );
+        }
+        else if( q.mode == TQ_BOUNCE )
+        {
+            const D3 cam = mk( c.p.cameraPosition[ 0 ], c.p.cameraPosition[ 1 ], c.p.cameraPosition[ 2 ] );
+            or_texture_sample_lod( tex, tu, tv, BounceLod( q.a, length( h.position - cam ), length( h.position - rayOrigin ), c.p.renderWidth, q.b ), s );
+        }
+        else
+        {
+            or_texture_sample_lod( tex, tu, tv, q.a, s );
+        }
+        h.albedo = mk( saturate( col.x * s[ 0 ] ), saturate( col.y * s[ 1 ] ), saturate( col.z * s[ 2 ] ) );
+    }
     h.roughness  = std::max( double( gr.roughness ), std::max( double( c.p.minRoughness ), MIN_ROUGH ) );
     h.metallic   = gr.metallic;
     h.emission   = h.albedo * double( gr.emissive );
@@ -721,6 +791,7 @@ struct RrPix
 {
     D3       thr, dir, vpos;
     double   fpl;
+    double   cone, coneLast; // ray-cone width now and at the last hit (RayCone.h:25-36)
     uint32_t media;
     bool     active, split;
 };
@@ -769,6 +840,7 @@ void ReflectRefract( Ctx& c, std::vector< RgExtRay >& rays, std::vector< RgExtHi
                     s.dir   = RayDir( p, ( double( x ) + 0.5 + p.jitterX ) / W, ( double( y ) + 0.5 + p.jitterY ) / H );
                     s.fpl   = fb.get( RG_EXT_PLANE_DEPTH_WORLD, cx, cy, 0 );
                     s.vpos  = pos;
+                    s.cone  = double( p.cameraRayConeSpreadAngle ) * s.fpl; // width 0 propagated over the primary hit
                     s.media = p.cameraMediaType & 3u;
                     s.split = false;
                 }
@@ -780,7 +852,9 @@ void ReflectRefract( Ctx& c, std::vector< RgExtRay >& rays, std::vector< RgExtHi
                         continue;
                     }
                     const RgExtRay& r = rays[ i ];
-                    const Hit       h = GetHit( c, hits[ i ].primitiveId, hits[ i ].u, hits[ i ].v, mk( r.origin[ 0 ], r.origin[ 1 ], r.origin[ 2 ] ) );
+                    TexQuery        cq;
+                    cq.mode = TQ_CONE, cq.a = s.coneLast, cq.dir = s.dir;
+                    const Hit       h = GetHit( c, hits[ i ].primitiveId, hits[ i ].u, hits[ i ].v, mk( r.origin[ 0 ], r.origin[ 1 ], r.origin[ 2 ] ), cq );
                     pos = h.position, nrm = h.normal, alb = h.albedo, rough = h.roughness, metal = h.metallic;
                     flags = c.sc->geoms[ h.geom ].flags;
                 }
@@ -888,8 +962,13 @@ void ReflectRefract( Ctx& c, std::vector< RgExtRay >& rays, std::vector< RgExtHi
                 }
                 const RgExtRay& r      = rays[ i ];
                 const D3        origin = mk( r.origin[ 0 ], r.origin[ 1 ], r.origin[ 2 ] );
-                const Hit       h      = GetHit( c, hits[ i ].primitiveId, hits[ i ].u, hits[ i ].v, origin );
+                TexQuery        cq;
+                cq.mode = TQ_CONE, cq.a = s.cone, cq.dir = s.dir;
+                const Hit       h      = GetHit( c, hits[ i ].primitiveId, hits[ i ].u, hits[ i ].v, origin, cq );
                 const double    len    = length( h.position - origin );
+                // propagateRayCone: the spread doubles at each propagation
+                s.coneLast = s.cone;
+                s.cone += double( p.cameraRayConeSpreadAngle ) * std::exp2( double( depth + 1 ) ) * len;
                 // virtual position for motion: along the camera ray by the path length
                 const D3 cam = RayDir( p, ( double( x ) + 0.5 + p.jitterX ) / W, ( double( y ) + 0.5 + p.jitterY ) / H );
                 s.vpos       = s.vpos + cam * len;
@@ -1249,19 +1328,25 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
                 fb.set4( RG_EXT_PLANE_THROUGHPUT, cx, cy, 1, 1, 1, 0 );
                 continue;
             }
-            const Hit     h  = GetHit( c, hit.primitiveId, hit.u, hit.v, camPos );
-            const OrGeom& gr = sc->geoms[ h.geom ];
             const float*  t  = sc->tris + size_t( hit.primitiveId ) * 9;
             const D3      p0 = mk( t[ 0 ], t[ 1 ], t[ 2 ] ), p1 = mk( t[ 3 ], t[ 4 ], t[ 5 ] ), p2 = mk( t[ 6 ], t[ 7 ], t[ 8 ] );
             // ray differentials on the same triangle (HitInfo.inl:181-268)
-            auto bary = [ & ]( D3 d ) {
+            auto baryOf = [ & ]( D3 d ) {
                 const D3     e1 = p1 - p0, e2 = p2 - p0, pv = cross( d, e2 );
                 const double inv = 1.0 / dot( e1, pv );
                 const D3     tv  = camPos - p0, qv = cross( tv, e1 );
                 const double bu  = dot( tv, pv ) * inv, bv = dot( d, qv ) * inv;
-                return p0 * ( 1.0 - bu - bv ) + p1 * bu + p2 * bv;
+                return mk( 1.0 - bu - bv, bu, bv );
             };
-            const D3 pAX = bary( RayDir( p, u + 1.0 / W, v ) ), pAY = bary( RayDir( p, u, v + 1.0 / H ) );
+            const D3 bAX = baryOf( RayDir( p, u + 1.0 / W, v ) ), bAY = baryOf( RayDir( p, u, v + 1.0 / H ) );
+            const D3 pAX = p0 * bAX.x + p1 * bAX.y + p2 * bAX.z, pAY = p0 * bAY.x + p1 * bAY.y + p2 * bAY.z;
+            TexQuery tq;
+            tq.mode       = TQ_GRAD;
+            const D3 b    = mk( 1.0 - double( hit.u ) - double( hit.v ), hit.u, hit.v );
+            tq.dbx        = bAX - b;
+            tq.dby        = bAY - b;
+            const Hit     h  = GetHit( c, hit.primitiveId, hit.u, hit.v, camPos, tq );
+            const OrGeom& gr = sc->geoms[ h.geom ];
             D3       prevPos = h.position;
             if( !gr.isStatic )
             {
@@ -1504,7 +1589,10 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
                 b1sky[ i ] = 1;
                 continue;
             }
-            const Hit      h    = GetHit( c, hit.primitiveId, hit.u, hit.v, org );
+            TexQuery       bq;
+            bq.mode             = TQ_BOUNCE; // FIRST_BOUNCE_MIP_BIAS 0 (RtRaygenIndirect.inl:63)
+            bq.a                = fb.get( RG_EXT_PLANE_METALLIC_ROUGHNESS, x, y, 1 );
+            const Hit      h    = GetHit( c, hit.primitiveId, hit.u, hit.v, org, bq );
             const Surface  hs   = HitToSurface( h, dir );
             const uint32_t seed = seedOf( x, y, p.frameId );
             D3             diff, lp = mk( 0, 0, 0 );
@@ -1549,7 +1637,10 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
                 if( hShadow[ i ] && hits2[ i ].primitiveId != RG_EXT_MISS ) diffuse = mk( 0, 0, 0 );
                 if( !hIsDir[ i ] ) diffuse = mk( clampd( diffuse.x, 0, 20 ), clampd( diffuse.y, 0, 20 ), clampd( diffuse.z, 0, 20 ) ); // SHIPPING_HACK
                 const RgExtHit h1   = hits[ i ];
-                const Hit      hi1  = GetHit( c, h1.primitiveId, h1.u, h1.v, s.position );
+                TexQuery       q1;
+                q1.mode             = TQ_BOUNCE;
+                q1.a                = s.roughness;
+                const Hit      hi1  = GetHit( c, h1.primitiveId, h1.u, h1.v, s.position + s.normal * 0.01, q1 );
                 const D3       emis = hi1.emission * double( p.emissionMapBoost );
                 D3             second;
                 const RgExtHit h2 = hits3[ i ];
@@ -1560,7 +1651,11 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
                 else
                 {
                     const RgExtRay br  = rays3[ i ];
-                    const Hit      hi2 = GetHit( c, h2.primitiveId, h2.u, h2.v, mk( br.origin[ 0 ], br.origin[ 1 ], br.origin[ 2 ] ) );
+                    TexQuery       q2;
+                    q2.mode            = TQ_BOUNCE;
+                    q2.a               = hi1.roughness;
+                    q2.b               = 32.0; // SECOND_BOUNCE_MIP_BIAS (RtRaygenIndirect.inl:64)
+                    const Hit      hi2 = GetHit( c, h2.primitiveId, h2.u, h2.v, mk( br.origin[ 0 ], br.origin[ 1 ], br.origin[ 2 ] ), q2 );
                     const Surface  hs2 = HitToSurface( hi2, b2dir[ i ] );
                     D3             diff2, lp;
                     bool           isDir2, ns2;

@@ -5,6 +5,7 @@
 
 #include "oracle.h"
 #include "oracle_denoise.h"
+#include "oracle_texture.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -22,6 +23,7 @@ typedef struct OrGeom
     uint32_t flags;             /* RgMeshPrimitiveFlags                     */
     uint32_t firstTri;
     uint32_t triCount;
+    uint32_t texture;           /* 1 + index into OrPtScene::textures, 0: none */
 } OrGeom;
 
 /* World-space triangles with their geometries and the frame's lights. */
@@ -42,6 +44,10 @@ typedef struct OrPtScene
      * others (null: they trace bvh), or miss everything with opaqueNone. */
     const struct OrBvh*                 bvhOpaque;
     uint32_t                            opaqueNone;
+    /* Albedo textures and per-triangle texture coordinates (6 floats). */
+    const struct OrTexture* const*      textures;
+    uint32_t                            textureCount;
+    const float*                        texcoords;
 } OrPtScene;
 
 typedef struct OrPathTracer OrPathTracer;

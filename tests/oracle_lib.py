@@ -69,6 +69,12 @@ class Oracle:
             "or_denoise_variance": (None, [_P, _P]),
             "or_denoise_atrous": (None, [_P, _P, _U32]),
             "or_denoise": (None, [_P, _P]),
+            "or_texture_create": (_P, [_P, _U32, _U32, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "or_texture_free": (None, [_P]),
+            "or_texture_mip_count": (_U32, [_P]),
+            "or_texture_copy_mip": (None, [_P, _U32, _P]),
+            "or_texture_sample_lod": (None, [_P, C.c_double, C.c_double, C.c_double, _P]),
+            "or_texture_sample_grad": (None, [_P, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, _P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(l, name)
@@ -231,13 +237,50 @@ class OracleFramebuffers:
 
 class OrGeomC(C.Structure):
     _fields_ = [("prevFromCur", C.c_float * 12), ("isStatic", _U32), ("color", _U32), ("emissive", _F),
-                ("roughness", _F), ("metallic", _F), ("flags", _U32), ("firstTri", _U32), ("triCount", _U32)]
+                ("roughness", _F), ("metallic", _F), ("flags", _U32), ("firstTri", _U32), ("triCount", _U32),
+                ("texture", _U32)]
+
+
+class OracleTexture:
+    """The oracle's albedo texture (oracle/oracle_texture.cpp) of
+    (height, width, 4) uint8 sRGB pixels."""
+
+    def __init__(self, o: "Oracle", rgba8, filter: int = 0, address_u: int = 0, address_v: int = 0, force_min_linear: bool = False):
+        self.o = o
+        self.px = np.ascontiguousarray(rgba8, np.uint8)
+        self.h = o.l.or_texture_create(self.px.ctypes.data, self.px.shape[1], self.px.shape[0], filter, address_u, address_v,
+                                       1 if force_min_linear else 0)
+
+    def mips(self) -> list:
+        out, w, h = [], self.px.shape[1], self.px.shape[0]
+        for lvl in range(self.o.l.or_texture_mip_count(self.h)):
+            a = np.zeros((h, w, 4), np.uint8)
+            self.o.l.or_texture_copy_mip(self.h, lvl, a.ctypes.data)
+            out.append(a)
+            w, h = max(1, w // 2), max(1, h // 2)
+        return out
+
+    def sample_lod(self, u, v, lod):
+        r = np.zeros(4)
+        self.o.l.or_texture_sample_lod(self.h, u, v, lod, r.ctypes.data)
+        return r
+
+    def sample_grad(self, u, v, dudx, dvdx, dudy, dvdy):
+        r = np.zeros(4)
+        self.o.l.or_texture_sample_grad(self.h, u, v, dudx, dvdx, dudy, dvdy, r.ctypes.data)
+        return r
+
+    def __del__(self):
+        try:
+            self.o.l.or_texture_free(self.h)
+        except Exception:
+            pass
 
 
 class OrPtSceneC(C.Structure):
     _fields_ = [("tris", _P), ("normals", _P), ("triGeom", _P), ("triCount", _U32), ("geoms", _P), ("geomIds", _P),
                 ("geomCount", _U32), ("dirLight", _P), ("spheres", _P), ("sphereCount", _U32),
-                ("bvhOpaque", _P), ("opaqueNone", _U32)]
+                ("bvhOpaque", _P), ("opaqueNone", _U32), ("textures", _P), ("textureCount", _U32), ("texcoords", _P)]
 
 # RgMeshPrimitiveFlags of the refracting primitives that are not alpha tested
 # (VertexCollectorFilterType.cpp:215-229): water 128, glass 64,
@@ -255,12 +298,20 @@ class OraclePathTracer:
     non-indexed primitive per object, identity transform, opaque colour,
     roughness 1, metallic 0."""
 
-    def __init__(self, o: Oracle, scene, width: int, height: int, leaf: int = 3, flags: dict | None = None):
-        """flags: object index -> RgMeshPrimitiveFlags, as rtgl.SceneUploader
-        takes them.  Refracting objects follow the others, in upload order,
-        as the frame assembles them."""
+    def __init__(self, o: Oracle, scene, width: int, height: int, leaf: int = 3, flags: dict | None = None,
+                 textures: dict | None = None, texcoords: dict | None = None, texture_pixels: dict | None = None):
+        """flags, textures (object index -> texture name) and texcoords
+        (object index -> (tris * 3, 2) floats) as rtgl.SceneUploader takes
+        them; texture_pixels: name -> (rgba8, filter, address u, address v)
+        as rgProvideOriginalTexture got them.  Refracting objects follow the
+        others, in upload order, as the frame assembles them."""
         self.o, self.scene = o, scene
         flags = dict(flags or {})
+        textures, texcoords = dict(textures or {}), dict(texcoords or {})
+        names = sorted(texture_pixels or {})
+        self.tex_objs = [OracleTexture(o, *texture_pixels[nm]) for nm in names]
+        self.tex_ptrs = (_P * max(1, len(names)))(*[t.h for t in self.tex_objs])
+        tex_id = {nm: i + 1 for i, nm in enumerate(names)}
         self.h = o.l.or_pt_create(width, height)
         objs = list(enumerate(scene.objects))
         objs = [ob for ob in objs if not refracts(flags.get(ob[0], 0))] + [ob for ob in objs if refracts(flags.get(ob[0], 0))]
@@ -270,6 +321,7 @@ class OraclePathTracer:
         self.normals = np.ascontiguousarray(scene.normals[order], np.float32)
         self.bvh = OracleBvh(o, self.tris, leaf)
         self.tri_geom = np.zeros(n, np.uint32)
+        self.texcoords = np.zeros((max(1, n), 6), np.float32)
         self.geoms = (OrGeomC * max(1, len(scene.objects)))()
         self.ids = np.zeros(max(1, len(scene.objects)), np.uint64)
         pos, opaque = 0, 0
@@ -281,6 +333,9 @@ class OraclePathTracer:
             gr.isStatic, gr.color = 1, int(scene.colors[first]) | 0xFF000000
             gr.emissive, gr.roughness, gr.metallic = 0.0, 1.0, 0.0
             gr.flags, gr.firstTri, gr.triCount = flags.get(oid, 0), pos, count
+            gr.texture = tex_id.get(textures.get(oid), 0)
+            if oid in texcoords:
+                self.texcoords[pos:pos + count] = np.asarray(texcoords[oid], np.float32).reshape(count, 6)
             self.ids[g] = oid << 32
             opaque += 0 if refracts(gr.flags) else count
             pos += count
@@ -288,7 +343,8 @@ class OraclePathTracer:
         self.bvh_opaque = OracleBvh(o, self.tris[:opaque], leaf) if opaque >= 2 and n - opaque >= 2 else None
         self.c = OrPtSceneC(self.tris.ctypes.data, self.normals.ctypes.data, self.tri_geom.ctypes.data, n,
                             C.cast(self.geoms, _P), self.ids.ctypes.data, len(scene.objects), None, None, 0,
-                            self.bvh_opaque.h if self.bvh_opaque else None, 1 if (opaque == 0 and n > 0) else 0)
+                            self.bvh_opaque.h if self.bvh_opaque else None, 1 if (opaque == 0 and n > 0) else 0,
+                            C.cast(self.tex_ptrs, _P) if names else None, len(names), self.texcoords.ctypes.data)
         self.scene_lights(scene)
 
     def scene_lights(self, scene):

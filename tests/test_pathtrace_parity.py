@@ -48,9 +48,12 @@ def _rel(g, e, scale):
     return np.abs(g - e) / (np.abs(e) + scale)
 
 
-def _parity_frames(oracle, flags=None, rr=None):
+def _parity_frames(oracle, flags=None, rr=None, tex=None):
     """Renders FRAMES frames through the C API and the oracle side by side;
-    yields (frame, gpu planes, oracle planes, metrics) after each."""
+    yields (frame, gpu planes, oracle planes, metrics) after each.  tex:
+    (object -> texture name, object -> texcoords, name -> (pixels, filter,
+    address u, address v))."""
+    textures, texcoords, pixels = tex if tex else ({}, {}, {})
     s = _scene_with_spheres()
     no_sun = scenes.Scene(s.tris, s.normals, s.colors, s.objects, {"spheres": s.lights["spheres"]})
     scenes_per_frame = [s, s, s, no_sun]
@@ -58,19 +61,21 @@ def _parity_frames(oracle, flags=None, rr=None):
     cams = [scenes.Camera(cam0.eye + np.array([0.05 * f, 0, 0.03 * f]), cam0.target + np.array([0.05 * f, 0, 0.03 * f]))
             for f in range(FRAMES)]
     ofb = OracleFramebuffers(oracle, W, H)
-    opt = OraclePathTracer(oracle, s, W, H, flags=flags)
+    opt = OraclePathTracer(oracle, s, W, H, flags=flags, textures=textures, texcoords=texcoords, texture_pixels=pixels)
     r = rtgl.Renderer(W, H)
     try:
         if rr is not None:
             r.set_reflect_refract(rr)
+        for name, (px, filt, au, av) in pixels.items():
+            r.provide_texture(name, px, filt, au, av)
         for f in range(FRAMES):
             sc = scenes_per_frame[f]
-            r.frame(rtgl.SceneUploader(sc, flags), cams[f])
+            r.frame(rtgl.SceneUploader(sc, flags, textures, texcoords), cams[f])
             prm = r.frame_params()
             fbv = r.framebuffers()
             g = {k: fbv.download(P[k]) for k in ("IS_SKY", "VISIBILITY_BUFFER", "DEPTH_WORLD", "MOTION", "RESERVOIRS_INITIAL",
                                                    "RESERVOIRS", "UNFILTERED_DIRECT", "UNFILTERED_SPECULAR", "UNFILTERED_INDIR",
-                                                   "GRADIENT_PREV_PIX", "DIS_GRADIENT_HISTORY", "PRE_FINAL", "THROUGHPUT")}
+                                                   "GRADIENT_PREV_PIX", "DIS_GRADIENT_HISTORY", "PRE_FINAL", "THROUGHPUT", "ALBEDO")}
             if f:
                 ofb.swap()
             opt.scene_lights(sc)
@@ -90,6 +95,7 @@ def _parity_frames(oracle, flags=None, rr=None):
             m["DEPTH_WORLD"] = np.mean(_rel(g["DEPTH_WORLD"][hit, 0], oc[P["DEPTH_WORLD"]][hit, 0], 0) < 1e-4)
             m["MOTION max"] = np.abs(g["MOTION"][hit, :2] - oc[P["MOTION"]][hit, :2]).max()
             m["THROUGHPUT"] = np.mean(np.all(np.abs(g["THROUGHPUT"] - oc[P["THROUGHPUT"]]) < 1e-4, -1))
+            m["ALBEDO"] = np.mean(np.all(np.abs(g["ALBEDO"][..., :3] - oc[P["ALBEDO"]][..., :3]) < 2.0 / 255.0, -1))
             for k in ("RESERVOIRS_INITIAL", "RESERVOIRS"):
                 m[k] = np.mean(g[k][hit, 0] == oc[P[k]][hit, 0].astype(np.uint32))
             for k in ("UNFILTERED_DIRECT", "UNFILTERED_SPECULAR", "UNFILTERED_INDIR"):
@@ -158,3 +164,42 @@ def test_reflection_refraction_frames_match_the_fp64_oracle(oracle):
         assert abs(m["PRE_FINAL mean"] - 1.0) < 0.03, f
     # the chain ran: some pixels carry a reflected or refracted surface
     assert seen > 0.01
+
+
+def test_textured_frames_match_the_fp64_oracle(oracle):
+    """Planar texture coordinates (x/6, z/6) on every object; the terrain and
+    half of the boxes carry a seeded 64x64 texture (bilinear, trilinear between
+    mips), the other boxes an alpha-tested 16x16 texture with transparent 4x4
+    blocks (nearest).  FP32 sampling on the GPU against FP64 in the oracle, and
+    mip levels that may differ by one sRGB code: albedo within 2/255 on >= 99%
+    of pixels; alpha-tested edges move a few pixels (visibility >= 99%)."""
+    from raytracedgl1_amd import rtgl as _r
+    s = scenes.terrain_boxes_scene()
+    rng = np.random.default_rng(11)
+    ground = rng.integers(0, 256, (64, 64, 4), dtype=np.uint8)
+    ground[..., 3] = 255
+    j, i = np.meshgrid(np.arange(16), np.arange(16), indexing="ij")
+    crate = np.zeros((16, 16, 4), np.uint8)
+    crate[..., 0], crate[..., 1], crate[..., 2] = 200, 150, 90
+    crate[..., 3] = np.where(((i // 4) + (j // 4)) % 2 == 0, 255, 0)
+    textures, texcoords, flags = {}, {}, {}
+    for k, (_, first, count) in enumerate(s.objects):
+        pos = s.tris[first:first + count].reshape(-1, 3).astype(np.float64)
+        texcoords[k] = np.stack([pos[:, 0] / 6.0, pos[:, 2] / 6.0], 1).astype(np.float32)
+        if k > 0 and k % 2 == 0:
+            textures[k], flags[k] = "crate", _r.RG_MESH_PRIMITIVE_ALPHA_TESTED
+        else:
+            textures[k] = "ground"
+    pixels = {"ground": (ground, _r.RG_SAMPLER_FILTER_LINEAR, 0, 0), "crate": (crate, _r.RG_SAMPLER_FILTER_NEAREST, 0, 0)}
+    print("parity: textured and alpha-tested frames vs FP64 oracle, terrain-boxes 160x90", flush=True)
+    for f, g, oc, m in _parity_frames(oracle, flags, None, (textures, texcoords, pixels)):
+        assert m["IS_SKY"] >= 0.995, f
+        for k in ("VISIBILITY_ID", "DEPTH_WORLD", "ALBEDO"):
+            assert m[k] >= 0.99, (f, k, m[k])
+        for k in ("RESERVOIRS_INITIAL", "RESERVOIRS"):
+            assert m[k] >= 0.98, (f, k)
+        for k, need in (("UNFILTERED_DIRECT", 0.975), ("UNFILTERED_SPECULAR", 0.95), ("UNFILTERED_INDIR", 0.9)):
+            assert m[k] >= need, (f, k, m[k])
+            assert abs(m[k + " mean"] - 1.0) < 0.05, (f, k)
+        assert m["PRE_FINAL"] >= 0.7, (f, m["PRE_FINAL"])
+        assert abs(m["PRE_FINAL mean"] - 1.0) < 0.03, f
