@@ -217,8 +217,8 @@ __device__ float BounceLod( float roughness, float viewDist, float hitDist, floa
 
 // Layer-0 albedo with its texture (processAlbedo*, HitInfo.inl:28-128, for
 // one opaque layer): colour * texture, saturated.
-__device__ v3 TexturedAlbedo( const SceneView& sc, const RgExtFrameParams& p, const GeomRecord& gr, uint32_t localPrim, float u, float v,
-                              v3 p0, v3 p1, v3 p2, v3 position, v3 normal, v3 rayOrigin, const TexQuery& q )
+__device__ __forceinline__ v3 TexturedAlbedo( const SceneView& sc, const RgExtFrameParams& p, const GeomRecord& gr, uint32_t localPrim,
+                                              float u, float v, v3 p0, v3 p1, v3 p2, v3 position, v3 normal, v3 rayOrigin, const TexQuery& q )
 {
     float2 t[ 3 ];
 #pragma unroll
