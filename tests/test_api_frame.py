@@ -406,3 +406,44 @@ def test_upload_order_does_not_change_the_image(boxes):
         same = np.all(a[k] == b[k], axis=-1) if a[k].ndim == 3 else a[k] == b[k]
         assert same.mean() >= 0.999, (k, same.mean())
     assert abs(a["PRE_FINAL"][..., :3].mean() / b["PRE_FINAL"][..., :3].mean() - 1.0) < 1e-4
+
+
+def test_fp16_nodes_with_refracting_split_and_alpha_test():
+    """The frame BVH's top node over the opaque and refracting subtrees and
+    the alpha-testing traversal kernels also exist in the FP16 node format:
+    frames with mirror / glass / water boxes and alpha-tested, textured boxes
+    are bit-identical to the FP32 nodes' (FP16 boxes only change the visit
+    order)."""
+    print("api: FP16 nodes with the refracting split and the alpha test", flush=True)
+    s = scenes.terrain_boxes_scene()
+    kinds = (rtgl.RG_MESH_PRIMITIVE_MIRROR, rtgl.RG_MESH_PRIMITIVE_GLASS, rtgl.RG_MESH_PRIMITIVE_WATER,
+             rtgl.RG_MESH_PRIMITIVE_ALPHA_TESTED)
+    flags = {i: kinds[i % 4] for i in range(1, len(s.objects))}
+    texcoords, textures = {}, {}
+    for k, (_, first, count) in enumerate(s.objects):
+        pos = s.tris[first:first + count].reshape(-1, 3).astype(np.float64)
+        texcoords[k] = np.stack([pos[:, 0] / 4.0, pos[:, 2] / 4.0], 1).astype(np.float32)
+        textures[k] = "t"
+    j, i = np.meshgrid(np.arange(8), np.arange(8), indexing="ij")
+    px = np.zeros((8, 8, 4), np.uint8)
+    px[..., 0], px[..., 1], px[..., 2] = 40 * (i % 6), 30 * (j % 8), 150
+    px[..., 3] = np.where((i + j) % 3 == 0, 0, 255)
+    up = rtgl.SceneUploader(s, flags, textures, texcoords)
+    cams = [scenes.terrain_boxes_camera()] * 2
+    out = {}
+    for prec in (0, ffi.RG_EXT_PRECISION_FP16_NODES):
+        r = rtgl.Renderer(W1, H1)
+        try:
+            r.provide_texture("t", px, rtgl.RG_SAMPLER_FILTER_NEAREST)
+            r.set_reflect_refract(rtgl.reflect_refract_params(maxReflectRefractDepth=3))
+            r.path.precisionFlags = prec
+            for c in cams:
+                r.frame(up, c)
+            fb = r.framebuffers()
+            out[prec] = {k: fb.download(PLANE[k]).copy() for k in ("VISIBILITY_BUFFER", "THROUGHPUT", "PRE_FINAL", "DEPTH_WORLD")}
+        finally:
+            r.close()
+    chain = np.abs(out[0]["THROUGHPUT"][..., 3]) == 1.0
+    assert chain.mean() > 0.01
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[ffi.RG_EXT_PRECISION_FP16_NODES][k]), k
