@@ -672,40 +672,78 @@ Hit GetHit( const Ctx& c, uint32_t prim, double u, double v, D3 rayOrigin, const
     }
     const D3 col = UnpackColor( gr.color );
     h.albedo     = mk( saturate( col.x ), saturate( col.y ), saturate( col.z ) );
-    if( gr.texture != 0 && gr.texture <= c.sc->textureCount )
+    double rough = gr.roughness, metal = gr.metallic;
+    bool   emissiveMap = false;
+    auto   valid       = [ & ]( uint32_t id ) { return id != 0 && id <= c.sc->textureCount; };
+    if( valid( gr.texture ) || valid( gr.texOrm ) || valid( gr.texNormal ) || valid( gr.texEmissive ) )
     {
-        const OrTexture* tex = c.sc->textures[ gr.texture - 1 ];
-        const float*     tc  = c.sc->texcoords + size_t( prim ) * 6;
-        const double     tu  = tc[ 0 ] * b0 + tc[ 2 ] * u + tc[ 4 ] * v, tv = tc[ 1 ] * b0 + tc[ 3 ] * u + tc[ 5 ] * v;
-        double           s[ 4 ];
+        // texture coordinates and the LOD source of the query (HitInfo.inl,
+        // RayCone.h:86-130, getBounceLOD)
+        const float* tc  = c.sc->texcoords + size_t( prim ) * 6;
+        const double tu  = tc[ 0 ] * b0 + tc[ 2 ] * u + tc[ 4 ] * v, tv = tc[ 1 ] * b0 + tc[ 3 ] * u + tc[ 5 ] * v;
+        double       g[ 4 ] = { 0, 0, 0, 0 }, lod = q.a;
         if( q.mode == TQ_GRAD )
         {
-            or_texture_sample_grad( tex, tu, tv, tc[ 0 ] * q.dbx.x + tc[ 2 ] * q.dbx.y + tc[ 4 ] * q.dbx.z,
-                                    tc[ 1 ] * q.dbx.x + tc[ 3 ] * q.dbx.y + tc[ 5 ] * q.dbx.z, tc[ 0 ] * q.dby.x + tc[ 2 ] * q.dby.y + tc[ 4 ] * q.dby.z,
-                                    tc[ 1 ] * q.dby.x + tc[ 3 ] * q.dby.y + tc[ 5 ] * q.dby.z, s );
+            g[ 0 ] = tc[ 0 ] * q.dbx.x + tc[ 2 ] * q.dbx.y + tc[ 4 ] * q.dbx.z, g[ 1 ] = tc[ 1 ] * q.dbx.x + tc[ 3 ] * q.dbx.y + tc[ 5 ] * q.dbx.z;
+            g[ 2 ] = tc[ 0 ] * q.dby.x + tc[ 2 ] * q.dby.y + tc[ 4 ] * q.dby.z, g[ 3 ] = tc[ 1 ] * q.dby.x + tc[ 3 ] * q.dby.y + tc[ 5 ] * q.dby.z;
         }
         else if( q.mode == TQ_CONE )
         {
             const double area   = length( cross( p1 - p0, p2 - p0 ) );
             const double pcw    = q.a / std::fabs( dot( q.dir, h.normal ) );
             const double uvArea = std::fabs( ( tc[ 2 ] - tc[ 0 ] ) * ( tc[ 5 ] - tc[ 1 ] ) - ( tc[ 4 ] - tc[ 0 ] ) * ( tc[ 3 ] - tc[ 1 ] ) );
-            const double U      = std::sqrt( uvArea * pcw * pcw / area );
-            or_texture_sample_grad( tex, tu, tv, U, 0, 0, U, s );
+            g[ 0 ] = g[ 3 ] = std::sqrt( uvArea * pcw * pcw / area );
         }
         else if( q.mode == TQ_BOUNCE )
         {
             const D3 cam = mk( c.p.cameraPosition[ 0 ], c.p.cameraPosition[ 1 ], c.p.cameraPosition[ 2 ] );
-            or_texture_sample_lod( tex, tu, tv, BounceLod( q.a, length( h.position - cam ), length( h.position - rayOrigin ), c.p.renderWidth, q.b ), s );
+            lod          = BounceLod( q.a, length( h.position - cam ), length( h.position - rayOrigin ), c.p.renderWidth, q.b );
         }
-        else
+        auto sample = [ & ]( uint32_t id, double scale, double out[ 4 ] ) {
+            if( q.mode == TQ_GRAD || q.mode == TQ_CONE )
+                or_texture_sample_grad( c.sc->textures[ id - 1 ], tu, tv, g[ 0 ] * scale, g[ 1 ] * scale, g[ 2 ] * scale, g[ 3 ] * scale, out );
+            else
+                or_texture_sample_lod( c.sc->textures[ id - 1 ], tu, tv, lod, out );
+        };
+        double s[ 4 ];
+        if( valid( gr.texture ) )
         {
-            or_texture_sample_lod( tex, tu, tv, q.a, s );
+            sample( gr.texture, 1.0, s );
+            h.albedo = mk( saturate( col.x * s[ 0 ] ), saturate( col.y * s[ 1 ] ), saturate( col.z * s[ 2 ] ) );
         }
-        h.albedo = mk( saturate( col.x * s[ 0 ] ), saturate( col.y * s[ 1 ] ), saturate( col.z * s[ 2 ] ) );
+        if( valid( gr.texOrm ) )
+        {
+            sample( gr.texOrm, 1.0, s );
+            rough = s[ 1 ], metal = s[ 2 ];
+        }
+        if( valid( gr.texEmissive ) )
+        {
+            sample( gr.texEmissive, 1.0, s );
+            h.emission  = mk( s[ 0 ], s[ 1 ], s[ 2 ] );
+            emissiveMap = true;
+        }
+        if( ( q.mode == TQ_GRAD || q.mode == TQ_CONE ) && valid( gr.texNormal ) )
+        {
+            // getTangent (VertexData.inl:249-269) on the world-space triangle,
+            // equal to the reference's model-space tangent times mat3(model)
+            // for rigid transforms; normal map with gradients x 5 on primary hits
+            sample( gr.texNormal, q.mode == TQ_GRAD ? 5.0 : 1.0, s );
+            const double nx = s[ 0 ] * 2.0 - 1.0, ny = s[ 1 ] * 2.0 - 1.0;
+            const D3     e1 = p1 - p0, e2 = p2 - p0;
+            const double u1x = tc[ 2 ] - tc[ 0 ], u1y = tc[ 3 ] - tc[ 1 ], u2x = tc[ 4 ] - tc[ 0 ], u2y = tc[ 5 ] - tc[ 1 ];
+            const double invDet = 1.0 / ( u1x * u2y - u2x * u1y );
+            const D3     T  = normalize( ( e1 * u2y - e2 * u1y ) * invDet );
+            const D3     Bg = normalize( ( e2 * u1x - e1 * u2x ) * invDet );
+            const D3     ns = safeNormalize( mk( n[ 0 ] + n[ 3 ] + n[ 6 ], n[ 1 ] + n[ 4 ] + n[ 7 ], n[ 2 ] + n[ 5 ] + n[ 8 ] ) );
+            const double w  = dot( cross( ns, T ), Bg ) > 0.0 ? 1.0 : -1.0;
+            const D3     B  = cross( h.normal, T ) * w;
+            const D3     nn = safeNormalize( T * nx + B * ny + h.normal );
+            h.normal        = safeNormalize( mix3( h.normal, nn, c.p.normalMapStrength ) );
+        }
     }
-    h.roughness  = std::max( double( gr.roughness ), std::max( double( c.p.minRoughness ), MIN_ROUGH ) );
-    h.metallic   = gr.metallic;
-    h.emission   = h.albedo * double( gr.emissive );
+    h.roughness  = std::max( rough, std::max( double( c.p.minRoughness ), MIN_ROUGH ) );
+    h.metallic   = metal;
+    if( !emissiveMap ) h.emission = h.albedo * double( gr.emissive );
     h.geom       = g;
     h.localPrim  = prim - gr.firstTri;
     return h;

@@ -24,6 +24,7 @@ typedef struct OrGeom
     uint32_t firstTri;
     uint32_t triCount;
     uint32_t texture;           /* 1 + index into OrPtScene::textures, 0: none */
+    uint32_t texOrm, texNormal, texEmissive; /* the _orm, _n and _e maps, same ids */
 } OrGeom;
 
 /* World-space triangles with their geometries and the frame's lights. */

@@ -19,7 +19,7 @@ struct OrTexture
 {
     std::vector< std::vector< uint32_t > > levels;
     std::vector< int >                     w, h;
-    bool                                   magNearest, minNearest, clampU, clampV;
+    bool                                   magNearest, minNearest, clampU, clampV, srgb;
 };
 
 namespace
@@ -44,12 +44,17 @@ int Wrap( int i, int n, bool clamp )
     return m < 0 ? m + n : m;
 }
 
+double Decode( bool srgb, uint32_t c8 )
+{
+    return srgb ? SrgbDecode( c8 ) : c8 / 255.0;
+}
+
 void Texel( const OrTexture& t, int l, int x, int y, double out[ 4 ] )
 {
     const uint32_t c = t.levels[ l ][ size_t( Wrap( y, t.h[ l ], t.clampV ) ) * t.w[ l ] + Wrap( x, t.w[ l ], t.clampU ) ];
-    out[ 0 ] = SrgbDecode( c & 0xFFu );
-    out[ 1 ] = SrgbDecode( ( c >> 8 ) & 0xFFu );
-    out[ 2 ] = SrgbDecode( ( c >> 16 ) & 0xFFu );
+    out[ 0 ] = Decode( t.srgb, c & 0xFFu );
+    out[ 1 ] = Decode( t.srgb, ( c >> 8 ) & 0xFFu );
+    out[ 2 ] = Decode( t.srgb, ( c >> 16 ) & 0xFFu );
     out[ 3 ] = ( c >> 24 ) / 255.0;
 }
 
@@ -74,9 +79,10 @@ void SampleLevel( const OrTexture& t, int l, double u, double v, bool nearest, d
 } // namespace
 
 extern "C" OrTexture* or_texture_create( const uint8_t* rgba, uint32_t width, uint32_t height, int filter, int addressU, int addressV,
-                                         int forceMinificationLinear )
+                                         int forceMinificationLinear, int srgb )
 {
     OrTexture* t   = new OrTexture;
+    t->srgb        = srgb != 0;
     const bool nearest = filter == RG_SAMPLER_FILTER_NEAREST;
     t->magNearest  = nearest;
     t->minNearest  = nearest && !forceMinificationLinear;
@@ -107,13 +113,20 @@ extern "C" OrTexture* or_texture_create( const uint8_t* rgba, uint32_t width, ui
                     const int   yy = std::min( std::max( int( fy ) + ( k >> 1 ), 0 ), sh - 1 );
                     const double wk = ( ( k & 1 ) ? ax : 1 - ax ) * ( ( k >> 1 ) ? ay : 1 - ay );
                     const uint32_t c = t->levels[ l ][ size_t( yy ) * sw + xx ];
-                    acc[ 0 ] += wk * SrgbDecode( c & 0xFFu );
-                    acc[ 1 ] += wk * SrgbDecode( ( c >> 8 ) & 0xFFu );
-                    acc[ 2 ] += wk * SrgbDecode( ( c >> 16 ) & 0xFFu );
-                    acc[ 3 ] += wk * ( ( c >> 24 ) / 255.0 );
+                    // sRGB channels filter in linear space; UNORM channels and
+                    // alpha filter the 8-bit codes (the same value, without a
+                    // /255 * 255 round trip before the rounding)
+                    for( int ch = 0; ch < 3; ch++ )
+                    {
+                        const uint32_t b = ( c >> ( 8 * ch ) ) & 0xFFu;
+                        acc[ ch ] += wk * ( t->srgb ? SrgbDecode( b ) : double( b ) );
+                    }
+                    acc[ 3 ] += wk * double( c >> 24 );
                 }
-                const uint32_t a = uint32_t( std::nearbyint( std::min( std::max( acc[ 3 ], 0.0 ), 1.0 ) * 255.0 ) );
-                dst[ size_t( y ) * dw + x ] = SrgbEncode( acc[ 0 ] ) | ( SrgbEncode( acc[ 1 ] ) << 8 ) | ( SrgbEncode( acc[ 2 ] ) << 16 ) | ( a << 24 );
+                auto code = []( double v ) { return uint32_t( std::nearbyint( std::min( std::max( v, 0.0 ), 255.0 ) ) ); };
+                uint32_t out = code( acc[ 3 ] ) << 24;
+                for( int ch = 0; ch < 3; ch++ ) out |= ( t->srgb ? SrgbEncode( acc[ ch ] ) : code( acc[ ch ] ) ) << ( 8 * ch );
+                dst[ size_t( y ) * dw + x ] = out;
             }
         }
         t->levels.push_back( std::move( dst ) );

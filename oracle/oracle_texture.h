@@ -15,8 +15,9 @@ extern "C" {
 typedef struct OrTexture OrTexture;
 
 /* filter / address modes: RgSamplerFilter, RgSamplerAddressMode values. */
+/* srgb: 0 for the UNORM maps (occlusion-roughness-metallic, normal). */
 OrTexture* or_texture_create( const uint8_t* rgba, uint32_t width, uint32_t height, int filter, int addressU, int addressV,
-                              int forceMinificationLinear );
+                              int forceMinificationLinear, int srgb );
 void       or_texture_free( OrTexture* t );
 uint32_t   or_texture_mip_count( const OrTexture* t );
 /* Level l's texels (RGBA8) into out (width_l * height_l * 4 bytes). */

@@ -122,6 +122,7 @@ void FillFrameParams( CameraState& cs, const RgDrawFrameInfo& di, uint32_t w, ui
     const auto* tx             = FindParams< RgDrawFrameTexturesParams >( di, RG_STRUCTURE_TYPE_TEXTURES );
     p.minRoughness             = tx ? std::clamp( tx->minRoughness, 0.0f, 1.0f ) : 0.0f;
     p.emissionMapBoost         = tx ? std::max( tx->emissionMapBoost, 0.0f ) : 100.0f;
+    p.normalMapStrength        = tx ? tx->normalMapStrength : 1.0f;
 
     p.rayCullBackFaces  = cullBackFaces ? 1u : 0u;
     p.rayLength         = std::clamp( di.rayLength, 0.1f, 10000.0f );

@@ -217,28 +217,39 @@ __device__ float BounceLod( float roughness, float viewDist, float hitDist, floa
 
 // Layer-0 albedo with its texture (processAlbedo*, HitInfo.inl:28-128, for
 // one opaque layer): colour * texture, saturated.
-__device__ __forceinline__ v3 TexturedAlbedo( const SceneView& sc, const RgExtFrameParams& p, const GeomRecord& gr, uint32_t localPrim,
-                                              float u, float v, v3 p0, v3 p1, v3 p2, v3 position, v3 normal, v3 rayOrigin, const TexQuery& q )
+// Texture coordinates of a hit and the LOD source of its query: one
+// sampler for all maps of a material (HitInfo.inl:28-128, :386-447).
+struct MaterialCoords
 {
     float2 t[ 3 ];
+    float  u, v;                     // interpolated texture coordinates
+    float  dux, dvx, duy, dvy;       // gradients (TQ_GRAD, TQ_CONE)
+    float  lod;                      // TQ_LOD, TQ_BOUNCE
+};
+
+__device__ __forceinline__ MaterialCoords GetMaterialCoords( const SceneView& sc, const RgExtFrameParams& p, const GeomRecord& gr, uint32_t localPrim,
+                                                             float u, float v, v3 p0, v3 p1, v3 p2, v3 position, v3 normal, v3 rayOrigin,
+                                                             const TexQuery& q )
+{
+    MaterialCoords m;
 #pragma unroll
     for( int k = 0; k < 3; k++ )
     {
         const uint32_t idx = gr.baseIndex == 0xFFFFFFFFu ? 3 * localPrim + k : sc.indices[ gr.baseIndex + 3 * localPrim + k ];
         const RgPrimitiveVertex& vx = sc.verts[ gr.baseVertex + idx ];
-        t[ k ] = make_float2( vx.texCoord[ 0 ], vx.texCoord[ 1 ] );
+        m.t[ k ] = make_float2( vx.texCoord[ 0 ], vx.texCoord[ 1 ] );
     }
     const float b0 = 1.0f - u - v;
-    const float tu = t[ 0 ].x * b0 + t[ 1 ].x * u + t[ 2 ].x * v;
-    const float tv = t[ 0 ].y * b0 + t[ 1 ].y * u + t[ 2 ].y * v;
-    float4      c;
+    m.u            = m.t[ 0 ].x * b0 + m.t[ 1 ].x * u + m.t[ 2 ].x * v;
+    m.v            = m.t[ 0 ].y * b0 + m.t[ 1 ].y * u + m.t[ 2 ].y * v;
+    m.dux = m.dvx = m.duy = m.dvy = 0.0f;
+    m.lod = 0.0f;
     if( q.mode == TQ_GRAD )
     {
-        const float dux = t[ 0 ].x * q.dbx.x + t[ 1 ].x * q.dbx.y + t[ 2 ].x * q.dbx.z;
-        const float dvx = t[ 0 ].y * q.dbx.x + t[ 1 ].y * q.dbx.y + t[ 2 ].y * q.dbx.z;
-        const float duy = t[ 0 ].x * q.dby.x + t[ 1 ].x * q.dby.y + t[ 2 ].x * q.dby.z;
-        const float dvy = t[ 0 ].y * q.dby.x + t[ 1 ].y * q.dby.y + t[ 2 ].y * q.dby.z;
-        c               = SampleGrad( sc.tex, gr.texture, tu, tv, dux, dvx, duy, dvy );
+        m.dux = m.t[ 0 ].x * q.dbx.x + m.t[ 1 ].x * q.dbx.y + m.t[ 2 ].x * q.dbx.z;
+        m.dvx = m.t[ 0 ].y * q.dbx.x + m.t[ 1 ].y * q.dbx.y + m.t[ 2 ].y * q.dbx.z;
+        m.duy = m.t[ 0 ].x * q.dby.x + m.t[ 1 ].x * q.dby.y + m.t[ 2 ].x * q.dby.z;
+        m.dvy = m.t[ 0 ].y * q.dby.x + m.t[ 1 ].y * q.dby.y + m.t[ 2 ].y * q.dby.z;
     }
     else if( q.mode == TQ_CONE )
     {
@@ -246,21 +257,95 @@ __device__ __forceinline__ v3 TexturedAlbedo( const SceneView& sc, const RgExtFr
         const float quadArea = length( cross( p1 - p0, p2 - p0 ) );
         const float pcw      = q.a / fabsf( dot( q.dir, normal ) );
         const float ratio    = pcw * pcw / quadArea;
-        const float uvArea   = fabsf( ( t[ 1 ].x - t[ 0 ].x ) * ( t[ 2 ].y - t[ 0 ].y ) - ( t[ 2 ].x - t[ 0 ].x ) * ( t[ 1 ].y - t[ 0 ].y ) );
-        const float U        = sqrtf( uvArea * ratio );
-        c                    = SampleGrad( sc.tex, gr.texture, tu, tv, U, 0.0f, 0.0f, U );
+        const float uvArea   = fabsf( ( m.t[ 1 ].x - m.t[ 0 ].x ) * ( m.t[ 2 ].y - m.t[ 0 ].y ) - ( m.t[ 2 ].x - m.t[ 0 ].x ) * ( m.t[ 1 ].y - m.t[ 0 ].y ) );
+        m.dux = m.dvy = sqrtf( uvArea * ratio );
     }
     else if( q.mode == TQ_BOUNCE )
     {
         const v3 cam = mk3( p.cameraPosition[ 0 ], p.cameraPosition[ 1 ], p.cameraPosition[ 2 ] );
-        c = SampleLod( sc.tex, gr.texture, tu, tv, BounceLod( q.a, length( position - cam ), length( position - rayOrigin ), float( p.renderWidth ), q.b ) );
+        m.lod        = BounceLod( q.a, length( position - cam ), length( position - rayOrigin ), float( p.renderWidth ), q.b );
     }
     else
     {
-        c = SampleLod( sc.tex, gr.texture, tu, tv, q.a );
+        m.lod = q.a;
     }
-    const v3 col = UnpackColor( gr.color );
-    return mk3( saturate( col.x * c.x ), saturate( col.y * c.y ), saturate( col.z * c.z ) );
+    return m;
+}
+
+// One map at the query's LOD; gradients scaled by `gradScale` (the normal
+// map's suppressDetails).
+__device__ __forceinline__ float4 SampleMaterial( const SceneView& sc, uint32_t id, const MaterialCoords& m, const TexQuery& q, float gradScale )
+{
+    if( q.mode == TQ_GRAD || q.mode == TQ_CONE )
+    {
+        return SampleGrad( sc.tex, id, m.u, m.v, m.dux * gradScale, m.dvx * gradScale, m.duy * gradScale, m.dvy * gradScale );
+    }
+    return SampleLod( sc.tex, id, m.u, m.v, m.lod );
+}
+
+// getTangent (VertexData.inl:249-269) from the model-space vertices, then
+// into world space by the model matrix's 3x3 part (:488-494): xyz tangent,
+// w handedness.
+__device__ float4 TriangleTangent( const SceneView& sc, const GeomRecord& gr, uint32_t localPrim, const MaterialCoords& m )
+{
+    v3 pos[ 3 ], nsum = mk3( 0.f, 0.f, 0.f );
+#pragma unroll
+    for( int k = 0; k < 3; k++ )
+    {
+        const uint32_t idx = gr.baseIndex == 0xFFFFFFFFu ? 3 * localPrim + k : sc.indices[ gr.baseIndex + 3 * localPrim + k ];
+        const RgPrimitiveVertex& vx = sc.verts[ gr.baseVertex + idx ];
+        pos[ k ] = mk3( vx.position[ 0 ], vx.position[ 1 ], vx.position[ 2 ] );
+        nsum     = nsum + mk3( vx.normal[ 0 ], vx.normal[ 1 ], vx.normal[ 2 ] );
+    }
+    const v3     e1 = pos[ 1 ] - pos[ 0 ], e2 = pos[ 2 ] - pos[ 0 ];
+    const float2 u1 = make_float2( m.t[ 1 ].x - m.t[ 0 ].x, m.t[ 1 ].y - m.t[ 0 ].y );
+    const float2 u2 = make_float2( m.t[ 2 ].x - m.t[ 0 ].x, m.t[ 2 ].y - m.t[ 0 ].y );
+    const float  invDet = 1.0f / ( u1.x * u2.y - u2.x * u1.y );
+    const v3     tg = normalize( ( e1 * u2.y - e2 * u1.y ) * invDet );
+    const v3     bt = normalize( ( e2 * u1.x - e1 * u2.x ) * invDet );
+    const float  w  = dot( cross( safeNormalize( nsum ), tg ), bt ) > 0.0f ? 1.0f : -1.0f;
+    const float* M  = gr.model;
+    return make_float4( M[ 0 ] * tg.x + M[ 1 ] * tg.y + M[ 2 ] * tg.z, M[ 4 ] * tg.x + M[ 5 ] * tg.y + M[ 6 ] * tg.z,
+                        M[ 8 ] * tg.x + M[ 9 ] * tg.y + M[ 10 ] * tg.z, w );
+}
+
+// Material maps of a textured hit (HitInfo.inl:28-128, :386-447): albedo =
+// colour x texture; roughness / metallic from the ORM map's g / b; emission
+// from the emission map; the normal map (primary and reflected hits only)
+// bends the shading normal by normalMapStrength.
+__device__ void ApplyMaterial( const SceneView& sc, const RgExtFrameParams& p, const GeomRecord& gr, uint32_t localPrim, float u, float v, v3 p0,
+                               v3 p1, v3 p2, v3 rayOrigin, const TexQuery& q, HitInfo& h, float* rough, float* metal, bool* emissiveMap )
+{
+    const MaterialCoords m = GetMaterialCoords( sc, p, gr, localPrim, u, v, p0, p1, p2, h.position, h.normal, rayOrigin, q );
+    if( TexValid( sc.tex, gr.texture ) )
+    {
+        const float4 c   = SampleMaterial( sc, gr.texture, m, q, 1.0f );
+        const v3     col = UnpackColor( gr.color );
+        h.albedo         = mk3( saturate( col.x * c.x ), saturate( col.y * c.y ), saturate( col.z * c.z ) );
+    }
+    if( TexValid( sc.tex, gr.texOrm ) )
+    {
+        const float4 orm = SampleMaterial( sc, gr.texOrm, m, q, 1.0f );
+        *rough           = orm.y;
+        *metal           = orm.z;
+    }
+    if( TexValid( sc.tex, gr.texEmissive ) )
+    {
+        const float4 e = SampleMaterial( sc, gr.texEmissive, m, q, 1.0f );
+        h.emission     = mk3( e.x, e.y, e.z );
+        *emissiveMap   = true;
+    }
+    if( ( q.mode == TQ_GRAD || q.mode == TQ_CONE ) && TexValid( sc.tex, gr.texNormal ) )
+    {
+        // less detail in normal maps for better denoising: gradients x 5
+        const float4 nm  = SampleMaterial( sc, gr.texNormal, m, q, q.mode == TQ_GRAD ? 5.0f : 1.0f );
+        const float  nx  = nm.x * 2.0f - 1.0f, ny = nm.y * 2.0f - 1.0f;
+        const float4 tg  = TriangleTangent( sc, gr, localPrim, m );
+        const v3     T   = mk3( tg.x, tg.y, tg.z );
+        const v3     B   = cross( h.normal, T ) * tg.w;
+        const v3     nn  = safeNormalize( T * nx + B * ny + h.normal );
+        h.normal         = safeNormalize( mix3( h.normal, nn, p.normalMapStrength ) );
+    }
 }
 
 // getHitInfo* (HitInfo.inl:204-466): albedo is the primitive colour (layer
@@ -288,18 +373,20 @@ __device__ HitInfo GetHit( const SceneView& sc, const RgExtFrameParams& p, uint3
             h.normal = -h.normal;
         }
     }
-    if( !TexValid( sc.tex, gr.texture ) )
+    const v3 c  = UnpackColor( gr.color );
+    h.albedo    = mk3( saturate( c.x ), saturate( c.y ), saturate( c.z ) );
+    float rough = gr.roughness, metal = gr.metallic;
+    bool  emissiveMap = false;
+    if( gr.texture != TEX_NONE || gr.texOrm != TEX_NONE || gr.texNormal != TEX_NONE || gr.texEmissive != TEX_NONE )
     {
-        const v3 c = UnpackColor( gr.color );
-        h.albedo   = mk3( saturate( c.x ), saturate( c.y ), saturate( c.z ) );
+        ApplyMaterial( sc, p, gr, prim - gr.firstTri, u, v, p0, p1, p2, rayOrigin, q, h, &rough, &metal, &emissiveMap );
     }
-    else
+    h.roughness = fmaxf( rough, fmaxf( p.minRoughness, MIN_GGX_ROUGH ) );
+    h.metallic  = metal;
+    if( !emissiveMap )
     {
-        h.albedo = TexturedAlbedo( sc, p, gr, prim - gr.firstTri, u, v, p0, p1, p2, h.position, h.normal, rayOrigin, q );
+        h.emission = h.albedo * gr.emissive;
     }
-    h.roughness = fmaxf( gr.roughness, fmaxf( p.minRoughness, MIN_GGX_ROUGH ) );
-    h.metallic  = gr.metallic;
-    h.emission  = h.albedo * gr.emissive;
     h.geom      = g;
     h.localPrim = prim - gr.firstTri;
     return h;

@@ -29,9 +29,12 @@ struct GeomRecord
     uint32_t flags;             // RgMeshPrimitiveFlags
     uint32_t isStatic;          // prevFromCur is the identity
     uint32_t texture;           // albedo texture id (pTextureName), TEX_NONE if none
-    uint32_t pad;
+    uint32_t texOrm;            // pTextureName + "_orm": occlusion-roughness-metallic
+    uint32_t texNormal;         // pTextureName + "_n": tangent-space normal
+    uint32_t texEmissive;       // pTextureName + "_e": emission
+    uint32_t pad[ 2 ];
 };
-static_assert( sizeof( GeomRecord ) == 144, "GeomRecord" );
+static_assert( sizeof( GeomRecord ) == 160, "GeomRecord" );
 
 // ShLightEncoded (Source/Generated/ShaderCommonC.h:350-357), 64 bytes.
 struct LightEncoded

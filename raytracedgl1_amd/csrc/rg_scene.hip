@@ -10,6 +10,8 @@
 #include "rg_scene.h"
 #include "rg_textures.h"
 
+#include <string>
+
 #include <algorithm>
 #include <thread>
 #include <vector>
@@ -181,7 +183,14 @@ void SceneCollector::AddPrimitive( const RgMeshInfo& mesh, const RgMeshPrimitive
     g.roughness    = pbr ? Saturate( prim.pEditorInfo->pbrInfo.roughnessDefault ) : 1.0f;
     g.metallic     = pbr ? Saturate( prim.pEditorInfo->pbrInfo.metallicDefault ) : 0.0f;
     g.flags        = prim.flags;
-    g.texture      = textures ? textures->Find( prim.pTextureName ) : TEX_NONE;
+    if( textures && prim.pTextureName && prim.pTextureName[ 0 ] )
+    {
+        const std::string base( prim.pTextureName );
+        g.texture     = textures->Find( base.c_str() );
+        g.texOrm      = textures->Find( ( base + "_orm" ).c_str() );
+        g.texNormal   = textures->Find( ( base + "_n" ).c_str() );
+        g.texEmissive = textures->Find( ( base + "_e" ).c_str() );
+    }
 
     // Previous transform of the same (object, primitive): UniqueID::MakeForPrimitive.
     const uint64_t uid = ( uint64_t( mesh.uniqueObjectID ) << 32 ) | prim.primitiveIndexInMesh;

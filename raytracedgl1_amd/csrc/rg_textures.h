@@ -18,6 +18,10 @@ namespace rg
 {
 
 // Texture ids in geometry records are table index + 1; 0 is no texture.
+// A material named N takes its maps from the textures named N (albedo, sRGB),
+// N_orm (occlusion-roughness-metallic, UNORM), N_n (normal, UNORM) and N_e
+// (emission, sRGB): the reference's override postfixes (Source/Const.h:40-43,
+// TextureManager.cpp:418-425), applied to provided textures.
 constexpr uint32_t TEX_NONE      = 0;
 constexpr uint32_t TEX_MAX_MIPS  = 16;
 constexpr uint32_t TEX_MAX_SIZE  = 1u << 15;
@@ -28,7 +32,7 @@ struct TextureDesc
     uint32_t        width, height, mips;
     uint32_t        magNearest, minNearest; // 1: nearest texel within a level
     uint32_t        clampU, clampV;         // 1: clamp to edge, 0: repeat
-    uint32_t        pad;
+    uint32_t        srgb;                   // 0: UNORM colour (the _orm and _n maps)
 };
 
 // What the kernels see of the texture table.
@@ -100,6 +104,11 @@ __device__ __forceinline__ int TexWrap( int i, int n, uint32_t clampMode )
 __device__ __forceinline__ float4 TexelAt( const TexView& tv, const TextureDesc& d, const uint32_t* level, int w, int h, int x, int y )
 {
     const uint32_t c = level[ size_t( TexWrap( y, h, d.clampV ) ) * w + TexWrap( x, w, d.clampU ) ];
+    if( !d.srgb )
+    {
+        return make_float4( float( c & 0xFFu ) * ( 1.0f / 255.0f ), float( ( c >> 8 ) & 0xFFu ) * ( 1.0f / 255.0f ),
+                            float( ( c >> 16 ) & 0xFFu ) * ( 1.0f / 255.0f ), float( c >> 24 ) * ( 1.0f / 255.0f ) );
+    }
     return make_float4( tv.srgb[ c & 0xFFu ], tv.srgb[ ( c >> 8 ) & 0xFFu ], tv.srgb[ ( c >> 16 ) & 0xFFu ], float( c >> 24 ) * ( 1.0f / 255.0f ) );
 }
 

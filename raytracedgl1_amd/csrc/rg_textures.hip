@@ -27,7 +27,8 @@ __device__ uint32_t EncodeChannel( float c )
 // One level of the chain from the level above, as a linear-filtered blit of
 // the whole image does: each destination texel centre maps into the source
 // and is filtered bilinearly there (clamp to edge), in linear space.
-__global__ __launch_bounds__( THREADS ) void k_mip_level( const uint32_t* src, int sw, int sh, uint32_t* dst, int dw, int dh, const float* lut )
+__global__ __launch_bounds__( THREADS ) void k_mip_level( const uint32_t* src, int sw, int sh, uint32_t* dst, int dw, int dh, const float* lut,
+                                                          uint32_t srgb )
 {
     const int i = blockIdx.x * THREADS + threadIdx.x;
     if( i >= dw * dh )
@@ -47,13 +48,23 @@ __global__ __launch_bounds__( THREADS ) void k_mip_level( const uint32_t* src, i
         const int   cx = xx < 0 ? 0 : ( xx >= sw ? sw - 1 : xx ), cy = yy < 0 ? 0 : ( yy >= sh ? sh - 1 : yy );
         const float wgt = ( ( k & 1 ) ? ax : 1.0f - ax ) * ( ( k >> 1 ) ? ay : 1.0f - ay );
         const uint32_t c = src[ size_t( cy ) * sw + cx ];
-        acc[ 0 ] += wgt * lut[ c & 0xFFu ];
-        acc[ 1 ] += wgt * lut[ ( c >> 8 ) & 0xFFu ];
-        acc[ 2 ] += wgt * lut[ ( c >> 16 ) & 0xFFu ];
-        acc[ 3 ] += wgt * ( float( c >> 24 ) * ( 1.0f / 255.0f ) );
+        // UNORM channels (and alpha) filter the 8-bit codes themselves: with
+        // the exact 1/4 weights of a halving blit the sums are exact, so the
+        // rounding of .5 cases does not depend on the arithmetic's precision
+        for( int ch = 0; ch < 3; ch++ )
+        {
+            const uint32_t b = ( c >> ( 8 * ch ) ) & 0xFFu;
+            acc[ ch ] += wgt * ( srgb ? lut[ b ] : float( b ) );
+        }
+        acc[ 3 ] += wgt * float( c >> 24 );
     }
-    const float a = rintf( ( acc[ 3 ] < 0.f ? 0.f : ( acc[ 3 ] > 1.f ? 1.f : acc[ 3 ] ) ) * 255.0f );
-    dst[ i ] = EncodeChannel( acc[ 0 ] ) | ( EncodeChannel( acc[ 1 ] ) << 8 ) | ( EncodeChannel( acc[ 2 ] ) << 16 ) | ( uint32_t( a ) << 24 );
+    auto code = []( float v ) { return uint32_t( rintf( v < 0.f ? 0.f : ( v > 255.f ? 255.f : v ) ) ); };
+    uint32_t out = code( acc[ 3 ] ) << 24;
+    for( int ch = 0; ch < 3; ch++ )
+    {
+        out |= ( srgb ? EncodeChannel( acc[ ch ] ) : code( acc[ ch ] ) ) << ( 8 * ch );
+    }
+    dst[ i ] = out;
 }
 } // namespace
 
@@ -74,6 +85,12 @@ void TextureManager::Provide( const RgOriginalTextureInfo& info, hipStream_t s )
         RG_HIP_CHECK( hipMemcpy( srgbLut.ptr, lut, sizeof( lut ), hipMemcpyHostToDevice ) );
     }
     const std::string name( info.pTextureName );
+    auto endsWith = [ & ]( const char* sfx ) {
+        const size_t n = std::strlen( sfx );
+        return name.size() > n && name.compare( name.size() - n, n, sfx ) == 0;
+    };
+    // the occlusion-roughness-metallic and normal maps are UNORM, the others sRGB
+    const uint32_t srgb = ( endsWith( "_orm" ) || endsWith( "_n" ) ) ? 0u : 1u;
     uint32_t          id;
     auto              it = byName.find( name );
     if( it != byName.end() )
@@ -109,7 +126,8 @@ void TextureManager::Provide( const RgOriginalTextureInfo& info, hipStream_t s )
     {
         const uint32_t nw = lw > 1 ? lw >> 1 : 1, nh = lh > 1 ? lh >> 1 : 1;
         uint32_t*      next = level + size_t( lw ) * lh;
-        k_mip_level<<< DivUp( nw * nh, THREADS ), THREADS, 0, s >>>( level, int( lw ), int( lh ), next, int( nw ), int( nh ), srgbLut.as< float >() );
+        k_mip_level<<< DivUp( nw * nh, THREADS ), THREADS, 0, s >>>( level, int( lw ), int( lh ), next, int( nw ), int( nh ), srgbLut.as< float >(),
+                                                                     srgb );
         CheckLaunch( "k_mip_level" );
         level = next, lw = nw, lh = nh;
     }
@@ -126,6 +144,7 @@ void TextureManager::Provide( const RgOriginalTextureInfo& info, hipStream_t s )
     e.desc.minNearest  = ( nearest && !forceMinLinear ) ? 1u : 0u;
     e.desc.clampU      = info.addressModeU == RG_SAMPLER_ADDRESS_MODE_CLAMP ? 1u : 0u;
     e.desc.clampV      = info.addressModeV == RG_SAMPLER_ADDRESS_MODE_CLAMP ? 1u : 0u;
+    e.desc.srgb        = srgb;
     e.live             = true;
     byName[ name ]     = id;
     dirty              = true;

@@ -69,7 +69,7 @@ class Oracle:
             "or_denoise_variance": (None, [_P, _P]),
             "or_denoise_atrous": (None, [_P, _P, _U32]),
             "or_denoise": (None, [_P, _P]),
-            "or_texture_create": (_P, [_P, _U32, _U32, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "or_texture_create": (_P, [_P, _U32, _U32, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
             "or_texture_free": (None, [_P]),
             "or_texture_mip_count": (_U32, [_P]),
             "or_texture_copy_mip": (None, [_P, _U32, _P]),
@@ -238,18 +238,19 @@ class OracleFramebuffers:
 class OrGeomC(C.Structure):
     _fields_ = [("prevFromCur", C.c_float * 12), ("isStatic", _U32), ("color", _U32), ("emissive", _F),
                 ("roughness", _F), ("metallic", _F), ("flags", _U32), ("firstTri", _U32), ("triCount", _U32),
-                ("texture", _U32)]
+                ("texture", _U32), ("texOrm", _U32), ("texNormal", _U32), ("texEmissive", _U32)]
 
 
 class OracleTexture:
     """The oracle's albedo texture (oracle/oracle_texture.cpp) of
     (height, width, 4) uint8 sRGB pixels."""
 
-    def __init__(self, o: "Oracle", rgba8, filter: int = 0, address_u: int = 0, address_v: int = 0, force_min_linear: bool = False):
+    def __init__(self, o: "Oracle", rgba8, filter: int = 0, address_u: int = 0, address_v: int = 0, force_min_linear: bool = False,
+                 srgb: bool = True):
         self.o = o
         self.px = np.ascontiguousarray(rgba8, np.uint8)
         self.h = o.l.or_texture_create(self.px.ctypes.data, self.px.shape[1], self.px.shape[0], filter, address_u, address_v,
-                                       1 if force_min_linear else 0)
+                                       1 if force_min_linear else 0, 1 if srgb else 0)
 
     def mips(self) -> list:
         out, w, h = [], self.px.shape[1], self.px.shape[0]
@@ -309,7 +310,8 @@ class OraclePathTracer:
         flags = dict(flags or {})
         textures, texcoords = dict(textures or {}), dict(texcoords or {})
         names = sorted(texture_pixels or {})
-        self.tex_objs = [OracleTexture(o, *texture_pixels[nm]) for nm in names]
+        # the _orm and _n maps are UNORM, the rest sRGB (as the library stores them)
+        self.tex_objs = [OracleTexture(o, *texture_pixels[nm], srgb=not nm.endswith(("_orm", "_n"))) for nm in names]
         self.tex_ptrs = (_P * max(1, len(names)))(*[t.h for t in self.tex_objs])
         tex_id = {nm: i + 1 for i, nm in enumerate(names)}
         self.h = o.l.or_pt_create(width, height)
@@ -333,7 +335,10 @@ class OraclePathTracer:
             gr.isStatic, gr.color = 1, int(scene.colors[first]) | 0xFF000000
             gr.emissive, gr.roughness, gr.metallic = 0.0, 1.0, 0.0
             gr.flags, gr.firstTri, gr.triCount = flags.get(oid, 0), pos, count
-            gr.texture = tex_id.get(textures.get(oid), 0)
+            base = textures.get(oid)
+            if base:
+                gr.texture = tex_id.get(base, 0)
+                gr.texOrm, gr.texNormal, gr.texEmissive = (tex_id.get(base + sfx, 0) for sfx in ("_orm", "_n", "_e"))
             if oid in texcoords:
                 self.texcoords[pos:pos + count] = np.asarray(texcoords[oid], np.float32).reshape(count, 6)
             self.ids[g] = oid << 32

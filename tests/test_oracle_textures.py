@@ -82,3 +82,12 @@ def test_oracle_alpha_tested_pane_shadow(oracle):
     cur = _frame(oracle, scene, {1: rtgl.RG_MESH_PRIMITIVE_ALPHA_TESTED}, {1: "pane"}, uv,
                  {"pane": (tc.alpha_strip(), NEAREST, clamp, clamp)}, rc.frame_params(tc.SHADOW_CAMERA))
     tc.check_alpha_shadow(cur[PLANE["UNFILTERED_DIRECT"]], cur[PLANE["SURFACE_POSITION"]])
+
+
+def test_oracle_material_maps(oracle):
+    scene, uv = tc.quad_scene()
+    pixels = {"tex": (tc.checker8(), NEAREST, 0, 0), "tex_orm": (tc.orm_map(), NEAREST, 0, 0), "tex_n": (tc.normal_map(), NEAREST, 0, 0)}
+    cur = _frame(oracle, scene, {}, {0: "tex"}, uv, pixels, rc.frame_params(rc.camera(60.0)))
+    tc.check_orm(cur[PLANE["METALLIC_ROUGHNESS"]])
+    tc.check_normal_map(cur[PLANE["NORMAL"]],
+                        lambda a: np.array([oracle.decode_normal(int(x)) for x in a.ravel()]).reshape(a.shape[:2] + (3,)))

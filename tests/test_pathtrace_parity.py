@@ -75,7 +75,8 @@ def _parity_frames(oracle, flags=None, rr=None, tex=None):
             fbv = r.framebuffers()
             g = {k: fbv.download(P[k]) for k in ("IS_SKY", "VISIBILITY_BUFFER", "DEPTH_WORLD", "MOTION", "RESERVOIRS_INITIAL",
                                                    "RESERVOIRS", "UNFILTERED_DIRECT", "UNFILTERED_SPECULAR", "UNFILTERED_INDIR",
-                                                   "GRADIENT_PREV_PIX", "DIS_GRADIENT_HISTORY", "PRE_FINAL", "THROUGHPUT", "ALBEDO")}
+                                                   "GRADIENT_PREV_PIX", "DIS_GRADIENT_HISTORY", "PRE_FINAL", "THROUGHPUT", "ALBEDO",
+                                                   "NORMAL", "METALLIC_ROUGHNESS")}
             if f:
                 ofb.swap()
             opt.scene_lights(sc)
@@ -96,12 +97,16 @@ def _parity_frames(oracle, flags=None, rr=None, tex=None):
             m["MOTION max"] = np.abs(g["MOTION"][hit, :2] - oc[P["MOTION"]][hit, :2]).max()
             m["THROUGHPUT"] = np.mean(np.all(np.abs(g["THROUGHPUT"] - oc[P["THROUGHPUT"]]) < 1e-4, -1))
             m["ALBEDO"] = np.mean(np.all(np.abs(g["ALBEDO"][..., :3] - oc[P["ALBEDO"]][..., :3]) < 2.0 / 255.0, -1))
+            m["NORMAL"] = np.mean(g["NORMAL"][hit, 0] == oc[P["NORMAL"]][hit, 0].astype(np.uint32))
+            m["ROUGHNESS"] = np.mean(np.abs(g["METALLIC_ROUGHNESS"][hit, 1] - oc[P["METALLIC_ROUGHNESS"]][hit, 1]) < 1e-4)
             for k in ("RESERVOIRS_INITIAL", "RESERVOIRS"):
                 m[k] = np.mean(g[k][hit, 0] == oc[P[k]][hit, 0].astype(np.uint32))
             for k in ("UNFILTERED_DIRECT", "UNFILTERED_SPECULAR", "UNFILTERED_INDIR"):
                 e, gg = oc[P[k]][hit, :3], g[k][hit, :3]
                 rr_ = _rel(gg, e, 1e-3 * np.abs(e).mean() + 1e-12).max(axis=1)
                 m[k] = np.mean(rr_ < 1e-3)
+                m[k + " 1e-2"] = np.mean(rr_ < 1e-2)
+                m[k + " p99"] = np.quantile(rr_, 0.99)
                 m[k + " mean"] = gg.mean() / max(e.mean(), 1e-30)
             m["GRADIENT_PREV_PIX"] = np.mean(g["GRADIENT_PREV_PIX"][..., 0] == oc[P["GRADIENT_PREV_PIX"]][..., 0])
             m["gradient max diff"] = np.abs(g["DIS_GRADIENT_HISTORY"][..., :3] - oc[P["DIS_GRADIENT_HISTORY"]][..., :3]).max()
@@ -169,10 +174,13 @@ def test_reflection_refraction_frames_match_the_fp64_oracle(oracle):
 def test_textured_frames_match_the_fp64_oracle(oracle):
     """Planar texture coordinates (x/6, z/6) on every object; the terrain and
     half of the boxes carry a seeded 64x64 texture (bilinear, trilinear between
-    mips), the other boxes an alpha-tested 16x16 texture with transparent 4x4
-    blocks (nearest).  FP32 sampling on the GPU against FP64 in the oracle, and
+    mips) with seeded occlusion-roughness-metallic, normal and emission maps,
+    the other boxes an alpha-tested 16x16 texture with transparent 4x4 blocks
+    (nearest).  FP32 sampling on the GPU against FP64 in the oracle, and
     mip levels that may differ by one sRGB code: albedo within 2/255 on >= 99%
-    of pixels; alpha-tested edges move a few pixels (visibility >= 99%)."""
+    of pixels; alpha-tested edges move a few pixels (visibility >= 99%); UNORM
+    and alpha mips filter the 8-bit codes so both sides round them alike
+    (roughness equal on >= 99.9%)."""
     from raytracedgl1_amd import rtgl as _r
     s = scenes.terrain_boxes_scene()
     rng = np.random.default_rng(11)
@@ -190,7 +198,17 @@ def test_textured_frames_match_the_fp64_oracle(oracle):
             textures[k], flags[k] = "crate", _r.RG_MESH_PRIMITIVE_ALPHA_TESTED
         else:
             textures[k] = "ground"
-    pixels = {"ground": (ground, _r.RG_SAMPLER_FILTER_LINEAR, 0, 0), "crate": (crate, _r.RG_SAMPLER_FILTER_NEAREST, 0, 0)}
+    # the ground material also has occlusion-roughness-metallic, normal and
+    # emission maps (ground_orm, ground_n, ground_e)
+    # roughness >= 0.5 and gently bent normals: a near-mirror lobe would turn
+    # the last-bit differences of FP32 and FP64 normals into large ones
+    orm = rng.integers(128, 256, (32, 32, 4), dtype=np.uint8)
+    nmap = rng.integers(112, 144, (32, 32, 4), dtype=np.uint8)
+    emis = np.zeros((16, 16, 4), np.uint8)
+    emis[4:8, 4:8, :3] = 40
+    pixels = {"ground": (ground, _r.RG_SAMPLER_FILTER_LINEAR, 0, 0), "crate": (crate, _r.RG_SAMPLER_FILTER_NEAREST, 0, 0),
+              "ground_orm": (orm, _r.RG_SAMPLER_FILTER_LINEAR, 0, 0), "ground_n": (nmap, _r.RG_SAMPLER_FILTER_LINEAR, 0, 0),
+              "ground_e": (emis, _r.RG_SAMPLER_FILTER_LINEAR, 0, 0)}
     print("parity: textured and alpha-tested frames vs FP64 oracle, terrain-boxes 160x90", flush=True)
     for f, g, oc, m in _parity_frames(oracle, flags, None, (textures, texcoords, pixels)):
         assert m["IS_SKY"] >= 0.995, f
@@ -198,8 +216,13 @@ def test_textured_frames_match_the_fp64_oracle(oracle):
             assert m[k] >= 0.99, (f, k, m[k])
         for k in ("RESERVOIRS_INITIAL", "RESERVOIRS"):
             assert m[k] >= 0.98, (f, k)
-        for k, need in (("UNFILTERED_DIRECT", 0.975), ("UNFILTERED_SPECULAR", 0.95), ("UNFILTERED_INDIR", 0.9)):
-            assert m[k] >= need, (f, k, m[k])
-            assert abs(m[k + " mean"] - 1.0) < 0.05, (f, k)
+        # the packed normals agree on >= 99.5% (the bent normal's last bits
+        # can cross a 16+16-bit code); lighting: direct within 1e-3 on
+        # >= 99.5%, the bounce-based planes within 1e-2 on >= 97%, means 1%
+        assert m["NORMAL"] >= 0.995 and m["ROUGHNESS"] >= 0.999, f
+        assert m["UNFILTERED_DIRECT"] >= 0.995, (f, m["UNFILTERED_DIRECT"])
+        for k in ("UNFILTERED_DIRECT", "UNFILTERED_SPECULAR", "UNFILTERED_INDIR"):
+            assert m[k + " 1e-2"] >= 0.97, (f, k, m[k + " 1e-2"])
+            assert abs(m[k + " mean"] - 1.0) < 0.01, (f, k)
         assert m["PRE_FINAL"] >= 0.7, (f, m["PRE_FINAL"])
         assert abs(m["PRE_FINAL mean"] - 1.0) < 0.03, f

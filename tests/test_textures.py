@@ -1,5 +1,6 @@
 """Albedo textures and the alpha test of a frame through the C API on the GPU
 against closed forms (the cases and their derivation: texture_cases.py)."""
+import numpy as np
 import pytest
 
 from raytracedgl1_amd import rtgl
@@ -60,3 +61,21 @@ def test_deleted_texture_falls_back_to_the_colour_and_names_resolve_at_upload():
     assert textured.std() > 0.05
     assert abs(plain - 1.0).max() < 1e-6  # white primitive colour, no texture
     assert (again == textured).all()
+
+
+def test_material_maps(oracle):
+    """tex_orm (UNORM) sets roughness and metallic, tex_n bends the shading
+    normal (closed forms: texture_cases.check_orm / check_normal_map)."""
+    scene, uv = tc.quad_scene()
+    r = rtgl.Renderer(rc.W, rc.H)
+    try:
+        r.provide_texture("tex", tc.checker8(), rtgl.RG_SAMPLER_FILTER_NEAREST)
+        r.provide_texture("tex_orm", tc.orm_map(), rtgl.RG_SAMPLER_FILTER_NEAREST)
+        r.provide_texture("tex_n", tc.normal_map(), rtgl.RG_SAMPLER_FILTER_NEAREST)
+        r.frame(rtgl.SceneUploader(scene, {}, {0: "tex"}, uv), rc.camera(60.0))
+        fb = r.framebuffers()
+        mr, nrm = fb.download(PLANE["METALLIC_ROUGHNESS"]), fb.download(PLANE["NORMAL"])
+    finally:
+        r.close()
+    tc.check_orm(mr)
+    tc.check_normal_map(nrm, lambda a: np.array([oracle.decode_normal(int(x)) for x in a.ravel()]).reshape(a.shape[:2] + (3,)))

@@ -164,3 +164,37 @@ def check_alpha_shadow(direct_cb, pos_cb):
     print(f"alpha shadow: kept {direct[under_kept].mean() / lit:.4f}, discarded {direct[under_dropped].mean() / lit:.4f}")
     assert direct[under_kept].mean() / lit < 0.05
     assert direct[under_dropped].mean() / lit > 0.9
+
+
+# ---- material maps: N_orm (UNORM), N_n (UNORM), N_e (sRGB) beside N
+
+def orm_map():
+    """Roughness 128 / 255 and metallic 1 everywhere (read as UNORM: 0.502,
+    not the sRGB-decoded 0.216)."""
+    px = np.zeros((4, 4, 4), np.uint8)
+    px[..., 0], px[..., 1], px[..., 2], px[..., 3] = 255, 128, 255, 255
+    return px
+
+
+def normal_map():
+    """A constant tangent-space normal (255, 128) -> (1, 0.0039)."""
+    px = np.zeros((4, 4, 4), np.uint8)
+    px[..., 0], px[..., 1], px[..., 2], px[..., 3] = 255, 128, 255, 255
+    return px
+
+
+def check_orm(mr_cb):
+    mr = _to_regular(mr_cb)
+    np.testing.assert_allclose(mr[..., 0], 1.0, atol=1e-6)        # metallic = b
+    np.testing.assert_allclose(mr[..., 1], 128 / 255, atol=1e-6)  # roughness = g
+
+
+def check_normal_map(normal_cb, decode):
+    """The quad's texture u grows with +x and v with -y, so the tangent is +x
+    and the bitangent cross(n, t) * w = (0, -1, 0) (w = -1: v runs against
+    cross(n, t)); the bent normal is normalize(t * 1 + b * 0.0039 + n)."""
+    nx, ny = 255 / 255 * 2 - 1, 128 / 255 * 2 - 1
+    want = np.array([nx, -ny, 1.0])
+    want /= np.linalg.norm(want)
+    got = decode(_to_regular(normal_cb))
+    np.testing.assert_allclose(got.reshape(-1, 3), np.broadcast_to(want, (got.size // 3, 3)), atol=2e-4)
