@@ -378,7 +378,7 @@ typedef struct RgExtFrameParams
     float    thinMediaWidth;          /* 0.1                                 */
     float    cameraRayConeSpreadAngle; /* atan(2 tan(fovY / 2) / renderHeight) (VulkanDevice.cpp:399-400) */
     float    normalMapStrength;       /* RgDrawFrameTexturesParams, 1          */
-    float    pad0;
+    float    waterWaveStrength;       /* waterWaveNormalStrength, 1            */
     float    waterColorAndDensity[ 4 ]; /* (0.3, 0.73, 0.63, 0)             */
     float    acidColorAndDensity[ 4 ];  /* (0.0, 0.66, 0.55, 10)            */
     float    worldUp[ 4 ];              /* normalised RgInstanceCreateInfo::worldUp */

@@ -908,7 +908,21 @@ void ReflectRefract( Ctx& c, std::vector< RgExtRay >& rays, std::vector< RgExtHi
                 const double n1 = Ior( p, s.media ), n2 = Ior( p, newMedia );
                 const bool   water = newMedia == MEDIA_WATER || s.media == MEDIA_WATER || newMedia == MEDIA_ACID || s.media == MEDIA_ACID;
                 if( dot( nrm, -s.dir ) < 0.0 ) nrm = -nrm;
-                nrm = normalize( nrm );
+                if( water )
+                {
+                    // getWaterNormal (RaygenPrimary.inl:152-199) over the
+                    // default water texel (127, 127, 255): each octave reads
+                    // (2 * 127/255 - 1, same, 1) = (-1/255, -1/255, 1)
+                    D3 b0, b1;
+                    revisedONB( nrm, &b0, &b1 );
+                    const double k  = double( p.waterWaveStrength ) * ( 0.25 + 0.2 + 0.1 );
+                    const D3     tn = normalize( mk( k * ( 2.0 * 127.0 / 255.0 - 1.0 ), k * ( 2.0 * 127.0 / 255.0 - 1.0 ), 1.0 + k ) );
+                    nrm             = b0 * tn.x + b1 * tn.y + nrm * tn.z;
+                }
+                else
+                {
+                    nrm = normalize( nrm );
+                }
                 D3     origin = pos, t;
                 bool   doSplit = !s.split, doRefr = false;
                 double F       = 1.0;

@@ -455,6 +455,7 @@ RgResult rgExtFrameParamsDefaults( uint32_t width, uint32_t height, RgExtFramePa
         std::memcpy( p->acidColorAndDensity, acid, sizeof( acid ) );
         p->worldUp[ 1 ] = 1.0f;
         p->normalMapStrength = 1.0f; // DrawFrameInfo.h:175
+        p->waterWaveStrength = 1.0f; // DrawFrameInfo.h:138
     } );
 }
 

@@ -145,6 +145,7 @@ void FillFrameParams( CameraState& cs, const RgDrawFrameInfo& di, uint32_t w, ui
     p.indexOfRefractionWater = std::max( 0.0f, rr ? rr->indexOfRefractionWater : 1.33f );
     p.thinMediaWidth         = std::max( 0.0f, rr ? rr->thinMediaWidth : 0.1f );
     p.cameraRayConeSpreadAngle = std::atan( ( 2.0f * std::tan( di.fovYRadians * 0.5f ) ) / float( h ) );
+    p.waterWaveStrength        = rr ? rr->waterWaveNormalStrength : 1.0f;
     const float water[ 3 ] = { 0.3f, 0.73f, 0.63f }, acid[ 3 ] = { 0.0f, 0.66f, 0.55f };
     for( int i = 0; i < 3; i++ )
     {

@@ -967,13 +967,27 @@ __global__ __launch_bounds__( THREADS ) void k_reflrefr_a( RgExtFrameParams p, F
     const float curIor  = IndexOfRefraction( p, curMedia );
     const float newIor  = IndexOfRefraction( p, newMedia );
     const bool  isWater = newMedia == MEDIA_WATER || curMedia == MEDIA_WATER || newMedia == MEDIA_ACID || curMedia == MEDIA_ACID;
-    // getNormal (RaygenPrimary.inl:246-270): face the ray; the animated
-    // water normal needs the water texture and is not applied
+    // getNormal (RaygenPrimary.inl:246-270): face the ray; water and acid
+    // take getWaterNormal (:152-199) with the reference's default water
+    // texture, one texel (127, 127, 255): every octave reads
+    // (-1/255, -1/255, 1), so the normal is the basis of getONB (Random.h:
+    // 160-169) times normalize((0,0,1) + strength * 0.55 * (-1/255, -1/255, 1))
     if( dot( normal, -dir ) < 0.0f )
     {
         normal = -normal;
     }
-    normal = normalize( normal );
+    if( isWater )
+    {
+        v3 b0, b1;
+        revisedONB( normal, &b0, &b1 );
+        const float k  = p.waterWaveStrength * 0.55f;
+        const v3    tn = normalize( mk3( k * ( -1.0f / 255.0f ), k * ( -1.0f / 255.0f ), 1.0f + k ) );
+        normal         = b0 * tn.x + b1 * tn.y + normal * tn.z;
+    }
+    else
+    {
+        normal = normalize( normal );
+    }
 
     v3       origin  = pos;
     bool     wasSplit = ( bits & RR_SPLIT ) != 0u;

@@ -94,7 +94,7 @@ class RgExtFrameParams(C.Structure):
         ("emissionMapBoost", C.c_float),
         ("reflectRefractMaxDepth", C.c_uint32), ("cameraMediaType", C.c_uint32),
         ("indexOfRefractionGlass", C.c_float), ("indexOfRefractionWater", C.c_float), ("thinMediaWidth", C.c_float),
-        ("cameraRayConeSpreadAngle", C.c_float), ("normalMapStrength", C.c_float), ("pad0", C.c_float), ("waterColorAndDensity", C.c_float * 4), ("acidColorAndDensity", C.c_float * 4),
+        ("cameraRayConeSpreadAngle", C.c_float), ("normalMapStrength", C.c_float), ("waterWaveStrength", C.c_float), ("waterColorAndDensity", C.c_float * 4), ("acidColorAndDensity", C.c_float * 4),
         ("worldUp", C.c_float * 4),
     ]
 

@@ -107,7 +107,7 @@ def camera(fovy_deg):
     return scenes.Camera(np.array([0.0, 0.0, 0.0]), np.array([0.0, 0.0, -1.0]), fovy=math.radians(fovy_deg))
 
 
-def frame_params(cam, max_depth=2, camera_media=0, ior_glass=1.52, ior_water=1.33, water=WATER):
+def frame_params(cam, max_depth=2, camera_media=0, ior_glass=1.52, ior_water=1.33, water=WATER, wave=0.0):
     """RgExtFrameParams of a static camera as FillFrameParams fills it
     (Source/VulkanDevice.cpp:142-482; reflection / refraction :306-339)."""
     p = ffi.frame_params(W, H)
@@ -124,13 +124,16 @@ def frame_params(cam, max_depth=2, camera_media=0, ior_glass=1.52, ior_water=1.3
     p.reflectRefractMaxDepth, p.cameraMediaType = max_depth, camera_media
     p.indexOfRefractionGlass, p.indexOfRefractionWater = ior_glass, ior_water
     p.waterColorAndDensity[:3] = water
+    p.waterWaveStrength = wave
     return p
 
 
-def draw_params(max_depth=2, camera_media=0, ior_glass=1.52, ior_water=1.33, water=WATER):
-    """The same settings as RgDrawFrameReflectRefractParams."""
+def draw_params(max_depth=2, camera_media=0, ior_glass=1.52, ior_water=1.33, water=WATER, wave=0.0):
+    """The same settings as RgDrawFrameReflectRefractParams.  The cases
+    switch the water-normal perturbation off (wave 0) unless they test it."""
     return rtgl.reflect_refract_params(maxReflectRefractDepth=max_depth, typeOfMediaAroundCamera=camera_media,
-                                       indexOfRefractionGlass=ior_glass, indexOfRefractionWater=ior_water, waterColor=water)
+                                       indexOfRefractionGlass=ior_glass, indexOfRefractionWater=ior_water, waterColor=water,
+                                       waterWaveNormalStrength=wave)
 
 
 # ---- the cases: (flags per object, settings, fovy) and the expected planes
@@ -211,6 +214,38 @@ def check_under_water(out):
            tol_depth=5e-4, tol_thr=1e-3)
 
 
+def duff_onb(n):
+    """Orthonormal basis of Duff et al., "Building an Orthonormal Basis,
+    Revisited" (JCGT 2017), listing 3."""
+    sign = math.copysign(1.0, n[2])
+    a = -1.0 / (sign + n[2])
+    b = n[0] * n[1] * a
+    return np.array([1 + sign * n[0] * n[0] * a, sign * b, -sign * n[0]]), np.array([b, sign + n[1] * n[1] * a, -n[1]])
+
+
+def check_water_tilt(out):
+    """Water at strength 1 with the default water texel (127, 127, 255):
+    each of the three octaves reads t = (-1/255, -1/255, 1), so the normal is
+    basis(n) . normalize((0, 0, 1) + 0.55 t) with the Duff et al. basis of the
+    plane normal n = (0, 0, 1); even pixels reflect about it."""
+    d = _camera_dirs(out["params"])
+    az = np.abs(d[..., 2])
+    t = np.array([-1 / 255, -1 / 255, 1.0]) * 0.55 + np.array([0, 0, 1.0])
+    t /= np.linalg.norm(t)
+    b0, b1 = duff_onb(np.array([0.0, 0.0, 1.0]))
+    nt = b0 * t[0] + b1 * t[1] + np.array([0.0, 0.0, 1.0]) * t[2]
+    assert 1e-3 < math.acos(min(1.0, nt[2])) < 3e-3  # a tilt of about 0.1 degrees
+    r = d - 2 * (d @ nt)[..., None] * nt
+    F = (1 - np.abs(d @ nt)) ** 5
+    odd = _odd()
+    # even: reflected about the tilted normal, F-weighted, 0.005 along r, to z = +5
+    _check(out, ~odd, _albedo(COLORS[BACK]), 5.0 / az + 10.0 / r[..., 2] - 0.005, np.repeat((2 * F)[..., None], 3, -1), 1.0, r,
+           tol_depth=2e-4, tol_thr=2e-3)
+    # odd: index 1 passes straight through; only F changes
+    trans = np.power(np.array(WATER)[None, None, :], (5.0 / az)[..., None])
+    _check(out, odd, _albedo(COLORS[FAR]), 10.0 / az, (2 * (1 - F))[..., None] * trans, 1.0, d, tol_thr=2e-4)
+
+
 MIRROR, GLASS, WATER_F = rtgl.RG_MESH_PRIMITIVE_MIRROR, rtgl.RG_MESH_PRIMITIVE_GLASS, rtgl.RG_MESH_PRIMITIVE_WATER
 CASES = {
     "mirror": ({Q: MIRROR}, {}, 60.0, check_mirror),
@@ -221,6 +256,7 @@ CASES = {
     "glass_ior1": ({Q: GLASS}, {"ior_glass": 1.0}, 60.0, check_glass_index_one),
     "water_ior1": ({Q: WATER_F}, {"ior_water": 1.0}, 60.0, check_water),
     "under_water": ({Q: WATER_F}, {"camera_media": rtgl.RG_MEDIA_TYPE_WATER}, 90.0, check_under_water),
+    "water_tilt": ({Q: WATER_F}, {"ior_water": 1.0, "wave": 1.0}, 60.0, check_water_tilt),
 }
 
 
