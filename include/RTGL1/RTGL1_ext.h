@@ -286,6 +286,10 @@ typedef enum RgExtPlane
     RG_EXT_PLANE_INDIR_RESERVOIR_SAMPLE, /* C u32x4 (position bits xyz, encoded normal), PREV */
     RG_EXT_PLANE_INDIR_RESERVOIR_WEIGHT, /* C u32x4 (E5 radiance, target pdf, weight sum, M), PREV */
     RG_EXT_PLANE_FINAL,                  /* R f32x4 tone-mapped LDR colour in [0,1]            */
+    /* framebufScreenEmisRT / framebufAcidFogRT (RaygenPrimary.inl:126-127,
+     * :326-328, :588-589): emission seen on screen and acid fog */
+    RG_EXT_PLANE_SCREEN_EMISSION,        /* R f32x4                                           */
+    RG_EXT_PLANE_ACID_FOG,               /* R f32x4                                           */
     RG_EXT_PLANE_COUNT
 } RgExtPlane;
 
@@ -382,6 +386,8 @@ typedef struct RgExtFrameParams
     float    waterColorAndDensity[ 4 ]; /* (0.3, 0.73, 0.63, 0)             */
     float    acidColorAndDensity[ 4 ];  /* (0.0, 0.66, 0.55, 10)            */
     float    worldUp[ 4 ];              /* normalised RgInstanceCreateInfo::worldUp */
+    float    emissionMaxScreenColor;    /* RgDrawFrameTexturesParams, 1.5     */
+    float    pad1[ 3 ];
 } RgExtFrameParams;
 
 /* Fills an RgExtFrameParams with the reference defaults
@@ -418,6 +424,12 @@ RGAPI RgResult RGCONV rgExtDenoise( RgExtFramebuffers fb, const RgExtFrameParams
  * ImageComposition.cpp:377-394). tonemapping may be null (defaults). */
 RGAPI RgResult RGCONV rgExtComposeFinal( RgExtFramebuffers framebuffers, const RgDrawFrameTonemappingParams* tonemapping,
                                          const float* d_avgLuminance, void* stream );
+/* The same with the screen emission weight of RgDrawFrameTexturesParams
+ * (emissionMaxScreenColor; textures may be null: 1.5):
+ * FINAL = LPM(PRE_FINAL * exposure + SCREEN_EMISSION * emissionMaxScreenColor)
+ * + ACID_FOG, the three resolved alike on split pixels. */
+RGAPI RgResult RGCONV rgExtComposeFinalEx( RgExtFramebuffers framebuffers, const RgDrawFrameTonemappingParams* tonemapping,
+                                           const RgDrawFrameTexturesParams* textures, const float* d_avgLuminance, void* stream );
 
 
 /* ------------------------------------------------------ instance extras */

@@ -137,14 +137,18 @@ extern "C" void or_lpm_map( const float saturation[ 3 ], const float crosstalk[ 
     Map( Setup( saturation, crosstalk ), in, out );
 }
 
-extern "C" void or_compose_final( const OrFramebuffers* f, const float saturation[ 3 ], const float crosstalk[ 3 ], double avgLuminance )
+extern "C" void or_compose_final( const OrFramebuffers* f, const float saturation[ 3 ], const float crosstalk[ 3 ], double avgLuminance,
+                                  double emissionMaxScreenColor )
 {
     const Lpm k = Setup( saturation, crosstalk );
     const int W = f->width, H = f->height, sep = W / 2;
     const double* pre = f->cur[ RG_EXT_PLANE_PRE_FINAL ];
     const double* thr = f->cur[ RG_EXT_PLANE_THROUGHPUT ];
     double*       fin = f->cur[ RG_EXT_PLANE_FINAL ];
-    auto at = [ & ]( int x, int y, int c ) { return ( x < 0 || y < 0 || x >= W || y >= H ) ? 0.0 : pre[ ( size_t( y ) * W + x ) * 4 + c ]; };
+    const double* emi = f->cur[ RG_EXT_PLANE_SCREEN_EMISSION ];
+    const double* fgp = f->cur[ RG_EXT_PLANE_ACID_FOG ];
+    auto atp = [ & ]( const double* pl, int x, int y, int c ) { return ( x < 0 || y < 0 || x >= W || y >= H ) ? 0.0 : pl[ ( size_t( y ) * W + x ) * 4 + c ]; };
+    auto at  = [ & ]( int x, int y, int c ) { return atp( pre, x, y, c ); };
     // ev100ToLuminousExposure(getAutoEV100()) (Exposure.h:29-56)
     const double lum      = std::max( 0.0, avgLuminance );
     const double maxLum   = 1.2 * std::exp2( std::log2( lum * 100.0 / 12.5 ) );
@@ -155,20 +159,29 @@ extern "C" void or_compose_final( const OrFramebuffers* f, const float saturatio
         {
             int cx, cy;
             or_checkerboard_from_regular( x, y, W, &cx, &cy );
-            double hdr[ 3 ] = { at( x, y, 0 ), at( x, y, 1 ), at( x, y, 2 ) };
+            const double* pl[ 3 ] = { pre, emi, fgp };
+            double        v[ 3 ][ 3 ];
+            for( int q = 0; q < 3; q++ )
+                for( int c = 0; c < 3; c++ ) v[ q ][ c ] = atp( pl[ q ], x, y, c );
             if( thr[ ( size_t( cy ) * W + cx ) * 4 + 3 ] > 0.5 )
             {
                 const double l = ( cx == 0 || cx == sep ) ? 0.0 : 1.0, r = ( cx == sep || cx == W ) ? 0.0 : 1.0;
-                for( int c = 0; c < 3; c++ )
+                for( int q = 0; q < 3; q++ )
                 {
-                    const double cross = 0.25 * at( x - 1, y, c ) * l + 0.25 * at( x + 1, y, c ) * r + 0.25 * at( x, y + 1, c ) + 0.25 * at( x, y - 1, c );
-                    hdr[ c ]           = 0.5 * hdr[ c ] + 0.5 * cross;
+                    for( int c = 0; c < 3; c++ )
+                    {
+                        const double cross = 0.25 * atp( pl[ q ], x - 1, y, c ) * l + 0.25 * atp( pl[ q ], x + 1, y, c ) * r +
+                                             0.25 * atp( pl[ q ], x, y + 1, c ) + 0.25 * atp( pl[ q ], x, y - 1, c );
+                        v[ q ][ c ]        = 0.5 * v[ q ][ c ] + 0.5 * cross;
+                    }
                 }
             }
-            double in[ 3 ] = { hdr[ 0 ] * exposure, hdr[ 1 ] * exposure, hdr[ 2 ] * exposure }, out[ 3 ];
+            // finalizeColor (CmPrepareFinal.comp:109-121), then the acid fog (:162-164)
+            double in[ 3 ], out[ 3 ];
+            for( int c = 0; c < 3; c++ ) in[ c ] = v[ 0 ][ c ] * exposure + v[ 1 ][ c ] * emissionMaxScreenColor;
             Map( k, in, out );
             double* o = fin + ( size_t( y ) * W + x ) * 4;
-            o[ 0 ] = out[ 0 ], o[ 1 ] = out[ 1 ], o[ 2 ] = out[ 2 ], o[ 3 ] = 0.0;
+            o[ 0 ] = out[ 0 ] + v[ 2 ][ 0 ], o[ 1 ] = out[ 1 ] + v[ 2 ][ 1 ], o[ 2 ] = out[ 2 ] + v[ 2 ][ 2 ], o[ 3 ] = 0.0;
         }
     }
 }

@@ -14,7 +14,10 @@ void or_lpm_luma_coefficients( double out[ 3 ] );
 /* One colour through the LPM 709->709 map. */
 void or_lpm_map( const float saturation[ 3 ], const float crosstalk[ 3 ], const double in[ 3 ], double out[ 3 ] );
 /* PRE_FINAL -> FINAL of fb (checkerboard resolve, auto exposure, LPM). */
-void or_compose_final( const OrFramebuffers* fb, const float saturation[ 3 ], const float crosstalk[ 3 ], double avgLuminance );
+/* FINAL = LPM(PRE_FINAL * exposure + SCREEN_EMISSION * emissionMaxScreenColor)
+ * + ACID_FOG, the three resolved alike on split pixels. */
+void or_compose_final( const OrFramebuffers* fb, const float saturation[ 3 ], const float crosstalk[ 3 ], double avgLuminance,
+                       double emissionMaxScreenColor );
 
 #ifdef __cplusplus
 }

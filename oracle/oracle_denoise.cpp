@@ -214,7 +214,9 @@ extern "C" int or_plane_layout( int plane )
         case RG_EXT_PLANE_ALBEDO:
         case RG_EXT_PLANE_HISTOGRAM_INPUT:
         case RG_EXT_PLANE_PRE_FINAL:
-        case RG_EXT_PLANE_FINAL: return 1;
+        case RG_EXT_PLANE_FINAL:
+        case RG_EXT_PLANE_SCREEN_EMISSION:
+        case RG_EXT_PLANE_ACID_FOG: return 1;
         case RG_EXT_PLANE_DIS_PING_GRADIENT:
         case RG_EXT_PLANE_DIS_PONG_GRADIENT:
         case RG_EXT_PLANE_DIS_GRADIENT_HISTORY:

@@ -815,6 +815,14 @@ D3 Transmittance( const RgExtFrameParams& p, uint32_t m, double dist )
     return mk( std::exp( dist * std::log( double( col[ 0 ] ) ) * k ), std::exp( dist * std::log( double( col[ 1 ] ) ) * k ),
                std::exp( dist * std::log( double( col[ 2 ] ) ) * k ) );
 }
+// getGlowingMediaFog (Media.h:63-77)
+D3 GlowingFog( const RgExtFrameParams& p, uint32_t m, double dist )
+{
+    if( m != MEDIA_ACID ) return mk( 0, 0, 0 );
+    const double fog = clampd( 1.0 - std::exp( -dist * 0.00005 * double( p.acidColorAndDensity[ 3 ] ) ), 0.0, 1.0 );
+    return mk( p.acidColorAndDensity[ 0 ] * fog, p.acidColorAndDensity[ 1 ] * fog, p.acidColorAndDensity[ 2 ] * fog );
+}
+
 // Snell's law in vector form; false on total internal reflection
 bool Refract( double n1, double n2, D3 I, D3 N, D3* T )
 {
@@ -830,6 +838,8 @@ struct RrPix
     D3       thr, dir, vpos;
     double   fpl;
     double   cone, coneLast; // ray-cone width now and at the last hit (RayCone.h:25-36)
+    D3       emis, fog;      // screen emission and acid fog of the loop
+    bool     doSplit;
     uint32_t media;
     bool     active, split;
 };
@@ -879,6 +889,9 @@ void ReflectRefract( Ctx& c, std::vector< RgExtRay >& rays, std::vector< RgExtHi
                     s.fpl   = fb.get( RG_EXT_PLANE_DEPTH_WORLD, cx, cy, 0 );
                     s.vpos  = pos;
                     s.cone  = double( p.cameraRayConeSpreadAngle ) * s.fpl; // width 0 propagated over the primary hit
+                    s.emis  = mk( fb.get( RG_EXT_PLANE_SCREEN_EMISSION, x, y, 0 ), fb.get( RG_EXT_PLANE_SCREEN_EMISSION, x, y, 1 ),
+                                  fb.get( RG_EXT_PLANE_SCREEN_EMISSION, x, y, 2 ) );
+                    s.fog   = mk( fb.get( RG_EXT_PLANE_ACID_FOG, x, y, 0 ), fb.get( RG_EXT_PLANE_ACID_FOG, x, y, 1 ), fb.get( RG_EXT_PLANE_ACID_FOG, x, y, 2 ) );
                     s.media = p.cameraMediaType & 3u;
                     s.split = false;
                 }
@@ -967,6 +980,7 @@ void ReflectRefract( Ctx& c, std::vector< RgExtRay >& rays, std::vector< RgExtHi
                         origin = origin + s.dir * 0.005; // the reference's SHIPPING_HACK offset
                     }
                 }
+                s.doSplit = doSplit;
                 if( doSplit )
                 {
                     s.thr   = s.thr * 2.0;
@@ -1009,6 +1023,8 @@ void ReflectRefract( Ctx& c, std::vector< RgExtRay >& rays, std::vector< RgExtHi
                     fb.set4( RG_EXT_PLANE_VISIBILITY_BUFFER, cx, cy, 4294967295.0, 4294967295.0, 4294967295.0, 4294967295.0 );
                     fb.set4( RG_EXT_PLANE_VIEW_DIRECTION, cx, cy, s.dir.x, s.dir.y, s.dir.z, 0 );
                     fb.set4( RG_EXT_PLANE_THROUGHPUT, cx, cy, s.thr.x, s.thr.y, s.thr.z, thrW );
+                    fb.set4( RG_EXT_PLANE_SCREEN_EMISSION, x, y, 0, 0, 0, 0 );
+                    fb.set4( RG_EXT_PLANE_ACID_FOG, x, y, 0, 0, 0, 0 );
                     s.active = false;
                     continue;
                 }
@@ -1031,6 +1047,11 @@ void ReflectRefract( Ctx& c, std::vector< RgExtRay >& rays, std::vector< RgExtHi
                 const D3 cPrev = MulCM( p.projectionPrev, vPrev, wp, &cwp );
                 s.thr          = s.thr * Transmittance( p, s.media, len );
                 s.fpl += len;
+                s.emis         = s.emis + h.emission * s.thr;
+                s.fog          = s.fog + GlowingFog( p, s.media, len ) * ( s.doSplit ? 2.0 : 1.0 );
+                const D3 semi  = p.cameraMediaType != MEDIA_ACID ? s.emis + s.fog * 0.05 : s.emis;
+                fb.set4( RG_EXT_PLANE_SCREEN_EMISSION, x, y, semi.x, semi.y, semi.z, 0 );
+                fb.set4( RG_EXT_PLANE_ACID_FOG, x, y, s.fog.x, s.fog.y, s.fog.z, 0 );
                 fb.set4( RG_EXT_PLANE_IS_SKY, cx, cy, 0 );
                 fb.set4( RG_EXT_PLANE_ALBEDO, x, y, h.albedo.x, h.albedo.y, h.albedo.z, 0 );
                 fb.set4( RG_EXT_PLANE_NORMAL, cx, cy, EncodeNormal( h.normal ) );
@@ -1378,6 +1399,8 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
                 fb.set4( RG_EXT_PLANE_VIEW_DIRECTION, cx, cy, dir.x, dir.y, dir.z, 0 );
                 fb.set4( RG_EXT_PLANE_DEPTH_GRAD, cx, cy, 0 );
                 fb.set4( RG_EXT_PLANE_THROUGHPUT, cx, cy, 1, 1, 1, 0 );
+                fb.set4( RG_EXT_PLANE_SCREEN_EMISSION, x, y, 0, 0, 0, 0 );
+                fb.set4( RG_EXT_PLANE_ACID_FOG, x, y, 0, 0, 0, 0 );
                 continue;
             }
             const float*  t  = sc->tris + size_t( hit.primitiveId ) * 9;
@@ -1432,7 +1455,13 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
             fb.set4( RG_EXT_PLANE_SURFACE_POSITION, cx, cy, h.position.x, h.position.y, h.position.z, 0 );
             fb.set4( RG_EXT_PLANE_VISIBILITY_BUFFER, cx, cy, 1, double( ( h.localPrim << 12 ) | ( h.geom & 0xFFFu ) ), hit.u, hit.v );
             fb.set4( RG_EXT_PLANE_VIEW_DIRECTION, cx, cy, dir.x, dir.y, dir.z, 0 );
-            fb.set4( RG_EXT_PLANE_THROUGHPUT, cx, cy, 1, 1, 1, 0 );
+            // the camera's medium attenuates the primary segment; screen
+            // emission and acid fog (RaygenPrimary.inl:321-328)
+            const D3 thr = Transmittance( p, p.cameraMediaType & 3u, depth );
+            const D3 fog = GlowingFog( p, p.cameraMediaType & 3u, depth );
+            fb.set4( RG_EXT_PLANE_THROUGHPUT, cx, cy, thr.x, thr.y, thr.z, 0 );
+            fb.set4( RG_EXT_PLANE_SCREEN_EMISSION, x, y, h.emission.x * thr.x, h.emission.y * thr.y, h.emission.z * thr.z, 0 );
+            fb.set4( RG_EXT_PLANE_ACID_FOG, x, y, fog.x, fog.y, fog.z, 0 );
         }
     } );
 

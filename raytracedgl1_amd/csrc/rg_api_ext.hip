@@ -1,5 +1,6 @@
 // extern "C" entry points of RTGL1_ext.h: device memory helpers, timers, the
 // LBVH and traversal stages, and eye adaptation.
+#include <algorithm>
 #include "rg_bvh.h"
 #include "rg_denoise.h"
 #include "rg_internal.h"
@@ -337,7 +338,8 @@ RgResult rgExtLuminanceHistogram( const float* d_luminance, uint32_t width, uint
     } );
 }
 
-RgResult rgExtComposeFinal( RgExtFramebuffers fbs, const RgDrawFrameTonemappingParams* tonemapping, const float* d_avgLuminance, void* stream )
+RgResult rgExtComposeFinalEx( RgExtFramebuffers fbs, const RgDrawFrameTonemappingParams* tonemapping, const RgDrawFrameTexturesParams* textures,
+                              const float* d_avgLuminance, void* stream )
 {
     return Guard( [ & ] {
         RG_REQUIRE( fbs && d_avgLuminance, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
@@ -347,8 +349,14 @@ RgResult rgExtComposeFinal( RgExtFramebuffers fbs, const RgDrawFrameTonemappingP
             std::memcpy( sat, tonemapping->saturation.data, sizeof( sat ) );
             std::memcpy( ct, tonemapping->crosstalk.data, sizeof( ct ) );
         }
-        ComposeFinal( fbs->fb, MakeLpmConsts( sat, ct ), d_avgLuminance, AsStream( stream ) );
+        const float emis = textures ? std::max( textures->emissionMaxScreenColor, 0.0f ) : 1.5f; // VulkanDevice.cpp:278
+        ComposeFinal( fbs->fb, MakeLpmConsts( sat, ct ), emis, d_avgLuminance, AsStream( stream ) );
     } );
+}
+
+RgResult rgExtComposeFinal( RgExtFramebuffers fbs, const RgDrawFrameTonemappingParams* tonemapping, const float* d_avgLuminance, void* stream )
+{
+    return rgExtComposeFinalEx( fbs, tonemapping, nullptr, d_avgLuminance, stream );
 }
 
 RgResult rgExtLuminanceAverage( uint32_t* d_histogram, uint32_t pixelCount, float minLogLuminance, float maxLogLuminance, float timeDelta, RgBool32 stopEyeAdaptation, float* d_avgLuminance, void* stream )
@@ -456,6 +464,7 @@ RgResult rgExtFrameParamsDefaults( uint32_t width, uint32_t height, RgExtFramePa
         p->worldUp[ 1 ] = 1.0f;
         p->normalMapStrength = 1.0f; // DrawFrameInfo.h:175
         p->waterWaveStrength = 1.0f; // DrawFrameInfo.h:138
+        p->emissionMaxScreenColor = 1.5f; // DrawFrameInfo.h:177
     } );
 }
 

@@ -123,6 +123,7 @@ void FillFrameParams( CameraState& cs, const RgDrawFrameInfo& di, uint32_t w, ui
     p.minRoughness             = tx ? std::clamp( tx->minRoughness, 0.0f, 1.0f ) : 0.0f;
     p.emissionMapBoost         = tx ? std::max( tx->emissionMapBoost, 0.0f ) : 100.0f;
     p.normalMapStrength        = tx ? tx->normalMapStrength : 1.0f;
+    p.emissionMaxScreenColor   = tx ? std::max( tx->emissionMaxScreenColor, 0.0f ) : 1.5f;
 
     p.rayCullBackFaces  = cullBackFaces ? 1u : 0u;
     p.rayLength         = std::clamp( di.rayLength, 0.1f, 10000.0f );
@@ -222,7 +223,7 @@ uint64_t EncodeFrame( FrameGpu& g, TraceContext& tc, const RgExtFrameParams& p, 
         std::memcpy( sat, tm->saturation.data, sizeof( sat ) );
         std::memcpy( ct, tm->crosstalk.data, sizeof( ct ) );
     }
-    ComposeFinal( fb, MakeLpmConsts( sat, ct ), g.avgLum.as< float >(), s );
+    ComposeFinal( fb, MakeLpmConsts( sat, ct ), p.emissionMaxScreenColor, g.avgLum.as< float >(), s );
     return rays;
 }
 

@@ -49,6 +49,8 @@ const PlaneDesc g_planes[ RG_EXT_PLANE_COUNT ] = {
     { LAYOUT_CHECKERBOARD, 4, true },  // INDIR_RESERVOIR_SAMPLE
     { LAYOUT_CHECKERBOARD, 4, true },  // INDIR_RESERVOIR_WEIGHT
     { LAYOUT_REGULAR, 4, false },      // FINAL
+    { LAYOUT_REGULAR, 4, false },      // SCREEN_EMISSION
+    { LAYOUT_REGULAR, 4, false },      // ACID_FOG
 };
 } // namespace
 

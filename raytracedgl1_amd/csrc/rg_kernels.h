@@ -24,6 +24,7 @@ void LuminanceHistogram( const float* d_lum, uint32_t count, float minLog, float
 void LuminanceAverage( uint32_t* d_hist, uint32_t pixelCount, float minLog, float maxLog, float timeDelta, bool stop, float* d_avg, hipStream_t stream );
 LpmConsts MakeLpmConsts( const float saturation[ 3 ], const float crosstalk[ 3 ] );
 // PRE_FINAL -> FINAL: checkerboard resolve, auto exposure from *d_avgLum, LPM tone map
-void ComposeFinal( FramebufferSet& fb, const LpmConsts& k, const float* d_avgLum, hipStream_t stream );
+// emissionMaxScreenColor weights SCREEN_EMISSION into the tone map's input.
+void ComposeFinal( FramebufferSet& fb, const LpmConsts& k, float emissionMaxScreenColor, const float* d_avgLum, hipStream_t stream );
 
 } // namespace rg

@@ -656,127 +656,8 @@ __device__ __forceinline__ v3 RayTriBary( v3 p0, v3 p1, v3 p2, v3 o, v3 d )
     return mk3( 1.0f - u - v, u, v );
 }
 
-// storeSky (RaygenPrimary.inl:90-139) for a colour sky; the motion of an
-// infinitely far point uses the checkerboard pixel, as :74-76 and :118 do.
-// The primary pass also clears DEPTH_GRAD; reflections keep it.
-__device__ void StoreSky( const RgExtFrameParams& p, const FbPtrs& P, size_t ci, size_t ri, int cx, int cy, v3 dir, v3 thr, float thrW, bool primary )
-{
-    const int   W  = p.renderWidth, H = p.renderHeight;
-    const float cu = ( float( cx ) + 0.5f + p.jitterX ) / float( W );
-    const float cv = ( float( cy ) + 0.5f + p.jitterY ) / float( H );
-    const v3    rd = GetRayDir( p, cu, cv );
-    float       w;
-    const v3    vc = MulM4( p.view, rd, 0.0f, &w ), vp = MulM4( p.viewPrev, rd, 0.0f, &w );
-    // mat3(projection) * v: the upper-left 3x3 block, no division by w
-    const v3 cc = mk3( p.projection[ 0 ] * vc.x + p.projection[ 4 ] * vc.y + p.projection[ 8 ] * vc.z,
-                       p.projection[ 1 ] * vc.x + p.projection[ 5 ] * vc.y + p.projection[ 9 ] * vc.z, 0.0f );
-    const v3 cp = mk3( p.projectionPrev[ 0 ] * vp.x + p.projectionPrev[ 4 ] * vp.y + p.projectionPrev[ 8 ] * vp.z,
-                       p.projectionPrev[ 1 ] * vp.x + p.projectionPrev[ 5 ] * vp.y + p.projectionPrev[ 9 ] * vp.z, 0.0f );
-    const float mx  = ( cp.x * 0.5f + 0.5f ) - ( cc.x * 0.5f + 0.5f );
-    const float my  = ( cp.y * 0.5f + 0.5f ) - ( cc.y * 0.5f + 0.5f );
-    const v3    sky = SkyColor( p );
-    Plane< uint32_t >( P, RG_EXT_PLANE_IS_SKY )[ ci ]           = 1u;
-    Plane< float4 >( P, RG_EXT_PLANE_ALBEDO )[ ri ]             = make_float4( sky.x, sky.y, sky.z, 0.0f );
-    Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL )[ ci ]           = EncodeNormal( mk3( 0, 0, 0 ) );
-    Plane< float2 >( P, RG_EXT_PLANE_METALLIC_ROUGHNESS )[ ci ] = make_float2( 0.f, 0.f );
-    Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD )[ ci ]         = MAX_RAY_LEN * 2.0f;
-    Plane< float4 >( P, RG_EXT_PLANE_MOTION )[ ci ]             = make_float4( mx, my, 0.f, 0.f );
-    Plane< float4 >( P, RG_EXT_PLANE_SURFACE_POSITION )[ ci ]   = make_float4( 1e7f, 1e7f, 1e7f, 1e7f );
-    Plane< float4 >( P, RG_EXT_PLANE_VISIBILITY_BUFFER )[ ci ]  = make_float4( 4294967295.0f, 4294967295.0f, 4294967295.0f, 4294967295.0f );
-    Plane< float4 >( P, RG_EXT_PLANE_VIEW_DIRECTION )[ ci ]     = make_float4( dir.x, dir.y, dir.z, 0.f );
-    Plane< float4 >( P, RG_EXT_PLANE_THROUGHPUT )[ ci ]         = make_float4( thr.x, thr.y, thr.z, thrW );
-    if( primary )
-    {
-        Plane< float >( P, RG_EXT_PLANE_DEPTH_GRAD )[ ci ] = 0.0f;
-    }
-}
-
-// G-buffer of the primary hit (RaygenPrimary.inl:280-348, storeSky :90-139).
-__global__ __launch_bounds__( THREADS ) void k_primary_shade( RgExtFrameParams p, const RgExtHit* hits, SceneView sc, FbPtrs P )
-{
-    const int x = blockIdx.x * BX + threadIdx.x % BX, y = blockIdx.y * BY + threadIdx.x / BX;
-    const int W = p.renderWidth, H = p.renderHeight;
-    if( x >= W || y >= H )
-    {
-        return;
-    }
-    int cx, cy;
-    cbFromRegular( x, y, W, &cx, &cy );
-    const size_t   ci  = size_t( cy ) * W + cx;
-    const size_t   ri  = size_t( y ) * W + x;
-    const RgExtHit hit = hits[ ri ];
-    const float    u   = ( float( x ) + 0.5f + p.jitterX ) / float( W );
-    const float    v   = ( float( y ) + 0.5f + p.jitterY ) / float( H );
-    const v3       dir = GetRayDir( p, u, v );
-    const v3       org = mk3( p.cameraPosition[ 0 ], p.cameraPosition[ 1 ], p.cameraPosition[ 2 ] );
-
-    if( hit.primitiveId == RG_EXT_MISS )
-    {
-        StoreSky( p, P, ci, ri, cx, cy, dir, mk3( 1.f, 1.f, 1.f ), 0.0f, true );
-        return;
-    }
-
-    const float*      t  = sc.tris + size_t( hit.primitiveId ) * 9;
-    const v3 p0 = mk3( t[ 0 ], t[ 1 ], t[ 2 ] ), p1 = mk3( t[ 3 ], t[ 4 ], t[ 5 ] ), p2 = mk3( t[ 6 ], t[ 7 ], t[ 8 ] );
-
-    // ray differentials on the same triangle (HitInfo.inl:243-268); their
-    // barycentric deltas give the texture gradients
-    const v3 dAX = GetRayDir( p, u + 1.0f / float( W ), v );
-    const v3 dAY = GetRayDir( p, u, v + 1.0f / float( H ) );
-    const v3 bAX = RayTriBary( p0, p1, p2, org, dAX );
-    const v3 bAY = RayTriBary( p0, p1, p2, org, dAY );
-    TexQuery tq;
-    tq.mode = TQ_GRAD;
-    tq.dbx  = bAX - mk3( 1.0f - hit.u - hit.v, hit.u, hit.v );
-    tq.dby  = bAY - mk3( 1.0f - hit.u - hit.v, hit.u, hit.v );
-    const HitInfo     h  = GetHit( sc, p, hit.primitiveId, hit.u, hit.v, org, tq );
-    const GeomRecord& gr = sc.geoms[ h.geom ];
-    const v3 prevPos = gr.isStatic ? h.position : Xform34( gr.prevFromCur, h.position );
-
-    float    wc, wp, wx, wy;
-    const v3 vCur  = MulM4( p.view, h.position, 1.0f, &wc );
-    const v3 vPrev = MulM4( p.viewPrev, prevPos, 1.0f, &wp );
-    const v3 vAX   = MulM4( p.view, p0 * bAX.x + p1 * bAX.y + p2 * bAX.z, 1.0f, &wx );
-    const v3 vAY   = MulM4( p.view, p0 * bAY.x + p1 * bAY.y + p2 * bAY.z, 1.0f, &wy );
-    float    cwc, cwp;
-    const v3 cCur  = MulM4( p.projection, vCur, wc, &cwc );
-    const v3 cPrev = MulM4( p.projectionPrev, vPrev, wp, &cwp );
-    // dot(projection[2], v): the third column of the matrix, as the reference writes it
-    const float* col2 = p.projection + 8;
-    const float  dAXc = col2[ 0 ] * vAX.x + col2[ 1 ] * vAX.y + col2[ 2 ] * vAX.z + col2[ 3 ] * wx;
-    const float  dAYc = col2[ 0 ] * vAY.x + col2[ 1 ] * vAY.y + col2[ 2 ] * vAY.z + col2[ 3 ] * wy;
-    const float  gx = dAXc - cCur.z, gy = dAYc - cCur.z;
-
-    const float sxc = ( cCur.x / cwc ) * 0.5f + 0.5f, syc = ( cCur.y / cwc ) * 0.5f + 0.5f;
-    const float sxp = ( cPrev.x / cwp ) * 0.5f + 0.5f, syp = ( cPrev.y / cwp ) * 0.5f + 0.5f;
-    const float depthLinear = length( vCur );
-    const float motionZ     = length( vPrev ) - depthLinear;
-
-    Plane< uint32_t >( P, RG_EXT_PLANE_IS_SKY )[ ci ]           = 0u;
-    Plane< float4 >( P, RG_EXT_PLANE_ALBEDO )[ ri ]             = make_float4( h.albedo.x, h.albedo.y, h.albedo.z, 0.0f );
-    Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL )[ ci ]           = EncodeNormal( h.normal );
-    Plane< float2 >( P, RG_EXT_PLANE_METALLIC_ROUGHNESS )[ ci ] = make_float2( h.metallic, h.roughness );
-    Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD )[ ci ]         = depthLinear;
-    Plane< float >( P, RG_EXT_PLANE_DEPTH_GRAD )[ ci ]          = sqrtf( gx * gx + gy * gy );
-    Plane< float4 >( P, RG_EXT_PLANE_MOTION )[ ci ]             = make_float4( sxp - sxc, syp - syc, motionZ, 0.0f );
-    Plane< float4 >( P, RG_EXT_PLANE_SURFACE_POSITION )[ ci ]   = make_float4( h.position.x, h.position.y, h.position.z, __uint_as_float( 1u ) );
-    Plane< uint4 >( P, RG_EXT_PLANE_VISIBILITY_BUFFER )[ ci ]   = make_uint4( 1u, ( h.localPrim << 12 ) | ( h.geom & 0xFFFu ), __float_as_uint( hit.u ), __float_as_uint( hit.v ) );
-    Plane< float4 >( P, RG_EXT_PLANE_VIEW_DIRECTION )[ ci ]     = make_float4( dir.x, dir.y, dir.z, 0.0f );
-    Plane< float4 >( P, RG_EXT_PLANE_THROUGHPUT )[ ci ]         = make_float4( 1.f, 1.f, 1.f, 0.f );
-}
-
-// ------------------------------------------- reflection / refraction chain
-// RAYGEN_REFL_REFR_SHADER (RaygenPrimary.inl:352-599) as a wavefront: each
-// depth is k_reflrefr_a (decide, build the ray), one closest-hit launch, then
-// k_reflrefr_b (store the hit or the sky into the G-buffer).  The loop's
-// local state lives in three float4 per pixel:
-//   S[0] = throughput.xyz, bits (1: still tracing, 2: wasSplit, media << 2)
-//   S[1] = ray direction.xyz, full path length
-//   S[2] = virtual position for motion.xyz
-//   S[3] = .x: the ray-cone width the last hit was shaded with
-// Portals, animated water normals, screen emission and acid fog are not
-// part of this path (they need the water texture or portal instances).
-constexpr uint32_t RR_ACTIVE = 1u, RR_SPLIT = 2u;
+// Media of the reflection / refraction chain and of the camera.
+constexpr uint32_t RR_ACTIVE = 1u, RR_SPLIT = 2u, RR_DOSPLIT = 32u;
 constexpr uint32_t MEDIA_VACUUM = 0u, MEDIA_WATER = 1u, MEDIA_GLASS = 2u, MEDIA_ACID = 3u;
 
 // getMediaTypeFromFlags / isRefractFromFlags / isReflectFromFlags
@@ -871,6 +752,148 @@ __device__ __forceinline__ v3 FresnelSpec( float nl, v3 f0 )
     return f0 + ( mk3( 1.0f, 1.0f, 1.0f ) - f0 ) * t;
 }
 
+// getGlowingMediaFog (Media.h:63-77): acid glows with distance
+__device__ v3 GlowingMediaFog( const RgExtFrameParams& p, uint32_t media, float distance )
+{
+    if( media != MEDIA_ACID )
+    {
+        return mk3( 0.0f, 0.0f, 0.0f );
+    }
+    const float density = 0.00005f * p.acidColorAndDensity[ 3 ];
+    const float fog     = clampf( 1.0f - expf( -distance * density ), 0.0f, 1.0f );
+    return mk3( p.acidColorAndDensity[ 0 ], p.acidColorAndDensity[ 1 ], p.acidColorAndDensity[ 2 ] ) * fog;
+}
+
+// storeSky (RaygenPrimary.inl:90-139) for a colour sky; the motion of an
+// infinitely far point uses the checkerboard pixel, as :74-76 and :118 do.
+// The primary pass also clears DEPTH_GRAD; reflections keep it.
+__device__ void StoreSky( const RgExtFrameParams& p, const FbPtrs& P, size_t ci, size_t ri, int cx, int cy, v3 dir, v3 thr, float thrW, bool primary )
+{
+    const int   W  = p.renderWidth, H = p.renderHeight;
+    const float cu = ( float( cx ) + 0.5f + p.jitterX ) / float( W );
+    const float cv = ( float( cy ) + 0.5f + p.jitterY ) / float( H );
+    const v3    rd = GetRayDir( p, cu, cv );
+    float       w;
+    const v3    vc = MulM4( p.view, rd, 0.0f, &w ), vp = MulM4( p.viewPrev, rd, 0.0f, &w );
+    // mat3(projection) * v: the upper-left 3x3 block, no division by w
+    const v3 cc = mk3( p.projection[ 0 ] * vc.x + p.projection[ 4 ] * vc.y + p.projection[ 8 ] * vc.z,
+                       p.projection[ 1 ] * vc.x + p.projection[ 5 ] * vc.y + p.projection[ 9 ] * vc.z, 0.0f );
+    const v3 cp = mk3( p.projectionPrev[ 0 ] * vp.x + p.projectionPrev[ 4 ] * vp.y + p.projectionPrev[ 8 ] * vp.z,
+                       p.projectionPrev[ 1 ] * vp.x + p.projectionPrev[ 5 ] * vp.y + p.projectionPrev[ 9 ] * vp.z, 0.0f );
+    const float mx  = ( cp.x * 0.5f + 0.5f ) - ( cc.x * 0.5f + 0.5f );
+    const float my  = ( cp.y * 0.5f + 0.5f ) - ( cc.y * 0.5f + 0.5f );
+    const v3    sky = SkyColor( p );
+    Plane< uint32_t >( P, RG_EXT_PLANE_IS_SKY )[ ci ]           = 1u;
+    Plane< float4 >( P, RG_EXT_PLANE_ALBEDO )[ ri ]             = make_float4( sky.x, sky.y, sky.z, 0.0f );
+    Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL )[ ci ]           = EncodeNormal( mk3( 0, 0, 0 ) );
+    Plane< float2 >( P, RG_EXT_PLANE_METALLIC_ROUGHNESS )[ ci ] = make_float2( 0.f, 0.f );
+    Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD )[ ci ]         = MAX_RAY_LEN * 2.0f;
+    Plane< float4 >( P, RG_EXT_PLANE_MOTION )[ ci ]             = make_float4( mx, my, 0.f, 0.f );
+    Plane< float4 >( P, RG_EXT_PLANE_SURFACE_POSITION )[ ci ]   = make_float4( 1e7f, 1e7f, 1e7f, 1e7f );
+    Plane< float4 >( P, RG_EXT_PLANE_VISIBILITY_BUFFER )[ ci ]  = make_float4( 4294967295.0f, 4294967295.0f, 4294967295.0f, 4294967295.0f );
+    Plane< float4 >( P, RG_EXT_PLANE_VIEW_DIRECTION )[ ci ]     = make_float4( dir.x, dir.y, dir.z, 0.f );
+    Plane< float4 >( P, RG_EXT_PLANE_THROUGHPUT )[ ci ]         = make_float4( thr.x, thr.y, thr.z, thrW );
+    Plane< float4 >( P, RG_EXT_PLANE_SCREEN_EMISSION )[ ri ]    = make_float4( 0.f, 0.f, 0.f, 0.f );
+    Plane< float4 >( P, RG_EXT_PLANE_ACID_FOG )[ ri ]           = make_float4( 0.f, 0.f, 0.f, 0.f );
+    if( primary )
+    {
+        Plane< float >( P, RG_EXT_PLANE_DEPTH_GRAD )[ ci ] = 0.0f;
+    }
+}
+
+// G-buffer of the primary hit (RaygenPrimary.inl:280-348, storeSky :90-139).
+__global__ __launch_bounds__( THREADS ) void k_primary_shade( RgExtFrameParams p, const RgExtHit* hits, SceneView sc, FbPtrs P )
+{
+    const int x = blockIdx.x * BX + threadIdx.x % BX, y = blockIdx.y * BY + threadIdx.x / BX;
+    const int W = p.renderWidth, H = p.renderHeight;
+    if( x >= W || y >= H )
+    {
+        return;
+    }
+    int cx, cy;
+    cbFromRegular( x, y, W, &cx, &cy );
+    const size_t   ci  = size_t( cy ) * W + cx;
+    const size_t   ri  = size_t( y ) * W + x;
+    const RgExtHit hit = hits[ ri ];
+    const float    u   = ( float( x ) + 0.5f + p.jitterX ) / float( W );
+    const float    v   = ( float( y ) + 0.5f + p.jitterY ) / float( H );
+    const v3       dir = GetRayDir( p, u, v );
+    const v3       org = mk3( p.cameraPosition[ 0 ], p.cameraPosition[ 1 ], p.cameraPosition[ 2 ] );
+
+    if( hit.primitiveId == RG_EXT_MISS )
+    {
+        StoreSky( p, P, ci, ri, cx, cy, dir, mk3( 1.f, 1.f, 1.f ), 0.0f, true );
+        return;
+    }
+
+    const float*      t  = sc.tris + size_t( hit.primitiveId ) * 9;
+    const v3 p0 = mk3( t[ 0 ], t[ 1 ], t[ 2 ] ), p1 = mk3( t[ 3 ], t[ 4 ], t[ 5 ] ), p2 = mk3( t[ 6 ], t[ 7 ], t[ 8 ] );
+
+    // ray differentials on the same triangle (HitInfo.inl:243-268); their
+    // barycentric deltas give the texture gradients
+    const v3 dAX = GetRayDir( p, u + 1.0f / float( W ), v );
+    const v3 dAY = GetRayDir( p, u, v + 1.0f / float( H ) );
+    const v3 bAX = RayTriBary( p0, p1, p2, org, dAX );
+    const v3 bAY = RayTriBary( p0, p1, p2, org, dAY );
+    TexQuery tq;
+    tq.mode = TQ_GRAD;
+    tq.dbx  = bAX - mk3( 1.0f - hit.u - hit.v, hit.u, hit.v );
+    tq.dby  = bAY - mk3( 1.0f - hit.u - hit.v, hit.u, hit.v );
+    const HitInfo     h  = GetHit( sc, p, hit.primitiveId, hit.u, hit.v, org, tq );
+    const GeomRecord& gr = sc.geoms[ h.geom ];
+    const v3 prevPos = gr.isStatic ? h.position : Xform34( gr.prevFromCur, h.position );
+
+    float    wc, wp, wx, wy;
+    const v3 vCur  = MulM4( p.view, h.position, 1.0f, &wc );
+    const v3 vPrev = MulM4( p.viewPrev, prevPos, 1.0f, &wp );
+    const v3 vAX   = MulM4( p.view, p0 * bAX.x + p1 * bAX.y + p2 * bAX.z, 1.0f, &wx );
+    const v3 vAY   = MulM4( p.view, p0 * bAY.x + p1 * bAY.y + p2 * bAY.z, 1.0f, &wy );
+    float    cwc, cwp;
+    const v3 cCur  = MulM4( p.projection, vCur, wc, &cwc );
+    const v3 cPrev = MulM4( p.projectionPrev, vPrev, wp, &cwp );
+    // dot(projection[2], v): the third column of the matrix, as the reference writes it
+    const float* col2 = p.projection + 8;
+    const float  dAXc = col2[ 0 ] * vAX.x + col2[ 1 ] * vAX.y + col2[ 2 ] * vAX.z + col2[ 3 ] * wx;
+    const float  dAYc = col2[ 0 ] * vAY.x + col2[ 1 ] * vAY.y + col2[ 2 ] * vAY.z + col2[ 3 ] * wy;
+    const float  gx = dAXc - cCur.z, gy = dAYc - cCur.z;
+
+    const float sxc = ( cCur.x / cwc ) * 0.5f + 0.5f, syc = ( cCur.y / cwc ) * 0.5f + 0.5f;
+    const float sxp = ( cPrev.x / cwp ) * 0.5f + 0.5f, syp = ( cPrev.y / cwp ) * 0.5f + 0.5f;
+    const float depthLinear = length( vCur );
+    const float motionZ     = length( vPrev ) - depthLinear;
+
+    Plane< uint32_t >( P, RG_EXT_PLANE_IS_SKY )[ ci ]           = 0u;
+    Plane< float4 >( P, RG_EXT_PLANE_ALBEDO )[ ri ]             = make_float4( h.albedo.x, h.albedo.y, h.albedo.z, 0.0f );
+    Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL )[ ci ]           = EncodeNormal( h.normal );
+    Plane< float2 >( P, RG_EXT_PLANE_METALLIC_ROUGHNESS )[ ci ] = make_float2( h.metallic, h.roughness );
+    Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD )[ ci ]         = depthLinear;
+    Plane< float >( P, RG_EXT_PLANE_DEPTH_GRAD )[ ci ]          = sqrtf( gx * gx + gy * gy );
+    Plane< float4 >( P, RG_EXT_PLANE_MOTION )[ ci ]             = make_float4( sxp - sxc, syp - syc, motionZ, 0.0f );
+    Plane< float4 >( P, RG_EXT_PLANE_SURFACE_POSITION )[ ci ]   = make_float4( h.position.x, h.position.y, h.position.z, __uint_as_float( 1u ) );
+    Plane< uint4 >( P, RG_EXT_PLANE_VISIBILITY_BUFFER )[ ci ]   = make_uint4( 1u, ( h.localPrim << 12 ) | ( h.geom & 0xFFFu ), __float_as_uint( hit.u ), __float_as_uint( hit.v ) );
+    Plane< float4 >( P, RG_EXT_PLANE_VIEW_DIRECTION )[ ci ]     = make_float4( dir.x, dir.y, dir.z, 0.0f );
+    // the camera's medium attenuates the primary segment (RaygenPrimary.inl:321-328)
+    const v3 thr  = MediaTransmittance( p, p.cameraMediaType & 3u, depthLinear );
+    const v3 emis = h.emission * thr;
+    const v3 fog  = GlowingMediaFog( p, p.cameraMediaType & 3u, depthLinear );
+    Plane< float4 >( P, RG_EXT_PLANE_THROUGHPUT )[ ci ]         = make_float4( thr.x, thr.y, thr.z, 0.f );
+    Plane< float4 >( P, RG_EXT_PLANE_SCREEN_EMISSION )[ ri ]    = make_float4( emis.x, emis.y, emis.z, 0.f );
+    Plane< float4 >( P, RG_EXT_PLANE_ACID_FOG )[ ri ]           = make_float4( fog.x, fog.y, fog.z, 0.f );
+}
+
+// ------------------------------------------- reflection / refraction chain
+// RAYGEN_REFL_REFR_SHADER (RaygenPrimary.inl:352-599) as a wavefront: each
+// depth is k_reflrefr_a (decide, build the ray), one closest-hit launch, then
+// k_reflrefr_b (store the hit or the sky into the G-buffer).  The loop's
+// local state lives in three float4 per pixel:
+//   S[0] = throughput.xyz, bits (1: still tracing, 2: wasSplit, media << 2)
+//   S[1] = ray direction.xyz, full path length
+//   S[2] = virtual position for motion.xyz
+//   S[3] = .x: the ray-cone width the last hit was shaded with
+//   S[4] = screen emission.xyz, S[5] = acid fog.xyz (framebufScreenEmisRT,
+//          framebufAcidFogRT of the loop)
+// Portals, animated water normals, screen emission and acid fog are not
+// part of this path (they need the water texture or portal instances).
 struct RrState
 {
     float4* s; // 3 x W*H float4
@@ -924,6 +947,8 @@ __global__ __launch_bounds__( THREADS ) void k_reflrefr_a( RgExtFrameParams p, F
         // ray cone (RaygenPrimary.inl:396-406): width 0, then propagated
         // over the primary hit's distance
         st.s[ 2 * st.n + ri ] = make_float4( pos.x, pos.y, pos.z, p.cameraRayConeSpreadAngle * fpl );
+        st.s[ 4 * st.n + ri ] = Plane< float4 >( P, RG_EXT_PLANE_SCREEN_EMISSION )[ ri ];
+        st.s[ 5 * st.n + ri ] = Plane< float4 >( P, RG_EXT_PLANE_ACID_FOG )[ ri ];
     }
     else
     {
@@ -1044,7 +1069,7 @@ __global__ __launch_bounds__( THREADS ) void k_reflrefr_a( RgExtFrameParams p, F
         thr      = thr * 2.0f;
         wasSplit = true;
     }
-    bits = RR_ACTIVE | ( wasSplit ? RR_SPLIT : 0u ) | ( ( media & 3u ) << 2 );
+    bits = RR_ACTIVE | ( wasSplit ? RR_SPLIT : 0u ) | ( doSplit ? RR_DOSPLIT : 0u ) | ( ( media & 3u ) << 2 );
     st.s[ ri ]        = make_float4( thr.x, thr.y, thr.z, __uint_as_float( bits ) );
     st.s[ st.n + ri ] = make_float4( dir.x, dir.y, dir.z, fpl );
     // traceReflectionRefractionRay (RaygenCommon.h:235-251)
@@ -1116,6 +1141,15 @@ __global__ __launch_bounds__( THREADS ) void k_reflrefr_b( RgExtFrameParams p, F
 
     thr             = thr * MediaTransmittance( p, media, rayLen );
     const float fpl = s1.w + rayLen;
+    // screen emission and acid fog along the chain (RaygenPrimary.inl:575-589)
+    const float4 e4   = st.s[ 4 * st.n + ri ], f4 = st.s[ 5 * st.n + ri ];
+    const v3     emis = mk3( e4.x, e4.y, e4.z ) + h.emission * thr;
+    const v3     fog  = mk3( f4.x, f4.y, f4.z ) + GlowingMediaFog( p, media, rayLen ) * ( ( bits & RR_DOSPLIT ) ? 2.0f : 1.0f );
+    const v3     semi = p.cameraMediaType != MEDIA_ACID ? emis + fog * 0.05f : emis;
+    st.s[ 4 * st.n + ri ] = make_float4( emis.x, emis.y, emis.z, 0.0f );
+    st.s[ 5 * st.n + ri ] = make_float4( fog.x, fog.y, fog.z, 0.0f );
+    Plane< float4 >( P, RG_EXT_PLANE_SCREEN_EMISSION )[ ri ] = make_float4( semi.x, semi.y, semi.z, 0.0f );
+    Plane< float4 >( P, RG_EXT_PLANE_ACID_FOG )[ ri ]        = make_float4( fog.x, fog.y, fog.z, 0.0f );
 
     Plane< uint32_t >( P, RG_EXT_PLANE_IS_SKY )[ ci ]           = 0u;
     Plane< float4 >( P, RG_EXT_PLANE_ALBEDO )[ ri ]             = make_float4( h.albedo.x, h.albedo.y, h.albedo.z, 0.0f );

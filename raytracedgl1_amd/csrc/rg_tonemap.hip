@@ -137,10 +137,9 @@ __device__ __forceinline__ float4 LdReg( const float4* p, int W, int H, int x, i
     return ( x < 0 || y < 0 || x >= W || y >= H ) ? make_float4( 0.f, 0.f, 0.f, 0.f ) : p[ size_t( y ) * W + x ];
 }
 
-// CmCheckerboard.comp:49-76 then CmPrepareFinal.comp:148-176 (no volumetrics,
-// screen emission or acid fog; classic shading is off at the default
-// lightmapScreenCoverage of 0)
-__global__ __launch_bounds__( CBX * CBY ) void k_compose( FbPtrs P, int W, int H, LpmConsts k, const float* avgLum )
+// CmCheckerboard.comp:49-76 then CmPrepareFinal.comp:148-176 (no volumetrics;
+// classic shading is off at the default lightmapScreenCoverage of 0)
+__global__ __launch_bounds__( CBX * CBY ) void k_compose( FbPtrs P, int W, int H, LpmConsts k, float emisWeight, const float* avgLum )
 {
     const int x = blockIdx.x * CBX + threadIdx.x % CBX, y = blockIdx.y * CBY + threadIdx.x / CBX;
     if( x >= W || y >= H )
@@ -152,22 +151,32 @@ __global__ __launch_bounds__( CBX * CBY ) void k_compose( FbPtrs P, int W, int H
     int           cx, cy;
     cbFromRegular( x, y, W, &cx, &cy );
     const float4 thr = reinterpret_cast< const float4* >( P.cur[ RG_EXT_PLANE_THROUGHPUT ] )[ size_t( cy ) * W + cx ];
-    v3           hdr = mk3( c4.x, c4.y, c4.z );
+    const float4* em  = reinterpret_cast< const float4* >( P.cur[ RG_EXT_PLANE_SCREEN_EMISSION ] );
+    const float4* fg  = reinterpret_cast< const float4* >( P.cur[ RG_EXT_PLANE_ACID_FOG ] );
+    const float4  e4  = em[ size_t( y ) * W + x ], f4 = fg[ size_t( y ) * W + x ];
+    v3           hdr = mk3( c4.x, c4.y, c4.z ), emis = mk3( e4.x, e4.y, e4.z ), fog = mk3( f4.x, f4.y, f4.z );
     if( thr.w > 0.5f ) // wasSplit: a refracted / reflected pixel is resolved with its cross
     {
-        const int   sep  = cbSeparator( W );
-        const float l    = ( cx == 0 || cx == sep ) ? 0.0f : 1.0f;
-        const float r    = ( cx == sep || cx == W ) ? 0.0f : 1.0f;
-        const float4 a = LdReg( pre, W, H, x - 1, y ), b = LdReg( pre, W, H, x + 1, y ), u = LdReg( pre, W, H, x, y + 1 ), d = LdReg( pre, W, H, x, y - 1 );
-        const v3    cross = mk3( a.x, a.y, a.z ) * ( 0.25f * l ) + mk3( b.x, b.y, b.z ) * ( 0.25f * r ) + mk3( u.x, u.y, u.z ) * 0.25f + mk3( d.x, d.y, d.z ) * 0.25f;
-        hdr               = mix3( hdr, cross, 0.5f );
+        const int   sep = cbSeparator( W );
+        const float l   = ( cx == 0 || cx == sep ) ? 0.0f : 1.0f;
+        const float r   = ( cx == sep || cx == W ) ? 0.0f : 1.0f;
+        auto resolve = [ & ]( const float4* src, v3 center ) {
+            const float4 a = LdReg( src, W, H, x - 1, y ), b = LdReg( src, W, H, x + 1, y ), u = LdReg( src, W, H, x, y + 1 ), d = LdReg( src, W, H, x, y - 1 );
+            const v3 cross = mk3( a.x, a.y, a.z ) * ( 0.25f * l ) + mk3( b.x, b.y, b.z ) * ( 0.25f * r ) + mk3( u.x, u.y, u.z ) * 0.25f + mk3( d.x, d.y, d.z ) * 0.25f;
+            return mix3( center, cross, 0.5f );
+        };
+        hdr  = resolve( pre, hdr );
+        emis = resolve( em, emis );
+        fog  = resolve( fg, fog );
     }
     // ev100ToLuminousExposure(getAutoEV100()) (Exposure.h:29-56)
     const float lum      = fmaxf( 0.0f, *avgLum );
     const float ev100    = log2f( lum * 100.0f / 12.5f );
     const float maxLum   = 1.2f * exp2f( ev100 );
     const float exposure = maxLum > 0.0f ? 1.0f / maxLum : 0.0f;
-    const v3    out      = LpmMap709( hdr * exposure, k );
+    // finalizeColor: hdr * exposure + screen emission * emissionMaxScreenColor,
+    // tone mapped; then the acid fog (CmPrepareFinal.comp:109-121, :162-164)
+    const v3    out      = LpmMap709( hdr * exposure + emis * emisWeight, k ) + fog;
     reinterpret_cast< float4* >( P.cur[ RG_EXT_PLANE_FINAL ] )[ size_t( y ) * W + x ] = make_float4( out.x, out.y, out.z, 0.0f );
 }
 
@@ -238,10 +247,10 @@ LpmConsts MakeLpmConsts( const float saturation[ 3 ], const float crosstalk[ 3 ]
     return k;
 }
 
-void ComposeFinal( FramebufferSet& fb, const LpmConsts& k, const float* d_avgLum, hipStream_t stream )
+void ComposeFinal( FramebufferSet& fb, const LpmConsts& k, float emissionMaxScreenColor, const float* d_avgLum, hipStream_t stream )
 {
     const dim3 grid( DivUp( fb.width, CBX ), DivUp( fb.height, CBY ) );
-    k_compose<<< grid, CBX * CBY, 0, stream >>>( fb.Ptrs(), int( fb.width ), int( fb.height ), k, d_avgLum );
+    k_compose<<< grid, CBX * CBY, 0, stream >>>( fb.Ptrs(), int( fb.width ), int( fb.height ), k, emissionMaxScreenColor, d_avgLum );
     CheckLaunch( "k_compose" );
 }
 

@@ -95,7 +95,7 @@ class RgExtFrameParams(C.Structure):
         ("reflectRefractMaxDepth", C.c_uint32), ("cameraMediaType", C.c_uint32),
         ("indexOfRefractionGlass", C.c_float), ("indexOfRefractionWater", C.c_float), ("thinMediaWidth", C.c_float),
         ("cameraRayConeSpreadAngle", C.c_float), ("normalMapStrength", C.c_float), ("waterWaveStrength", C.c_float), ("waterColorAndDensity", C.c_float * 4), ("acidColorAndDensity", C.c_float * 4),
-        ("worldUp", C.c_float * 4),
+        ("worldUp", C.c_float * 4), ("emissionMaxScreenColor", C.c_float), ("pad1", C.c_float * 3),
     ]
 
 
@@ -143,6 +143,7 @@ _SIGS = {
     "rgExtLuminanceHistogram": (_RES, [_P, _U32, _U32, _F, _F, _P, _P]),
     "rgExtLuminanceAverage": (_RES, [_P, _U32, _F, _F, _F, _U32, _P, _P]),
     "rgExtComposeFinal": (_RES, [_P, _P, _P, _P]),
+    "rgExtComposeFinalEx": (_RES, [_P, _P, _P, _P, _P]),
     "rgExtFramebuffersCreate": (_RES, [_U32, _U32, C.POINTER(_P)]),
     "rgExtFramebuffersCreateWithStorage": (_RES, [_U32, _U32, _U32, C.POINTER(_P)]),
     "rgExtFramebuffersDestroy": (_RES, [_P]),
@@ -403,6 +404,8 @@ PLANES = [
     ("INDIR_RESERVOIR_SAMPLE", 0, 4, np.uint32, True),
     ("INDIR_RESERVOIR_WEIGHT", 0, 4, np.uint32, True),
     ("FINAL", 1, 4, np.float32, False),
+    ("SCREEN_EMISSION", 1, 4, np.float32, False),
+    ("ACID_FOG", 1, 4, np.float32, False),
 ]
 PLANE = {name: i for i, (name, *_rest) in enumerate(PLANES)}
 

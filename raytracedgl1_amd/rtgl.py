@@ -108,6 +108,14 @@ RG_SAMPLER_ADDRESS_MODE_REPEAT, RG_SAMPLER_ADDRESS_MODE_CLAMP = 0, 1
 RG_MESH_PRIMITIVE_ALPHA_TESTED = 1
 
 
+class RgDrawFrameTexturesParams(C.Structure):
+    _fields_ = [("sType", C.c_int), ("pNext", _P), ("dynamicSamplerFilter", C.c_int), ("normalMapStrength", _F),
+                ("emissionMapBoost", _F), ("emissionMaxScreenColor", _F), ("minRoughness", _F)]
+
+
+RG_STRUCTURE_TYPE_TEXTURES = 8
+
+
 class RgDrawFrameReflectRefractParams(C.Structure):
     _fields_ = [("sType", C.c_int), ("pNext", _P), ("maxReflectRefractDepth", _U32), ("typeOfMediaAroundCamera", C.c_int),
                 ("indexOfRefractionGlass", _F), ("indexOfRefractionWater", _F), ("thinMediaWidth", _F),
@@ -262,13 +270,15 @@ class SceneUploader:
     """Prebuilt per-object vertex arrays of a scenes.Scene, uploaded each frame
     with rgUploadMeshPrimitive (non-indexed triangle lists, identity transform)."""
 
-    def __init__(self, scene, flags: dict | None = None, textures: dict | None = None, texcoords: dict | None = None):
+    def __init__(self, scene, flags: dict | None = None, textures: dict | None = None, texcoords: dict | None = None,
+                 emissive: dict | None = None):
         """flags: object index -> RgMeshPrimitiveFlags; textures: object
         index -> pTextureName; texcoords: object index -> (tris * 3, 2)
         float32 vertex texture coordinates."""
         self.scene = scene
         self.flags = dict(flags or {})  # object index -> RgMeshPrimitiveFlags
         self.textures = {k: v.encode() for k, v in (textures or {}).items()}
+        self.emissive = dict(emissive or {})  # object index -> RgMeshPrimitiveInfo::emissive
         self.objects = []
         for oid, (name, first, count) in enumerate(scene.objects):
             v = np.zeros(count * 3, VERTEX_DTYPE)
@@ -309,7 +319,7 @@ class SceneUploader:
             self.prim.pIndices = None
             self.prim.indexCount = 0
             self.prim.color = color | 0xFF000000
-            self.prim.emissive = 0.0
+            self.prim.emissive = float(self.emissive.get(oid, 0.0))
             self.prim.flags = self.flags.get(oid, 0)
             self.prim.pTextureName = self.textures.get(oid)
             ffi.check(prefix + "UploadMeshPrimitive", mesh_fn(inst, C.byref(self.mesh), C.byref(self.prim)))

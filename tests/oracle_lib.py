@@ -60,7 +60,7 @@ class Oracle:
             "or_denoise_gradient_atrous": (None, [_P, _P, _U32]),
             "or_lpm_luma_coefficients": (None, [_P]),
             "or_lpm_map": (None, [_P, _P, _P, _P]),
-            "or_compose_final": (None, [_P, _P, _P, C.c_double]),
+            "or_compose_final": (None, [_P, _P, _P, C.c_double, C.c_double]),
             "or_pt_create": (_P, [C.c_int, C.c_int]),
             "or_pt_destroy": (None, [_P]),
             "or_pt_frame": (None, [_P, _P, _P, _P, _P, C.c_int]),
@@ -192,7 +192,7 @@ class OracleBvh:
         return hits
 
 
-PLANE_COUNT = 42
+PLANE_COUNT = 44
 
 
 class OrFramebuffersC(C.Structure):
@@ -300,7 +300,8 @@ class OraclePathTracer:
     roughness 1, metallic 0."""
 
     def __init__(self, o: Oracle, scene, width: int, height: int, leaf: int = 3, flags: dict | None = None,
-                 textures: dict | None = None, texcoords: dict | None = None, texture_pixels: dict | None = None):
+                 textures: dict | None = None, texcoords: dict | None = None, texture_pixels: dict | None = None,
+                 emissive: dict | None = None):
         """flags, textures (object index -> texture name) and texcoords
         (object index -> (tris * 3, 2) floats) as rtgl.SceneUploader takes
         them; texture_pixels: name -> (rgba8, filter, address u, address v)
@@ -333,7 +334,7 @@ class OraclePathTracer:
             for k in range(12):
                 gr.prevFromCur[k] = 1.0 if k % 5 == 0 else 0.0
             gr.isStatic, gr.color = 1, int(scene.colors[first]) | 0xFF000000
-            gr.emissive, gr.roughness, gr.metallic = 0.0, 1.0, 0.0
+            gr.emissive, gr.roughness, gr.metallic = min(max(float((emissive or {}).get(oid, 0.0)), 0.0), 1.0), 1.0, 0.0
             gr.flags, gr.firstTri, gr.triCount = flags.get(oid, 0), pos, count
             base = textures.get(oid)
             if base:

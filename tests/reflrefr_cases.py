@@ -99,7 +99,9 @@ def _check(out, mask, albedo, depth, thr, thr_w, view_dir, tol_depth=2e-4, tol_t
     np.testing.assert_allclose(vd[mask][:, :3], view_dir[mask], atol=2e-4)
 
 
-NAMES = ("ALBEDO", "DEPTH_WORLD", "THROUGHPUT", "VIEW_DIRECTION", "IS_SKY", "SURFACE_POSITION", "MOTION")
+NAMES = ("ALBEDO", "DEPTH_WORLD", "THROUGHPUT", "VIEW_DIRECTION", "IS_SKY", "SURFACE_POSITION", "MOTION", "SCREEN_EMISSION",
+         "ACID_FOG")
+ACID = (0.0, 0.66, 0.55)
 WATER = (0.3, 0.73, 0.63)
 
 
@@ -198,9 +200,11 @@ def check_under_water(out):
     margin = 0.01  # keep away from the critical angle, where rounding may pick either branch
     tir = sin_i * n1 / n2 > 1 + margin
     assert tir.sum() > 100 and (~tir).sum() > 100
+    # the camera's water attenuates the primary segment too (RaygenPrimary.inl:321-322)
+    prim = np.power(np.array(WATER)[None, None, :], (5.0 / az)[..., None])
     # total internal reflection: F = 1, no split, back through water to z = +5
     L = 10.0 / az - 0.005
-    trans = np.power(np.array(WATER)[None, None, :], L[..., None])
+    trans = prim * np.power(np.array(WATER)[None, None, :], L[..., None])
     _check(out, tir, _albedo(COLORS[BACK]), 5.0 / az + L, trans, -1.0, d * np.array([1, 1, -1]), tol_thr=2e-3)
     # refraction out of the water (odd pixels well inside the critical angle)
     ok = odd & (sin_i * n1 / n2 < 1 - margin)
@@ -210,7 +214,7 @@ def check_under_water(out):
     t_dir = np.concatenate([lateral * sin_t[..., None], -cos_t[..., None]], -1)
     r0 = ((n1 - n2) / (n1 + n2)) ** 2
     F = r0 + (1 - r0) * (1 - az) ** 5
-    _check(out, ok, _albedo(COLORS[FAR]), 5.0 / az + 5.0 / cos_t, np.repeat((2 * (1 - F))[..., None], 3, -1), 1.0, t_dir,
+    _check(out, ok, _albedo(COLORS[FAR]), 5.0 / az + 5.0 / cos_t, (2 * (1 - F))[..., None] * prim, 1.0, t_dir,
            tol_depth=5e-4, tol_thr=1e-3)
 
 
@@ -246,6 +250,34 @@ def check_water_tilt(out):
     _check(out, odd, _albedo(COLORS[FAR]), 10.0 / az, (2 * (1 - F))[..., None] * trans, 1.0, d, tol_thr=2e-4)
 
 
+def check_acid_fog(out):
+    """Acid of index 1 (acid takes the water index): odd pixels refract into
+    it, are attenuated by colour^(distance * max(1, sqrt(density))) and
+    gather the glowing fog (1 - exp(-distance * 0.00005 * density)) * colour,
+    doubled on the split (Media.h:43-77); the screen emission carries 5% of
+    the fog.  Even pixels reflect in vacuum: no fog."""
+    d = _camera_dirs(out["params"])
+    az = np.abs(d[..., 2])
+    odd = _odd()
+    F = (1 - az) ** 5
+    dist = 5.0 / az
+    trans = np.power(np.array(ACID)[None, None, :], (dist * math.sqrt(10.0))[..., None])
+    _check(out, odd, _albedo(COLORS[FAR]), 10.0 / az, (2 * (1 - F))[..., None] * trans, 1.0, d)
+    fog = (1 - np.exp(-dist * 0.00005 * 10.0))[..., None] * np.array(ACID) * 2.0
+    np.testing.assert_allclose(out["ACID_FOG"][..., :3][odd], fog[odd], rtol=1e-4, atol=1e-9)
+    np.testing.assert_allclose(out["SCREEN_EMISSION"][..., :3][odd], 0.05 * fog[odd], rtol=1e-4, atol=1e-9)
+    assert np.abs(out["ACID_FOG"][..., :3][~odd]).max() == 0.0
+
+
+def check_emission_in_mirror(out):
+    """The wall behind the camera emits (emissive 1: albedo x 1); a mirror
+    shows it weighted by the mirror's Fresnel term (RaygenPrimary.inl:575)."""
+    d = _camera_dirs(out["params"])
+    az = np.abs(d[..., 2])
+    want = _albedo(COLORS[BACK])[None, None, :] * _schlick_spec(az)[..., None]
+    np.testing.assert_allclose(out["SCREEN_EMISSION"][..., :3], want, rtol=2e-4, atol=1e-7)
+
+
 MIRROR, GLASS, WATER_F = rtgl.RG_MESH_PRIMITIVE_MIRROR, rtgl.RG_MESH_PRIMITIVE_GLASS, rtgl.RG_MESH_PRIMITIVE_WATER
 CASES = {
     "mirror": ({Q: MIRROR}, {}, 60.0, check_mirror),
@@ -257,6 +289,8 @@ CASES = {
     "water_ior1": ({Q: WATER_F}, {"ior_water": 1.0}, 60.0, check_water),
     "under_water": ({Q: WATER_F}, {"camera_media": rtgl.RG_MEDIA_TYPE_WATER}, 90.0, check_under_water),
     "water_tilt": ({Q: WATER_F}, {"ior_water": 1.0, "wave": 1.0}, 60.0, check_water_tilt),
+    "acid_fog": ({Q: rtgl.RG_MESH_PRIMITIVE_ACID}, {"ior_water": 1.0}, 60.0, check_acid_fog),
+    "emission_in_mirror": ({Q: MIRROR}, {}, 60.0, check_emission_in_mirror, {BACK: 1.0}),
 }
 
 

@@ -101,13 +101,32 @@ def test_crosstalk_moves_clipped_energy(oracle):
     assert out0[2] < out[2] and out0[1] > out[1]
 
 
-def _compose_oracle(o, pre, thr, avg, sat=ZERO3, ct=ONE3):
+def _compose_oracle(o, pre, thr, avg, sat=ZERO3, ct=ONE3, emis=None, fog=None, weight=1.5):
     H, W = pre.shape[:2]
     fb = OracleFramebuffers(o, W, H)
     fb.set(PLANE["PRE_FINAL"], pre)
     fb.set(PLANE["THROUGHPUT"], thr)
-    o.l.or_compose_final(C.byref(fb.c), sat, ct, float(avg))
+    fb.set(PLANE["SCREEN_EMISSION"], emis if emis is not None else np.zeros((H, W, 4)))
+    fb.set(PLANE["ACID_FOG"], fog if fog is not None else np.zeros((H, W, 4)))
+    o.l.or_compose_final(C.byref(fb.c), sat, ct, float(avg), float(weight))
     return fb.cur[PLANE["FINAL"]]
+
+
+def test_compose_screen_emission_and_acid_fog(oracle):
+    """finalizeColor (CmPrepareFinal.comp:109-121): the screen emission joins
+    the exposed HDR colour before the tone map, weighted by
+    emissionMaxScreenColor; the acid fog is added after it."""
+    W, H = 8, 6
+    pre = np.zeros((H, W, 4)); pre[..., :3] = 2.0
+    emis = np.zeros((H, W, 4)); emis[..., :3] = (0.3, 0.05, 0.6)
+    fog = np.zeros((H, W, 4)); fog[..., :3] = (0.01, 0.02, 0.005)
+    thr = np.zeros((H, W, 4))
+    out = _compose_oracle(oracle, pre, thr, 0.5, emis=emis, fog=fog, weight=2.5)
+    want = lpm(oracle, np.array([2.0 / 4.8] * 3) + 2.5 * np.array([0.3, 0.05, 0.6])) + np.array([0.01, 0.02, 0.005])
+    assert np.allclose(out[..., :3], want, rtol=1e-12)
+    # with no exposure (black average) only the emission and the fog remain
+    out0 = _compose_oracle(oracle, pre, thr, 0.0, emis=emis, fog=fog, weight=2.5)
+    assert np.allclose(out0[..., :3], lpm(oracle, 2.5 * np.array([0.3, 0.05, 0.6])) + np.array([0.01, 0.02, 0.005]), rtol=1e-12)
 
 
 def test_compose_exposure(oracle):
@@ -167,10 +186,17 @@ def _inputs(W, H, seed):
 def test_gpu_compose_final(gpu, oracle, W, H, custom):
     print(f"compose {W}x{H} custom={custom}", flush=True)
     pre, thr = _inputs(W, H, seed=W + H)
+    r = np.random.default_rng(W * H)
+    emis = np.zeros((H, W, 4), np.float32)
+    emis[..., :3] = r.uniform(0, 0.5, (H, W, 3)) * (r.uniform(0, 1, (H, W, 1)) < 0.2)
+    fog = np.zeros((H, W, 4), np.float32)
+    fog[..., :3] = r.uniform(0, 0.05, (H, W, 3)) * (r.uniform(0, 1, (H, W, 1)) < 0.1)
     avg = np.array([0.37], np.float32)
     g = FramebufferSet(W, H)
     g.upload(PLANE["PRE_FINAL"], pre)
     g.upload(PLANE["THROUGHPUT"], thr)
+    g.upload(PLANE["SCREEN_EMISSION"], emis)
+    g.upload(PLANE["ACID_FOG"], fog)
     d_avg = DeviceArray.from_numpy(avg)
     sat, ct = ZERO3, ONE3
     tm = None
@@ -181,13 +207,21 @@ def test_gpu_compose_final(gpu, oracle, W, H, custom):
         tm.crosstalk.data[:] = (1.0, 0.5, 0.25)
         sat, ct = (C.c_float * 3)(*tm.saturation.data), (C.c_float * 3)(*tm.crosstalk.data)
         tm = C.byref(tm)
-    ffi.call("rgExtComposeFinal", g.h, tm, d_avg.ptr, None)
+    weight = 1.5
+    if custom:
+        tx = rtgl.RgDrawFrameTexturesParams()
+        tx.sType = rtgl.RG_STRUCTURE_TYPE_TEXTURES
+        tx.emissionMaxScreenColor = weight = 3.0
+        ffi.call("rgExtComposeFinalEx", g.h, tm, C.byref(tx), d_avg.ptr, None)
+    else:
+        ffi.call("rgExtComposeFinal", g.h, tm, d_avg.ptr, None)
     got = g.download(PLANE["FINAL"]).astype(np.float64)
-    ref = _compose_oracle(oracle, pre.astype(np.float64), thr.astype(np.float64), float(avg[0]), sat, ct)
+    ref = _compose_oracle(oracle, pre.astype(np.float64), thr.astype(np.float64), float(avg[0]), sat, ct,
+                          emis.astype(np.float64), fog.astype(np.float64), weight)
     assert np.isfinite(got).all()
     err = np.abs(got - ref).max()
     assert err < 1e-4, err
-    assert got[..., :3].min() >= 0.0 and got[..., :3].max() <= 1.0
+    assert got[..., :3].min() >= 0.0 and got[..., :3].max() <= 1.0 + 0.05  # the fog adds after the tone map
 
 
 @pytest.mark.gpu
@@ -207,10 +241,12 @@ def test_gpu_frame_final_plane(gpu, oracle):
         fbv = r.framebuffers()
         pre = fbv.download(PLANE["PRE_FINAL"]).astype(np.float64)
         thr = fbv.download(PLANE["THROUGHPUT"]).astype(np.float64)
+        emis = fbv.download(PLANE["SCREEN_EMISSION"]).astype(np.float64)
+        fog = fbv.download(PLANE["ACID_FOG"]).astype(np.float64)
         got = fbv.download(PLANE["FINAL"]).astype(np.float64)
     finally:
         r.close()
     assert st.avgLuminance > 0
-    ref = _compose_oracle(oracle, pre, thr, st.avgLuminance)
+    ref = _compose_oracle(oracle, pre, thr, st.avgLuminance, emis=emis, fog=fog)
     assert np.abs(got - ref).max() < 1e-4
     assert 0.05 < got[..., :3].mean() < 0.95  # exposed, not black or blown out

@@ -11,18 +11,18 @@ from oracle_lib import OracleFramebuffers, OraclePathTracer
 import reflrefr_cases as rc
 
 
-def _run(oracle, scene, flags, params):
+def _run(oracle, scene, flags, params, emissive=None):
     fb = OracleFramebuffers(oracle, rc.W, rc.H)
-    pt = OraclePathTracer(oracle, scene, rc.W, rc.H, flags=flags)
+    pt = OraclePathTracer(oracle, scene, rc.W, rc.H, flags=flags, emissive=emissive)
     pt.frame(params, fb, threads=8)
     return fb.cur
 
 
 @pytest.mark.parametrize("case", list(rc.CASES))
 def test_oracle_chain_closed_forms(oracle, case):
-    flags, settings, fovy, check = rc.CASES[case]
+    flags, settings, fovy, check, *emissive = rc.CASES[case]
     p = rc.frame_params(rc.camera(fovy), **settings)
-    cur = _run(oracle, rc._planes_scene(), flags, p)
+    cur = _run(oracle, rc._planes_scene(), flags, p, emissive[0] if emissive else None)
     out = {k: cur[PLANE[k]] for k in rc.NAMES}
     out["params"] = p
     check(out)

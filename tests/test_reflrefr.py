@@ -12,11 +12,11 @@ import reflrefr_cases as rc
 pytestmark = pytest.mark.gpu
 
 
-def _render(flags, settings, fovy):
+def _render(flags, settings, fovy, emissive=None):
     r = rtgl.Renderer(rc.W, rc.H)
     try:
         r.set_reflect_refract(rc.draw_params(**settings))
-        r.frame(rtgl.SceneUploader(rc._planes_scene(), flags), rc.camera(fovy))
+        r.frame(rtgl.SceneUploader(rc._planes_scene(), flags, emissive=emissive), rc.camera(fovy))
         fb = r.framebuffers()
         out = {k: fb.download(PLANE[k]) for k in rc.NAMES}
         out["params"] = r.frame_params()
@@ -27,9 +27,9 @@ def _render(flags, settings, fovy):
 
 @pytest.mark.parametrize("case", list(rc.CASES))
 def test_chain_closed_forms(case):
-    flags, settings, fovy, check = rc.CASES[case]
+    flags, settings, fovy, check, *emissive = rc.CASES[case]
     print(f"reflection / refraction case {case}", flush=True)
-    check(_render(flags, settings, fovy))
+    check(_render(flags, settings, fovy, emissive[0] if emissive else None))
 
 
 @pytest.mark.parametrize("pane_flags, lit", rc.SHADOW_CASES)
