@@ -350,6 +350,7 @@ __device__ void ApplyMaterial( const SceneView& sc, const RgExtFrameParams& p, c
 
 // getHitInfo* (HitInfo.inl:204-466): albedo is the primitive colour (layer
 // 0) times its albedo texture if it has one, normals interpolated or exact.
+template< bool TEX = false >
 __device__ HitInfo GetHit( const SceneView& sc, const RgExtFrameParams& p, uint32_t prim, float u, float v, v3 rayOrigin,
                            const TexQuery& q = TexQuery{} )
 {
@@ -377,7 +378,7 @@ __device__ HitInfo GetHit( const SceneView& sc, const RgExtFrameParams& p, uint3
     h.albedo    = mk3( saturate( c.x ), saturate( c.y ), saturate( c.z ) );
     float rough = gr.roughness, metal = gr.metallic;
     bool  emissiveMap = false;
-    if( gr.texture != TEX_NONE || gr.texOrm != TEX_NONE || gr.texNormal != TEX_NONE || gr.texEmissive != TEX_NONE )
+    if( TEX && ( gr.texture != TEX_NONE || gr.texOrm != TEX_NONE || gr.texNormal != TEX_NONE || gr.texEmissive != TEX_NONE ) )
     {
         ApplyMaterial( sc, p, gr, prim - gr.firstTri, u, v, p0, p1, p2, rayOrigin, q, h, &rough, &metal, &emissiveMap );
     }
@@ -802,6 +803,7 @@ __device__ void StoreSky( const RgExtFrameParams& p, const FbPtrs& P, size_t ci,
 }
 
 // G-buffer of the primary hit (RaygenPrimary.inl:280-348, storeSky :90-139).
+template< bool TEX >
 __global__ __launch_bounds__( THREADS ) void k_primary_shade( RgExtFrameParams p, const RgExtHit* hits, SceneView sc, FbPtrs P )
 {
     const int x = blockIdx.x * BX + threadIdx.x % BX, y = blockIdx.y * BY + threadIdx.x / BX;
@@ -839,7 +841,7 @@ __global__ __launch_bounds__( THREADS ) void k_primary_shade( RgExtFrameParams p
     tq.mode = TQ_GRAD;
     tq.dbx  = bAX - mk3( 1.0f - hit.u - hit.v, hit.u, hit.v );
     tq.dby  = bAY - mk3( 1.0f - hit.u - hit.v, hit.u, hit.v );
-    const HitInfo     h  = GetHit( sc, p, hit.primitiveId, hit.u, hit.v, org, tq );
+    const HitInfo     h  = GetHit< TEX >( sc, p, hit.primitiveId, hit.u, hit.v, org, tq );
     const GeomRecord& gr = sc.geoms[ h.geom ];
     const v3 prevPos = gr.isStatic ? h.position : Xform34( gr.prevFromCur, h.position );
 
@@ -903,6 +905,7 @@ struct RrState
 // One depth of the loop up to traceReflectionRefractionRay.  Depth 0 reads
 // the primary surface from the G-buffer (as the reference restores it) and
 // its flags from the primary hit; later depths recompute the last hit.
+template< bool TEX >
 __global__ __launch_bounds__( THREADS ) void k_reflrefr_a( RgExtFrameParams p, FbPtrs P, SceneView sc, RgExtRay* rays, const RgExtHit* hits,
                                                            RrState st, uint32_t depth )
 {
@@ -966,7 +969,7 @@ __global__ __launch_bounds__( THREADS ) void k_reflrefr_a( RgExtFrameParams p, F
         cq.mode            = TQ_CONE;
         cq.a               = st.s[ 3 * st.n + ri ].x; // the cone the last hit was shaded with
         cq.dir             = mk3( s1.x, s1.y, s1.z );
-        const HitInfo  h   = GetHit( sc, p, hit.primitiveId, hit.u, hit.v, mk3( r.origin[ 0 ], r.origin[ 1 ], r.origin[ 2 ] ), cq );
+        const HitInfo  h   = GetHit< TEX >( sc, p, hit.primitiveId, hit.u, hit.v, mk3( r.origin[ 0 ], r.origin[ 1 ], r.origin[ 2 ] ), cq );
         pos                = h.position;
         normal             = h.normal;
         albedo             = h.albedo;
@@ -1079,6 +1082,7 @@ __global__ __launch_bounds__( THREADS ) void k_reflrefr_a( RgExtFrameParams p, F
 // After the trace: the sky (RaygenPrimary.inl:550-556) or the new surface
 // (getHitInfoWithRayCone_ReflectionRefraction, HitInfo.inl:213-330, and the
 // G-buffer stores of :584-597), with its virtual-position motion.
+template< bool TEX >
 __global__ __launch_bounds__( THREADS ) void k_reflrefr_b( RgExtFrameParams p, FbPtrs P, SceneView sc, const RgExtRay* rays, const RgExtHit* hits,
                                                            RrState st, uint32_t depth )
 {
@@ -1119,7 +1123,7 @@ __global__ __launch_bounds__( THREADS ) void k_reflrefr_b( RgExtFrameParams p, F
     cq.mode               = TQ_CONE;
     cq.a                  = s2.w;
     cq.dir                = dir;
-    const HitInfo  h      = GetHit( sc, p, hit.primitiveId, hit.u, hit.v, origin, cq );
+    const HitInfo  h      = GetHit< TEX >( sc, p, hit.primitiveId, hit.u, hit.v, origin, cq );
     const float    rayLen = length( h.position - origin );
     // propagateRayCone (RayCone.h:30-36): the spread angle doubles at every
     // propagation, so at this depth it is the camera's times 2^(depth + 1)
@@ -1456,6 +1460,7 @@ __device__ void IndirectDirect( const RgExtFrameParams& p, const LightView& lv, 
 }
 
 // Bounce-1 hit: direct light at the hit (shadow ray) and the second bounce.
+template< bool TEX >
 __global__ __launch_bounds__( THREADS ) void k_indirect_b( RgExtFrameParams p, FbPtrs P, SceneView sc, LightView lv, const RgExtRay* rays1, const RgExtHit* hits1,
                                                            RgExtRay* shadowRays, RgExtRay* bounceRays, IndirPend pend )
 {
@@ -1484,7 +1489,7 @@ __global__ __launch_bounds__( THREADS ) void k_indirect_b( RgExtFrameParams p, F
     TexQuery bq;
     bq.mode             = TQ_BOUNCE; // FIRST_BOUNCE_MIP_BIAS 0 (RtRaygenIndirect.inl:63)
     bq.a                = Plane< float2 >( P, RG_EXT_PLANE_METALLIC_ROUGHNESS )[ i ].y;
-    const HitInfo  h    = GetHit( sc, p, hit.primitiveId, hit.u, hit.v, org, bq );
+    const HitInfo  h    = GetHit< TEX >( sc, p, hit.primitiveId, hit.u, hit.v, org, bq );
     const Surface  hs   = HitToSurface( h, dir );
     const uint32_t seed = randomSeed( x, y, p.frameId );
     v3             diff, lightPos = mk3( 0, 0, 0 );
@@ -1511,6 +1516,7 @@ __global__ __launch_bounds__( THREADS ) void k_indirect_b( RgExtFrameParams p, F
 
 // Second-bounce hit and the final indirect sample; shading of the primary
 // surface with it (RtRaygenIndirect.inl:66-161, :402-433 without reuse).
+template< bool TEX >
 __global__ __launch_bounds__( THREADS ) void k_indirect_c( RgExtFrameParams p, FbPtrs P, SceneView sc, LightView lv, const RgExtHit* shadowHits,
                                                            const RgExtRay* bounceRays, const RgExtHit* bounceHits, const RgExtHit* hits1, IndirPend pend )
 {
@@ -1560,7 +1566,7 @@ __global__ __launch_bounds__( THREADS ) void k_indirect_c( RgExtFrameParams p, F
         TexQuery       q1;
         q1.mode             = TQ_BOUNCE;
         q1.a                = s.roughness;
-        const HitInfo  hi1  = GetHit( sc, p, h1.primitiveId, h1.u, h1.v, s.position + s.normal * 0.01f, q1 );
+        const HitInfo  hi1  = GetHit< TEX >( sc, p, h1.primitiveId, h1.u, h1.v, s.position + s.normal * 0.01f, q1 );
         const v3       emis = hi1.emission * p.emissionMapBoost;
         // processSecondDiffuseBounce (RtRaygenIndirect.inl:82-101)
         const v3       d2 = mk3( b2.x, b2.y, b2.z );
@@ -1577,7 +1583,7 @@ __global__ __launch_bounds__( THREADS ) void k_indirect_c( RgExtFrameParams p, F
             q2.mode             = TQ_BOUNCE;
             q2.a                = hi1.roughness;
             q2.b                = 32.0f; // SECOND_BOUNCE_MIP_BIAS (RtRaygenIndirect.inl:64)
-            const HitInfo  hi2  = GetHit( sc, p, h2.primitiveId, h2.u, h2.v, mk3( br.origin[ 0 ], br.origin[ 1 ], br.origin[ 2 ] ), q2 );
+            const HitInfo  hi2  = GetHit< TEX >( sc, p, h2.primitiveId, h2.u, h2.v, mk3( br.origin[ 0 ], br.origin[ 1 ], br.origin[ 2 ] ), q2 );
             const Surface  hs2  = HitToSurface( hi2, d2 );
             const uint32_t seed = randomSeed( x, y, p.frameId );
             v3             diff2, lp;
@@ -2268,7 +2274,7 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
     k_primary_rays<<< grid2, THREADS, 0, s >>>( p, R );
     CheckLaunch( "k_primary_rays" );
     trace( R, Hh, RG_EXT_TRACE_CLOSEST, n, true );
-    k_primary_shade<<< grid2, THREADS, 0, s >>>( p, Hh, sv, P );
+    ( tex.count > 0 ? k_primary_shade< true > : k_primary_shade< false > )<<< grid2, THREADS, 0, s >>>( p, Hh, sv, P );
     CheckLaunch( "k_primary_shade" );
 
     // reflections and refractions replace the G-buffer surface; without
@@ -2278,10 +2284,10 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
         RrState st{ pendA.as< float4 >(), n };
         for( uint32_t d = 0; d < p.reflectRefractMaxDepth && d < 16u; d++ )
         {
-            k_reflrefr_a<<< grid2, THREADS, 0, s >>>( p, P, sv, R, Hh, st, d );
+            ( tex.count > 0 ? k_reflrefr_a< true > : k_reflrefr_a< false > )<<< grid2, THREADS, 0, s >>>( p, P, sv, R, Hh, st, d );
             CheckLaunch( "k_reflrefr_a" );
             trace( R, Hh, RG_EXT_TRACE_CLOSEST, n, true );
-            k_reflrefr_b<<< grid2, THREADS, 0, s >>>( p, P, sv, R, Hh, st, d );
+            ( tex.count > 0 ? k_reflrefr_b< true > : k_reflrefr_b< false > )<<< grid2, THREADS, 0, s >>>( p, P, sv, R, Hh, st, d );
             CheckLaunch( "k_reflrefr_b" );
         }
     }
@@ -2325,11 +2331,11 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
     k_indirect_a<<< grid2, THREADS, 0, s >>>( p, P, lv, R, ip );
     CheckLaunch( "k_indirect_a" );
     trace( R, Hh, RG_EXT_TRACE_CLOSEST, n );
-    k_indirect_b<<< grid2, THREADS, 0, s >>>( p, P, sv, lv, R, Hh, shadowR, R2, ip );
+    ( tex.count > 0 ? k_indirect_b< true > : k_indirect_b< false > )<<< grid2, THREADS, 0, s >>>( p, P, sv, lv, R, Hh, shadowR, R2, ip );
     CheckLaunch( "k_indirect_b" );
     trace( shadowR, shadowH, RG_EXT_TRACE_ANY_HIT, n );
     trace( R2, H2, RG_EXT_TRACE_CLOSEST, n );
-    k_indirect_c<<< grid2, THREADS, 0, s >>>( p, P, sv, lv, shadowH, R2, H2, Hh, ip );
+    ( tex.count > 0 ? k_indirect_c< true > : k_indirect_c< false > )<<< grid2, THREADS, 0, s >>>( p, P, sv, lv, shadowH, R2, H2, Hh, ip );
     CheckLaunch( "k_indirect_c" );
     k_indirect_final<<< grid2, THREADS, 0, s >>>( p, P );
     CheckLaunch( "k_indirect_final" );
