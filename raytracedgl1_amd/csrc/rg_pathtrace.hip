@@ -530,7 +530,7 @@ __global__ __launch_bounds__( THREADS ) void k_light_grid( RgExtFrameParams p, L
 // visibility test.  Returns the regular and directional reservoirs and the
 // salt to continue with.
 __device__ void InitialCandidates( const RgExtFrameParams& p, const LightView& lv, const Surface& surf, uint32_t seed, uint32_t& salt,
-                                   float pu, float pv, bool loose, Reservoir& regular, Reservoir& dir )
+                                   const LightPoint& lp, bool loose, Reservoir& regular, Reservoir& dir )
 {
     const Grid g = MakeGrid( p );
     regular      = emptyReservoir();
@@ -554,7 +554,7 @@ __device__ void InitialCandidates( const RgExtFrameParams& p, const LightView& l
                 continue;
             }
             const float       oneOverSource = r.weightSum * safePositiveRcp( r.targetPdf );
-            const LightSample ls            = sampleLight( GetLight( lv, r.selected ), surf.position, pu, pv, p.polyLightSpotlightFactor );
+            const LightSample ls            = sampleLight( GetLight( lv, r.selected ), surf.position, lp, p.polyLightSpotlightFactor );
             const float       tp            = targetPdf( surf, ls, loose );
             updateReservoir( regular, r.selected, tp, oneOverSource, rnd16( seed, salt++ ) );
         }
@@ -565,7 +565,7 @@ __device__ void InitialCandidates( const RgExtFrameParams& p, const LightView& l
         {
             const float       rnd = rnd16( seed, salt++ );
             const uint32_t    xi  = 1u + min( uint32_t( rnd * float( p.lightCount ) ), p.lightCount - 1u );
-            const LightSample ls  = sampleLight( GetLight( lv, xi ), surf.position, pu, pv, p.polyLightSpotlightFactor );
+            const LightSample ls  = sampleLight( GetLight( lv, xi ), surf.position, lp, p.polyLightSpotlightFactor );
             const float       tp  = targetPdf( surf, ls, loose );
             updateReservoir( regular, xi, tp, float( p.lightCount ), rnd16( seed, salt++ ) );
         }
@@ -574,7 +574,7 @@ __device__ void InitialCandidates( const RgExtFrameParams& p, const LightView& l
     dir = emptyReservoir();
     if( p.directionalLightExists )
     {
-        const LightSample ls = sampleLight( GetLight( lv, 0 ), surf.position, pu, pv, p.polyLightSpotlightFactor );
+        const LightSample ls = sampleLight( GetLight( lv, 0 ), surf.position, lp, p.polyLightSpotlightFactor );
         const float       tp = targetPdf( surf, ls, loose );
         updateReservoir( dir, 0, tp, 1.0f, rnd16( seed, salt++ ) );
     }
@@ -599,11 +599,11 @@ __device__ __forceinline__ RgExtRay ShadowRay( const Surface& s, v3 lightPos )
     return MakeRay( start, l / dist, 0.001f, dist - 0.01f );
 }
 
-__device__ __forceinline__ void LightPointRnd( uint32_t seed, float* u, float* v )
+__device__ __forceinline__ LightPoint LightPointRnd( uint32_t seed )
 {
-    rnd16_2( seed, SALT_LIGHT_POINT, u, v );
-    *u *= 0.99f;
-    *v *= 0.99f;
+    float u, v;
+    rnd16_2( seed, SALT_LIGHT_POINT, &u, &v );
+    return MakeLightPoint( u * 0.99f, v * 0.99f );
 }
 
 __device__ __forceinline__ bool DirectValid( const RgExtFrameParams& p, int bounce )
@@ -1191,16 +1191,15 @@ __global__ __launch_bounds__( THREADS ) void k_initial_a( RgExtFrameParams p, Fb
     }
     const uint32_t seed = randomSeed( x, y, p.frameId );
     uint32_t       salt = SALT_INITIAL_RESERVOIRS;
-    float          pu, pv;
-    LightPointRnd( seed, &pu, &pv );
+    const LightPoint lp = LightPointRnd( seed );
     Reservoir regular, dir;
-    InitialCandidates( p, lv, s, seed, salt, pu, pv, false, regular, dir );
+    InitialCandidates( p, lv, s, seed, salt, lp, false, regular, dir );
     Plane< uint4 >( P, RG_EXT_PLANE_RESERVOIRS_INITIAL )[ i ] = packReservoir( regular );
     const uint4 pd = packReservoir( dir );
     pend[ i ]      = make_uint4( pd.x, pd.y, pd.z, salt );
     if( isValid( dir ) )
     {
-        const LightSample ls = sampleLight( GetLight( lv, 0 ), s.position, pu, pv, p.polyLightSpotlightFactor );
+        const LightSample ls = sampleLight( GetLight( lv, 0 ), s.position, lp, p.polyLightSpotlightFactor );
         rays[ i ]            = ShadowRay( s, ls.position );
     }
     else
@@ -1272,8 +1271,7 @@ __global__ __launch_bounds__( THREADS ) void k_direct_a( RgExtFrameParams p, FbP
     Reservoir      res  = emptyReservoir();
     if( DirectValid( p, 0 ) )
     {
-        float pu, pv;
-        LightPointRnd( seed, &pu, &pv );
+        const LightPoint lp = LightPointRnd( seed );
         // selectLight_Direct (RaygenCommon.h:503-599)
         int lo, hi;
         cbRenderArea( x, W, &lo, &hi );
@@ -1304,7 +1302,7 @@ __global__ __launch_bounds__( THREADS ) void k_direct_a( RgExtFrameParams p, FbP
                 const uint32_t cur = t.selected < lv.prevCount ? lv.prevToCur[ t.selected ] : 0xFFFFFFFFu;
                 if( cur != 0xFFFFFFFFu && cur != LIGHT_INDEX_NONE )
                 {
-                    tpCur      = targetPdf( s, sampleLight( GetLight( lv, cur ), s.position, pu, pv, p.polyLightSpotlightFactor ), false );
+                    tpCur      = targetPdf( s, sampleLight( GetLight( lv, cur ), s.position, lp, p.polyLightSpotlightFactor ), false );
                     t.selected = cur;
                 }
             }
@@ -1329,7 +1327,7 @@ __global__ __launch_bounds__( THREADS ) void k_direct_a( RgExtFrameParams p, FbP
         {
             res = comb;
             // traceDirectIllumination (RaygenCommon.h:603-637)
-            const LightSample ls = sampleLight( GetLight( lv, res.selected ), s.position, pu, pv, p.polyLightSpotlightFactor );
+            const LightSample ls = sampleLight( GetLight( lv, res.selected ), s.position, lp, p.polyLightSpotlightFactor );
             shadeLight( s, ls, selectedSampleWeight( res ), false, &diff, &spec );
             if( luminance( diff + spec ) <= 0.0f )
             {
@@ -1436,17 +1434,16 @@ __device__ void IndirectDirect( const RgExtFrameParams& p, const LightView& lv, 
     {
         return;
     }
-    float pu, pv;
-    LightPointRnd( seed, &pu, &pv );
+    const LightPoint lp = LightPointRnd( seed );
     uint32_t  salt = SALT_LIGHT_CHOOSE_INDIRECT;
     Reservoir regular, dir;
-    InitialCandidates( p, lv, hs, seed, salt, pu, pv, true, regular, dir );
+    InitialCandidates( p, lv, hs, seed, salt, lp, true, regular, dir );
     const Reservoir r = CombineInitial( regular, dir, seed, salt );
     if( !isValid( r ) )
     {
         return;
     }
-    const LightSample ls = sampleLight( GetLight( lv, r.selected ), hs.position, pu, pv, p.polyLightSpotlightFactor );
+    const LightSample ls = sampleLight( GetLight( lv, r.selected ), hs.position, lp, p.polyLightSpotlightFactor );
     v3                d, sp;
     shadeLight( hs, ls, selectedSampleWeight( r ), true, &d, &sp );
     if( luminance( d + sp ) <= 0.0f )
@@ -1999,9 +1996,8 @@ __global__ __launch_bounds__( THREADS ) void k_grad_a( RgExtFrameParams p, FbPtr
                 r.selected = cur;
                 // processDirectIllumination for LIGHT_SAMPLE_METHOD_GRADIENTS
                 // (RaygenCommon.h:742-760) -> traceDirectIllumination (:639-674)
-                float pu, pv;
-                LightPointRnd( oldSeed, &pu, &pv );
-                const LightSample ls = sampleLight( GetLight( lv, cur ), ps.position, pu, pv, p.polyLightSpotlightFactor );
+                const LightPoint lp = LightPointRnd( oldSeed );
+                const LightSample ls = sampleLight( GetLight( lv, cur ), ps.position, lp, p.polyLightSpotlightFactor );
                 v3                d, sp;
                 shadeLight( ps, ls, selectedSampleWeight( r ), false, &d, &sp );
                 if( luminance( d + sp ) > 0.0f )

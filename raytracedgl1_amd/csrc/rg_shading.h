@@ -58,6 +58,33 @@ RG_HD v3 safeNormalize( v3 v )
     return len > 0.001f ? v / len : mk3( 0.0f, 1.0f, 0.0f );
 }
 
+// Hardware reciprocal and reciprocal square root (v_rcp_f32 / v_rsq_f32,
+// 1 ulp) for the light sampling and BRDF evaluated per RIS candidate: a
+// correctly rounded divide is about ten instructions, and the candidate loop
+// runs up to 32 times per pixel.  Zero, infinite and NaN inputs give the
+// same specials as the division they replace.
+__device__ __forceinline__ float fastRcp( float x )
+{
+    return __builtin_amdgcn_rcpf( x );
+}
+__device__ __forceinline__ float fastLength( v3 a )
+{
+    return sqrtf( dot( a, a ) );
+}
+__device__ __forceinline__ v3 fastNormalize( v3 a )
+{
+    return a * __builtin_amdgcn_rsqf( dot( a, a ) );
+}
+__device__ __forceinline__ v3 fastSafeNormalize( v3 v )
+{
+    const float len = fastLength( v );
+    return len > 0.001f ? v * fastRcp( len ) : mk3( 0.0f, 1.0f, 0.0f );
+}
+__device__ __forceinline__ float fastSafePositiveRcp( float f )
+{
+    return f <= 0.0f ? 0.0f : fastRcp( f );
+}
+
 // ----------------------------------------------------------- sampling
 
 // Random.h:36-51
@@ -103,6 +130,46 @@ __device__ __forceinline__ v3 sampleOrientedHemisphere( v3 n, float u1, float u2
     return r;
 }
 
+// The light-point random numbers of a pixel and the terms of sampleDisk and
+// sampleOrientedHemisphere that depend only on them.  Every candidate light
+// of a pixel is sampled at the same (u, v) (Light.h:225-313 takes one
+// pointRnd per surface), so the sines and cosines are taken once per pixel;
+// each term is the same expression the warps evaluate.
+struct LightPoint
+{
+    float pu, pv;
+    float diskR, diskCos, diskSin;      // sampleDisk( 1, pu, pv ): sqrt(0.99 u1), cos / sin(2 pi 0.99 u2)
+    float hemiX, hemiY, hemiZ;          // sampleOrientedHemisphere( n, pu, pv ) = normalize( n + hemi )
+    float triU1, triU2;                 // the triangle light's 0.99-scaled pair
+};
+
+__device__ __forceinline__ LightPoint MakeLightPoint( float pu, float pv )
+{
+    LightPoint lp;
+    lp.pu = pu;
+    lp.pv = pv;
+    {
+        const float u1 = pu * 0.99f, u2 = pv * 0.99f;
+        const float phi = 2.0f * RG_PI * u2;
+        lp.diskR        = sqrtf( u1 );
+        lp.diskCos      = cosf( phi );
+        lp.diskSin      = sinf( phi );
+        lp.triU1        = u1;
+        lp.triU2        = u2;
+    }
+    {
+        float       a   = 1.0f - 2.0f * pu;
+        float       b   = sqrtf( 1.0f - a * a );
+        const float phi = 2.0f * RG_PI * pv;
+        a *= 0.98f;
+        b *= 0.98f;
+        lp.hemiX = b * cosf( phi );
+        lp.hemiY = b * sinf( phi );
+        lp.hemiZ = a;
+    }
+    return lp;
+}
+
 // ---------------------------------------------------------------- BRDF
 
 // BRDF.h:83-103; loose = FORCE_EVALBRDF_GGX_LOOSE (alpha >= 0.2), defined
@@ -115,7 +182,7 @@ __device__ __forceinline__ float D_GGX( float nm, float alpha, bool loose )
     }
     const float a2 = alpha * alpha;
     nm             = fmaxf( 0.0f, nm );
-    return a2 / RG_PI / square( nm * nm * ( a2 - 1.0f ) + 1.0f );
+    return a2 * ( 1.0f / RG_PI ) * fastRcp( square( nm * nm * ( a2 - 1.0f ) + 1.0f ) );
 }
 
 __device__ __forceinline__ float G1_GGX( float ns, float alpha )
@@ -134,10 +201,10 @@ __device__ __forceinline__ v3 evalBRDFSmithGGX( v3 n, v3 v, v3 l, float alpha, v
     }
     const float f  = powf( 1.0f - fmaxf( nl, 0.0f ), 5.0f );
     const v3    F  = specColor + ( mk3( 1, 1, 1 ) - specColor ) * f;
-    const v3    h  = normalize( v + l );
+    const v3    h  = fastNormalize( v + l );
     const float D  = D_GGX( dot( n, h ), alpha, loose );
     const float nv = fmaxf( dot( n, v ), 0.0f );
-    const float G2 = 0.5f / mixf( 2.0f * nl * nv, nl + nv, alpha );
+    const float G2 = 0.5f * fastRcp( mixf( 2.0f * nl * nv, nl + nv, alpha ) );
     return F * ( G2 * D );
 }
 
@@ -187,7 +254,7 @@ __device__ __forceinline__ float safeSolidAngle( float a )
 }
 __device__ __forceinline__ float solidAngleForSphere( float radius, float dist )
 {
-    const float sinT = radius / fmaxf( radius, dist );
+    const float sinT = radius >= dist ? 1.0f : radius * fastRcp( dist );
     const float cosT = sqrtf( 1.0f - sinT * sinT );
     return safeSolidAngle( 2.0f * RG_PI * ( 1.0f - cosT ) );
 }
@@ -228,8 +295,8 @@ __device__ float lightWeight( const LightEncoded& e, v3 cellCenter, float cellRa
     }
 }
 
-// sampleLight (Light.h:225-313)
-__device__ LightSample sampleLight( const LightEncoded& e, v3 surfPos, float pu, float pv, float polyFactor )
+// sampleLight (Light.h:225-313), at the pixel's light point.
+__device__ LightSample sampleLight( const LightEncoded& e, v3 surfPos, const LightPoint& lp, float polyFactor )
 {
     LightSample r;
     r.position = mk3( 0, 0, 0 );
@@ -240,57 +307,52 @@ __device__ LightSample sampleLight( const LightEncoded& e, v3 surfPos, float pu,
         case LIGHT_TYPE_DIRECTIONAL: {
             const v3    dir  = L3( e.data0 );
             const float rad  = sinf( fmaxf( 0.01f, e.data0[ 3 ] ) );
-            float       dx, dy;
-            sampleDisk( rad, pu, pv, &dx, &dy );
-            v3 b0, b1;
+            const float rr   = rad * lp.diskR;
+            v3          b0, b1;
             revisedONB( dir, &b0, &b1 );
-            const v3 ln = normalize( dir + b0 * dx + b1 * dy );
+            const v3 ln = fastNormalize( dir + b0 * ( rr * lp.diskCos ) + b1 * ( rr * lp.diskSin ) );
             r.position  = surfPos - ln * MAX_RAY_LEN;
             r.color     = L3( e.color );
             r.dw        = 1.0f;
             return r;
         }
         case LIGHT_TYPE_SPHERE: {
+            // the hemisphere around the direction to the surface
             const v3    c   = L3( e.data0 );
-            v3          d   = c - surfPos;
-            const float len = length( d );
-            d               = d / len;
-            float       pdf;
-            const v3    ln = sampleOrientedHemisphere( -d, pu, pv, &pdf );
-            r.position     = c + ln * e.data0[ 3 ];
-            r.color        = L3( e.color );
-            r.dw           = solidAngleForSphere( e.data0[ 3 ], len );
+            const v3    d   = c - surfPos;
+            const float len = fastLength( d );
+            const v3    ln  = fastNormalize( d * -fastRcp( len ) + mk3( lp.hemiX, lp.hemiY, lp.hemiZ ) );
+            r.position      = c + ln * e.data0[ 3 ];
+            r.color         = L3( e.color );
+            r.dw            = solidAngleForSphere( e.data0[ 3 ], len );
             return r;
         }
         case LIGHT_TYPE_TRIANGLE: {
             const v3 p0 = L3( e.data0 ), p1 = L3( e.data1 ), p2 = L3( e.data2 );
             v3       nrm = mk3( e.data0[ 3 ], e.data1[ 3 ], e.data2[ 3 ] );
-            const float nl = length( nrm );
-            nrm            = nrm / nl;
+            const float nl = fastLength( nrm );
+            nrm            = nrm * fastRcp( nl );
             const float area = nl * 0.5f;
-            const float u1 = pu * 0.99f, u2 = pv * 0.99f;
-            const float beta = 1.0f - sqrtf( u1 ), gamma = ( 1.0f - beta ) * u2, alpha = 1.0f - beta - gamma;
+            const float beta = 1.0f - sqrtf( lp.triU1 ), gamma = ( 1.0f - beta ) * lp.triU2, alpha = 1.0f - beta - gamma;
             r.position       = p0 * alpha + p1 * beta + p2 * gamma;
-            const float ll   = fmaxf( dot( nrm, normalize( surfPos - r.position ) ), 0.0f );
-            r.color          = L3( e.color ) * powf( ll, polyFactor );
             v3          tl   = surfPos - r.position;
-            const float dist = length( tl );
-            tl               = tl / dist;
-            r.dw             = safeSolidAngle( area * ( fabsf( dot( nrm, tl ) ) / square( dist ) ) );
+            const float dist = fastLength( tl );
+            tl               = tl * fastRcp( dist );
+            const float ll   = fmaxf( dot( nrm, tl ), 0.0f );
+            r.color          = L3( e.color ) * powf( ll, polyFactor );
+            r.dw             = safeSolidAngle( area * ( fabsf( dot( nrm, tl ) ) * fastRcp( square( dist ) ) ) );
             return r;
         }
         case LIGHT_TYPE_SPOT: {
-            const v3 c = L3( e.data0 ), dir = L3( e.data1 );
-            float    dx, dy;
-            sampleDisk( e.data0[ 3 ], pu, pv, &dx, &dy );
-            v3 b0, b1;
+            const v3    c = L3( e.data0 ), dir = L3( e.data1 );
+            const float rr = e.data0[ 3 ] * lp.diskR;
+            v3          b0, b1;
             revisedONB( dir, &b0, &b1 );
-            r.position      = c + b0 * dx + b1 * dy;
-            v3          d   = c - surfPos;
-            const float len = length( d );
-            d               = d / len;
-            const float cosA = fmaxf( dot( dir, -d ), 0.0f );
-            float       t    = clampf( ( cosA - e.data2[ 1 ] ) / ( e.data2[ 0 ] - e.data2[ 1 ] ), 0.0f, 1.0f );
+            r.position      = c + b0 * ( rr * lp.diskCos ) + b1 * ( rr * lp.diskSin );
+            const v3    d   = c - surfPos;
+            const float len = fastLength( d );
+            const float cosA = fmaxf( -dot( dir, d ) * fastRcp( len ), 0.0f );
+            float       t    = clampf( ( cosA - e.data2[ 1 ] ) * fastRcp( e.data2[ 0 ] - e.data2[ 1 ] ), 0.0f, 1.0f );
             t                = t * t * ( 3.0f - 2.0f * t );
             r.color          = L3( e.color ) * square( t );
             r.dw             = solidAngleForSphere( e.data0[ 3 ], len );
@@ -303,7 +365,7 @@ __device__ LightSample sampleLight( const LightEncoded& e, v3 surfPos, float pu,
 // shade (RaygenCommon.h:364-380)
 __device__ __forceinline__ void shadeLight( const Surface& s, const LightSample& lt, float oneOverPdf, bool loose, v3* diff, v3* spec )
 {
-    const v3    l  = safeNormalize( lt.position - s.position );
+    const v3    l  = fastSafeNormalize( lt.position - s.position );
     const float nl = dot( s.normal, l );
     if( nl <= 0.0f )
     {
