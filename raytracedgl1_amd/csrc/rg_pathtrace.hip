@@ -1288,13 +1288,16 @@ __global__ __launch_bounds__( THREADS ) void k_direct_a( RgExtFrameParams p, FbP
             rnd8_4( seed, salt++, r4 );
             const int   qx    = int( floorf( px + ( r4[ 0 ] * 2.0f - 1.0f ) * 2.0f ) );
             const int   qy    = int( floorf( py + ( r4[ 1 ] * 2.0f - 1.0f ) * 2.0f ) );
-            const float dPrev = LdPlane( Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD, true ), W, H, qx, qy, 0.0f );
-            const v3    nPrev = DecodeNormal( LdPlane( Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL, true ), W, H, qx, qy, 0u ) );
-            if( !ReuseOk( lo, hi, H, qx, qy, depthCur, dPrev - mot.z, s.normal, nPrev ) )
+            // one round trip at the clamped pixel (an outside one fails ReuseOk)
+            const size_t   qi    = size_t( min( max( qy, 0 ), H - 1 ) ) * W + min( max( qx, 0 ), W - 1 );
+            const float    dPrev = Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD, true )[ qi ];
+            const uint32_t nPrev = Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL, true )[ qi ];
+            const uint4    tRaw  = Plane< uint4 >( P, RG_EXT_PLANE_RESERVOIRS, true )[ qi ];
+            if( !ReuseOk( lo, hi, H, qx, qy, depthCur, dPrev - mot.z, s.normal, DecodeNormal( nPrev ) ) )
             {
                 continue;
             }
-            Reservoir t = unpackReservoir( LdPlane( Plane< uint4 >( P, RG_EXT_PLANE_RESERVOIRS, true ), W, H, qx, qy, make_uint4( 0u, 0u, 0u, 0u ) ) );
+            Reservoir t = unpackReservoir( tRaw );
             normalizeReservoir( t, init.M * 20 );
             float tpCur = 0.0f;
             if( t.selected != LIGHT_INDEX_NONE )
@@ -1314,13 +1317,15 @@ __global__ __launch_bounds__( THREADS ) void k_direct_a( RgExtFrameParams p, FbP
             rnd8_4( seed, salt++, r4 );
             const int   qx = x + int( ( r4[ 0 ] * 2.0f - 1.0f ) * 30.0f );
             const int   qy = y + int( ( r4[ 1 ] * 2.0f - 1.0f ) * 30.0f );
-            const float dO = LdPlane( Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD ), W, H, qx, qy, 0.0f );
-            const v3    nO = DecodeNormal( LdPlane( Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL ), W, H, qx, qy, 0u ) );
-            if( !ReuseOk( lo, hi, H, qx, qy, depthCur, dO, s.normal, nO ) )
+            const size_t   qi  = size_t( min( max( qy, 0 ), H - 1 ) ) * W + min( max( qx, 0 ), W - 1 );
+            const float    dO  = Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD )[ qi ];
+            const uint32_t nO  = Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL )[ qi ];
+            const uint4    spR = Plane< uint4 >( P, RG_EXT_PLANE_RESERVOIRS_INITIAL )[ qi ];
+            if( !ReuseOk( lo, hi, H, qx, qy, depthCur, dO, s.normal, DecodeNormal( nO ) ) )
             {
                 continue;
             }
-            const Reservoir sp = unpackReservoir( LdPlane( Plane< uint4 >( P, RG_EXT_PLANE_RESERVOIRS_INITIAL ), W, H, qx, qy, make_uint4( 0u, 0u, 0u, 0u ) ) );
+            const Reservoir sp = unpackReservoir( spR );
             updateCombined( comb, sp, rnd16( seed, salt++ ) );
         }
         if( !( comb.weightSum <= 0.0f || comb.selected == LIGHT_INDEX_NONE ) )
@@ -1672,10 +1677,8 @@ __device__ __forceinline__ void CombineIndirNewSurf( ReservoirIndir& c, const Re
 
 // loadInitialSampleAsReservoir (RtRaygenIndirect.inl:246-255):
 // updateReservoirIndirect (ReservoirIndirect.h:70-84) with rnd = 0.5
-__device__ __forceinline__ ReservoirIndir InitialAsReservoir( const FbPtrs& P, size_t i )
+__device__ __forceinline__ ReservoirIndir InitialAsReservoir( uint4 a, uint2 b )
 {
-    const uint4    a = Plane< uint4 >( P, RG_EXT_PLANE_INDIR_INITIAL_SAMPLE )[ i ];
-    const uint2    b = Plane< uint2 >( P, RG_EXT_PLANE_INDIR_INITIAL_RADIANCE )[ i ];
     SampleIndir    s{ mk3( __uint_as_float( a.x ), __uint_as_float( a.y ), __uint_as_float( a.z ) ), DecodeNormal( a.w ), DecodeE5B9G9R9( b.x ) };
     const float    oneOverSourcePdf = __uint_as_float( b.y );
     const float    tp               = luminance( s.radiance ); // targetPdfForIndirectSample
@@ -1689,6 +1692,11 @@ __device__ __forceinline__ ReservoirIndir InitialAsReservoir( const FbPtrs& P, s
         r.selTargetPdf = tp;
     }
     return r;
+}
+
+__device__ __forceinline__ ReservoirIndir InitialAsReservoir( const FbPtrs& P, size_t i )
+{
+    return InitialAsReservoir( Plane< uint4 >( P, RG_EXT_PLANE_INDIR_INITIAL_SAMPLE )[ i ], Plane< uint2 >( P, RG_EXT_PLANE_INDIR_INITIAL_RADIANCE )[ i ] );
 }
 
 __device__ __forceinline__ ReservoirIndir LoadReservoirIndir( const FbPtrs& P, size_t i, bool prev )
@@ -1782,28 +1790,25 @@ __global__ __launch_bounds__( THREADS ) void k_indirect_final( RgExtFrameParams 
         const float oy = ( r4[ 1 ] * 2.0f - 1.0f ) * square( dw );
         const int   qx = int( floorf( px + ox * 2.0f ) );
         const int   qy = int( floorf( py + oy * 2.0f ) );
-        bool        use = LdPlane( Plane< uint32_t >( P, RG_EXT_PLANE_IS_SKY ), W, H, qx, qy, 0u ) == 0u;
-        if( use )
+        // the previous frame's planes in one round trip, at the pixel clamped
+        // into the image: a pixel outside fails the render-area check
+        const size_t   qi    = size_t( min( max( qy, 0 ), H - 1 ) ) * W + min( max( qx, 0 ), W - 1 );
+        const uint32_t skyQ  = Plane< uint32_t >( P, RG_EXT_PLANE_IS_SKY )[ qi ];
+        const float    dPrev = Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD, true )[ qi ];
+        const uint32_t nPrev = Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL, true )[ qi ];
+        // antilag of the indirect signal in the gradient history (strata pixels)
+        const int      GW    = ( W + 1 ) / 3 > 1 ? ( W + 1 ) / 3 : 1;
+        const int      GH    = ( H + 1 ) / 3 > 1 ? ( H + 1 ) / 3 : 1;
+        const float4   gh    = LdPlane( Plane< float4 >( P, RG_EXT_PLANE_DIS_GRADIENT_HISTORY ), GW, GH, qx / 3, qy / 3, make_float4( 0.f, 0.f, 0.f, 0.f ) );
+        ReservoirIndir t     = LoadReservoirIndir( P, qi, true );
+        bool           use   = skyQ == 0u && ReuseIndirOk( lo, hi, H, qx, qy, depthCur, dPrev - mot.z, s.normal, DecodeNormal( nPrev ) );
+        if( use && gh.y > 0.25f )
         {
-            const float dPrev = LdPlane( Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD, true ), W, H, qx, qy, 0.0f );
-            const v3    nPrev = DecodeNormal( LdPlane( Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL, true ), W, H, qx, qy, 0u ) );
-            use               = ReuseIndirOk( lo, hi, H, qx, qy, depthCur, dPrev - mot.z, s.normal, nPrev );
+            spatialCount++;
+            use = false;
         }
         if( use )
         {
-            // antilag of the indirect signal in the gradient history (strata pixels)
-            const int    GW    = ( W + 1 ) / 3 > 1 ? ( W + 1 ) / 3 : 1;
-            const int    GH    = ( H + 1 ) / 3 > 1 ? ( H + 1 ) / 3 : 1;
-            const float4 gh    = LdPlane( Plane< float4 >( P, RG_EXT_PLANE_DIS_GRADIENT_HISTORY ), GW, GH, qx / 3, qy / 3, make_float4( 0.f, 0.f, 0.f, 0.f ) );
-            if( gh.y > 0.25f )
-            {
-                spatialCount++;
-                use = false;
-            }
-        }
-        if( use )
-        {
-            ReservoirIndir t = LoadReservoirIndir( P, size_t( qy ) * W + qx, true );
             NormalizeIndir( t, 20u );
             CombineIndir( comb, t, rnd16( seed, salt++ ) );
         }
@@ -1819,19 +1824,21 @@ __global__ __launch_bounds__( THREADS ) void k_indirect_final( RgExtFrameParams 
             rnd8_4( seed, salt++, r4 );
             const int qx = x + int( ( r4[ 0 ] * 2.0f - 1.0f ) * radius );
             const int qy = y + int( ( r4[ 1 ] * 2.0f - 1.0f ) * radius );
-            if( LdPlane( Plane< uint32_t >( P, RG_EXT_PLANE_IS_SKY ), W, H, qx, qy, 0u ) != 0u )
+            // all of the neighbour's planes in one round trip, at the pixel
+            // clamped into the image; a pixel outside fails the reuse test
+            // (its render-area check), so the clamped values are never used
+            const size_t   qi   = size_t( min( max( qy, 0 ), H - 1 ) ) * W + min( max( qx, 0 ), W - 1 );
+            const uint32_t skyO = Plane< uint32_t >( P, RG_EXT_PLANE_IS_SKY )[ qi ];
+            const float    dO   = Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD )[ qi ];
+            const uint32_t nOe  = Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL )[ qi ];
+            const uint4    sA   = Plane< uint4 >( P, RG_EXT_PLANE_INDIR_INITIAL_SAMPLE )[ qi ];
+            const uint2    sB   = Plane< uint2 >( P, RG_EXT_PLANE_INDIR_INITIAL_RADIANCE )[ qi ];
+            const float4   x1q  = Plane< float4 >( P, RG_EXT_PLANE_SURFACE_POSITION )[ qi ];
+            if( skyO != 0u || !ReuseIndirOk( lo, hi, H, qx, qy, depthCur, dO, s.normal, DecodeNormal( nOe ) ) )
             {
                 continue;
             }
-            const float dO = LdPlane( Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD ), W, H, qx, qy, 0.0f );
-            const v3    nO = DecodeNormal( LdPlane( Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL ), W, H, qx, qy, 0u ) );
-            if( !ReuseIndirOk( lo, hi, H, qx, qy, depthCur, dO, s.normal, nO ) )
-            {
-                continue;
-            }
-            const size_t         qi  = size_t( qy ) * W + qx;
-            const ReservoirIndir rq  = InitialAsReservoir( P, qi );
-            const float4         x1q = Plane< float4 >( P, RG_EXT_PLANE_SURFACE_POSITION )[ qi ];
+            const ReservoirIndir rq = InitialAsReservoir( sA, sB );
             float oneOverJacobian    = safePositiveRcp( GeometryFactorSafe( rq.sel.normal, rq.sel.position, s.position ) ) *
                                     GeometryFactorSafe( rq.sel.normal, rq.sel.position, mk3( x1q.x, x1q.y, x1q.z ) );
             oneOverJacobian          = clampf( oneOverJacobian, 0.0f, 1.0f ); // SHIPPING_HACK
