@@ -187,7 +187,7 @@ __device__ __forceinline__ float4 f4( v3 a, float w )
 __device__ __forceinline__ v3 DecodeNormal( uint32_t p )
 {
     float sp, cp, st, ct;
-    sincospif( float( p >> 16 ) / 65535.0f, &sp, &cp );
+    sincospif( unitQuotient< 65535u >( p >> 16 ), &sp, &cp );
     sincospif( float( p & 0xFFFFu ) / 32768.0f, &st, &ct );
     return mk3( sp * ct, sp * st, cp );
 }

@@ -125,26 +125,41 @@ RG_HD uint32_t wellonsLowBias32( uint32_t x )
     return x;
 }
 
+// float( x ) / float( D ) for an integer 0 <= x <= D, without a divide: the
+// product with the rounded reciprocal and one FMA correction step.  For
+// D = 255 and D = 65535 it equals the correctly rounded quotient for every x
+// (an exhaustive check over both ranges is part of the CPU tests), so the
+// random numbers stay bit-exact while costing three instructions, not ten.
+template< uint32_t D >
+RG_HD float unitQuotient( uint32_t x )
+{
+    constexpr float C  = 1.0f / float( D );
+    const float     xf = float( x );
+    const float     q  = xf * C;
+    const float     r  = fmaf( -q, float( D ), xf );
+    return fmaf( r, C, q );
+}
+
 RG_HD float rnd16( uint32_t seed, uint32_t salt )
 {
     uint32_t r = wellonsLowBias32( seed + salt );
-    return float( r & 0xFFFFu ) / 65535.0f;
+    return unitQuotient< 65535u >( r & 0xFFFFu );
 }
 
 RG_HD void rnd16_2( uint32_t seed, uint32_t salt, float* a, float* b )
 {
     uint32_t r = wellonsLowBias32( seed + salt );
-    *a         = float( r & 0xFFFFu ) / 65535.0f;
-    *b         = float( r >> 16 ) / 65535.0f;
+    *a         = unitQuotient< 65535u >( r & 0xFFFFu );
+    *b         = unitQuotient< 65535u >( r >> 16 );
 }
 
 RG_HD void rnd8_4( uint32_t seed, uint32_t salt, float out[ 4 ] )
 {
     uint32_t r = wellonsLowBias32( seed + salt );
-    out[ 0 ]   = float( r & 0xFFu ) / 255.0f;
-    out[ 1 ]   = float( ( r >> 8 ) & 0xFFu ) / 255.0f;
-    out[ 2 ]   = float( ( r >> 16 ) & 0xFFu ) / 255.0f;
-    out[ 3 ]   = float( r >> 24 ) / 255.0f;
+    out[ 0 ]   = unitQuotient< 255u >( r & 0xFFu );
+    out[ 1 ]   = unitQuotient< 255u >( ( r >> 8 ) & 0xFFu );
+    out[ 2 ]   = unitQuotient< 255u >( ( r >> 16 ) & 0xFFu );
+    out[ 3 ]   = unitQuotient< 255u >( r >> 24 );
 }
 
 RG_HD void murmurHash33( uint32_t sx, uint32_t sy, uint32_t sz, uint32_t out[ 3 ] )

@@ -91,7 +91,7 @@ __device__ __forceinline__ T* Plane( const FbPtrs& P, int plane, bool prev = fal
 __device__ __forceinline__ v3 DecodeNormal( uint32_t p )
 {
     float sp, cp, st, ct;
-    sincospif( float( p >> 16 ) / 65535.0f, &sp, &cp );
+    sincospif( unitQuotient< 65535u >( p >> 16 ), &sp, &cp );
     sincospif( float( p & 0xFFFFu ) / 32768.0f, &st, &ct );
     return mk3( sp * ct, sp * st, cp );
 }
@@ -157,7 +157,7 @@ __device__ __forceinline__ v3 GetRayDir( const RgExtFrameParams& p, float u, flo
 
 __device__ __forceinline__ v3 UnpackColor( uint32_t c )
 {
-    return mk3( float( c & 0xFFu ) / 255.0f, float( ( c >> 8 ) & 0xFFu ) / 255.0f, float( ( c >> 16 ) & 0xFFu ) / 255.0f );
+    return mk3( unitQuotient< 255u >( c & 0xFFu ), unitQuotient< 255u >( ( c >> 8 ) & 0xFFu ), unitQuotient< 255u >( ( c >> 16 ) & 0xFFu ) );
 }
 
 // getSky / adjustSky (RaygenCommon.h:272-306) for RG_SKY_TYPE_COLOR

@@ -58,18 +58,23 @@ RG_HD v3 safeNormalize( v3 v )
     return len > 0.001f ? v / len : mk3( 0.0f, 1.0f, 0.0f );
 }
 
-// Hardware reciprocal and reciprocal square root (v_rcp_f32 / v_rsq_f32,
-// 1 ulp) for the light sampling and BRDF evaluated per RIS candidate: a
-// correctly rounded divide is about ten instructions, and the candidate loop
-// runs up to 32 times per pixel.  Zero, infinite and NaN inputs give the
-// same specials as the division they replace.
+// Hardware reciprocal, square root and reciprocal square root (v_rcp_f32,
+// v_sqrt_f32, v_rsq_f32: 1 ulp) for the light sampling and BRDF evaluated
+// per RIS candidate: a correctly rounded divide or square root is about ten
+// instructions, and the candidate loop runs up to 32 times per pixel.  Zero,
+// infinite and NaN inputs give the same specials as the operations they
+// replace.
 __device__ __forceinline__ float fastRcp( float x )
 {
     return __builtin_amdgcn_rcpf( x );
 }
+__device__ __forceinline__ float fastSqrt( float x )
+{
+    return __builtin_amdgcn_sqrtf( x );
+}
 __device__ __forceinline__ float fastLength( v3 a )
 {
-    return sqrtf( dot( a, a ) );
+    return fastSqrt( dot( a, a ) );
 }
 __device__ __forceinline__ v3 fastNormalize( v3 a )
 {
@@ -255,7 +260,7 @@ __device__ __forceinline__ float safeSolidAngle( float a )
 __device__ __forceinline__ float solidAngleForSphere( float radius, float dist )
 {
     const float sinT = radius >= dist ? 1.0f : radius * fastRcp( dist );
-    const float cosT = sqrtf( 1.0f - sinT * sinT );
+    const float cosT = fastSqrt( 1.0f - sinT * sinT );
     return safeSolidAngle( 2.0f * RG_PI * ( 1.0f - cosT ) );
 }
 __device__ __forceinline__ float lightColorWeight( v3 c )
