@@ -142,7 +142,6 @@ __device__ __forceinline__ v3 sampleOrientedHemisphere( v3 n, float u1, float u2
 // each term is the same expression the warps evaluate.
 struct LightPoint
 {
-    float pu, pv;
     float diskR, diskCos, diskSin;      // sampleDisk( 1, pu, pv ): sqrt(0.99 u1), cos / sin(2 pi 0.99 u2)
     float hemiX, hemiY, hemiZ;          // sampleOrientedHemisphere( n, pu, pv ) = normalize( n + hemi )
     float triU1, triU2;                 // the triangle light's 0.99-scaled pair
@@ -151,8 +150,6 @@ struct LightPoint
 __device__ __forceinline__ LightPoint MakeLightPoint( float pu, float pv )
 {
     LightPoint lp;
-    lp.pu = pu;
-    lp.pv = pv;
     {
         const float u1 = pu * 0.99f, u2 = pv * 0.99f;
         const float phi = 2.0f * RG_PI * u2;
