@@ -711,7 +711,7 @@ __device__ void AtrousPixel( const FbPtrs& P, const Dims& d, int x, int y, int k
     nws *= clampf( hSpec / 8.0f, 0.0f, 1.0f );
 
     float ws = 1.0f, wss = 1.0f, wsi = 1.0f;
-#pragma unroll
+#pragma unroll 1
     for( int yy = -1; yy <= 1; yy++ )
     {
 #pragma unroll
