@@ -387,7 +387,17 @@ typedef struct RgExtFrameParams
     float    acidColorAndDensity[ 4 ];  /* (0.0, 0.66, 0.55, 10)            */
     float    worldUp[ 4 ];              /* normalised RgInstanceCreateInfo::worldUp */
     float    emissionMaxScreenColor;    /* RgDrawFrameTexturesParams, 1.5     */
-    float    pad1[ 3 ];
+
+    /* Ray cull masks (Source/VulkanDevice.cpp:344-393, RaygenCommon.h:103-177):
+     * INSTANCE_MASK_WORLD_0/1/2 bits (1, 2, 4) of RgDrawFrameInfo::
+     * rayCullMaskWorld; world instances outside it are not traced at all. */
+    uint32_t rayCullMaskWorld;           /* default 7 (every world)                        */
+    uint32_t rayCullMaskWorldShadow;     /* the same without world 2 when sky geometry is allowed */
+    uint32_t lightIndexIgnoreFPVShadows; /* light whose shadow rays skip first-person-viewer
+                                            geometry (VulkanDevice.cpp:288-289); 32767: none */
+    uint32_t skyGeometry;                /* RgInstanceCreateInfo::allowGeometryWithSkyFlag:
+                                            hits on world-2 geometry read as sky (RaygenCommon.h:191-205) */
+    uint32_t pad2[ 3 ];
 } RgExtFrameParams;
 
 /* Fills an RgExtFrameParams with the reference defaults

@@ -215,13 +215,14 @@ Light EncodeSphere( const RgSphericalLightUploadInfo& i )
 
 struct Surface
 {
-    D3     position{ 0, 0, 0 };
-    D3     normal{ 0, 0, 0 };
-    double roughness = 1;
-    D3     albedo{ 0, 0, 0 };
-    bool   isSky     = true;
-    D3     specColor{ 0, 0, 0 };
-    D3     toViewer{ 0, 0, 0 };
+    D3       position{ 0, 0, 0 };
+    D3       normal{ 0, 0, 0 };
+    double   roughness = 1;
+    D3       albedo{ 0, 0, 0 };
+    bool     isSky     = true;
+    D3       specColor{ 0, 0, 0 };
+    D3       toViewer{ 0, 0, 0 };
+    uint32_t custom    = 1; // instanceCustomIndex of the surface's instance (Surface.inl:31)
 };
 struct LSample
 {
@@ -473,6 +474,8 @@ struct OrPathTracer
 namespace
 {
 
+bool IsNull( const RgExtRay& r );
+
 struct Ctx
 {
     const OrPtScene*       sc;
@@ -496,45 +499,134 @@ const Light& LightAt( const Ctx& c, uint32_t i )
     return i < c.lights.size() ? c.lights[ i ] : none;
 }
 
-// allGeometry: primary and reflection / refraction rays; the others skip the
-// refracting triangles (INSTANCE_MASK_REFRACT, RaygenCommon.h:103-170)
+// ---- instances and cull masks
+// The reference puts each primitive into a BLAS by its flags
+// (VertexCollectorFilterTypeFlags_GetForGeometry,
+// VertexCollectorFilterType.cpp:200-257) and gives that BLAS's TLAS instance
+// a mask and custom-index flags (ASManager::SetupTLASInstanceFromBLAS,
+// ASManager.cpp:837-963).  Here each triangle is tested against the ray's
+// cull mask on its own, over one tree of all triangles: a triangle is hit
+// only if its instance is in the TLAS and (instance mask & cull mask) != 0.
+const uint32_t M_WORLD_0 = 1, M_WORLD_1 = 2, M_WORLD_2 = 4, M_REFRACT = 32, M_FP = 64, M_FPV = 128; // ShaderCommonC.h:64-71
+const uint32_t CI_DYNAMIC = 1, CI_FP = 2, CI_FPV = 4, CI_SKY = 8;                                  // ShaderCommonC.h:60-63
+
+// The instance mask of a primitive's TLAS instance; *inTlas is false when
+// the world it belongs to is not in rayCullMaskWorld.
+uint32_t InstanceMask( uint32_t flags, uint32_t worldMask, bool* inTlas )
+{
+    *inTlas = true;
+    if( flags & RG_MESH_PRIMITIVE_FIRST_PERSON ) return M_FP;
+    if( flags & RG_MESH_PRIMITIVE_FIRST_PERSON_VIEWER ) return M_FPV;
+    const uint32_t world = ( flags & RG_MESH_PRIMITIVE_SKY_VISIBILITY ) ? M_WORLD_2 : M_WORLD_0;
+    *inTlas              = ( worldMask & world ) != 0;
+    const bool alpha     = ( flags & RG_MESH_PRIMITIVE_ALPHA_TESTED ) != 0;
+    const bool refract   = !alpha && ( flags & ( RG_MESH_PRIMITIVE_WATER | RG_MESH_PRIMITIVE_GLASS | RG_MESH_PRIMITIVE_GLASS_IF_SMOOTH |
+                                                 RG_MESH_PRIMITIVE_ACID ) ) != 0;
+    return refract ? M_REFRACT : world;
+}
+
+uint32_t CustomIndex( uint32_t flags, bool skyGeometry )
+{
+    if( flags & RG_MESH_PRIMITIVE_FIRST_PERSON ) return CI_DYNAMIC | CI_FP;
+    if( flags & RG_MESH_PRIMITIVE_FIRST_PERSON_VIEWER ) return CI_DYNAMIC | CI_FPV;
+    if( skyGeometry && ( flags & RG_MESH_PRIMITIVE_SKY_VISIBILITY ) ) return CI_DYNAMIC | CI_SKY;
+    return CI_DYNAMIC;
+}
+
+uint32_t TriCustom( const Ctx& c, uint32_t prim )
+{
+    return CustomIndex( c.sc->geoms[ c.sc->triGeom[ prim ] ].flags, c.p.skyGeometry != 0 );
+}
+
+// doesPayloadContainHitInfo (RaygenCommon.h:191-205)
+bool IsSkyHit( const Ctx& c, const RgExtHit& h )
+{
+    return h.primitiveId == RG_EXT_MISS || ( TriCustom( c, h.primitiveId ) & CI_SKY ) != 0;
+}
+
+// getPrimaryVisibilityCullMask .. getIndirectIlluminationCullMask
+// (RaygenCommon.h:103-177) and traceVisibility's first-person-viewer
+// exception (:350-358)
+uint8_t PrimaryMask( const RgExtFrameParams& p ) { return uint8_t( p.rayCullMaskWorld | M_REFRACT | M_FP ); }
+uint8_t ReflRefrMask( const RgExtFrameParams& p, uint32_t custom, uint32_t flags, bool refraction )
+{
+    uint32_t w = p.rayCullMaskWorld | M_REFRACT;
+    if( flags & RG_MESH_PRIMITIVE_IGNORE_REFRACT_AFTER )
+    {
+        w &= ~M_REFRACT;
+        if( custom & CI_FP ) return uint8_t( w );
+    }
+    if( custom & CI_FP ) return uint8_t( w | M_FP );
+    return uint8_t( w | ( refraction ? M_FP : M_FPV ) );
+}
+uint8_t ShadowMask( const RgExtFrameParams& p, uint32_t custom, uint32_t lightIndex )
+{
+    uint32_t m = p.rayCullMaskWorldShadow;
+    if( custom & CI_FP ) m |= M_FP;
+    else m |= M_FPV; // first-person viewer and world surfaces alike
+    if( lightIndex == p.lightIndexIgnoreFPVShadows ) m &= ~M_FPV;
+    return uint8_t( m );
+}
+uint8_t IndirectMask( const RgExtFrameParams& p, uint32_t custom )
+{
+    return uint8_t( p.rayCullMaskWorld | ( ( custom & CI_FP ) ? M_FP : M_FPV ) );
+}
+
 // RtAlphaTest.rahit: alpha-tested primitives with a texture ignore hits
 // where the layer-0 colour times the base-level texel has
 // (r+g+b)/3 * a + a < 0.5.
-int AlphaAccept( const void* ctx, uint32_t prim, float u, float v )
+bool AlphaPass( const OrPtScene* sc, uint32_t prim, float u, float v )
 {
-    const OrPtScene* sc = static_cast< const OrPtScene* >( ctx );
-    const OrGeom&    g  = sc->geoms[ sc->triGeom[ prim ] ];
-    if( !( g.flags & RG_MESH_PRIMITIVE_ALPHA_TESTED ) || g.texture == 0 || g.texture > sc->textureCount ) return 1;
+    const OrGeom& g = sc->geoms[ sc->triGeom[ prim ] ];
+    if( !( g.flags & RG_MESH_PRIMITIVE_ALPHA_TESTED ) || g.texture == 0 || g.texture > sc->textureCount ) return true;
     const float* t  = sc->texcoords + size_t( prim ) * 6;
     const double b0 = 1.0 - double( u ) - double( v );
     double       c[ 4 ];
     or_texture_sample_lod( sc->textures[ g.texture - 1 ], t[ 0 ] * b0 + t[ 2 ] * u + t[ 4 ] * v, t[ 1 ] * b0 + t[ 3 ] * u + t[ 5 ] * v, 0.0, c );
     const double r = c[ 0 ] * ( g.color & 0xFFu ) / 255.0, gg = c[ 1 ] * ( ( g.color >> 8 ) & 0xFFu ) / 255.0;
     const double b = c[ 2 ] * ( ( g.color >> 16 ) & 0xFFu ) / 255.0, a = c[ 3 ] * ( ( g.color >> 24 ) & 0xFFu ) / 255.0;
-    return ( r + gg + b ) / 3.0 * a + a >= 0.5 ? 1 : 0;
+    return ( r + gg + b ) / 3.0 * a + a >= 0.5;
 }
 
-void Trace( const Ctx& c, std::vector< RgExtRay >& rays, std::vector< RgExtHit >& hits, uint32_t flags, bool allGeometry = false )
+struct FilterCtx
+{
+    const OrPtScene* sc;
+    uint32_t         worldMask, rayMask;
+};
+
+int MaskedAccept( const void* ctx, uint32_t prim, float u, float v )
+{
+    const FilterCtx* f = static_cast< const FilterCtx* >( ctx );
+    bool             inTlas;
+    const uint32_t   m = InstanceMask( f->sc->geoms[ f->sc->triGeom[ prim ] ].flags, f->worldMask, &inTlas );
+    if( !inTlas || ( m & f->rayMask ) == 0 ) return 0;
+    return AlphaPass( f->sc, prim, u, v ) ? 1 : 0;
+}
+
+// Traces rays[i] with cull mask masks[i] (a group per distinct mask).
+void Trace( const Ctx& c, std::vector< RgExtRay >& rays, const std::vector< uint8_t >& masks, std::vector< RgExtHit >& hits, uint32_t flags )
 {
     hits.resize( rays.size() );
-    const OrHitFilter filter = c.sc->textures ? AlphaAccept : nullptr;
-    if( !allGeometry && c.sc->bvhOpaque )
+    for( auto& h : hits )
     {
-        or_trace_rays_filtered( c.sc->bvhOpaque, rays.data(), uint32_t( rays.size() ), hits.data(), flags | c.traceFlags, c.threads, nullptr,
-                                nullptr, filter, c.sc );
-        return;
+        h.t = 0, h.u = h.v = 0, h.primitiveId = RG_EXT_MISS;
     }
-    if( c.sc->triCount == 0 || ( !allGeometry && c.sc->opaqueNone ) )
+    if( c.sc->triCount == 0 ) return;
+    std::vector< std::vector< uint32_t > > groups( 256 );
+    for( uint32_t i = 0; i < rays.size(); i++ )
     {
-        for( auto& h : hits )
-        {
-            h.t = 0, h.u = h.v = 0, h.primitiveId = RG_EXT_MISS;
-        }
-        return;
+        if( !IsNull( rays[ i ] ) ) groups[ masks[ i ] ].push_back( i );
     }
-    or_trace_rays_filtered( c.bvh, rays.data(), uint32_t( rays.size() ), hits.data(), flags | c.traceFlags, c.threads, nullptr, nullptr, filter,
-                            c.sc );
+    for( uint32_t m = 0; m < 256; m++ )
+    {
+        if( groups[ m ].empty() ) continue;
+        std::vector< RgExtRay > rr( groups[ m ].size() );
+        std::vector< RgExtHit > hh( groups[ m ].size() );
+        for( size_t k = 0; k < rr.size(); k++ ) rr[ k ] = rays[ groups[ m ][ k ] ];
+        FilterCtx f{ c.sc, c.p.rayCullMaskWorld, m };
+        or_trace_rays_filtered( c.bvh, rr.data(), uint32_t( rr.size() ), hh.data(), flags | c.traceFlags, c.threads, nullptr, nullptr, MaskedAccept, &f );
+        for( size_t k = 0; k < rr.size(); k++ ) hits[ groups[ m ][ k ] ] = hh[ k ];
+    }
 }
 
 RgExtRay MakeRay( D3 o, D3 d, double tmin, double tmax )
@@ -749,9 +841,10 @@ Hit GetHit( const Ctx& c, uint32_t prim, double u, double v, D3 rayOrigin, const
     return h;
 }
 
-Surface HitToSurface( const Hit& h, D3 rayDir )
+Surface HitToSurface( const Ctx& c, const Hit& h, D3 rayDir )
 {
     Surface s;
+    s.custom    = CustomIndex( c.sc->geoms[ h.geom ].flags, c.p.skyGeometry != 0 );
     s.position  = h.position;
     s.normal    = h.normal;
     s.roughness = h.roughness;
@@ -775,6 +868,7 @@ Surface FetchSurface( const Ctx& c, int x, int y )
     s.roughness = c.fb.get( RG_EXT_PLANE_METALLIC_ROUGHNESS, x, y, 1 );
     s.normal    = DecodeNormal( c.fb.get( RG_EXT_PLANE_NORMAL, x, y, 0 ) );
     s.toViewer  = -mk( c.fb.get( RG_EXT_PLANE_VIEW_DIRECTION, x, y, 0 ), c.fb.get( RG_EXT_PLANE_VIEW_DIRECTION, x, y, 1 ), c.fb.get( RG_EXT_PLANE_VIEW_DIRECTION, x, y, 2 ) );
+    s.custom    = uint32_t( c.fb.get( RG_EXT_PLANE_SURFACE_POSITION, x, y, 3 ) ); // the custom index as a number
     return s;
 }
 
@@ -846,6 +940,7 @@ struct RrPix
 
 void ReflectRefract( Ctx& c, std::vector< RgExtRay >& rays, std::vector< RgExtHit >& hits )
 {
+    std::vector< uint8_t > masks( rays.size(), 0 );
     const RgExtFrameParams& p = c.p;
     const Fb&               fb = c.fb;
     const int               W = fb.W, H = fb.H;
@@ -869,7 +964,7 @@ void ReflectRefract( Ctx& c, std::vector< RgExtRay >& rays, std::vector< RgExtHi
                 RrPix&   s = st[ i ];
                 D3       pos, nrm, alb;
                 double   rough, metal;
-                uint32_t flags;
+                uint32_t flags, custom;
                 if( depth == 0 )
                 {
                     s.active = false;
@@ -882,7 +977,8 @@ void ReflectRefract( Ctx& c, std::vector< RgExtRay >& rays, std::vector< RgExtHi
                     nrm   = DecodeNormal( fb.get( RG_EXT_PLANE_NORMAL, cx, cy, 0 ) );
                     metal = fb.get( RG_EXT_PLANE_METALLIC_ROUGHNESS, cx, cy, 0 );
                     rough = fb.get( RG_EXT_PLANE_METALLIC_ROUGHNESS, cx, cy, 1 );
-                    flags = c.sc->geoms[ c.sc->triGeom[ hits[ i ].primitiveId ] ].flags;
+                    flags  = c.sc->geoms[ c.sc->triGeom[ hits[ i ].primitiveId ] ].flags;
+                    custom = uint32_t( fb.get( RG_EXT_PLANE_SURFACE_POSITION, cx, cy, 3 ) );
                     s.thr   = mk( fb.get( RG_EXT_PLANE_THROUGHPUT, cx, cy, 0 ), fb.get( RG_EXT_PLANE_THROUGHPUT, cx, cy, 1 ),
                                   fb.get( RG_EXT_PLANE_THROUGHPUT, cx, cy, 2 ) );
                     s.dir   = RayDir( p, ( double( x ) + 0.5 + p.jitterX ) / W, ( double( y ) + 0.5 + p.jitterY ) / H );
@@ -907,7 +1003,8 @@ void ReflectRefract( Ctx& c, std::vector< RgExtRay >& rays, std::vector< RgExtHi
                     cq.mode = TQ_CONE, cq.a = s.coneLast, cq.dir = s.dir;
                     const Hit       h = GetHit( c, hits[ i ].primitiveId, hits[ i ].u, hits[ i ].v, mk( r.origin[ 0 ], r.origin[ 1 ], r.origin[ 2 ] ), cq );
                     pos = h.position, nrm = h.normal, alb = h.albedo, rough = h.roughness, metal = h.metallic;
-                    flags = c.sc->geoms[ h.geom ].flags;
+                    flags  = c.sc->geoms[ h.geom ].flags;
+                    custom = CustomIndex( flags, p.skyGeometry != 0 );
                 }
                 // getNewRayMedia (RaygenPrimary.inl:141-150)
                 uint32_t   newMedia = ( depth == 0 && p.cameraMediaType != MEDIA_VACUUM ) ? MEDIA_VACUUM : MediaOf( flags, rough );
@@ -986,11 +1083,12 @@ void ReflectRefract( Ctx& c, std::vector< RgExtRay >& rays, std::vector< RgExtHi
                     s.thr   = s.thr * 2.0;
                     s.split = true;
                 }
-                s.active  = true;
-                rays[ i ] = MakeRay( origin, s.dir, 0.001, p.rayLength );
+                s.active   = true;
+                rays[ i ]  = MakeRay( origin, s.dir, 0.001, p.rayLength );
+                masks[ i ] = ReflRefrMask( p, custom, flags, doRefr );
             }
         } );
-        Trace( c, rays, hits, RG_EXT_TRACE_CLOSEST, true );
+        Trace( c, rays, masks, hits, RG_EXT_TRACE_CLOSEST );
         ParallelFor( H, c.threads, [ & ]( int y ) {
             for( int x = 0; x < W; x++ )
             {
@@ -1000,7 +1098,7 @@ void ReflectRefract( Ctx& c, std::vector< RgExtRay >& rays, std::vector< RgExtHi
                 int cx, cy;
                 or_checkerboard_from_regular( x, y, W, &cx, &cy );
                 const double thrW = s.split ? 1.0 : -1.0;
-                if( hits[ i ].primitiveId == RG_EXT_MISS )
+                if( IsSkyHit( c, hits[ i ] ) )
                 {
                     const double a = std::fabs( dot( s.dir, up ) );
                     s.thr          = s.thr * Transmittance( p, s.media, 1.0 / ( a * a * a ) );
@@ -1059,7 +1157,8 @@ void ReflectRefract( Ctx& c, std::vector< RgExtRay >& rays, std::vector< RgExtHi
                 fb.set4( RG_EXT_PLANE_DEPTH_WORLD, cx, cy, s.fpl );
                 fb.set4( RG_EXT_PLANE_MOTION, cx, cy, ( cPrev.x / cwp ) * 0.5 - ( cCur.x / cwc ) * 0.5, ( cPrev.y / cwp ) * 0.5 - ( cCur.y / cwc ) * 0.5,
                          length( vPrev ) - length( vCur ), 0 );
-                fb.set4( RG_EXT_PLANE_SURFACE_POSITION, cx, cy, h.position.x, h.position.y, h.position.z, 0 );
+                fb.set4( RG_EXT_PLANE_SURFACE_POSITION, cx, cy, h.position.x, h.position.y, h.position.z,
+                         CustomIndex( c.sc->geoms[ h.geom ].flags, p.skyGeometry != 0 ) );
                 fb.set4( RG_EXT_PLANE_VISIBILITY_BUFFER, cx, cy, 1, double( ( h.localPrim << 12 ) | ( h.geom & 0xFFFu ) ), hits[ i ].u, hits[ i ].v );
                 fb.set4( RG_EXT_PLANE_VIEW_DIRECTION, cx, cy, s.dir.x, s.dir.y, s.dir.z, 0 );
                 fb.set4( RG_EXT_PLANE_THROUGHPUT, cx, cy, s.thr.x, s.thr.y, s.thr.z, thrW );
@@ -1238,7 +1337,8 @@ void RenderArea( int cx, int W, int* lo, int* hi )
 }
 
 // Direct light at a bounce hit: RIS without visibility (RaygenCommon.h:701-741)
-void IndirectDirect( const Ctx& c, const Surface& hs, uint32_t seed, int bounce, D3* diffOut, bool* isDir, bool* needShadow, D3* lightPos )
+void IndirectDirect( const Ctx& c, const Surface& hs, uint32_t seed, int bounce, D3* diffOut, bool* isDir, bool* needShadow, D3* lightPos,
+                     uint32_t* lightIndex = nullptr )
 {
     *diffOut = mk( 0, 0, 0 ), *needShadow = false, *isDir = false;
     if( !DirectValid( c, bounce ) ) return;
@@ -1257,6 +1357,7 @@ void IndirectDirect( const Ctx& c, const Surface& hs, uint32_t seed, int bounce,
     *isDir      = r.sel == 0u;
     *needShadow = uint32_t( bounce ) < c.p.maxBounceShadowsLights;
     *lightPos   = ls.position;
+    if( lightIndex ) *lightIndex = r.sel;
 }
 
 double Antilag( double g, double n ) { return clampd( n > 0.0001 ? clampd( std::fabs( g ) / n, 0.0, 1.0 ) : 0.0, 0.0, 1.0 ); }
@@ -1352,6 +1453,7 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
 
     std::vector< RgExtRay > rays( n ), rays2( n ), rays3( n );
     std::vector< RgExtHit > hits, hits2, hits3;
+    std::vector< uint8_t >  masks( n, PrimaryMask( p ) ), masks2( n, 0 ), masks3( n, 0 );
 
     // ---- primary rays (FP32, the kernel's operations) and G-buffer
     for( int y = 0; y < H; y++ )
@@ -1368,7 +1470,7 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
             r.tmin = p.primaryRayMinDist, r.tmax = p.rayLength;
         }
     }
-    Trace( c, rays, hits, RG_EXT_TRACE_CLOSEST, true );
+    Trace( c, rays, masks, hits, RG_EXT_TRACE_CLOSEST );
     ParallelFor( H, threads, [ & ]( int y ) {
         for( int x = 0; x < W; x++ )
         {
@@ -1378,7 +1480,7 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
             const double    u   = ( double( x ) + 0.5 + p.jitterX ) / W;
             const double    v   = ( double( y ) + 0.5 + p.jitterY ) / H;
             const D3        dir = RayDir( p, u, v );
-            if( hit.primitiveId == RG_EXT_MISS )
+            if( IsSkyHit( c, hit ) )
             {
                 const double cu = ( double( cx ) + 0.5 + p.jitterX ) / W, cv = ( double( cy ) + 0.5 + p.jitterY ) / H;
                 const D3     rd = RayDir( p, cu, cv );
@@ -1452,7 +1554,7 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
             fb.set4( RG_EXT_PLANE_DEPTH_WORLD, cx, cy, depth );
             fb.set4( RG_EXT_PLANE_DEPTH_GRAD, cx, cy, std::sqrt( gx * gx + gy * gy ) );
             fb.set4( RG_EXT_PLANE_MOTION, cx, cy, sxp - sxc, syp - syc, length( vPrev ) - depth, 0 );
-            fb.set4( RG_EXT_PLANE_SURFACE_POSITION, cx, cy, h.position.x, h.position.y, h.position.z, 0 );
+            fb.set4( RG_EXT_PLANE_SURFACE_POSITION, cx, cy, h.position.x, h.position.y, h.position.z, CustomIndex( gr.flags, p.skyGeometry != 0 ) );
             fb.set4( RG_EXT_PLANE_VISIBILITY_BUFFER, cx, cy, 1, double( ( h.localPrim << 12 ) | ( h.geom & 0xFFFu ) ), hit.u, hit.v );
             fb.set4( RG_EXT_PLANE_VIEW_DIRECTION, cx, cy, dir.x, dir.y, dir.z, 0 );
             // the camera's medium attenuates the primary segment; screen
@@ -1497,13 +1599,14 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
             pendSalt[ i ] = salt;
             if( valid( dir ) )
             {
-                rays[ i ] = ShadowRay( s, sampleLight( c.lights[ 0 ], s.position, pu, pv ).position );
+                rays[ i ]  = ShadowRay( s, sampleLight( c.lights[ 0 ], s.position, pu, pv ).position );
+                masks[ i ] = ShadowMask( p, s.custom, 0u );
             }
         }
     } );
     if( p.directionalLightExists )
     {
-        Trace( c, rays, hits, RG_EXT_TRACE_ANY_HIT );
+        Trace( c, rays, masks, hits, RG_EXT_TRACE_ANY_HIT );
     }
     ParallelFor( H, threads, [ & ]( int y ) {
         for( int x = 0; x < W; x++ )
@@ -1597,8 +1700,9 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
                     }
                     else if( 0u < p.maxBounceShadowsLights )
                     {
-                        rays[ i ] = ShadowRay( s, ls.position );
-                        dist      = length( ls.position - s.position );
+                        rays[ i ]  = ShadowRay( s, ls.position );
+                        masks[ i ] = ShadowMask( p, s.custom, res.sel );
+                        dist       = length( ls.position - s.position );
                     }
                 }
             }
@@ -1606,7 +1710,7 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
             pD[ i ] = diff, pS[ i ] = spec, pDist[ i ] = dist;
         }
     } );
-    Trace( c, rays, hits, RG_EXT_TRACE_ANY_HIT );
+    Trace( c, rays, masks, hits, RG_EXT_TRACE_ANY_HIT );
     ParallelFor( H, threads, [ & ]( int y ) {
         for( int x = 0; x < W; x++ )
         {
@@ -1650,10 +1754,11 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
             double   opdf;
             const D3 d = sampleSmithGGX( s.normal, s.toViewer, s.roughness, bn[ 0 ], bn[ 1 ], &opdf, true );
             rays[ i ]  = MakeRay( s.position + s.normal * 0.01, d, 0.001, p.rayLength );
+            masks[ i ] = IndirectMask( p, s.custom );
             b1dir[ i ] = d, b1pdf[ i ] = opdf;
         }
     } );
-    Trace( c, rays, hits, RG_EXT_TRACE_CLOSEST );
+    Trace( c, rays, masks, hits, RG_EXT_TRACE_CLOSEST );
     ParallelFor( H, threads, [ & ]( int y ) {
         for( int x = 0; x < W; x++ )
         {
@@ -1665,7 +1770,7 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
             const RgExtHit hit = hits[ i ];
             const D3       dir = mk( r.direction[ 0 ], r.direction[ 1 ], r.direction[ 2 ] );
             const D3       org = mk( r.origin[ 0 ], r.origin[ 1 ], r.origin[ 2 ] );
-            if( hit.primitiveId == RG_EXT_MISS )
+            if( IsSkyHit( c, hit ) )
             {
                 b1sky[ i ] = 1;
                 continue;
@@ -1674,23 +1779,29 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
             bq.mode             = TQ_BOUNCE; // FIRST_BOUNCE_MIP_BIAS 0 (RtRaygenIndirect.inl:63)
             bq.a                = fb.get( RG_EXT_PLANE_METALLIC_ROUGHNESS, x, y, 1 );
             const Hit      h    = GetHit( c, hit.primitiveId, hit.u, hit.v, org, bq );
-            const Surface  hs   = HitToSurface( h, dir );
+            const Surface  hs   = HitToSurface( c, h, dir );
             const uint32_t seed = seedOf( x, y, p.frameId );
             D3             diff, lp = mk( 0, 0, 0 );
             bool           isDir, needShadow;
-            IndirectDirect( c, hs, seed, 1, &diff, &isDir, &needShadow, &lp );
-            if( needShadow ) rays2[ i ] = ShadowRay( hs, lp );
+            uint32_t       lightIndex = LIGHT_NONE;
+            IndirectDirect( c, hs, seed, 1, &diff, &isDir, &needShadow, &lp, &lightIndex );
+            if( needShadow )
+            {
+                rays2[ i ]  = ShadowRay( hs, lp );
+                masks2[ i ] = ShadowMask( p, hs.custom, lightIndex );
+            }
             double bn[ 4 ];
             BlueNoise8( pt, seed, SALT_DIFF_BOUNCE_BASE + 2, bn );
             double   pdf2;
             const D3 d2 = sampleOrientedHemisphere( hs.normal, bn[ 0 ], bn[ 1 ], &pdf2 );
             rays3[ i ]  = MakeRay( hs.position + hs.normal * 0.01, d2, 0.001, p.rayLength );
+            masks3[ i ] = IndirectMask( p, hs.custom );
             hpos[ i ] = h.position, hnrm[ i ] = h.normal, halb[ i ] = h.albedo, hdiff[ i ] = diff;
             hIsDir[ i ] = isDir, hShadow[ i ] = needShadow, b2dir[ i ] = d2, b2pdf[ i ] = pdf2;
         }
     } );
-    Trace( c, rays2, hits2, RG_EXT_TRACE_ANY_HIT );
-    Trace( c, rays3, hits3, RG_EXT_TRACE_CLOSEST );
+    Trace( c, rays2, masks2, hits2, RG_EXT_TRACE_ANY_HIT );
+    Trace( c, rays3, masks3, hits3, RG_EXT_TRACE_CLOSEST );
     // initial samples (RtRaygenIndirect.inl:221-240); positions / radiance kept in FP64
     std::vector< D3 >     sPos( n ), sNrm( n ), sRad( n );
     std::vector< double > sPdf( n );
@@ -1725,7 +1836,7 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
                 const D3       emis = hi1.emission * double( p.emissionMapBoost );
                 D3             second;
                 const RgExtHit h2 = hits3[ i ];
-                if( h2.primitiveId == RG_EXT_MISS )
+                if( IsSkyHit( c, h2 ) )
                 {
                     second = SkyColor( p ) * b2pdf[ i ];
                 }
@@ -1737,7 +1848,7 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
                     q2.a               = hi1.roughness;
                     q2.b               = 32.0; // SECOND_BOUNCE_MIP_BIAS (RtRaygenIndirect.inl:64)
                     const Hit      hi2 = GetHit( c, h2.primitiveId, h2.u, h2.v, mk( br.origin[ 0 ], br.origin[ 1 ], br.origin[ 2 ] ), q2 );
-                    const Surface  hs2 = HitToSurface( hi2, b2dir[ i ] );
+                    const Surface  hs2 = HitToSurface( c, hi2, b2dir[ i ] );
                     D3             diff2, lp;
                     bool           isDir2, ns2;
                     IndirectDirect( c, hs2, seedOf( x, y, p.frameId ), 2, &diff2, &isDir2, &ns2, &lp );
@@ -1917,6 +2028,7 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
 
     // ---- gradient samples (RtGradients.rgen:44-181)
     std::vector< RgExtRay > grays( size_t( GW ) * GH, NullRay() );
+    std::vector< uint8_t >  gmasks( size_t( GW ) * GH, 0 );
     std::vector< RgExtHit > ghits;
     struct GP
     {
@@ -1975,6 +2087,7 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
             ps.specColor = SpecColor( ps.albedo, fb.get( RG_EXT_PLANE_METALLIC_ROUGHNESS, psx, psy, 0, true ) );
             ps.roughness = fb.get( RG_EXT_PLANE_METALLIC_ROUGHNESS, psx, psy, 1, true );
             ps.normal    = DecodeNormal( fb.get( RG_EXT_PLANE_NORMAL, psx, psy, 0, true ) );
+            ps.custom    = uint32_t( fb.get( RG_EXT_PLANE_SURFACE_POSITION, psx, psy, 3, true ) );
             bool matched = false;
             if( fb.get( RG_EXT_PLANE_VISIBILITY_BUFFER, psx, psy, 0, true ) == 1.0 )
             {
@@ -2010,8 +2123,9 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
                             g.lumD = lum( d ), g.lumS = lum( sp );
                             if( 0u < p.maxBounceShadowsLights )
                             {
-                                grays[ gi ] = ShadowRay( ps, ls.position );
-                                g.state     = 2;
+                                grays[ gi ]  = ShadowRay( ps, ls.position );
+                                gmasks[ gi ] = ShadowMask( p, ps.custom, cur );
+                                g.state      = 2;
                             }
                         }
                     }
@@ -2020,7 +2134,7 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
             g.prevD = pl[ 0 ], g.prevS = pl[ 1 ], g.psx = psx, g.psy = psy, g.strata = curStrata;
         }
     } );
-    Trace( c, grays, ghits, RG_EXT_TRACE_ANY_HIT );
+    Trace( c, grays, gmasks, ghits, RG_EXT_TRACE_ANY_HIT );
     for( int gy = 0; gy < GH; gy++ )
     {
         for( int gx = 0; gx < GW; gx++ )
@@ -2048,6 +2162,28 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
                      mixd( gr[ 1 ], fb.get( RG_EXT_PLANE_DIS_GRADIENT_HISTORY, hx, hy, 1 ), 0.5 ),
                      mixd( gr[ 2 ], fb.get( RG_EXT_PLANE_DIS_GRADIENT_HISTORY, hx, hy, 2 ), 0.5 ), 0 );
             fb.set4( RG_EXT_PLANE_GRADIENT_PREV_PIX, gx, gy, double( g.strata ) );
+        }
+    }
+}
+
+// VertexPreprocessPartial.inl:62-120 (GEOM_INST_FLAG_GENERATE_NORMALS with
+// normalSign +1): one pass over the triangles in index order.
+extern "C" void or_flat_normals( const float* positions, uint32_t vertexCount, const uint32_t* indices, uint32_t indexCount, float* normals )
+{
+    const uint32_t triCount = indices ? indexCount / 3 : vertexCount / 3;
+    for( uint32_t t = 0; t < triCount; t++ )
+    {
+        uint32_t v[ 3 ];
+        for( int k = 0; k < 3; k++ ) v[ k ] = indices ? indices[ 3 * t + k ] : 3 * t + k;
+        const D3 p0 = mk( positions[ 3 * v[ 0 ] ], positions[ 3 * v[ 0 ] + 1 ], positions[ 3 * v[ 0 ] + 2 ] );
+        const D3 p1 = mk( positions[ 3 * v[ 1 ] ], positions[ 3 * v[ 1 ] + 1 ], positions[ 3 * v[ 1 ] + 2 ] );
+        const D3 p2 = mk( positions[ 3 * v[ 2 ] ], positions[ 3 * v[ 2 ] + 1 ], positions[ 3 * v[ 2 ] + 2 ] );
+        const D3 n  = normalize( cross( p1 - p0, p2 - p0 ) );
+        for( int k = 0; k < 3; k++ )
+        {
+            normals[ 3 * v[ k ] + 0 ] = float( n.x );
+            normals[ 3 * v[ k ] + 1 ] = float( n.y );
+            normals[ 3 * v[ k ] + 2 ] = float( n.z );
         }
     }
 }

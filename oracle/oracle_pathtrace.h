@@ -40,16 +40,20 @@ typedef struct OrPtScene
     const RgDirectionalLightUploadInfo* dirLight; /* may be null */
     const RgSphericalLightUploadInfo*   spheres;
     uint32_t                            sphereCount;
-    /* Refracting geometry (water, glass, acid) occupies the last triangles;
-     * shadow, bounce and gradient rays trace bvhOpaque, a tree over the
-     * others (null: they trace bvh), or miss everything with opaqueNone. */
-    const struct OrBvh*                 bvhOpaque;
-    uint32_t                            opaqueNone;
     /* Albedo textures and per-triangle texture coordinates (6 floats). */
     const struct OrTexture* const*      textures;
     uint32_t                            textureCount;
     const float*                        texcoords;
 } OrPtScene;
+
+/* Flat normals of one primitive (CmVertexPreprocess.comp with
+ * VertexPreprocessPartial.inl:43-120): for each triangle in index order,
+ * normalize(cross(p1 - p0, p2 - p0)) of its positions is written to its
+ * three vertices, so a vertex shared by several triangles keeps the last
+ * one's normal.  positions and normals hold vertexCount * 3 floats (normals
+ * in and out: vertices no triangle uses keep theirs); indices null for a
+ * non-indexed primitive. */
+void or_flat_normals( const float* positions, uint32_t vertexCount, const uint32_t* indices, uint32_t indexCount, float* normals );
 
 typedef struct OrPathTracer OrPathTracer;
 OrPathTracer* or_pt_create( int width, int height );

@@ -465,6 +465,10 @@ RgResult rgExtFrameParamsDefaults( uint32_t width, uint32_t height, RgExtFramePa
         p->normalMapStrength = 1.0f; // DrawFrameInfo.h:175
         p->waterWaveStrength = 1.0f; // DrawFrameInfo.h:138
         p->emissionMaxScreenColor = 1.5f; // DrawFrameInfo.h:177
+        p->rayCullMaskWorld           = 7u; // every world
+        p->rayCullMaskWorldShadow     = 7u;
+        p->lightIndexIgnoreFPVShadows = 32767u; // LIGHT_INDEX_NONE
+        p->skyGeometry                = 0u;
     } );
 }
 

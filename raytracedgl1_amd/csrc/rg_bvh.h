@@ -101,20 +101,19 @@ struct BvhObject
 
     uint32_t NodeStride() const { return nodeFormat == RG_EXT_BVH_NODE_FP16 ? uint32_t( sizeof( BvhNode8H ) ) : uint32_t( sizeof( BvhNode8 ) ); }
 
-    // Frame BVH with a refracting range (BuildFrameBvh): device words
-    // { top node, opaque root, 0 }.  FRAME_SINGLE: one tree, every ray type
-    // sees every triangle; FRAME_SPLIT: primary and reflection/refraction
-    // rays start at the top node over both subtrees, shadow and bounce rays
-    // at the opaque root; FRAME_REFRACT_ONLY: no opaque triangles, so
-    // shadow and bounce rays miss.
-    enum FrameMode : uint32_t
+    // Frame BVH (BuildFrameBvh): one segment per non-empty class of the
+    // frame's triangles, in class order.  A ray starts at the roots of the
+    // segments its cull mask admits, the way a TLAS instance is visited
+    // when (instance mask & cull mask) != 0.
+    struct FrameSegment
     {
-        FRAME_SINGLE       = 0,
-        FRAME_SPLIT        = 1,
-        FRAME_REFRACT_ONLY = 2,
+        uint32_t cls  = 0;       // the class (index into BuildFrameBvh's counts)
+        uint32_t root = 0;       // 8-wide node, or the leaf of a one-triangle segment
+        float    box[ 6 ] = {};  // lo x, y, z, hi x, y, z (infinite for a single segment)
     };
-    uint32_t     frameMode = FRAME_SINGLE;
-    DeviceBuffer frameRoots;
+    std::vector< FrameSegment > frameSegs;
+    DeviceBuffer                segBoxes;     // 8 floats per segment
+    std::vector< float >        hostSegBoxes;
 };
 
 void BuildBvh( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtBvhBuildParams& params, hipStream_t stream );
@@ -124,13 +123,11 @@ void BuildBvh( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtBvhBu
 void BuildBvhBatch( BvhObject& bvh, const float* d_tris, const uint32_t* offsets, uint32_t sceneCount, const RgExtBvhBuildParams& params,
                     hipStream_t stream );
 
-// The frame's BVH over n triangles whose last n - opaqueN refract (water,
-// glass, acid).  With at least two triangles on each side the two ranges
-// become separate subtrees of one batched build plus a top node over both
-// (FRAME_SPLIT), so rays that must not see refracting geometry start below
-// it (the reference's INSTANCE_MASK_REFRACT, RaygenCommon.h:103-170).
-// Otherwise a single tree (FRAME_SINGLE; FRAME_REFRACT_ONLY when opaqueN is 0).
-void BuildFrameBvh( BvhObject& bvh, const float* d_tris, uint32_t n, uint32_t opaqueN, const RgExtBvhBuildParams& params, hipStream_t stream );
+// The frame's BVH over the triangles of classTris[c] triangles per class,
+// classes in order.  One class: a single-scene build.  Several: one batched
+// build with a segment per non-empty class (1..8 of them).
+void BuildFrameBvh( BvhObject& bvh, const float* d_tris, const uint32_t* classTris, uint32_t classCount, const RgExtBvhBuildParams& params,
+                    hipStream_t stream );
 
 constexpr int TRACE_STACK_VARIANTS = 3;
 
@@ -162,13 +159,15 @@ struct TraceAccum
 // in ctx (read back by ReadTraceStats / ReadTraceErrors).
 void TraceRays( const BvhObject& bvh, TraceContext& ctx, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags, hipStream_t stream,
                 hipEvent_t evBefore = nullptr, hipEvent_t evAfter = nullptr, const TraceAccum* accum = nullptr );
-// TraceRays over a frame BVH (BuildFrameBvh); opaqueOnly rays skip the
-// refracting range.
+// TraceRays over a frame BVH (BuildFrameBvh).  classMasks[c] is the
+// instance mask of class c, 0 when the class is not in the frame's TLAS; a
+// ray sees the segments whose mask shares a bit with its own cull mask,
+// rayMasks[i] (or defaultMask for every ray when rayMasks is null).
 // alpha (with tex): per-triangle alpha-test records; the hits of
 // alpha-tested triangles whose texture is transparent there are ignored.
-void TraceRaysFrame( const BvhObject& bvh, TraceContext& ctx, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags, bool opaqueOnly,
-                     hipStream_t stream, hipEvent_t evBefore, hipEvent_t evAfter, const TraceAccum* accum,
-                     const struct AlphaRec* alpha = nullptr, const struct TexView* tex = nullptr );
+void TraceRaysFrame( const BvhObject& bvh, TraceContext& ctx, const RgExtRay* d_rays, const uint8_t* d_rayMasks, uint32_t defaultMask, uint32_t count,
+                     RgExtHit* d_hits, uint32_t flags, const uint32_t* classMasks, uint32_t classCount, hipStream_t stream, hipEvent_t evBefore,
+                     hipEvent_t evAfter, const TraceAccum* accum, const struct AlphaRec* alpha = nullptr, const struct TexView* tex = nullptr );
 // One persistent launch over the rays of every scene of a batched BVH: rays
 // of scene s are [rayOffsets[s], rayOffsets[s+1]) (host array, sceneCount + 1
 // entries); primitive ids in the hits are relative to the scene.

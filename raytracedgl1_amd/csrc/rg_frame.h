@@ -44,9 +44,11 @@ struct CameraState
 };
 
 // FillUniform's hot-path fields for this frame; the previous frame's camera
-// becomes the *Prev fields.
+// becomes the *Prev fields.  allowSkyGeometry: RgInstanceCreateInfo::
+// allowGeometryWithSkyFlag; ignoreFpvLightIndex: this frame's index of the
+// light whose shadows skip first-person-viewer geometry (LIGHT_INDEX_NONE).
 void FillFrameParams( CameraState& cs, const RgDrawFrameInfo& di, uint32_t w, uint32_t h, uint32_t lightArraySize, uint32_t prevLightArraySize,
-                      bool hasDirectional, bool cullBackFaces );
+                      bool hasDirectional, bool cullBackFaces, bool allowSkyGeometry = false, uint32_t ignoreFpvLightIndex = LIGHT_INDEX_NONE );
 
 // Device state of one camera's frames besides its framebuffers.
 struct FrameGpu

@@ -64,7 +64,7 @@ CameraState::CameraState()
 
 // FillUniform (Source/VulkanDevice.cpp:142-482), hot-path fields only.
 void FillFrameParams( CameraState& cs, const RgDrawFrameInfo& di, uint32_t w, uint32_t h, uint32_t lightArraySize, uint32_t prevLightArraySize,
-                      bool hasDirectional, bool cullBackFaces )
+                      bool hasDirectional, bool cullBackFaces, bool allowSkyGeometry, uint32_t ignoreFpvLightIndex )
 {
     RgExtFrameParams& p = cs.params;
     std::memcpy( p.viewPrev, p.view, sizeof( p.view ) );
@@ -162,6 +162,33 @@ void FillFrameParams( CameraState& cs, const RgDrawFrameInfo& di, uint32_t w, ui
     p.worldUp[ 1 ] = ul > 1e-12 ? float( uy / ul ) : 1.0f;
     p.worldUp[ 2 ] = ul > 1e-12 ? float( uz / ul ) : 0.0f;
     p.worldUp[ 3 ] = 0.0f;
+
+    // ray cull masks (Source/VulkanDevice.cpp:344-393)
+    uint32_t world = 0;
+    if( di.rayCullMaskWorld & RG_DRAW_FRAME_RAY_CULL_WORLD_0_BIT )
+    {
+        world |= IMASK_WORLD_0;
+    }
+    if( di.rayCullMaskWorld & RG_DRAW_FRAME_RAY_CULL_WORLD_1_BIT )
+    {
+        world |= IMASK_WORLD_1;
+    }
+    if( di.rayCullMaskWorld & RG_DRAW_FRAME_RAY_CULL_WORLD_2_BIT )
+    {
+        RG_REQUIRE( !allowSkyGeometry, RG_RESULT_WRONG_FUNCTION_ARGUMENT,
+                    "RG_DRAW_FRAME_RAY_CULL_WORLD_2_BIT cannot be used, as RgInstanceCreateInfo::allowGeometryWithSkyFlag was true" );
+        world |= IMASK_WORLD_2;
+    }
+    if( di.rayCullMaskWorld & RG_DRAW_FRAME_RAY_CULL_SKY_BIT )
+    {
+        RG_REQUIRE( allowSkyGeometry, RG_RESULT_WRONG_FUNCTION_ARGUMENT,
+                    "RG_DRAW_FRAME_RAY_CULL_SKY_BIT cannot be used, as RgInstanceCreateInfo::allowGeometryWithSkyFlag was false" );
+        world |= IMASK_WORLD_2;
+    }
+    p.rayCullMaskWorld           = world;
+    p.rayCullMaskWorldShadow     = allowSkyGeometry ? ( world & ~IMASK_WORLD_2 ) : world;
+    p.lightIndexIgnoreFPVShadows = ignoreFpvLightIndex;
+    p.skyGeometry                = allowSkyGeometry ? 1u : 0u;
 }
 
 void RenderSizeOf( const RgDrawFrameInfo& di, uint32_t* w, uint32_t* h )

@@ -115,8 +115,10 @@ struct Instance
         uint32_t w, h;
         RenderSizeOf( di, &w, &h );
         scene.FinishLights();
+        const auto* illum = FindParams< RgDrawFrameIlluminationParams >( di, RG_STRUCTURE_TYPE_ILLUMINATION );
         FillFrameParams( cam, di, w, h, uint32_t( scene.lights.size() ), scene.prevLightCount, scene.hasDirectional,
-                         info.rayCullBackFacingTriangles != 0 );
+                         info.rayCullBackFacingTriangles != 0, info.allowGeometryWithSkyFlag != 0,
+                         scene.LightIndexOf( illum ? illum->lightUniqueIdIgnoreFirstPersonViewerShadows : nullptr ) );
         {
             // styles for the lights of the next frame's uploads
             const auto* il = FindParams< RgDrawFrameIlluminationParams >( di, RG_STRUCTURE_TYPE_ILLUMINATION );
@@ -172,15 +174,16 @@ struct Instance
         RG_HIP_CHECK( hipEventRecord( ev[ ST_ASSEMBLED ], s ) );
         stats.bvhRebuilt = 0;
         if( dscene.triCount > 0 &&
-            ( changed || !dscene.bvhValid || bvh.nodeFormat != nodeFmt || dscene.builtOpaqueTriCount != dscene.opaqueTriCount ) )
+            ( changed || !dscene.bvhValid || bvh.nodeFormat != nodeFmt ||
+              std::memcmp( dscene.builtClassTris, dscene.classTris, sizeof( dscene.classTris ) ) != 0 ) )
         {
             // one triangle per leaf: measured fastest at 1M triangles (more
             // nodes, fewer triangle tests per leaf visit)
             RgExtBvhBuildParams bp = { 1, nodeFmt };
-            BuildFrameBvh( bvh, dscene.tris.as< float >(), dscene.triCount, dscene.opaqueTriCount, bp, s );
-            dscene.bvhValid            = true;
-            dscene.builtTriCount       = dscene.triCount;
-            dscene.builtOpaqueTriCount = dscene.opaqueTriCount;
+            BuildFrameBvh( bvh, dscene.tris.as< float >(), dscene.classTris, GC_COUNT, bp, s );
+            dscene.bvhValid      = true;
+            dscene.builtTriCount = dscene.triCount;
+            std::memcpy( dscene.builtClassTris, dscene.classTris, sizeof( dscene.classTris ) );
             stats.bvhRebuilt     = 1;
         }
         RG_HIP_CHECK( hipEventRecord( ev[ ST_BVH ], s ) );

@@ -287,57 +287,80 @@ __global__ __launch_bounds__( THREADS ) void k_scene_roots( uint32_t internal, c
     }
 }
 
+// Seeds the collapse with every scene root; a one-triangle scene (frame
+// segments only) has no internal node and is left out.
 __global__ void k_seed_frontier_seg( const uint32_t* roots, uint32_t S, uint32_t* frontier, uint32_t* counts )
 {
-    for( uint32_t s = threadIdx.x; s < S; s += blockDim.x )
-    {
-        frontier[ s ] = roots[ s ];
-    }
     if( threadIdx.x == 0 )
     {
-        counts[ 0 ] = S;
+        uint32_t k = 0;
+        for( uint32_t s = 0; s < S; s++ )
+        {
+            if( roots[ s ] != 0xFFFFFFFFu )
+            {
+                frontier[ k++ ] = roots[ s ];
+            }
+        }
+        counts[ 0 ] = k;
     }
 }
 
-// Node `top` over the 8-wide roots of S <= 8 scenes: each child box is the
-// union of that root's child boxes.
-__global__ void k_top_node( BvhNode8* nodes, const uint32_t* roots8, uint32_t S, uint32_t top )
+// 8-wide root of each scene; a one-triangle scene's root is the leaf of its
+// triangle, which the sort left at the scene's first position.
+__global__ void k_gather_roots8( const uint32_t* roots, const uint32_t* offsets, uint32_t S, const uint32_t* node8Index, uint32_t* roots8 )
 {
-    // roots8 entries index nodes below `top`; anything else stays an empty slot
-    const int k = threadIdx.x;
-    if( k >= 8 )
+    for( uint32_t s = threadIdx.x; s < S; s += blockDim.x )
+    {
+        roots8[ s ] = roots[ s ] != 0xFFFFFFFFu ? node8Index[ roots[ s ] ] : ( RG_EXT_BVH_CHILD_LEAF_BIT | ( offsets[ s ] << 3 ) );
+    }
+}
+
+// Box of each frame segment's root: the union of an 8-wide root's child
+// boxes, or a one-triangle root's triangle box.  Out: 8 floats per segment
+// (lo xyz, hi xyz, 2 unused).
+__global__ void k_segment_boxes( const BvhNode8* nodes, const float4* sortedTris, const uint32_t* roots8, uint32_t S, float* out )
+{
+    const uint32_t s = threadIdx.x;
+    if( s >= S )
     {
         return;
     }
-    BvhNode8& t = nodes[ top ];
-    float     lo[ 3 ] = { INFINITY, INFINITY, INFINITY }, hi[ 3 ] = { -INFINITY, -INFINITY, -INFINITY };
-    uint32_t  child   = RG_EXT_BVH_CHILD_EMPTY;
-    if( uint32_t( k ) < S && roots8[ k ] < top )
+    float          lo[ 3 ] = { INFINITY, INFINITY, INFINITY }, hi[ 3 ] = { -INFINITY, -INFINITY, -INFINITY };
+    const uint32_t r       = roots8[ s ];
+    if( r & RG_EXT_BVH_CHILD_LEAF_BIT )
     {
-        const BvhNode8& r = nodes[ roots8[ k ] ];
-        child             = roots8[ k ];
-        for( int j = 0; j < 8; j++ )
+        const uint32_t first = ( r >> 3 ) & 0x0FFFFFFFu;
+        for( uint32_t k = 0; k <= ( r & 7u ); k++ )
         {
-            if( r.child[ j ] != RG_EXT_BVH_CHILD_EMPTY )
+            for( int v = 0; v < 3; v++ )
             {
-                lo[ 0 ] = fminf( lo[ 0 ], r.lo_x[ j ] ), hi[ 0 ] = fmaxf( hi[ 0 ], r.hi_x[ j ] );
-                lo[ 1 ] = fminf( lo[ 1 ], r.lo_y[ j ] ), hi[ 1 ] = fmaxf( hi[ 1 ], r.hi_y[ j ] );
-                lo[ 2 ] = fminf( lo[ 2 ], r.lo_z[ j ] ), hi[ 2 ] = fmaxf( hi[ 2 ], r.hi_z[ j ] );
+                const float4 q = sortedTris[ size_t( first + k ) * 3 + v ];
+                lo[ 0 ] = fminf( lo[ 0 ], q.x ), hi[ 0 ] = fmaxf( hi[ 0 ], q.x );
+                lo[ 1 ] = fminf( lo[ 1 ], q.y ), hi[ 1 ] = fmaxf( hi[ 1 ], q.y );
+                lo[ 2 ] = fminf( lo[ 2 ], q.z ), hi[ 2 ] = fmaxf( hi[ 2 ], q.z );
             }
         }
     }
-    t.lo_x[ k ] = lo[ 0 ], t.lo_y[ k ] = lo[ 1 ], t.lo_z[ k ] = lo[ 2 ];
-    t.hi_x[ k ] = hi[ 0 ], t.hi_y[ k ] = hi[ 1 ], t.hi_z[ k ] = hi[ 2 ];
-    t.child[ k ] = child;
-    t.pad[ k ]   = 0;
-}
-
-__global__ void k_gather_roots8( const uint32_t* roots, uint32_t S, const uint32_t* node8Index, uint32_t* roots8 )
-{
-    for( uint32_t s = threadIdx.x; s < S; s += blockDim.x )
+    else
     {
-        roots8[ s ] = node8Index[ roots[ s ] ];
+        const BvhNode8& nd = nodes[ r ];
+        for( int j = 0; j < 8; j++ )
+        {
+            if( nd.child[ j ] != RG_EXT_BVH_CHILD_EMPTY )
+            {
+                lo[ 0 ] = fminf( lo[ 0 ], nd.lo_x[ j ] ), hi[ 0 ] = fmaxf( hi[ 0 ], nd.hi_x[ j ] );
+                lo[ 1 ] = fminf( lo[ 1 ], nd.lo_y[ j ] ), hi[ 1 ] = fmaxf( hi[ 1 ], nd.hi_y[ j ] );
+                lo[ 2 ] = fminf( lo[ 2 ], nd.lo_z[ j ] ), hi[ 2 ] = fmaxf( hi[ 2 ], nd.hi_z[ j ] );
+            }
+        }
     }
+    for( int k = 0; k < 3; k++ )
+    {
+        out[ s * 8 + k ]     = lo[ k ];
+        out[ s * 8 + 3 + k ] = hi[ k ];
+    }
+    out[ s * 8 + 6 ] = 0.0f;
+    out[ s * 8 + 7 ] = 0.0f;
 }
 
 // --------------------------------------------------------- radix sort
@@ -881,7 +904,7 @@ namespace
 // S + 1 of them) switches steps 1-2 to per-scene bounds and keys and seeds
 // the collapse with every scene's root.
 void BuildBvhImpl( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtBvhBuildParams& params, const uint32_t* batchOffsets,
-                   uint32_t sceneCount, hipStream_t stream, bool topNode = false )
+                   uint32_t sceneCount, hipStream_t stream, bool segmentBoxes = false )
 {
     RG_REQUIRE( n > 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "triangle count must be > 0" );
     RG_REQUIRE( n < ( 1u << 28 ), RG_RESULT_WRONG_FUNCTION_ARGUMENT, "triangle count must be < 2^28" );
@@ -1118,37 +1141,41 @@ void BuildBvhImpl( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtB
     RG_HIP_CHECK( hipMemcpyAsync( bvh.hostCounts, nodeTotal, 4, hipMemcpyDeviceToHost, stream ) );
     RG_HIP_CHECK( hipStreamSynchronize( stream ) );
     bvh.nodeCount = bvh.hostCounts[ 0 ];
-    RG_REQUIRE( bvh.nodeCount >= 1, RG_RESULT_INTERNAL_ERROR, "BVH collapse produced no nodes" );
+    // a batch of one-triangle frame segments has leaves for roots and no node
+    RG_REQUIRE( bvh.nodeCount >= 1 || segmentBoxes, RG_RESULT_INTERNAL_ERROR, "BVH collapse produced no nodes" );
 
     const bool    half    = params.nodeFormat == RG_EXT_BVH_NODE_FP16;
     DeviceBuffer& fp32Out = half ? bvh.nodes32 : bvh.nodes;
-    const bool    withTop = batched && topNode;
-    RG_REQUIRE( !withTop || sceneCount <= 8, RG_RESULT_INTERNAL_ERROR, "a top node holds at most 8 scenes" );
-    fp32Out.Reserve( size_t( bvh.nodeCount + ( withTop ? 1u : 0u ) ) * sizeof( BvhNode8 ) );
+    fp32Out.Reserve( size_t( bvh.nodeCount ) * sizeof( BvhNode8 ) );
     k_write_nodes<<< DivUp( slots * 8u, THREADS ), THREADS, 0, stream >>>( slots, n, bvh.leafMax, bvh.isNode8.as< uint32_t >(), bvh.node8Index.as< uint32_t >(), bvh.expansion.as< uint32_t >(), rf, rl, bounds, fp32Out.as< BvhNode8 >() );
     CheckLaunch( "k_write_nodes" );
+    k_sorted_tris<<< DivUp( n, THREADS ), THREADS, 0, stream >>>( d_tris, vin, n, bvh.sortedTris.as< float4 >() );
+    CheckLaunch( "k_sorted_tris" );
     if( batched )
     {
-        k_gather_roots8<<< 1, 256, 0, stream >>>( bvh.sceneRoots.as< uint32_t >(), sceneCount, bvh.node8Index.as< uint32_t >(),
-                                                  bvh.sceneRoots8.as< uint32_t >() );
+        k_gather_roots8<<< 1, 256, 0, stream >>>( bvh.sceneRoots.as< uint32_t >(), bvh.sceneOffsets.as< uint32_t >(), sceneCount,
+                                                  bvh.node8Index.as< uint32_t >(), bvh.sceneRoots8.as< uint32_t >() );
         CheckLaunch( "k_gather_roots8" );
         bvh.hostSceneRoots.resize( sceneCount );
         RG_HIP_CHECK( hipMemcpyAsync( bvh.hostSceneRoots.data(), bvh.sceneRoots8.ptr, size_t( sceneCount ) * 4, hipMemcpyDeviceToHost, stream ) );
+        if( segmentBoxes )
+        {
+            RG_REQUIRE( sceneCount <= 64, RG_RESULT_INTERNAL_ERROR, "too many frame segments" );
+            bvh.segBoxes.Reserve( size_t( sceneCount ) * 8 * sizeof( float ) );
+            k_segment_boxes<<< 1, 64, 0, stream >>>( fp32Out.as< BvhNode8 >(), bvh.sortedTris.as< float4 >(), bvh.sceneRoots8.as< uint32_t >(), sceneCount,
+                                                     bvh.segBoxes.as< float >() );
+            CheckLaunch( "k_segment_boxes" );
+            bvh.hostSegBoxes.resize( size_t( sceneCount ) * 8 );
+            RG_HIP_CHECK( hipMemcpyAsync( bvh.hostSegBoxes.data(), bvh.segBoxes.ptr, size_t( sceneCount ) * 8 * sizeof( float ), hipMemcpyDeviceToHost,
+                                          stream ) );
+        }
     }
-    if( withTop )
-    {
-        k_top_node<<< 1, 64, 0, stream >>>( fp32Out.as< BvhNode8 >(), bvh.sceneRoots8.as< uint32_t >(), sceneCount, bvh.nodeCount );
-        CheckLaunch( "k_top_node" );
-        bvh.nodeCount += 1;
-    }
-    if( half )
+    if( half && bvh.nodeCount > 0 )
     {
         bvh.nodes.Reserve( size_t( bvh.nodeCount ) * sizeof( BvhNode8H ) );
         k_half_nodes<<< DivUp( bvh.nodeCount, THREADS ), THREADS, 0, stream >>>( bvh.nodes32.as< BvhNode8 >(), bvh.nodeCount, bvh.nodes.as< BvhNode8H >() );
         CheckLaunch( "k_half_nodes" );
     }
-    k_sorted_tris<<< DivUp( n, THREADS ), THREADS, 0, stream >>>( d_tris, vin, n, bvh.sortedTris.as< float4 >() );
-    CheckLaunch( "k_sorted_tris" );
 
     RG_HIP_CHECK( hipMemcpyAsync( bvh.sceneMin, sceneBounds, 3 * sizeof( double ), hipMemcpyDeviceToHost, stream ) );
     RG_HIP_CHECK( hipMemcpyAsync( bvh.sceneMax, sceneBounds + 3, 3 * sizeof( double ), hipMemcpyDeviceToHost, stream ) );
@@ -1159,7 +1186,7 @@ void BuildBvhImpl( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtB
 void BuildBvh( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtBvhBuildParams& params, hipStream_t stream )
 {
     BuildBvhImpl( bvh, d_tris, n, params, nullptr, 0, stream );
-    bvh.frameMode = BvhObject::FRAME_SINGLE;
+    bvh.frameSegs.clear();
 }
 
 void BuildBvhBatch( BvhObject& bvh, const float* d_tris, const uint32_t* offsets, uint32_t sceneCount, const RgExtBvhBuildParams& params,
@@ -1172,37 +1199,55 @@ void BuildBvhBatch( BvhObject& bvh, const float* d_tris, const uint32_t* offsets
         RG_REQUIRE( offsets[ s + 1 ] >= offsets[ s ] + 2, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "every scene needs at least 2 triangles" );
     }
     BuildBvhImpl( bvh, d_tris, offsets[ sceneCount ], params, offsets, sceneCount, stream );
-    bvh.frameMode = BvhObject::FRAME_SINGLE;
+    bvh.frameSegs.clear();
     for( uint32_t s = 0; s < sceneCount; s++ )
     {
         RG_REQUIRE( bvh.hostSceneRoots[ s ] < bvh.nodeCount, RG_RESULT_INTERNAL_ERROR, "a scene has no root node" );
     }
 }
 
-void BuildFrameBvh( BvhObject& bvh, const float* d_tris, uint32_t n, uint32_t opaqueN, const RgExtBvhBuildParams& params, hipStream_t stream )
+void BuildFrameBvh( BvhObject& bvh, const float* d_tris, const uint32_t* classTris, uint32_t classCount, const RgExtBvhBuildParams& params,
+                    hipStream_t stream )
 {
-    RG_REQUIRE( opaqueN <= n, RG_RESULT_INTERNAL_ERROR, "opaque range exceeds the triangle count" );
-    uint32_t roots[ 3 ] = { 0, 0, 0 };
-    if( opaqueN >= 2 && n - opaqueN >= 2 )
+    std::vector< uint32_t > offsets( 1, 0u ), cls;
+    for( uint32_t c = 0; c < classCount; c++ )
     {
-        const uint32_t offsets[ 3 ] = { 0, opaqueN, n };
-        BuildBvhImpl( bvh, d_tris, n, params, offsets, 2, stream, true );
-        RG_REQUIRE( bvh.hostSceneRoots[ 0 ] < bvh.nodeCount && bvh.hostSceneRoots[ 1 ] < bvh.nodeCount, RG_RESULT_INTERNAL_ERROR,
-                    "a triangle range has no root node" );
-        bvh.frameMode = BvhObject::FRAME_SPLIT;
-        roots[ 0 ]    = bvh.nodeCount - 1; // the top node
-        roots[ 1 ]    = bvh.hostSceneRoots[ 0 ];
+        if( classTris[ c ] > 0 )
+        {
+            offsets.push_back( offsets.back() + classTris[ c ] );
+            cls.push_back( c );
+        }
     }
-    else
+    const uint32_t S = uint32_t( cls.size() ), n = offsets.back();
+    RG_REQUIRE( S > 0 && S <= 8, RG_RESULT_INTERNAL_ERROR, "a frame BVH has 1..8 segments" );
+    bvh.frameSegs.assign( S, BvhObject::FrameSegment{} );
+    if( S == 1 )
     {
-        // A range of one triangle has no subtree of its own: a single tree,
-        // seen by every ray type unless nothing in it is opaque.
+        // one class: the single-scene build (21 Morton bits per axis); rays
+        // whose mask admits it start at its root
         BuildBvhImpl( bvh, d_tris, n, params, nullptr, 0, stream );
-        bvh.frameMode = opaqueN == 0 && n > 0 ? BvhObject::FRAME_REFRACT_ONLY : BvhObject::FRAME_SINGLE;
+        bvh.frameSegs[ 0 ].cls  = cls[ 0 ];
+        bvh.frameSegs[ 0 ].root = 0;
+        for( int k = 0; k < 3; k++ )
+        {
+            bvh.frameSegs[ 0 ].box[ k ]     = -INFINITY;
+            bvh.frameSegs[ 0 ].box[ 3 + k ] = INFINITY;
+        }
+        return;
     }
-    bvh.frameRoots.Reserve( sizeof( roots ) );
-    RG_HIP_CHECK( hipMemcpyAsync( bvh.frameRoots.ptr, roots, sizeof( roots ), hipMemcpyHostToDevice, stream ) );
-    RG_HIP_CHECK( hipStreamSynchronize( stream ) );
+    // several classes: one batched build, each class a segment with its own
+    // root (a leaf for a one-triangle class) and root box
+    BuildBvhImpl( bvh, d_tris, n, params, offsets.data(), S, stream, true );
+    for( uint32_t s = 0; s < S; s++ )
+    {
+        const uint32_t r = bvh.hostSceneRoots[ s ];
+        RG_REQUIRE( ( r & RG_EXT_BVH_CHILD_LEAF_BIT ) ? ( ( r >> 3 ) & 0x0FFFFFFFu ) == offsets[ s ] : r < bvh.nodeCount, RG_RESULT_INTERNAL_ERROR,
+                    "a frame segment has no root" );
+        bvh.frameSegs[ s ].cls  = cls[ s ];
+        bvh.frameSegs[ s ].root = r;
+        std::memcpy( bvh.frameSegs[ s ].box, &bvh.hostSegBoxes[ size_t( s ) * 8 ], 6 * sizeof( float ) );
+    }
+    bvh.sceneCount = 0; // traced as a frame (TraceRaysFrame), not as a batch of scenes
 }
 
 } // namespace rg

@@ -27,6 +27,7 @@ struct Wavefront
 {
     DeviceBuffer rays, hits, rays2, hits2; // W*H
     DeviceBuffer pendA, pendB, pendC;      // W*H x (6 float4 / ray / hit)
+    DeviceBuffer masks, masks2, masksB;    // W*H cull masks of rays, rays2, pendB's rays
     size_t       capPx = 0;
 
     void          Reserve( size_t pixels );

@@ -420,11 +420,11 @@ __device__ Surface FetchSurface( const FbPtrs& P, int W, int x, int y )
 }
 
 // hitInfoToSurface_Indirect (Surface.inl:96-108)
-__device__ Surface HitToSurface( const HitInfo& h, v3 rayDir )
+__device__ Surface HitToSurface( const HitInfo& h, v3 rayDir, uint32_t custom )
 {
     Surface s;
     s.position        = h.position;
-    s.instCustomIndex = 1u;
+    s.instCustomIndex = custom;
     s.normal          = h.normal;
     s.roughness       = h.roughness;
     s.albedo          = h.albedo;
@@ -597,6 +597,58 @@ __device__ __forceinline__ RgExtRay ShadowRay( const Surface& s, v3 lightPos )
     v3          l     = lightPos - start;
     const float dist  = length( l );
     return MakeRay( start, l / dist, 0.001f, dist - 0.01f );
+}
+
+// Cull masks of the ray types from the origin surface's instance custom
+// index (RaygenCommon.h:103-177).
+__host__ __device__ __forceinline__ uint32_t PrimaryCullMask( const RgExtFrameParams& p )
+{
+    return p.rayCullMaskWorld | IMASK_REFRACT | IMASK_FIRST_PERSON;
+}
+__device__ __forceinline__ uint32_t ReflRefrCullMask( const RgExtFrameParams& p, uint32_t custom, uint32_t geomFlags, bool refraction )
+{
+    uint32_t world = p.rayCullMaskWorld | IMASK_REFRACT;
+    if( geomFlags & RG_MESH_PRIMITIVE_IGNORE_REFRACT_AFTER )
+    {
+        world &= ~IMASK_REFRACT;
+        if( custom & CUSTOM_FIRST_PERSON )
+        {
+            return world;
+        }
+    }
+    if( custom & CUSTOM_FIRST_PERSON )
+    {
+        return world | IMASK_FIRST_PERSON;
+    }
+    return world | ( refraction ? IMASK_FIRST_PERSON : IMASK_FIRST_PERSON_VIEWER );
+}
+// getShadowCullMask, and traceVisibility's ignoreFirstPersonViewer for the
+// light lightIndexIgnoreFPVShadows (RaygenCommon.h:350-358)
+__device__ __forceinline__ uint32_t ShadowCullMask( const RgExtFrameParams& p, uint32_t custom, uint32_t lightIndex )
+{
+    uint32_t m = p.rayCullMaskWorldShadow | ( ( custom & CUSTOM_FIRST_PERSON ) ? IMASK_FIRST_PERSON : IMASK_FIRST_PERSON_VIEWER );
+    if( lightIndex == p.lightIndexIgnoreFPVShadows )
+    {
+        m &= ~IMASK_FIRST_PERSON_VIEWER;
+    }
+    return m;
+}
+__device__ __forceinline__ uint32_t IndirectCullMask( const RgExtFrameParams& p, uint32_t custom )
+{
+    return p.rayCullMaskWorld | ( ( custom & CUSTOM_FIRST_PERSON ) ? IMASK_FIRST_PERSON : IMASK_FIRST_PERSON_VIEWER );
+}
+
+// instanceCustomIndex of a hit's instance
+__device__ __forceinline__ uint32_t HitCustomIndex( const RgExtFrameParams& p, const SceneView& sc, uint32_t prim )
+{
+    return CustomIndexOf( sc.geoms[ sc.triGeom[ prim ] ].flags, p.skyGeometry != 0u );
+}
+
+// doesPayloadContainHitInfo (RaygenCommon.h:191-205): a miss, or a hit on an
+// instance with the SKY custom-index flag, reads as the sky.
+__device__ __forceinline__ bool HitIsSky( const RgExtFrameParams& p, const SceneView& sc, const RgExtHit& h )
+{
+    return h.primitiveId == RG_EXT_MISS || ( p.skyGeometry != 0u && ( HitCustomIndex( p, sc, h.primitiveId ) & CUSTOM_SKY ) != 0u );
 }
 
 __device__ __forceinline__ LightPoint LightPointRnd( uint32_t seed )
@@ -822,7 +874,7 @@ __global__ __launch_bounds__( THREADS ) void k_primary_shade( RgExtFrameParams p
     const v3       dir = GetRayDir( p, u, v );
     const v3       org = mk3( p.cameraPosition[ 0 ], p.cameraPosition[ 1 ], p.cameraPosition[ 2 ] );
 
-    if( hit.primitiveId == RG_EXT_MISS )
+    if( HitIsSky( p, sc, hit ) )
     {
         StoreSky( p, P, ci, ri, cx, cy, dir, mk3( 1.f, 1.f, 1.f ), 0.0f, true );
         return;
@@ -871,7 +923,8 @@ __global__ __launch_bounds__( THREADS ) void k_primary_shade( RgExtFrameParams p
     Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD )[ ci ]         = depthLinear;
     Plane< float >( P, RG_EXT_PLANE_DEPTH_GRAD )[ ci ]          = sqrtf( gx * gx + gy * gy );
     Plane< float4 >( P, RG_EXT_PLANE_MOTION )[ ci ]             = make_float4( sxp - sxc, syp - syc, motionZ, 0.0f );
-    Plane< float4 >( P, RG_EXT_PLANE_SURFACE_POSITION )[ ci ]   = make_float4( h.position.x, h.position.y, h.position.z, __uint_as_float( 1u ) );
+    Plane< float4 >( P, RG_EXT_PLANE_SURFACE_POSITION )[ ci ] =
+        make_float4( h.position.x, h.position.y, h.position.z, __uint_as_float( CustomIndexOf( gr.flags, p.skyGeometry != 0u ) ) );
     Plane< uint4 >( P, RG_EXT_PLANE_VISIBILITY_BUFFER )[ ci ]   = make_uint4( 1u, ( h.localPrim << 12 ) | ( h.geom & 0xFFFu ), __float_as_uint( hit.u ), __float_as_uint( hit.v ) );
     Plane< float4 >( P, RG_EXT_PLANE_VIEW_DIRECTION )[ ci ]     = make_float4( dir.x, dir.y, dir.z, 0.0f );
     // the camera's medium attenuates the primary segment (RaygenPrimary.inl:321-328)
@@ -906,8 +959,8 @@ struct RrState
 // the primary surface from the G-buffer (as the reference restores it) and
 // its flags from the primary hit; later depths recompute the last hit.
 template< bool TEX >
-__global__ __launch_bounds__( THREADS ) void k_reflrefr_a( RgExtFrameParams p, FbPtrs P, SceneView sc, RgExtRay* rays, const RgExtHit* hits,
-                                                           RrState st, uint32_t depth )
+__global__ __launch_bounds__( THREADS ) void k_reflrefr_a( RgExtFrameParams p, FbPtrs P, SceneView sc, RgExtRay* rays, uint8_t* masks,
+                                                           const RgExtHit* hits, RrState st, uint32_t depth )
 {
     const int x = blockIdx.x * BX + threadIdx.x % BX, y = blockIdx.y * BY + threadIdx.x / BX;
     const int W = p.renderWidth, H = p.renderHeight;
@@ -922,7 +975,7 @@ __global__ __launch_bounds__( THREADS ) void k_reflrefr_a( RgExtFrameParams p, F
 
     v3       pos, normal, albedo, thr, dir;
     float    roughness, metallic, fpl;
-    uint32_t flags, bits;
+    uint32_t flags, bits, custom;
     if( depth == 0 )
     {
         if( Plane< uint32_t >( P, RG_EXT_PLANE_IS_SKY )[ ci ] != 0u )
@@ -946,6 +999,7 @@ __global__ __launch_bounds__( THREADS ) void k_reflrefr_a( RgExtFrameParams p, F
         dir              = GetRayDir( p, u, v );
         fpl              = Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD )[ ci ];
         flags            = sc.geoms[ sc.triGeom[ hits[ ri ].primitiveId ] ].flags;
+        custom           = __float_as_uint( sp.w );
         bits             = RR_ACTIVE | ( ( p.cameraMediaType & 3u ) << 2 );
         // ray cone (RaygenPrimary.inl:396-406): width 0, then propagated
         // over the primary hit's distance
@@ -976,6 +1030,7 @@ __global__ __launch_bounds__( THREADS ) void k_reflrefr_a( RgExtFrameParams p, F
         metallic           = h.metallic;
         roughness          = h.roughness;
         flags              = sc.geoms[ h.geom ].flags;
+        custom             = CustomIndexOf( flags, p.skyGeometry != 0u );
         thr                = mk3( s0.x, s0.y, s0.z );
         dir                = mk3( s1.x, s1.y, s1.z );
         fpl                = s1.w;
@@ -1076,7 +1131,8 @@ __global__ __launch_bounds__( THREADS ) void k_reflrefr_a( RgExtFrameParams p, F
     st.s[ ri ]        = make_float4( thr.x, thr.y, thr.z, __uint_as_float( bits ) );
     st.s[ st.n + ri ] = make_float4( dir.x, dir.y, dir.z, fpl );
     // traceReflectionRefractionRay (RaygenCommon.h:235-251)
-    rays[ ri ] = MakeRay( origin, dir, 0.001f, p.rayLength );
+    rays[ ri ]  = MakeRay( origin, dir, 0.001f, p.rayLength );
+    masks[ ri ] = uint8_t( ReflRefrCullMask( p, custom, flags, doRefr ) );
 }
 
 // After the trace: the sky (RaygenPrimary.inl:550-556) or the new surface
@@ -1108,7 +1164,7 @@ __global__ __launch_bounds__( THREADS ) void k_reflrefr_b( RgExtFrameParams p, F
     const float    thrW  = ( bits & RR_SPLIT ) ? 1.0f : -1.0f;
     v3             thr   = mk3( s0.x, s0.y, s0.z );
     const RgExtHit hit   = hits[ ri ];
-    if( hit.primitiveId == RG_EXT_MISS )
+    if( HitIsSky( p, sc, hit ) )
     {
         const float up = fabsf( dot( dir, mk3( p.worldUp[ 0 ], p.worldUp[ 1 ], p.worldUp[ 2 ] ) ) );
         thr            = thr * MediaTransmittance( p, media, 1.0f / ( up * up * up ) );
@@ -1161,7 +1217,8 @@ __global__ __launch_bounds__( THREADS ) void k_reflrefr_b( RgExtFrameParams p, F
     Plane< float2 >( P, RG_EXT_PLANE_METALLIC_ROUGHNESS )[ ci ] = make_float2( h.metallic, h.roughness );
     Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD )[ ci ]         = fpl;
     Plane< float4 >( P, RG_EXT_PLANE_MOTION )[ ci ]             = make_float4( sxp - sxc, syp - syc, motionZ, 0.0f );
-    Plane< float4 >( P, RG_EXT_PLANE_SURFACE_POSITION )[ ci ]   = make_float4( h.position.x, h.position.y, h.position.z, __uint_as_float( 1u ) );
+    Plane< float4 >( P, RG_EXT_PLANE_SURFACE_POSITION )[ ci ] =
+        make_float4( h.position.x, h.position.y, h.position.z, __uint_as_float( CustomIndexOf( sc.geoms[ h.geom ].flags, p.skyGeometry != 0u ) ) );
     Plane< uint4 >( P, RG_EXT_PLANE_VISIBILITY_BUFFER )[ ci ]   = make_uint4( 1u, ( h.localPrim << 12 ) | ( h.geom & 0xFFFu ), __float_as_uint( hit.u ), __float_as_uint( hit.v ) );
     Plane< float4 >( P, RG_EXT_PLANE_VIEW_DIRECTION )[ ci ]     = make_float4( dir.x, dir.y, dir.z, 0.0f );
     Plane< float4 >( P, RG_EXT_PLANE_THROUGHPUT )[ ci ]         = make_float4( thr.x, thr.y, thr.z, thrW );
@@ -1172,7 +1229,7 @@ __global__ __launch_bounds__( THREADS ) void k_reflrefr_b( RgExtFrameParams p, F
 }
 
 // RtInitialReservoirs.rgen, first half: candidates + directional visibility ray.
-__global__ __launch_bounds__( THREADS ) void k_initial_a( RgExtFrameParams p, FbPtrs P, LightView lv, RgExtRay* rays, uint4* pend )
+__global__ __launch_bounds__( THREADS ) void k_initial_a( RgExtFrameParams p, FbPtrs P, LightView lv, RgExtRay* rays, uint8_t* masks, uint4* pend )
 {
     const int x = blockIdx.x * BX + threadIdx.x % BX, y = blockIdx.y * BY + threadIdx.x / BX;
     const int W = p.renderWidth, H = p.renderHeight;
@@ -1201,6 +1258,7 @@ __global__ __launch_bounds__( THREADS ) void k_initial_a( RgExtFrameParams p, Fb
     {
         const LightSample ls = sampleLight( GetLight( lv, 0 ), s.position, lp, p.polyLightSpotlightFactor );
         rays[ i ]            = ShadowRay( s, ls.position );
+        masks[ i ]           = uint8_t( ShadowCullMask( p, s.instCustomIndex, 0u ) );
     }
     else
     {
@@ -1247,7 +1305,8 @@ __device__ __forceinline__ T LdPlane( const T* base, int W, int H, int x, int y,
 }
 
 // RtRaygenDirect.rgen first half: spatiotemporal reuse, shading, shadow ray.
-__global__ __launch_bounds__( THREADS ) void k_direct_a( RgExtFrameParams p, FbPtrs P, LightView lv, RgExtRay* rays, float4* pendD, float4* pendS )
+__global__ __launch_bounds__( THREADS ) void k_direct_a( RgExtFrameParams p, FbPtrs P, LightView lv, RgExtRay* rays, uint8_t* masks, float4* pendD,
+                                                         float4* pendS )
 {
     const int x = blockIdx.x * BX + threadIdx.x % BX, y = blockIdx.y * BY + threadIdx.x / BX;
     const int W = p.renderWidth, H = p.renderHeight;
@@ -1340,8 +1399,9 @@ __global__ __launch_bounds__( THREADS ) void k_direct_a( RgExtFrameParams p, FbP
             }
             else if( 0u < p.maxBounceShadowsLights )
             {
-                rays[ i ] = ShadowRay( s, ls.position );
-                dist      = length( ls.position - s.position );
+                rays[ i ]  = ShadowRay( s, ls.position );
+                masks[ i ] = uint8_t( ShadowCullMask( p, s.instCustomIndex, res.selected ) );
+                dist       = length( ls.position - s.position );
             }
         }
     }
@@ -1400,7 +1460,7 @@ struct IndirPend
 };
 
 // RT_RAYGEN_INDIRECT_INIT: specular (GGX-VNDF) bounce from the primary surface.
-__global__ __launch_bounds__( THREADS ) void k_indirect_a( RgExtFrameParams p, FbPtrs P, LightView lv, RgExtRay* rays, IndirPend pend )
+__global__ __launch_bounds__( THREADS ) void k_indirect_a( RgExtFrameParams p, FbPtrs P, LightView lv, RgExtRay* rays, uint8_t* masks, IndirPend pend )
 {
     const int x = blockIdx.x * BX + threadIdx.x % BX, y = blockIdx.y * BY + threadIdx.x / BX;
     const int W = p.renderWidth, H = p.renderHeight;
@@ -1423,6 +1483,7 @@ __global__ __launch_bounds__( THREADS ) void k_indirect_a( RgExtFrameParams p, F
     float    oneOverPdf;
     const v3 d = sampleSmithGGX( s.normal, s.toViewerDir, s.roughness, bn[ 0 ], bn[ 1 ], &oneOverPdf, true );
     rays[ i ]  = MakeRay( s.position + s.normal * 0.01f, d, 0.001f, p.rayLength );
+    masks[ i ] = uint8_t( IndirectCullMask( p, s.instCustomIndex ) );
     pend.at( 0, i ) = make_float4( d.x, d.y, d.z, oneOverPdf );
 }
 
@@ -1430,7 +1491,7 @@ __global__ __launch_bounds__( THREADS ) void k_indirect_a( RgExtFrameParams p, F
 // LIGHT_SAMPLE_METHOD_INDIR, RaygenCommon.h:701-741): RIS without
 // visibility; the shadow ray is traced if bounce < maxBounceShadows.
 __device__ void IndirectDirect( const RgExtFrameParams& p, const LightView& lv, const Surface& hs, uint32_t seed, int bounce,
-                                v3* diffOut, bool* isDirectional, bool* needShadow, v3* lightPos )
+                                v3* diffOut, bool* isDirectional, bool* needShadow, v3* lightPos, uint32_t* lightIndex = nullptr )
 {
     *diffOut       = mk3( 0, 0, 0 );
     *needShadow    = false;
@@ -1459,12 +1520,17 @@ __device__ void IndirectDirect( const RgExtFrameParams& p, const LightView& lv, 
     *isDirectional = r.selected == 0u;
     *needShadow    = uint32_t( bounce ) < p.maxBounceShadowsLights;
     *lightPos      = ls.position;
+    if( lightIndex )
+    {
+        *lightIndex = r.selected;
+    }
 }
 
 // Bounce-1 hit: direct light at the hit (shadow ray) and the second bounce.
 template< bool TEX >
 __global__ __launch_bounds__( THREADS ) void k_indirect_b( RgExtFrameParams p, FbPtrs P, SceneView sc, LightView lv, const RgExtRay* rays1, const RgExtHit* hits1,
-                                                           RgExtRay* shadowRays, RgExtRay* bounceRays, IndirPend pend )
+                                                           RgExtRay* shadowRays, uint8_t* shadowMasks, RgExtRay* bounceRays, uint8_t* bounceMasks,
+                                                           IndirPend pend )
 {
     const int x = blockIdx.x * BX + threadIdx.x % BX, y = blockIdx.y * BY + threadIdx.x / BX;
     const int W = p.renderWidth, H = p.renderHeight;
@@ -1483,7 +1549,7 @@ __global__ __launch_bounds__( THREADS ) void k_indirect_b( RgExtFrameParams p, F
     const RgExtHit hit = hits1[ i ];
     const v3       dir = mk3( r.direction[ 0 ], r.direction[ 1 ], r.direction[ 2 ] );
     const v3       org = mk3( r.origin[ 0 ], r.origin[ 1 ], r.origin[ 2 ] );
-    if( hit.primitiveId == RG_EXT_MISS )
+    if( HitIsSky( p, sc, hit ) )
     {
         pend.at( 1, i ) = make_float4( 0.f, 0.f, 0.f, 1.0f ); // sky
         return;
@@ -1492,14 +1558,16 @@ __global__ __launch_bounds__( THREADS ) void k_indirect_b( RgExtFrameParams p, F
     bq.mode             = TQ_BOUNCE; // FIRST_BOUNCE_MIP_BIAS 0 (RtRaygenIndirect.inl:63)
     bq.a                = Plane< float2 >( P, RG_EXT_PLANE_METALLIC_ROUGHNESS )[ i ].y;
     const HitInfo  h    = GetHit< TEX >( sc, p, hit.primitiveId, hit.u, hit.v, org, bq );
-    const Surface  hs   = HitToSurface( h, dir );
+    const Surface  hs   = HitToSurface( h, dir, CustomIndexOf( sc.geoms[ h.geom ].flags, p.skyGeometry != 0u ) );
     const uint32_t seed = randomSeed( x, y, p.frameId );
     v3             diff, lightPos = mk3( 0, 0, 0 );
     bool           isDir, needShadow;
-    IndirectDirect( p, lv, hs, seed, 1, &diff, &isDir, &needShadow, &lightPos );
+    uint32_t       lightIndex = LIGHT_INDEX_NONE;
+    IndirectDirect( p, lv, hs, seed, 1, &diff, &isDir, &needShadow, &lightPos, &lightIndex );
     if( needShadow )
     {
-        shadowRays[ i ] = ShadowRay( hs, lightPos );
+        shadowRays[ i ]  = ShadowRay( hs, lightPos );
+        shadowMasks[ i ] = uint8_t( ShadowCullMask( p, hs.instCustomIndex, lightIndex ) );
     }
     const v3 emis = h.emission * p.emissionMapBoost;
     // second diffuse bounce (getDiffuseBounce, RtRaygenIndirect.inl:57-62)
@@ -1507,7 +1575,8 @@ __global__ __launch_bounds__( THREADS ) void k_indirect_b( RgExtFrameParams p, F
     BlueNoise8( lv.blueNoise, seed, SALT_DIFF_BOUNCE( 2 ), bn );
     float    pdf2;
     const v3 d2     = sampleOrientedHemisphere( hs.normal, bn[ 0 ], bn[ 1 ], &pdf2 );
-    bounceRays[ i ] = MakeRay( hs.position + hs.normal * 0.01f, d2, 0.001f, p.rayLength );
+    bounceRays[ i ]  = MakeRay( hs.position + hs.normal * 0.01f, d2, 0.001f, p.rayLength );
+    bounceMasks[ i ] = uint8_t( IndirectCullMask( p, hs.instCustomIndex ) );
     pend.at( 1, i ) = make_float4( h.position.x, h.position.y, h.position.z, 0.0f );
     pend.at( 2, i ) = make_float4( h.normal.x, h.normal.y, h.normal.z, isDir ? 1.0f : 0.0f );
     pend.at( 3, i ) = make_float4( h.albedo.x, h.albedo.y, h.albedo.z, needShadow ? 1.0f : 0.0f );
@@ -1574,7 +1643,7 @@ __global__ __launch_bounds__( THREADS ) void k_indirect_c( RgExtFrameParams p, F
         const v3       d2 = mk3( b2.x, b2.y, b2.z );
         const RgExtHit h2 = bounceHits[ i ];
         v3             second;
-        if( h2.primitiveId == RG_EXT_MISS )
+        if( HitIsSky( p, sc, h2 ) )
         {
             second = SkyColor( p ) * b2.w;
         }
@@ -1586,7 +1655,7 @@ __global__ __launch_bounds__( THREADS ) void k_indirect_c( RgExtFrameParams p, F
             q2.a                = hi1.roughness;
             q2.b                = 32.0f; // SECOND_BOUNCE_MIP_BIAS (RtRaygenIndirect.inl:64)
             const HitInfo  hi2  = GetHit< TEX >( sc, p, h2.primitiveId, h2.u, h2.v, mk3( br.origin[ 0 ], br.origin[ 1 ], br.origin[ 2 ] ), q2 );
-            const Surface  hs2  = HitToSurface( hi2, d2 );
+            const Surface  hs2  = HitToSurface( hi2, d2, CustomIndexOf( sc.geoms[ hi2.geom ].flags, p.skyGeometry != 0u ) );
             const uint32_t seed = randomSeed( x, y, p.frameId );
             v3             diff2, lp;
             bool           isDir2, needShadow2;
@@ -1900,7 +1969,7 @@ __device__ __forceinline__ void StoreGradient( const FbPtrs& P, int GW, int gx, 
 // previous frame (not the one used last frame) is re-shaded with its old
 // reservoir, light point and seed on the surface as it is now.
 __global__ __launch_bounds__( THREADS ) void k_grad_a( RgExtFrameParams p, FbPtrs P, SceneView sc, LightView lv, const uint32_t* geomPrevToCur,
-                                                       uint32_t geomPrevCount, uint32_t geomCurCount, RgExtRay* rays, GradPend pend )
+                                                       uint32_t geomPrevCount, uint32_t geomCurCount, RgExtRay* rays, uint8_t* masks, GradPend pend )
 {
     const int W = p.renderWidth, H = p.renderHeight;
     const int GW = ( W + 1 ) / 3 > 1 ? ( W + 1 ) / 3 : 1, GH = ( H + 1 ) / 3 > 1 ? ( H + 1 ) / 3 : 1;
@@ -2013,8 +2082,9 @@ __global__ __launch_bounds__( THREADS ) void k_grad_a( RgExtFrameParams p, FbPtr
                     lumS = luminance( sp );
                     if( 0u < p.maxBounceShadowsLights )
                     {
-                        rays[ gi ] = ShadowRay( ps, ls.position );
-                        state      = 2u;
+                        rays[ gi ]  = ShadowRay( ps, ls.position );
+                        masks[ gi ] = uint8_t( ShadowCullMask( p, ps.instCustomIndex, cur ) );
+                        state       = 2u;
                     }
                 }
             }
@@ -2145,7 +2215,7 @@ void PrimaryRays( const RgExtFrameParams& p, RgExtRay* rays, hipStream_t s )
 
 size_t Wavefront::BytesFor( size_t n )
 {
-    return n * ( 2 * sizeof( RgExtRay ) + 2 * sizeof( RgExtHit ) + 16 * 6 + sizeof( RgExtRay ) + sizeof( RgExtHit ) );
+    return n * ( 2 * sizeof( RgExtRay ) + 2 * sizeof( RgExtHit ) + 16 * 6 + sizeof( RgExtRay ) + sizeof( RgExtHit ) + 3 );
 }
 
 void Wavefront::Reserve( size_t n )
@@ -2161,6 +2231,9 @@ void Wavefront::Reserve( size_t n )
     pendA.Reserve( n * 16 * 6 );
     pendB.Reserve( n * sizeof( RgExtRay ) );
     pendC.Reserve( n * sizeof( RgExtHit ) );
+    masks.Reserve( n );
+    masks2.Reserve( n );
+    masksB.Reserve( n );
     capPx = n;
 }
 
@@ -2247,22 +2320,30 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
     DeviceBuffer& pendB = wf->pendB;
     DeviceBuffer& pendC = wf->pendC;
     uint64_t  traced = 0;
-    const bool haveGeometry = scene.triCount > 0 && bvh.nodeCount > 0;
+    const bool haveGeometry = scene.triCount > 0 && !bvh.frameSegs.empty();
 
     traceLaunches = 0;
     RG_HIP_CHECK( hipMemsetAsync( frameAccum.ptr, 0, 3 * sizeof( uint64_t ), s ) );
     TraceAccum accum;
     accum.stats  = frameAccum.as< uint64_t >();
     accum.errors = reinterpret_cast< uint32_t* >( frameAccum.as< uint64_t >() + 2 );
-    // Primary and reflection / refraction rays see every triangle; shadow,
-    // bounce and gradient rays skip the refracting range (the reference's
-    // cull masks, RaygenCommon.h:103-170).
-    auto trace = [ & ]( RgExtRay* rr, RgExtHit* hh, uint32_t flags, uint32_t count, bool allGeometry = false ) {
+    // The frame's TLAS: each geometry class with its instance mask, or 0
+    // when rayCullMaskWorld leaves its world out (ASManager.cpp:837-963).
+    uint32_t classMasks[ GC_COUNT ];
+    for( uint32_t c = 0; c < GC_COUNT; c++ )
+    {
+        classMasks[ c ] = GeomClassPresent( c, p.rayCullMaskWorld ) ? GeomClassMask( c ) : 0u;
+    }
+    uint8_t* M  = wf->masks.as< uint8_t >();
+    uint8_t* M2 = wf->masks2.as< uint8_t >();
+    uint8_t* MB = wf->masksB.as< uint8_t >();
+    // rr's rays carry their cull masks in mm (null: every ray has mask)
+    auto trace = [ & ]( RgExtRay* rr, const uint8_t* mm, uint32_t mask, RgExtHit* hh, uint32_t flags, uint32_t count ) {
         if( haveGeometry )
         {
             const uint32_t k = traceLaunches < TRACE_EVENT_PAIRS ? traceLaunches : TRACE_EVENT_PAIRS - 1;
-            TraceRaysFrame( bvh, tc, rr, count, hh, flags | traceFlags | ( collectStats ? RG_EXT_TRACE_COLLECT_STATS : 0u ), !allGeometry, s,
-                            traceEv[ 2 * k ], traceEv[ 2 * k + 1 ], &accum, alpha, &tex );
+            TraceRaysFrame( bvh, tc, rr, mm, mask, count, hh, flags | traceFlags | ( collectStats ? RG_EXT_TRACE_COLLECT_STATS : 0u ), classMasks,
+                            GC_COUNT, s, traceEv[ 2 * k ], traceEv[ 2 * k + 1 ], &accum, alpha, &tex );
             traceLaunches = k + 1;
         }
         else
@@ -2276,7 +2357,7 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
     // primary
     k_primary_rays<<< grid2, THREADS, 0, s >>>( p, R );
     CheckLaunch( "k_primary_rays" );
-    trace( R, Hh, RG_EXT_TRACE_CLOSEST, n, true );
+    trace( R, nullptr, PrimaryCullMask( p ), Hh, RG_EXT_TRACE_CLOSEST, n );
     ( tex.count > 0 ? k_primary_shade< true > : k_primary_shade< false > )<<< grid2, THREADS, 0, s >>>( p, Hh, sv, P );
     CheckLaunch( "k_primary_shade" );
 
@@ -2287,9 +2368,9 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
         RrState st{ pendA.as< float4 >(), n };
         for( uint32_t d = 0; d < p.reflectRefractMaxDepth && d < 16u; d++ )
         {
-            ( tex.count > 0 ? k_reflrefr_a< true > : k_reflrefr_a< false > )<<< grid2, THREADS, 0, s >>>( p, P, sv, R, Hh, st, d );
+            ( tex.count > 0 ? k_reflrefr_a< true > : k_reflrefr_a< false > )<<< grid2, THREADS, 0, s >>>( p, P, sv, R, M, Hh, st, d );
             CheckLaunch( "k_reflrefr_a" );
-            trace( R, Hh, RG_EXT_TRACE_CLOSEST, n, true );
+            trace( R, M, 0u, Hh, RG_EXT_TRACE_CLOSEST, n );
             ( tex.count > 0 ? k_reflrefr_b< true > : k_reflrefr_b< false > )<<< grid2, THREADS, 0, s >>>( p, P, sv, R, Hh, st, d );
             CheckLaunch( "k_reflrefr_b" );
         }
@@ -2308,12 +2389,12 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
     }
 
     // initial reservoirs (directional visibility)
-    k_initial_a<<< grid2, THREADS, 0, s >>>( p, P, lv, R, pendA.as< uint4 >() );
+    k_initial_a<<< grid2, THREADS, 0, s >>>( p, P, lv, R, M, pendA.as< uint4 >() );
     CheckLaunch( "k_initial_a" );
     if( p.directionalLightExists )
     {
         // without a directional light every ray is null and k_initial_b reads no hit
-        trace( R, Hh, RG_EXT_TRACE_ANY_HIT, n );
+        trace( R, M, 0u, Hh, RG_EXT_TRACE_ANY_HIT, n );
     }
     k_initial_b<<< grid2, THREADS, 0, s >>>( p, P, Hh, pendA.as< uint4 >() );
     CheckLaunch( "k_initial_b" );
@@ -2321,9 +2402,9 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
     // direct illumination
     float4* pd = pendA.as< float4 >();
     float4* ps = pendA.as< float4 >() + n;
-    k_direct_a<<< grid2, THREADS, 0, s >>>( p, P, lv, R, pd, ps );
+    k_direct_a<<< grid2, THREADS, 0, s >>>( p, P, lv, R, M, pd, ps );
     CheckLaunch( "k_direct_a" );
-    trace( R, Hh, RG_EXT_TRACE_ANY_HIT, n );
+    trace( R, M, 0u, Hh, RG_EXT_TRACE_ANY_HIT, n );
     k_direct_b<<< grid2, THREADS, 0, s >>>( p, P, R, Hh, pd, ps );
     CheckLaunch( "k_direct_b" );
 
@@ -2331,13 +2412,13 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
     IndirPend ip{ pendA.as< float4 >(), n };
     RgExtRay* shadowR = pendB.as< RgExtRay >();
     RgExtHit* shadowH = pendC.as< RgExtHit >();
-    k_indirect_a<<< grid2, THREADS, 0, s >>>( p, P, lv, R, ip );
+    k_indirect_a<<< grid2, THREADS, 0, s >>>( p, P, lv, R, M, ip );
     CheckLaunch( "k_indirect_a" );
-    trace( R, Hh, RG_EXT_TRACE_CLOSEST, n );
-    ( tex.count > 0 ? k_indirect_b< true > : k_indirect_b< false > )<<< grid2, THREADS, 0, s >>>( p, P, sv, lv, R, Hh, shadowR, R2, ip );
+    trace( R, M, 0u, Hh, RG_EXT_TRACE_CLOSEST, n );
+    ( tex.count > 0 ? k_indirect_b< true > : k_indirect_b< false > )<<< grid2, THREADS, 0, s >>>( p, P, sv, lv, R, Hh, shadowR, MB, R2, M2, ip );
     CheckLaunch( "k_indirect_b" );
-    trace( shadowR, shadowH, RG_EXT_TRACE_ANY_HIT, n );
-    trace( R2, H2, RG_EXT_TRACE_CLOSEST, n );
+    trace( shadowR, MB, 0u, shadowH, RG_EXT_TRACE_ANY_HIT, n );
+    trace( R2, M2, 0u, H2, RG_EXT_TRACE_CLOSEST, n );
     ( tex.count > 0 ? k_indirect_c< true > : k_indirect_c< false > )<<< grid2, THREADS, 0, s >>>( p, P, sv, lv, shadowH, R2, H2, Hh, ip );
     CheckLaunch( "k_indirect_c" );
     k_indirect_final<<< grid2, THREADS, 0, s >>>( p, P );
@@ -2348,9 +2429,9 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
         const uint32_t GW = ( W + 1 ) / 3 > 1 ? ( W + 1 ) / 3 : 1, GH = ( H + 1 ) / 3 > 1 ? ( H + 1 ) / 3 : 1;
         const dim3     gridG( DivUp( GW, BX ), DivUp( GH, BY ) );
         GradPend       gp{ pendA.as< float4 >(), reinterpret_cast< uint4* >( pendA.as< float4 >() + size_t( GW ) * GH ) };
-        k_grad_a<<< gridG, THREADS, 0, s >>>( p, P, sv, lv, geomMap.as< uint32_t >(), geomMapCount, scene.geomCount, R, gp );
+        k_grad_a<<< gridG, THREADS, 0, s >>>( p, P, sv, lv, geomMap.as< uint32_t >(), geomMapCount, scene.geomCount, R, M, gp );
         CheckLaunch( "k_grad_a" );
-        trace( R, Hh, RG_EXT_TRACE_ANY_HIT, GW * GH );
+        trace( R, M, 0u, Hh, RG_EXT_TRACE_ANY_HIT, GW * GH );
         k_grad_b<<< gridG, THREADS, 0, s >>>( p, P, Hh, gp );
         CheckLaunch( "k_grad_b" );
     }

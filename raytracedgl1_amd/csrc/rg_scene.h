@@ -114,19 +114,118 @@ struct PinnedArray
     }
 };
 
+// Instance classes of a frame's geometry: the reference's TLAS instances
+// (one per BLAS bucket, ASManager::SetupTLASInstanceFromBLAS,
+// Source/ASManager.cpp:837-963) reduced to what decides visibility, their
+// instance mask and whether rayCullMaskWorld drops them.  A frame's
+// triangles are ordered by class, each class one segment of the frame BVH.
+enum GeomClass : uint32_t
+{
+    GC_WORLD_0     = 0, // PV_WORLD_0, opaque or alpha tested
+    GC_WORLD_2     = 1, // RG_MESH_PRIMITIVE_SKY_VISIBILITY (PV_WORLD_2)
+    GC_REFRACT_W0  = 2, // PT_REFRACT of world 0: mask REFRACT
+    GC_REFRACT_W2  = 3, // PT_REFRACT of world 2: mask REFRACT, dropped with world 2
+    GC_FIRST_PERSON        = 4, // PV_FIRST_PERSON (refracting or not)
+    GC_FIRST_PERSON_VIEWER = 5, // PV_FIRST_PERSON_VIEWER (refracting or not)
+    GC_COUNT       = 6,
+};
+
+// Instance masks and custom-index flags (Source/Generated/ShaderCommonC.h:60-71).
+constexpr uint32_t IMASK_WORLD_0 = 1u << 0, IMASK_WORLD_1 = 1u << 1, IMASK_WORLD_2 = 1u << 2;
+constexpr uint32_t IMASK_REFRACT = 1u << 5, IMASK_FIRST_PERSON = 1u << 6, IMASK_FIRST_PERSON_VIEWER = 1u << 7;
+constexpr uint32_t CUSTOM_DYNAMIC = 1u << 0, CUSTOM_FIRST_PERSON = 1u << 1, CUSTOM_FIRST_PERSON_VIEWER = 1u << 2, CUSTOM_SKY = 1u << 3;
+
+// Refracting primitives: water, glass, glass-if-smooth and acid that are not
+// alpha tested (VertexCollectorFilterType.cpp:215-229).
+__host__ __device__ inline bool IsRefractPrimitive( uint32_t flags )
+{
+    return ( flags & RG_MESH_PRIMITIVE_ALPHA_TESTED ) == 0u &&
+           ( flags & ( RG_MESH_PRIMITIVE_WATER | RG_MESH_PRIMITIVE_GLASS | RG_MESH_PRIMITIVE_GLASS_IF_SMOOTH | RG_MESH_PRIMITIVE_ACID ) ) != 0u;
+}
+
+// VertexCollectorFilterTypeFlags_GetForGeometry (VertexCollectorFilterType.cpp:
+// 233-251): first person before first-person viewer before sky visibility.
+__host__ __device__ inline uint32_t GeomClassOf( uint32_t flags )
+{
+    if( flags & RG_MESH_PRIMITIVE_FIRST_PERSON )
+    {
+        return GC_FIRST_PERSON;
+    }
+    if( flags & RG_MESH_PRIMITIVE_FIRST_PERSON_VIEWER )
+    {
+        return GC_FIRST_PERSON_VIEWER;
+    }
+    const bool w2 = ( flags & RG_MESH_PRIMITIVE_SKY_VISIBILITY ) != 0u;
+    if( IsRefractPrimitive( flags ) )
+    {
+        return w2 ? GC_REFRACT_W2 : GC_REFRACT_W0;
+    }
+    return w2 ? GC_WORLD_2 : GC_WORLD_0;
+}
+
+// The class's instance mask: first-person masks stay as they are, world
+// refracting instances are rewritten to REFRACT (ASManager.cpp:866-933).
+__host__ __device__ inline uint32_t GeomClassMask( uint32_t c )
+{
+    switch( c )
+    {
+        case GC_WORLD_0: return IMASK_WORLD_0;
+        case GC_WORLD_2: return IMASK_WORLD_2;
+        case GC_REFRACT_W0:
+        case GC_REFRACT_W2: return IMASK_REFRACT;
+        case GC_FIRST_PERSON: return IMASK_FIRST_PERSON;
+        default: return IMASK_FIRST_PERSON_VIEWER;
+    }
+}
+
+// Whether the class is in the frame's TLAS: world instances whose world bit
+// is missing from rayCullMaskWorld are left out (ASManager.cpp:880-918).
+__host__ __device__ inline bool GeomClassPresent( uint32_t c, uint32_t worldMask )
+{
+    switch( c )
+    {
+        case GC_WORLD_0:
+        case GC_REFRACT_W0: return ( worldMask & IMASK_WORLD_0 ) != 0u;
+        case GC_WORLD_2:
+        case GC_REFRACT_W2: return ( worldMask & IMASK_WORLD_2 ) != 0u;
+        default: return true;
+    }
+}
+
+// instanceCustomIndex of a primitive's instance (ASManager.cpp:859-912): every
+// uploaded primitive is dynamic; world-2 instances carry SKY when the
+// instance allows geometry with the sky flag.
+__host__ __device__ inline uint32_t CustomIndexOf( uint32_t flags, bool skyGeometry )
+{
+    const uint32_t c = GeomClassOf( flags );
+    uint32_t       r = CUSTOM_DYNAMIC;
+    if( c == GC_FIRST_PERSON )
+    {
+        r |= CUSTOM_FIRST_PERSON;
+    }
+    else if( c == GC_FIRST_PERSON_VIEWER )
+    {
+        r |= CUSTOM_FIRST_PERSON_VIEWER;
+    }
+    else if( skyGeometry && ( c == GC_WORLD_2 || c == GC_REFRACT_W2 ) )
+    {
+        r |= CUSTOM_SKY;
+    }
+    return r;
+}
+
 struct SceneCollector
 {
     PinnedArray< RgPrimitiveVertex > vertices;
     PinnedArray< uint32_t >          indices;
     PinnedArray< GeomRecord >        geoms;
     uint32_t                         triCount = 0;
-    // Water, glass and acid primitives (the reference's PT_REFRACT BLAS,
-    // VertexCollectorFilterType.cpp:215-229) collect here and follow all other
-    // geometry once Seal() runs, so their triangles form one trailing range
-    // that shadow and bounce rays skip.
-    PinnedArray< GeomRecord >        refrGeoms;
-    std::vector< uint64_t >          refrUids;
-    uint32_t                         refrTriCount = 0;
+    // Geometry of world 0 (the common class) collects in geoms directly,
+    // the other classes here; Seal() appends them in class order, so each
+    // class is one trailing range of triangles.
+    std::vector< GeomRecord >        classGeoms[ GC_COUNT ];
+    std::vector< uint64_t >          classUids[ GC_COUNT ];
+    uint32_t                         classTris[ GC_COUNT ] = {};
     bool                             sealed       = false;
     const struct TextureManager*     textures     = nullptr; // resolves pTextureName at upload
 
@@ -154,7 +253,6 @@ struct SceneCollector
     // Appends the refracting geometry after the rest (idempotent until the
     // next BeginFrame); UploadScene and GeomPrevToCur call it.
     void Seal();
-    uint32_t OpaqueTriCount() const { return triCount - refrTriCount; }
     void AddDirectional( const RgDirectionalLightUploadInfo& info );
     void AddSphere( const RgSphericalLightUploadInfo& info );
     void AddSpot( const RgSpotLightUploadInfo& info );
@@ -163,6 +261,10 @@ struct SceneCollector
     float LightStyle( const RgLightExtraInfo& extra ) const;
     // Builds prevToCur; call once all lights of the frame are added.
     void FinishLights();
+    // This frame's array index of the light with a uniqueID, LIGHT_INDEX_NONE
+    // if none or no id (LightManager::GetLightIndexForShaders,
+    // Source/LightManager.cpp:673-689).
+    uint32_t LightIndexOf( const uint64_t* uniqueId ) const;
     // Last frame's geometry index -> this frame's, 0xFFFFFFFF when gone
     // (getCurrentGeometryIndexByPrev, Shaders/VertexData.inl:636-658).
     void GeomPrevToCur( std::vector< uint32_t >& out );
@@ -177,23 +279,17 @@ struct DeviceScene
     DeviceBuffer triGeom;    // n u32
     DeviceBuffer trisPrev;   // last built copy, to skip identical rebuilds
     DeviceBuffer diffFlag;   // 1 u32
-    uint32_t     triCount = 0, geomCount = 0, builtTriCount = 0;
-    uint32_t     opaqueTriCount = 0; // triangles [opaqueTriCount, triCount) refract
-    uint32_t     builtOpaqueTriCount = 0;
+    DeviceBuffer lastTri;    // per vertex: 1 + the last triangle of its geometry that uses it (flat normals)
+    uint32_t     triCount = 0, geomCount = 0, builtTriCount = 0, vertexCount = 0;
+    uint32_t     classTris[ GC_COUNT ] = {};      // triangles per GeomClass, in class order
+    uint32_t     builtClassTris[ GC_COUNT ] = {}; // the same for the last BVH build
+    bool         generateNormals = false;         // some geometry generates flat normals
     bool         reflRefrGeometry = false; // some geometry reflects or refracts
     bool         alphaTested      = false; // some geometry is alpha tested with a texture
     DeviceBuffer alphaRecs;                // n AlphaRec when alphaTested
     TexView      tex;                      // the instance's textures for this frame
     bool         bvhValid = false;
 };
-
-// Refracting primitives: water, glass, glass-if-smooth and acid that are not
-// alpha tested (VertexCollectorFilterType.cpp:215-229).
-inline bool IsRefractPrimitive( uint32_t flags )
-{
-    return ( flags & RG_MESH_PRIMITIVE_ALPHA_TESTED ) == 0u &&
-           ( flags & ( RG_MESH_PRIMITIVE_WATER | RG_MESH_PRIMITIVE_GLASS | RG_MESH_PRIMITIVE_GLASS_IF_SMOOTH | RG_MESH_PRIMITIVE_ACID ) ) != 0u;
-}
 
 // Copies the collected geometry to the device (asynchronous).
 void UploadScene( SceneCollector& sc, DeviceScene& ds, hipStream_t s );

@@ -75,11 +75,14 @@ void SceneCollector::BeginFrame()
     vertices.Clear();
     indices.Clear();
     geoms.Clear();
-    refrGeoms.Clear();
-    refrUids.clear();
-    triCount     = 0;
-    refrTriCount = 0;
-    sealed       = false;
+    for( uint32_t c = 0; c < GC_COUNT; c++ )
+    {
+        classGeoms[ c ].clear();
+        classUids[ c ].clear();
+        classTris[ c ] = 0;
+    }
+    triCount = 0;
+    sealed   = false;
     prevModels.swap( curModels );
     curModels.clear();
     prevGeomUids.swap( geomUids );
@@ -169,13 +172,13 @@ void SceneCollector::AddPrimitive( const RgMeshInfo& mesh, const RgMeshPrimitive
     RG_REQUIRE( uint64_t( triCount ) + tris < ( 1ull << 28 ), RG_RESULT_WRONG_FUNCTION_ARGUMENT, "too many triangles in one frame" );
 
     RG_REQUIRE( !sealed, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "primitive uploaded after the frame's geometry was sealed" );
-    const bool refract = IsRefractPrimitive( prim.flags );
+    const uint32_t cls = GeomClassOf( prim.flags );
 
     GeomRecord g{};
     std::memcpy( g.model, &mesh.transform.matrix[ 0 ][ 0 ], sizeof( g.model ) );
     g.baseVertex = uint32_t( vertices.size );
     g.baseIndex  = 0xFFFFFFFFu;
-    g.firstTri   = refract ? refrTriCount : triCount - refrTriCount; // within its own list until Seal()
+    g.firstTri   = classTris[ cls ]; // within its class until Seal()
     g.triCount   = tris;
     g.color      = prim.color;
     g.emissive   = Saturate( prim.emissive );
@@ -220,17 +223,17 @@ void SceneCollector::AddPrimitive( const RgMeshInfo& mesh, const RgMeshPrimitive
         g.baseIndex = uint32_t( indices.size );
         StagingCopy( indices.Append( prim.indexCount ), prim.pIndices, size_t( prim.indexCount ) * 4 );
     }
-    if( refract )
-    {
-        *refrGeoms.Append( 1 ) = g;
-        refrUids.push_back( uid );
-        refrTriCount += tris;
-    }
-    else
+    if( cls == GC_WORLD_0 )
     {
         *geoms.Append( 1 ) = g;
         geomUids.push_back( uid );
     }
+    else
+    {
+        classGeoms[ cls ].push_back( g );
+        classUids[ cls ].push_back( uid );
+    }
+    classTris[ cls ] += tris;
     triCount += tris;
 }
 
@@ -240,17 +243,20 @@ void SceneCollector::Seal()
     {
         return;
     }
-    sealed              = true;
-    const uint32_t base = triCount - refrTriCount;
-    for( size_t i = 0; i < refrGeoms.size; i++ )
+    sealed        = true;
+    uint32_t base = classTris[ GC_WORLD_0 ];
+    for( uint32_t c = GC_WORLD_0 + 1; c < GC_COUNT; c++ )
     {
-        GeomRecord g = refrGeoms.data[ i ];
-        g.firstTri += base;
-        *geoms.Append( 1 ) = g;
+        for( GeomRecord g : classGeoms[ c ] )
+        {
+            g.firstTri += base;
+            *geoms.Append( 1 ) = g;
+        }
+        geomUids.insert( geomUids.end(), classUids[ c ].begin(), classUids[ c ].end() );
+        classGeoms[ c ].clear();
+        classUids[ c ].clear();
+        base += classTris[ c ];
     }
-    geomUids.insert( geomUids.end(), refrUids.begin(), refrUids.end() );
-    refrGeoms.Clear();
-    refrUids.clear();
 }
 
 // Light encoders: LightManager.cpp:99-234.
@@ -394,6 +400,22 @@ void SceneCollector::FinishLights()
     }
 }
 
+uint32_t SceneCollector::LightIndexOf( const uint64_t* uniqueId ) const
+{
+    if( !uniqueId )
+    {
+        return LIGHT_INDEX_NONE;
+    }
+    for( uint32_t i = 0; i < lights.size(); i++ )
+    {
+        if( lights[ i ].lightType != LIGHT_TYPE_NONE && lightIds[ i ] == *uniqueId )
+        {
+            return i;
+        }
+    }
+    return LIGHT_INDEX_NONE;
+}
+
 void SceneCollector::GeomPrevToCur( std::vector< uint32_t >& out )
 {
     Seal();
@@ -493,6 +515,77 @@ __global__ __launch_bounds__( THREADS ) void k_assemble( const RgPrimitiveVertex
     }
 }
 
+// Flat normals (CmVertexPreprocess.comp, VertexPreprocessPartial.inl:43-120):
+// a geometry without RG_MESH_PRIMITIVE_DONT_GENERATE_NORMALS
+// (GeomInfoManager.cpp:173-176) gets normalize(cross(p1 - p0, p2 - p0)) of
+// its local positions written to the vertices of each triangle, triangles
+// in index order, so a vertex shared by several triangles keeps the last
+// one's normal.  The reference loops over a geometry's triangles in one
+// thread; here one thread per triangle, with the last triangle of each
+// vertex found first (k_flat_last) so every vertex has one writer.
+// GEOM_INST_FLAG_INVERTED_NORMALS is never set by the reference's
+// GeomInfoManager, so the sign is +1.
+__device__ __forceinline__ bool GeneratesNormals( const GeomRecord& g )
+{
+    return ( g.flags & RG_MESH_PRIMITIVE_DONT_GENERATE_NORMALS ) == 0u;
+}
+
+__global__ __launch_bounds__( THREADS ) void k_flat_last( const uint32_t* indices, const GeomRecord* geoms, const uint32_t* triGeom,
+                                                          uint32_t triCount, uint32_t* lastTri )
+{
+    const uint32_t t = blockIdx.x * THREADS + threadIdx.x;
+    if( t >= triCount )
+    {
+        return;
+    }
+    const GeomRecord& g = geoms[ triGeom[ t ] ];
+    if( !GeneratesNormals( g ) || g.baseIndex == 0xFFFFFFFFu )
+    {
+        return;
+    }
+    const uint32_t i = t - g.firstTri;
+#pragma unroll
+    for( int k = 0; k < 3; k++ )
+    {
+        atomicMax( &lastTri[ g.baseVertex + indices[ g.baseIndex + 3 * i + k ] ], i + 1u );
+    }
+}
+
+__global__ __launch_bounds__( THREADS ) void k_flat_write( RgPrimitiveVertex* verts, const uint32_t* indices, const GeomRecord* geoms,
+                                                           const uint32_t* triGeom, uint32_t triCount, const uint32_t* lastTri )
+{
+    const uint32_t t = blockIdx.x * THREADS + threadIdx.x;
+    if( t >= triCount )
+    {
+        return;
+    }
+    const GeomRecord& g = geoms[ triGeom[ t ] ];
+    if( !GeneratesNormals( g ) )
+    {
+        return;
+    }
+    const uint32_t i = t - g.firstTri;
+    uint32_t       v[ 3 ];
+    v3             p[ 3 ];
+#pragma unroll
+    for( int k = 0; k < 3; k++ )
+    {
+        v[ k ] = g.baseVertex + ( g.baseIndex == 0xFFFFFFFFu ? 3 * i + k : indices[ g.baseIndex + 3 * i + k ] );
+        p[ k ] = mk3( verts[ v[ k ] ].position[ 0 ], verts[ v[ k ] ].position[ 1 ], verts[ v[ k ] ].position[ 2 ] );
+    }
+    const v3 n = normalize( cross( p[ 1 ] - p[ 0 ], p[ 2 ] - p[ 0 ] ) );
+#pragma unroll
+    for( int k = 0; k < 3; k++ )
+    {
+        if( g.baseIndex == 0xFFFFFFFFu || lastTri[ v[ k ] ] == i + 1u )
+        {
+            verts[ v[ k ] ].normal[ 0 ] = n.x;
+            verts[ v[ k ] ].normal[ 1 ] = n.y;
+            verts[ v[ k ] ].normal[ 2 ] = n.z;
+        }
+    }
+}
+
 __global__ __launch_bounds__( THREADS ) void k_compare( const uint32_t* a, const uint32_t* b, size_t words, uint32_t* diff )
 {
     bool d = false;
@@ -512,11 +605,13 @@ void UploadScene( SceneCollector& sc, DeviceScene& ds, hipStream_t s )
     sc.Seal();
     const uint32_t n = sc.triCount;
     ds.triCount       = n;
-    ds.opaqueTriCount = sc.OpaqueTriCount();
+    std::memcpy( ds.classTris, sc.classTris, sizeof( ds.classTris ) );
     ds.reflRefrGeometry = false;
     ds.alphaTested      = false;
+    ds.generateNormals  = false;
     for( size_t i = 0; i < sc.geoms.size; i++ )
     {
+        ds.generateNormals = ds.generateNormals || ( sc.geoms.data[ i ].flags & RG_MESH_PRIMITIVE_DONT_GENERATE_NORMALS ) == 0u;
         ds.alphaTested = ds.alphaTested || ( ( sc.geoms.data[ i ].flags & RG_MESH_PRIMITIVE_ALPHA_TESTED ) && sc.geoms.data[ i ].texture != TEX_NONE );
         constexpr uint32_t RR = RG_MESH_PRIMITIVE_MIRROR | RG_MESH_PRIMITIVE_WATER | RG_MESH_PRIMITIVE_GLASS | RG_MESH_PRIMITIVE_ACID |
                                 RG_MESH_PRIMITIVE_GLASS_IF_SMOOTH | RG_MESH_PRIMITIVE_MIRROR_IF_SMOOTH;
@@ -527,6 +622,7 @@ void UploadScene( SceneCollector& sc, DeviceScene& ds, hipStream_t s )
     {
         return;
     }
+    ds.vertexCount = uint32_t( sc.vertices.size );
     ds.vertices.Reserve( sc.vertices.size * sizeof( RgPrimitiveVertex ) );
     ds.indices.Reserve( ( sc.indices.size + 1 ) * 4 );
     ds.geoms.Reserve( sc.geoms.size * sizeof( GeomRecord ) );
@@ -560,6 +656,19 @@ bool AssembleScene( DeviceScene& ds, hipStream_t s )
 
     k_tri_geom<<< DivUp( n, THREADS ), THREADS, 0, s >>>( ds.geoms.as< GeomRecord >(), ds.geomCount, n, ds.triGeom.as< uint32_t >() );
     CheckLaunch( "k_tri_geom" );
+    if( ds.generateNormals )
+    {
+        // vertex preprocessing before the acceleration structure, as
+        // Scene::SubmitForFrame does (Source/Scene.cpp:99-100)
+        ds.lastTri.Reserve( size_t( ds.vertexCount ) * 4 + 4 );
+        RG_HIP_CHECK( hipMemsetAsync( ds.lastTri.ptr, 0, size_t( ds.vertexCount ) * 4, s ) );
+        k_flat_last<<< DivUp( n, THREADS ), THREADS, 0, s >>>( ds.indices.as< uint32_t >(), ds.geoms.as< GeomRecord >(), ds.triGeom.as< uint32_t >(), n,
+                                                               ds.lastTri.as< uint32_t >() );
+        CheckLaunch( "k_flat_last" );
+        k_flat_write<<< DivUp( n, THREADS ), THREADS, 0, s >>>( ds.vertices.as< RgPrimitiveVertex >(), ds.indices.as< uint32_t >(),
+                                                                ds.geoms.as< GeomRecord >(), ds.triGeom.as< uint32_t >(), n, ds.lastTri.as< uint32_t >() );
+        CheckLaunch( "k_flat_write" );
+    }
     if( ds.alphaTested )
     {
         ds.alphaRecs.Reserve( size_t( n ) * sizeof( AlphaRec ) );

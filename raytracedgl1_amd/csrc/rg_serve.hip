@@ -494,7 +494,7 @@ RgResult rgExtSceneBuild( RgExtScene scene )
         if( scene->ds.triCount > 0 )
         {
             RgExtBvhBuildParams bp = { 1, s.nodeFormat };
-            BuildFrameBvh( scene->bvh, scene->ds.tris.as< float >(), scene->ds.triCount, scene->ds.opaqueTriCount, bp, s.setup );
+            BuildFrameBvh( scene->bvh, scene->ds.tris.as< float >(), scene->ds.classTris, GC_COUNT, bp, s.setup );
             scene->ds.bvhValid      = true;
             scene->ds.builtTriCount = scene->ds.triCount;
         }

@@ -58,12 +58,22 @@ struct TraceArgs
     // the texture table
     const AlphaRec* alpha;
     TexView         tex;
+    // frame launches (rootCount > 0): a ray starts at the entry roots whose
+    // instance mask shares a bit with its cull mask (rayMasks[ray], or
+    // defaultMask); with several, their boxes are slab-tested first, like
+    // the children of a node, without counting a node visit
+    const uint8_t*  rayMasks;
+    uint32_t        defaultMask;
+    uint32_t        rootCount;
+    uint32_t        rootMask[ 8 ];
+    uint32_t        rootRef[ 8 ];
+    float           rootBox[ 8 ][ 6 ];
 };
 
 // RtAlphaTest.rahit: the layer-0 colour times the texture at the base level
 // (an any-hit stage has no derivatives); the hit is ignored when
 // (r+g+b)/3 * a + a < 0.5.
-__device__ __noinline__ bool AlphaPasses( const TexView& tv, const AlphaRec& ar, float u, float v )
+__device__ __forceinline__ bool AlphaPasses( const TexView& tv, const AlphaRec& ar, float u, float v )
 {
     const float  b0 = 1.0f - u - v;
     const float  tu = ar.uv[ 0 ] * b0 + ar.uv[ 2 ] * u + ar.uv[ 4 ] * v;
@@ -187,6 +197,23 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
     uint32_t chunkNext = 0, chunkEnd = 0; // this wave's rays [chunkNext, chunkEnd), wave-uniform
     bool     drained   = false;           // the global counter has passed the last ray
 
+    auto push = [ & ]( uint32_t v ) __attribute__( ( always_inline ) ) {
+        if( sp < STACK_LDS )
+        {
+            s_stack[ sp * TR_THREADS + tid ] = v;
+        }
+        else if( sp < STACK_TOTAL )
+        {
+            ovf[ sp - STACK_LDS ] = v;
+        }
+        else
+        {
+            atomicOr( a.errorFlag, RG_EXT_TRACE_ERROR_STACK_OVERFLOW ); // dropped entry
+            return;
+        }
+        sp++;
+    };
+
     while( true )
     {
         // Refill idle lanes.  Each wave owns a chunk of a.chunk consecutive
@@ -245,6 +272,55 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
                     sp            = 0;
                     steps         = 0;
                     active        = true;
+                    if( a.rootCount )
+                    {
+                        // frame launch: the entry roots the ray's cull mask admits
+                        const uint32_t rm      = a.rayMasks ? uint32_t( a.rayMasks[ my ] ) : a.defaultMask;
+                        uint32_t       admit   = 0, only = RG_EXT_BVH_CHILD_EMPTY;
+#pragma unroll
+                        for( int k = 0; k < 8; k++ )
+                        {
+                            if( uint32_t( k ) < a.rootCount && ( a.rootMask[ k ] & rm ) != 0u )
+                            {
+                                admit++;
+                                only = a.rootRef[ k ];
+                            }
+                        }
+                        node = only;
+                        if( admit > 1u )
+                        {
+                            // several segments: those whose box the ray
+                            // meets, in segment order (closest hit does
+                            // not depend on the order)
+                            node = RG_EXT_BVH_CHILD_EMPTY;
+#pragma unroll
+                            for( int k = 7; k >= 0; k-- )
+                            {
+                                float tn;
+                                if( uint32_t( k ) < a.rootCount && ( a.rootMask[ k ] & rm ) != 0u &&
+                                    SlabTest( ray, a.rootBox[ k ][ 0 ], a.rootBox[ k ][ 1 ], a.rootBox[ k ][ 2 ], a.rootBox[ k ][ 3 ],
+                                              a.rootBox[ k ][ 4 ], a.rootBox[ k ][ 5 ], tmin, bestT, &tn ) )
+                                {
+                                    if( node != RG_EXT_BVH_CHILD_EMPTY )
+                                    {
+                                        push( node );
+                                    }
+                                    node = a.rootRef[ k ];
+                                }
+                            }
+                        }
+                        if( node == RG_EXT_BVH_CHILD_EMPTY )
+                        {
+                            // no segment to visit: a miss
+                            RgExtHit h;
+                            h.t         = bestT;
+                            h.u         = 0.0f;
+                            h.v         = 0.0f;
+                            h.primitiveId = RG_EXT_MISS;
+                            a.hits[ my ] = h;
+                            active       = false;
+                        }
+                    }
                 }
                 else
                 {
@@ -268,6 +344,10 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
         }
         if( __ballot( active ) == 0ull )
         {
+            if( __ballot( !exhausted ) != 0ull )
+            {
+                continue; // rays that ended at the frame roots: take the next ones
+            }
             if( a.stats )
             {
                 uint64_t nv = nodeVisits, tt = triTests;
@@ -349,22 +429,6 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
                     hitCount += h ? 1 : 0;
                 }
             }
-            auto push = [ & ]( uint32_t v ) {
-                if( sp < STACK_LDS )
-                {
-                    s_stack[ sp * TR_THREADS + tid ] = v;
-                }
-                else if( sp < STACK_TOTAL )
-                {
-                    ovf[ sp - STACK_LDS ] = v;
-                }
-                else
-                {
-                    atomicOr( a.errorFlag, RG_EXT_TRACE_ERROR_STACK_OVERFLOW ); // dropped entry
-                    return;
-                }
-                sp++;
-            };
             if( hitCount == 0 )
             {
                 node = RG_EXT_BVH_CHILD_EMPTY;
@@ -552,11 +616,22 @@ void InitGrids( TraceContext& s )
     s.gridBlocks = maxGrid;
 }
 
+// Entry roots of a frame launch (rootCount 0: none).
+struct FrameRoots
+{
+    uint32_t       rootCount = 0;
+    uint32_t       mask[ 8 ] = {};
+    uint32_t       ref[ 8 ]  = {};
+    float          box[ 8 ][ 6 ] = {};
+    const uint8_t* rayMasks    = nullptr;
+    uint32_t       defaultMask = 0;
+};
+
 void TraceRaysImpl( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags,
                     hipStream_t stream, hipEvent_t evBefore, hipEvent_t evAfter, const TraceAccum* accum, const uint32_t* d_segRays,
-                    const uint32_t* d_frameRoot = nullptr, const AlphaRec* alpha = nullptr, const TexView* tex = nullptr )
+                    const FrameRoots* frame = nullptr, const AlphaRec* alpha = nullptr, const TexView* tex = nullptr )
 {
-    RG_REQUIRE( bvh.nodeCount > 0, RG_RESULT_WRONG_FUNCTION_CALL, "BVH is not built" );
+    RG_REQUIRE( bvh.nodeCount > 0 || ( frame && frame->rootCount > 0 ), RG_RESULT_WRONG_FUNCTION_CALL, "BVH is not built" );
     RG_REQUIRE( bvh.nodeFormat == RG_EXT_BVH_NODE_FP32 || bvh.nodeFormat == RG_EXT_BVH_NODE_FP16, RG_RESULT_WRONG_FUNCTION_ARGUMENT,
                 "unsupported node format" );
     // the chunk counter runs up to count + (waves + 1) * chunk without wrapping
@@ -590,15 +665,32 @@ void TraceRaysImpl( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_ray
     a.segRoots      = d_segRays ? bvh.sceneRoots8.as< uint32_t >() : nullptr;
     a.segTris       = d_segRays ? bvh.sceneOffsets.as< uint32_t >() : nullptr;
     a.segCount      = d_segRays ? bvh.sceneCount : 0u;
-    if( d_frameRoot )
+    a.rayMasks    = nullptr;
+    a.defaultMask = 0;
+    a.rootCount   = 0;
+    for( int k = 0; k < 8; k++ )
     {
-        // one segment: every ray starts at *d_frameRoot; the third word of
-        // the frame roots is 0, so primitive ids stay global (a single
-        // segment never reads segRays)
-        a.segRays  = d_frameRoot;
-        a.segRoots = d_frameRoot;
-        a.segTris  = bvh.frameRoots.as< uint32_t >() + 2;
-        a.segCount = 1;
+        a.rootMask[ k ] = 0;
+        a.rootRef[ k ]  = RG_EXT_BVH_CHILD_EMPTY;
+        for( int j = 0; j < 6; j++ )
+        {
+            a.rootBox[ k ][ j ] = 0.0f;
+        }
+    }
+    if( frame )
+    {
+        a.rayMasks    = frame->rayMasks;
+        a.defaultMask = frame->defaultMask;
+        a.rootCount   = frame->rootCount;
+        for( uint32_t k = 0; k < frame->rootCount && k < 8; k++ )
+        {
+            a.rootMask[ k ] = frame->mask[ k ];
+            a.rootRef[ k ]  = frame->ref[ k ];
+            for( int j = 0; j < 6; j++ )
+            {
+                a.rootBox[ k ][ j ] = frame->box[ k ][ j ];
+            }
+        }
     }
     a.chunk         = s.rayChunk >= 16 && s.rayChunk <= 1024 ? s.rayChunk : RAY_CHUNK_DEFAULT;
     a.alpha         = alpha && tex ? alpha : nullptr;
@@ -634,32 +726,47 @@ void TraceRays( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, u
     TraceRaysImpl( bvh, s, d_rays, count, d_hits, flags, stream, evBefore, evAfter, accum, nullptr );
 }
 
-void TraceRaysFrame( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags, bool opaqueOnly,
-                     hipStream_t stream, hipEvent_t evBefore, hipEvent_t evAfter, const TraceAccum* accum, const AlphaRec* alpha,
-                     const TexView* tex )
+void TraceRaysFrame( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, const uint8_t* d_rayMasks, uint32_t defaultMask, uint32_t count,
+                     RgExtHit* d_hits, uint32_t flags, const uint32_t* classMasks, uint32_t classCount, hipStream_t stream, hipEvent_t evBefore,
+                     hipEvent_t evAfter, const TraceAccum* accum, const AlphaRec* alpha, const TexView* tex )
 {
-    if( bvh.frameMode == BvhObject::FRAME_SPLIT )
+    RG_REQUIRE( !bvh.frameSegs.empty(), RG_RESULT_WRONG_FUNCTION_CALL, "not a frame BVH" );
+    FrameRoots fr;
+    fr.rayMasks    = d_rayMasks;
+    fr.defaultMask = defaultMask;
+    for( const auto& seg : bvh.frameSegs )
     {
-        TraceRaysImpl( bvh, s, d_rays, count, d_hits, flags, stream, evBefore, evAfter, accum, nullptr,
-                       bvh.frameRoots.as< uint32_t >() + ( opaqueOnly ? 1 : 0 ), alpha, tex );
-        return;
+        const uint32_t m = seg.cls < classCount ? classMasks[ seg.cls ] : 0u;
+        if( m == 0u )
+        {
+            continue; // not in this frame's TLAS
+        }
+        fr.mask[ fr.rootCount ] = m;
+        fr.ref[ fr.rootCount ]  = seg.root;
+        for( int j = 0; j < 6; j++ )
+        {
+            fr.box[ fr.rootCount ][ j ] = seg.box[ j ];
+        }
+        fr.rootCount++;
     }
-    RG_REQUIRE( bvh.sceneCount == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "a batched BVH is traced with TraceRaysBatch" );
-    if( opaqueOnly && bvh.frameMode == BvhObject::FRAME_REFRACT_ONLY )
+    if( fr.rootCount == 0 || count == 0 )
     {
-        // nothing opaque to hit
+        // nothing in the TLAS: every ray misses
         if( evBefore )
         {
             RG_HIP_CHECK( hipEventRecord( evBefore, stream ) );
         }
-        RG_HIP_CHECK( hipMemsetAsync( d_hits, 0xFF, size_t( count ) * sizeof( RgExtHit ), stream ) );
+        if( count )
+        {
+            RG_HIP_CHECK( hipMemsetAsync( d_hits, 0xFF, size_t( count ) * sizeof( RgExtHit ), stream ) );
+        }
         if( evAfter )
         {
             RG_HIP_CHECK( hipEventRecord( evAfter, stream ) );
         }
         return;
     }
-    TraceRaysImpl( bvh, s, d_rays, count, d_hits, flags, stream, evBefore, evAfter, accum, nullptr, nullptr, alpha, tex );
+    TraceRaysImpl( bvh, s, d_rays, count, d_hits, flags, stream, evBefore, evAfter, accum, nullptr, &fr, alpha, tex );
 }
 
 void TraceRaysBatch( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, const uint32_t* rayOffsets, RgExtHit* d_hits, uint32_t flags,

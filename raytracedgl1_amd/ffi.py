@@ -95,7 +95,9 @@ class RgExtFrameParams(C.Structure):
         ("reflectRefractMaxDepth", C.c_uint32), ("cameraMediaType", C.c_uint32),
         ("indexOfRefractionGlass", C.c_float), ("indexOfRefractionWater", C.c_float), ("thinMediaWidth", C.c_float),
         ("cameraRayConeSpreadAngle", C.c_float), ("normalMapStrength", C.c_float), ("waterWaveStrength", C.c_float), ("waterColorAndDensity", C.c_float * 4), ("acidColorAndDensity", C.c_float * 4),
-        ("worldUp", C.c_float * 4), ("emissionMaxScreenColor", C.c_float), ("pad1", C.c_float * 3),
+        ("worldUp", C.c_float * 4), ("emissionMaxScreenColor", C.c_float),
+        ("rayCullMaskWorld", C.c_uint32), ("rayCullMaskWorldShadow", C.c_uint32), ("lightIndexIgnoreFPVShadows", C.c_uint32),
+        ("skyGeometry", C.c_uint32), ("pad2", C.c_uint32 * 3),
     ]
 
 

@@ -182,6 +182,8 @@ RG_STRUCTURE_TYPE_TONEMAPPING = 4
 RG_STRUCTURE_TYPE_SKY = 7
 RG_STRUCTURE_TYPE_REFLECTREFRACT = 6
 # RgMeshPrimitiveFlags (RTGL1.h) used by the reflection / refraction chain
+RG_MESH_PRIMITIVE_FIRST_PERSON = 4
+RG_MESH_PRIMITIVE_FIRST_PERSON_VIEWER = 8
 RG_MESH_PRIMITIVE_MIRROR = 32
 RG_MESH_PRIMITIVE_GLASS = 64
 RG_MESH_PRIMITIVE_WATER = 128
@@ -190,10 +192,16 @@ RG_MESH_PRIMITIVE_MIRROR_IF_SMOOTH = 1024
 RG_MESH_PRIMITIVE_GLASS_IF_SMOOTH = 2048
 RG_MESH_PRIMITIVE_ACID = 8192
 RG_MESH_PRIMITIVE_THIN_MEDIA = 16384
+RG_MESH_PRIMITIVE_DONT_GENERATE_NORMALS = 256
+RG_MESH_PRIMITIVE_IGNORE_REFRACT_AFTER = 4096
+RG_MESH_PRIMITIVE_SKY_VISIBILITY = 32768
 # RgMediaType
 RG_MEDIA_TYPE_VACUUM, RG_MEDIA_TYPE_WATER, RG_MEDIA_TYPE_GLASS, RG_MEDIA_TYPE_ACID = 0, 1, 2, 3
 RG_EXT_STRUCTURE_TYPE_PATH_PARAMS = 0x52470001
 RG_DRAW_FRAME_RAY_CULL_WORLD_0_BIT = 1
+RG_DRAW_FRAME_RAY_CULL_WORLD_1_BIT = 2
+RG_DRAW_FRAME_RAY_CULL_WORLD_2_BIT = 4
+RG_DRAW_FRAME_RAY_CULL_SKY_BIT = 8
 
 VERTEX_DTYPE = np.dtype([("position", "<f4", 3), ("_p0", "<u4"), ("normal", "<f4", 3), ("_p1", "<u4"),
                          ("tangent", "<f4", 4), ("texCoord", "<f4", 2), ("color", "<u4"), ("_p2", "<u4")])
@@ -252,15 +260,16 @@ def _style(info, light: dict) -> None:
         info.extra.lightstyle = int(light["lightstyle"])
 
 
-def draw_info(cam, current_time: float, params_ptr) -> RgDrawFrameInfo:
-    """RgDrawFrameInfo for a camera (world culling mask 0, ray length 10000)."""
+def draw_info(cam, current_time: float, params_ptr, cull_mask: int = RG_DRAW_FRAME_RAY_CULL_WORLD_0_BIT) -> RgDrawFrameInfo:
+    """RgDrawFrameInfo for a camera (ray length 10000; by default only
+    world 0 is traced, as Tests/RtglTest.cpp:689 does)."""
     di = RgDrawFrameInfo()
     di.view[:] = cam.view().tolist()
     di.fovYRadians = cam.fovy
     di.cameraNear = cam.znear
     di.cameraFar = cam.zfar
     di.rayLength = 10000.0
-    di.rayCullMaskWorld = RG_DRAW_FRAME_RAY_CULL_WORLD_0_BIT
+    di.rayCullMaskWorld = cull_mask
     di.currentTime = current_time
     di.pParams = params_ptr
     return di
@@ -348,11 +357,12 @@ class SceneUploader:
 class Renderer:
     """One RTGL1 instance rendering a synthetic scene at a fixed render size."""
 
-    def __init__(self, width: int, height: int, cull_back: bool = False):
+    def __init__(self, width: int, height: int, cull_back: bool = False, allow_sky: bool = False):
         l = api()
         ci = RgInstanceCreateInfo()
         ci.pAppName = b"raytracedgl1_amd"
         ci.rayCullBackFacingTriangles = 1 if cull_back else 0
+        ci.allowGeometryWithSkyFlag = 1 if allow_sky else 0
         h = _P()
         ffi.check("rgCreateInstance", l.rgCreateInstance(C.byref(ci), C.byref(h)))
         self.h = h.value
@@ -371,6 +381,9 @@ class Renderer:
         self.illum = None
         self.rr = None
         self.time = 0.0
+        self.cull_mask = RG_DRAW_FRAME_RAY_CULL_WORLD_0_BIT  # RgDrawFrameInfo::rayCullMaskWorld
+        self.styles = None
+        self.ignore_fpv = None
 
     def _link(self) -> None:
         chain = [self.sky, self.path] + [x for x in (self.illum, self.rr) if x is not None]
@@ -385,11 +398,23 @@ class Renderer:
     def set_lightstyles(self, values) -> None:
         """Chains RgDrawFrameIlluminationParams (defaults of DrawFrameInfo.h)
         with these light style values into every following rgDrawFrame."""
-        if values is None:
+        self._illumination(values, self.ignore_fpv)
+
+    def set_ignore_fpv_shadows(self, light_unique_id) -> None:
+        """RgDrawFrameIlluminationParams::lightUniqueIdIgnoreFirstPersonViewerShadows
+        (None: no such light)."""
+        self._illumination(self.styles_values if self.styles is not None else None, light_unique_id)
+
+    def _illumination(self, values, ignore_fpv) -> None:
+        self.styles_values = values
+        self.ignore_fpv = ignore_fpv
+        if values is None and ignore_fpv is None:
             self.illum = None
+            self.styles = None
             self._link()
             return
-        self.styles = (C.c_float * max(1, len(values)))(*values)
+        self.styles = (C.c_float * max(1, len(values or [])))(*(values or []))
+        self.fpv_id = C.c_uint64(ignore_fpv or 0)
         il = RgDrawFrameIlluminationParams()
         il.sType = RG_STRUCTURE_TYPE_ILLUMINATION
         il.maxBounceShadows = 2
@@ -399,8 +424,9 @@ class Renderer:
         il.indirectDiffuseSensitivityToChange = 0.2
         il.specularSensitivityToChange = 0.5
         il.polygonalLightSpotlightFactor = 2.0
-        il.lightstyleValuesCount = len(values)
-        il.pLightstyleValues = C.cast(self.styles, _P)
+        il.lightstyleValuesCount = len(values or [])
+        il.pLightstyleValues = C.cast(self.styles, _P) if values else None
+        il.lightUniqueIdIgnoreFirstPersonViewerShadows = C.cast(C.pointer(self.fpv_id), _P) if ignore_fpv is not None else None
         self.illum = il
         self._link()
 
@@ -428,7 +454,7 @@ class Renderer:
         ffi.check("rgStartFrame", l.rgStartFrame(self.h, C.byref(sfi)))
         uploader.upload(self.h, geometry)
         self.time += dt
-        di = draw_info(cam, self.time, C.cast(C.pointer(self.sky), _P))
+        di = draw_info(cam, self.time, C.cast(C.pointer(self.sky), _P), self.cull_mask)
         ffi.check("rgDrawFrame", l.rgDrawFrame(self.h, C.byref(di)))
         return self.stats()
 

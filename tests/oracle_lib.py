@@ -64,6 +64,7 @@ class Oracle:
             "or_pt_create": (_P, [C.c_int, C.c_int]),
             "or_pt_destroy": (None, [_P]),
             "or_pt_frame": (None, [_P, _P, _P, _P, _P, C.c_int]),
+            "or_flat_normals": (None, [_P, _U32, _P, _U32, _P]),
             "or_denoise_temporal": (None, [_P, _P]),
             "or_denoise_antifirefly": (None, [_P, _P]),
             "or_denoise_variance": (None, [_P, _P]),
@@ -281,16 +282,37 @@ class OracleTexture:
 class OrPtSceneC(C.Structure):
     _fields_ = [("tris", _P), ("normals", _P), ("triGeom", _P), ("triCount", _U32), ("geoms", _P), ("geomIds", _P),
                 ("geomCount", _U32), ("dirLight", _P), ("spheres", _P), ("sphereCount", _U32),
-                ("bvhOpaque", _P), ("opaqueNone", _U32), ("textures", _P), ("textureCount", _U32), ("texcoords", _P)]
+                ("textures", _P), ("textureCount", _U32), ("texcoords", _P)]
 
 # RgMeshPrimitiveFlags of the refracting primitives that are not alpha tested
 # (VertexCollectorFilterType.cpp:215-229): water 128, glass 64,
 # glass-if-smooth 2048, acid 8192; alpha tested 1.
 _REFRACT_FLAGS, _ALPHA_TESTED = 128 | 64 | 2048 | 8192, 1
+FIRST_PERSON, FIRST_PERSON_VIEWER, DONT_GENERATE_NORMALS, SKY_VISIBILITY = 4, 8, 256, 32768
 
 
 def refracts(flags: int) -> bool:
     return (flags & _ALPHA_TESTED) == 0 and (flags & _REFRACT_FLAGS) != 0
+
+
+def frame_class(flags: int) -> int:
+    """The position of a primitive's triangles in the library's frame order:
+    world 0, world 2 (sky visibility), refracting of world 0, refracting of
+    world 2, first person, first-person viewer; upload order within each."""
+    if flags & FIRST_PERSON:
+        return 4
+    if flags & FIRST_PERSON_VIEWER:
+        return 5
+    w2 = (flags & SKY_VISIBILITY) != 0
+    return (3 if w2 else 2) if refracts(flags) else (1 if w2 else 0)
+
+
+def flat_normals(o: "Oracle", tris: np.ndarray) -> np.ndarray:
+    """or_flat_normals over a non-indexed triangle list ((n, 9) floats)."""
+    pos = np.ascontiguousarray(tris, np.float32).reshape(-1, 3)
+    nrm = np.zeros_like(pos)
+    o.l.or_flat_normals(pos.ctypes.data, len(pos), None, 0, nrm.ctypes.data)
+    return nrm.reshape(-1, 9)
 
 
 class OraclePathTracer:
@@ -317,11 +339,18 @@ class OraclePathTracer:
         tex_id = {nm: i + 1 for i, nm in enumerate(names)}
         self.h = o.l.or_pt_create(width, height)
         objs = list(enumerate(scene.objects))
-        objs = [ob for ob in objs if not refracts(flags.get(ob[0], 0))] + [ob for ob in objs if refracts(flags.get(ob[0], 0))]
+        objs = sorted(objs, key=lambda ob: frame_class(flags.get(ob[0], 0)))  # stable: upload order within a class
         order = np.concatenate([np.arange(first, first + count) for _, (_, first, count) in objs]) if objs else np.zeros(0, np.int64)
         n = len(order)
         self.tris = np.ascontiguousarray(scene.tris[order], np.float32)
-        self.normals = np.ascontiguousarray(scene.normals[order], np.float32)
+        normals = []
+        for oid, (_, first, count) in objs:
+            # flat normals unless the primitive opts out (GeomInfoManager.cpp:173-176)
+            if flags.get(oid, 0) & DONT_GENERATE_NORMALS:
+                normals.append(scene.normals[first:first + count])
+            else:
+                normals.append(flat_normals(o, scene.tris[first:first + count]))
+        self.normals = np.ascontiguousarray(np.concatenate(normals) if normals else np.zeros((0, 9)), np.float32)
         self.bvh = OracleBvh(o, self.tris, leaf)
         self.tri_geom = np.zeros(n, np.uint32)
         self.texcoords = np.zeros((max(1, n), 6), np.float32)
@@ -343,13 +372,9 @@ class OraclePathTracer:
             if oid in texcoords:
                 self.texcoords[pos:pos + count] = np.asarray(texcoords[oid], np.float32).reshape(count, 6)
             self.ids[g] = oid << 32
-            opaque += 0 if refracts(gr.flags) else count
             pos += count
-        # the frame's split: a subtree per side when both have >= 2 triangles
-        self.bvh_opaque = OracleBvh(o, self.tris[:opaque], leaf) if opaque >= 2 and n - opaque >= 2 else None
         self.c = OrPtSceneC(self.tris.ctypes.data, self.normals.ctypes.data, self.tri_geom.ctypes.data, n,
                             C.cast(self.geoms, _P), self.ids.ctypes.data, len(scene.objects), None, None, 0,
-                            self.bvh_opaque.h if self.bvh_opaque else None, 1 if (opaque == 0 and n > 0) else 0,
                             C.cast(self.tex_ptrs, _P) if names else None, len(names), self.texcoords.ctypes.data)
         self.scene_lights(scene)
 

@@ -29,8 +29,12 @@ COLORS = {Q: (0.6, 0.6, 0.6), BACK: (0.2, 0.5, 0.8), FAR: (0.9, 0.3, 0.1)}
 
 
 def _quad(center, a, b, normal, color):
+    """Two triangles wound counter-clockwise around `normal`, so that the
+    flat normals the library generates by default equal it."""
     c, a, b = (np.asarray(v, np.float64) for v in (center, a, b))
     p = [c - a - b, c + a - b, c + a + b, c - a + b]
+    if np.dot(np.cross(a, b), np.asarray(normal, np.float64)) < 0.0:
+        p = [p[0], p[3], p[2], p[1]]
     tris = np.array([np.concatenate([p[0], p[1], p[2]]), np.concatenate([p[0], p[2], p[3]])], np.float32)
     nrm = np.tile(np.asarray(normal, np.float32), (2, 3))
     col = np.full(2, scenes.pack_rgba8(*color), np.uint32)

@@ -1,0 +1,286 @@
+"""Flat-normal generation and ray cull masks through the C API, against
+closed forms.
+
+Flat normals (CmVertexPreprocess.comp, VertexPreprocessPartial.inl:43-120):
+every primitive without RG_MESH_PRIMITIVE_DONT_GENERATE_NORMALS
+(GeomInfoManager.cpp:173-176) shades with normalize(cross(p1 - p0, p2 - p0))
+written to the vertices of each triangle in index order, so a vertex shared
+by two triangles keeps the later one's normal.
+
+Cull masks (VertexCollectorFilterType.cpp:200-257, ASManager.cpp:837-963,
+RaygenCommon.h:103-177, VulkanDevice.cpp:344-393):
+- first-person-viewer geometry is invisible to primary rays, casts shadows on
+  the world unless its light is lightUniqueIdIgnoreFirstPersonViewerShadows,
+  and shows in reflections;
+- first-person geometry is visible to primary rays, casts no shadow on the
+  world and is missing from reflections off world surfaces;
+- world 2 (RG_MESH_PRIMITIVE_SKY_VISIBILITY) and world 0 are traced only when
+  rayCullMaskWorld names them; with allowGeometryWithSkyFlag, world-2 hits
+  read as the sky and world 2 casts no shadow.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from raytracedgl1_amd import ffi, rtgl, scenes
+from raytracedgl1_amd.ffi import PLANE
+
+pytestmark = pytest.mark.gpu
+
+W, H = 96, 64
+FPV, FP = rtgl.RG_MESH_PRIMITIVE_FIRST_PERSON_VIEWER, rtgl.RG_MESH_PRIMITIVE_FIRST_PERSON
+SKYVIS, DONT_GEN = 32768, 256
+W0, W1_, W2, SKY = (rtgl.RG_DRAW_FRAME_RAY_CULL_WORLD_0_BIT, rtgl.RG_DRAW_FRAME_RAY_CULL_WORLD_1_BIT,
+                    rtgl.RG_DRAW_FRAME_RAY_CULL_WORLD_2_BIT, rtgl.RG_DRAW_FRAME_RAY_CULL_SKY_BIT)
+SUN = {"color": (1.0, 1.0, 1.0), "intensity": 5.0, "direction": (0.0, -1.0, 0.0), "angularDiameterDegrees": 0.5}
+
+
+class Prims:
+    """Uploads a list of primitives (vertices, indices or None, flags,
+    colour) and a sun, the way rtgl.SceneUploader uploads a scene."""
+
+    def __init__(self, prims, sun=SUN):
+        self.prims, self.sun = prims, sun
+
+    def upload(self, inst, geometry=True, **_):
+        l = rtgl.api()
+        mesh = rtgl.RgMeshInfo()
+        mesh.transform = rtgl.identity_transform()
+        for i, (v, idx, flags, color) in enumerate(self.prims if geometry else []):
+            prim = rtgl.RgMeshPrimitiveInfo()
+            mesh.uniqueObjectID = i
+            prim.pVertices, prim.vertexCount = v.ctypes.data, len(v)
+            prim.pIndices = idx.ctypes.data if idx is not None else None
+            prim.indexCount = len(idx) if idx is not None else 0
+            prim.flags, prim.color = flags, color | 0xFF000000
+            ffi.check("rgUploadMeshPrimitive", l.rgUploadMeshPrimitive(inst, C.byref(mesh), C.byref(prim)))
+        if self.sun:
+            d = rtgl.RgDirectionalLightUploadInfo()
+            d.uniqueID = 1
+            d.color = l.rgUtilPackColorFloat4D(*self.sun["color"], 1.0)
+            d.intensity = self.sun["intensity"]
+            d.direction.data[:] = self.sun["direction"]
+            d.angularDiameterDegrees = self.sun["angularDiameterDegrees"]
+            ffi.check("rgUploadDirectionalLight", l.rgUploadDirectionalLight(inst, C.byref(d)))
+
+
+def verts(positions, normals=None):
+    p = np.asarray(positions, np.float32).reshape(-1, 3)
+    v = np.zeros(len(p), rtgl.VERTEX_DTYPE)
+    v["position"] = p
+    v["normal"] = np.asarray(normals, np.float32).reshape(-1, 3) if normals is not None else (1.0, 0.0, 0.0)
+    return v
+
+
+def quad(center, half, y_normal_up=True):
+    """A horizontal square (two triangles, counter-clockwise seen from +y)."""
+    cx, cy, cz = center
+    a = [(cx - half, cy, cz - half), (cx - half, cy, cz + half), (cx + half, cy, cz + half), (cx + half, cy, cz - half)]
+    tri = [a[0], a[1], a[2], a[0], a[2], a[3]]
+    return verts(tri, [(0.0, 1.0, 0.0)] * 6)
+
+
+def render(prims, cam, cull=W0, allow_sky=False, ignore_fpv=None, rr=None, frames=1):
+    r = rtgl.Renderer(W, H, allow_sky=allow_sky)
+    try:
+        r.cull_mask = cull
+        if ignore_fpv is not None:
+            r.set_ignore_fpv_shadows(ignore_fpv)
+        if rr is not None:
+            r.set_reflect_refract(rr)
+        for _ in range(frames):
+            r.frame(Prims(prims), cam)
+        fb = r.framebuffers()
+        return {k: fb.download(PLANE[k]) for k in ("IS_SKY", "VISIBILITY_BUFFER", "NORMAL", "UNFILTERED_DIRECT", "SURFACE_POSITION")}
+    finally:
+        r.close()
+
+
+def decode_normal(packed):
+    """decodeNormal (Utils.h:126-140): 16-bit polar and azimuth angles."""
+    packed = np.asarray(packed, np.uint64)
+    phi = (packed >> 16).astype(np.float64) / 65535.0 * math.pi
+    theta = (packed & 0xFFFF).astype(np.float64) / 32768.0 * math.pi
+    return np.stack([np.sin(phi) * np.cos(theta), np.sin(phi) * np.sin(theta), np.cos(phi)], -1)
+
+
+CAM = scenes.Camera(np.array([0.0, 6.0, 6.0]), np.array([0.0, 0.0, 0.0]), fovy=math.radians(60.0))
+
+
+def _hits(out):
+    vis = out["VISIBILITY_BUFFER"].reshape(-1, 4)
+    hit = vis[:, 0] == 1
+    return vis, hit
+
+
+# ---------------------------------------------------------------- normals
+
+def test_flat_normals_of_an_icosphere():
+    print("normals: icosphere level 2, default flags and DONT_GENERATE_NORMALS", flush=True)
+    t = scenes.icosphere(2).reshape(-1, 3, 3) * 2.0 + np.array([0.0, 1.0, 0.0])
+    smooth = (t - np.array([0.0, 1.0, 0.0])) / 2.0  # unit-sphere positions: smooth normals
+    face = np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0])
+    face /= np.linalg.norm(face, axis=1, keepdims=True)
+    for flags in (0, DONT_GEN):
+        out = render([(verts(t, smooth), None, flags, 0x808080)], CAM)
+        vis, hit = _hits(out)
+        assert 0.1 < hit.mean() < 0.9
+        local = vis[hit, 1] >> 12
+        n = decode_normal(out["NORMAL"].reshape(-1)[hit])
+        if flags == 0:
+            want = face[local]
+        else:
+            u, v = vis[hit, 2].view(np.float32).astype(np.float64), vis[hit, 3].view(np.float32).astype(np.float64)
+            s = smooth[local]
+            want = s[:, 0] * (1 - u - v)[:, None] + s[:, 1] * u[:, None] + s[:, 2] * v[:, None]
+            want /= np.linalg.norm(want, axis=1, keepdims=True)
+        err = np.abs(n - want).max()
+        print(f"  flags {flags}: {hit.sum()} pixels, max |n - expected| {err:.2e}", flush=True)
+        # the 16+16-bit normal codec: 4.8e-5 rad in the polar angle, 9.6e-5 in the azimuth
+        assert err < 2e-4
+
+
+def test_indexed_shared_vertices_keep_the_last_triangles_normal():
+    """A two-triangle roof over 4 indexed vertices; v0, v1 are shared.  In
+    index order [0,1,2, 1,0,3] the shared vertices take triangle 1's normal
+    n1, so triangle 0 shades with normalize((1 - v) n1 + v n0) at
+    barycentrics (u, v) and triangle 1 with n1; order [1,0,3, 0,1,2] swaps
+    the roles.  The supplied normals (1, 0, 0) are overwritten."""
+    print("normals: indexed roof, last triangle wins", flush=True)
+    p = np.array([(-2, 1, 0), (2, 1, 0), (0, 0, -2), (0, 0, 2)], np.float32)
+    n0 = np.cross(p[1] - p[0], p[2] - p[0]).astype(np.float64)
+    n1 = np.cross(p[0] - p[1], p[3] - p[1]).astype(np.float64)
+    n0, n1 = n0 / np.linalg.norm(n0), n1 / np.linalg.norm(n1)
+    for idx, first, second in (([0, 1, 2, 1, 0, 3], n0, n1), ([1, 0, 3, 0, 1, 2], n1, n0)):
+        # `first` is the normal of the triangle indexed first, `second` of the last
+        out = render([(verts(p), np.array(idx, np.uint32), 0, 0x808080)], CAM)
+        vis, hit = _hits(out)
+        local = vis[hit, 1] >> 12
+        v = vis[hit, 3].view(np.float32).astype(np.float64)
+        n = decode_normal(out["NORMAL"].reshape(-1)[hit])
+        want = np.where((local == 0)[:, None], (1.0 - v)[:, None] * second + v[:, None] * first, second)
+        want /= np.linalg.norm(want, axis=1, keepdims=True)
+        assert (local == 0).sum() > 50 and (local == 1).sum() > 50
+        err = np.abs(n - want).max()
+        print(f"  indices {idx}: max |n - expected| {err:.2e}", flush=True)
+        assert err < 2e-4
+    # DONT_GENERATE_NORMALS: the supplied (1, 0, 0) everywhere
+    out = render([(verts(p), np.array([0, 1, 2, 1, 0, 3], np.uint32), DONT_GEN, 0x808080)], CAM)
+    vis, hit = _hits(out)
+    assert np.abs(decode_normal(out["NORMAL"].reshape(-1)[hit]) - np.array([1.0, 0.0, 0.0])).max() < 2e-4
+
+
+# ------------------------------------------------------------------ masks
+
+FLOOR = quad((0.0, 0.0, 0.0), 10.0)
+BLOCKER = quad((0.0, 2.0, 0.0), 1.0)
+
+
+def _under_around(out):
+    """Mean direct light on the floor under the blocker's shadow (|x|, |z| <
+    0.8) and around it (1.3 < |x| < 3, |z| < 3)."""
+    pos = out["SURFACE_POSITION"].reshape(-1, 4)
+    d = out["UNFILTERED_DIRECT"].reshape(-1, 4)[:, :3].sum(-1)
+    sky = out["IS_SKY"].reshape(-1) != 0
+    floor = ~sky & (np.abs(pos[:, 1]) < 1e-3)
+    under = floor & (np.abs(pos[:, 0]) < 0.8) & (np.abs(pos[:, 2]) < 0.8)
+    around = floor & (np.abs(pos[:, 0]) > 1.3) & (np.abs(pos[:, 0]) < 3) & (np.abs(pos[:, 2]) < 3)
+    assert under.sum() > 20 and around.sum() > 20
+    return d[under].mean(), d[around].mean()
+
+
+def _blocker_pixels(out, geom):
+    vis, hit = _hits(out)
+    return hit & ((vis[:, 1] & 0xFFF) == geom)
+
+
+def test_first_person_viewer_geometry():
+    print("masks: first-person viewer blocker", flush=True)
+    alone = render([(FLOOR, None, 0, 0xC0C0C0)], CAM)
+    out = render([(FLOOR, None, 0, 0xC0C0C0), (BLOCKER, None, FPV, 0x404040)], CAM)
+    # invisible to primary rays: the visibility buffer is the floor's alone
+    assert np.array_equal(out["VISIBILITY_BUFFER"], alone["VISIBILITY_BUFFER"])
+    under, around = _under_around(out)
+    print(f"  shadow under / around {under / around:.4f}", flush=True)
+    assert around > 0 and under / around < 0.05  # casts a shadow on the world
+    # ... except for the light named in lightUniqueIdIgnoreFirstPersonViewerShadows:
+    # then the frame is the floor's alone
+    lit = render([(FLOOR, None, 0, 0xC0C0C0), (BLOCKER, None, FPV, 0x404040)], CAM, ignore_fpv=1)
+    assert np.array_equal(lit["UNFILTERED_DIRECT"], alone["UNFILTERED_DIRECT"])
+
+
+def test_first_person_geometry():
+    print("masks: first-person blocker", flush=True)
+    out = render([(FLOOR, None, 0, 0xC0C0C0), (BLOCKER, None, FP, 0x404040)], CAM)
+    assert _blocker_pixels(out, 1).sum() > 20  # visible to primary rays
+    under, around = _under_around(out)
+    # no shadow on world surfaces (the reservoir combine halves some pixels at random)
+    assert under / around > 0.85
+    world = render([(FLOOR, None, 0, 0xC0C0C0), (BLOCKER, None, 0, 0x404040)], CAM)
+    assert np.array_equal(_blocker_pixels(world, 1), _blocker_pixels(out, 1))
+    under, around = _under_around(world)
+    assert under / around < 0.05  # the same blocker as world geometry does cast one
+
+
+def test_reflections_show_the_viewer_and_hide_first_person():
+    """A mirror floor: reflection rays off world surfaces see first-person-
+    viewer geometry and not first-person geometry (RaygenCommon.h:108-135)."""
+    print("masks: mirror floor with FP / FPV blockers", flush=True)
+    mirror = rtgl.RG_MESH_PRIMITIVE_MIRROR
+    cam = scenes.Camera(np.array([0.0, 3.0, 6.0]), np.array([0.0, 0.0, 0.0]), fovy=math.radians(60.0))
+    for flags, seen in ((FPV, True), (FP, False), (0, True)):
+        out = render([(FLOOR, None, mirror, 0xC0C0C0), (quad((0.0, 1.0, -4.0), 1.0), None, flags, 0x404040)], cam,
+                     rr=rtgl.reflect_refract_params(maxReflectRefractDepth=1))
+        vis, hit = _hits(out)
+        pos = out["SURFACE_POSITION"].reshape(-1, 4)
+        # blocker surfaces in the G-buffer lie at y = 1 (only the mirror shows them when FPV)
+        shown = (hit & (np.abs(pos[:, 1] - 1.0) < 1e-3)).sum()
+        print(f"  flags {flags}: blocker surfaces in the G-buffer {shown}", flush=True)
+        if flags == FPV:
+            # none of them is the primary view: the mirror is in front of all of it
+            direct = render([(FLOOR, None, 0, 0xC0C0C0), (quad((0.0, 1.0, -4.0), 1.0), None, FPV, 0x404040)], cam)
+            assert _blocker_pixels(direct, 1).sum() == 0
+        assert (shown > 10) == seen, flags
+
+
+def test_world_two_and_the_world_cull_mask():
+    print("masks: world 2 and rayCullMaskWorld", flush=True)
+    alone = render([(FLOOR, None, 0, 0xC0C0C0)], CAM)
+    scene = [(FLOOR, None, 0, 0xC0C0C0), (BLOCKER, None, SKYVIS, 0x404040)]
+    # world 2 left out of the cull mask: not in the TLAS at all
+    out = render(scene, CAM, cull=W0)
+    assert np.array_equal(out["VISIBILITY_BUFFER"], alone["VISIBILITY_BUFFER"])
+    assert np.array_equal(out["UNFILTERED_DIRECT"], alone["UNFILTERED_DIRECT"])
+    # named: ordinary geometry
+    out = render(scene, CAM, cull=W0 | W2)
+    assert _blocker_pixels(out, 1).sum() > 20
+    under, around = _under_around(out)
+    assert under / around < 0.05
+    # with allowGeometryWithSkyFlag it is sky geometry: hits read as sky and
+    # it casts no shadow
+    out = render(scene, CAM, cull=W0 | SKY, allow_sky=True)
+    sky = out["IS_SKY"].reshape(-1) != 0
+    named = render(scene, CAM, cull=W0 | W2)
+    assert np.array_equal(sky, _blocker_pixels(named, 1) | (alone["IS_SKY"].reshape(-1) != 0))
+    under, around = _under_around(out)
+    assert under / around > 0.85
+    # only world 1 traced: nothing is visible
+    out = render(scene, CAM, cull=W1_)
+    assert (out["IS_SKY"] != 0).all()
+
+
+def test_cull_mask_arguments_are_checked():
+    print("masks: RG_DRAW_FRAME_RAY_CULL_WORLD_2_BIT / SKY_BIT against allowGeometryWithSkyFlag", flush=True)
+    for allow, mask in ((True, W0 | W2), (False, W0 | SKY)):
+        r = rtgl.Renderer(W, H, allow_sky=allow)
+        try:
+            l = rtgl.api()
+            sfi = rtgl.RgStartFrameInfo()
+            ffi.check("rgStartFrame", l.rgStartFrame(r.h, C.byref(sfi)))
+            Prims([(FLOOR, None, 0, 0xC0C0C0)]).upload(r.h)
+            di = rtgl.draw_info(CAM, 0.1, C.cast(C.pointer(r.sky), C.c_void_p), mask)
+            assert l.rgDrawFrame(r.h, C.byref(di)) == 11  # RG_RESULT_WRONG_FUNCTION_ARGUMENT
+        finally:
+            r.close()

@@ -140,6 +140,42 @@ def test_frames_match_the_fp64_oracle(oracle):
     assert oc[P["DIS_GRADIENT_HISTORY"]][..., 0].max() > 0.1
 
 
+def test_instance_classes_match_the_fp64_oracle(oracle):
+    """Boxes of every instance class beside the world-0 terrain: first
+    person (every 5th from box 1), first-person viewer (from box 2), world 2
+    (from box 3, left out by the world-0 cull mask), supplied normals kept
+    (from box 4, DONT_GENERATE_NORMALS) and plain world 0.  The GPU segments
+    the frame BVH by class and starts each ray at the segments its cull mask
+    admits; the oracle tests every triangle's instance mask against the ray's
+    on one tree.  Same thresholds as the plain frames."""
+    from raytracedgl1_amd import rtgl as _r
+    s = scenes.terrain_boxes_scene()
+    kinds = (_r.RG_MESH_PRIMITIVE_FIRST_PERSON, _r.RG_MESH_PRIMITIVE_FIRST_PERSON_VIEWER, 32768, 256, 0)
+    flags = {i: kinds[(i - 1) % 5] for i in range(1, len(s.objects))}
+    print("parity: first-person / viewer / world-2 boxes vs FP64 oracle, terrain-boxes 160x90", flush=True)
+    for f, g, oc, m in _parity_frames(oracle, flags):
+        assert m["IS_SKY"] == 1.0 and m["VISIBILITY_BUFFER"] == 1.0, f
+        assert m["DEPTH_WORLD max"] < 1e-5 and m["MOTION max"] < 1e-5, f
+        for k in ("RESERVOIRS_INITIAL", "RESERVOIRS"):
+            assert m[k] >= 0.999, (f, k)
+        for k, need in (("UNFILTERED_DIRECT", 0.995), ("UNFILTERED_SPECULAR", 0.97), ("UNFILTERED_INDIR", 0.97)):
+            assert m[k] >= need, (f, k, m[k])
+            assert abs(m[k + " mean"] - 1.0) < 0.03, (f, k)
+        assert m["PRE_FINAL"] >= 0.8, (f, m["PRE_FINAL"])
+        assert abs(m["PRE_FINAL mean"] - 1.0) < 0.01, f
+        # first-person-viewer and world-2 boxes are absent from the G-buffer
+        geom = g["VISIBILITY_BUFFER"][..., 1] & 0xFFF
+        seen = {int(x) for x in np.unique(geom[g["VISIBILITY_BUFFER"][..., 0] == 1])}
+        order = sorted(range(len(s.objects)), key=lambda i: _class_of(flags.get(i, 0)))
+        hidden = {order.index(i) for i in flags if flags[i] in (kinds[1], kinds[2])}
+        assert not (seen & hidden), f
+
+
+def _class_of(fl):
+    from oracle_lib import frame_class
+    return frame_class(fl)
+
+
 def test_reflection_refraction_frames_match_the_fp64_oracle(oracle):
     """Boxes as mirrors, glass and water (every third box each; depth 3).
     Reflected and refracted rays are FP32 on the GPU and FP64 in the oracle,
