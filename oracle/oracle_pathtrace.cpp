@@ -46,7 +46,7 @@ const uint32_t SALT_RESAMPLE_INDIRECT_BASE = 132;
 const int      GX = 16, GY = 16, GZ = 16, GCELL = 128;
 const uint32_t GRID_ENTRIES = GX * GY * GZ * GCELL;
 
-const uint32_t LT_NONE = 0, LT_DIRECTIONAL = 1, LT_SPHERE = 2;
+const uint32_t LT_NONE = 0, LT_DIRECTIONAL = 1, LT_SPHERE = 2, LT_TRIANGLE = 3, LT_SPOT = 4;
 const uint32_t LIGHT_NONE = 32767; // LIGHT_INDEX_NONE
 
 struct D3
@@ -181,6 +181,8 @@ struct Light
     uint32_t type = LT_NONE;
     D3       color{ 0, 0, 0 };
     double   d0[ 4 ] = { 0, 0, 0, 0 };
+    double   d1[ 4 ] = { 0, 0, 0, 0 };
+    double   d2[ 4 ] = { 0, 0, 0, 0 };
 };
 
 D3 UnpackColor( uint32_t c )
@@ -210,6 +212,42 @@ Light EncodeSphere( const RgSphericalLightUploadInfo& i )
     l.color           = UnpackColor( i.color ) * ( double( i.intensity ) / area );
     l.d0[ 0 ] = i.position.data[ 0 ], l.d0[ 1 ] = i.position.data[ 1 ], l.d0[ 2 ] = i.position.data[ 2 ];
     l.d0[ 3 ] = r;
+    return l;
+}
+// EncodeAsSpotLight (LightManager.cpp:194-234): cone angles clamped to
+// [0, 89] degrees, the inner one at least 1 degree inside the outer one.
+Light EncodeSpot( const RgSpotLightUploadInfo& i )
+{
+    Light        l;
+    const double r    = std::max( 0.005, double( i.radius ) );
+    const double area = PI * r * r;
+    const D3     d    = mk( i.direction.data[ 0 ], i.direction.data[ 1 ], i.direction.data[ 2 ] );
+    const double len  = length( d );
+    const double deg  = PI / 180.0;
+    auto         cl   = [ & ]( double a ) { return std::min( std::max( a, 0.0 ), 89.0 * deg ); };
+    const double inner = std::min( double( i.angleInner ), double( i.angleOuter ) - 1.0 * deg );
+    l.type             = LT_SPOT;
+    l.color            = UnpackColor( i.color ) * ( double( i.intensity ) / area );
+    l.d0[ 0 ] = i.position.data[ 0 ], l.d0[ 1 ] = i.position.data[ 1 ], l.d0[ 2 ] = i.position.data[ 2 ], l.d0[ 3 ] = r;
+    const D3 dn = len > 0 ? d / len : mk( 0, 0, 0 );
+    l.d1[ 0 ] = dn.x, l.d1[ 1 ] = dn.y, l.d1[ 2 ] = dn.z;
+    l.d2[ 0 ] = std::cos( cl( inner ) ), l.d2[ 1 ] = std::cos( cl( double( i.angleOuter ) ) );
+    return l;
+}
+// EncodeAsTriangleLight (LightManager.cpp:150-192) with the unnormalised
+// normal cross(p1 - p0, p2 - p0) (Utils::GetUnnormalizedNormal, Utils.cpp:346-359)
+Light EncodeTriangle( const RgPolygonalLightUploadInfo& i )
+{
+    Light    l;
+    const D3 p0 = mk( i.positions[ 0 ].data[ 0 ], i.positions[ 0 ].data[ 1 ], i.positions[ 0 ].data[ 2 ] );
+    const D3 p1 = mk( i.positions[ 1 ].data[ 0 ], i.positions[ 1 ].data[ 1 ], i.positions[ 1 ].data[ 2 ] );
+    const D3 p2 = mk( i.positions[ 2 ].data[ 0 ], i.positions[ 2 ].data[ 1 ], i.positions[ 2 ].data[ 2 ] );
+    const D3 n  = cross( p1 - p0, p2 - p0 );
+    l.type      = LT_TRIANGLE;
+    l.color     = UnpackColor( i.color ) * ( double( i.intensity ) / ( 0.5 * length( n ) ) );
+    l.d0[ 0 ] = p0.x, l.d0[ 1 ] = p0.y, l.d0[ 2 ] = p0.z, l.d0[ 3 ] = n.x;
+    l.d1[ 0 ] = p1.x, l.d1[ 1 ] = p1.y, l.d1[ 2 ] = p1.z, l.d1[ 3 ] = n.y;
+    l.d2[ 0 ] = p2.x, l.d2[ 1 ] = p2.y, l.d2[ 2 ] = p2.z, l.d2[ 3 ] = n.z;
     return l;
 }
 
@@ -332,17 +370,34 @@ double solidAngleSphere( double radius, double dist )
     return safeSolidAngle( 2 * PI * ( 1.0 - c ) );
 }
 double colorWeight( D3 c ) { return clampd( lum( c ) * 0.1 + 0.9, 1.0, 10.0 ); }
+D3     P0( const Light& l ) { return mk( l.d0[ 0 ], l.d0[ 1 ], l.d0[ 2 ] ); }
+D3     P1( const Light& l ) { return mk( l.d1[ 0 ], l.d1[ 1 ], l.d1[ 2 ] ); }
+D3     P2( const Light& l ) { return mk( l.d2[ 0 ], l.d2[ 1 ], l.d2[ 2 ] ); }
+D3     TriNormal( const Light& l ) { return normalize( mk( l.d0[ 3 ], l.d1[ 3 ], l.d2[ 3 ] ) ); }
+// isSphereInFront (Light.h:111-114)
+double inFront( D3 planeNormal, D3 planePos, D3 center, double radius ) { return dot( planeNormal, center - planePos ) > -radius ? 1.0 : 0.0; }
+// getLightWeight (Light.h:162-204, :286-296)
 double lightWeight( const Light& l, D3 cellCenter, double cellRadius )
 {
     switch( l.type )
     {
         case LT_DIRECTIONAL: return colorWeight( l.color );
         case LT_SPHERE:
-            return colorWeight( l.color ) * solidAngleSphere( l.d0[ 3 ], std::max( length( mk( l.d0[ 0 ], l.d0[ 1 ], l.d0[ 2 ] ) - cellCenter ), cellRadius ) );
+            return colorWeight( l.color ) * solidAngleSphere( l.d0[ 3 ], std::max( length( P0( l ) - cellCenter ), cellRadius ) );
+        case LT_TRIANGLE:
+        {
+            const D3     c = ( P0( l ) + P1( l ) + P2( l ) ) / 3.0;
+            const double r = ( length( P0( l ) - c ) + length( P1( l ) - c ) + length( P2( l ) - c ) ) / 3.0;
+            return colorWeight( l.color ) * solidAngleSphere( r, std::max( length( c - cellCenter ), cellRadius ) ) *
+                   inFront( TriNormal( l ), c, cellCenter, cellRadius );
+        }
+        case LT_SPOT:
+            return colorWeight( l.color ) * solidAngleSphere( l.d0[ 3 ], std::max( length( P0( l ) - cellCenter ), cellRadius ) ) *
+                   inFront( P1( l ), P0( l ), cellCenter, cellRadius );
         default: return 0.0;
     }
 }
-LSample sampleLight( const Light& l, D3 surfPos, double pu, double pv )
+LSample sampleLight( const Light& l, D3 surfPos, double pu, double pv, double polyFactor )
 {
     LSample r;
     if( l.type == LT_DIRECTIONAL )
@@ -369,6 +424,40 @@ LSample sampleLight( const Light& l, D3 surfPos, double pu, double pv )
         r.position  = c + ln * l.d0[ 3 ];
         r.color     = l.color;
         r.dw        = solidAngleSphere( l.d0[ 3 ], len );
+    }
+    else if( l.type == LT_TRIANGLE )
+    {
+        // sampleTriangle (Random.h:58-69): uniform over the triangle's area
+        const double u1 = pu * 0.99, u2 = pv * 0.99;
+        const double beta = 1.0 - std::sqrt( u1 ), gamma = ( 1.0 - beta ) * u2, alpha = 1.0 - beta - gamma;
+        r.position          = P0( l ) * alpha + P1( l ) * beta + P2( l ) * gamma;
+        const D3     n      = TriNormal( l );
+        const double area   = 0.5 * length( mk( l.d0[ 3 ], l.d1[ 3 ], l.d2[ 3 ] ) );
+        D3           toSurf = surfPos - r.position;
+        const double dist   = length( toSurf );
+        toSurf              = toSurf / dist;
+        // getPolySpotFactor (Light.h:101-105); calcSolidAngleForArea (:153-158)
+        r.color = l.color * std::pow( std::max( dot( n, toSurf ), 0.0 ), polyFactor );
+        r.dw    = safeSolidAngle( area * std::fabs( dot( n, toSurf ) ) / ( dist * dist ) );
+    }
+    else if( l.type == LT_SPOT )
+    {
+        // sampleSpotLight (Light.h:282-297): a point of the disk facing the
+        // spot direction; the cone factor smoothstep(cosOuter, cosInner, cosA)^2
+        // (getSpotFactor, Light.h:107-110) of the direction to the centre
+        const D3 dir = P1( l );
+        double   dx, dy;
+        sampleDisk( l.d0[ 3 ], pu, pv, &dx, &dy );
+        D3 b0, b1;
+        revisedONB( dir, &b0, &b1 );
+        r.position          = P0( l ) + b0 * dx + b1 * dy;
+        const D3     d      = P0( l ) - surfPos;
+        const double len    = length( d );
+        const double cosA   = std::max( -dot( dir, d ) / len, 0.0 );
+        const double t      = clampd( ( cosA - l.d2[ 1 ] ) / ( l.d2[ 0 ] - l.d2[ 1 ] ), 0.0, 1.0 );
+        const double smooth = t * t * ( 3.0 - 2.0 * t );
+        r.color             = l.color * ( smooth * smooth );
+        r.dw                = solidAngleSphere( l.d0[ 3 ], len );
     }
     return r;
 }
@@ -1293,7 +1382,7 @@ void InitialCandidates( const Ctx& c, const Surface& s, uint32_t seed, uint32_t&
             r.M = 1;
             if( !valid( r ) ) continue;
             const double  oneOverSource = r.wsum * safeRcp( r.tp );
-            const LSample ls            = sampleLight( LightAt( c, r.sel ), s.position, pu, pv );
+            const LSample ls            = sampleLight( LightAt( c, r.sel ), s.position, pu, pv, c.p.polyLightSpotlightFactor );
             update( reg, r.sel, targetPdf( s, ls, loose ), oneOverSource, rnd16( seed, salt++ ) );
         }
     }
@@ -1303,7 +1392,7 @@ void InitialCandidates( const Ctx& c, const Surface& s, uint32_t seed, uint32_t&
         {
             const double   rnd = rnd16( seed, salt++ );
             const uint32_t xi  = 1u + std::min( uint32_t( float( rnd ) * float( c.lightCount ) ), c.lightCount - 1u );
-            const LSample  ls  = sampleLight( LightAt( c, xi ), s.position, pu, pv );
+            const LSample  ls  = sampleLight( LightAt( c, xi ), s.position, pu, pv, c.p.polyLightSpotlightFactor );
             update( reg, xi, targetPdf( s, ls, loose ), double( float( c.lightCount ) ), rnd16( seed, salt++ ) );
         }
     }
@@ -1311,7 +1400,7 @@ void InitialCandidates( const Ctx& c, const Surface& s, uint32_t seed, uint32_t&
     dir = Res{};
     if( c.hasDir )
     {
-        const LSample ls = sampleLight( c.lights[ 0 ], s.position, pu, pv );
+        const LSample ls = sampleLight( c.lights[ 0 ], s.position, pu, pv, c.p.polyLightSpotlightFactor );
         update( dir, 0, targetPdf( s, ls, loose ), 1.0, rnd16( seed, salt++ ) );
     }
     normalizeRes( dir, 1 );
@@ -1349,7 +1438,7 @@ void IndirectDirect( const Ctx& c, const Surface& hs, uint32_t seed, int bounce,
     InitialCandidates( c, hs, seed, salt, pu, pv, true, reg, dir );
     const Res r = CombineInitial( reg, dir, seed, salt );
     if( !valid( r ) ) return;
-    const LSample ls = sampleLight( LightAt( c, r.sel ), hs.position, pu, pv );
+    const LSample ls = sampleLight( LightAt( c, r.sel ), hs.position, pu, pv, c.p.polyLightSpotlightFactor );
     D3            d, sp;
     shade( hs, ls, selWeight( r ), true, &d, &sp );
     if( lum( d + sp ) <= 0.0 ) return;
@@ -1413,6 +1502,22 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
         const auto& s = sc->spheres[ i ];
         if( s.intensity <= 0.00001f || ( s.color & 0x00FFFFFFu ) == 0 ) continue;
         c.lights.push_back( EncodeSphere( s ) );
+        ids.push_back( s.uniqueID );
+    }
+    for( uint32_t i = 0; i < sc->spotCount; i++ )
+    {
+        const auto& s = sc->spots[ i ];
+        if( s.intensity <= 0.00001f || ( s.color & 0x00FFFFFFu ) == 0 || s.radius < 0.0f || s.angleOuter <= 0.0f ) continue;
+        c.lights.push_back( EncodeSpot( s ) );
+        ids.push_back( s.uniqueID );
+    }
+    for( uint32_t i = 0; i < sc->polyCount; i++ )
+    {
+        const auto& s = sc->polys[ i ];
+        if( s.intensity <= 0.00001f || ( s.color & 0x00FFFFFFu ) == 0 ) continue;
+        const Light l = EncodeTriangle( s );
+        if( !( l.d0[ 3 ] * l.d0[ 3 ] + l.d1[ 3 ] * l.d1[ 3 ] + l.d2[ 3 ] * l.d2[ 3 ] > 0.0 ) ) continue;
+        c.lights.push_back( l );
         ids.push_back( s.uniqueID );
     }
     c.lightCount = uint32_t( c.lights.size() ) - 1;
@@ -1599,7 +1704,7 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
             pendSalt[ i ] = salt;
             if( valid( dir ) )
             {
-                rays[ i ]  = ShadowRay( s, sampleLight( c.lights[ 0 ], s.position, pu, pv ).position );
+                rays[ i ]  = ShadowRay( s, sampleLight( c.lights[ 0 ], s.position, pu, pv, c.p.polyLightSpotlightFactor ).position );
                 masks[ i ] = ShadowMask( p, s.custom, 0u );
             }
         }
@@ -1672,7 +1777,7 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
                         const uint32_t cur = t.sel < c.prevToCur.size() ? c.prevToCur[ t.sel ] : 0xFFFFFFFFu;
                         if( cur != 0xFFFFFFFFu && cur != LIGHT_NONE )
                         {
-                            tpCur = targetPdf( s, sampleLight( LightAt( c, cur ), s.position, pu, pv ), false );
+                            tpCur = targetPdf( s, sampleLight( LightAt( c, cur ), s.position, pu, pv, c.p.polyLightSpotlightFactor ), false );
                             t.sel = cur;
                         }
                     }
@@ -1692,7 +1797,7 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
                 if( !( comb.wsum <= 0.0 || comb.sel == LIGHT_NONE ) )
                 {
                     res              = comb;
-                    const LSample ls = sampleLight( LightAt( c, res.sel ), s.position, pu, pv );
+                    const LSample ls = sampleLight( LightAt( c, res.sel ), s.position, pu, pv, c.p.polyLightSpotlightFactor );
                     shade( s, ls, selWeight( res ), false, &diff, &spec );
                     if( lum( diff + spec ) <= 0.0 )
                     {
@@ -2115,7 +2220,7 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
                         r.sel = cur;
                         double pu, pv;
                         LightPointRnd( oldSeed, &pu, &pv );
-                        const LSample ls = sampleLight( LightAt( c, cur ), ps.position, pu, pv );
+                        const LSample ls = sampleLight( LightAt( c, cur ), ps.position, pu, pv, c.p.polyLightSpotlightFactor );
                         D3            d, sp;
                         shade( ps, ls, selWeight( r ), false, &d, &sp );
                         if( lum( d + sp ) > 0.0 )

@@ -40,6 +40,11 @@ typedef struct OrPtScene
     const RgDirectionalLightUploadInfo* dirLight; /* may be null */
     const RgSphericalLightUploadInfo*   spheres;
     uint32_t                            sphereCount;
+    /* regular lights in this order: spheres, spots, polygons */
+    const RgSpotLightUploadInfo*        spots;
+    uint32_t                            spotCount;
+    const RgPolygonalLightUploadInfo*   polys;
+    uint32_t                            polyCount;
     /* Albedo textures and per-triangle texture coordinates (6 floats). */
     const struct OrTexture* const*      textures;
     uint32_t                            textureCount;

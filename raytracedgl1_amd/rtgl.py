@@ -70,6 +70,17 @@ class RgSphericalLightUploadInfo(C.Structure):
                 ("intensity", _F), ("position", RgFloat3D), ("radius", _F)]
 
 
+class RgSpotLightUploadInfo(C.Structure):
+    _fields_ = [("uniqueID", C.c_uint64), ("isExportable", _U32), ("extra", RgLightExtraInfo), ("color", _U32),
+                ("intensity", _F), ("position", RgFloat3D), ("direction", RgFloat3D), ("radius", _F), ("angleOuter", _F),
+                ("angleInner", _F)]
+
+
+class RgPolygonalLightUploadInfo(C.Structure):
+    _fields_ = [("uniqueID", C.c_uint64), ("isExportable", _U32), ("extra", RgLightExtraInfo), ("color", _U32),
+                ("intensity", _F), ("positions", RgFloat3D * 3)]
+
+
 class RgStartFrameInfo(C.Structure):
     _fields_ = [("pMapName", C.c_char_p), ("ignoreExternalGeometry", _U32)]
 
@@ -215,6 +226,8 @@ _API = {
     "rgUploadMeshPrimitive": (C.c_int, [_P, C.POINTER(RgMeshInfo), C.POINTER(RgMeshPrimitiveInfo)]),
     "rgUploadDirectionalLight": (C.c_int, [_P, C.POINTER(RgDirectionalLightUploadInfo)]),
     "rgUploadSphericalLight": (C.c_int, [_P, C.POINTER(RgSphericalLightUploadInfo)]),
+    "rgUploadSpotLight": (C.c_int, [_P, C.POINTER(RgSpotLightUploadInfo)]),
+    "rgUploadPolygonalLight": (C.c_int, [_P, C.POINTER(RgPolygonalLightUploadInfo)]),
     "rgStartFrame": (C.c_int, [_P, C.POINTER(RgStartFrameInfo)]),
     "rgDrawFrame": (C.c_int, [_P, C.POINTER(RgDrawFrameInfo)]),
     "rgUtilGetResultDescription": (C.c_char_p, [C.c_int]),
@@ -352,6 +365,37 @@ class SceneUploader:
             s.radius = sp["radius"]
             _style(s, sp)
             ffi.check(prefix + "UploadSphericalLight", sph_fn(inst, C.byref(s)))
+        for i, sp in enumerate(lights.get("spots", [])):
+            ffi.check("rgUploadSpotLight", l.rgUploadSpotLight(inst, C.byref(spot_info(sp, 200 + i))))
+        for i, pl in enumerate(lights.get("polys", [])):
+            ffi.check("rgUploadPolygonalLight", l.rgUploadPolygonalLight(inst, C.byref(poly_info(pl, 300 + i))))
+
+
+def spot_info(sp: dict, uid: int) -> RgSpotLightUploadInfo:
+    """A spot light dict (color, intensity, position, direction, radius,
+    angleOuter, angleInner in radians) -> RgSpotLightUploadInfo."""
+    s = RgSpotLightUploadInfo()
+    s.uniqueID = uid
+    s.color = api().rgUtilPackColorFloat4D(*sp["color"], 1.0)
+    s.intensity = sp["intensity"]
+    s.position.data[:] = sp["position"]
+    s.direction.data[:] = sp["direction"]
+    s.radius, s.angleOuter, s.angleInner = sp["radius"], sp["angleOuter"], sp["angleInner"]
+    _style(s, sp)
+    return s
+
+
+def poly_info(pl: dict, uid: int) -> RgPolygonalLightUploadInfo:
+    """A triangle light dict (color, intensity, positions: 3 points) ->
+    RgPolygonalLightUploadInfo."""
+    s = RgPolygonalLightUploadInfo()
+    s.uniqueID = uid
+    s.color = api().rgUtilPackColorFloat4D(*pl["color"], 1.0)
+    s.intensity = pl["intensity"]
+    for k in range(3):
+        s.positions[k].data[:] = pl["positions"][k]
+    _style(s, pl)
+    return s
 
 
 class Renderer:
@@ -384,6 +428,7 @@ class Renderer:
         self.cull_mask = RG_DRAW_FRAME_RAY_CULL_WORLD_0_BIT  # RgDrawFrameInfo::rayCullMaskWorld
         self.styles = None
         self.ignore_fpv = None
+        self.illum_over = {}
 
     def _link(self) -> None:
         chain = [self.sky, self.path] + [x for x in (self.illum, self.rr) if x is not None]
@@ -405,10 +450,16 @@ class Renderer:
         (None: no such light)."""
         self._illumination(self.styles_values if self.styles is not None else None, light_unique_id)
 
+    def set_illumination(self, **fields) -> None:
+        """Overrides RgDrawFrameIlluminationParams fields (cellWorldSize,
+        polygonalLightSpotlightFactor, ...) in every following rgDrawFrame."""
+        self.illum_over.update(fields)
+        self._illumination(self.styles_values if self.styles is not None else None, self.ignore_fpv)
+
     def _illumination(self, values, ignore_fpv) -> None:
         self.styles_values = values
         self.ignore_fpv = ignore_fpv
-        if values is None and ignore_fpv is None:
+        if values is None and ignore_fpv is None and not self.illum_over:
             self.illum = None
             self.styles = None
             self._link()
@@ -427,6 +478,8 @@ class Renderer:
         il.lightstyleValuesCount = len(values or [])
         il.pLightstyleValues = C.cast(self.styles, _P) if values else None
         il.lightUniqueIdIgnoreFirstPersonViewerShadows = C.cast(C.pointer(self.fpv_id), _P) if ignore_fpv is not None else None
+        for k, v in self.illum_over.items():
+            setattr(il, k, v)
         self.illum = il
         self._link()
 

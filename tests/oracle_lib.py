@@ -282,6 +282,7 @@ class OracleTexture:
 class OrPtSceneC(C.Structure):
     _fields_ = [("tris", _P), ("normals", _P), ("triGeom", _P), ("triCount", _U32), ("geoms", _P), ("geomIds", _P),
                 ("geomCount", _U32), ("dirLight", _P), ("spheres", _P), ("sphereCount", _U32),
+                ("spots", _P), ("spotCount", _U32), ("polys", _P), ("polyCount", _U32),
                 ("textures", _P), ("textureCount", _U32), ("texcoords", _P)]
 
 # RgMeshPrimitiveFlags of the refracting primitives that are not alpha tested
@@ -374,7 +375,7 @@ class OraclePathTracer:
             self.ids[g] = oid << 32
             pos += count
         self.c = OrPtSceneC(self.tris.ctypes.data, self.normals.ctypes.data, self.tri_geom.ctypes.data, n,
-                            C.cast(self.geoms, _P), self.ids.ctypes.data, len(scene.objects), None, None, 0,
+                            C.cast(self.geoms, _P), self.ids.ctypes.data, len(scene.objects), None, None, 0, None, 0, None, 0,
                             C.cast(self.tex_ptrs, _P) if names else None, len(names), self.texcoords.ctypes.data)
         self.scene_lights(scene)
 
@@ -402,9 +403,14 @@ class OraclePathTracer:
             s.intensity = sp["intensity"]
             s.position.data[:] = sp["position"]
             s.radius = sp["radius"]
+        spots, polys = lights.get("spots", []), lights.get("polys", [])
+        self.spots = (rtgl.RgSpotLightUploadInfo * max(1, len(spots)))(*[rtgl.spot_info(sp, 200 + i) for i, sp in enumerate(spots)])
+        self.polys = (rtgl.RgPolygonalLightUploadInfo * max(1, len(polys)))(*[rtgl.poly_info(pl, 300 + i) for i, pl in enumerate(polys)])
         self.c.dirLight = C.cast(C.pointer(self.sun), _P) if self.sun else None
         self.c.spheres = C.cast(self.spheres, _P)
         self.c.sphereCount = len(sph)
+        self.c.spots, self.c.spotCount = C.cast(self.spots, _P), len(spots)
+        self.c.polys, self.c.polyCount = C.cast(self.polys, _P), len(polys)
 
     def frame(self, params, fb: "OracleFramebuffers", threads: int = 8):
         self.o.l.or_pt_frame(self.h, C.byref(self.c), self.bvh.h, C.byref(params), C.byref(fb.c), threads)

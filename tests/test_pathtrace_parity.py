@@ -48,14 +48,37 @@ def _rel(g, e, scale):
     return np.abs(g - e) / (np.abs(e) + scale)
 
 
-def _parity_frames(oracle, flags=None, rr=None, tex=None):
+def _scene_with_all_lights():
+    """The sphere-lit scene plus three spot and three triangle lights."""
+    s = _scene_with_spheres()
+    r = np.random.default_rng(5)
+    lights = dict(s.lights)
+    lights["spots"] = [{"color": tuple(float(c) for c in r.uniform(0.6, 1.0, 3)), "intensity": float(r.uniform(100, 400)),
+                        "position": (float(r.uniform(-15, 15)), float(r.uniform(5, 9)), float(r.uniform(-15, 15))),
+                        "direction": (float(r.uniform(-0.3, 0.3)), -1.0, float(r.uniform(-0.3, 0.3))), "radius": float(r.uniform(0.1, 0.5)),
+                        "angleOuter": float(np.radians(r.uniform(30, 60))), "angleInner": float(np.radians(r.uniform(5, 25)))}
+                       for _ in range(3)]
+    polys = []
+    for _ in range(3):
+        c = np.array([r.uniform(-15, 15), r.uniform(5, 9), r.uniform(-15, 15)])
+        a, b = r.uniform(-1.5, 1.5, 3), r.uniform(-1.5, 1.5, 3)
+        pts = [c, c + a, c + b]
+        if np.cross(pts[1] - pts[0], pts[2] - pts[0])[1] > 0:  # face the terrain below
+            pts = [pts[0], pts[2], pts[1]]
+        polys.append({"color": tuple(float(x) for x in r.uniform(0.6, 1.0, 3)), "intensity": float(r.uniform(50, 300)),
+                      "positions": [tuple(float(x) for x in q) for q in pts]})
+    lights["polys"] = polys
+    return scenes.Scene(s.tris, s.normals, s.colors, s.objects, lights)
+
+
+def _parity_frames(oracle, flags=None, rr=None, tex=None, scene_fn=None):
     """Renders FRAMES frames through the C API and the oracle side by side;
     yields (frame, gpu planes, oracle planes, metrics) after each.  tex:
     (object -> texture name, object -> texcoords, name -> (pixels, filter,
     address u, address v))."""
     textures, texcoords, pixels = tex if tex else ({}, {}, {})
-    s = _scene_with_spheres()
-    no_sun = scenes.Scene(s.tris, s.normals, s.colors, s.objects, {"spheres": s.lights["spheres"]})
+    s = (scene_fn or _scene_with_spheres)()
+    no_sun = scenes.Scene(s.tris, s.normals, s.colors, s.objects, {k: v for k, v in s.lights.items() if k != "sun"})
     scenes_per_frame = [s, s, s, no_sun]
     cam0 = scenes.terrain_boxes_camera()
     cams = [scenes.Camera(cam0.eye + np.array([0.05 * f, 0, 0.03 * f]), cam0.target + np.array([0.05 * f, 0, 0.03 * f]))
@@ -138,6 +161,22 @@ def test_frames_match_the_fp64_oracle(oracle):
         assert abs(m["PRE_FINAL mean"] - 1.0) < 0.01, f
     # the sun's removal in the last frame shows in the gradients
     assert oc[P["DIS_GRADIENT_HISTORY"]][..., 0].max() > 0.1
+
+
+def test_spot_and_triangle_lights_match_the_fp64_oracle(oracle):
+    """The plain frames with three spot and three triangle lights beside the
+    sun and the eight spheres (Light.h:225-297 on both sides).  Same
+    thresholds as the plain frames."""
+    print("parity: sun + sphere + spot + triangle lights vs FP64 oracle, terrain-boxes 160x90", flush=True)
+    for f, g, oc, m in _parity_frames(oracle, scene_fn=_scene_with_all_lights):
+        assert m["IS_SKY"] == 1.0 and m["VISIBILITY_BUFFER"] == 1.0, f
+        for k in ("RESERVOIRS_INITIAL", "RESERVOIRS"):
+            assert m[k] >= 0.999, (f, k)
+        for k, need in (("UNFILTERED_DIRECT", 0.995), ("UNFILTERED_SPECULAR", 0.97), ("UNFILTERED_INDIR", 0.97)):
+            assert m[k] >= need, (f, k, m[k])
+            assert abs(m[k + " mean"] - 1.0) < 0.03, (f, k)
+        assert m["PRE_FINAL"] >= 0.8, (f, m["PRE_FINAL"])
+        assert abs(m["PRE_FINAL mean"] - 1.0) < 0.01, f
 
 
 def test_instance_classes_match_the_fp64_oracle(oracle):
