@@ -191,9 +191,9 @@ def run_frames(a, scene, precision: int, flags=None, planes=("PRE_FINAL",)):
         r.path.precisionFlags = precision
         frame = [0]
 
-        def step(geometry=False):
+        def step(geometry=False, stats=True):
             geometry = geometry or a.dynamic_upload
-            st = r.frame(up, scenes.orbit_camera(frame[0] % ORBIT_FRAMES), geometry=geometry)
+            st = r.frame(up, scenes.orbit_camera(frame[0] % ORBIT_FRAMES), geometry=geometry, stats=stats)
             frame[0] += 1
             return st
 
@@ -203,10 +203,15 @@ def run_frames(a, scene, precision: int, flags=None, planes=("PRE_FINAL",)):
             print(f"warmup {i + 1}/{max(1, a.warmup)} (precision flags {precision})", file=sys.stderr, flush=True)
         ffi.call("rgExtSynchronize")
         t0 = time.perf_counter()
-        stats = [step() for _ in range(a.steps)]
+        for _ in range(a.steps):
+            step(stats=False)  # two frames in flight, as an application runs them
         ffi.call("rgExtSynchronize")
         wall = time.perf_counter() - t0
         print(f"timed {a.steps} frames in {wall:.3f} s (precision flags {precision})", file=sys.stderr, flush=True)
+        # per-stage GPU times and traversal counts of the next a.steps orbit
+        # frames, each read once its frame has finished (outside the timed
+        # region)
+        stats = [step() for _ in range(a.steps)]
         assert all(st.bvhRebuilt == 0 for st in stats), "static geometry was rebuilt inside the timed region"
         fb = r.framebuffers()
         if planes == ("PRE_FINAL",):

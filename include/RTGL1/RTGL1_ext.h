@@ -29,6 +29,8 @@ extern "C" {
  * (its staging is VMA-mapped memory, Source/MemoryAllocator.cpp). */
 RGAPI RgResult RGCONV rgExtDeviceAlloc( size_t bytes, void** pOut );
 RGAPI RgResult RGCONV rgExtDeviceFree( void* d_ptr );
+/* The blocking copies and the memset first wait for all work on the device
+ * (an instance keeps up to two frames in flight on its own stream). */
 RGAPI RgResult RGCONV rgExtCopyHostToDevice( void* d_dst, const void* src, size_t bytes );
 RGAPI RgResult RGCONV rgExtCopyDeviceToHost( void* dst, const void* d_src, size_t bytes );
 RGAPI RgResult RGCONV rgExtMemsetDevice( void* d_dst, int value, size_t bytes );
@@ -469,7 +471,10 @@ typedef struct RgExtDrawFramePathParams
 #define RG_EXT_PRECISION_FP16_NODES   2u
 
 /* Per-frame statistics of the last rgDrawFrame; GPU times from HIP events
- * recorded on the instance stream around each stage. */
+ * recorded on the instance stream around each stage.  rgDrawFrame returns
+ * once its frame is enqueued (at most two frames in flight, as the
+ * reference's MAX_FRAMES_IN_FLIGHT, Source/Common.h:38), so
+ * rgExtGetFrameStats waits for the last frame to finish. */
 typedef struct RgExtFrameStats
 {
     uint32_t frameId;
@@ -496,7 +501,8 @@ typedef struct RgExtFrameStats
 
 RGAPI RgResult RGCONV rgExtGetFrameStats( RgInstance instance, RgExtFrameStats* pOut );
 /* The instance's framebuffer set (valid after the first rgDrawFrame); owned
- * by the instance, never pass it to rgExtFramebuffersDestroy. */
+ * by the instance, never pass it to rgExtFramebuffersDestroy.  Waits for the
+ * frames in flight, so the planes hold the last frame's results. */
 RGAPI RgResult RGCONV rgExtGetFramebuffers( RgInstance instance, RgExtFramebuffers* pOut );
 /* The frame constants used by the last rgDrawFrame. */
 RGAPI RgResult RGCONV rgExtGetFrameParams( RgInstance instance, RgExtFrameParams* pOut );

@@ -79,6 +79,9 @@ RgResult rgExtCopyHostToDevice( void* d_dst, const void* src, size_t bytes )
         RG_REQUIRE( ( d_dst && src ) || bytes == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null pointer" );
         if( bytes )
         {
+            // after the work of every stream (an instance keeps frames in
+            // flight on its own non-blocking stream)
+            RG_HIP_CHECK( hipDeviceSynchronize() );
             RG_HIP_CHECK( hipMemcpy( d_dst, src, bytes, hipMemcpyHostToDevice ) );
         }
     } );
@@ -90,6 +93,9 @@ RgResult rgExtCopyDeviceToHost( void* dst, const void* d_src, size_t bytes )
         RG_REQUIRE( ( dst && d_src ) || bytes == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null pointer" );
         if( bytes )
         {
+            // after the work of every stream (an instance keeps frames in
+            // flight on its own non-blocking stream)
+            RG_HIP_CHECK( hipDeviceSynchronize() );
             RG_HIP_CHECK( hipMemcpy( dst, d_src, bytes, hipMemcpyDeviceToHost ) );
         }
     } );
@@ -119,6 +125,9 @@ RgResult rgExtMemsetDevice( void* d_dst, int value, size_t bytes )
         RG_REQUIRE( d_dst || bytes == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null pointer" );
         if( bytes )
         {
+            // after the work of every stream (an instance keeps frames in
+            // flight on its own non-blocking stream)
+            RG_HIP_CHECK( hipDeviceSynchronize() );
             RG_HIP_CHECK( hipMemset( d_dst, value, bytes ) );
         }
     } );

@@ -52,6 +52,15 @@ struct Instance
     bool                                        frameStarted = false;
     bool                                        staticGeometry = false;
     hipEvent_t                                  ev[ ST_COUNT ] = {};
+    // Two frames in flight (MAX_FRAMES_IN_FLIGHT, Source/Common.h:38):
+    // rgDrawFrame returns once its frame is enqueued; frameDone[k] marks the
+    // end of the last frame drawn in slot k = frame count mod 2, and a frame
+    // waits for the one two frames back before it is enqueued.
+    hipEvent_t                                  frameDone[ 2 ] = {};
+    uint32_t                                    framesDrawn  = 0;
+    // The GPU times, traversal counts and adapted luminance of the last
+    // frame are read once it has finished, when they are first asked for.
+    bool                                        statsPending = false;
 
     Instance( const RgInstanceCreateInfo& ci )
     {
@@ -72,6 +81,10 @@ struct Instance
         {
             RG_HIP_CHECK( hipEventCreate( &e ) );
         }
+        for( auto& e : frameDone )
+        {
+            RG_HIP_CHECK( hipEventCreateWithFlags( &e, hipEventDisableTiming ) );
+        }
         g.Init( stream );
         RG_HIP_CHECK( hipStreamSynchronize( stream ) );
     }
@@ -82,6 +95,13 @@ struct Instance
             ( void )hipStreamSynchronize( stream );
         }
         for( auto& e : ev )
+        {
+            if( e )
+            {
+                ( void )hipEventDestroy( e );
+            }
+        }
+        for( auto& e : frameDone )
         {
             if( e )
             {
@@ -102,7 +122,45 @@ struct Instance
             throw Error( RG_RESULT_FRAME_WASNT_ENDED, "rgStartFrame was called twice without rgDrawFrame" );
         }
         frameStarted = true;
+        if( framesDrawn > 0 )
+        {
+            // the last frame's uploads read the pinned staging arrays, the
+            // light list and the geometry map that this frame refills
+            RG_HIP_CHECK( hipEventSynchronize( ev[ ST_UPLOADED ] ) );
+        }
         scene.BeginFrame();
+    }
+
+    // Waits for every enqueued frame (before host-side readback of device
+    // memory, texture changes and teardown).
+    void Quiesce() { RG_HIP_CHECK( hipStreamSynchronize( stream ) ); }
+
+    // Completes the statistics of the last drawn frame (waits for it).
+    const RgExtFrameStats& Stats()
+    {
+        if( !statsPending )
+        {
+            return stats;
+        }
+        RG_HIP_CHECK( hipEventSynchronize( ev[ ST_END ] ) );
+        auto ms = [ & ]( Stage a, Stage b ) {
+            float v = 0.0f;
+            ( void )hipEventElapsedTime( &v, ev[ a ], ev[ b ] );
+            return v;
+        };
+        stats.gpuMsUpload       = ms( ST_BEGIN, ST_UPLOADED );
+        stats.gpuMsAssemble     = ms( ST_UPLOADED, ST_ASSEMBLED );
+        stats.gpuMsTraversal    = g.pt.TraversalMs();
+        stats.traversalLaunches = g.pt.traceLaunches;
+        g.pt.TraversalCounts( &stats.nodeVisits, &stats.triangleTests, &stats.traversalErrors );
+        stats.gpuMsBvh     = ms( ST_ASSEMBLED, ST_BVH );
+        stats.gpuMsTrace   = ms( ST_BVH, ST_TRACED );
+        stats.gpuMsDenoise = ms( ST_TRACED, ST_DENOISED );
+        stats.gpuMsTonemap = ms( ST_DENOISED, ST_END );
+        stats.gpuMsTotal   = ms( ST_BEGIN, ST_END );
+        RG_HIP_CHECK( hipMemcpy( &stats.avgLuminance, g.avgLum.ptr, 4, hipMemcpyDeviceToHost ) );
+        statsPending = false;
+        return stats;
     }
 
     void DrawFrame( const RgDrawFrameInfo& di )
@@ -138,8 +196,12 @@ struct Instance
                     "unknown precisionFlags bits" );
         const uint32_t storage  = ( prec & RG_EXT_PRECISION_FP16_STORAGE ) ? RG_EXT_STORAGE_FP16 : RG_EXT_STORAGE_FP32;
         const uint32_t nodeFmt  = ( prec & RG_EXT_PRECISION_FP16_NODES ) ? RG_EXT_BVH_NODE_FP16 : RG_EXT_BVH_NODE_FP32;
+        // at most two frames in flight: the frame two back has finished
+        const uint32_t slot = framesDrawn % 2u;
+        RG_HIP_CHECK( hipEventSynchronize( frameDone[ slot ] ) );
         if( !fb || fb->fb.width != w || fb->fb.height != h || fb->fb.storage != storage )
         {
+            Quiesce(); // the last frame may still use the old planes
             fb = std::make_unique< RgExtFramebuffers_T >();
             fb->fb.Create( w, h, storage );
             cam.firstFrame = true;
@@ -172,7 +234,7 @@ struct Instance
         RG_HIP_CHECK( hipEventRecord( ev[ ST_UPLOADED ], s ) );
         const bool changed = keep ? false : AssembleScene( dscene, s );
         RG_HIP_CHECK( hipEventRecord( ev[ ST_ASSEMBLED ], s ) );
-        stats.bvhRebuilt = 0;
+        uint32_t rebuilt = 0;
         if( dscene.triCount > 0 &&
             ( changed || !dscene.bvhValid || bvh.nodeFormat != nodeFmt ||
               std::memcmp( dscene.builtClassTris, dscene.classTris, sizeof( dscene.classTris ) ) != 0 ) )
@@ -184,37 +246,25 @@ struct Instance
             dscene.bvhValid      = true;
             dscene.builtTriCount = dscene.triCount;
             std::memcpy( dscene.builtClassTris, dscene.classTris, sizeof( dscene.classTris ) );
-            stats.bvhRebuilt     = 1;
+            rebuilt              = 1;
         }
         RG_HIP_CHECK( hipEventRecord( ev[ ST_BVH ], s ) );
         dscene.tex          = textures.View( s );
         const uint64_t rays = EncodeFrame( g, tc, cam.params, bvh, dscene, fb->fb, di, s, ev[ ST_TRACED ], ev[ ST_DENOISED ] );
         RG_HIP_CHECK( hipEventRecord( ev[ ST_END ], s ) );
-        // The staging arrays are reused by the next frame's uploads.
-        RG_HIP_CHECK( hipStreamSynchronize( s ) );
+        RG_HIP_CHECK( hipEventRecord( frameDone[ slot ], s ) );
+        framesDrawn++;
 
-        auto ms = [ & ]( Stage a, Stage b ) {
-            float v = 0.0f;
-            ( void )hipEventElapsedTime( &v, ev[ a ], ev[ b ] );
-            return v;
-        };
-        stats.frameId        = cam.frameId;
-        stats.renderWidth    = w;
-        stats.renderHeight   = h;
-        stats.triangleCount  = dscene.triCount;
-        stats.bvhNodeCount   = bvh.nodeCount;
-        stats.raysTraced     = rays;
-        stats.gpuMsUpload    = ms( ST_BEGIN, ST_UPLOADED );
-        stats.gpuMsAssemble  = ms( ST_UPLOADED, ST_ASSEMBLED );
-        stats.gpuMsTraversal = g.pt.TraversalMs();
-        stats.traversalLaunches = g.pt.traceLaunches;
-        g.pt.TraversalCounts( &stats.nodeVisits, &stats.triangleTests, &stats.traversalErrors );
-        stats.gpuMsBvh       = ms( ST_ASSEMBLED, ST_BVH );
-        stats.gpuMsTrace     = ms( ST_BVH, ST_TRACED );
-        stats.gpuMsDenoise   = ms( ST_TRACED, ST_DENOISED );
-        stats.gpuMsTonemap   = ms( ST_DENOISED, ST_END );
-        stats.gpuMsTotal     = ms( ST_BEGIN, ST_END );
-        RG_HIP_CHECK( hipMemcpy( &stats.avgLuminance, g.avgLum.ptr, 4, hipMemcpyDeviceToHost ) );
+        // what the host knows now; the GPU figures follow in Stats()
+        stats               = RgExtFrameStats{};
+        stats.frameId       = cam.frameId;
+        stats.renderWidth   = w;
+        stats.renderHeight  = h;
+        stats.triangleCount = dscene.triCount;
+        stats.bvhNodeCount  = bvh.nodeCount;
+        stats.raysTraced    = rays;
+        stats.bvhRebuilt    = rebuilt;
+        statsPending        = true;
         cam.frameId++;
         cam.firstFrame = false;
     }
@@ -332,6 +382,7 @@ RgResult rgProvideOriginalTexture( RgInstance instance, const RgOriginalTextureI
     return Guard( [ & ] {
         Instance& in = Get( instance );
         RG_REQUIRE( pInfo, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "Argument is null" );
+        in.Quiesce(); // frames in flight may sample the texture table
         in.textures.Provide( *pInfo, in.stream );
     } );
 }
@@ -345,6 +396,7 @@ RgResult rgMarkOriginalTextureAsDeleted( RgInstance instance, const char* pTextu
 {
     return Guard( [ & ] {
         Instance& in = Get( instance );
+        in.Quiesce();
         in.textures.MarkDeleted( pTextureName );
     } );
 }
@@ -445,7 +497,7 @@ RgResult rgExtGetFrameStats( RgInstance instance, RgExtFrameStats* pOut )
 {
     return Guard( [ & ] {
         RG_REQUIRE( pOut, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
-        *pOut = Get( instance ).stats;
+        *pOut = Get( instance ).Stats();
     } );
 }
 
@@ -455,6 +507,7 @@ RgResult rgExtGetFramebuffers( RgInstance instance, RgExtFramebuffers* pOut )
         RG_REQUIRE( pOut, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
         Instance& in = Get( instance );
         RG_REQUIRE( in.fb, RG_RESULT_WRONG_FUNCTION_CALL, "no frame was drawn yet" );
+        in.Quiesce(); // the planes are read and written by the host after this
         *pOut = in.fb.get();
     } );
 }
@@ -472,6 +525,7 @@ RgResult rgExtReadbackHistogram( RgInstance instance, uint32_t* pOut256 )
     return Guard( [ & ] {
         RG_REQUIRE( pOut256, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
         Instance& in = Get( instance );
+        in.Quiesce();
         RG_HIP_CHECK( hipMemcpy( pOut256, in.g.histCopy.ptr, 256 * 4, hipMemcpyDeviceToHost ) );
     } );
 }

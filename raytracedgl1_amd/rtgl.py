@@ -119,6 +119,24 @@ RG_SAMPLER_ADDRESS_MODE_REPEAT, RG_SAMPLER_ADDRESS_MODE_CLAMP = 0, 1
 RG_MESH_PRIMITIVE_ALPHA_TESTED = 1
 
 
+class RgOriginalCubemapInfo(C.Structure):
+    _fields_ = [("pTextureName", C.c_char_p), ("pPixelsPositiveX", _P), ("pPixelsNegativeX", _P),
+                ("pPixelsPositiveY", _P), ("pPixelsNegativeY", _P), ("pPixelsPositiveZ", _P),
+                ("pPixelsNegativeZ", _P), ("sideSize", _U32)]
+
+
+class RgDrawFrameRenderResolutionParams(C.Structure):
+    _fields_ = [("sType", C.c_int), ("pNext", _P), ("upscaleTechnique", C.c_int), ("sharpenTechnique", C.c_int),
+                ("resolutionMode", C.c_int), ("customRenderSize", RgExtent2D), ("pPixelizedRenderSize", _P),
+                ("resetUpscalerHistory", _U32)]
+
+
+RG_STRUCTURE_TYPE_RENDER_RESOLUTION = 1
+RG_RENDER_UPSCALE_TECHNIQUE_LINEAR = 0
+RG_RENDER_RESOLUTION_MODE_CUSTOM = 0
+RG_SKY_TYPE_COLOR = 0
+
+
 class RgDrawFrameTexturesParams(C.Structure):
     _fields_ = [("sType", C.c_int), ("pNext", _P), ("dynamicSamplerFilter", C.c_int), ("normalMapStrength", _F),
                 ("emissionMapBoost", _F), ("emissionMaxScreenColor", _F), ("minRoughness", _F)]
@@ -221,6 +239,7 @@ assert VERTEX_DTYPE.itemsize == 64
 _API = {
     "rgProvideOriginalTexture": (C.c_int, [_P, C.POINTER(RgOriginalTextureInfo)]),
     "rgMarkOriginalTextureAsDeleted": (C.c_int, [_P, C.c_char_p]),
+    "rgProvideOriginalCubemapTexture": (C.c_int, [_P, C.POINTER(RgOriginalCubemapInfo)]),
     "rgCreateInstance": (C.c_int, [C.POINTER(RgInstanceCreateInfo), C.POINTER(_P)]),
     "rgDestroyInstance": (C.c_int, [_P]),
     "rgUploadMeshPrimitive": (C.c_int, [_P, C.POINTER(RgMeshInfo), C.POINTER(RgMeshPrimitiveInfo)]),
@@ -500,8 +519,11 @@ class Renderer:
     def set_static_geometry(self, enable: bool) -> None:
         ffi.check("rgExtSetStaticGeometry", api().rgExtSetStaticGeometry(self.h, 1 if enable else 0))
 
-    def frame(self, uploader: SceneUploader, cam, dt: float = 1.0 / 60.0, geometry: bool = True) -> RgExtFrameStats:
-        """rgStartFrame, uploads, rgDrawFrame; geometry=False uploads lights only."""
+    def frame(self, uploader: SceneUploader, cam, dt: float = 1.0 / 60.0, geometry: bool = True,
+              stats: bool = True):
+        """rgStartFrame, uploads, rgDrawFrame; geometry=False uploads lights only.
+        Returns the frame's statistics (waiting for the frame), or None with
+        stats=False (the frame stays in flight)."""
         l = api()
         sfi = RgStartFrameInfo()
         ffi.check("rgStartFrame", l.rgStartFrame(self.h, C.byref(sfi)))
@@ -509,7 +531,7 @@ class Renderer:
         self.time += dt
         di = draw_info(cam, self.time, C.cast(C.pointer(self.sky), _P), self.cull_mask)
         ffi.check("rgDrawFrame", l.rgDrawFrame(self.h, C.byref(di)))
-        return self.stats()
+        return self.stats() if stats else None
 
     def stats(self) -> RgExtFrameStats:
         st = RgExtFrameStats()
