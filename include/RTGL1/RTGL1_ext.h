@@ -292,6 +292,10 @@ typedef enum RgExtPlane
      * :326-328, :588-589): emission seen on screen and acid fog */
     RG_EXT_PLANE_SCREEN_EMISSION,        /* R f32x4                                           */
     RG_EXT_PLANE_ACID_FOG,               /* R f32x4                                           */
+    /* the denoiser's guide: decoded normal and linear depth of each pixel,
+     * written by the temporal pass for the stencil passes (no counterpart in
+     * the reference, whose passes decode the normal image at every tap) */
+    RG_EXT_PLANE_DENOISE_GUIDE,          /* C f32x4 (normal xyz, depth)                      */
     RG_EXT_PLANE_COUNT
 } RgExtPlane;
 

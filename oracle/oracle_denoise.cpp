@@ -191,6 +191,7 @@ extern "C" int or_plane_components( int plane )
 {
     switch( plane )
     {
+        case OR_PLANE_GI_TIE:
         case RG_EXT_PLANE_NORMAL:
         case RG_EXT_PLANE_DEPTH_WORLD:
         case RG_EXT_PLANE_DEPTH_GRAD:
@@ -229,6 +230,7 @@ extern "C" int or_plane_has_prev( int plane )
 {
     switch( plane )
     {
+        case OR_PLANE_GI_TIE:
         case RG_EXT_PLANE_NORMAL:
         case RG_EXT_PLANE_METALLIC_ROUGHNESS:
         case RG_EXT_PLANE_DEPTH_WORLD:

@@ -10,6 +10,18 @@
 extern "C" {
 #endif
 
+/* An oracle-only plane after the product's planes (C f64, PREV): 1 where
+ * the pixel's ReSTIR GI reservoir depends on a decision that FP32 and FP64
+ * arithmetic can take differently, else 0.  Such a decision is the sign of
+ * the cosine at a spatial neighbour's sample point towards this pixel's
+ * surface when the surface lies in the sample's own plane (flat-shaded
+ * geometry: the pixel and the neighbour's bounce hit on one triangle's
+ * plane), which decides whether the sample counts (RtRaygenIndirect.inl:
+ * 365-403); a pixel whose temporal sample comes from such a pixel of the
+ * previous frame inherits the mark. */
+#define OR_PLANE_GI_TIE RG_EXT_PLANE_COUNT
+#define OR_PLANE_COUNT ( RG_EXT_PLANE_COUNT + 1 )
+
 /* Host planes of the framebuffer set, one double array per plane
  * (components interleaved); prev[] holds the previous-frame copy of planes
  * with a PREV flag (others may be null). */
@@ -17,8 +29,8 @@ typedef struct OrFramebuffers
 {
     int     width;
     int     height;
-    double* cur[ RG_EXT_PLANE_COUNT ];
-    double* prev[ RG_EXT_PLANE_COUNT ];
+    double* cur[ OR_PLANE_COUNT ];
+    double* prev[ OR_PLANE_COUNT ];
 } OrFramebuffers;
 
 int or_plane_components( int plane );

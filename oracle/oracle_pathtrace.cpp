@@ -2006,12 +2006,14 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
             {
                 fb.set4( RG_EXT_PLANE_INDIR_RESERVOIR_SAMPLE, x, y, 0, 0, 0, 0 );
                 fb.set4( RG_EXT_PLANE_INDIR_RESERVOIR_WEIGHT, x, y, 0, 0, 0, 0 );
+                fb.set4( OR_PLANE_GI_TIE, x, y, 0.0 );
                 continue;
             }
             s.position          = s.position + s.toViewer * 0.01;
             const uint32_t seed = seedOf( x, y, p.frameId );
             uint32_t       salt = SALT_RESAMPLE_INDIRECT_BASE;
             RI             comb = initialAsRes( x, y );
+            bool           tie  = false; // see OR_PLANE_GI_TIE
             int            lo, hi;
             RenderArea( x, W, &lo, &hi );
             const double mx = fb.get( RG_EXT_PLANE_MOTION, x, y, 0 ), my = fb.get( RG_EXT_PLANE_MOTION, x, y, 1 ), mz = fb.get( RG_EXT_PLANE_MOTION, x, y, 2 );
@@ -2039,6 +2041,7 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
                 }
                 if( use )
                 {
+                    tie = tie || fb.get( OR_PLANE_GI_TIE, qx, qy, 0, true ) != 0.0;
                     RI t;
                     t.pos  = mk( fb.get( RG_EXT_PLANE_INDIR_RESERVOIR_SAMPLE, qx, qy, 0, true ), fb.get( RG_EXT_PLANE_INDIR_RESERVOIR_SAMPLE, qx, qy, 1, true ),
                                  fb.get( RG_EXT_PLANE_INDIR_RESERVOIR_SAMPLE, qx, qy, 2, true ) );
@@ -2083,6 +2086,23 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
                         d                = d / len;
                         return std::max( 0.0, dot( rq.nrm, d ) ) * safeRcp( square( len ) );
                     };
+                    {
+                        // The cosines at the sample towards both receivers, and the
+                        // error of each when evaluated in FP32: the positions carry
+                        // one rounding each (2^-24 of their magnitude), so a receiver
+                        // near the sample's plane gets a cosine whose relative error
+                        // can exceed 1e-3 (FP32 cancellation), and one at the plane
+                        // a cosine of either sign.
+                        auto illConditioned = [ & ]( D3 recv ) {
+                            const D3     d   = recv - rq.pos;
+                            const double len = std::max( length( d ), 1e-30 );
+                            const double mag = std::max( { std::fabs( recv.x ), std::fabs( recv.y ), std::fabs( recv.z ) } ) +
+                                               std::max( { std::fabs( rq.pos.x ), std::fabs( rq.pos.y ), std::fabs( rq.pos.z ) } );
+                            const double err = 6e-8 * ( mag / len + 1.0 );
+                            return std::fabs( dot( rq.nrm, d ) / len ) < 1e3 * err;
+                        };
+                        tie = tie || illConditioned( s.position ) || illConditioned( x1q );
+                    }
                     const double ooj   = clampd( safeRcp( G( rq.pos, s.position ) ) * G( rq.pos, x1q ), 0.0, 1.0 ); // SHIPPING_HACK
                     const double tpCur = lum( rq.rad ) * ooj;
                     const double w     = tpCur * safeRcp( rq.tp ) * rq.wsum;
@@ -2097,6 +2117,7 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
                 }
                 comb.M = nobiasM;
             }
+            fb.set4( OR_PLANE_GI_TIE, x, y, tie ? 1.0 : 0.0 );
             if( std::isfinite( comb.wsum ) && comb.wsum >= 0.0 )
             {
                 const double rgb[ 3 ] = { comb.rad.x, comb.rad.y, comb.rad.z };
@@ -2272,7 +2293,12 @@ extern "C" void or_pt_frame( OrPathTracer* pt, const OrPtScene* sc, const OrBvh*
 }
 
 // VertexPreprocessPartial.inl:62-120 (GEOM_INST_FLAG_GENERATE_NORMALS with
-// normalSign +1): one pass over the triangles in index order.
+// normalSign +1): one pass over the triangles in index order.  The shader
+// computes normalize(cross(p1 - p0, p2 - p0)) in FP32; this restates it in
+// FP32 with one rounding per operation in the order written (edges, cross
+// product, the dot product left to right, a correctly rounded square root,
+// three divisions), so the vertex normals are the same floats as the
+// kernel's.
 extern "C" void or_flat_normals( const float* positions, uint32_t vertexCount, const uint32_t* indices, uint32_t indexCount, float* normals )
 {
     const uint32_t triCount = indices ? indexCount / 3 : vertexCount / 3;
@@ -2280,15 +2306,18 @@ extern "C" void or_flat_normals( const float* positions, uint32_t vertexCount, c
     {
         uint32_t v[ 3 ];
         for( int k = 0; k < 3; k++ ) v[ k ] = indices ? indices[ 3 * t + k ] : 3 * t + k;
-        const D3 p0 = mk( positions[ 3 * v[ 0 ] ], positions[ 3 * v[ 0 ] + 1 ], positions[ 3 * v[ 0 ] + 2 ] );
-        const D3 p1 = mk( positions[ 3 * v[ 1 ] ], positions[ 3 * v[ 1 ] + 1 ], positions[ 3 * v[ 1 ] + 2 ] );
-        const D3 p2 = mk( positions[ 3 * v[ 2 ] ], positions[ 3 * v[ 2 ] + 1 ], positions[ 3 * v[ 2 ] + 2 ] );
-        const D3 n  = normalize( cross( p1 - p0, p2 - p0 ) );
+        const float* p0 = positions + 3 * v[ 0 ];
+        const float* p1 = positions + 3 * v[ 1 ];
+        const float* p2 = positions + 3 * v[ 2 ];
+        const float  ax = p1[ 0 ] - p0[ 0 ], ay = p1[ 1 ] - p0[ 1 ], az = p1[ 2 ] - p0[ 2 ];
+        const float  bx = p2[ 0 ] - p0[ 0 ], by = p2[ 1 ] - p0[ 1 ], bz = p2[ 2 ] - p0[ 2 ];
+        const float  cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+        const float  len = std::sqrt( cx * cx + cy * cy + cz * cz );
         for( int k = 0; k < 3; k++ )
         {
-            normals[ 3 * v[ k ] + 0 ] = float( n.x );
-            normals[ 3 * v[ k ] + 1 ] = float( n.y );
-            normals[ 3 * v[ k ] + 2 ] = float( n.z );
+            normals[ 3 * v[ k ] + 0 ] = cx / len;
+            normals[ 3 * v[ k ] + 1 ] = cy / len;
+            normals[ 3 * v[ k ] + 2 ] = cz / len;
         }
     }
 }

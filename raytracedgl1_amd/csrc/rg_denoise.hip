@@ -8,10 +8,12 @@
 //
 // Layout: 256-thread blocks of 32 x 8 pixels; a wave covers two rows of 32
 // pixels, so each wave's loads of a 16-byte plane are two 512-byte
-// contiguous runs.  Stencil passes stage their footprint (tile + halo) in
-// LDS once per block and decode normals once per tile pixel instead of once
-// per tap.  A fetch outside the image returns zeros, like texelFetch with
-// robust image access.
+// contiguous runs.  The temporal pass decodes each pixel's normal once and
+// writes it with the depth to the guide plane (16 bytes per pixel); the
+// variance and a-trous passes read one guide value per tap instead of
+// decoding the normal (two sincospi) at every tap.  Iteration-0 passes
+// stage their footprint (tile + halo) in LDS once per block.  A fetch
+// outside the image returns zeros, like texelFetch with robust image access.
 #include "rg_denoise.h"
 #include "rg_device_math.h"
 
@@ -190,6 +192,22 @@ __device__ __forceinline__ v3 DecodeNormal( uint32_t p )
     sincospif( unitQuotient< 65535u >( p >> 16 ), &sp, &cp );
     sincospif( float( p & 0xFFFFu ) / 32768.0f, &st, &ct );
     return mk3( sp * ct, sp * st, cp );
+}
+
+// The denoise guide of a pixel: its decoded normal and linear depth, as
+// DecodeNormal(NORMAL) and DEPTH_WORLD would give them; outside the image
+// the values of a zero fetch of both (decoded normal code 0, depth 0).
+__device__ __forceinline__ void LdGuide( const FbPtrs& P, int w, int h, int x, int y, v3& n, float& depth )
+{
+    if( x < 0 || y < 0 || x >= w || y >= h )
+    {
+        n     = DecodeNormal( 0u );
+        depth = 0.0f;
+        return;
+    }
+    const float4 g = reinterpret_cast< const float4* >( P.cur[ RG_EXT_PLANE_DENOISE_GUIDE ] )[ size_t( y ) * w + x ];
+    n              = mk3( g.x, g.y, g.z );
+    depth          = g.w;
 }
 
 __device__ __forceinline__ bool TestDepth( float z, float zPrev, float zMotion )
@@ -469,6 +487,7 @@ __global__ __launch_bounds__( THREADS ) void k_temporal( FbPtrs P, Dims d, RgExt
         indirAccum = unfIndir;
         indirHist  = 1.0f;
     }
+    St4( P.cur[ RG_EXT_PLANE_DENOISE_GUIDE ], W, x, y, f4( normal, depth ) );
     StP4< F16 >( P.cur[ RG_EXT_PLANE_DIFF_TEMPORARY ], W, x, y, f4( diffAccum, 0.0f ) );
     StP2< F16 >( P.cur[ RG_EXT_PLANE_DIFF_ACCUM_MOMENTS ], W, x, y, diffMom );
     StP4< F16 >( P.cur[ RG_EXT_PLANE_ACCUM_HISTORY_LENGTH ], W, x, y, make_float4( diffHist, indirHist, specHistAccum, specHitAccum ) );
@@ -529,6 +548,22 @@ __global__ __launch_bounds__( THREADS ) void k_antifirefly( FbPtrs P, Dims d, ui
     }
 }
 
+// Writes the denoise guide from NORMAL and DEPTH_WORLD, for the stencil
+// passes when they run on their own (the temporal pass writes it in the
+// full chain).
+__global__ __launch_bounds__( THREADS ) void k_guide( FbPtrs P, Dims d )
+{
+    const int x = blockIdx.x * BX + threadIdx.x % BX;
+    const int y = blockIdx.y * BY + threadIdx.x / BX;
+    const int W = d.W, H = d.H;
+    if( x >= W || y >= H )
+    {
+        return;
+    }
+    St4( P.cur[ RG_EXT_PLANE_DENOISE_GUIDE ], W, x, y,
+         f4( DecodeNormal( LdU( P.cur[ RG_EXT_PLANE_NORMAL ], W, H, x, y ) ), Ld1( P.cur[ RG_EXT_PLANE_DEPTH_WORLD ], W, H, x, y ) ) );
+}
+
 // ------------------------------------------------------------ variance
 
 template< bool F16 >
@@ -544,8 +579,7 @@ __global__ __launch_bounds__( THREADS ) void k_variance( FbPtrs P, Dims d )
     {
         const int tx = i % TW, ty = i / TW;
         s_col[ ty ][ tx ] = xyz( LdP4< F16 >( P.cur[ RG_EXT_PLANE_DIFF_ACCUM_COLOR ], W, H, bx0 + tx, by0 + ty ) );
-        s_nrm[ ty ][ tx ] = DecodeNormal( LdU( P.cur[ RG_EXT_PLANE_NORMAL ], W, H, bx0 + tx, by0 + ty ) );
-        s_dep[ ty ][ tx ] = Ld1( P.cur[ RG_EXT_PLANE_DEPTH_WORLD ], W, H, bx0 + tx, by0 + ty );
+        LdGuide( P, W, H, bx0 + tx, by0 + ty, s_nrm[ ty ][ tx ], s_dep[ ty ][ tx ] );
     }
     __syncthreads();
     const int lx = threadIdx.x % BX + R, ly = threadIdx.x / BX + R;
@@ -786,8 +820,7 @@ __global__ __launch_bounds__( THREADS ) void k_atrous0( FbPtrs P, Dims d, Atrous
         s_diff[ ty ][ tx ] = LdP4< F16 >( P.cur[ ap.srcD ], W, H, gx, gy );
         s_spec[ ty ][ tx ] = xyz( LdP4< F16 >( P.cur[ ap.srcS ], W, H, gx, gy ) );
         s_ind[ ty ][ tx ]  = xyz( LdP4< F16 >( P.cur[ ap.srcI ], W, H, gx, gy ) );
-        s_nrm[ ty ][ tx ]  = DecodeNormal( LdU( P.cur[ RG_EXT_PLANE_NORMAL ], W, H, gx, gy ) );
-        s_dep[ ty ][ tx ]  = Ld1( P.cur[ RG_EXT_PLANE_DEPTH_WORLD ], W, H, gx, gy );
+        LdGuide( P, W, H, gx, gy, s_nrm[ ty ][ tx ], s_dep[ ty ][ tx ] );
         s_rgh[ ty ][ tx ]  = Ld2( P.cur[ RG_EXT_PLANE_METALLIC_ROUGHNESS ], W, H, gx, gy ).y;
     }
     __syncthreads();
@@ -851,7 +884,9 @@ __global__ __launch_bounds__( THREADS ) void k_atrousN( FbPtrs P, Dims d, Atrous
     const float4 cd    = LdP4< F16 >( P.cur[ ap.srcD ], W, H, x, y );
     const v3     cs    = xyz( LdP4< F16 >( P.cur[ ap.srcS ], W, H, x, y ) );
     const v3     ci    = xyz( LdP4< F16 >( P.cur[ ap.srcI ], W, H, x, y ) );
-    const float  depth = Ld1( P.cur[ RG_EXT_PLANE_DEPTH_WORLD ], W, H, x, y );
+    v3           n;
+    float        depth;
+    LdGuide( P, W, H, x, y, n, depth );
     if( depth < 0.0f || depth > MAX_RAY_LENGTH )
     {
         if( ap.dstD >= 0 )
@@ -866,13 +901,11 @@ __global__ __launch_bounds__( THREADS ) void k_atrousN( FbPtrs P, Dims d, Atrous
         }
         return;
     }
-    const v3    n     = DecodeNormal( LdU( P.cur[ RG_EXT_PLANE_NORMAL ], W, H, x, y ) );
     const float rough = Ld2( P.cur[ RG_EXT_PLANE_METALLIC_ROUGHNESS ], W, H, x, y ).y;
     const float pvar  = LdP1< F16 >( P.cur[ RG_EXT_PLANE_ATROUS_FILTERED_VARIANCE ], W, H, x, y );
     AtrousPixel< F16 >( P, d, x, y, k, 1 << k, ap, n, depth, Ld1( P.cur[ RG_EXT_PLANE_DEPTH_GRAD ], W, H, x, y ), rough, cd, cs, ci, pvar,
                  [ & ]( int, int, int qx, int qy, v3& nq, float& dq, float& rq, float4& cq, v3& sq, v3& iq ) {
-                     nq = DecodeNormal( LdU( P.cur[ RG_EXT_PLANE_NORMAL ], W, H, qx, qy ) );
-                     dq = Ld1( P.cur[ RG_EXT_PLANE_DEPTH_WORLD ], W, H, qx, qy );
+                     LdGuide( P, W, H, qx, qy, nq, dq );
                      rq = Ld2( P.cur[ RG_EXT_PLANE_METALLIC_ROUGHNESS ], W, H, qx, qy ).y;
                      cq = LdP4< F16 >( P.cur[ ap.srcD ], W, H, qx, qy );
                      sq = xyz( LdP4< F16 >( P.cur[ ap.srcS ], W, H, qx, qy ) );
@@ -935,11 +968,25 @@ void DenoiseAntiFirefly( FramebufferSet& fb, const RgExtFrameParams& p, hipStrea
     CheckLaunch( "k_antifirefly" );
 }
 
-void DenoiseVariance( FramebufferSet& fb, const RgExtFrameParams& p, hipStream_t s )
+namespace
+{
+void WriteGuide( FramebufferSet& fb, hipStream_t s )
+{
+    const Dims d = GetDims( fb );
+    k_guide<<< dim3( DivUp( d.W, BX ), DivUp( d.H, BY ) ), THREADS, 0, s >>>( fb.Ptrs(), d );
+    CheckLaunch( "k_guide" );
+}
+} // namespace
+
+void DenoiseVariance( FramebufferSet& fb, const RgExtFrameParams& p, hipStream_t s, bool writeGuide )
 {
     CheckParams( fb, p );
     const Dims d = GetDims( fb );
     const dim3 grid( DivUp( d.W, BX ), DivUp( d.H, BY ) );
+    if( writeGuide )
+    {
+        WriteGuide( fb, s );
+    }
     if( fb.Half() )
     {
         k_variance< true ><<< grid, THREADS, 0, s >>>( fb.Ptrs(), d );
@@ -951,10 +998,14 @@ void DenoiseVariance( FramebufferSet& fb, const RgExtFrameParams& p, hipStream_t
     CheckLaunch( "k_variance" );
 }
 
-void DenoiseAtrous( FramebufferSet& fb, const RgExtFrameParams& p, uint32_t k, hipStream_t s )
+void DenoiseAtrous( FramebufferSet& fb, const RgExtFrameParams& p, uint32_t k, hipStream_t s, bool writeGuide )
 {
     CheckParams( fb, p );
     RG_REQUIRE( k < p.atrousIterations, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "a-trous iteration out of range" );
+    if( writeGuide )
+    {
+        WriteGuide( fb, s );
+    }
     const Dims         d  = GetDims( fb );
     const AtrousPlanes ap = GetAtrousPlanes( k, p.atrousIterations );
     const dim3         grid( DivUp( d.W, BX ), DivUp( d.H, BY ) );
@@ -992,10 +1043,11 @@ void Denoise( FramebufferSet& fb, const RgExtFrameParams& p, hipStream_t s )
     }
     DenoiseTemporal( fb, p, s );
     DenoiseAntiFirefly( fb, p, s );
-    DenoiseVariance( fb, p, s );
+    // the temporal pass has written the guide
+    DenoiseVariance( fb, p, s, false );
     for( uint32_t k = 0; k < p.atrousIterations; k++ )
     {
-        DenoiseAtrous( fb, p, k, s );
+        DenoiseAtrous( fb, p, k, s, false );
     }
 }
 

@@ -51,6 +51,7 @@ const PlaneDesc g_planes[ RG_EXT_PLANE_COUNT ] = {
     { LAYOUT_REGULAR, 4, false },      // FINAL
     { LAYOUT_REGULAR, 4, false },      // SCREEN_EMISSION
     { LAYOUT_REGULAR, 4, false },      // ACID_FOG
+    { LAYOUT_CHECKERBOARD, 4, false }, // DENOISE_GUIDE
 };
 } // namespace
 

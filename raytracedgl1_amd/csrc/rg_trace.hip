@@ -44,6 +44,8 @@ struct TraceArgs
     uint32_t        partitions;
     uint32_t*       rayCounter;
     uint32_t*       overflowStack;
+    uint32_t        ovfStride;  // global stack entries per lane
+    uint32_t        stackTotal; // LDS + global stack entries per lane
     uint32_t*       errorFlag;
     uint64_t*       stats; // [0] node visits, [1] triangle tests (RG_EXT_TRACE_COLLECT_STATS)
     // rays of several scenes of a batched BVH (segCount > 0): rays of scene s
@@ -174,12 +176,29 @@ template< bool H, int STACK_LDS, bool ALPHA >
 __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
 {
     __shared__ uint32_t s_stack[ STACK_LDS * TR_THREADS ];
+    // The frame's entry roots, staged in LDS once per block: read from the
+    // kernel arguments inside the loop they would hold 64 scalar registers,
+    // which the compiler spills to vector-register lanes and reloads.
+    __shared__ uint32_t s_rootMask[ 8 ], s_rootRef[ 8 ];
+    __shared__ float    s_rootBox[ 8 ][ 6 ];
 
     const int      tid       = threadIdx.x;
+    if( tid < 8 )
+    {
+        s_rootMask[ tid ] = a.rootMask[ tid ];
+        s_rootRef[ tid ]  = a.rootRef[ tid ];
+#pragma unroll
+        for( int j = 0; j < 6; j++ )
+        {
+            s_rootBox[ tid ][ j ] = a.rootBox[ tid ][ j ];
+        }
+    }
+    __syncthreads();
     const int      lane      = tid & 63;
     const uint64_t ltMask    = lane == 0 ? 0ull : ( ~0ull >> ( 64 - lane ) );
     const uint32_t gthread   = blockIdx.x * TR_THREADS + tid;
-    uint32_t*      ovf       = a.overflowStack + size_t( gthread ) * STACK_OVF_STRIDE;
+    uint32_t*      ovf       = a.overflowStack + size_t( gthread ) * a.ovfStride;
+    const int      stackTotal = int( a.stackTotal );
     const bool     anyHit    = ( a.flags & RG_EXT_TRACE_ANY_HIT ) != 0;
     const bool     cullFront = ( a.flags & RG_EXT_TRACE_CULL_FRONT_FACE ) != 0;
 
@@ -202,7 +221,7 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
         {
             s_stack[ sp * TR_THREADS + tid ] = v;
         }
-        else if( sp < STACK_TOTAL )
+        else if( sp < stackTotal )
         {
             ovf[ sp - STACK_LDS ] = v;
         }
@@ -280,10 +299,10 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
 #pragma unroll
                         for( int k = 0; k < 8; k++ )
                         {
-                            if( uint32_t( k ) < a.rootCount && ( a.rootMask[ k ] & rm ) != 0u )
+                            if( uint32_t( k ) < a.rootCount && ( s_rootMask[ k ] & rm ) != 0u )
                             {
                                 admit++;
-                                only = a.rootRef[ k ];
+                                only = s_rootRef[ k ];
                             }
                         }
                         node = only;
@@ -297,15 +316,15 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
                             for( int k = 7; k >= 0; k-- )
                             {
                                 float tn;
-                                if( uint32_t( k ) < a.rootCount && ( a.rootMask[ k ] & rm ) != 0u &&
-                                    SlabTest( ray, a.rootBox[ k ][ 0 ], a.rootBox[ k ][ 1 ], a.rootBox[ k ][ 2 ], a.rootBox[ k ][ 3 ],
-                                              a.rootBox[ k ][ 4 ], a.rootBox[ k ][ 5 ], tmin, bestT, &tn ) )
+                                if( uint32_t( k ) < a.rootCount && ( s_rootMask[ k ] & rm ) != 0u &&
+                                    SlabTest( ray, s_rootBox[ k ][ 0 ], s_rootBox[ k ][ 1 ], s_rootBox[ k ][ 2 ], s_rootBox[ k ][ 3 ],
+                                              s_rootBox[ k ][ 4 ], s_rootBox[ k ][ 5 ], tmin, bestT, &tn ) )
                                 {
                                     if( node != RG_EXT_BVH_CHILD_EMPTY )
                                     {
                                         push( node );
                                     }
-                                    node = a.rootRef[ k ];
+                                    node = s_rootRef[ k ];
                                 }
                             }
                         }
@@ -549,6 +568,7 @@ void ReadTraceStats( const TraceContext& s, uint64_t* nodeVisits, uint64_t* triT
     uint64_t v[ 2 ] = { 0, 0 };
     if( s.stats.ptr )
     {
+        RG_HIP_CHECK( hipDeviceSynchronize() ); // the trace may run on any stream
         RG_HIP_CHECK( hipMemcpy( v, s.stats.ptr, sizeof( v ), hipMemcpyDeviceToHost ) );
     }
     *nodeVisits = v[ 0 ];
@@ -560,6 +580,7 @@ uint32_t ReadTraceErrors( const TraceContext& s )
     uint32_t e = 0;
     if( s.counter.ptr )
     {
+        RG_HIP_CHECK( hipDeviceSynchronize() );
         RG_HIP_CHECK( hipMemcpy( &e, s.counter.as< uint32_t >() + RAY_PARTITIONS * COUNTER_STRIDE, sizeof( e ), hipMemcpyDeviceToHost ) );
     }
     return e;
@@ -660,6 +681,8 @@ void TraceRaysImpl( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_ray
     a.errorFlag     = s.counter.as< uint32_t >() + RAY_PARTITIONS * COUNTER_STRIDE;
     a.partitions    = blocks < RAY_PARTITIONS ? blocks : RAY_PARTITIONS;
     a.overflowStack = s.overflow.as< uint32_t >();
+    a.ovfStride     = STACK_OVF_STRIDE;
+    a.stackTotal    = STACK_TOTAL;
     a.stats         = nullptr;
     a.segRays       = d_segRays;
     a.segRoots      = d_segRays ? bvh.sceneRoots8.as< uint32_t >() : nullptr;

@@ -408,6 +408,7 @@ PLANES = [
     ("FINAL", 1, 4, np.float32, False),
     ("SCREEN_EMISSION", 1, 4, np.float32, False),
     ("ACID_FOG", 1, 4, np.float32, False),
+    ("DENOISE_GUIDE", 0, 4, np.float32, False),
 ]
 PLANE = {name: i for i, (name, *_rest) in enumerate(PLANES)}
 

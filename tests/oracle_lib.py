@@ -11,6 +11,8 @@ from pathlib import Path
 
 import numpy as np
 
+from raytracedgl1_amd.ffi import PLANES as _PLANES
+
 REPO = Path(__file__).resolve().parents[1]
 ORACLE_DIR = REPO / "oracle"
 ORACLE_LIB = ORACLE_DIR / "_build" / "liboracle.so"
@@ -193,7 +195,9 @@ class OracleBvh:
         return hits
 
 
-PLANE_COUNT = 44
+# the product's planes, then OR_PLANE_GI_TIE (oracle/oracle_denoise.h)
+OR_PLANE_GI_TIE = len(_PLANES)
+PLANE_COUNT = len(_PLANES) + 1
 
 
 class OrFramebuffersC(C.Structure):

@@ -62,9 +62,12 @@ def test_direct_reuse_keeps_the_mean_gpu():
                 fb = r.framebuffers()
                 ok = fb.download(P["IS_SKY"])[..., 0] == 0
                 d = fb.download(P["UNFILTERED_DIRECT"])[..., 0][ok]
-                print(f"  frame {f}: mean / expected {d.mean() / expect:.6f}", flush=True)
+                empty = (fb.download(P["RESERVOIRS"])[..., 0][ok] & 0xFFFF) == tr.LIGHT_INDEX_NONE
+                print(f"  frame {f}: mean / expected {d.mean() / expect:.6f}, {empty.sum()} empty reservoirs", flush=True)
                 assert abs(d.mean() / expect - 1.0) < 0.2 * jitter / expect
-                assert np.abs(d - expect).max() <= 4.0 * jitter
+                assert empty.sum() <= tr.MAX_EMPTY_FRACTION * ok.sum()
+                assert np.abs(d[~empty] - expect).max() <= 4.0 * jitter
+                assert (d[empty] == 0.0).all()
         finally:
             r.close()
 

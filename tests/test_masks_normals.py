@@ -162,7 +162,8 @@ def test_indexed_shared_vertices_keep_the_last_triangles_normal():
         n = decode_normal(out["NORMAL"].reshape(-1)[hit])
         want = np.where((local == 0)[:, None], (1.0 - v)[:, None] * second + v[:, None] * first, second)
         want /= np.linalg.norm(want, axis=1, keepdims=True)
-        assert (local == 0).sum() > 50 and (local == 1).sum() > 50
+        # the back slope (triangle 0) is seen at a grazing angle: ~40 pixels
+        assert (local == 0).sum() > 20 and (local == 1).sum() > 20
         err = np.abs(n - want).max()
         print(f"  indices {idx}: max |n - expected| {err:.2e}", flush=True)
         assert err < 2e-4
@@ -226,23 +227,30 @@ def test_first_person_geometry():
 
 def test_reflections_show_the_viewer_and_hide_first_person():
     """A mirror floor: reflection rays off world surfaces see first-person-
-    viewer geometry and not first-person geometry (RaygenCommon.h:108-135)."""
+    viewer geometry and not first-person geometry (RaygenCommon.h:108-135);
+    primary rays see first-person geometry and not first-person-viewer
+    geometry (:103-106)."""
     print("masks: mirror floor with FP / FPV blockers", flush=True)
     mirror = rtgl.RG_MESH_PRIMITIVE_MIRROR
     cam = scenes.Camera(np.array([0.0, 3.0, 6.0]), np.array([0.0, 0.0, 0.0]), fovy=math.radians(60.0))
-    for flags, seen in ((FPV, True), (FP, False), (0, True)):
-        out = render([(FLOOR, None, mirror, 0xC0C0C0), (quad((0.0, 1.0, -4.0), 1.0), None, flags, 0x404040)], cam,
-                     rr=rtgl.reflect_refract_params(maxReflectRefractDepth=1))
+    blocker = quad((0.0, 1.0, -4.0), 1.0)
+
+    def at_blocker(out):
+        # blocker surfaces in the G-buffer lie at y = 1 (the chain writes
+        # the reflected surface into the G-buffer)
         vis, hit = _hits(out)
-        pos = out["SURFACE_POSITION"].reshape(-1, 4)
-        # blocker surfaces in the G-buffer lie at y = 1 (only the mirror shows them when FPV)
-        shown = (hit & (np.abs(pos[:, 1] - 1.0) < 1e-3)).sum()
-        print(f"  flags {flags}: blocker surfaces in the G-buffer {shown}", flush=True)
-        if flags == FPV:
-            # none of them is the primary view: the mirror is in front of all of it
-            direct = render([(FLOOR, None, 0, 0xC0C0C0), (quad((0.0, 1.0, -4.0), 1.0), None, FPV, 0x404040)], cam)
-            assert _blocker_pixels(direct, 1).sum() == 0
-        assert (shown > 10) == seen, flags
+        return hit & (np.abs(out["SURFACE_POSITION"].reshape(-1, 4)[:, 1] - 1.0) < 1e-3)
+
+    for flags, primary, mirrored in ((FPV, False, True), (FP, True, False), (0, True, True)):
+        out = render([(FLOOR, None, mirror, 0xC0C0C0), (blocker, None, flags, 0x404040)], cam,
+                     rr=rtgl.reflect_refract_params(maxReflectRefractDepth=1))
+        plain = render([(FLOOR, None, 0, 0xC0C0C0), (blocker, None, flags, 0x404040)], cam)
+        direct = at_blocker(plain)  # seen by primary rays (the floor cannot hide it)
+        through = at_blocker(out) & ~direct  # seen only in the mirror
+        print(f"  flags {flags}: blocker pixels seen directly {direct.sum()}, in the mirror {through.sum()}", flush=True)
+        assert (direct.sum() > 10) == primary, flags
+        assert (through.sum() > 10) == mirrored, flags
+        assert np.array_equal(at_blocker(out) & direct, direct), flags
 
 
 def test_world_two_and_the_world_cull_mask():

@@ -14,9 +14,21 @@ Tolerances (per frame, over non-sky pixels):
 - Unfiltered direct light within 1e-3 relative on >= 99.5% of pixels
   (measured 100% under the sun, 99.89% with sphere lights only: their
   sampled points are FP32 on the GPU).
-- Unfiltered specular and indirect within 1e-3 on >= 97%: bounce directions
-  are FP32 on the GPU and FP64 in the oracle, so a bounce grazing an edge can
-  hit a different triangle.
+- Unfiltered specular and indirect within 1e-3 on >= 97% of the pixels
+  outside the ReSTIR GI tie class, and their image means within 3% over all
+  pixels: bounce directions are FP32 on the GPU and FP64 in the oracle, so a
+  bounce grazing an edge can hit a different triangle.  The tie class
+  (OR_PLANE_GI_TIE, marked by the oracle) holds the pixels whose GI reservoir
+  depends on a cosine at a reused sample that FP32 cannot resolve: with flat
+  normals (the default, GEOM_INST_FLAG_GENERATE_NORMALS) a rough surface's
+  GGX bounce that leaves below the surface re-hits its own triangle 0.01 away,
+  and that sample's cosine towards any receiver on the same plane is 0 up to
+  rounding; the GI reuse weights it by a ratio of two such cosines
+  (RtRaygenIndirect.inl:365-379), which is rounding noise in FP32 and in the
+  reference alike.  Its size is printed (GI_TIE); the image means over every
+  pixel stay within 3%, and the same frames with DONT_GENERATE_NORMALS
+  (smooth normals: such a sample's normal is not in its own plane) hold the
+  bound on every pixel.
 - Gradient stratum choice equal on >= 99.9%; filtered gradients within 0.02.
 - Denoised radiance (PRE_FINAL) within 1e-2 relative on >= 80% of pixels and
   its image mean within 1%: the a-trous filter spreads the few differing
@@ -26,7 +38,7 @@ import numpy as np
 import pytest
 
 from raytracedgl1_amd import ffi, rtgl, scenes
-from oracle_lib import OracleFramebuffers, OraclePathTracer
+from oracle_lib import DONT_GENERATE_NORMALS, OR_PLANE_GI_TIE, OracleFramebuffers, OraclePathTracer
 
 pytestmark = pytest.mark.gpu
 
@@ -124,13 +136,22 @@ def _parity_frames(oracle, flags=None, rr=None, tex=None, scene_fn=None):
             m["ROUGHNESS"] = np.mean(np.abs(g["METALLIC_ROUGHNESS"][hit, 1] - oc[P["METALLIC_ROUGHNESS"]][hit, 1]) < 1e-4)
             for k in ("RESERVOIRS_INITIAL", "RESERVOIRS"):
                 m[k] = np.mean(g[k][hit, 0] == oc[P[k]][hit, 0].astype(np.uint32))
+            # the oracle's tie class of ReSTIR GI (OR_PLANE_GI_TIE): pixels whose
+            # reservoir depends on a cosine that FP32 cannot resolve
+            tie = oc[OR_PLANE_GI_TIE][..., 0] != 0
+            m["GI_TIE"] = np.mean(tie[hit])
             for k in ("UNFILTERED_DIRECT", "UNFILTERED_SPECULAR", "UNFILTERED_INDIR"):
-                e, gg = oc[P[k]][hit, :3], g[k][hit, :3]
+                sel = hit if k == "UNFILTERED_DIRECT" else hit & ~tie
+                e, gg = oc[P[k]][sel, :3], g[k][sel, :3]
                 rr_ = _rel(gg, e, 1e-3 * np.abs(e).mean() + 1e-12).max(axis=1)
                 m[k] = np.mean(rr_ < 1e-3)
                 m[k + " 1e-2"] = np.mean(rr_ < 1e-2)
                 m[k + " p99"] = np.quantile(rr_, 0.99)
                 m[k + " mean"] = gg.mean() / max(e.mean(), 1e-30)
+                # every pixel, ties included
+                e_all, g_all = oc[P[k]][hit, :3], g[k][hit, :3]
+                m[k + " all"] = np.mean(_rel(g_all, e_all, 1e-3 * np.abs(e_all).mean() + 1e-12).max(axis=1) < 1e-3)
+                m[k + " mean all"] = g_all.mean() / max(e_all.mean(), 1e-30)
             m["GRADIENT_PREV_PIX"] = np.mean(g["GRADIENT_PREV_PIX"][..., 0] == oc[P["GRADIENT_PREV_PIX"]][..., 0])
             m["gradient max diff"] = np.abs(g["DIS_GRADIENT_HISTORY"][..., :3] - oc[P["DIS_GRADIENT_HISTORY"]][..., :3]).max()
             e, gg = oc[P["PRE_FINAL"]][..., :3], g["PRE_FINAL"][..., :3]
@@ -140,6 +161,23 @@ def _parity_frames(oracle, flags=None, rr=None, tex=None, scene_fn=None):
             yield f, g, oc, m
     finally:
         r.close()
+
+
+def test_frames_match_the_fp64_oracle_smooth_normals(oracle):
+    """The same frames with every primitive's supplied (smooth) normals:
+    no GI ties, and the bounce bounds hold on every pixel."""
+    print("parity: traced passes vs FP64 oracle with DONT_GENERATE_NORMALS, terrain-boxes 160x90", flush=True)
+    s = _scene_with_spheres()
+    flags = {i: DONT_GENERATE_NORMALS for i in range(len(s.objects))}
+    for f, g, oc, m in _parity_frames(oracle, flags=flags):
+        assert m["IS_SKY"] == 1.0 and m["VISIBILITY_BUFFER"] == 1.0, f
+        for k in ("RESERVOIRS_INITIAL", "RESERVOIRS"):
+            assert m[k] >= 0.999, (f, k)
+        # over every pixel: the tie class is conservative here (short
+        # bounces flag their samples), and the reuse decisions still agree
+        for k, need in (("UNFILTERED_DIRECT", 0.995), ("UNFILTERED_SPECULAR", 0.97), ("UNFILTERED_INDIR", 0.97)):
+            assert m[k + " all"] >= need, (f, k, m[k + " all"])
+            assert abs(m[k + " mean all"] - 1.0) < 0.03, (f, k)
 
 
 def test_frames_match_the_fp64_oracle(oracle):
@@ -155,6 +193,7 @@ def test_frames_match_the_fp64_oracle(oracle):
         for k, need in (("UNFILTERED_DIRECT", 0.995), ("UNFILTERED_SPECULAR", 0.97), ("UNFILTERED_INDIR", 0.97)):
             assert m[k] >= need, (f, k, m[k])
             assert abs(m[k + " mean"] - 1.0) < 0.03, (f, k)
+            assert abs(m[k + " mean all"] - 1.0) < 0.03, (f, k)
         assert m["GRADIENT_PREV_PIX"] >= 0.999, f
         assert m["gradient max diff"] < 0.02, (f, m["gradient max diff"])
         assert m["PRE_FINAL"] >= 0.8, (f, m["PRE_FINAL"])
