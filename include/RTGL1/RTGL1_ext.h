@@ -151,6 +151,26 @@ RGAPI RgResult RGCONV rgExtTraceRays( RgExtBvh        bvh,
                                       RgExtHit*       d_hits,
                                       uint32_t        flags,
                                       void*           stream );
+/* Allocation-free rgExtTraceRays, for callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  No counterpart in the reference.  Same arguments and
+ * results as rgExtTraceRays on a single-scene BVH (rgExtBvhBuild).  Its
+ * scratch is one fixed device array per GPU of 40 MiB + 8.25 KiB, held
+ * until the process ends: the ray-chunk counters, the error word, the
+ * statistics and the global traversal stacks of a persistent grid of at
+ * most 1024 blocks (64 stack entries per ray, 24 of them in LDS; a deeper
+ * stack sets RG_EXT_TRACE_ERROR_STACK_OVERFLOW).  Every call on a GPU
+ * shares that array, so calls that overlap on one GPU race.  After a call,
+ * rgExtBvhGetTraceErrors / rgExtBvhGetTraceStats of the BVH report that
+ * call's error bits and counts.  More than RG_EXT_TRACE_NOALLOC_MAX_RAYS
+ * rays are rejected with RG_RESULT_WRONG_FUNCTION_ARGUMENT. */
+#define RG_EXT_TRACE_NOALLOC_MAX_RAYS 0xF0000000u
+RGAPI RgResult RGCONV rgExtTraceRaysNoAlloc( RgExtBvh        bvh,
+                                             const RgExtRay* d_rays,
+                                             uint32_t        rayCount,
+                                             RgExtHit*       d_hits,
+                                             uint32_t        flags,
+                                             void*           stream );
 /* ------------------------------------------------------ batched scenes */
 /* Several scenes in one BVH object, built by one pass sequence whose every
  * kernel covers all scenes: the triangles of scene s are

@@ -35,6 +35,7 @@ struct RgExtBvh_T
 {
     BvhObject    bvh;
     TraceContext trace;
+    bool         lastNoAlloc = false; // the last trace was rgExtTraceRaysNoAlloc
 };
 
 namespace
@@ -262,7 +263,18 @@ RgResult rgExtTraceRays( RgExtBvh bvh, const RgExtRay* d_rays, uint32_t rayCount
     return Guard( [ & ] {
         RG_REQUIRE( bvh && ( ( d_rays && d_hits ) || rayCount == 0 ), RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
         RG_REQUIRE( ( flags & ~7u ) == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "unknown trace flags" );
+        bvh->lastNoAlloc = false;
         TraceRays( bvh->bvh, bvh->trace, d_rays, rayCount, d_hits, flags, AsStream( stream ) );
+    } );
+}
+
+RgResult rgExtTraceRaysNoAlloc( RgExtBvh bvh, const RgExtRay* d_rays, uint32_t rayCount, RgExtHit* d_hits, uint32_t flags, void* stream )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( bvh && ( ( d_rays && d_hits ) || rayCount == 0 ), RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        RG_REQUIRE( ( flags & ~7u ) == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "unknown trace flags" );
+        TraceRaysNoAlloc( bvh->bvh, d_rays, rayCount, d_hits, flags, AsStream( stream ) );
+        bvh->lastNoAlloc = true;
     } );
 }
 
@@ -325,6 +337,11 @@ RgResult rgExtBvhGetTraceStats( RgExtBvh bvh, uint64_t* pNodeVisits, uint64_t* p
 {
     return Guard( [ & ] {
         RG_REQUIRE( bvh && pNodeVisits && pTriangleTests, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        if( bvh->lastNoAlloc )
+        {
+            ReadTraceNoAllocWords( nullptr, pNodeVisits, pTriangleTests );
+            return;
+        }
         ReadTraceStats( bvh->trace, pNodeVisits, pTriangleTests );
     } );
 }
@@ -333,6 +350,11 @@ RgResult rgExtBvhGetTraceErrors( RgExtBvh bvh, uint32_t* pErrors )
 {
     return Guard( [ & ] {
         RG_REQUIRE( bvh && pErrors, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        if( bvh->lastNoAlloc )
+        {
+            ReadTraceNoAllocWords( pErrors, nullptr, nullptr );
+            return;
+        }
         *pErrors = ReadTraceErrors( bvh->trace );
     } );
 }

@@ -178,6 +178,11 @@ void TraceRaysBatch( const BvhObject& bvh, TraceContext& ctx, const RgExtRay* d_
 void AutotuneTrace( const BvhObject& bvh, TraceContext& ctx, const RgExtRay* d_rays, uint32_t count, const uint32_t* rayOffsets, RgExtHit* d_hits,
                     uint32_t flags, hipStream_t stream, RgExtTraceTuning* out );
 // Synchronous reads of the last un-accumulated launch's results.
+// rgExtTraceRays without allocation: scratch in a fixed per-GPU device
+// array (rg_trace.hip); ReadTraceNoAllocWords reads the last such call's
+// error bits and counts on the current device (synchronises it).
+void TraceRaysNoAlloc( const BvhObject& bvh, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags, hipStream_t stream );
+void ReadTraceNoAllocWords( uint32_t* errors, uint64_t* nodeVisits, uint64_t* triTests );
 void     ReadTraceStats( const TraceContext& ctx, uint64_t* nodeVisits, uint64_t* triTests );
 uint32_t ReadTraceErrors( const TraceContext& ctx );
 

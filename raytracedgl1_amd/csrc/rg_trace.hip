@@ -20,6 +20,8 @@
 #include "rg_trace_common.h"
 #include "rg_textures.h"
 
+#include <mutex>
+
 namespace rg
 {
 namespace
@@ -747,6 +749,155 @@ void TraceRays( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, u
 {
     RG_REQUIRE( bvh.sceneCount == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "a batched BVH is traced with TraceRaysBatch" );
     TraceRaysImpl( bvh, s, d_rays, count, d_hits, flags, stream, evBefore, evAfter, accum, nullptr );
+}
+
+// ------------------------------------------------ allocation-free traversal
+//
+// TraceRaysNoAlloc keeps its scratch in one file-scope device array per GPU
+// instead of a TraceContext: the 32 ray-chunk counters (256 bytes apart),
+// the error word, the statistics and the global overflow stacks of a
+// persistent grid of at most NOALLOC_MAX_BLOCKS blocks, NOALLOC_OVF entries
+// per lane behind the 24 in LDS (a 64-entry stack; deeper entries set
+// RG_EXT_TRACE_ERROR_STACK_OVERFLOW).  Every call on a GPU uses that GPU's
+// array, so calls that overlap on one GPU race.
+namespace
+{
+constexpr uint32_t NOALLOC_MAX_BLOCKS = 1024;
+constexpr uint32_t NOALLOC_OVF        = 40;
+constexpr int      NOALLOC_VARIANT    = 1; // the 24-entry LDS stack
+constexpr uint32_t NOALLOC_ERROR      = RAY_PARTITIONS * COUNTER_STRIDE; // u32 index of the error word
+constexpr uint32_t NOALLOC_STATS      = NOALLOC_ERROR + 2;               // u32 index of 2 x u64
+constexpr uint32_t NOALLOC_HEADER     = NOALLOC_ERROR + 64;              // u32 index of the overflow stacks
+constexpr size_t   NOALLOC_WORDS      = NOALLOC_HEADER + size_t( NOALLOC_MAX_BLOCKS ) * TR_THREADS * NOALLOC_OVF;
+static_assert( ( NOALLOC_STATS * 4 ) % 8 == 0, "statistics alignment" );
+static_assert( ( NOALLOC_HEADER * 4 ) % 256 == 0, "stack alignment" );
+static_assert( TRACE_STACK_DEPTH[ NOALLOC_VARIANT ] == 24, "no-alloc stack variant" );
+
+// Clears the counters, the error word and the statistics.
+__global__ __launch_bounds__( 64 ) void k_noalloc_clear( uint32_t* hdr )
+{
+    const uint32_t t = threadIdx.x;
+    if( t < RAY_PARTITIONS )
+    {
+        hdr[ t * COUNTER_STRIDE ] = 0u;
+    }
+    if( t < 6 )
+    {
+        hdr[ NOALLOC_ERROR + t ] = 0u; // error word, pad, 2 x u64 statistics
+    }
+}
+
+struct NoAllocDevice
+{
+    uint32_t* scratch = nullptr; // the device address of `scratch` on this GPU
+    uint32_t  grid[ 2 ] = {};    // persistent blocks per node format (FP32, FP16)
+};
+constexpr int MAX_DEVICES = 64;
+std::mutex    g_noAllocMutex;
+NoAllocDevice g_noAlloc[ MAX_DEVICES ];
+} // namespace
+
+static __device__ __align__( 256 ) uint32_t scratch[ NOALLOC_WORDS ];
+
+namespace
+{
+// Set-up once per device, with that device current: the array's device
+// address (the host-side name `scratch` is not one) and the grid sizes.
+const NoAllocDevice& NoAllocFor( int dev )
+{
+    RG_REQUIRE( dev >= 0 && dev < MAX_DEVICES, RG_RESULT_INTERNAL_ERROR, "device index out of range" );
+    std::lock_guard< std::mutex > lock( g_noAllocMutex );
+    NoAllocDevice&                d = g_noAlloc[ dev ];
+    if( !d.scratch )
+    {
+        void* ptr = nullptr;
+        RG_HIP_CHECK( hipGetSymbolAddress( &ptr, HIP_SYMBOL( scratch ) ) );
+        int cus = 0;
+        RG_HIP_CHECK( hipDeviceGetAttribute( &cus, hipDeviceAttributeMultiprocessorCount, dev ) );
+        for( int h = 0; h < 2; h++ )
+        {
+            int perCu = 0;
+            RG_HIP_CHECK( hipOccupancyMaxActiveBlocksPerMultiprocessor( &perCu, TraceKernel( h, NOALLOC_VARIANT ), TR_THREADS, 0 ) );
+            const uint32_t g = uint32_t( cus * ( perCu > 0 ? perCu : 1 ) );
+            d.grid[ h ]      = g < NOALLOC_MAX_BLOCKS ? g : NOALLOC_MAX_BLOCKS;
+        }
+        d.scratch = static_cast< uint32_t* >( ptr );
+    }
+    return d;
+}
+} // namespace
+
+void TraceRaysNoAlloc( const BvhObject& bvh, const RgExtRay* d_rays, uint32_t count, RgExtHit* d_hits, uint32_t flags, hipStream_t stream )
+{
+    RG_REQUIRE( bvh.sceneCount == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "a batched BVH is traced with TraceRaysBatch" );
+    RG_REQUIRE( bvh.nodeCount > 0, RG_RESULT_WRONG_FUNCTION_CALL, "BVH is not built" );
+    RG_REQUIRE( bvh.nodeFormat == RG_EXT_BVH_NODE_FP32 || bvh.nodeFormat == RG_EXT_BVH_NODE_FP16, RG_RESULT_WRONG_FUNCTION_ARGUMENT,
+                "unsupported node format" );
+    RG_REQUIRE( count <= RG_EXT_TRACE_NOALLOC_MAX_RAYS, RG_RESULT_WRONG_FUNCTION_ARGUMENT,
+                "too many rays for one call (at most RG_EXT_TRACE_NOALLOC_MAX_RAYS)" );
+    int dev = 0;
+    RG_HIP_CHECK( hipGetDevice( &dev ) );
+    const NoAllocDevice& nd = NoAllocFor( dev );
+    if( count == 0 )
+    {
+        return;
+    }
+    const int      h      = bvh.nodeFormat == RG_EXT_BVH_NODE_FP16 ? 1 : 0;
+    const uint32_t blocks = DivUp( count, TR_THREADS ) < nd.grid[ h ] ? DivUp( count, TR_THREADS ) : nd.grid[ h ];
+    RG_REQUIRE( size_t( blocks ) * TR_THREADS * NOALLOC_OVF <= NOALLOC_WORDS - NOALLOC_HEADER, RG_RESULT_INTERNAL_ERROR,
+                "no-alloc traversal grid exceeds its scratch" );
+    k_noalloc_clear<<< 1, 64, 0, stream >>>( nd.scratch );
+    CheckLaunch( "k_noalloc_clear" );
+
+    TraceArgs a{};
+    a.nodes         = bvh.nodes.ptr;
+    a.tris          = reinterpret_cast< const float4* >( bvh.sortedTris.ptr );
+    a.rays          = d_rays;
+    a.hits          = d_hits;
+    a.count         = count;
+    a.flags         = flags;
+    a.partitions    = blocks < RAY_PARTITIONS ? blocks : RAY_PARTITIONS;
+    a.rayCounter    = nd.scratch;
+    a.errorFlag     = nd.scratch + NOALLOC_ERROR;
+    a.overflowStack = nd.scratch + NOALLOC_HEADER;
+    a.ovfStride     = NOALLOC_OVF;
+    a.stackTotal    = TRACE_STACK_DEPTH[ NOALLOC_VARIANT ] + NOALLOC_OVF;
+    a.stats         = ( flags & RG_EXT_TRACE_COLLECT_STATS ) ? reinterpret_cast< uint64_t* >( nd.scratch + NOALLOC_STATS ) : nullptr;
+    a.segCount      = 0;
+    a.rootCount     = 0;
+    for( int k = 0; k < 8; k++ )
+    {
+        a.rootRef[ k ] = RG_EXT_BVH_CHILD_EMPTY;
+    }
+    a.chunk = RAY_CHUNK_DEFAULT;
+    a.alpha = nullptr;
+    a.tex   = TexView{};
+    hipLaunchKernelGGL( TraceKernel( h, NOALLOC_VARIANT, false ), dim3( blocks ), dim3( TR_THREADS ), 0, stream, a );
+    CheckLaunch( "k_trace" );
+}
+
+void ReadTraceNoAllocWords( uint32_t* errors, uint64_t* nodeVisits, uint64_t* triTests )
+{
+    int dev = 0;
+    RG_HIP_CHECK( hipGetDevice( &dev ) );
+    const NoAllocDevice& nd = NoAllocFor( dev );
+    uint32_t             w[ 6 ] = {};
+    RG_HIP_CHECK( hipDeviceSynchronize() );
+    RG_HIP_CHECK( hipMemcpy( w, nd.scratch + NOALLOC_ERROR, sizeof( w ), hipMemcpyDeviceToHost ) );
+    if( errors )
+    {
+        *errors = w[ 0 ];
+    }
+    uint64_t st[ 2 ];
+    std::memcpy( st, w + 2, sizeof( st ) );
+    if( nodeVisits )
+    {
+        *nodeVisits = st[ 0 ];
+    }
+    if( triTests )
+    {
+        *triTests = st[ 1 ];
+    }
 }
 
 void TraceRaysFrame( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, const uint8_t* d_rayMasks, uint32_t defaultMask, uint32_t count,

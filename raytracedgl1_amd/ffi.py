@@ -135,6 +135,7 @@ _SIGS = {
     "rgExtBvhBuild": (_RES, [_P, _P, _U32, C.POINTER(RgExtBvhBuildParams), _P]),
     "rgExtBvhGetView": (_RES, [_P, C.POINTER(RgExtBvhView)]),
     "rgExtTraceRays": (_RES, [_P, _P, _U32, _P, _U32, _P]),
+    "rgExtTraceRaysNoAlloc": (_RES, [_P, _P, _U32, _P, _U32, _P]),
     "rgExtBvhBuildBatch": (_RES, [_P, _P, _P, _U32, C.POINTER(RgExtBvhBuildParams), _P]),
     "rgExtBvhGetSceneRoots": (_RES, [_P, _P, _U32, C.POINTER(_U32)]),
     "rgExtTraceRaysBatch": (_RES, [_P, _P, _P, _P, _U32, _P]),
