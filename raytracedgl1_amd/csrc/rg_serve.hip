@@ -187,6 +187,7 @@ struct RgExtServer_T
     std::condition_variable                     cvWork, cvIdle;
     std::thread                                 worker;
     bool                                        stop = false;
+    uint32_t                                    launching  = 0; // frames being encoded outside the lock
     uint32_t                                    roundRobin = 0;
     uint64_t                                    sessionBytes = 0, sharedBytes = 0, sceneBytes = 0;
     uint64_t                                    done = 0, dropped = 0, failed = 0;
@@ -207,7 +208,7 @@ struct RgExtServer_T
 
     bool Busy() const
     {
-        if( !ready.empty() )
+        if( !ready.empty() || launching > 0 )
         {
             return true;
         }
@@ -241,12 +242,24 @@ struct RgExtServer_T
         }
     }
 
-    void Launch( RgExtSession_T& s, StreamCtx& st )
+    // Called with the lock held; encodes the frame with the lock released,
+    // so submissions, polls and statistics do not wait for the host-side
+    // encoding.  The session counts the frame as in flight meanwhile (a
+    // destroy waits for it) and the server stays busy (WaitIdle waits).
+    // Only this thread launches frames and touches the session's camera,
+    // framebuffers and the stream's state.
+    void Launch( RgExtSession_T& s, StreamCtx& st, std::unique_lock< std::mutex >& lk )
     {
         FrameRequest req = std::move( *s.pending );
         s.pending.reset();
         req.Relink();
-        const double start = NowSeconds();
+        s.inflight++;
+        launching++;
+        lk.unlock();
+        const double start   = NowSeconds();
+        hipEvent_t   ev      = nullptr;
+        bool         ok      = false;
+        std::string  failure;
         try
         {
             RgExtScene_T& sc = *s.scene;
@@ -257,16 +270,26 @@ struct RgExtServer_T
                 s.fbs.fb.Swap();
             }
             EncodeFrame( s.g, st.tc, s.cam.params, sc.bvh, sc.ds, s.fbs.fb, req.di, st.stream );
-            hipEvent_t ev = st.events[ st.nextEvent++ % st.events.size() ];
+            ev = st.events[ st.nextEvent++ % st.events.size() ];
             RG_HIP_CHECK( hipEventRecord( ev, st.stream ) );
-            st.inflight.push_back( { &s, req.tag, s.cam.frameId, req.submit, start, ev } );
-            s.inflight++;
-            s.cam.frameId++;
-            s.cam.firstFrame = false;
+            ok = true;
         }
         catch( const Error& e )
         {
-            SetLastError( e.what() );
+            failure = e.what();
+        }
+        lk.lock();
+        launching--;
+        if( ok )
+        {
+            st.inflight.push_back( { &s, req.tag, s.cam.frameId, req.submit, start, ev } );
+            s.cam.frameId++;
+            s.cam.firstFrame = false;
+        }
+        else
+        {
+            s.inflight--;
+            SetLastError( failure.c_str() );
             Result( &s, req.tag, RG_EXT_FRAME_FAILED, s.cam.frameId, req.submit, start, NowSeconds() );
         }
     }
@@ -305,10 +328,12 @@ struct RgExtServer_T
                         s->fbs.fb.shared  = &st.planes;
                         s->g.pt.wf        = &st.wf;
                     }
-                    it        = ready.erase( it );
+                    ready.erase( it );
                     s->queued = false;
-                    Launch( *s, st );
+                    Launch( *s, st, lk );
                     launched = true;
+                    // the ready list may have changed while the lock was released
+                    it = ready.begin();
                 }
                 else
                 {

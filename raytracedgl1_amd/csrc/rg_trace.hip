@@ -32,6 +32,12 @@ constexpr int      STACK_TOTAL   = 128; // entries per lane: LDS part + global o
 constexpr int      STACK_OVF_STRIDE = 112; // global entries per lane, enough for the smallest LDS part (16)
 constexpr uint32_t MAX_STEPS_PER_RAY = 1u << 22;
 constexpr uint32_t RAY_CHUNK_DEFAULT = 64; // rays per counter atomic
+// A wave refills its idle lanes once at least this many are idle (or none
+// is busy): the refill block (ballots, the chunk atomic, the ray set-up with
+// its six divisions) then runs for many lanes at once instead of for the
+// one or two lanes that finish in a typical iteration.  Measured on the
+// 1080p frame: 1 -> 3.77 ms of traversal, 8 -> 3.64, 16 -> 3.57, 32 -> 3.60.
+constexpr uint32_t REFILL_MIN_LANES = 16;
 constexpr uint32_t RAY_PARTITIONS    = 32;  // ray counters
 constexpr uint32_t COUNTER_STRIDE    = 64;  // u32 between counters (256 B)
 
@@ -242,7 +248,7 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
         // take rays from the chunk (wave-uniform bookkeeping, no atomic).
         const bool     need = !active && !exhausted;
         const uint64_t mask = __ballot( need );
-        if( mask != 0ull )
+        if( mask != 0ull && ( __popcll( mask ) >= REFILL_MIN_LANES || __ballot( active ) == 0ull ) )
         {
             const uint32_t cnt    = __popcll( mask );
             const uint32_t rank   = __popcll( mask & ltMask );

@@ -292,3 +292,41 @@ def test_cull_mask_arguments_are_checked():
             assert l.rgDrawFrame(r.h, C.byref(di)) == 11  # RG_RESULT_WRONG_FUNCTION_ARGUMENT
         finally:
             r.close()
+
+
+def test_one_triangle_classes_keep_their_masks():
+    """A class of exactly one triangle is its own segment of the frame BVH
+    (a leaf root): one glass triangle over a lit floor casts no shadow
+    (INSTANCE_MASK_REFRACT is not in the shadow mask, RaygenCommon.h:137-150)
+    and is seen by primary rays; one opaque triangle beside glass does cast
+    its shadow."""
+    print("masks: one-triangle glass and opaque classes", flush=True)
+    glass = rtgl.RG_MESH_PRIMITIVE_GLASS
+    tri = verts([(-0.8, 2.0, -0.8), (-0.8, 2.0, 0.8), (0.8, 2.0, 0.0)], [(0.0, 1.0, 0.0)] * 3)
+    alone = render([(FLOOR, None, 0, 0xC0C0C0)], CAM)
+    out = render([(FLOOR, None, 0, 0xC0C0C0), (tri, None, glass, 0x404040)], CAM)
+    assert _blocker_pixels(out, 1).sum() > 10  # the glass triangle is visible
+    # no shadow: every pixel showing the same floor point as without the
+    # glass (all but those seen through it) is lit exactly as without it,
+    # the floor under the triangle included
+    p_out, p_alone = out["SURFACE_POSITION"].reshape(-1, 4), alone["SURFACE_POSITION"].reshape(-1, 4)
+    vis, hit = _hits(alone)
+    same = hit & np.all(p_out == p_alone, -1)
+    assert same.sum() > 0.9 * hit.sum()
+    under = same & (np.abs(p_alone[:, 0]) < 0.2) & (np.abs(p_alone[:, 2]) < 0.2)
+    assert under.sum() > 5
+    d_out = out["UNFILTERED_DIRECT"].reshape(-1, 4)[same, :3]
+    d_alone = alone["UNFILTERED_DIRECT"].reshape(-1, 4)[same, :3]
+    assert np.array_equal(d_out, d_alone)
+    # one opaque triangle (its own class) beside the glass triangle: a shadow
+    opaque = verts([(-0.8, 2.0, -0.8), (-0.8, 2.0, 0.8), (0.8, 2.0, 0.0)], [(0.0, 1.0, 0.0)] * 3)
+    out = render([(FLOOR, None, 0, 0xC0C0C0), (opaque, None, SKYVIS, 0x404040),
+                  (quad((3.0, 1.0, 3.0), 0.5), None, glass, 0x404040)], CAM, cull=W0 | W2)
+    pos = out["SURFACE_POSITION"].reshape(-1, 4)
+    d = out["UNFILTERED_DIRECT"].reshape(-1, 4)[:, :3].sum(-1)
+    vis, hit = _hits(out)
+    on_floor = hit & ((vis[:, 1] & 0xFFF) == 0)
+    under = on_floor & (np.abs(pos[:, 0]) < 0.2) & (np.abs(pos[:, 2]) < 0.2)
+    around = on_floor & (np.abs(pos[:, 0]) > 1.5) & (np.abs(pos[:, 0]) < 2.5) & (np.abs(pos[:, 2]) < 1.0)
+    assert under.sum() > 5 and around.sum() > 5
+    assert d[under].mean() < 0.05 * d[around].mean()
