@@ -316,6 +316,13 @@ typedef enum RgExtPlane
      * written by the temporal pass for the stencil passes (no counterpart in
      * the reference, whose passes decode the normal image at every tap) */
     RG_EXT_PLANE_DENOISE_GUIDE,          /* C f32x4 (normal xyz, depth)                      */
+    /* what the ReSTIR GI spatial reuse reads of a neighbour, in one 64-byte
+     * record written after the initial samples (no counterpart in the
+     * reference, which reads the G-buffer and initial-sample images): f32x4
+     * surface position xyz + linear depth, f32x4 decoded normal xyz + is-sky,
+     * f32x4 initial sample position xyz + 1/source pdf, f32x4 decoded sample
+     * normal xyz + E5B9G9R9 radiance code */
+    RG_EXT_PLANE_GI_REUSE,               /* C u32x16                                          */
     RG_EXT_PLANE_COUNT
 } RgExtPlane;
 
@@ -558,27 +565,52 @@ RGAPI RgResult RGCONV rgExtSetStaticGeometry( RgInstance instance, RgBool32 enab
  * Scheduling: a session with a frame in flight stays on that frame's
  * stream, so its frames stay in order; an idle session's next frame goes to
  * the stream with the fewest frames queued.  A dispatcher thread launches
- * the oldest waiting request whose stream has fewer than
- * framesInFlightPerStream frames queued, and retires frames as their
- * completion events fire.  A
- * request that is still waiting when its session submits again is replaced
- * by the newer one and reported as dropped (the newest camera wins).
+ * waiting requests while a stream has fewer than framesInFlightPerStream
+ * launches queued, and retires frames as their completion events fire.
+ * The request it launches first is the oldest waiting one
+ * (RG_EXT_SCHEDULE_OLDEST_FIRST) or the newest (RG_EXT_SCHEDULE_NEWEST_FIRST,
+ * which keeps the latency of served frames low when more is offered than
+ * the GPU can render).  A request that is still waiting when its session
+ * submits again is replaced by the newer one and reported as dropped (the
+ * newest camera wins); one that has waited maxQueueSeconds is dropped as
+ * expired.
  *
- * A session renders the same pixels alone or among others: its frames
- * depend only on its own requests and the scene. */
+ * Batching: with batchFrames > 1, one launch takes up to batchFrames
+ * waiting requests of sessions of the same scene (and equal ray cull masks
+ * and reflection depth) and renders them together: each traversal of a
+ * frame stage covers every frame's rays in one persistent launch, the
+ * shading, A-SVGF and compose kernels run per frame.  Each stream holds
+ * batchFrames sets of within-frame planes.
+ *
+ * Readback: with RG_EXT_SERVER_READBACK, each frame ends with its FINAL
+ * plane packed to RGBA8 (sRGB transfer, the swapchain encoding) and copied
+ * to pinned host memory of the session (two images per session, used in
+ * turn); a frame is finished when that copy is.  RgExtFrameResult::pImage
+ * points at the frame's pixels, valid until the session finishes two more
+ * frames.
+ *
+ * A session renders the same pixels alone, among others and in a batch:
+ * its frames depend only on its own requests and the scene. */
 typedef struct RgExtServer_T*  RgExtServer;
 typedef struct RgExtScene_T*   RgExtScene;
 typedef struct RgExtSession_T* RgExtSession;
 
+#define RG_EXT_SCHEDULE_OLDEST_FIRST 0u
+#define RG_EXT_SCHEDULE_NEWEST_FIRST 1u
+#define RG_EXT_SERVER_READBACK       1u
+
 typedef struct RgExtServerCreateInfo
 {
     uint32_t streamCount;             /* 1..16; 0 = 4                                  */
-    uint32_t framesInFlightPerStream; /* 1..8; 0 = 2                                   */
+    uint32_t framesInFlightPerStream; /* launches queued per stream, 1..8; 0 = 2       */
     uint32_t maxRenderWidth;          /* capacity of the per-stream within-frame planes */
     uint32_t maxRenderHeight;
     uint32_t precisionFlags;          /* RG_EXT_PRECISION_* for every scene and session */
-    uint32_t _reserved;
+    uint32_t batchFrames;             /* frames per launch, 1..16; 0 = 1                */
     uint64_t memoryBudgetBytes;       /* session bytes admitted; 0 = no limit           */
+    double   maxQueueSeconds;         /* waiting longer drops a request; 0 = no limit   */
+    uint32_t schedule;                /* RG_EXT_SCHEDULE_*                              */
+    uint32_t flags;                   /* RG_EXT_SERVER_READBACK                         */
 } RgExtServerCreateInfo;
 
 /* rgExtSessionCreate: the session's bytes do not fit the budget. */
@@ -587,6 +619,7 @@ typedef struct RgExtServerCreateInfo
 #define RG_EXT_FRAME_DONE    0u
 #define RG_EXT_FRAME_DROPPED 1u /* replaced by a newer request before it started */
 #define RG_EXT_FRAME_FAILED  2u
+#define RG_EXT_FRAME_EXPIRED 3u /* waited maxQueueSeconds without starting       */
 
 typedef struct RgExtFrameResult
 {
@@ -596,7 +629,10 @@ typedef struct RgExtFrameResult
     uint32_t     frameId;    /* the session's frame index (DONE)                    */
     double       submitTime; /* seconds on the server's steady clock                */
     double       startTime;  /* the frame's work was enqueued (DONE)                */
-    double       endTime;    /* completion seen (DONE) or drop time                  */
+    double       endTime;    /* completion seen (DONE; after the readback), or drop time */
+    const void*  pImage;     /* RGBA8 pixels, width x height rows (DONE with READBACK) */
+    uint32_t     batchSize;  /* frames in the frame's launch (DONE)                 */
+    uint32_t     _pad;
 } RgExtFrameResult;
 
 typedef struct RgExtServerStats
@@ -607,7 +643,7 @@ typedef struct RgExtServerStats
     uint64_t sharedBytes;    /* per-stream within-frame planes, wavefronts, traversal stacks */
     uint64_t sceneBytes;
     uint64_t framesDone;
-    uint64_t framesDropped;
+    uint64_t framesDropped;  /* replaced or expired                               */
     uint64_t framesFailed;
     double   now;            /* the server's steady clock, seconds                 */
 } RgExtServerStats;
@@ -639,6 +675,21 @@ RGAPI RgResult RGCONV rgExtServerPoll( RgExtServer server, RgExtFrameResult* pRe
 /* Blocks until no request is waiting or in flight. */
 RGAPI RgResult RGCONV rgExtServerWaitIdle( RgExtServer server );
 RGAPI RgResult RGCONV rgExtServerGetStats( RgExtServer server, RgExtServerStats* pOut );
+/* The dispatch policy of the launches that follow (rgExtServerSetPolicy). */
+typedef struct RgExtServerPolicy
+{
+    uint32_t schedule;                /* RG_EXT_SCHEDULE_*                                  */
+    uint32_t batchFrames;             /* 1 .. the batchFrames the server was created with  */
+    uint32_t framesInFlightPerStream; /* launches queued per stream, 1..8                  */
+    uint32_t streams;                 /* streams that take new launches, 1 .. streamCount  */
+    double   maxQueueSeconds;         /* 0 = requests do not expire                        */
+} RgExtServerPolicy;
+
+/* Changes the dispatch policy: fewer streams, frames in flight or frames per
+ * launch bound the latency of served frames, more raise the throughput.  A
+ * session with a frame in flight on a stream no longer in use finishes its
+ * frames there before it moves. */
+RGAPI RgResult RGCONV rgExtServerSetPolicy( RgExtServer server, const RgExtServerPolicy* pPolicy );
 /* The session's framebuffer set: its own persistent planes (FINAL holds the
  * last finished frame) and views of its stream's within-frame planes, which
  * the stream's next frame overwrites.  Owned by the session. */

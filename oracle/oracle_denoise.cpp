@@ -12,10 +12,44 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 namespace
 {
+
+// Rows of a pass on g_threads host threads (or_denoise_set_threads).  Every
+// pass writes planes (or scratch arrays) that it does not read, so the rows
+// are independent and the result does not depend on the thread count.
+int g_threads = 1;
+template< typename F >
+void ParallelRows( int n, F&& f )
+{
+    const int threads = std::max( 1, std::min( g_threads, n ) );
+    if( threads == 1 )
+    {
+        for( int i = 0; i < n; i++ )
+        {
+            f( i );
+        }
+        return;
+    }
+    std::vector< std::thread > ts;
+    for( int t = 0; t < threads; t++ )
+    {
+        ts.emplace_back( [ &, t ] {
+            for( int i = t; i < n; i += threads )
+            {
+                f( i );
+            }
+        } );
+    }
+    for( auto& t : ts )
+    {
+        t.join();
+    }
+}
+
 const double PI            = 3.14159265358979323846;
 const double MAX_RAY_LEN   = 10000.0; // MAX_RAY_LENGTH, ShaderCommonC.h:197
 const int    STRATA        = 3;       // COMPUTE_ASVGF_STRATA_SIZE
@@ -203,6 +237,7 @@ extern "C" int or_plane_components( int plane )
         case RG_EXT_PLANE_DIFF_ACCUM_MOMENTS:
         case RG_EXT_PLANE_GRADIENT_INPUTS:
         case RG_EXT_PLANE_INDIR_INITIAL_RADIANCE: return 2;
+        case RG_EXT_PLANE_GI_REUSE: return 16;
         default: return 4;
     }
 }
@@ -257,8 +292,7 @@ extern "C" void or_denoise_gradient_atrous( const OrFramebuffers* fbp, const RgE
     const int step = 1 << iter;
     const double kernel[ 2 ][ 2 ] = { { 1.0, 0.5 }, { 0.5, 0.25 } };
     std::vector< double > out( size_t( fb.GW ) * fb.GH * 4 );
-    for( int gy = 0; gy < fb.GH; gy++ )
-    {
+    ParallelRows( fb.GH, [ & ]( int gy ) {
         for( int gx = 0; gx < fb.GW; gx++ )
         {
             int lo, hi;
@@ -282,7 +316,7 @@ extern "C" void or_denoise_gradient_atrous( const OrFramebuffers* fbp, const RgE
             double* o = &out[ ( size_t( gy ) * fb.GW + gx ) * 4 ];
             o[ 0 ] = r.x; o[ 1 ] = r.y; o[ 2 ] = r.z; o[ 3 ] = 0.0;
         }
-    }
+    } );
     for( int gy = 0; gy < fb.GH; gy++ )
     {
         for( int gx = 0; gx < fb.GW; gx++ )
@@ -302,8 +336,7 @@ extern "C" void or_denoise_temporal( const OrFramebuffers* fbp, const RgExtFrame
 {
     Fb        fb( fbp );
     const int W = fb.W, H = fb.H;
-    for( int y = 0; y < H; y++ )
-    {
+    ParallelRows( H, [ & ]( int y ) {
         for( int x = 0; x < W; x++ )
         {
             const D3     unfDiff  = fb.get3( RG_EXT_PLANE_UNFILTERED_DIRECT, x, y );
@@ -479,7 +512,7 @@ extern "C" void or_denoise_temporal( const OrFramebuffers* fbp, const RgExtFrame
             fb.set4( RG_EXT_PLANE_SPEC_ACCUM_COLOR, x, y, specAccum, 0.0 );
             fb.set4( RG_EXT_PLANE_INDIR_ACCUM, x, y, indirAccum, 0.0 );
         }
-    }
+    } );
 }
 
 // CmAntiFirefly.comp:96-159 (5x5 min/max clamp, radius 2, centre excluded)
@@ -489,8 +522,7 @@ extern "C" void or_denoise_antifirefly( const OrFramebuffers* fbp, const RgExtFr
     const int W = fb.W, H = fb.H;
     const int srcs[ 3 ] = { RG_EXT_PLANE_DIFF_TEMPORARY, RG_EXT_PLANE_SPEC_ACCUM_COLOR, RG_EXT_PLANE_INDIR_ACCUM };
     const int dsts[ 3 ] = { RG_EXT_PLANE_DIFF_ACCUM_COLOR, RG_EXT_PLANE_SPEC_PING, RG_EXT_PLANE_INDIR_PING };
-    for( int y = 0; y < H; y++ )
-    {
+    ParallelRows( H, [ & ]( int y ) {
         for( int x = 0; x < W; x++ )
         {
             for( int s = 0; s < 3; s++ )
@@ -518,7 +550,7 @@ extern "C" void or_denoise_antifirefly( const OrFramebuffers* fbp, const RgExtFr
                 fb.set4( dsts[ s ], x, y, c, 0.0 );
             }
         }
-    }
+    } );
 }
 
 // CmSVGFEstimateVariance.comp:129-206
@@ -527,8 +559,7 @@ extern "C" void or_denoise_variance( const OrFramebuffers* fbp, const RgExtFrame
     Fb        fb( fbp );
     const int W = fb.W, H = fb.H;
     const double HT = 4.0; // HISTORY_LENGTH_THRESHOLD
-    for( int y = 0; y < H; y++ )
-    {
+    ParallelRows( H, [ & ]( int y ) {
         for( int x = 0; x < W; x++ )
         {
             D3           color = fb.get3( RG_EXT_PLANE_DIFF_ACCUM_COLOR, x, y );
@@ -582,7 +613,7 @@ extern "C" void or_denoise_variance( const OrFramebuffers* fbp, const RgExtFrame
             const double var = std::max( 0.0, s1 - s0 * s0 ) * ( 1.0 + 2.0 * ( 1.0 - histLen / HT ) );
             fb.set4( RG_EXT_PLANE_DIFF_PING, x, y, color, var );
         }
-    }
+    } );
 }
 
 namespace
@@ -637,8 +668,7 @@ extern "C" void or_denoise_atrous( const OrFramebuffers* fbp, const RgExtFramePa
     const size_t np = size_t( W ) * H;
     std::vector< double > oD( np * 4 ), oS( np * 3 ), oI( np * 3 ), fvar( np, -1.0 );
 
-    for( int y = 0; y < H; y++ )
-    {
+    ParallelRows( H, [ & ]( int y ) {
         for( int x = 0; x < W; x++ )
         {
             const size_t i = size_t( y ) * W + x;
@@ -736,7 +766,7 @@ extern "C" void or_denoise_atrous( const OrFramebuffers* fbp, const RgExtFramePa
             oS[ i * 3 + 0 ] = outS.x; oS[ i * 3 + 1 ] = outS.y; oS[ i * 3 + 2 ] = outS.z;
             oI[ i * 3 + 0 ] = outI.x; oI[ i * 3 + 1 ] = outI.y; oI[ i * 3 + 2 ] = outI.z;
         }
-    }
+    } );
 
     for( int y = 0; y < H; y++ )
     {
@@ -797,4 +827,11 @@ extern "C" void or_denoise( const OrFramebuffers* fb, const RgExtFrameParams* g 
     {
         or_denoise_atrous( fb, g, k );
     }
+}
+
+// Host threads for the passes above (1 by default); the results do not
+// depend on it.
+extern "C" void or_denoise_set_threads( int threads )
+{
+    g_threads = std::max( 1, std::min( threads, 256 ) );
 }

@@ -45,6 +45,9 @@ void or_denoise_antifirefly( const OrFramebuffers* fb, const RgExtFrameParams* p
 void or_denoise_variance( const OrFramebuffers* fb, const RgExtFrameParams* p );
 void or_denoise_atrous( const OrFramebuffers* fb, const RgExtFrameParams* p, uint32_t iteration );
 void or_denoise( const OrFramebuffers* fb, const RgExtFrameParams* p );
+/* Host threads the passes use (rows in parallel; the results do not depend
+ * on the count).  1 by default. */
+void or_denoise_set_threads( int threads );
 
 #ifdef __cplusplus
 }

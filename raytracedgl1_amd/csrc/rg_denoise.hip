@@ -210,6 +210,51 @@ __device__ __forceinline__ void LdGuide( const FbPtrs& P, int w, int h, int x, i
     depth          = g.w;
 }
 
+// Transcendentals of filter weights and accumulation speeds on the hardware
+// units: v_exp_f32 / v_log_f32 (base 2) and v_rcp_f32, each within about
+// one ulp, instead of the libm sequences (powf alone is some sixty
+// instructions).  These values only weight normalised averages or set
+// blend factors, so their relative error (about 1e-6 over the arguments
+// the passes produce) moves the denoised radiance by no more than that,
+// against its 1e-3 bound.  Comparisons that choose a branch (reprojection
+// tests, history thresholds) keep IEEE division.
+__device__ __forceinline__ float FastExp( float x )
+{
+    return __builtin_amdgcn_exp2f( x * 1.44269504088896341f );
+}
+__device__ __forceinline__ float FastExp2( float x )
+{
+    return __builtin_amdgcn_exp2f( x );
+}
+// x^y for x >= 0 and y > 0 (pow(0, y) = 0 through log2(0) = -inf)
+__device__ __forceinline__ float FastPow( float x, float y )
+{
+    return __builtin_amdgcn_exp2f( y * __builtin_amdgcn_logf( x ) );
+}
+__device__ __forceinline__ float FastLn( float x )
+{
+    return __builtin_amdgcn_logf( x ) * 0.693147180559945309f;
+}
+__device__ __forceinline__ float Pow5( float x )
+{
+    const float x2 = x * x;
+    return x2 * x2 * x;
+}
+__device__ __forceinline__ float Pow10( float x )
+{
+    const float x5 = Pow5( x );
+    return x5 * x5;
+}
+__device__ __forceinline__ float Pow128( float x )
+{
+#pragma unroll
+    for( int i = 0; i < 7; i++ )
+    {
+        x *= x;
+    }
+    return x;
+}
+
 __device__ __forceinline__ bool TestDepth( float z, float zPrev, float zMotion )
 {
     return fabsf( z - zPrev + zMotion ) / fabsf( z ) < 0.1f;
@@ -255,18 +300,18 @@ __device__ float Parallax( const RgExtFrameParams& g, v3 pos, v3 posPrev )
 }
 __device__ float SpecAccumSpeed( const RgExtFrameParams& g, float Amax, float roughness, float NoV, float parallax )
 {
-    const float a                   = powf( saturate( 1.0f - NoV ), 0.66f );
+    const float a                   = FastPow( saturate( 1.0f - NoV ), 0.66f );
     const float b                   = 1.1f + roughness * roughness;
     const float parallaxSensitivity = ( b + a ) / ( b - a );
     const float powerScale          = 1.0f + parallax * parallaxSensitivity;
-    float       f                   = 1.0f - exp2f( -200.0f * roughness * roughness );
-    f *= powf( saturate( roughness ), 0.5f * powerScale );
+    float       f                   = 1.0f - FastExp2( -200.0f * roughness * roughness );
+    f *= FastPow( saturate( roughness ), 0.5f * powerScale );
     return clampf( 256.0f * f, SpecMinAccumFrames( g ), Amax );
 }
 __device__ float SpecularDominantFactor( float nv, float roughness )
 {
-    const float a = 0.298475f * logf( 39.4115f - 39.0029f * roughness );
-    const float f = powf( clampf( 1.0f - nv, 0.0f, 1.0f ), 10.8649f ) * ( 1.0f - a ) + a;
+    const float a = 0.298475f * FastLn( 39.4115f - 39.0029f * roughness );
+    const float f = FastPow( clampf( 1.0f - nv, 0.0f, 1.0f ), 10.8649f ) * ( 1.0f - a ) + a;
     return clampf( f, 0.0f, 1.0f );
 }
 
@@ -387,7 +432,7 @@ __global__ __launch_bounds__( THREADS ) void k_temporal( FbPtrs P, Dims d, RgExt
         const v3    surfPos      = xyz( Ld4( P.cur[ RG_EXT_PLANE_SURFACE_POSITION ], W, H, x, y ) );
         const v3    surfPosPrev  = xyz( Ld4( P.prev[ RG_EXT_PLANE_SURFACE_POSITION ], W, H, ppx, ppy ) );
         const float nv           = fmaxf( 0.0f, dot( normal, -rayDir ) );
-        const float A            = clampf( 1.0f + specHistLen * powf( 1.0f - aSpec, 5.0f ), SpecMinAccumFrames( g ), 256.0f );
+        const float A            = clampf( 1.0f + specHistLen * Pow5( 1.0f - aSpec ), SpecMinAccumFrames( g ), 256.0f );
         const float parallax     = Parallax( g, surfPos, surfPosPrev );
         const float AOnSurf      = SpecAccumSpeed( g, A, roughness, nv, parallax );
         const v3    specOnSurf   = mix3( specPrev, unfSpec, 1.0f / ( 1.0f + AOnSurf ) );
@@ -462,7 +507,7 @@ __global__ __launch_bounds__( THREADS ) void k_temporal( FbPtrs P, Dims d, RgExt
         float dh  = hist.x * inv;
         float ih  = hist.y * inv;
 
-        dh *= powf( 1.0f - aDiff, 10.0f );
+        dh *= Pow10( 1.0f - aDiff );
         dh           = clampf( dh + 1.0f, 1.0f, 256.0f );
         float alphaC = fmaxf( 0.01f, 1.0f / dh );
         float alphaM = fmaxf( 0.01f, 1.0f / dh );
@@ -472,7 +517,7 @@ __global__ __launch_bounds__( THREADS ) void k_temporal( FbPtrs P, Dims d, RgExt
         diffMom      = make_float2( mixf( momPrev.x, l, alphaM ), mixf( momPrev.y, l * l, alphaM ) );
         diffHist     = dh;
 
-        ih *= powf( 1.0f - aIndir, 10.0f );
+        ih *= Pow10( 1.0f - aIndir );
         ih         = clampf( ih + 1.0f, 1.0f, 256.0f );
         float aI   = fmaxf( 0.01f, 1.0f / ih );
         aI         = mixf( aI, 1.0f, aIndir );
@@ -623,8 +668,9 @@ __global__ __launch_bounds__( THREADS ) void k_variance( FbPtrs P, Dims d )
             const v3    cq  = s_col[ ly + yy ][ lx + xx ];
             const float lq  = luminance( cq );
             const float dq  = s_dep[ ly + yy ][ lx + xx ];
-            const float w_z = expf( -fabsf( depth - dq ) * safePositiveRcp( fmaxf( depth, dq ) ) );
-            const float w_n = powf( fmaxf( 0.0f, dot( n, s_nrm[ ly + yy ][ lx + xx ] ) ), 128.0f );
+            const float dm  = fmaxf( depth, dq );
+            const float w_z = FastExp( -fabsf( depth - dq ) * ( dm <= 0.0f ? 0.0f : __builtin_amdgcn_rcpf( dm ) ) );
+            const float w_n = Pow128( fmaxf( 0.0f, dot( n, s_nrm[ ly + yy ][ lx + xx ] ) ) );
             const float w   = w_z * w_n;
             if( !isnan( w ) )
             {
@@ -744,6 +790,8 @@ __device__ void AtrousPixel( const FbPtrs& P, const Dims& d, int x, int y, int k
     float       nws    = clampf( RoughnessSquaredToSpecPower( c_rough * c_rough ), 8.0f, 1024.0f );
     nws *= clampf( hSpec / 8.0f, 0.0f, 1.0f );
 
+    // 1 / max(grad * (|dx| + |dy|), 0.01) of the edge and diagonal taps
+    const float zInv1 = 1.0f / fmaxf( c_grad, 0.01f ), zInv2 = 1.0f / fmaxf( c_grad * 2.0f, 0.01f );
     float ws = 1.0f, wss = 1.0f, wsi = 1.0f;
 #pragma unroll 1
     for( int yy = -1; yy <= 1; yy++ )
@@ -762,18 +810,18 @@ __device__ void AtrousPixel( const FbPtrs& P, const Dims& d, int x, int y, int k
             tap( xx, yy, qx, qy, nq, dq, rq, cq, sq, iq );
             const float lq  = luminance( xyz( cq ) );
             const float nn  = fmaxf( 0.0f, dot( c_n, nq ) );
-            const float w_z = fabsf( c_depth - dq ) / fmaxf( c_grad * float( abs( xx ) + abs( yy ) ), 0.01f );
-            const float w_n = powf( nn, 128.0f );
+            const float w_z = fabsf( c_depth - dq ) * ( xx != 0 && yy != 0 ? zInv2 : zInv1 );
+            const float w_n = Pow128( nn );
             const float w_l = fabsf( l - lq ) * wLum;
             float       w_r = fmaxf( 0.0f, 1.0f - 10.0f * fabsf( c_rough - rq ) ) * wRough;
             if( nws > 0.0f )
             {
-                w_r *= powf( nn, nws );
+                w_r *= FastPow( nn, nws );
             }
             const float kw     = ( xx == 0 ? 1.0f : 0.5f ) * ( yy == 0 ? 1.0f : 0.5f );
             const float inside = inRenderArea( qx, qy, alo, ahi, H ) ? 1.0f : 0.0f;
-            const float wBase  = expf( -w_z * w_z ) * w_n * kw * inside;
-            const float wD = wBase * expf( -w_l ), wS = wBase * w_r, wI = wBase;
+            const float wBase  = FastExp( -w_z * w_z ) * w_n * kw * inside;
+            const float wD = wBase * FastExp( -w_l ), wS = wBase * w_r, wI = wBase;
             outD = outD + xyz( cq ) * wD;
             outS = outS + sq * wS;
             outI = outI + iq * wI;

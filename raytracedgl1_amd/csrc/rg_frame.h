@@ -68,4 +68,20 @@ uint64_t EncodeFrame( FrameGpu& g, TraceContext& tc, const RgExtFrameParams& p, 
                       FramebufferSet& fb, const RgDrawFrameInfo& di, hipStream_t s, hipEvent_t evTraced = nullptr,
                       hipEvent_t evDenoised = nullptr );
 
+// One frame of a batch for EncodeFrames.
+struct EncodeSlot
+{
+    FrameGpu*               g;
+    const RgExtFrameParams* p;
+    FramebufferSet*         fb;
+    const RgDrawFrameInfo*  di;
+};
+
+// B frames of one scene: their traced passes share each stage's traversal
+// launch (TraceFrames over the wavefront wf, which must hold every frame's
+// pixels); A-SVGF, eye adaptation and compose then run frame by frame.  A
+// frame's result equals EncodeFrame's for it alone.
+uint64_t EncodeFrames( const EncodeSlot* slots, uint32_t B, TraceContext& tc, const BvhObject& bvh, const DeviceScene& scene, Wavefront& wf,
+                       hipStream_t s );
+
 } // namespace rg

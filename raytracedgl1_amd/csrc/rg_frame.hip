@@ -222,16 +222,12 @@ void FrameGpu::Init( hipStream_t s )
     pt.Init( s );
 }
 
-uint64_t EncodeFrame( FrameGpu& g, TraceContext& tc, const RgExtFrameParams& p, const BvhObject& bvh, const DeviceScene& scene,
-                      FramebufferSet& fb, const RgDrawFrameInfo& di, hipStream_t s, hipEvent_t evTraced, hipEvent_t evDenoised )
+namespace
+{
+// A-SVGF, eye adaptation and the final compose of one traced frame.
+void FinishFrame( FrameGpu& g, const RgExtFrameParams& p, FramebufferSet& fb, const RgDrawFrameInfo& di, hipStream_t s, hipEvent_t evDenoised )
 {
     const uint32_t w = p.renderWidth, h = p.renderHeight;
-    const uint32_t flags = p.rayCullBackFaces ? RG_EXT_TRACE_CULL_FRONT_FACE : 0u;
-    const uint64_t rays  = g.pt.Trace( p, bvh, tc, scene, fb, flags, s, nullptr );
-    if( evTraced )
-    {
-        RG_HIP_CHECK( hipEventRecord( evTraced, s ) );
-    }
     Denoise( fb, p, s );
     if( evDenoised )
     {
@@ -251,6 +247,37 @@ uint64_t EncodeFrame( FrameGpu& g, TraceContext& tc, const RgExtFrameParams& p, 
         std::memcpy( ct, tm->crosstalk.data, sizeof( ct ) );
     }
     ComposeFinal( fb, MakeLpmConsts( sat, ct ), p.emissionMaxScreenColor, g.avgLum.as< float >(), s );
+}
+} // namespace
+
+uint64_t EncodeFrame( FrameGpu& g, TraceContext& tc, const RgExtFrameParams& p, const BvhObject& bvh, const DeviceScene& scene,
+                      FramebufferSet& fb, const RgDrawFrameInfo& di, hipStream_t s, hipEvent_t evTraced, hipEvent_t evDenoised )
+{
+    const uint32_t flags = p.rayCullBackFaces ? RG_EXT_TRACE_CULL_FRONT_FACE : 0u;
+    const uint64_t rays  = g.pt.Trace( p, bvh, tc, scene, fb, flags, s, nullptr );
+    if( evTraced )
+    {
+        RG_HIP_CHECK( hipEventRecord( evTraced, s ) );
+    }
+    FinishFrame( g, p, fb, di, s, evDenoised );
+    return rays;
+}
+
+uint64_t EncodeFrames( const EncodeSlot* slots, uint32_t B, TraceContext& tc, const BvhObject& bvh, const DeviceScene& scene, Wavefront& wf,
+                       hipStream_t s )
+{
+    RG_REQUIRE( B >= 1 && B <= MAX_TRACE_SLOTS, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "frames per batch out of range" );
+    TraceSlot ts[ MAX_TRACE_SLOTS ];
+    for( uint32_t k = 0; k < B; k++ )
+    {
+        ts[ k ] = TraceSlot{ &slots[ k ].g->pt, slots[ k ].p, slots[ k ].fb };
+    }
+    const uint32_t flags = slots[ 0 ].p->rayCullBackFaces ? RG_EXT_TRACE_CULL_FRONT_FACE : 0u;
+    const uint64_t rays  = TraceFrames( ts, B, bvh, tc, scene, wf, flags, s );
+    for( uint32_t k = 0; k < B; k++ )
+    {
+        FinishFrame( *slots[ k ].g, *slots[ k ].p, *slots[ k ].fb, *slots[ k ].di, s, nullptr );
+    }
     return rays;
 }
 

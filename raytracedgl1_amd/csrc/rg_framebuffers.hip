@@ -52,6 +52,7 @@ const PlaneDesc g_planes[ RG_EXT_PLANE_COUNT ] = {
     { LAYOUT_REGULAR, 4, false },      // SCREEN_EMISSION
     { LAYOUT_REGULAR, 4, false },      // ACID_FOG
     { LAYOUT_CHECKERBOARD, 4, false }, // DENOISE_GUIDE
+    { LAYOUT_CHECKERBOARD, 16, false }, // GI_REUSE
 };
 } // namespace
 

@@ -26,5 +26,8 @@ LpmConsts MakeLpmConsts( const float saturation[ 3 ], const float crosstalk[ 3 ]
 // PRE_FINAL -> FINAL: checkerboard resolve, auto exposure from *d_avgLum, LPM tone map
 // emissionMaxScreenColor weights SCREEN_EMISSION into the tone map's input.
 void ComposeFinal( FramebufferSet& fb, const LpmConsts& k, float emissionMaxScreenColor, const float* d_avgLum, hipStream_t stream );
+// FINAL (pixels float RGBA) -> RGBA8, sRGB transfer, alpha 255: the frame a
+// server reads back.
+void PackFinalRgba8( const float* d_final, uint32_t* d_out, uint32_t pixels, hipStream_t stream );
 
 } // namespace rg

@@ -71,4 +71,27 @@ struct PathTracer
                     FramebufferSet& fb, uint32_t traceFlags, hipStream_t s, FrameTimings* t );
 };
 
+// One frame of a batch: its path tracer (light grid, reservoir history),
+// frame constants and framebuffers.
+struct TraceSlot
+{
+    PathTracer*             pt;
+    const RgExtFrameParams* p;
+    FramebufferSet*         fb;
+};
+constexpr uint32_t MAX_TRACE_SLOTS = 16;
+
+// Frames that can share traversal launches: the same ray cull masks,
+// reflection / refraction depth, face culling and light kinds (their scene
+// must be the same too).
+bool TraceCompatible( const RgExtFrameParams& a, const RgExtFrameParams& b );
+
+// The traced passes of B frames of one scene, stage by stage: each stage's
+// shading kernels run per frame, and each traversal covers every frame's
+// rays of that stage in one launch (frame k's rays are a slice of wf).
+// Every frame's result equals the frame traced alone.  The first slot's
+// path tracer times and counts the launches.  Returns the rays traced.
+uint64_t TraceFrames( const TraceSlot* slots, uint32_t B, const BvhObject& bvh, TraceContext& tc, const DeviceScene& scene, Wavefront& wf,
+                      uint32_t traceFlags, hipStream_t s );
+
 } // namespace rg

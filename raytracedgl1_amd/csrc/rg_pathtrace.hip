@@ -1587,6 +1587,23 @@ __global__ __launch_bounds__( THREADS ) void k_indirect_b( RgExtFrameParams p, F
 
 // Second-bounce hit and the final indirect sample; shading of the primary
 // surface with it (RtRaygenIndirect.inl:66-161, :402-433 without reuse).
+// The GI_REUSE record of pixel i: what the spatial reuse of other pixels
+// reads of it (its G-buffer surface and its initial sample, normals decoded
+// once here instead of at every reuse), 64 bytes in one cache line.
+__device__ __forceinline__ void StoreGiReuse( const FbPtrs& P, size_t i, uint4 sA, uint2 sB )
+{
+    const float4 x1  = Plane< float4 >( P, RG_EXT_PLANE_SURFACE_POSITION )[ i ];
+    const float  d   = Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD )[ i ];
+    const v3     n   = DecodeNormal( Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL )[ i ] );
+    const v3     sn  = DecodeNormal( sA.w );
+    auto         u   = []( float f ) { return __float_as_uint( f ); };
+    uint4*       rec = Plane< uint4 >( P, RG_EXT_PLANE_GI_REUSE ) + 4 * i;
+    rec[ 0 ]         = make_uint4( u( x1.x ), u( x1.y ), u( x1.z ), u( d ) );
+    rec[ 1 ]         = make_uint4( u( n.x ), u( n.y ), u( n.z ), Plane< uint32_t >( P, RG_EXT_PLANE_IS_SKY )[ i ] );
+    rec[ 2 ]         = make_uint4( sA.x, sA.y, sA.z, sB.y );
+    rec[ 3 ]         = make_uint4( u( sn.x ), u( sn.y ), u( sn.z ), sB.x );
+}
+
 template< bool TEX >
 __global__ __launch_bounds__( THREADS ) void k_indirect_c( RgExtFrameParams p, FbPtrs P, SceneView sc, LightView lv, const RgExtHit* shadowHits,
                                                            const RgExtRay* bounceRays, const RgExtHit* bounceHits, const RgExtHit* hits1, IndirPend pend )
@@ -1605,6 +1622,7 @@ __global__ __launch_bounds__( THREADS ) void k_indirect_c( RgExtFrameParams p, F
         Plane< float4 >( P, RG_EXT_PLANE_UNFILTERED_INDIR )[ i ]        = make_float4( 0.f, 0.f, 0.f, 0.f );
         Plane< uint4 >( P, RG_EXT_PLANE_INDIR_INITIAL_SAMPLE )[ i ]     = make_uint4( 0u, 0u, 0u, 0u );
         Plane< uint2 >( P, RG_EXT_PLANE_INDIR_INITIAL_RADIANCE )[ i ] = make_uint2( 0u, 0u );
+        StoreGiReuse( P, i, make_uint4( 0u, 0u, 0u, 0u ), make_uint2( 0u, 0u ) );
         return;
     }
     Surface s  = FetchSurface( P, W, x, y );
@@ -1674,9 +1692,11 @@ __global__ __launch_bounds__( THREADS ) void k_indirect_c( RgExtFrameParams p, F
         sampleRad = ( emis + diffuse ) * mk3( ha.x, ha.y, ha.z );
     }
     // restirIndirect_StoreInitialSample (ReservoirIndirect.h:160-177)
-    Plane< uint4 >( P, RG_EXT_PLANE_INDIR_INITIAL_SAMPLE )[ i ] =
-        make_uint4( __float_as_uint( samplePos.x ), __float_as_uint( samplePos.y ), __float_as_uint( samplePos.z ), EncodeNormal( sampleNrm ) );
-    Plane< uint2 >( P, RG_EXT_PLANE_INDIR_INITIAL_RADIANCE )[ i ] = make_uint2( EncodeE5B9G9R9( sampleRad ), __float_as_uint( b1.w ) );
+    const uint4 sA = make_uint4( __float_as_uint( samplePos.x ), __float_as_uint( samplePos.y ), __float_as_uint( samplePos.z ), EncodeNormal( sampleNrm ) );
+    const uint2 sB = make_uint2( EncodeE5B9G9R9( sampleRad ), __float_as_uint( b1.w ) );
+    Plane< uint4 >( P, RG_EXT_PLANE_INDIR_INITIAL_SAMPLE )[ i ]   = sA;
+    Plane< uint2 >( P, RG_EXT_PLANE_INDIR_INITIAL_RADIANCE )[ i ] = sB;
+    StoreGiReuse( P, i, sA, sB );
 }
 
 // ------------------------------------------------------------ ReSTIR GI
@@ -1746,10 +1766,9 @@ __device__ __forceinline__ void CombineIndirNewSurf( ReservoirIndir& c, const Re
 
 // loadInitialSampleAsReservoir (RtRaygenIndirect.inl:246-255):
 // updateReservoirIndirect (ReservoirIndirect.h:70-84) with rnd = 0.5
-__device__ __forceinline__ ReservoirIndir InitialAsReservoir( uint4 a, uint2 b )
+__device__ __forceinline__ ReservoirIndir InitialAsReservoir( v3 position, v3 normal, uint32_t radianceCode, float oneOverSourcePdf )
 {
-    SampleIndir    s{ mk3( __uint_as_float( a.x ), __uint_as_float( a.y ), __uint_as_float( a.z ) ), DecodeNormal( a.w ), DecodeE5B9G9R9( b.x ) };
-    const float    oneOverSourcePdf = __uint_as_float( b.y );
+    SampleIndir    s{ position, normal, DecodeE5B9G9R9( radianceCode ) };
     const float    tp               = luminance( s.radiance ); // targetPdfForIndirectSample
     ReservoirIndir r                = EmptyReservoirIndir();
     const float    w                = tp * oneOverSourcePdf;
@@ -1761,6 +1780,12 @@ __device__ __forceinline__ ReservoirIndir InitialAsReservoir( uint4 a, uint2 b )
         r.selTargetPdf = tp;
     }
     return r;
+}
+
+__device__ __forceinline__ ReservoirIndir InitialAsReservoir( uint4 a, uint2 b )
+{
+    return InitialAsReservoir( mk3( __uint_as_float( a.x ), __uint_as_float( a.y ), __uint_as_float( a.z ) ), DecodeNormal( a.w ), b.x,
+                               __uint_as_float( b.y ) );
 }
 
 __device__ __forceinline__ ReservoirIndir InitialAsReservoir( const FbPtrs& P, size_t i )
@@ -1893,21 +1918,19 @@ __global__ __launch_bounds__( THREADS ) void k_indirect_final( RgExtFrameParams 
             rnd8_4( seed, salt++, r4 );
             const int qx = x + int( ( r4[ 0 ] * 2.0f - 1.0f ) * radius );
             const int qy = y + int( ( r4[ 1 ] * 2.0f - 1.0f ) * radius );
-            // all of the neighbour's planes in one round trip, at the pixel
+            // the neighbour's GI_REUSE record (one cache line), at the pixel
             // clamped into the image; a pixel outside fails the reuse test
             // (its render-area check), so the clamped values are never used
-            const size_t   qi   = size_t( min( max( qy, 0 ), H - 1 ) ) * W + min( max( qx, 0 ), W - 1 );
-            const uint32_t skyO = Plane< uint32_t >( P, RG_EXT_PLANE_IS_SKY )[ qi ];
-            const float    dO   = Plane< float >( P, RG_EXT_PLANE_DEPTH_WORLD )[ qi ];
-            const uint32_t nOe  = Plane< uint32_t >( P, RG_EXT_PLANE_NORMAL )[ qi ];
-            const uint4    sA   = Plane< uint4 >( P, RG_EXT_PLANE_INDIR_INITIAL_SAMPLE )[ qi ];
-            const uint2    sB   = Plane< uint2 >( P, RG_EXT_PLANE_INDIR_INITIAL_RADIANCE )[ qi ];
-            const float4   x1q  = Plane< float4 >( P, RG_EXT_PLANE_SURFACE_POSITION )[ qi ];
-            if( skyO != 0u || !ReuseIndirOk( lo, hi, H, qx, qy, depthCur, dO, s.normal, DecodeNormal( nOe ) ) )
+            const size_t  qi  = size_t( min( max( qy, 0 ), H - 1 ) ) * W + min( max( qx, 0 ), W - 1 );
+            const uint4*  rec = Plane< uint4 >( P, RG_EXT_PLANE_GI_REUSE ) + 4 * qi;
+            const uint4   r0 = rec[ 0 ], r1 = rec[ 1 ], r2 = rec[ 2 ], r3 = rec[ 3 ];
+            auto          f  = []( uint32_t v ) { return __uint_as_float( v ); };
+            const float4  x1q = make_float4( f( r0.x ), f( r0.y ), f( r0.z ), 0.0f );
+            if( r1.w != 0u || !ReuseIndirOk( lo, hi, H, qx, qy, depthCur, f( r0.w ), s.normal, mk3( f( r1.x ), f( r1.y ), f( r1.z ) ) ) )
             {
                 continue;
             }
-            const ReservoirIndir rq = InitialAsReservoir( sA, sB );
+            const ReservoirIndir rq = InitialAsReservoir( mk3( f( r2.x ), f( r2.y ), f( r2.z ) ), mk3( f( r3.x ), f( r3.y ), f( r3.z ) ), r3.w, f( r2.w ) );
             float oneOverJacobian    = safePositiveRcp( GeometryFactorSafe( rq.sel.normal, rq.sel.position, s.position ) ) *
                                     GeometryFactorSafe( rq.sel.normal, rq.sel.position, mk3( x1q.x, x1q.y, x1q.z ) );
             oneOverJacobian          = clampf( oneOverJacobian, 0.0f, 1.0f ); // SHIPPING_HACK
@@ -2302,140 +2325,240 @@ void PathTracer::UploadLights( const SceneCollector& sc, hipStream_t s )
 uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, TraceContext& tc, const DeviceScene& scene,
                             FramebufferSet& fb, uint32_t traceFlags, hipStream_t s, FrameTimings* timing )
 {
-    Resize( p.renderWidth, p.renderHeight );
-    const uint32_t n = W * H;
-    const TexView  tex   = scene.tex;
+    ( void )timing;
+    TraceSlot one{ this, &p, &fb };
+    return TraceFrames( &one, 1, bvh, tc, scene, *wf, traceFlags, s );
+}
+
+bool TraceCompatible( const RgExtFrameParams& a, const RgExtFrameParams& b )
+{
+    return PrimaryCullMask( a ) == PrimaryCullMask( b ) && a.rayCullMaskWorld == b.rayCullMaskWorld &&
+           a.reflectRefractMaxDepth == b.reflectRefractMaxDepth && a.rayCullBackFaces == b.rayCullBackFaces &&
+           a.directionalLightExists == b.directionalLightExists && ( a.lightCount > 0 ) == ( b.lightCount > 0 );
+}
+
+uint64_t TraceFrames( const TraceSlot* slots, uint32_t B, const BvhObject& bvh, TraceContext& tc, const DeviceScene& scene, Wavefront& wf,
+                      uint32_t traceFlags, hipStream_t s )
+{
+    RG_REQUIRE( B >= 1 && B <= MAX_TRACE_SLOTS, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "frames per batch out of range" );
+    const RgExtFrameParams& p0 = *slots[ 0 ].p;
+    for( uint32_t k = 1; k < B; k++ )
+    {
+        RG_REQUIRE( TraceCompatible( p0, *slots[ k ].p ), RG_RESULT_WRONG_FUNCTION_ARGUMENT,
+                    "frames of one batch differ in their ray masks, reflection depth or lights" );
+    }
+    // Each frame's rays, hits and pending records are one slice of the
+    // wavefront: frame k's pixel i is entry off[k] + i, so a traversal
+    // launch covers every frame's rays of a stage at once.
+    size_t   off[ MAX_TRACE_SLOTS ], offG[ MAX_TRACE_SLOTS ];
+    uint32_t nPx[ MAX_TRACE_SLOTS ], gw[ MAX_TRACE_SLOTS ], gh[ MAX_TRACE_SLOTS ];
+    size_t   total = 0, totalG = 0;
+    for( uint32_t k = 0; k < B; k++ )
+    {
+        PathTracer& pt = *slots[ k ].pt;
+        pt.Resize( slots[ k ].p->renderWidth, slots[ k ].p->renderHeight );
+        nPx[ k ] = pt.W * pt.H;
+        gw[ k ]  = ( pt.W + 1 ) / 3 > 1 ? ( pt.W + 1 ) / 3 : 1;
+        gh[ k ]  = ( pt.H + 1 ) / 3 > 1 ? ( pt.H + 1 ) / 3 : 1;
+        off[ k ] = total;
+        offG[ k ] = totalG;
+        total += nPx[ k ];
+        totalG += size_t( gw[ k ] ) * gh[ k ];
+    }
+    RG_REQUIRE( total <= wf.capPx, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "the batch's pixels exceed the wavefront capacity" );
+    PathTracer&     lead  = *slots[ 0 ].pt; // times and counts the batch's traversal launches
+    const TexView   tex   = scene.tex;
     const AlphaRec* alpha = scene.alphaTested ? scene.alphaRecs.as< AlphaRec >() : nullptr;
-    const dim3     grid2( DivUp( W, BX ), DivUp( H, BY ) );
-    const FbPtrs   P = fb.Ptrs();
-    SceneView      sv{ scene.tris.as< float >(), scene.triNormals.as< float >(), scene.triGeom.as< uint32_t >(), scene.geoms.as< GeomRecord >(),
-                       scene.vertices.as< RgPrimitiveVertex >(), scene.indices.as< uint32_t >(), tex };
-    LightView      lv{ lights.as< LightEncoded >(), lightArraySize, prevToCur.as< uint32_t >(), prevToCurCount,
-                  grid[ gridParity ].as< uint4 >(), grid[ gridParity ^ 1 ].as< uint4 >(), blueNoise };
-    RgExtRay* R  = wf->rays.as< RgExtRay >();
-    RgExtHit* Hh = wf->hits.as< RgExtHit >();
-    RgExtRay* R2 = wf->rays2.as< RgExtRay >();
-    RgExtHit* H2 = wf->hits2.as< RgExtHit >();
-    DeviceBuffer& pendA = wf->pendA;
-    DeviceBuffer& pendB = wf->pendB;
-    DeviceBuffer& pendC = wf->pendC;
+    const SceneView sv{ scene.tris.as< float >(), scene.triNormals.as< float >(), scene.triGeom.as< uint32_t >(), scene.geoms.as< GeomRecord >(),
+                        scene.vertices.as< RgPrimitiveVertex >(), scene.indices.as< uint32_t >(), tex };
+    RgExtRay* R  = wf.rays.as< RgExtRay >();
+    RgExtHit* Hh = wf.hits.as< RgExtHit >();
+    RgExtRay* R2 = wf.rays2.as< RgExtRay >();
+    RgExtHit* H2 = wf.hits2.as< RgExtHit >();
+    float4*   PA = wf.pendA.as< float4 >();
+    RgExtRay* PB = wf.pendB.as< RgExtRay >();
+    RgExtHit* PC = wf.pendC.as< RgExtHit >();
+    uint8_t*  M  = wf.masks.as< uint8_t >();
+    uint8_t*  M2 = wf.masks2.as< uint8_t >();
+    uint8_t*  MB = wf.masksB.as< uint8_t >();
     uint64_t  traced = 0;
     const bool haveGeometry = scene.triCount > 0 && !bvh.frameSegs.empty();
 
-    traceLaunches = 0;
-    RG_HIP_CHECK( hipMemsetAsync( frameAccum.ptr, 0, 3 * sizeof( uint64_t ), s ) );
+    lead.traceLaunches = 0;
+    RG_HIP_CHECK( hipMemsetAsync( lead.frameAccum.ptr, 0, 3 * sizeof( uint64_t ), s ) );
     TraceAccum accum;
-    accum.stats  = frameAccum.as< uint64_t >();
-    accum.errors = reinterpret_cast< uint32_t* >( frameAccum.as< uint64_t >() + 2 );
+    accum.stats  = lead.frameAccum.as< uint64_t >();
+    accum.errors = reinterpret_cast< uint32_t* >( lead.frameAccum.as< uint64_t >() + 2 );
     // The frame's TLAS: each geometry class with its instance mask, or 0
     // when rayCullMaskWorld leaves its world out (ASManager.cpp:837-963).
     uint32_t classMasks[ GC_COUNT ];
     for( uint32_t c = 0; c < GC_COUNT; c++ )
     {
-        classMasks[ c ] = GeomClassPresent( c, p.rayCullMaskWorld ) ? GeomClassMask( c ) : 0u;
+        classMasks[ c ] = GeomClassPresent( c, p0.rayCullMaskWorld ) ? GeomClassMask( c ) : 0u;
     }
-    uint8_t* M  = wf->masks.as< uint8_t >();
-    uint8_t* M2 = wf->masks2.as< uint8_t >();
-    uint8_t* MB = wf->masksB.as< uint8_t >();
     // rr's rays carry their cull masks in mm (null: every ray has mask)
-    auto trace = [ & ]( RgExtRay* rr, const uint8_t* mm, uint32_t mask, RgExtHit* hh, uint32_t flags, uint32_t count ) {
+    auto trace = [ & ]( RgExtRay* rr, const uint8_t* mm, uint32_t mask, RgExtHit* hh, uint32_t flags, size_t count ) {
         if( haveGeometry )
         {
-            const uint32_t k = traceLaunches < TRACE_EVENT_PAIRS ? traceLaunches : TRACE_EVENT_PAIRS - 1;
-            TraceRaysFrame( bvh, tc, rr, mm, mask, count, hh, flags | traceFlags | ( collectStats ? RG_EXT_TRACE_COLLECT_STATS : 0u ), classMasks,
-                            GC_COUNT, s, traceEv[ 2 * k ], traceEv[ 2 * k + 1 ], &accum, alpha, &tex );
-            traceLaunches = k + 1;
+            const uint32_t k = lead.traceLaunches < PathTracer::TRACE_EVENT_PAIRS ? lead.traceLaunches : PathTracer::TRACE_EVENT_PAIRS - 1;
+            TraceRaysFrame( bvh, tc, rr, mm, mask, uint32_t( count ), hh,
+                            flags | traceFlags | ( lead.collectStats ? RG_EXT_TRACE_COLLECT_STATS : 0u ), classMasks, GC_COUNT, s,
+                            lead.traceEv[ 2 * k ], lead.traceEv[ 2 * k + 1 ], &accum, alpha, &tex );
+            lead.traceLaunches = k + 1;
         }
         else
         {
             // no geometry: every ray misses
-            RG_HIP_CHECK( hipMemsetAsync( hh, 0xFF, size_t( count ) * sizeof( RgExtHit ), s ) );
+            RG_HIP_CHECK( hipMemsetAsync( hh, 0xFF, count * sizeof( RgExtHit ), s ) );
         }
         traced += count;
     };
+    struct Frame
+    {
+        const RgExtFrameParams* p;
+        PathTracer*             pt;
+        FbPtrs                  P;
+        LightView               lv;
+        dim3                    grid2;
+        size_t                  o, n;
+    };
+    Frame f[ MAX_TRACE_SLOTS ];
+    for( uint32_t k = 0; k < B; k++ )
+    {
+        PathTracer& pt = *slots[ k ].pt;
+        f[ k ].p     = slots[ k ].p;
+        f[ k ].pt    = &pt;
+        f[ k ].P     = slots[ k ].fb->Ptrs();
+        f[ k ].lv    = LightView{ pt.lights.as< LightEncoded >(), pt.lightArraySize, pt.prevToCur.as< uint32_t >(), pt.prevToCurCount,
+                                  pt.grid[ pt.gridParity ].as< uint4 >(), pt.grid[ pt.gridParity ^ 1 ].as< uint4 >(), pt.blueNoise };
+        f[ k ].grid2 = dim3( DivUp( pt.W, BX ), DivUp( pt.H, BY ) );
+        f[ k ].o     = off[ k ];
+        f[ k ].n     = nPx[ k ];
+    }
+    auto each = [ & ]( auto&& fn ) {
+        for( uint32_t k = 0; k < B; k++ )
+        {
+            fn( f[ k ] );
+        }
+    };
 
     // primary
-    k_primary_rays<<< grid2, THREADS, 0, s >>>( p, R );
-    CheckLaunch( "k_primary_rays" );
-    trace( R, nullptr, PrimaryCullMask( p ), Hh, RG_EXT_TRACE_CLOSEST, n );
-    ( tex.count > 0 ? k_primary_shade< true > : k_primary_shade< false > )<<< grid2, THREADS, 0, s >>>( p, Hh, sv, P );
-    CheckLaunch( "k_primary_shade" );
+    each( [ & ]( Frame& q ) {
+        k_primary_rays<<< q.grid2, THREADS, 0, s >>>( *q.p, R + q.o );
+        CheckLaunch( "k_primary_rays" );
+    } );
+    trace( R, nullptr, PrimaryCullMask( p0 ), Hh, RG_EXT_TRACE_CLOSEST, total );
+    each( [ & ]( Frame& q ) {
+        ( tex.count > 0 ? k_primary_shade< true > : k_primary_shade< false > )<<< q.grid2, THREADS, 0, s >>>( *q.p, Hh + q.o, sv, q.P );
+        CheckLaunch( "k_primary_shade" );
+    } );
 
     // reflections and refractions replace the G-buffer surface; without
     // such geometry the reference's loop ends at depth 0 with no writes
-    if( p.reflectRefractMaxDepth > 0 && haveGeometry && scene.reflRefrGeometry )
+    if( p0.reflectRefractMaxDepth > 0 && haveGeometry && scene.reflRefrGeometry )
     {
-        RrState st{ pendA.as< float4 >(), n };
-        for( uint32_t d = 0; d < p.reflectRefractMaxDepth && d < 16u; d++ )
+        for( uint32_t d = 0; d < p0.reflectRefractMaxDepth && d < 16u; d++ )
         {
-            ( tex.count > 0 ? k_reflrefr_a< true > : k_reflrefr_a< false > )<<< grid2, THREADS, 0, s >>>( p, P, sv, R, M, Hh, st, d );
-            CheckLaunch( "k_reflrefr_a" );
-            trace( R, M, 0u, Hh, RG_EXT_TRACE_CLOSEST, n );
-            ( tex.count > 0 ? k_reflrefr_b< true > : k_reflrefr_b< false > )<<< grid2, THREADS, 0, s >>>( p, P, sv, R, Hh, st, d );
-            CheckLaunch( "k_reflrefr_b" );
+            each( [ & ]( Frame& q ) {
+                RrState st{ PA + 6 * q.o, q.n };
+                ( tex.count > 0 ? k_reflrefr_a< true > : k_reflrefr_a< false > )<<< q.grid2, THREADS, 0, s >>>( *q.p, q.P, sv, R + q.o, M + q.o,
+                                                                                                                Hh + q.o, st, d );
+                CheckLaunch( "k_reflrefr_a" );
+            } );
+            trace( R, M, 0u, Hh, RG_EXT_TRACE_CLOSEST, total );
+            each( [ & ]( Frame& q ) {
+                RrState st{ PA + 6 * q.o, q.n };
+                ( tex.count > 0 ? k_reflrefr_b< true > : k_reflrefr_b< false > )<<< q.grid2, THREADS, 0, s >>>( *q.p, q.P, sv, R + q.o, Hh + q.o,
+                                                                                                                st, d );
+                CheckLaunch( "k_reflrefr_b" );
+            } );
         }
     }
 
     // light grid (current grid = gridParity, previous = the other one)
-    if( p.lightCount > 0 )
-    {
-        lv.grid     = grid[ gridParity ^ 1 ].as< uint4 >();
-        lv.gridPrev = grid[ gridParity ].as< uint4 >();
-        k_light_grid<<< DivUp( GRID_ENTRIES, THREADS ), THREADS, 0, s >>>( p, lv, grid[ gridParity ^ 1 ].as< uint4 >() );
-        CheckLaunch( "k_light_grid" );
-        gridParity ^= 1;
-        lv.grid     = grid[ gridParity ].as< uint4 >();
-        lv.gridPrev = grid[ gridParity ^ 1 ].as< uint4 >();
-    }
+    each( [ & ]( Frame& q ) {
+        PathTracer& pt = *q.pt;
+        if( q.p->lightCount > 0 )
+        {
+            q.lv.grid     = pt.grid[ pt.gridParity ^ 1 ].as< uint4 >();
+            q.lv.gridPrev = pt.grid[ pt.gridParity ].as< uint4 >();
+            k_light_grid<<< DivUp( GRID_ENTRIES, THREADS ), THREADS, 0, s >>>( *q.p, q.lv, pt.grid[ pt.gridParity ^ 1 ].as< uint4 >() );
+            CheckLaunch( "k_light_grid" );
+            pt.gridParity ^= 1;
+            q.lv.grid     = pt.grid[ pt.gridParity ].as< uint4 >();
+            q.lv.gridPrev = pt.grid[ pt.gridParity ^ 1 ].as< uint4 >();
+        }
+    } );
 
     // initial reservoirs (directional visibility)
-    k_initial_a<<< grid2, THREADS, 0, s >>>( p, P, lv, R, M, pendA.as< uint4 >() );
-    CheckLaunch( "k_initial_a" );
-    if( p.directionalLightExists )
+    each( [ & ]( Frame& q ) {
+        k_initial_a<<< q.grid2, THREADS, 0, s >>>( *q.p, q.P, q.lv, R + q.o, M + q.o, reinterpret_cast< uint4* >( PA + 6 * q.o ) );
+        CheckLaunch( "k_initial_a" );
+    } );
+    if( p0.directionalLightExists )
     {
         // without a directional light every ray is null and k_initial_b reads no hit
-        trace( R, M, 0u, Hh, RG_EXT_TRACE_ANY_HIT, n );
+        trace( R, M, 0u, Hh, RG_EXT_TRACE_ANY_HIT, total );
     }
-    k_initial_b<<< grid2, THREADS, 0, s >>>( p, P, Hh, pendA.as< uint4 >() );
-    CheckLaunch( "k_initial_b" );
+    each( [ & ]( Frame& q ) {
+        k_initial_b<<< q.grid2, THREADS, 0, s >>>( *q.p, q.P, Hh + q.o, reinterpret_cast< uint4* >( PA + 6 * q.o ) );
+        CheckLaunch( "k_initial_b" );
+    } );
 
     // direct illumination
-    float4* pd = pendA.as< float4 >();
-    float4* ps = pendA.as< float4 >() + n;
-    k_direct_a<<< grid2, THREADS, 0, s >>>( p, P, lv, R, M, pd, ps );
-    CheckLaunch( "k_direct_a" );
-    trace( R, M, 0u, Hh, RG_EXT_TRACE_ANY_HIT, n );
-    k_direct_b<<< grid2, THREADS, 0, s >>>( p, P, R, Hh, pd, ps );
-    CheckLaunch( "k_direct_b" );
+    each( [ & ]( Frame& q ) {
+        k_direct_a<<< q.grid2, THREADS, 0, s >>>( *q.p, q.P, q.lv, R + q.o, M + q.o, PA + 6 * q.o, PA + 6 * q.o + q.n );
+        CheckLaunch( "k_direct_a" );
+    } );
+    trace( R, M, 0u, Hh, RG_EXT_TRACE_ANY_HIT, total );
+    each( [ & ]( Frame& q ) {
+        k_direct_b<<< q.grid2, THREADS, 0, s >>>( *q.p, q.P, R + q.o, Hh + q.o, PA + 6 * q.o, PA + 6 * q.o + q.n );
+        CheckLaunch( "k_direct_b" );
+    } );
 
     // indirect: bounce 1, then the shadow ray at its hit and bounce 2
-    IndirPend ip{ pendA.as< float4 >(), n };
-    RgExtRay* shadowR = pendB.as< RgExtRay >();
-    RgExtHit* shadowH = pendC.as< RgExtHit >();
-    k_indirect_a<<< grid2, THREADS, 0, s >>>( p, P, lv, R, M, ip );
-    CheckLaunch( "k_indirect_a" );
-    trace( R, M, 0u, Hh, RG_EXT_TRACE_CLOSEST, n );
-    ( tex.count > 0 ? k_indirect_b< true > : k_indirect_b< false > )<<< grid2, THREADS, 0, s >>>( p, P, sv, lv, R, Hh, shadowR, MB, R2, M2, ip );
-    CheckLaunch( "k_indirect_b" );
-    trace( shadowR, MB, 0u, shadowH, RG_EXT_TRACE_ANY_HIT, n );
-    trace( R2, M2, 0u, H2, RG_EXT_TRACE_CLOSEST, n );
-    ( tex.count > 0 ? k_indirect_c< true > : k_indirect_c< false > )<<< grid2, THREADS, 0, s >>>( p, P, sv, lv, shadowH, R2, H2, Hh, ip );
-    CheckLaunch( "k_indirect_c" );
-    k_indirect_final<<< grid2, THREADS, 0, s >>>( p, P );
-    CheckLaunch( "k_indirect_final" );
+    each( [ & ]( Frame& q ) {
+        k_indirect_a<<< q.grid2, THREADS, 0, s >>>( *q.p, q.P, q.lv, R + q.o, M + q.o, IndirPend{ PA + 6 * q.o, q.n } );
+        CheckLaunch( "k_indirect_a" );
+    } );
+    trace( R, M, 0u, Hh, RG_EXT_TRACE_CLOSEST, total );
+    each( [ & ]( Frame& q ) {
+        ( tex.count > 0 ? k_indirect_b< true > : k_indirect_b< false > )<<< q.grid2, THREADS, 0, s >>>(
+            *q.p, q.P, sv, q.lv, R + q.o, Hh + q.o, PB + q.o, MB + q.o, R2 + q.o, M2 + q.o, IndirPend{ PA + 6 * q.o, q.n } );
+        CheckLaunch( "k_indirect_b" );
+    } );
+    trace( PB, MB, 0u, PC, RG_EXT_TRACE_ANY_HIT, total );
+    trace( R2, M2, 0u, H2, RG_EXT_TRACE_CLOSEST, total );
+    each( [ & ]( Frame& q ) {
+        ( tex.count > 0 ? k_indirect_c< true > : k_indirect_c< false > )<<< q.grid2, THREADS, 0, s >>>(
+            *q.p, q.P, sv, q.lv, PC + q.o, R2 + q.o, H2 + q.o, Hh + q.o, IndirPend{ PA + 6 * q.o, q.n } );
+        CheckLaunch( "k_indirect_c" );
+        k_indirect_final<<< q.grid2, THREADS, 0, s >>>( *q.p, q.P );
+        CheckLaunch( "k_indirect_final" );
+    } );
 
-    // gradient samples (CalculateGradientsSamples, PathTracer.cpp:196-219)
+    // gradient samples (CalculateGradientsSamples, PathTracer.cpp:196-219):
+    // frame k's strata are entries offG[k] .. of the ray and hit arrays
+    for( uint32_t k = 0; k < B; k++ )
     {
-        const uint32_t GW = ( W + 1 ) / 3 > 1 ? ( W + 1 ) / 3 : 1, GH = ( H + 1 ) / 3 > 1 ? ( H + 1 ) / 3 : 1;
-        const dim3     gridG( DivUp( GW, BX ), DivUp( GH, BY ) );
-        GradPend       gp{ pendA.as< float4 >(), reinterpret_cast< uint4* >( pendA.as< float4 >() + size_t( GW ) * GH ) };
-        k_grad_a<<< gridG, THREADS, 0, s >>>( p, P, sv, lv, geomMap.as< uint32_t >(), geomMapCount, scene.geomCount, R, M, gp );
+        Frame&     q = f[ k ];
+        const dim3 gridG( DivUp( gw[ k ], BX ), DivUp( gh[ k ], BY ) );
+        float4*    lum = PA + 6 * q.o;
+        GradPend   gp{ lum, reinterpret_cast< uint4* >( lum + size_t( gw[ k ] ) * gh[ k ] ) };
+        k_grad_a<<< gridG, THREADS, 0, s >>>( *q.p, q.P, sv, q.lv, q.pt->geomMap.as< uint32_t >(), q.pt->geomMapCount, scene.geomCount,
+                                             R + offG[ k ], M + offG[ k ], gp );
         CheckLaunch( "k_grad_a" );
-        trace( R, M, 0u, Hh, RG_EXT_TRACE_ANY_HIT, GW * GH );
-        k_grad_b<<< gridG, THREADS, 0, s >>>( p, P, Hh, gp );
+    }
+    trace( R, M, 0u, Hh, RG_EXT_TRACE_ANY_HIT, totalG );
+    for( uint32_t k = 0; k < B; k++ )
+    {
+        Frame&     q = f[ k ];
+        const dim3 gridG( DivUp( gw[ k ], BX ), DivUp( gh[ k ], BY ) );
+        float4*    lum = PA + 6 * q.o;
+        GradPend   gp{ lum, reinterpret_cast< uint4* >( lum + size_t( gw[ k ] ) * gh[ k ] ) };
+        k_grad_b<<< gridG, THREADS, 0, s >>>( *q.p, q.P, Hh + offG[ k ], gp );
         CheckLaunch( "k_grad_b" );
     }
-    ( void )timing;
     return traced;
 }
 

@@ -5,12 +5,16 @@
 // A-SVGF history and moments, ReSTIR DI/GI reservoirs, the light grid and the
 // adapted luminance), kept resident in HBM; a frame is the part of
 // VulkanDevice::Render (Source/VulkanDevice.cpp:540-588) from the traced
-// passes to the final compose, as EncodeFrame runs it.  Within-frame planes
+// passes to the final compose, as EncodeFrames runs it.  Within-frame planes
 // and wavefront buffers belong to a stream, so a session costs only what it
-// carries from frame to frame.
+// carries from frame to frame.  One launch renders a batch of waiting
+// requests of sessions of one scene: every traversal stage of the batch is
+// one persistent launch over all its frames' rays.
 #include "rg_frame.h"
 #include "rg_internal.h"
+#include "rg_kernels.h"
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -45,6 +49,15 @@ struct FrameRequest
     bool                          hasTm = false, hasSky = false, hasIl = false, hasTx = false, hasPath = false, hasRr = false;
     uint64_t                      tag    = 0;
     double                        submit = 0.0;
+
+    // What frames of one batch must share: the ray cull masks and the
+    // reflection / refraction depth (FillFrameParams derives the rest of
+    // TraceCompatible's fields from the scene).
+    uint64_t BatchKey() const
+    {
+        const uint32_t depth = std::min( 16u, hasRr ? rr.maxReflectRefractDepth : 2u );
+        return ( uint64_t( di.rayCullMaskWorld ) << 32 ) | depth;
+    }
 
     void Capture( const RgDrawFrameInfo& in )
     {
@@ -133,6 +146,19 @@ struct RgExtSession_T
     bool                          dying    = false;
     uint32_t                      inflight = 0;
     size_t                        bytes    = 0;
+    void*                         image[ 2 ] = { nullptr, nullptr }; // pinned RGBA8 readback images, used in turn
+    uint32_t                      imageNext  = 0;
+
+    ~RgExtSession_T()
+    {
+        for( void* p : image )
+        {
+            if( p )
+            {
+                ( void )hipHostFree( p );
+            }
+        }
+    }
 };
 
 namespace rg
@@ -141,18 +167,26 @@ namespace
 {
 struct StreamCtx
 {
-    struct InFlight
+    struct Member
     {
         RgExtSession_T* s;
-        uint64_t     tag;
-        uint32_t     frameId;
-        double       submit, start;
-        hipEvent_t   ev;
+        uint64_t        tag;
+        uint32_t        frameId;
+        double          submit;
+        const void*     image;
     };
-    hipStream_t               stream = nullptr;
-    TransientPlanes           planes;
-    Wavefront                 wf;
-    TraceContext              tc;
+    // One launch: the frames of a batch, finished together.
+    struct InFlight
+    {
+        std::vector< Member > m;
+        double                start;
+        hipEvent_t            ev;
+    };
+    hipStream_t                    stream = nullptr;
+    std::vector< std::unique_ptr< TransientPlanes > > planes; // one set per frame of a batch
+    Wavefront                      wf;     // every frame of a batch at the largest render size
+    TraceContext                   tc;
+    DeviceBuffer                   packed; // RGBA8 of a batch's frames for the readback
     std::vector< hipEvent_t > events;
     uint32_t                  nextEvent = 0;
     std::deque< InFlight >    inflight;
@@ -176,6 +210,8 @@ struct RgExtServer_T
 {
     RgExtServerCreateInfo                       info{};
     uint32_t                                    nodeFormat = RG_EXT_BVH_NODE_FP32;
+    uint32_t                                    batchCapacity = 1; // within-frame plane sets per stream
+    uint32_t                                    activeStreams = 1; // streams that take an idle session's next launch
     uint32_t                                    storage    = RG_EXT_STORAGE_FP32;
     hipStream_t                                 setup      = nullptr; // scene builds and session set-up
     std::vector< std::unique_ptr< StreamCtx > > streams;
@@ -192,9 +228,10 @@ struct RgExtServer_T
     uint64_t                                    sessionBytes = 0, sharedBytes = 0, sceneBytes = 0;
     uint64_t                                    done = 0, dropped = 0, failed = 0;
 
-    void Result( RgExtSession_T* s, uint64_t tag, uint32_t status, uint32_t frameId, double submit, double start, double end )
+    void Result( RgExtSession_T* s, uint64_t tag, uint32_t status, uint32_t frameId, double submit, double start, double end,
+                 const void* image = nullptr, uint32_t batch = 0 )
     {
-        RgExtFrameResult r;
+        RgExtFrameResult r{};
         r.session    = reinterpret_cast< RgExtSession >( s );
         r.tag        = tag;
         r.status     = status;
@@ -202,8 +239,10 @@ struct RgExtServer_T
         r.submitTime = submit;
         r.startTime  = start;
         r.endTime    = end;
+        r.pImage     = image;
+        r.batchSize  = batch;
         results.push_back( r );
-        ( status == RG_EXT_FRAME_DONE ? done : status == RG_EXT_FRAME_DROPPED ? dropped : failed )++;
+        ( status == RG_EXT_FRAME_DONE ? done : status == RG_EXT_FRAME_FAILED ? failed : dropped )++;
     }
 
     bool Busy() const
@@ -235,41 +274,186 @@ struct RgExtServer_T
                 {
                     break;
                 }
-                Result( f.s, f.tag, q == hipSuccess ? RG_EXT_FRAME_DONE : RG_EXT_FRAME_FAILED, f.frameId, f.submit, f.start, NowSeconds() );
-                f.s->inflight--;
+                const double end = NowSeconds();
+                for( const auto& m : f.m )
+                {
+                    Result( m.s, m.tag, q == hipSuccess ? RG_EXT_FRAME_DONE : RG_EXT_FRAME_FAILED, m.frameId, m.submit, f.start, end, m.image,
+                            uint32_t( f.m.size() ) );
+                    m.s->inflight--;
+                }
                 st->inflight.pop_front();
             }
         }
     }
 
-    // Called with the lock held; encodes the frame with the lock released,
-    // so submissions, polls and statistics do not wait for the host-side
-    // encoding.  The session counts the frame as in flight meanwhile (a
-    // destroy waits for it) and the server stays busy (WaitIdle waits).
-    // Only this thread launches frames and touches the session's camera,
-    // framebuffers and the stream's state.
-    void Launch( RgExtSession_T& s, StreamCtx& st, std::unique_lock< std::mutex >& lk )
+    // Drops waiting requests older than maxQueueSeconds.
+    void Expire()
     {
-        FrameRequest req = std::move( *s.pending );
-        s.pending.reset();
-        req.Relink();
-        s.inflight++;
+        if( info.maxQueueSeconds <= 0.0 )
+        {
+            return;
+        }
+        const double now = NowSeconds();
+        for( auto it = ready.begin(); it != ready.end(); )
+        {
+            RgExtSession_T* s = *it;
+            if( now - s->pending->submit > info.maxQueueSeconds )
+            {
+                Result( s, s->pending->tag, RG_EXT_FRAME_EXPIRED, 0, s->pending->submit, 0.0, now );
+                s->pending.reset();
+                s->queued = false;
+                it        = ready.erase( it );
+            }
+            else
+            {
+                ++it;
+            }
+        }
+    }
+
+    // The stream a session's next frame goes to: its own while a frame of
+    // it is in flight (its frames must stay ordered), else the one with the
+    // fewest launches queued (its persistent planes are its own, the
+    // within-frame planes are taken from that stream).
+    uint32_t TargetStream( const RgExtSession_T& s ) const
+    {
+        uint32_t target = s.stream;
+        if( s.inflight == 0 )
+        {
+            if( target >= activeStreams )
+            {
+                target = 0;
+            }
+            for( uint32_t k = 0; k < activeStreams; k++ )
+            {
+                if( streams[ k ]->inflight.size() < streams[ target ]->inflight.size() )
+                {
+                    target = k;
+                }
+            }
+        }
+        return target;
+    }
+
+    // Called with the lock held: picks the next batch (the lead request by
+    // the schedule, then up to batchFrames - 1 more waiting requests of the
+    // same scene and batch key that may run on the lead's stream, oldest
+    // first) and launches it.  False when no stream has room for a lead.
+    bool LaunchNext( std::unique_lock< std::mutex >& lk )
+    {
+        auto lead = ready.end();
+        for( auto it = ready.begin(); it != ready.end(); ++it )
+        {
+            if( streams[ TargetStream( **it ) ]->inflight.size() >= info.framesInFlightPerStream )
+            {
+                continue;
+            }
+            if( lead == ready.end() ||
+                ( info.schedule == RG_EXT_SCHEDULE_NEWEST_FIRST ? ( *it )->pending->submit > ( *lead )->pending->submit
+                                                                 : ( *it )->pending->submit < ( *lead )->pending->submit ) )
+            {
+                lead = it;
+            }
+        }
+        if( lead == ready.end() )
+        {
+            return false;
+        }
+        RgExtSession_T*                s0     = *lead;
+        const uint32_t                 target = TargetStream( *s0 );
+        const uint64_t                 key    = s0->pending->BatchKey();
+        std::vector< RgExtSession_T* > batch{ s0 };
+        ready.erase( lead );
+        if( info.batchFrames > 1 )
+        {
+            std::vector< RgExtSession_T* > cand;
+            for( RgExtSession_T* s : ready )
+            {
+                if( s->scene == s0->scene && s->pending->BatchKey() == key && ( s->inflight == 0 || s->stream == target ) )
+                {
+                    cand.push_back( s );
+                }
+            }
+            std::stable_sort( cand.begin(), cand.end(),
+                              []( const RgExtSession_T* a, const RgExtSession_T* b ) { return a->pending->submit < b->pending->submit; } );
+            for( RgExtSession_T* s : cand )
+            {
+                if( batch.size() >= info.batchFrames )
+                {
+                    break;
+                }
+                batch.push_back( s );
+                ready.erase( std::find( ready.begin(), ready.end(), s ) );
+            }
+        }
+        LaunchBatch( batch, *streams[ target ], target, lk );
+        return true;
+    }
+
+    // Called with the lock held; encodes the batch with the lock released,
+    // so submissions, polls and statistics do not wait for the host-side
+    // encoding.  Each session counts its frame as in flight meanwhile (a
+    // destroy waits for it) and the server stays busy (WaitIdle waits).
+    // Only this thread launches frames and touches the sessions' cameras,
+    // framebuffers and the stream's state.
+    void LaunchBatch( const std::vector< RgExtSession_T* >& batch, StreamCtx& st, uint32_t target, std::unique_lock< std::mutex >& lk )
+    {
+        const uint32_t              B = uint32_t( batch.size() );
+        std::vector< FrameRequest > req( B );
+        for( uint32_t k = 0; k < B; k++ )
+        {
+            RgExtSession_T& s = *batch[ k ];
+            req[ k ]          = std::move( *s.pending );
+            s.pending.reset();
+            s.queued = false;
+            req[ k ].Relink();
+            s.inflight++;
+            s.stream = target;
+        }
         launching++;
         lk.unlock();
-        const double start   = NowSeconds();
-        hipEvent_t   ev      = nullptr;
-        bool         ok      = false;
-        std::string  failure;
+        const double                     start = NowSeconds();
+        hipEvent_t                       ev    = nullptr;
+        bool                             ok    = false;
+        std::string                      failure;
+        std::vector< StreamCtx::Member > members;
         try
         {
-            RgExtScene_T& sc = *s.scene;
-            FillFrameParams( s.cam, req.di, s.width, s.height, uint32_t( sc.sc.lights.size() ), uint32_t( sc.sc.lights.size() ),
-                             sc.sc.hasDirectional, false );
-            if( !s.cam.firstFrame )
+            RgExtScene_T& sc = *batch[ 0 ]->scene;
+            EncodeSlot    slots[ MAX_TRACE_SLOTS ];
+            for( uint32_t k = 0; k < B; k++ )
             {
-                s.fbs.fb.Swap();
+                RgExtSession_T& s = *batch[ k ];
+                FillFrameParams( s.cam, req[ k ].di, s.width, s.height, uint32_t( sc.sc.lights.size() ), uint32_t( sc.sc.lights.size() ),
+                                 sc.sc.hasDirectional, false );
+                if( !s.cam.firstFrame )
+                {
+                    s.fbs.fb.Swap();
+                }
+                s.fbs.fb.shared = st.planes[ k ].get();
+                slots[ k ]      = EncodeSlot{ &s.g, &s.cam.params, &s.fbs.fb, &req[ k ].di };
             }
-            EncodeFrame( s.g, st.tc, s.cam.params, sc.bvh, sc.ds, s.fbs.fb, req.di, st.stream );
+            EncodeFrames( slots, B, st.tc, sc.bvh, sc.ds, st.wf, st.stream );
+            size_t packOff = 0;
+            for( uint32_t k = 0; k < B; k++ )
+            {
+                RgExtSession_T& s     = *batch[ k ];
+                const void*     image = nullptr;
+                if( info.flags & RG_EXT_SERVER_READBACK )
+                {
+                    // FINAL -> RGBA8 in the stream's staging, then to the
+                    // session's pinned image of this frame
+                    const size_t n = size_t( s.width ) * s.height;
+                    uint32_t*    d = st.packed.as< uint32_t >() + packOff;
+                    PackFinalRgba8( reinterpret_cast< const float* >( s.fbs.fb.Cur( RG_EXT_PLANE_FINAL ) ), d, uint32_t( n ), st.stream );
+                    void* h = s.image[ s.imageNext ];
+                    s.imageNext ^= 1u;
+                    RG_HIP_CHECK( hipMemcpyAsync( h, d, n * 4, hipMemcpyDeviceToHost, st.stream ) );
+                    image = h;
+                    packOff += n;
+                }
+                members.push_back( { &s, req[ k ].tag, s.cam.frameId, req[ k ].submit, image } );
+            }
             ev = st.events[ st.nextEvent++ % st.events.size() ];
             RG_HIP_CHECK( hipEventRecord( ev, st.stream ) );
             ok = true;
@@ -282,15 +466,21 @@ struct RgExtServer_T
         launching--;
         if( ok )
         {
-            st.inflight.push_back( { &s, req.tag, s.cam.frameId, req.submit, start, ev } );
-            s.cam.frameId++;
-            s.cam.firstFrame = false;
+            st.inflight.push_back( { std::move( members ), start, ev } );
+            for( RgExtSession_T* s : batch )
+            {
+                s->cam.frameId++;
+                s->cam.firstFrame = false;
+            }
         }
         else
         {
-            s.inflight--;
             SetLastError( failure.c_str() );
-            Result( &s, req.tag, RG_EXT_FRAME_FAILED, s.cam.frameId, req.submit, start, NowSeconds() );
+            for( uint32_t k = 0; k < B; k++ )
+            {
+                batch[ k ]->inflight--;
+                Result( batch[ k ], req[ k ].tag, RG_EXT_FRAME_FAILED, batch[ k ]->cam.frameId, req[ k ].submit, start, NowSeconds() );
+            }
         }
     }
 
@@ -300,45 +490,12 @@ struct RgExtServer_T
         for( ;; )
         {
             Retire();
+            Expire();
             bool launched = false;
-            for( auto it = ready.begin(); it != ready.end(); )
+            while( !ready.empty() && LaunchNext( lk ) )
             {
-                RgExtSession_T* s = *it;
-                // A session with a frame in flight stays on its stream (its
-                // frames must stay ordered); an idle one moves to the stream
-                // with the fewest frames queued: its persistent planes are its
-                // own, the within-frame planes are taken from that stream.
-                uint32_t target = s->stream;
-                if( s->inflight == 0 )
-                {
-                    for( uint32_t k = 0; k < streams.size(); k++ )
-                    {
-                        if( streams[ k ]->inflight.size() < streams[ target ]->inflight.size() )
-                        {
-                            target = k;
-                        }
-                    }
-                }
-                StreamCtx& st = *streams[ target ];
-                if( st.inflight.size() < info.framesInFlightPerStream )
-                {
-                    if( target != s->stream )
-                    {
-                        s->stream         = target;
-                        s->fbs.fb.shared  = &st.planes;
-                        s->g.pt.wf        = &st.wf;
-                    }
-                    ready.erase( it );
-                    s->queued = false;
-                    Launch( *s, st, lk );
-                    launched = true;
-                    // the ready list may have changed while the lock was released
-                    it = ready.begin();
-                }
-                else
-                {
-                    ++it;
-                }
+                launched = true;
+                Retire();
             }
             cvIdle.notify_all();
             if( stop && !Busy() )
@@ -355,6 +512,10 @@ struct RgExtServer_T
                 if( inflight )
                 {
                     cvWork.wait_for( lk, std::chrono::microseconds( 50 ) );
+                }
+                else if( info.maxQueueSeconds > 0.0 && !ready.empty() )
+                {
+                    cvWork.wait_for( lk, std::chrono::milliseconds( 1 ) );
                 }
                 else
                 {
@@ -409,12 +570,19 @@ RgResult rgExtServerCreate( const RgExtServerCreateInfo* pInfo, RgExtServer* pOu
         RgExtServerCreateInfo ci = *pInfo;
         ci.streamCount             = ci.streamCount ? ci.streamCount : 4u;
         ci.framesInFlightPerStream = ci.framesInFlightPerStream ? ci.framesInFlightPerStream : 2u;
+        ci.batchFrames             = ci.batchFrames ? ci.batchFrames : 1u;
         RG_REQUIRE( ci.streamCount <= 16 && ci.framesInFlightPerStream <= 8, RG_RESULT_WRONG_FUNCTION_ARGUMENT,
                     "streamCount must be <= 16 and framesInFlightPerStream <= 8" );
+        RG_REQUIRE( ci.batchFrames <= MAX_TRACE_SLOTS, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "batchFrames must be <= 16" );
+        RG_REQUIRE( ci.schedule <= RG_EXT_SCHEDULE_NEWEST_FIRST, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "unknown schedule" );
+        RG_REQUIRE( ( ci.flags & ~RG_EXT_SERVER_READBACK ) == 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "unknown server flags" );
+        RG_REQUIRE( ci.maxQueueSeconds >= 0.0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "maxQueueSeconds must not be negative" );
         RG_REQUIRE( ( ci.precisionFlags & ~( RG_EXT_PRECISION_FP16_STORAGE | RG_EXT_PRECISION_FP16_NODES ) ) == 0,
                     RG_RESULT_WRONG_FUNCTION_ARGUMENT, "unknown precisionFlags bits" );
         auto srv        = std::make_unique< RgExtServer_T >();
         srv->info       = ci;
+        srv->batchCapacity = ci.batchFrames;
+        srv->activeStreams = ci.streamCount;
         srv->storage    = ( ci.precisionFlags & RG_EXT_PRECISION_FP16_STORAGE ) ? RG_EXT_STORAGE_FP16 : RG_EXT_STORAGE_FP32;
         srv->nodeFormat = ( ci.precisionFlags & RG_EXT_PRECISION_FP16_NODES ) ? RG_EXT_BVH_NODE_FP16 : RG_EXT_BVH_NODE_FP32;
         RG_HIP_CHECK( hipStreamCreateWithFlags( &srv->setup, hipStreamNonBlocking ) );
@@ -423,16 +591,25 @@ RgResult rgExtServerCreate( const RgExtServerCreateInfo* pInfo, RgExtServer* pOu
         {
             auto st = std::make_unique< StreamCtx >();
             RG_HIP_CHECK( hipStreamCreateWithFlags( &st->stream, hipStreamNonBlocking ) );
-            st->planes.Create( ci.maxRenderWidth, ci.maxRenderHeight, srv->storage );
-            st->wf.Reserve( capPx );
+            for( uint32_t k = 0; k < ci.batchFrames; k++ )
+            {
+                st->planes.push_back( std::make_unique< TransientPlanes >() );
+                st->planes.back()->Create( ci.maxRenderWidth, ci.maxRenderHeight, srv->storage );
+            }
+            st->wf.Reserve( capPx * ci.batchFrames );
+            if( ci.flags & RG_EXT_SERVER_READBACK )
+            {
+                st->packed.Reserve( capPx * ci.batchFrames * 4 );
+            }
             st->events.resize( ci.framesInFlightPerStream );
             for( auto& e : st->events )
             {
                 RG_HIP_CHECK( hipEventCreateWithFlags( &e, hipEventDisableTiming ) );
             }
-            srv->sharedBytes += FramebufferSet::BytesFor( ci.maxRenderWidth, ci.maxRenderHeight, srv->storage, false ) -
-                                FramebufferSet::BytesFor( ci.maxRenderWidth, ci.maxRenderHeight, srv->storage, true ) +
-                                Wavefront::BytesFor( capPx );
+            srv->sharedBytes += ( FramebufferSet::BytesFor( ci.maxRenderWidth, ci.maxRenderHeight, srv->storage, false ) -
+                                  FramebufferSet::BytesFor( ci.maxRenderWidth, ci.maxRenderHeight, srv->storage, true ) + Wavefront::BytesFor( capPx ) ) *
+                                    ci.batchFrames +
+                                st->packed.bytes;
             srv->streams.push_back( std::move( st ) );
         }
         RgExtServer_T* raw = srv.get();
@@ -587,8 +764,15 @@ RgResult rgExtSessionCreate( RgExtScene scene, uint32_t width, uint32_t height, 
             ss->stream = streamIdx;
             ss->bytes  = bytes;
             StreamCtx& st = *s.streams[ streamIdx ];
-            ss->fbs.fb.Create( width, height, s.storage, &st.planes );
+            ss->fbs.fb.Create( width, height, s.storage, st.planes[ 0 ].get() );
             ss->g.pt.wf = &st.wf;
+            if( s.info.flags & RG_EXT_SERVER_READBACK )
+            {
+                for( void*& img : ss->image )
+                {
+                    RG_HIP_CHECK( hipHostMalloc( &img, size_t( width ) * height * 4, hipHostMallocDefault ) );
+                }
+            }
             ss->g.Init( s.setup );
             ss->g.pt.UploadLights( scene->sc, s.setup );
             ss->g.pt.UploadGeomMap( scene->geomMap, s.setup );
@@ -708,6 +892,29 @@ RgResult rgExtServerGetStats( RgExtServer server, RgExtServerStats* pOut )
         pOut->framesDropped = s.dropped;
         pOut->framesFailed  = s.failed;
         pOut->now           = NowSeconds();
+    } );
+}
+
+RgResult rgExtServerSetPolicy( RgExtServer server, const RgExtServerPolicy* pPolicy )
+{
+    return Guard( [ & ] {
+        RgExtServer_T& s = Srv( server );
+        RG_REQUIRE( pPolicy, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "pPolicy is null" );
+        const RgExtServerPolicy& p = *pPolicy;
+        RG_REQUIRE( p.schedule <= RG_EXT_SCHEDULE_NEWEST_FIRST, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "unknown schedule" );
+        RG_REQUIRE( p.batchFrames >= 1 && p.batchFrames <= s.batchCapacity, RG_RESULT_WRONG_FUNCTION_ARGUMENT,
+                    "batchFrames must be 1 .. the server's batchFrames" );
+        RG_REQUIRE( p.framesInFlightPerStream >= 1 && p.framesInFlightPerStream <= 8, RG_RESULT_WRONG_FUNCTION_ARGUMENT,
+                    "framesInFlightPerStream must be 1..8" );
+        RG_REQUIRE( p.streams >= 1 && p.streams <= s.streams.size(), RG_RESULT_WRONG_FUNCTION_ARGUMENT, "streams must be 1 .. streamCount" );
+        RG_REQUIRE( p.maxQueueSeconds >= 0.0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "maxQueueSeconds must not be negative" );
+        std::lock_guard< std::mutex > lk( s.m );
+        s.info.schedule                = p.schedule;
+        s.info.batchFrames             = p.batchFrames;
+        s.info.framesInFlightPerStream = p.framesInFlightPerStream;
+        s.info.maxQueueSeconds         = p.maxQueueSeconds;
+        s.activeStreams                = p.streams;
+        s.cvWork.notify_one();
     } );
 }
 

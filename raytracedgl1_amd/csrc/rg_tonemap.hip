@@ -3,7 +3,8 @@
 // CmLuminanceHistogram.comp:41-86 builds a 256-bin histogram per 16x16
 // group in shared memory and adds it to the global one; here a persistent
 // grid of 256-thread blocks walks the image with a grid stride, so each block
-// folds many pixels into its LDS bins before one global atomic per bin.
+// folds many pixels into its LDS bins before one global atomic per bin, and
+// the lanes of a wave that share a bin add to it with one LDS atomic.
 // CmLuminanceAvg.comp:39-90 is a single 256-thread group.
 #include "rg_device_math.h"
 #include "rg_framebuffers.h"
@@ -40,10 +41,31 @@ __global__ __launch_bounds__( HIST_THREADS ) void k_lum_histogram( const float* 
     __shared__ uint32_t s_bins[ HIST_BINS ];
     s_bins[ threadIdx.x ] = 0;
     __syncthreads();
-    const float invRange = 1.0f / ( maxLog - minLog );
+    const float    invRange = 1.0f / ( maxLog - minLog );
+    const uint32_t lane     = threadIdx.x & 63u;
     for( uint32_t i = blockIdx.x * HIST_THREADS + threadIdx.x; i < count; i += gridDim.x * HIST_THREADS )
     {
-        atomicAdd( &s_bins[ ColorBin( lum[ i ], minLog, invRange ) ], 1u );
+        // Wave-ballot aggregation: the lanes of a wave holding the same bin
+        // add their count with one LDS atomic from their first lane (image
+        // regions of one luminance put a whole wave into a few bins).
+        const uint32_t bin     = ColorBin( lum[ i ], minLog, invRange );
+        bool           pending = true;
+        for( ;; )
+        {
+            const uint64_t act = __ballot( pending );
+            if( act == 0ull )
+            {
+                break;
+            }
+            const int      leader = __ffsll( ( long long )act ) - 1;
+            const uint32_t b      = __shfl( bin, leader, 64 );
+            const uint64_t same   = __ballot( pending && bin == b );
+            if( int( lane ) == leader )
+            {
+                atomicAdd( &s_bins[ b ], uint32_t( __popcll( same ) ) );
+            }
+            pending = pending && bin != b;
+        }
     }
     __syncthreads();
     const uint32_t c = s_bins[ threadIdx.x ];
@@ -252,6 +274,37 @@ void ComposeFinal( FramebufferSet& fb, const LpmConsts& k, float emissionMaxScre
     const dim3 grid( DivUp( fb.width, CBX ), DivUp( fb.height, CBY ) );
     k_compose<<< grid, CBX * CBY, 0, stream >>>( fb.Ptrs(), int( fb.width ), int( fb.height ), k, emissionMaxScreenColor, d_avgLum );
     CheckLaunch( "k_compose" );
+}
+
+namespace
+{
+// FINAL (float RGBA, regular layout) -> RGBA8 with the sRGB transfer of the
+// swapchain's *_SRGB formats (IEC 61966-2-1), R in the low byte, alpha 255.
+__global__ __launch_bounds__( 256 ) void k_pack_rgba8( const float4* src, uint32_t* dst, uint32_t n )
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if( i >= n )
+    {
+        return;
+    }
+    const float4 c   = src[ i ];
+    auto         enc = []( float v ) {
+        v             = fminf( fmaxf( v, 0.0f ), 1.0f );
+        const float e = v <= 0.0031308f ? 12.92f * v : 1.055f * powf( v, 1.0f / 2.4f ) - 0.055f;
+        return uint32_t( fminf( fmaxf( e, 0.0f ), 1.0f ) * 255.0f + 0.5f );
+    };
+    dst[ i ] = enc( c.x ) | ( enc( c.y ) << 8 ) | ( enc( c.z ) << 16 ) | ( 255u << 24 );
+}
+} // namespace
+
+void PackFinalRgba8( const float* d_final, uint32_t* d_out, uint32_t pixels, hipStream_t stream )
+{
+    if( pixels == 0 )
+    {
+        return;
+    }
+    k_pack_rgba8<<< DivUp( pixels, 256u ), 256, 0, stream >>>( reinterpret_cast< const float4* >( d_final ), d_out, pixels );
+    CheckLaunch( "k_pack_rgba8" );
 }
 
 void LuminanceHistogram( const float* d_lum, uint32_t count, float minLog, float maxLog, uint32_t* d_hist, hipStream_t stream )

@@ -146,33 +146,35 @@ __device__ __forceinline__ void Sort8( uint64_t k[ 8 ] )
 #undef CS
 }
 
-// Child boxes of an FP16 node as FP32 (rows lo x, y, z, hi x, y, z) plus
-// the child references: origin + float(offset), the sum its build rounded
-// outward against.  9 16-byte loads.
-__device__ __forceinline__ void LoadHalfNode( const void* nodes, uint32_t node, float box[ 6 ][ 8 ], uint32_t child[ 8 ] )
+// Child boxes of an FP16 node as FP32, the planes the ray enters through
+// (near[axis][child]) and leaves through (far), plus the child references:
+// origin + float(offset), the sum its build rounded outward against.  Row
+// q[1 + 2 * axis] holds the lo offsets, q[2 + 2 * axis] the hi ones; `sgn`
+// bit `axis` is set where the ray's direction is negative on that axis, so
+// hi is its entering side.  9 16-byte loads.
+__device__ __forceinline__ void LoadHalfNode( const void* nodes, uint32_t node, uint32_t sgn, float nearP[ 3 ][ 8 ], float farP[ 3 ][ 8 ],
+                                              uint32_t child[ 8 ] )
 {
     typedef _Float16 half8 __attribute__( ( ext_vector_type( 8 ) ) );
     const float4* np = reinterpret_cast< const float4* >( reinterpret_cast< const BvhNode8H* >( nodes ) + node );
-    float4        q[ 9 ];
+    const float4  q0 = np[ 0 ];
+    const float   o[ 3 ] = { q0.x, q0.y, q0.z };
 #pragma unroll
-    for( int i = 0; i < 9; i++ )
+    for( int ax = 0; ax < 3; ax++ )
     {
-        q[ i ] = np[ i ];
-    }
-    const float o[ 3 ] = { q[ 0 ].x, q[ 0 ].y, q[ 0 ].z };
-    // q[1..6]: lo_x hi_x lo_y hi_y lo_z hi_z, 8 halves each
-    const int src[ 6 ] = { 1, 3, 5, 2, 4, 6 };
-#pragma unroll
-    for( int r = 0; r < 6; r++ )
-    {
-        const half8 hv = __builtin_bit_cast( half8, q[ src[ r ] ] );
+        const uint32_t s  = ( sgn >> ax ) & 1u;
+        const half8    hn = __builtin_bit_cast( half8, np[ 1 + 2 * ax + s ] );
+        const half8    hf = __builtin_bit_cast( half8, np[ 2 + 2 * ax - s ] );
 #pragma unroll
         for( int k = 0; k < 8; k++ )
         {
-            box[ r ][ k ] = o[ r % 3 ] + float( hv[ k ] );
+            nearP[ ax ][ k ] = o[ ax ] + float( hn[ k ] );
+            farP[ ax ][ k ]  = o[ ax ] + float( hf[ k ] );
         }
     }
-    const uint32_t* c = reinterpret_cast< const uint32_t* >( &q[ 7 ] );
+    const float4 c0 = np[ 7 ], c1 = np[ 8 ];
+    const uint32_t c[ 8 ] = { __float_as_uint( c0.x ), __float_as_uint( c0.y ), __float_as_uint( c0.z ), __float_as_uint( c0.w ),
+                              __float_as_uint( c1.x ), __float_as_uint( c1.y ), __float_as_uint( c1.z ), __float_as_uint( c1.w ) };
 #pragma unroll
     for( int k = 0; k < 8; k++ )
     {
@@ -216,6 +218,7 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
     float    tmin = 0.0f, bestT = 0.0f, bestU = 0.0f, bestV = 0.0f;
     uint32_t bestPrim = RG_EXT_MISS;
     uint32_t node     = 0;
+    uint32_t sgn      = 0; // bit a: the ray's direction is negative on axis a
     uint32_t triBase  = 0; // first triangle of the ray's scene (batched BVHs)
     int      sp       = 0;
     uint32_t steps    = 0;
@@ -284,6 +287,7 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
                     rayId            = my;
                     ray              = MakeRayPre( mk3( r.origin[ 0 ], r.origin[ 1 ], r.origin[ 2 ] ),
                                                    mk3( r.direction[ 0 ], r.direction[ 1 ], r.direction[ 2 ] ) );
+                    sgn              = ( ray.invd.x < 0.0f ? 1u : 0u ) | ( ray.invd.y < 0.0f ? 2u : 0u ) | ( ray.invd.z < 0.0f ? 4u : 0u );
                     tmin             = r.tmin;
                     bestT            = r.tmax;
                     bestU = bestV = 0.0f;
@@ -414,26 +418,32 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
             int      hitCount = 0;
             if constexpr( !H )
             {
-                const float4* np = reinterpret_cast< const float4* >( reinterpret_cast< const BvhNode8* >( a.nodes ) + node );
+                // rows of 8 floats: lo_x hi_x lo_y hi_y lo_z hi_z child; each
+                // axis's entering row (hi where the direction is negative)
+                // and leaving row are loaded in that order, so the slab test
+                // needs no min / max to order them
+                const float4*  np = reinterpret_cast< const float4* >( reinterpret_cast< const BvhNode8* >( a.nodes ) + node );
+                const uint32_t sx = sgn & 1u, sy = ( sgn >> 1 ) & 1u, sz = ( sgn >> 2 ) & 1u;
                 // two halves of four children (7 loads each) keep fewer
                 // registers live than the whole node
 #pragma unroll
                 for( int half = 0; half < 2; half++ )
                 {
-                    float4 q[ 7 ];
-#pragma unroll
-                    for( int i = 0; i < 7; i++ )
-                    {
-                        q[ i ] = np[ 2 * i + half ];
-                    }
-                    const float* f = reinterpret_cast< const float* >( q ); // lo_x hi_x lo_y hi_y lo_z hi_z child, 4 each
+                    const float4 qnx = np[ 2 * ( 0 + sx ) + half ], qfx = np[ 2 * ( 1 - sx ) + half ];
+                    const float4 qny = np[ 2 * ( 2 + sy ) + half ], qfy = np[ 2 * ( 3 - sy ) + half ];
+                    const float4 qnz = np[ 2 * ( 4 + sz ) + half ], qfz = np[ 2 * ( 5 - sz ) + half ];
+                    const float4 qc  = np[ 12 + half ];
+                    const float  nx[ 4 ] = { qnx.x, qnx.y, qnx.z, qnx.w }, fx[ 4 ] = { qfx.x, qfx.y, qfx.z, qfx.w };
+                    const float  ny[ 4 ] = { qny.x, qny.y, qny.z, qny.w }, fy[ 4 ] = { qfy.x, qfy.y, qfy.z, qfy.w };
+                    const float  nz[ 4 ] = { qnz.x, qnz.y, qnz.z, qnz.w }, fz[ 4 ] = { qfz.x, qfz.y, qfz.z, qfz.w };
+                    const float  cf[ 4 ] = { qc.x, qc.y, qc.z, qc.w };
 #pragma unroll
                     for( int j = 0; j < 4; j++ )
                     {
                         const int      k = half * 4 + j;
-                        const uint32_t c = __float_as_uint( f[ 24 + j ] );
+                        const uint32_t c = __float_as_uint( cf[ j ] );
                         float          tn;
-                        const bool     inBox = SlabTest( ray, f[ j ], f[ 8 + j ], f[ 16 + j ], f[ 4 + j ], f[ 12 + j ], f[ 20 + j ], tmin, bestT, &tn );
+                        const bool     inBox = SlabTestOrdered( ray, nx[ j ], ny[ j ], nz[ j ], fx[ j ], fy[ j ], fz[ j ], tmin, bestT, &tn );
                         const bool     h     = inBox && c != RG_EXT_BVH_CHILD_EMPTY;
                         key[ k ] = h ? SortKey( tn, c ) : ~0ull;
                         hitCount += h ? 1 : 0;
@@ -442,15 +452,16 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
             }
             else
             {
-                float    box[ 6 ][ 8 ];
+                float    nearP[ 3 ][ 8 ], farP[ 3 ][ 8 ];
                 uint32_t child[ 8 ];
-                LoadHalfNode( a.nodes, node, box, child );
+                LoadHalfNode( a.nodes, node, sgn, nearP, farP, child );
 #pragma unroll
                 for( int k = 0; k < 8; k++ )
                 {
                     const uint32_t c = child[ k ];
                     float          tn;
-                    const bool     inBox = SlabTest( ray, box[ 0 ][ k ], box[ 1 ][ k ], box[ 2 ][ k ], box[ 3 ][ k ], box[ 4 ][ k ], box[ 5 ][ k ], tmin, bestT, &tn );
+                    const bool     inBox = SlabTestOrdered( ray, nearP[ 0 ][ k ], nearP[ 1 ][ k ], nearP[ 2 ][ k ], farP[ 0 ][ k ], farP[ 1 ][ k ],
+                                                            farP[ 2 ][ k ], tmin, bestT, &tn );
                     const bool     h     = inBox && c != RG_EXT_BVH_CHILD_EMPTY;
                     key[ k ] = h ? SortKey( tn, c ) : ~0ull;
                     hitCount += h ? 1 : 0;
@@ -464,12 +475,30 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
             {
                 // closest hit: near to far, so nearer hits shrink the interval early
                 Sort8( key );
-#pragma unroll
-                for( int k = 7; k >= 1; k-- )
+                const int np = hitCount - 1; // key[np] .. key[1] are pushed, farthest first
+                if( sp + 7 <= STACK_LDS )
                 {
-                    if( k < hitCount )
+                    // room in LDS for any node: predicated stores, no
+                    // per-entry overflow branches
+#pragma unroll
+                    for( int k = 7; k >= 1; k-- )
                     {
-                        push( uint32_t( key[ k ] ) );
+                        if( k <= np )
+                        {
+                            s_stack[ ( sp + np - k ) * TR_THREADS + tid ] = uint32_t( key[ k ] );
+                        }
+                    }
+                    sp += np;
+                }
+                else
+                {
+#pragma unroll
+                    for( int k = 7; k >= 1; k-- )
+                    {
+                        if( k < hitCount )
+                        {
+                            push( uint32_t( key[ k ] ) );
+                        }
                     }
                 }
                 node = uint32_t( key[ 0 ] );
@@ -480,16 +509,37 @@ __global__ __launch_bounds__( TR_THREADS ) void k_trace( TraceArgs a )
                 // the order does not change which nodes an unoccluded ray
                 // visits; children go in slot order, unsorted
                 uint32_t next = RG_EXT_BVH_CHILD_EMPTY;
-#pragma unroll
-                for( int k = 7; k >= 0; k-- )
+                if( sp + 7 <= STACK_LDS )
                 {
-                    if( key[ k ] != ~0ull )
+                    int pos = sp;
+#pragma unroll
+                    for( int k = 7; k >= 0; k-- )
                     {
-                        if( next != RG_EXT_BVH_CHILD_EMPTY )
+                        if( key[ k ] != ~0ull )
                         {
-                            push( next );
+                            if( next != RG_EXT_BVH_CHILD_EMPTY )
+                            {
+                                s_stack[ pos * TR_THREADS + tid ] = next;
+                                pos++;
+                            }
+                            next = uint32_t( key[ k ] );
                         }
-                        next = uint32_t( key[ k ] );
+                    }
+                    sp = pos;
+                }
+                else
+                {
+#pragma unroll
+                    for( int k = 7; k >= 0; k-- )
+                    {
+                        if( key[ k ] != ~0ull )
+                        {
+                            if( next != RG_EXT_BVH_CHILD_EMPTY )
+                            {
+                                push( next );
+                            }
+                            next = uint32_t( key[ k ] );
+                        }
                     }
                 }
                 node = next;

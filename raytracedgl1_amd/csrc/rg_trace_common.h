@@ -76,6 +76,27 @@ __host__ __device__ __forceinline__ bool SlabTest( const RayPre& r,
     return tn <= fminf( tf, tmax );
 }
 
+// SlabTest with the planes already ordered for the ray: n* is the box side
+// the ray enters through on each axis (lo where 1/d > 0, hi where 1/d < 0),
+// f* the side it leaves through.  (b - o) * (1/d) rounds monotonically in
+// b, so for lo <= hi the entering side's distance is the min of the two and
+// the leaving side's the max: the same tn and tf as SlabTest, bit for bit
+// (up to the sign of a zero tn, which no comparison or sort key sees),
+// without the six min / max per box.
+__host__ __device__ __forceinline__ bool SlabTestOrdered( const RayPre& r,
+                                                          float nx, float ny, float nz,
+                                                          float fx, float fy, float fz,
+                                                          float tmin, float tmax, float* tnearOut )
+{
+    const float tx0 = ( nx - r.o.x ) * r.invd.x, tx1 = ( fx - r.o.x ) * r.invd.x;
+    const float ty0 = ( ny - r.o.y ) * r.invd.y, ty1 = ( fy - r.o.y ) * r.invd.y;
+    const float tz0 = ( nz - r.o.z ) * r.invd.z, tz1 = ( fz - r.o.z ) * r.invd.z;
+    const float tn  = fmaxf( fmaxf( tx0, ty0 ), fmaxf( tz0, tmin ) );
+    const float tf  = fminf( fminf( tx1, ty1 ), tz1 ) * 1.0000004f;
+    *tnearOut       = tn;
+    return tn <= fminf( tf, tmax );
+}
+
 // Watertight intersection in [tmin, tmax] (both inclusive).  u, v are the
 // barycentric weights of p1 and p2.  cullFront rejects triangles whose
 // vertices appear counter-clockwise from the ray origin (dot(n, d) < 0 for

@@ -410,6 +410,7 @@ PLANES = [
     ("SCREEN_EMISSION", 1, 4, np.float32, False),
     ("ACID_FOG", 1, 4, np.float32, False),
     ("DENOISE_GUIDE", 0, 4, np.float32, False),
+    ("GI_REUSE", 0, 16, np.uint32, False),
 ]
 PLANE = {name: i for i, (name, *_rest) in enumerate(PLANES)}
 

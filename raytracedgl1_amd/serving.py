@@ -19,17 +19,25 @@ from . import ffi, rtgl, scenes
 _P, _U32, _F = C.c_void_p, C.c_uint32, C.c_float
 
 RG_EXT_RESULT_OVER_BUDGET = 0x52470101
-RG_EXT_FRAME_DONE, RG_EXT_FRAME_DROPPED, RG_EXT_FRAME_FAILED = 0, 1, 2
+RG_EXT_FRAME_DONE, RG_EXT_FRAME_DROPPED, RG_EXT_FRAME_FAILED, RG_EXT_FRAME_EXPIRED = 0, 1, 2, 3
+RG_EXT_SCHEDULE_OLDEST_FIRST, RG_EXT_SCHEDULE_NEWEST_FIRST = 0, 1
+RG_EXT_SERVER_READBACK = 1
 
 
 class RgExtServerCreateInfo(C.Structure):
     _fields_ = [("streamCount", _U32), ("framesInFlightPerStream", _U32), ("maxRenderWidth", _U32),
-                ("maxRenderHeight", _U32), ("precisionFlags", _U32), ("_reserved", _U32), ("memoryBudgetBytes", C.c_uint64)]
+                ("maxRenderHeight", _U32), ("precisionFlags", _U32), ("batchFrames", _U32), ("memoryBudgetBytes", C.c_uint64),
+                ("maxQueueSeconds", C.c_double), ("schedule", _U32), ("flags", _U32)]
 
 
 class RgExtFrameResult(C.Structure):
     _fields_ = [("session", _P), ("tag", C.c_uint64), ("status", _U32), ("frameId", _U32), ("submitTime", C.c_double),
-                ("startTime", C.c_double), ("endTime", C.c_double)]
+                ("startTime", C.c_double), ("endTime", C.c_double), ("pImage", _P), ("batchSize", _U32), ("_pad", _U32)]
+
+
+class RgExtServerPolicy(C.Structure):
+    _fields_ = [("schedule", _U32), ("batchFrames", _U32), ("framesInFlightPerStream", _U32), ("streams", _U32),
+                ("maxQueueSeconds", C.c_double)]
 
 
 class RgExtServerStats(C.Structure):
@@ -38,7 +46,7 @@ class RgExtServerStats(C.Structure):
                 ("framesFailed", C.c_uint64), ("now", C.c_double)]
 
 
-assert C.sizeof(RgExtServerCreateInfo) == 32 and C.sizeof(RgExtFrameResult) == 48 and C.sizeof(RgExtServerStats) == 64
+assert C.sizeof(RgExtServerPolicy) == 24 and C.sizeof(RgExtServerCreateInfo) == 48 and C.sizeof(RgExtFrameResult) == 64 and C.sizeof(RgExtServerStats) == 64
 
 _SIGS = {
     "rgExtServerCreate": [C.POINTER(RgExtServerCreateInfo), C.POINTER(_P)],
@@ -53,6 +61,7 @@ _SIGS = {
     "rgExtServerWaitIdle": [_P],
     "rgExtServerGetStats": [_P, C.POINTER(RgExtServerStats)],
     "rgExtSessionGetFramebuffers": [_P, C.POINTER(_P)],
+    "rgExtServerSetPolicy": [_P, C.c_void_p],
 }
 _bound = False
 
@@ -86,9 +95,11 @@ class Server:
 
     def __init__(self, max_width: int, max_height: int, streams: int = 4, frames_in_flight: int = 2,
                  precision: int = 0, budget_bytes: int = 0, atrous: int = 4, candidates: int = 8,
-                 temporal: int = 1, spatial: int = 8):
+                 temporal: int = 1, spatial: int = 8, batch: int = 1, schedule: int = RG_EXT_SCHEDULE_OLDEST_FIRST,
+                 max_queue_s: float = 0.0, readback: bool = False):
         l = _lib()
-        ci = RgExtServerCreateInfo(streams, frames_in_flight, max_width, max_height, precision, 0, budget_bytes)
+        ci = RgExtServerCreateInfo(streams, frames_in_flight, max_width, max_height, precision, batch, budget_bytes,
+                                   max_queue_s, schedule, RG_EXT_SERVER_READBACK if readback else 0)
         h = _P()
         ffi.check("rgExtServerCreate", l.rgExtServerCreate(C.byref(ci), C.byref(h)))
         self.h = h.value
@@ -137,10 +148,14 @@ class Server:
         n = _U32()
         while True:
             ffi.check("rgExtServerPoll", _lib().rgExtServerPoll(self.h, self._results, len(self._results), C.byref(n)))
-            out.extend((r.session, r.tag, r.status, r.frameId, r.submitTime, r.startTime, r.endTime)
+            out.extend((r.session, r.tag, r.status, r.frameId, r.submitTime, r.startTime, r.endTime, r.pImage or 0, r.batchSize)
                        for r in self._results[:n.value])
             if n.value < len(self._results):
                 return out
+
+    def set_policy(self, schedule: int, batch: int, frames_in_flight: int, streams: int, max_queue_s: float = 0.0) -> None:
+        p = RgExtServerPolicy(schedule, batch, frames_in_flight, streams, max_queue_s)
+        ffi.check("rgExtServerSetPolicy", _lib().rgExtServerSetPolicy(self.h, C.byref(p)))
 
     def wait_idle(self) -> None:
         ffi.check("rgExtServerWaitIdle", _lib().rgExtServerWaitIdle(self.h))
@@ -201,7 +216,9 @@ def run_poisson(server: Server, sessions: list, rate_hz: float, duration_s: floa
     tail_s more seconds so that measured requests still meet newer ones (a
     request left waiting at the end of the load would otherwise count the
     drain as latency), then the server drains.  Latency of a finished frame
-    = its completion - its request's submit."""
+    = its completion (after the readback when the server reads frames back)
+    - its request's submit; requests replaced by a newer one of their
+    session, or expired in the queue, are counted apart."""
     rng = np.random.default_rng(seed)
     n = len(sessions)
     span = duration_s + tail_s
@@ -247,7 +264,9 @@ def run_poisson(server: Server, sessions: list, rate_hz: float, duration_s: floa
         "requests": measured_tags,
         "frames_done": len(done),
         "frames_dropped": sum(1 for r in results if r[2] == RG_EXT_FRAME_DROPPED),
+        "frames_expired": sum(1 for r in results if r[2] == RG_EXT_FRAME_EXPIRED),
         "frames_failed": sum(1 for r in results if r[2] == RG_EXT_FRAME_FAILED),
+        "mean_batch": float(np.mean([r[8] for r in done])) if done else None,
         "frames_per_s": len(done) / max(last - first, 1e-9),
         "mpix_per_s": sum(px.get(r[0], 0) for r in done) / max(last - first, 1e-9) / 1e6,
         "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,

@@ -72,6 +72,7 @@ class Oracle:
             "or_denoise_variance": (None, [_P, _P]),
             "or_denoise_atrous": (None, [_P, _P, _U32]),
             "or_denoise": (None, [_P, _P]),
+            "or_denoise_set_threads": (None, [C.c_int]),
             "or_texture_create": (_P, [_P, _U32, _U32, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
             "or_texture_free": (None, [_P]),
             "or_texture_mip_count": (_U32, [_P]),

@@ -201,6 +201,25 @@ def test_compose_closed_form(oracle):
     assert np.allclose(pre[0, 0, :3], 0.5)
 
 
+def test_oracle_denoise_threads_do_not_change_results(oracle):
+    """The oracle's passes run rows on several host threads for the CPU
+    baseline; every pass writes planes it does not read, so the planes are
+    bit-identical to a single-threaded run."""
+    out = []
+    for threads in (1, 5):
+        cur, prev = make_gbuffer(96, 54, seed=11)
+        o = OracleFramebuffers(oracle, 96, 54)
+        fill_planes(cur, prev, None, o)
+        oracle.l.or_denoise_set_threads(threads)
+        try:
+            o.run("", ffi.frame_params(96, 54))
+        finally:
+            oracle.l.or_denoise_set_threads(1)
+        out.append({k: v.copy() for k, v in o.cur.items()})
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[1][k]), k
+
+
 # ------------------------------------------------------------ GPU parity
 
 def _pair(oracle, W, H, seed, **kw):

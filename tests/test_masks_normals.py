@@ -305,19 +305,31 @@ def test_one_triangle_classes_keep_their_masks():
     tri = verts([(-0.8, 2.0, -0.8), (-0.8, 2.0, 0.8), (0.8, 2.0, 0.0)], [(0.0, 1.0, 0.0)] * 3)
     alone = render([(FLOOR, None, 0, 0xC0C0C0)], CAM)
     out = render([(FLOOR, None, 0, 0xC0C0C0), (tri, None, glass, 0x404040)], CAM)
-    assert _blocker_pixels(out, 1).sum() > 10  # the glass triangle is visible
+    # the glass triangle is seen by primary rays: the pixels it covers carry
+    # the surface its reflection / refraction chain ends on (the G-buffer of
+    # a refracting hit is the chain's, RaygenCommon.h), so they differ from
+    # the frame without it (even pixels reflect to the sky)
+    p_out, p_alone = out["SURFACE_POSITION"].reshape(-1, 4), alone["SURFACE_POSITION"].reshape(-1, 4)
+    sky_out, sky_alone = out["IS_SKY"].reshape(-1), alone["IS_SKY"].reshape(-1)
+    changed = (sky_out != sky_alone) | ((sky_alone == 0) & ~np.all(p_out == p_alone, -1))
+    assert changed.sum() > 10
     # no shadow: every pixel showing the same floor point as without the
     # glass (all but those seen through it) is lit exactly as without it,
     # the floor under the triangle included
-    p_out, p_alone = out["SURFACE_POSITION"].reshape(-1, 4), alone["SURFACE_POSITION"].reshape(-1, 4)
     vis, hit = _hits(alone)
     same = hit & np.all(p_out == p_alone, -1)
     assert same.sum() > 0.9 * hit.sum()
-    under = same & (np.abs(p_alone[:, 0]) < 0.2) & (np.abs(p_alone[:, 2]) < 0.2)
-    assert under.sum() > 5
-    d_out = out["UNFILTERED_DIRECT"].reshape(-1, 4)[same, :3]
-    d_alone = alone["UNFILTERED_DIRECT"].reshape(-1, 4)[same, :3]
-    assert np.array_equal(d_out, d_alone)
+    # the floor straight under the triangle (its shadow with the sun overhead)
+    under = same & (np.abs(p_alone[:, 0]) < 0.4) & (np.abs(p_alone[:, 2]) < 0.3)
+    assert under.sum() > 10
+    # (not bit for bit: ReSTIR's spatial reuse takes sun-disk samples from
+    # neighbours, and the pixels seen through the glass carry other ones)
+    d_out = out["UNFILTERED_DIRECT"].reshape(-1, 4)[:, :3].sum(-1)
+    d_alone = alone["UNFILTERED_DIRECT"].reshape(-1, 4)[:, :3].sum(-1)
+    ratio = d_out[under].mean() / d_alone[under].mean()
+    print(f"  direct light under the glass triangle / without it: {ratio:.5f}", flush=True)
+    assert abs(ratio - 1.0) < 1e-2
+    assert np.all(np.abs(d_out[under] - d_alone[under]) <= 1e-2 * d_alone[under])
     # one opaque triangle (its own class) beside the glass triangle: a shadow
     opaque = verts([(-0.8, 2.0, -0.8), (-0.8, 2.0, 0.8), (0.8, 2.0, 0.0)], [(0.0, 1.0, 0.0)] * 3)
     out = render([(FLOOR, None, 0, 0xC0C0C0), (opaque, None, SKYVIS, 0x404040),

@@ -10,7 +10,17 @@
   light to itself, so the two may differ in a few samples).
 - Sessions are admitted by the memory budget, and a waiting request is
   replaced by a newer one.
+- Batched launches (batchFrames > 1: one traversal launch per stage over
+  the rays of several sessions of a scene) render every session's frames
+  bit for bit as alone.
+- Readback: a finished frame's pImage holds FINAL as RGBA8 with the sRGB
+  transfer (within one code of a float64 encoding of the FINAL plane).
+- Scheduling: NEWEST_FIRST starts the newest waiting request first, and
+  requests that wait longer than maxQueueSeconds are reported expired.
 """
+import ctypes as C
+import time
+
 import numpy as np
 import pytest
 
@@ -39,15 +49,19 @@ def _scenes():
 SPECS = [(1, 128, 72), (0, 160, 90), (0, 200, 100), (1, 160, 90)]
 
 
-def _render(server, sess, frames):
+def _render(server, sess, frames, batches=None):
     """Frames for every session in `sess`, one request per session per round,
-    all of a round in flight together; persistent planes after the last."""
+    all of a round in flight together; persistent planes after the last.
+    `batches` collects the batch size of every finished frame."""
+    res = []
     for f in range(frames):
         for k, s in sess:
             server.submit(s, serving.session_camera(k, 0.1 * f), (f + 1) / 60.0, f)
         server.wait_idle()
-    res = server.poll()
+        res += server.poll()
     assert all(r[2] == serving.RG_EXT_FRAME_DONE for r in res), res
+    if batches is not None:
+        batches.extend(r[8] for r in res)
     out = {}
     for k, s in sess:
         fb = server.framebuffers(s)
@@ -55,17 +69,22 @@ def _render(server, sess, frames):
     return out
 
 
-def test_session_same_alone_and_in_the_mix():
-    print("serving: 4 sessions alone vs mixed, 2 streams", flush=True)
-    sc = _scenes()
+def _alone(sc, specs):
     alone = {}
-    for k, (si, w, h) in enumerate(SPECS):
+    for k, (si, w, h) in enumerate(specs):
         srv = serving.Server(256, 128, streams=2)
         try:
             scn = srv.scene(sc[si])
             alone.update(_render(srv, [(k, srv.session(scn, w, h))], FRAMES))
         finally:
             srv.close()
+    return alone
+
+
+def test_session_same_alone_and_in_the_mix():
+    print("serving: 4 sessions alone vs mixed, 2 streams", flush=True)
+    sc = _scenes()
+    alone = _alone(sc, SPECS)
     srv = serving.Server(256, 128, streams=2, frames_in_flight=2)
     try:
         handles = [srv.scene(s) for s in sc]
@@ -152,3 +171,96 @@ def test_waiting_request_is_replaced():
     done = [r for r in res if r[2] == serving.RG_EXT_FRAME_DONE]
     assert [r[3] for r in done] == list(range(len(done)))  # frame ids count rendered frames
     assert all(r[4] <= r[5] <= r[6] for r in done)
+
+
+# six sessions over two scenes; batches form among the sessions of a scene
+BATCH_SPECS = SPECS + [(0, 96, 64), (0, 256, 128)]
+
+
+def test_batched_sessions_match_alone():
+    print("serving: 6 sessions in batches of up to 4, one stream", flush=True)
+    sc = _scenes()
+    alone = _alone(sc, BATCH_SPECS)
+    srv = serving.Server(256, 128, streams=1, frames_in_flight=1, batch=4)
+    batches = []
+    try:
+        handles = [srv.scene(s) for s in sc]
+        sess = [(k, srv.session(handles[si], w, h)) for k, (si, w, h) in enumerate(BATCH_SPECS)]
+        got = _render(srv, sess, FRAMES, batches)
+    finally:
+        srv.close()
+    print(f"  batch sizes of the finished frames: {sorted(batches)}", flush=True)
+    assert max(batches) > 1 and max(batches) <= 4
+    for k in range(len(BATCH_SPECS)):
+        for p in PERSISTENT:
+            assert np.array_equal(alone[k][p], got[k][p]), (k, p)
+
+
+def _srgb8(c):
+    """RGBA8 code of linear values (IEC 61966-2-1), float64."""
+    c = np.clip(c.astype(np.float64), 0.0, 1.0)
+    e = np.where(c <= 0.0031308, 12.92 * c, 1.055 * np.power(c, 1.0 / 2.4) - 0.055)
+    return np.floor(np.clip(e, 0.0, 1.0) * 255.0 + 0.5).astype(np.int64)
+
+
+def test_readback_image_is_the_final_plane():
+    print("serving: readback of FINAL as RGBA8", flush=True)
+    a, _ = _scenes()
+    srv = serving.Server(160, 90, streams=1, readback=True)
+    try:
+        s = srv.session(srv.scene(a), 160, 90)
+        res = []
+        for f in range(3):
+            srv.submit(s, serving.session_camera(1, 0.1 * f), (f + 1) / 60.0, f)
+            srv.wait_idle()
+            res += srv.poll()
+        assert all(r[2] == serving.RG_EXT_FRAME_DONE and r[7] for r in res)
+        assert res[-1][7] != res[-2][7]  # the two images are used in turn
+        img = np.frombuffer(C.string_at(res[-1][7], 160 * 90 * 4), np.uint8).reshape(90, 160, 4).astype(np.int64)
+        fin = srv.framebuffers(s).download(ffi.PLANE["FINAL"])
+    finally:
+        srv.close()
+    ref = _srgb8(fin[..., :3])
+    d = np.abs(img[..., :3] - ref)
+    print(f"  readback vs FINAL: max code difference {d.max()}, differing {np.mean(d > 0):.4f}", flush=True)
+    assert d.max() <= 1 and np.mean(d > 0) < 0.01
+    assert (img[..., 3] == 255).all()
+    assert 0 < img[..., :3].mean() < 255
+
+
+def test_newest_first_and_expiry():
+    print("serving: NEWEST_FIRST order and maxQueueSeconds", flush=True)
+    a, _ = _scenes()
+    srv = serving.Server(2560, 1440, streams=1, frames_in_flight=1, schedule=serving.RG_EXT_SCHEDULE_NEWEST_FIRST)
+    try:
+        scn = srv.scene(a)
+        big = srv.session(scn, 2560, 1440)
+        small = [srv.session(scn, 160, 90) for _ in range(3)]
+        srv.submit(big, serving.session_camera(0, 0.0), 1 / 60.0, 100)
+        time.sleep(0.003)  # the 2560x1440 frame is launched and keeps the stream busy for several ms
+        for i, s in enumerate(small):
+            srv.submit(s, serving.session_camera(i + 1, 0.0), 1 / 60.0, i)
+        srv.wait_idle()
+        res = {r[1]: r for r in srv.poll()}
+    finally:
+        srv.close()
+    assert all(r[2] == serving.RG_EXT_FRAME_DONE for r in res.values())
+    # the three waited behind the 2560x1440 frame; the newest started first
+    starts = [res[i][5] for i in range(3)]
+    print(f"  start order of tags 0, 1, 2: {np.argsort(starts)}", flush=True)
+    assert starts[2] <= starts[1] <= starts[0]
+    srv = serving.Server(160, 90, streams=1, frames_in_flight=1, max_queue_s=1e-9)
+    try:
+        scn = srv.scene(a)
+        sess = [srv.session(scn, 160, 90) for _ in range(4)]
+        for i, s in enumerate(sess):
+            srv.submit(s, serving.session_camera(i, 0.0), 1 / 60.0, i)
+        srv.wait_idle()
+        res = srv.poll()
+        st = srv.stats()
+    finally:
+        srv.close()
+    status = [r[2] for r in res]
+    assert len(res) == 4 and all(x in (serving.RG_EXT_FRAME_DONE, serving.RG_EXT_FRAME_EXPIRED) for x in status)
+    assert status.count(serving.RG_EXT_FRAME_EXPIRED) >= 1
+    assert st.framesDropped == status.count(serving.RG_EXT_FRAME_EXPIRED)
