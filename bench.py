@@ -67,6 +67,9 @@ SERVING_POLICIES = {
     "full_load_latency": (1, 1, 1, 4, 0.02, 3, "one frame per launch, 1 launch queued on each of 4 streams, newest request "
                                                "first, requests waiting over 20 ms expire; a waiting request is replaced by "
                                                "its session's newer one"),
+    "full_load_batched_latency": (1, 2, 1, 4, 0.02, 4, "up to 2 frames of one scene per launch, 1 launch queued on each of "
+                                                       "4 streams, newest request first, requests waiting over 20 ms expire; "
+                                                       "a waiting request is replaced by its session's newer one"),
 }
 
 
