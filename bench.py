@@ -335,7 +335,8 @@ def batch_run(a) -> dict:
     resl = ((256, 256), (640, 360), (1280, 720), (1920, 1080))
     res = [resl[k] for k in rng.integers(0, 4, S)]
     t = time.perf_counter()
-    tris = np.concatenate([scenes.soup_tris(scenes.SEED_SCENE_BATCH + 1 + i, int(c)) for i, c in enumerate(counts)])
+    soups = [scenes.soup_tris(scenes.SEED_SCENE_BATCH + 1 + i, int(c)) for i, c in enumerate(counts)]
+    tris = np.concatenate(soups)
     offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint32)
     print(f"batch: {S} scenes, {len(tris)} triangles generated in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     d_tris = ffi.DeviceArray.from_numpy(tris)
@@ -365,9 +366,11 @@ def batch_run(a) -> dict:
     n_rays = int(roffs[-1])
     d_rays = ffi.DeviceArray(n_rays, ffi.RAY_DTYPE)
     d_hits = ffi.DeviceArray(n_rays, ffi.HIT_DTYPE)
+    eyes = []
     for k, (w, h) in enumerate(res):
         ang = rng.uniform(0, 2 * np.pi)
         eye = np.array([180 * np.cos(ang), rng.uniform(20, 80), 180 * np.sin(ang)])
+        eyes.append(eye)
         cam = scenes.Camera(eye, np.zeros(3))
         p = ffi.frame_params(w, h)
         view = cam.view().astype(np.float32)
@@ -406,6 +409,8 @@ def batch_run(a) -> dict:
     hit_frac = float((hits[:, 3] != ffi.RG_EXT_MISS).mean())
     print(f"batch trace {t_batch_trace * 1e3:.1f} ms (tuned {t_tuned_trace * 1e3:.1f}), 64 launches {t_single_trace * 1e3:.1f} ms",
           file=sys.stderr, flush=True)
+    del d_rays, d_hits, d_tris, bb, singles
+    frames = group_frames(soups, res, eyes)
     return {
         "scenes": S, "triangles": int(offs[-1]), "rays": n_rays, "hit_fraction": hit_frac,
         "resolutions": [f"{w}x{h}" for w, h in res],
@@ -415,7 +420,63 @@ def batch_run(a) -> dict:
         "trace_mrays_per_s": {"batched": n_rays / t_batch_trace / 1e6, "batched_tuned": n_rays / t_tuned_trace / 1e6,
                               "separate": n_rays / t_single_trace / 1e6},
         "autotune": {"stack_lds_entries": tune.stackLdsEntries, "ray_chunk": tune.rayChunk},
+        "frames": frames,
     }
+
+
+def group_frames(soups, res, eyes, rounds: int = 3) -> dict:
+    """Whole frames of the batched scenes: each soup as a plain scene (a sun,
+    grey surfaces) of a render server, the scenes in one scene group, one
+    session per scene at its resolution and camera.  A round submits every
+    session's next frame and waits for all of them; frames/s over `rounds`
+    rounds, with up to 16 frames of different scenes per launch (each traced
+    stage one traversal launch over all their rays, through the group's
+    batched BVH; then A-SVGF and compose per frame) and with one frame per
+    launch through the same BVH."""
+    import numpy as np
+
+    from raytracedgl1_amd import scenes, serving
+
+    srv = serving.Server(max(w for w, _ in res), max(h for _, h in res), streams=2, frames_in_flight=2, batch=16, atrous=5,
+                         candidates=8, temporal=1, spatial=8)
+    try:
+        t = time.perf_counter()
+        handles = [srv.scene(scenes.soup_scene(s)) for s in soups]
+        t_scenes = time.perf_counter() - t
+        t = time.perf_counter()
+        g = srv.group(handles)
+        t_group = time.perf_counter() - t
+        sess = [srv.session(handles[k], w, h) for k, (w, h) in enumerate(res)]
+        cams = [scenes.Camera(e, np.zeros(3)) for e in eyes]
+        print(f"group frames: {len(handles)} scenes built in {t_scenes:.1f} s, group BVH {t_group * 1e3:.1f} ms", file=sys.stderr, flush=True)
+        frame = [0]
+
+        def run(batch):
+            srv.set_policy(serving.RG_EXT_SCHEDULE_OLDEST_FIRST, batch, 2, 2, 0.0)
+
+            def one_round():
+                for k, s in enumerate(sess):
+                    srv.submit(s, cams[k], frame[0] / 60.0, frame[0])
+                frame[0] += 1
+                srv.wait_idle()
+                return srv.poll()
+            one_round()  # warm
+            t0 = time.perf_counter()
+            done = []
+            for _ in range(rounds):
+                done += one_round()
+            dt = time.perf_counter() - t0
+            assert all(r[2] == serving.RG_EXT_FRAME_DONE for r in done), "a frame of the group run failed"
+            px = sum(s.width * s.height for s in sess) * rounds
+            return {"frames_per_s": len(done) / dt, "mpix_per_s": px / dt / 1e6, "mean_batch": float(np.mean([r[8] for r in done]))}
+        batched = run(16)
+        single = run(1)
+        srv.destroy_group(g)
+        print(f"group frames: batched {batched}, one per launch {single}", file=sys.stderr, flush=True)
+        return {"sessions": len(sess), "rounds": rounds, "group_build_ms": t_group * 1e3, "batched_16": batched,
+                "one_per_launch": single, "speedup": batched["frames_per_s"] / single["frames_per_s"]}
+    finally:
+        srv.close()
 
 
 def load_profile(path):

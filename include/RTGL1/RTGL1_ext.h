@@ -589,11 +589,20 @@ RGAPI RgResult RGCONV rgExtSetStaticGeometry( RgInstance instance, RgBool32 enab
  * points at the frame's pixels, valid until the session finishes two more
  * frames.
  *
+ * Scene groups: frames of different scenes share a launch when their scenes
+ * are members of one group (rgExtSceneGroupCreate), which builds one
+ * batched BVH over the members' triangles, each scene its own subtree and
+ * root.  Each traversal stage of such a batch is one launch whose rays
+ * start at their own scene's root.  A grouped scene's frames always trace
+ * the group's BVH, alone or batched.
+ *
  * A session renders the same pixels alone, among others and in a batch:
- * its frames depend only on its own requests and the scene. */
-typedef struct RgExtServer_T*  RgExtServer;
-typedef struct RgExtScene_T*   RgExtScene;
-typedef struct RgExtSession_T* RgExtSession;
+ * its frames depend only on its own requests and the scene (and, for a
+ * grouped scene, the group it was traced in). */
+typedef struct RgExtServer_T*     RgExtServer;
+typedef struct RgExtScene_T*      RgExtScene;
+typedef struct RgExtSession_T*    RgExtSession;
+typedef struct RgExtSceneGroup_T* RgExtSceneGroup;
 
 #define RG_EXT_SCHEDULE_OLDEST_FIRST 0u
 #define RG_EXT_SCHEDULE_NEWEST_FIRST 1u
@@ -659,8 +668,19 @@ RGAPI RgResult RGCONV rgExtSceneUploadMeshPrimitive( RgExtScene scene, const RgM
 RGAPI RgResult RGCONV rgExtSceneUploadDirectionalLight( RgExtScene scene, const RgDirectionalLightUploadInfo* pInfo );
 RGAPI RgResult RGCONV rgExtSceneUploadSphericalLight( RgExtScene scene, const RgSphericalLightUploadInfo* pInfo );
 RGAPI RgResult RGCONV rgExtSceneBuild( RgExtScene scene );
-/* Fails with RG_RESULT_WRONG_FUNCTION_CALL while sessions use the scene. */
+/* Fails with RG_RESULT_WRONG_FUNCTION_CALL while sessions use the scene or
+ * it is a member of a group. */
 RGAPI RgResult RGCONV rgExtSceneDestroy( RgExtScene scene );
+/* A group of 2..128 built scenes whose frames may share launches: every
+ * primitive opaque world-0 geometry (no alpha test, no sky visibility,
+ * first-person, water, glass or acid primitives), at least 2 triangles per
+ * scene, the same light kinds in every scene (a directional light or not,
+ * sphere lights or not).  A scene belongs to at most one group.  Builds the
+ * group's BVH synchronously. */
+RGAPI RgResult RGCONV rgExtSceneGroupCreate( RgExtServer server, const RgExtScene* pScenes, uint32_t sceneCount, RgExtSceneGroup* pOut );
+/* Waits until no request is waiting or in flight, then frees the group; its
+ * scenes trace their own BVHs again. */
+RGAPI RgResult RGCONV rgExtSceneGroupDestroy( RgExtSceneGroup group );
 /* A camera session of a fixed render size over a built scene; returns
  * RG_EXT_RESULT_OVER_BUDGET when its bytes do not fit. */
 RGAPI RgResult RGCONV rgExtSessionCreate( RgExtScene scene, uint32_t width, uint32_t height, RgExtSession* pOut );

@@ -172,7 +172,7 @@ void TraceRaysFrame( const BvhObject& bvh, TraceContext& ctx, const RgExtRay* d_
 // of scene s are [rayOffsets[s], rayOffsets[s+1]) (host array, sceneCount + 1
 // entries); primitive ids in the hits are relative to the scene.
 void TraceRaysBatch( const BvhObject& bvh, TraceContext& ctx, const RgExtRay* d_rays, const uint32_t* rayOffsets, RgExtHit* d_hits, uint32_t flags,
-                     hipStream_t stream, hipEvent_t evBefore = nullptr, hipEvent_t evAfter = nullptr );
+                     hipStream_t stream, hipEvent_t evBefore = nullptr, hipEvent_t evAfter = nullptr, const TraceAccum* accum = nullptr );
 // Times every LDS stack depth x ray chunk on these rays (a batched BVH takes
 // rayOffsets, else count) and keeps the fastest on ctx for later launches.
 void AutotuneTrace( const BvhObject& bvh, TraceContext& ctx, const RgExtRay* d_rays, uint32_t count, const uint32_t* rayOffsets, RgExtHit* d_hits,

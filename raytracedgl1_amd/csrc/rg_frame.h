@@ -75,13 +75,18 @@ struct EncodeSlot
     const RgExtFrameParams* p;
     FramebufferSet*         fb;
     const RgDrawFrameInfo*  di;
+    const DeviceScene*      scene = nullptr; // its scene, when the frames' scenes differ
+    uint32_t                group = 0;       // its scene's index in the grouped BVH
 };
 
-// B frames of one scene: their traced passes share each stage's traversal
-// launch (TraceFrames over the wavefront wf, which must hold every frame's
-// pixels); A-SVGF, eye adaptation and compose then run frame by frame.  A
-// frame's result equals EncodeFrame's for it alone.
-uint64_t EncodeFrames( const EncodeSlot* slots, uint32_t B, TraceContext& tc, const BvhObject& bvh, const DeviceScene& scene, Wavefront& wf,
+// B frames: their traced passes share each stage's traversal launch
+// (TraceFrames over the wavefront wf, which must hold every frame's pixels);
+// A-SVGF, eye adaptation and compose then run frame by frame.  With `scene`
+// every frame is of that scene and bvh is its frame BVH; with a null scene
+// the frames are of their slots' plain scenes, in group order, and bvh is
+// the batched BVH of the group.  A frame's result equals EncodeFrame's for
+// it alone.
+uint64_t EncodeFrames( const EncodeSlot* slots, uint32_t B, TraceContext& tc, const BvhObject& bvh, const DeviceScene* scene, Wavefront& wf,
                        hipStream_t s );
 
 } // namespace rg

@@ -263,14 +263,14 @@ uint64_t EncodeFrame( FrameGpu& g, TraceContext& tc, const RgExtFrameParams& p, 
     return rays;
 }
 
-uint64_t EncodeFrames( const EncodeSlot* slots, uint32_t B, TraceContext& tc, const BvhObject& bvh, const DeviceScene& scene, Wavefront& wf,
+uint64_t EncodeFrames( const EncodeSlot* slots, uint32_t B, TraceContext& tc, const BvhObject& bvh, const DeviceScene* scene, Wavefront& wf,
                        hipStream_t s )
 {
     RG_REQUIRE( B >= 1 && B <= MAX_TRACE_SLOTS, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "frames per batch out of range" );
     TraceSlot ts[ MAX_TRACE_SLOTS ];
     for( uint32_t k = 0; k < B; k++ )
     {
-        ts[ k ] = TraceSlot{ &slots[ k ].g->pt, slots[ k ].p, slots[ k ].fb };
+        ts[ k ] = TraceSlot{ &slots[ k ].g->pt, slots[ k ].p, slots[ k ].fb, slots[ k ].scene, slots[ k ].group };
     }
     const uint32_t flags = slots[ 0 ].p->rayCullBackFaces ? RG_EXT_TRACE_CULL_FRONT_FACE : 0u;
     const uint64_t rays  = TraceFrames( ts, B, bvh, tc, scene, wf, flags, s );

@@ -78,6 +78,8 @@ struct TraceSlot
     PathTracer*             pt;
     const RgExtFrameParams* p;
     FramebufferSet*         fb;
+    const DeviceScene*      scene = nullptr; // its scene, when the frames' scenes differ
+    uint32_t                group = 0;       // its scene's index in the grouped BVH
 };
 constexpr uint32_t MAX_TRACE_SLOTS = 16;
 
@@ -86,12 +88,21 @@ constexpr uint32_t MAX_TRACE_SLOTS = 16;
 // must be the same too).
 bool TraceCompatible( const RgExtFrameParams& a, const RgExtFrameParams& b );
 
-// The traced passes of B frames of one scene, stage by stage: each stage's
-// shading kernels run per frame, and each traversal covers every frame's
-// rays of that stage in one launch (frame k's rays are a slice of wf).
+// Scenes whose frames can share traversal launches although they differ:
+// every triangle opaque world-0 geometry, no alpha test, no reflecting or
+// refracting geometry (one BVH segment, no per-ray cull masks to honour).
+bool GroupableScene( const DeviceScene& scene );
+
+// The traced passes of B frames, stage by stage: each stage's shading
+// kernels run per frame, and each traversal covers every frame's rays of
+// that stage in one launch (frame k's rays are a slice of wf).  With
+// `scene`, every frame is of that scene and bvh is its frame BVH; with a
+// null scene, frame k is of slots[k].scene (GroupableScene), bvh is one
+// batched BVH over those scenes (BuildBvhBatch) in which that scene is
+// number slots[k].group, and the slots come in non-decreasing group order.
 // Every frame's result equals the frame traced alone.  The first slot's
 // path tracer times and counts the launches.  Returns the rays traced.
-uint64_t TraceFrames( const TraceSlot* slots, uint32_t B, const BvhObject& bvh, TraceContext& tc, const DeviceScene& scene, Wavefront& wf,
+uint64_t TraceFrames( const TraceSlot* slots, uint32_t B, const BvhObject& bvh, TraceContext& tc, const DeviceScene* scene, Wavefront& wf,
                       uint32_t traceFlags, hipStream_t s );
 
 } // namespace rg

@@ -2327,7 +2327,19 @@ uint64_t PathTracer::Trace( const RgExtFrameParams& p, const BvhObject& bvh, Tra
 {
     ( void )timing;
     TraceSlot one{ this, &p, &fb };
-    return TraceFrames( &one, 1, bvh, tc, scene, *wf, traceFlags, s );
+    return TraceFrames( &one, 1, bvh, tc, &scene, *wf, traceFlags, s );
+}
+
+bool GroupableScene( const DeviceScene& scene )
+{
+    for( uint32_t c = 0; c < GC_COUNT; c++ )
+    {
+        if( c != GC_WORLD_0 && scene.classTris[ c ] != 0 )
+        {
+            return false;
+        }
+    }
+    return !scene.alphaTested && !scene.reflRefrGeometry;
 }
 
 bool TraceCompatible( const RgExtFrameParams& a, const RgExtFrameParams& b )
@@ -2337,15 +2349,29 @@ bool TraceCompatible( const RgExtFrameParams& a, const RgExtFrameParams& b )
            a.directionalLightExists == b.directionalLightExists && ( a.lightCount > 0 ) == ( b.lightCount > 0 );
 }
 
-uint64_t TraceFrames( const TraceSlot* slots, uint32_t B, const BvhObject& bvh, TraceContext& tc, const DeviceScene& scene, Wavefront& wf,
+uint64_t TraceFrames( const TraceSlot* slots, uint32_t B, const BvhObject& bvh, TraceContext& tc, const DeviceScene* sharedScene, Wavefront& wf,
                       uint32_t traceFlags, hipStream_t s )
 {
     RG_REQUIRE( B >= 1 && B <= MAX_TRACE_SLOTS, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "frames per batch out of range" );
-    const RgExtFrameParams& p0 = *slots[ 0 ].p;
+    const RgExtFrameParams& p0      = *slots[ 0 ].p;
+    const bool              grouped = sharedScene == nullptr;
     for( uint32_t k = 1; k < B; k++ )
     {
         RG_REQUIRE( TraceCompatible( p0, *slots[ k ].p ), RG_RESULT_WRONG_FUNCTION_ARGUMENT,
                     "frames of one batch differ in their ray masks, reflection depth or lights" );
+    }
+    if( grouped )
+    {
+        RG_REQUIRE( bvh.sceneCount > 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "frames of several scenes need their grouped BVH" );
+        for( uint32_t k = 0; k < B; k++ )
+        {
+            RG_REQUIRE( slots[ k ].scene && GroupableScene( *slots[ k ].scene ) && slots[ k ].group < bvh.sceneCount &&
+                            ( k == 0 || slots[ k ].group >= slots[ k - 1 ].group ),
+                        RG_RESULT_WRONG_FUNCTION_ARGUMENT, "grouped frames need plain scenes of the group in group order" );
+            // every ray mask then admits the one class there is
+            RG_REQUIRE( ( slots[ k ].p->rayCullMaskWorld & slots[ k ].p->rayCullMaskWorldShadow & IMASK_WORLD_0 ) != 0u,
+                        RG_RESULT_WRONG_FUNCTION_ARGUMENT, "grouped frames must see world 0 with every ray" );
+        }
     }
     // Each frame's rays, hits and pending records are one slice of the
     // wavefront: frame k's pixel i is entry off[k] + i, so a traversal
@@ -2366,11 +2392,10 @@ uint64_t TraceFrames( const TraceSlot* slots, uint32_t B, const BvhObject& bvh, 
         totalG += size_t( gw[ k ] ) * gh[ k ];
     }
     RG_REQUIRE( total <= wf.capPx, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "the batch's pixels exceed the wavefront capacity" );
-    PathTracer&     lead  = *slots[ 0 ].pt; // times and counts the batch's traversal launches
-    const TexView   tex   = scene.tex;
-    const AlphaRec* alpha = scene.alphaTested ? scene.alphaRecs.as< AlphaRec >() : nullptr;
-    const SceneView sv{ scene.tris.as< float >(), scene.triNormals.as< float >(), scene.triGeom.as< uint32_t >(), scene.geoms.as< GeomRecord >(),
-                        scene.vertices.as< RgPrimitiveVertex >(), scene.indices.as< uint32_t >(), tex };
+    PathTracer&        lead  = *slots[ 0 ].pt; // times and counts the batch's traversal launches
+    const DeviceScene& scene = grouped ? *slots[ 0 ].scene : *sharedScene; // scene-wide switches (equal in a group)
+    const TexView      tex   = scene.tex;
+    const AlphaRec*    alpha = !grouped && scene.alphaTested ? scene.alphaRecs.as< AlphaRec >() : nullptr;
     RgExtRay* R  = wf.rays.as< RgExtRay >();
     RgExtHit* Hh = wf.hits.as< RgExtHit >();
     RgExtRay* R2 = wf.rays2.as< RgExtRay >();
@@ -2382,7 +2407,7 @@ uint64_t TraceFrames( const TraceSlot* slots, uint32_t B, const BvhObject& bvh, 
     uint8_t*  M2 = wf.masks2.as< uint8_t >();
     uint8_t*  MB = wf.masksB.as< uint8_t >();
     uint64_t  traced = 0;
-    const bool haveGeometry = scene.triCount > 0 && !bvh.frameSegs.empty();
+    const bool haveGeometry = grouped ? bvh.nodeCount > 0 : scene.triCount > 0 && !bvh.frameSegs.empty();
 
     lead.traceLaunches = 0;
     RG_HIP_CHECK( hipMemsetAsync( lead.frameAccum.ptr, 0, 3 * sizeof( uint64_t ), s ) );
@@ -2396,14 +2421,35 @@ uint64_t TraceFrames( const TraceSlot* slots, uint32_t B, const BvhObject& bvh, 
     {
         classMasks[ c ] = GeomClassPresent( c, p0.rayCullMaskWorld ) ? GeomClassMask( c ) : 0u;
     }
-    // rr's rays carry their cull masks in mm (null: every ray has mask)
-    auto trace = [ & ]( RgExtRay* rr, const uint8_t* mm, uint32_t mask, RgExtHit* hh, uint32_t flags, size_t count ) {
+    // rr's rays carry their cull masks in mm (null: every ray has mask);
+    // frame k's rays start at offs[k].  A grouped launch gives each scene of
+    // the group the rays of its frames (plain scenes: the masks admit every
+    // triangle, null rays have an empty interval).
+    std::vector< uint32_t > groupOffs;
+    auto trace = [ & ]( RgExtRay* rr, const uint8_t* mm, uint32_t mask, RgExtHit* hh, uint32_t flags, size_t count, const size_t* offs ) {
         if( haveGeometry )
         {
-            const uint32_t k = lead.traceLaunches < PathTracer::TRACE_EVENT_PAIRS ? lead.traceLaunches : PathTracer::TRACE_EVENT_PAIRS - 1;
-            TraceRaysFrame( bvh, tc, rr, mm, mask, uint32_t( count ), hh,
-                            flags | traceFlags | ( lead.collectStats ? RG_EXT_TRACE_COLLECT_STATS : 0u ), classMasks, GC_COUNT, s,
-                            lead.traceEv[ 2 * k ], lead.traceEv[ 2 * k + 1 ], &accum, alpha, &tex );
+            const uint32_t k  = lead.traceLaunches < PathTracer::TRACE_EVENT_PAIRS ? lead.traceLaunches : PathTracer::TRACE_EVENT_PAIRS - 1;
+            const uint32_t fl = flags | traceFlags | ( lead.collectStats ? RG_EXT_TRACE_COLLECT_STATS : 0u );
+            if( grouped )
+            {
+                groupOffs.assign( bvh.sceneCount + 1, uint32_t( count ) );
+                uint32_t f = 0;
+                for( uint32_t g = 0; g < bvh.sceneCount; g++ )
+                {
+                    while( f < B && slots[ f ].group < g )
+                    {
+                        f++;
+                    }
+                    groupOffs[ g ] = f < B ? uint32_t( offs[ f ] ) : uint32_t( count );
+                }
+                TraceRaysBatch( bvh, tc, rr, groupOffs.data(), hh, fl, s, lead.traceEv[ 2 * k ], lead.traceEv[ 2 * k + 1 ], &accum );
+            }
+            else
+            {
+                TraceRaysFrame( bvh, tc, rr, mm, mask, uint32_t( count ), hh, fl, classMasks, GC_COUNT, s, lead.traceEv[ 2 * k ],
+                                lead.traceEv[ 2 * k + 1 ], &accum, alpha, &tex );
+            }
             lead.traceLaunches = k + 1;
         }
         else
@@ -2417,6 +2463,8 @@ uint64_t TraceFrames( const TraceSlot* slots, uint32_t B, const BvhObject& bvh, 
     {
         const RgExtFrameParams* p;
         PathTracer*             pt;
+        const DeviceScene*      sc;
+        SceneView               sv;
         FbPtrs                  P;
         LightView               lv;
         dim3                    grid2;
@@ -2428,6 +2476,10 @@ uint64_t TraceFrames( const TraceSlot* slots, uint32_t B, const BvhObject& bvh, 
         PathTracer& pt = *slots[ k ].pt;
         f[ k ].p     = slots[ k ].p;
         f[ k ].pt    = &pt;
+        f[ k ].sc    = grouped ? slots[ k ].scene : sharedScene;
+        const DeviceScene& sk = *f[ k ].sc;
+        f[ k ].sv    = SceneView{ sk.tris.as< float >(), sk.triNormals.as< float >(), sk.triGeom.as< uint32_t >(), sk.geoms.as< GeomRecord >(),
+                                  sk.vertices.as< RgPrimitiveVertex >(), sk.indices.as< uint32_t >(), sk.tex };
         f[ k ].P     = slots[ k ].fb->Ptrs();
         f[ k ].lv    = LightView{ pt.lights.as< LightEncoded >(), pt.lightArraySize, pt.prevToCur.as< uint32_t >(), pt.prevToCurCount,
                                   pt.grid[ pt.gridParity ].as< uint4 >(), pt.grid[ pt.gridParity ^ 1 ].as< uint4 >(), pt.blueNoise };
@@ -2447,9 +2499,9 @@ uint64_t TraceFrames( const TraceSlot* slots, uint32_t B, const BvhObject& bvh, 
         k_primary_rays<<< q.grid2, THREADS, 0, s >>>( *q.p, R + q.o );
         CheckLaunch( "k_primary_rays" );
     } );
-    trace( R, nullptr, PrimaryCullMask( p0 ), Hh, RG_EXT_TRACE_CLOSEST, total );
+    trace( R, nullptr, PrimaryCullMask( p0 ), Hh, RG_EXT_TRACE_CLOSEST, total, off );
     each( [ & ]( Frame& q ) {
-        ( tex.count > 0 ? k_primary_shade< true > : k_primary_shade< false > )<<< q.grid2, THREADS, 0, s >>>( *q.p, Hh + q.o, sv, q.P );
+        ( q.sv.tex.count > 0 ? k_primary_shade< true > : k_primary_shade< false > )<<< q.grid2, THREADS, 0, s >>>( *q.p, Hh + q.o, q.sv, q.P );
         CheckLaunch( "k_primary_shade" );
     } );
 
@@ -2461,14 +2513,14 @@ uint64_t TraceFrames( const TraceSlot* slots, uint32_t B, const BvhObject& bvh, 
         {
             each( [ & ]( Frame& q ) {
                 RrState st{ PA + 6 * q.o, q.n };
-                ( tex.count > 0 ? k_reflrefr_a< true > : k_reflrefr_a< false > )<<< q.grid2, THREADS, 0, s >>>( *q.p, q.P, sv, R + q.o, M + q.o,
+                ( q.sv.tex.count > 0 ? k_reflrefr_a< true > : k_reflrefr_a< false > )<<< q.grid2, THREADS, 0, s >>>( *q.p, q.P, q.sv, R + q.o, M + q.o,
                                                                                                                 Hh + q.o, st, d );
                 CheckLaunch( "k_reflrefr_a" );
             } );
-            trace( R, M, 0u, Hh, RG_EXT_TRACE_CLOSEST, total );
+            trace( R, M, 0u, Hh, RG_EXT_TRACE_CLOSEST, total, off );
             each( [ & ]( Frame& q ) {
                 RrState st{ PA + 6 * q.o, q.n };
-                ( tex.count > 0 ? k_reflrefr_b< true > : k_reflrefr_b< false > )<<< q.grid2, THREADS, 0, s >>>( *q.p, q.P, sv, R + q.o, Hh + q.o,
+                ( q.sv.tex.count > 0 ? k_reflrefr_b< true > : k_reflrefr_b< false > )<<< q.grid2, THREADS, 0, s >>>( *q.p, q.P, q.sv, R + q.o, Hh + q.o,
                                                                                                                 st, d );
                 CheckLaunch( "k_reflrefr_b" );
             } );
@@ -2498,7 +2550,7 @@ uint64_t TraceFrames( const TraceSlot* slots, uint32_t B, const BvhObject& bvh, 
     if( p0.directionalLightExists )
     {
         // without a directional light every ray is null and k_initial_b reads no hit
-        trace( R, M, 0u, Hh, RG_EXT_TRACE_ANY_HIT, total );
+        trace( R, M, 0u, Hh, RG_EXT_TRACE_ANY_HIT, total, off );
     }
     each( [ & ]( Frame& q ) {
         k_initial_b<<< q.grid2, THREADS, 0, s >>>( *q.p, q.P, Hh + q.o, reinterpret_cast< uint4* >( PA + 6 * q.o ) );
@@ -2510,7 +2562,7 @@ uint64_t TraceFrames( const TraceSlot* slots, uint32_t B, const BvhObject& bvh, 
         k_direct_a<<< q.grid2, THREADS, 0, s >>>( *q.p, q.P, q.lv, R + q.o, M + q.o, PA + 6 * q.o, PA + 6 * q.o + q.n );
         CheckLaunch( "k_direct_a" );
     } );
-    trace( R, M, 0u, Hh, RG_EXT_TRACE_ANY_HIT, total );
+    trace( R, M, 0u, Hh, RG_EXT_TRACE_ANY_HIT, total, off );
     each( [ & ]( Frame& q ) {
         k_direct_b<<< q.grid2, THREADS, 0, s >>>( *q.p, q.P, R + q.o, Hh + q.o, PA + 6 * q.o, PA + 6 * q.o + q.n );
         CheckLaunch( "k_direct_b" );
@@ -2521,17 +2573,17 @@ uint64_t TraceFrames( const TraceSlot* slots, uint32_t B, const BvhObject& bvh, 
         k_indirect_a<<< q.grid2, THREADS, 0, s >>>( *q.p, q.P, q.lv, R + q.o, M + q.o, IndirPend{ PA + 6 * q.o, q.n } );
         CheckLaunch( "k_indirect_a" );
     } );
-    trace( R, M, 0u, Hh, RG_EXT_TRACE_CLOSEST, total );
+    trace( R, M, 0u, Hh, RG_EXT_TRACE_CLOSEST, total, off );
     each( [ & ]( Frame& q ) {
-        ( tex.count > 0 ? k_indirect_b< true > : k_indirect_b< false > )<<< q.grid2, THREADS, 0, s >>>(
-            *q.p, q.P, sv, q.lv, R + q.o, Hh + q.o, PB + q.o, MB + q.o, R2 + q.o, M2 + q.o, IndirPend{ PA + 6 * q.o, q.n } );
+        ( q.sv.tex.count > 0 ? k_indirect_b< true > : k_indirect_b< false > )<<< q.grid2, THREADS, 0, s >>>(
+            *q.p, q.P, q.sv, q.lv, R + q.o, Hh + q.o, PB + q.o, MB + q.o, R2 + q.o, M2 + q.o, IndirPend{ PA + 6 * q.o, q.n } );
         CheckLaunch( "k_indirect_b" );
     } );
-    trace( PB, MB, 0u, PC, RG_EXT_TRACE_ANY_HIT, total );
-    trace( R2, M2, 0u, H2, RG_EXT_TRACE_CLOSEST, total );
+    trace( PB, MB, 0u, PC, RG_EXT_TRACE_ANY_HIT, total, off );
+    trace( R2, M2, 0u, H2, RG_EXT_TRACE_CLOSEST, total, off );
     each( [ & ]( Frame& q ) {
-        ( tex.count > 0 ? k_indirect_c< true > : k_indirect_c< false > )<<< q.grid2, THREADS, 0, s >>>(
-            *q.p, q.P, sv, q.lv, PC + q.o, R2 + q.o, H2 + q.o, Hh + q.o, IndirPend{ PA + 6 * q.o, q.n } );
+        ( q.sv.tex.count > 0 ? k_indirect_c< true > : k_indirect_c< false > )<<< q.grid2, THREADS, 0, s >>>(
+            *q.p, q.P, q.sv, q.lv, PC + q.o, R2 + q.o, H2 + q.o, Hh + q.o, IndirPend{ PA + 6 * q.o, q.n } );
         CheckLaunch( "k_indirect_c" );
         k_indirect_final<<< q.grid2, THREADS, 0, s >>>( *q.p, q.P );
         CheckLaunch( "k_indirect_final" );
@@ -2545,11 +2597,11 @@ uint64_t TraceFrames( const TraceSlot* slots, uint32_t B, const BvhObject& bvh, 
         const dim3 gridG( DivUp( gw[ k ], BX ), DivUp( gh[ k ], BY ) );
         float4*    lum = PA + 6 * q.o;
         GradPend   gp{ lum, reinterpret_cast< uint4* >( lum + size_t( gw[ k ] ) * gh[ k ] ) };
-        k_grad_a<<< gridG, THREADS, 0, s >>>( *q.p, q.P, sv, q.lv, q.pt->geomMap.as< uint32_t >(), q.pt->geomMapCount, scene.geomCount,
+        k_grad_a<<< gridG, THREADS, 0, s >>>( *q.p, q.P, q.sv, q.lv, q.pt->geomMap.as< uint32_t >(), q.pt->geomMapCount, q.sc->geomCount,
                                              R + offG[ k ], M + offG[ k ], gp );
         CheckLaunch( "k_grad_a" );
     }
-    trace( R, M, 0u, Hh, RG_EXT_TRACE_ANY_HIT, totalG );
+    trace( R, M, 0u, Hh, RG_EXT_TRACE_ANY_HIT, totalG, offG );
     for( uint32_t k = 0; k < B; k++ )
     {
         Frame&     q = f[ k ];

@@ -8,8 +8,10 @@
 // passes to the final compose, as EncodeFrames runs it.  Within-frame planes
 // and wavefront buffers belong to a stream, so a session costs only what it
 // carries from frame to frame.  One launch renders a batch of waiting
-// requests of sessions of one scene: every traversal stage of the batch is
-// one persistent launch over all its frames' rays.
+// requests of sessions of one scene, or of the scenes of one scene group:
+// every traversal stage of the batch is one persistent launch over all its
+// frames' rays (a group's rays each start at their own scene's root of the
+// group's batched BVH).
 #include "rg_frame.h"
 #include "rg_internal.h"
 #include "rg_kernels.h"
@@ -120,6 +122,7 @@ struct FrameRequest
 using namespace rg;
 
 struct RgExtServer_T;
+struct RgExtSceneGroup_T;
 
 struct RgExtScene_T
 {
@@ -131,6 +134,19 @@ struct RgExtScene_T
     bool                    built    = false;
     uint32_t                sessions = 0;
     size_t                  bytes    = 0;
+    RgExtSceneGroup_T*      group      = nullptr; // traced through the group's BVH while set
+    uint32_t                groupIndex = 0;       // the scene's number in the group's BVH
+    bool                    joining    = false;   // a group with it is being built
+};
+
+// Scenes whose frames share launches: one batched BVH over their triangles
+// (scene k = subtree k).
+struct RgExtSceneGroup_T
+{
+    RgExtServer_T*                server = nullptr;
+    std::vector< RgExtScene_T* > members;
+    BvhObject                    bvh;
+    size_t                       bytes = 0;
 };
 
 struct RgExtSession_T
@@ -216,6 +232,7 @@ struct RgExtServer_T
     hipStream_t                                 setup      = nullptr; // scene builds and session set-up
     std::vector< std::unique_ptr< StreamCtx > > streams;
     std::list< std::unique_ptr< RgExtScene_T > >   scenes;
+    std::list< std::unique_ptr< RgExtSceneGroup_T > > groups;
     std::list< std::unique_ptr< RgExtSession_T > > sessions;
     std::deque< RgExtSession_T* >                  ready; // sessions with a waiting request
     std::deque< RgExtFrameResult >              results;
@@ -337,8 +354,9 @@ struct RgExtServer_T
 
     // Called with the lock held: picks the next batch (the lead request by
     // the schedule, then up to batchFrames - 1 more waiting requests of the
-    // same scene and batch key that may run on the lead's stream, oldest
-    // first) and launches it.  False when no stream has room for a lead.
+    // same scene, or of a scene of the lead scene's group, and batch key
+    // that may run on the lead's stream, oldest first) and launches it.
+    // False when no stream has room for a lead.
     bool LaunchNext( std::unique_lock< std::mutex >& lk )
     {
         auto lead = ready.end();
@@ -362,6 +380,10 @@ struct RgExtServer_T
         RgExtSession_T*                s0     = *lead;
         const uint32_t                 target = TargetStream( *s0 );
         const uint64_t                 key    = s0->pending->BatchKey();
+        // a frame that culls world 0 sees nothing of a grouped scene: it
+        // traces the scene's own BVH, which drops that class the usual way
+        RgExtSceneGroup_T* const grp =
+            ( s0->pending->di.rayCullMaskWorld & RG_DRAW_FRAME_RAY_CULL_WORLD_0_BIT ) ? s0->scene->group : nullptr;
         std::vector< RgExtSession_T* > batch{ s0 };
         ready.erase( lead );
         if( info.batchFrames > 1 )
@@ -369,7 +391,8 @@ struct RgExtServer_T
             std::vector< RgExtSession_T* > cand;
             for( RgExtSession_T* s : ready )
             {
-                if( s->scene == s0->scene && s->pending->BatchKey() == key && ( s->inflight == 0 || s->stream == target ) )
+                const bool sameTrace = grp ? s->scene->group == grp : s->scene == s0->scene;
+                if( sameTrace && s->pending->BatchKey() == key && ( s->inflight == 0 || s->stream == target ) )
                 {
                     cand.push_back( s );
                 }
@@ -386,7 +409,13 @@ struct RgExtServer_T
                 ready.erase( std::find( ready.begin(), ready.end(), s ) );
             }
         }
-        LaunchBatch( batch, *streams[ target ], target, lk );
+        if( grp )
+        {
+            // a grouped launch takes each scene's rays as one range
+            std::stable_sort( batch.begin(), batch.end(),
+                              []( const RgExtSession_T* a, const RgExtSession_T* b ) { return a->scene->groupIndex < b->scene->groupIndex; } );
+        }
+        LaunchBatch( batch, grp, *streams[ target ], target, lk );
         return true;
     }
 
@@ -395,8 +424,12 @@ struct RgExtServer_T
     // encoding.  Each session counts its frame as in flight meanwhile (a
     // destroy waits for it) and the server stays busy (WaitIdle waits).
     // Only this thread launches frames and touches the sessions' cameras,
-    // framebuffers and the stream's state.
-    void LaunchBatch( const std::vector< RgExtSession_T* >& batch, StreamCtx& st, uint32_t target, std::unique_lock< std::mutex >& lk )
+    // framebuffers and the stream's state.  grp: the group whose BVH the
+    // batch traces (its scenes' frames, in group order), or null for the
+    // frames of one ungrouped scene; a group outlives the launches that
+    // took it (rgExtSceneGroupDestroy waits for them).
+    void LaunchBatch( const std::vector< RgExtSession_T* >& batch, RgExtSceneGroup_T* grp, StreamCtx& st, uint32_t target,
+                      std::unique_lock< std::mutex >& lk )
     {
         const uint32_t              B = uint32_t( batch.size() );
         std::vector< FrameRequest > req( B );
@@ -419,11 +452,11 @@ struct RgExtServer_T
         std::vector< StreamCtx::Member > members;
         try
         {
-            RgExtScene_T& sc = *batch[ 0 ]->scene;
-            EncodeSlot    slots[ MAX_TRACE_SLOTS ];
+            EncodeSlot slots[ MAX_TRACE_SLOTS ];
             for( uint32_t k = 0; k < B; k++ )
             {
-                RgExtSession_T& s = *batch[ k ];
+                RgExtSession_T& s  = *batch[ k ];
+                RgExtScene_T&   sc = *s.scene;
                 FillFrameParams( s.cam, req[ k ].di, s.width, s.height, uint32_t( sc.sc.lights.size() ), uint32_t( sc.sc.lights.size() ),
                                  sc.sc.hasDirectional, false );
                 if( !s.cam.firstFrame )
@@ -431,9 +464,16 @@ struct RgExtServer_T
                     s.fbs.fb.Swap();
                 }
                 s.fbs.fb.shared = st.planes[ k ].get();
-                slots[ k ]      = EncodeSlot{ &s.g, &s.cam.params, &s.fbs.fb, &req[ k ].di };
+                slots[ k ]      = EncodeSlot{ &s.g, &s.cam.params, &s.fbs.fb, &req[ k ].di, &sc.ds, sc.groupIndex };
             }
-            EncodeFrames( slots, B, st.tc, sc.bvh, sc.ds, st.wf, st.stream );
+            if( grp )
+            {
+                EncodeFrames( slots, B, st.tc, grp->bvh, nullptr, st.wf, st.stream );
+            }
+            else
+            {
+                EncodeFrames( slots, B, st.tc, batch[ 0 ]->scene->bvh, &batch[ 0 ]->scene->ds, st.wf, st.stream );
+            }
             size_t packOff = 0;
             for( uint32_t k = 0; k < B; k++ )
             {
@@ -543,6 +583,7 @@ struct RgExtServer_T
             worker.join();
         }
         sessions.clear();
+        groups.clear();
         scenes.clear();
         streams.clear();
         if( setup )
@@ -720,6 +761,7 @@ RgResult rgExtSceneDestroy( RgExtScene scene )
         {
             std::lock_guard< std::mutex > lk( s.m );
             RG_REQUIRE( scene->sessions == 0, RG_RESULT_WRONG_FUNCTION_CALL, "sessions still use the scene" );
+            RG_REQUIRE( !scene->group && !scene->joining, RG_RESULT_WRONG_FUNCTION_CALL, "the scene is a member of a group" );
             for( auto it = s.scenes.begin(); it != s.scenes.end(); ++it )
             {
                 if( it->get() == scene )
@@ -731,6 +773,109 @@ RgResult rgExtSceneDestroy( RgExtScene scene )
             }
             s.sceneBytes -= scene->bytes;
         }
+    } );
+}
+
+RgResult rgExtSceneGroupCreate( RgExtServer server, const RgExtScene* pScenes, uint32_t sceneCount, RgExtSceneGroup* pOut )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( pScenes && pOut, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        RgExtServer_T& s = Srv( server );
+        RG_REQUIRE( sceneCount >= 2 && sceneCount <= 128, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "a group has 2..128 scenes" );
+        auto grp    = std::make_unique< RgExtSceneGroup_T >();
+        grp->server = &s;
+        std::vector< uint32_t > offsets( 1, 0u );
+        {
+            std::lock_guard< std::mutex > lk( s.m );
+            for( uint32_t k = 0; k < sceneCount; k++ )
+            {
+                RgExtScene_T* sc = pScenes[ k ];
+                RG_REQUIRE( sc && sc->server == &s, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "a scene is null or of another server" );
+                RG_REQUIRE( sc->built, RG_RESULT_WRONG_FUNCTION_CALL, "a scene is not built" );
+                RG_REQUIRE( !sc->group && !sc->joining, RG_RESULT_WRONG_FUNCTION_CALL, "a scene is already in a group" );
+                RG_REQUIRE( std::find( grp->members.begin(), grp->members.end(), sc ) == grp->members.end(), RG_RESULT_WRONG_FUNCTION_ARGUMENT,
+                            "a scene is listed twice" );
+                RG_REQUIRE( GroupableScene( sc->ds ), RG_RESULT_WRONG_FUNCTION_ARGUMENT,
+                            "a scene has primitives other than opaque world-0 geometry" );
+                RG_REQUIRE( sc->ds.triCount >= 2, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "a scene has fewer than 2 triangles" );
+                const RgExtScene_T& first = *pScenes[ 0 ];
+                // lights[0] is the directional light's slot, the rest are
+                // the other lights (TraceCompatible's light kinds)
+                RG_REQUIRE( sc->sc.hasDirectional == first.sc.hasDirectional && ( sc->sc.lights.size() > 1 ) == ( first.sc.lights.size() > 1 ),
+                            RG_RESULT_WRONG_FUNCTION_ARGUMENT, "the scenes differ in their light kinds" );
+                grp->members.push_back( sc );
+                offsets.push_back( offsets.back() + sc->ds.triCount );
+            }
+            for( RgExtScene_T* sc : grp->members )
+            {
+                sc->joining = true; // no other group takes it, no destroy frees it
+            }
+        }
+        try
+        {
+            // the members' triangles, scene after scene, for the batched build
+            DeviceBuffer tris;
+            tris.Reserve( size_t( offsets.back() ) * 9 * sizeof( float ) );
+            for( uint32_t k = 0; k < sceneCount; k++ )
+            {
+                RG_HIP_CHECK( hipMemcpyAsync( tris.as< float >() + size_t( offsets[ k ] ) * 9, grp->members[ k ]->ds.tris.ptr,
+                                              size_t( grp->members[ k ]->ds.triCount ) * 9 * sizeof( float ), hipMemcpyDeviceToDevice, s.setup ) );
+            }
+            const RgExtBvhBuildParams bp = { 1, s.nodeFormat };
+            BuildBvhBatch( grp->bvh, tris.as< float >(), offsets.data(), sceneCount, bp, s.setup );
+            RG_HIP_CHECK( hipStreamSynchronize( s.setup ) );
+        }
+        catch( ... )
+        {
+            std::lock_guard< std::mutex > lk( s.m );
+            for( RgExtScene_T* sc : grp->members )
+            {
+                sc->joining = false;
+            }
+            throw;
+        }
+        grp->bytes = size_t( grp->bvh.nodeCount ) * grp->bvh.NodeStride() + size_t( offsets.back() ) * 48;
+        RgExtSceneGroup out = grp.get();
+        std::lock_guard< std::mutex > lk( s.m );
+        for( uint32_t k = 0; k < sceneCount; k++ )
+        {
+            grp->members[ k ]->joining    = false;
+            grp->members[ k ]->group      = out;
+            grp->members[ k ]->groupIndex = k;
+        }
+        s.sceneBytes += grp->bytes;
+        s.groups.push_back( std::move( grp ) );
+        *pOut = out;
+    } );
+}
+
+RgResult rgExtSceneGroupDestroy( RgExtSceneGroup group )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( group, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "group is null" );
+        RgExtServer_T& s = *group->server;
+        {
+            // later launches trace the scenes' own BVHs
+            std::lock_guard< std::mutex > lk( s.m );
+            for( RgExtScene_T* sc : group->members )
+            {
+                sc->group      = nullptr;
+                sc->groupIndex = 0;
+            }
+        }
+        s.WaitIdle(); // launches that took the group have finished
+        std::unique_ptr< RgExtSceneGroup_T > keep;
+        std::lock_guard< std::mutex >        lk( s.m );
+        for( auto it = s.groups.begin(); it != s.groups.end(); ++it )
+        {
+            if( it->get() == group )
+            {
+                keep = std::move( *it );
+                s.groups.erase( it );
+                break;
+            }
+        }
+        s.sceneBytes -= group->bytes;
     } );
 }
 

@@ -1000,7 +1000,7 @@ void TraceRaysFrame( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_ra
 }
 
 void TraceRaysBatch( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, const uint32_t* rayOffsets, RgExtHit* d_hits, uint32_t flags,
-                     hipStream_t stream, hipEvent_t evBefore, hipEvent_t evAfter )
+                     hipStream_t stream, hipEvent_t evBefore, hipEvent_t evAfter, const TraceAccum* accum )
 {
     const uint32_t S = bvh.sceneCount;
     RG_REQUIRE( S > 0, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "the BVH was not built as a batch" );
@@ -1012,7 +1012,7 @@ void TraceRaysBatch( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_ra
     s.segHost.assign( rayOffsets, rayOffsets + S + 1 ); // kept alive until the copy has run
     s.segRays.Reserve( size_t( S + 1 ) * 4 );
     RG_HIP_CHECK( hipMemcpyAsync( s.segRays.ptr, s.segHost.data(), size_t( S + 1 ) * 4, hipMemcpyHostToDevice, stream ) );
-    TraceRaysImpl( bvh, s, d_rays, rayOffsets[ S ], d_hits, flags, stream, evBefore, evAfter, nullptr, s.segRays.as< uint32_t >() );
+    TraceRaysImpl( bvh, s, d_rays, rayOffsets[ S ], d_hits, flags, stream, evBefore, evAfter, accum, s.segRays.as< uint32_t >() );
 }
 
 void AutotuneTrace( const BvhObject& bvh, TraceContext& s, const RgExtRay* d_rays, uint32_t count, const uint32_t* rayOffsets, RgExtHit* d_hits,

@@ -210,6 +210,20 @@ def soup_tris(seed: int, n: int, extent: float = 100.0) -> np.ndarray:
     return np.concatenate(parts).astype(np.float32)
 
 
+def soup_scene(tris: np.ndarray) -> Scene:
+    """A soup (soup_tris) as a plain scene: face normals, grey surfaces, a
+    sun; one object per 65,536 triangles."""
+    t = tris.reshape(-1, 3, 3).astype(np.float64)
+    n = np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0])
+    n /= np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-30)
+    nrm = np.repeat(n[:, None, :], 3, 1).reshape(-1, 9).astype(np.float32)
+    cols = np.full(len(tris), pack_rgba8(0.6, 0.6, 0.6), np.uint32)
+    step = 65536
+    objects = [(f"soup{i // step}", i, min(step, len(tris) - i)) for i in range(0, len(tris), step)]
+    sun = {"direction": (-1.0, -6.0, -2.0), "angularDiameterDegrees": 0.5, "intensity": 7000.0, "color": (1.0, 1.0, 1.0)}
+    return Scene(np.ascontiguousarray(tris, np.float32), nrm, cols, objects, {"sun": sun, "spheres": []})
+
+
 def _finish(parts, objects, lights=None) -> Scene:
     tris = np.concatenate([p[0] for p in parts]).astype(np.float32)
     nrms = np.concatenate([p[1] for p in parts]).astype(np.float32)

@@ -54,6 +54,8 @@ _SIGS = {
     "rgExtSceneCreate": [_P, C.POINTER(_P)],
     "rgExtSceneBuild": [_P],
     "rgExtSceneDestroy": [_P],
+    "rgExtSceneGroupCreate": [_P, C.POINTER(_P), _U32, C.POINTER(_P)],
+    "rgExtSceneGroupDestroy": [_P],
     "rgExtSessionCreate": [_P, _U32, _U32, C.POINTER(_P)],
     "rgExtSessionDestroy": [_P],
     "rgExtSessionSubmit": [_P, C.POINTER(rtgl.RgDrawFrameInfo), C.c_uint64],
@@ -126,6 +128,16 @@ class Server:
         rtgl.SceneUploader(scene).upload(h.value, True, prefix="rgExtScene")
         ffi.check("rgExtSceneBuild", l.rgExtSceneBuild(h.value))
         return h.value
+
+    def group(self, scene_handles: list) -> int:
+        """rgExtSceneGroupCreate: frames of these scenes may share launches."""
+        arr = (_P * len(scene_handles))(*scene_handles)
+        h = _P()
+        ffi.check("rgExtSceneGroupCreate", _lib().rgExtSceneGroupCreate(self.h, arr, len(scene_handles), C.byref(h)))
+        return h.value
+
+    def destroy_group(self, group: int) -> None:
+        ffi.check("rgExtSceneGroupDestroy", _lib().rgExtSceneGroupDestroy(group))
 
     def session(self, scene: int, width: int, height: int) -> Session:
         h = _P()

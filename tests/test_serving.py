@@ -13,6 +13,13 @@
 - Batched launches (batchFrames > 1: one traversal launch per stage over
   the rays of several sessions of a scene) render every session's frames
   bit for bit as alone.
+- Scene groups: sessions of two scenes in one group share launches; every
+  session's frames are bit for bit its frames rendered alone through the
+  group and its frames through its scene's own BVH (the group's BVH is
+  another tree over the same triangles, and the closest-hit rule orders
+  hits by distance, then primitive, whatever the visiting order).  Group
+  creation refuses scenes with other light kinds, and a grouped scene
+  cannot be destroyed.
 - Readback: a finished frame's pImage holds FINAL as RGBA8 with the sRGB
   transfer (within one code of a float64 encoding of the FINAL plane).
 - Scheduling: NEWEST_FIRST starts the newest waiting request first, and
@@ -194,6 +201,79 @@ def test_batched_sessions_match_alone():
     for k in range(len(BATCH_SPECS)):
         for p in PERSISTENT:
             assert np.array_equal(alone[k][p], got[k][p]), (k, p)
+
+
+GROUP_SPECS = [(0, 160, 90), (1, 128, 72), (0, 200, 100), (1, 160, 90), (0, 128, 72), (1, 200, 100)]
+
+
+def _group_scenes():
+    """Two plain scenes with the same light kinds (a sun each)."""
+    return scenes.terrain_boxes_scene(), scenes.terrain_boxes_scene(seed=scenes.SEED_TERRAIN_BOXES + 7)
+
+
+def _grouped(specs, batch):
+    """specs rendered on one server whose two scenes form a group; returns
+    (planes per session index, batch sizes)."""
+    sc = _group_scenes()
+    srv = serving.Server(256, 128, streams=1, frames_in_flight=1, batch=batch)
+    batches = []
+    try:
+        handles = [srv.scene(s) for s in sc]
+        g = srv.group(handles)
+        sess = [(k, srv.session(handles[si], w, h)) for k, (si, w, h) in specs]
+        got = _render(srv, sess, FRAMES, batches)
+        with pytest.raises(ffi.RgError):  # a grouped scene stays
+            ffi.check("rgExtSceneDestroy", serving._lib().rgExtSceneDestroy(handles[0]))
+        srv.destroy_group(g)
+    finally:
+        srv.close()
+    return got, batches
+
+
+def test_scene_group_batches_match_alone():
+    print("serving: 6 sessions of two grouped scenes in batches of up to 4", flush=True)
+    specs = list(enumerate(GROUP_SPECS))
+    alone = {}
+    for k, spec in specs:
+        alone.update(_grouped([(k, spec)], 1)[0])
+    got, batches = _grouped(specs, 4)
+    print(f"  batch sizes of the finished frames: {sorted(batches)}", flush=True)
+    # 6 sessions, 3 per scene: a batch of 4 holds frames of both scenes
+    assert max(batches) == 4
+    for k in range(len(GROUP_SPECS)):
+        for p in PERSISTENT:
+            assert np.array_equal(alone[k][p], got[k][p]), (k, p)
+    # the same sessions through each scene's own BVH
+    sc = _group_scenes()
+    own = {}
+    for k, (si, w, h) in specs:
+        srv = serving.Server(256, 128, streams=1)
+        try:
+            own.update(_render(srv, [(k, srv.session(srv.scene(sc[si]), w, h))], FRAMES))
+        finally:
+            srv.close()
+    for k in range(len(GROUP_SPECS)):
+        for p in PERSISTENT:
+            assert np.array_equal(own[k][p], got[k][p]), (k, p)
+
+
+def test_scene_group_refuses_other_light_kinds():
+    a, _ = _scenes()
+    _, b = _scenes()  # a sun and sphere lights
+    srv = serving.Server(160, 90, streams=1)
+    try:
+        ha, hb = srv.scene(a), srv.scene(b)
+        with pytest.raises(ffi.RgError):
+            srv.group([ha, hb])
+        with pytest.raises(ffi.RgError):
+            srv.group([ha])
+        g = srv.group([ha, srv.scene(scenes.terrain_boxes_scene(seed=scenes.SEED_TERRAIN_BOXES + 9))])
+        with pytest.raises(ffi.RgError):
+            srv.group([ha, hb])  # ha is in a group already
+        srv.destroy_group(g)
+        ffi.check("rgExtSceneDestroy", serving._lib().rgExtSceneDestroy(ha))
+    finally:
+        srv.close()
 
 
 def _srgb8(c):
