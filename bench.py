@@ -231,6 +231,16 @@ def run_frames(a, scene, precision: int, flags=None, planes=("PRE_FINAL",)):
             step(geometry=(i == 0))
             print(f"warmup {i + 1}/{max(1, a.warmup)} (precision flags {precision})", file=sys.stderr, flush=True)
         ffi.call("rgExtSynchronize")
+        # the a-trous workgroup tile, tuned on the warm frame's planes (the
+        # timed frames use the kept width; results do not depend on it)
+        import ctypes as C
+        width, ms = C.c_uint32(0), (C.c_float * 4)()
+        ffi.call("rgExtDenoiseAutotune", r.framebuffers().h, r.frame_params(), None, C.byref(width), ms)
+        tune = {"tile_width": width.value, "tile": f"{width.value}x{256 // width.value}",
+                "iteration1_ms": {f"{w}x{256 // w}": round(float(m), 4) for w, m in zip((16, 32, 64, 128), ms)}}
+        if precision == 0:
+            a.atrous_tune = tune
+        print(f"a-trous tile autotune (precision flags {precision}): {tune}", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         for _ in range(a.steps):
             step(stats=False)  # two frames in flight, as an application runs them
@@ -575,6 +585,7 @@ def main(argv=None) -> int:
             },
         },
         "denoise_kernels": None,
+        "atrous_autotune": getattr(a, "atrous_tune", None),
         "cpu_baseline": None,
     }
     dprof = load_profile(DENOISE_TRAFFIC_FILE)

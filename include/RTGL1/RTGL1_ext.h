@@ -460,6 +460,16 @@ RGAPI RgResult RGCONV rgExtDenoiseVariance( RgExtFramebuffers fb, const RgExtFra
 RGAPI RgResult RGCONV rgExtDenoiseAtrous( RgExtFramebuffers fb, const RgExtFrameParams* p, uint32_t iteration, void* stream );
 /* The whole chain as Denoiser::Denoise runs it. */
 RGAPI RgResult RGCONV rgExtDenoise( RgExtFramebuffers fb, const RgExtFrameParams* p, void* stream );
+/* No counterpart (the reference's compute shaders have fixed 16x16 groups):
+ * the workgroup tile of a-trous iterations >= 1, 256 pixels of tileWidth
+ * 16, 32 (the default), 64 or 128 columns.  Results do not depend on it.
+ * rgExtDenoiseAutotune times iteration 1 on the set's current planes with
+ * each width (it writes that iteration's outputs and synchronises the
+ * stream), keeps the fastest for later a-trous calls on the set and returns
+ * it; pMs (optional) receives the four times in milliseconds, in width
+ * order. */
+RGAPI RgResult RGCONV rgExtFramebuffersSetAtrousTile( RgExtFramebuffers fb, uint32_t tileWidth );
+RGAPI RgResult RGCONV rgExtDenoiseAutotune( RgExtFramebuffers fb, const RgExtFrameParams* p, void* stream, uint32_t* pTileWidth, float* pMs );
 /* Final compose: CmCheckerboard.comp:49-76 and CmPrepareFinal.comp:148-176
  * (ImageComposition.cpp:87-156). PRE_FINAL -> FINAL: checkerboard resolve of
  * split pixels, auto exposure from *d_avgLuminance (Exposure.h:29-56) and

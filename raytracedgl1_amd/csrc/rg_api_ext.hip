@@ -541,4 +541,23 @@ RgResult rgExtDenoise( RgExtFramebuffers fb, const RgExtFrameParams* p, void* st
     RG_DENOISE_ENTRY( Denoise( fb->fb, *p, AsStream( stream ) ) );
 }
 
+RgResult rgExtFramebuffersSetAtrousTile( RgExtFramebuffers fb, uint32_t tileWidth )
+{
+    return Guard( [ & ] {
+        RG_REQUIRE( fb, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "null argument" );
+        SetAtrousTile( fb->fb, tileWidth );
+    } );
+}
+
+RgResult rgExtDenoiseAutotune( RgExtFramebuffers fb, const RgExtFrameParams* p, void* stream, uint32_t* pTileWidth, float* pMs )
+{
+    RG_DENOISE_ENTRY( {
+        const uint32_t w = AutotuneAtrous( fb->fb, *p, AsStream( stream ), pMs );
+        if( pTileWidth )
+        {
+            *pTileWidth = w;
+        }
+    } );
+}
+
 } // extern "C"

@@ -14,4 +14,12 @@ void DenoiseAntiFirefly( FramebufferSet& fb, const RgExtFrameParams& p, hipStrea
 void DenoiseVariance( FramebufferSet& fb, const RgExtFrameParams& p, hipStream_t s, bool writeGuide = true );
 void DenoiseAtrous( FramebufferSet& fb, const RgExtFrameParams& p, uint32_t iteration, hipStream_t s, bool writeGuide = true );
 void Denoise( FramebufferSet& fb, const RgExtFrameParams& p, hipStream_t s );
+// Workgroup tile widths of the a-trous iterations >= 1 (256 threads each).
+constexpr uint32_t ATROUS_TILE_WIDTHS[ 4 ] = { 16, 32, 64, 128 };
+void SetAtrousTile( FramebufferSet& fb, uint32_t tileWidth );
+// Times a-trous iteration 1 on fb's current planes with every tile width
+// (best of 3 after a warm-up, on stream s, which it synchronises) and keeps
+// the fastest in fb.  Writes iteration 1's output planes.  msOut (optional)
+// receives the four times in ATROUS_TILE_WIDTHS order.
+uint32_t AutotuneAtrous( FramebufferSet& fb, const RgExtFrameParams& p, hipStream_t s, float* msOut );
 } // namespace rg

@@ -287,7 +287,13 @@ __device__ __forceinline__ float SpecMinAccumFrames( const RgExtFrameParams& g )
     return clampf( 0.2f * safePositiveRcp( g.timeDelta ), 0.0f, 128.0f );
 }
 
-// CmSVGFTemporalAccumulation.comp:40-86
+// CmSVGFTemporalAccumulation.comp:40-86: 60 tan of the angle between the
+// surface point's view directions of this and the previous frame.  The sine
+// is |v x vPrev| rather than sqrt(1 - cos^2): the same value for unit
+// vectors, but sqrt(1 - cos^2) cancels catastrophically in FP32 at the small
+// angles of a slowly moving camera (cos within an ulp of 1 leaves no digits
+// of the angle), where the cross product keeps every digit.  At cos <= 0
+// the clamped cosine makes the sine 1, as in the shader.
 __device__ float Parallax( const RgExtFrameParams& g, v3 pos, v3 posPrev )
 {
     const v3    camDelta = mk3( g.cameraPositionPrev[ 0 ] - g.cameraPosition[ 0 ],
@@ -295,8 +301,10 @@ __device__ float Parallax( const RgExtFrameParams& g, v3 pos, v3 posPrev )
                              g.cameraPositionPrev[ 2 ] - g.cameraPosition[ 2 ] );
     const v3    v        = normalize( pos );
     const v3    vPrev    = normalize( posPrev - camDelta );
-    const float cosa     = saturate( dot( v, vPrev ) );
-    return sqrtf( 1.0f - cosa * cosa ) / fmaxf( cosa, 1e-6f ) * 60.0f;
+    const float c        = dot( v, vPrev );
+    const float cosa     = saturate( c );
+    const float sina     = c > 0.0f ? fminf( length( cross( v, vPrev ) ), 1.0f ) : 1.0f;
+    return sina / fmaxf( cosa, 1e-6f ) * 60.0f;
 }
 __device__ float SpecAccumSpeed( const RgExtFrameParams& g, float Amax, float roughness, float NoV, float parallax )
 {
@@ -919,11 +927,13 @@ __global__ __launch_bounds__( THREADS ) void k_atrous0( FbPtrs P, Dims d, Atrous
 }
 
 // Iterations >= 1 (steps 2, 4, 8, 16): taps gathered from global memory.
-template< bool F16 >
+// Each pixel is computed on its own, so the workgroup tile (TBX x
+// THREADS / TBX pixels) changes only which tap rows share a cache.
+template< bool F16, int TBX >
 __global__ __launch_bounds__( THREADS ) void k_atrousN( FbPtrs P, Dims d, AtrousPlanes ap, int k )
 {
-    const int x = blockIdx.x * BX + threadIdx.x % BX;
-    const int y = blockIdx.y * BY + threadIdx.x / BX;
+    const int x = blockIdx.x * TBX + threadIdx.x % TBX;
+    const int y = blockIdx.y * ( THREADS / TBX ) + threadIdx.x / TBX;
     const int W = d.W, H = d.H;
     if( x >= W || y >= H )
     {
@@ -1046,6 +1056,35 @@ void DenoiseVariance( FramebufferSet& fb, const RgExtFrameParams& p, hipStream_t
     CheckLaunch( "k_variance" );
 }
 
+namespace
+{
+template< int TBX >
+void LaunchAtrousNTile( FramebufferSet& fb, const Dims& d, const AtrousPlanes& ap, uint32_t k, hipStream_t s )
+{
+    const dim3 grid( DivUp( d.W, TBX ), DivUp( d.H, THREADS / TBX ) );
+    if( fb.Half() )
+    {
+        k_atrousN< true, TBX ><<< grid, THREADS, 0, s >>>( fb.Ptrs(), d, ap, int( k ) );
+    }
+    else
+    {
+        k_atrousN< false, TBX ><<< grid, THREADS, 0, s >>>( fb.Ptrs(), d, ap, int( k ) );
+    }
+    CheckLaunch( "k_atrousN" );
+}
+
+void LaunchAtrousN( FramebufferSet& fb, const Dims& d, const AtrousPlanes& ap, uint32_t k, uint32_t tileW, hipStream_t s )
+{
+    switch( tileW )
+    {
+        case 16: LaunchAtrousNTile< 16 >( fb, d, ap, k, s ); break;
+        case 64: LaunchAtrousNTile< 64 >( fb, d, ap, k, s ); break;
+        case 128: LaunchAtrousNTile< 128 >( fb, d, ap, k, s ); break;
+        default: LaunchAtrousNTile< 32 >( fb, d, ap, k, s ); break;
+    }
+}
+} // namespace
+
 void DenoiseAtrous( FramebufferSet& fb, const RgExtFrameParams& p, uint32_t k, hipStream_t s, bool writeGuide )
 {
     CheckParams( fb, p );
@@ -1071,16 +1110,70 @@ void DenoiseAtrous( FramebufferSet& fb, const RgExtFrameParams& p, uint32_t k, h
     }
     else
     {
-        if( fb.Half() )
-        {
-            k_atrousN< true ><<< grid, THREADS, 0, s >>>( fb.Ptrs(), d, ap, int( k ) );
-        }
-        else
-        {
-            k_atrousN< false ><<< grid, THREADS, 0, s >>>( fb.Ptrs(), d, ap, int( k ) );
-        }
-        CheckLaunch( "k_atrousN" );
+        LaunchAtrousN( fb, d, ap, k, fb.atrousTileW, s );
     }
+}
+
+void SetAtrousTile( FramebufferSet& fb, uint32_t tileWidth )
+{
+    bool known = false;
+    for( uint32_t w : ATROUS_TILE_WIDTHS )
+    {
+        known = known || w == tileWidth;
+    }
+    RG_REQUIRE( known, RG_RESULT_WRONG_FUNCTION_ARGUMENT, "a-trous tile width must be 16, 32, 64 or 128" );
+    fb.atrousTileW = tileWidth;
+}
+
+uint32_t AutotuneAtrous( FramebufferSet& fb, const RgExtFrameParams& p, hipStream_t s, float* msOut )
+{
+    CheckParams( fb, p );
+    WriteGuide( fb, s );
+    const Dims         d  = GetDims( fb );
+    const AtrousPlanes ap = GetAtrousPlanes( 1, p.atrousIterations );
+    hipEvent_t         e0, e1;
+    RG_HIP_CHECK( hipEventCreate( &e0 ) );
+    RG_HIP_CHECK( hipEventCreate( &e1 ) );
+    uint32_t best   = fb.atrousTileW;
+    float    bestMs = 1e30f;
+    try
+    {
+        for( uint32_t i = 0; i < 4; i++ )
+        {
+            const uint32_t w = ATROUS_TILE_WIDTHS[ i ];
+            LaunchAtrousN( fb, d, ap, 1, w, s ); // warm-up
+            float ms = 1e30f;
+            for( int r = 0; r < 3; r++ )
+            {
+                RG_HIP_CHECK( hipEventRecord( e0, s ) );
+                LaunchAtrousN( fb, d, ap, 1, w, s );
+                RG_HIP_CHECK( hipEventRecord( e1, s ) );
+                RG_HIP_CHECK( hipEventSynchronize( e1 ) );
+                float t = 0.0f;
+                RG_HIP_CHECK( hipEventElapsedTime( &t, e0, e1 ) );
+                ms = std::min( ms, t );
+            }
+            if( msOut )
+            {
+                msOut[ i ] = ms;
+            }
+            if( ms < bestMs )
+            {
+                bestMs = ms;
+                best   = w;
+            }
+        }
+    }
+    catch( ... )
+    {
+        ( void )hipEventDestroy( e0 );
+        ( void )hipEventDestroy( e1 );
+        throw;
+    }
+    RG_HIP_CHECK( hipEventDestroy( e0 ) );
+    RG_HIP_CHECK( hipEventDestroy( e1 ) );
+    fb.atrousTileW = best;
+    return best;
 }
 
 void Denoise( FramebufferSet& fb, const RgExtFrameParams& p, hipStream_t s )

@@ -159,6 +159,8 @@ _SIGS = {
     "rgExtDenoiseVariance": (_RES, [_P, C.POINTER(RgExtFrameParams), _P]),
     "rgExtDenoiseAtrous": (_RES, [_P, C.POINTER(RgExtFrameParams), _U32, _P]),
     "rgExtDenoise": (_RES, [_P, C.POINTER(RgExtFrameParams), _P]),
+    "rgExtFramebuffersSetAtrousTile": (_RES, [_P, _U32]),
+    "rgExtDenoiseAutotune": (_RES, [_P, C.POINTER(RgExtFrameParams), _P, C.POINTER(_U32), C.POINTER(C.c_float)]),
 }
 
 

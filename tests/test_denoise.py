@@ -5,6 +5,7 @@ Tolerance: every output channel within 1e-3 relative (denominator
 |ref| + 1e-6 max|ref|) of the FP64 oracle, the bound stated for the denoised
 radiance.
 """
+import ctypes
 import math
 
 import numpy as np
@@ -368,3 +369,32 @@ def test_gpu_full_chain_fp16_storage(gpu, oracle, W, H):
     assert rel.max() < 3e-2
     assert np.quantile(rel, 0.99) < 1e-3
     assert abs(got.mean() / ref.mean() - 1.0) < 1e-4
+
+
+@pytest.mark.gpu
+def test_gpu_atrous_tile_autotune(gpu):
+    """The a-trous workgroup tile (16, 32, 64 or 128 columns) changes no
+    result: the whole chain gives bit-identical planes with every width, and
+    rgExtDenoiseAutotune keeps one of them after timing all four."""
+    print("a-trous tile widths and autotune at 640x360", flush=True)
+    W, H = 640, 360
+    p = frame_params(W, H)
+    p.atrousIterations = 5
+    outs = {}
+    for width in (32, 16, 64, 128):
+        cur, prev = make_gbuffer(W, H, 91)
+        g = FramebufferSet(W, H)
+        fill_planes(cur, prev, g, None)
+        ffi.call("rgExtFramebuffersSetAtrousTile", g.h, width)
+        ffi.call("rgExtDenoise", g.h, p, None)
+        outs[width] = {name: g.download(PLANE[name]).copy() for name in ("PRE_FINAL", "DIFF_PONG", "SPEC_PONG", "INDIR_PONG")}
+    for width in (16, 64, 128):
+        for name, a in outs[32].items():
+            assert np.array_equal(a, outs[width][name]), (width, name)
+    chosen, ms = ctypes.c_uint32(0), (ctypes.c_float * 4)()
+    ffi.call("rgExtDenoiseAutotune", g.h, p, None, ctypes.byref(chosen), ms)
+    print(f"  iteration-1 ms for widths 16/32/64/128: {[round(m, 4) for m in ms]}, kept {chosen.value}", flush=True)
+    assert chosen.value in (16, 32, 64, 128) and all(m > 0.0 for m in ms)
+    assert ms[(16, 32, 64, 128).index(chosen.value)] == min(ms)
+    with pytest.raises(ffi.RgError):
+        ffi.call("rgExtFramebuffersSetAtrousTile", g.h, 48)
