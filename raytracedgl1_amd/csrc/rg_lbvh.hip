@@ -670,8 +670,25 @@ __global__ __launch_bounds__( THREADS ) void k_collapse_level( const uint32_t* f
     for( uint32_t t = blockIdx.x * THREADS + threadIdx.x; t < count; t += gridDim.x * THREADS )
     {
         const uint32_t b = frontierIn[ t ];
-        uint32_t       ch[ 8 ];
-        int            nch = 0;
+        // The node's children so far, each with whether it may be opened
+        // (internal, more than leafMax triangles) and its half area.  Every
+        // index into these arrays is a compile-time constant (the loops are
+        // unrolled and the updates predicated), so they stay in registers.
+        uint32_t ch[ 8 ];
+        float    ar[ 8 ];
+        bool     ex[ 8 ];
+#pragma unroll
+        for( int k = 0; k < 8; k++ )
+        {
+            ch[ k ] = 0xFFFFFFFFu;
+            ar[ k ] = -1.0f;
+            ex[ k ] = false;
+        }
+        auto info = [ & ]( uint32_t c, float& a, bool& e ) {
+            e = c < n - 1 && NodeCount( c, n, rf, rl ) > leafMax;
+            a = e ? HalfArea( bounds + size_t( c ) * 6 ) : -1.0f;
+        };
+        int nch = 0;
         if( b >= n - 1 || NodeCount( b, n, rf, rl ) <= leafMax )
         {
             ch[ 0 ] = b; // a root that is small enough is a single leaf
@@ -681,50 +698,85 @@ __global__ __launch_bounds__( THREADS ) void k_collapse_level( const uint32_t* f
         {
             ch[ 0 ] = children[ b * 2 + 0 ];
             ch[ 1 ] = children[ b * 2 + 1 ];
-            nch     = 2;
+            info( ch[ 0 ], ar[ 0 ], ex[ 0 ] );
+            info( ch[ 1 ], ar[ 1 ], ex[ 1 ] );
+            nch = 2;
             while( nch < 8 )
             {
+                // the first child of largest area among those that open
                 int   best     = -1;
                 float bestArea = -1.0f;
-                for( int k = 0; k < nch; k++ )
+#pragma unroll
+                for( int k = 0; k < 8; k++ )
                 {
-                    const uint32_t c = ch[ k ];
-                    if( c < n - 1 && NodeCount( c, n, rf, rl ) > leafMax )
+                    if( k < nch && ex[ k ] && ar[ k ] > bestArea )
                     {
-                        const float a = HalfArea( bounds + size_t( c ) * 6 );
-                        if( a > bestArea )
-                        {
-                            bestArea = a;
-                            best     = k;
-                        }
+                        bestArea = ar[ k ];
+                        best     = k;
                     }
                 }
                 if( best < 0 )
                 {
                     break;
                 }
-                const uint32_t c = ch[ best ];
-                for( int k = nch; k > best + 1; k-- )
+                uint32_t c = 0;
+#pragma unroll
+                for( int k = 0; k < 8; k++ )
                 {
-                    ch[ k ] = ch[ k - 1 ];
+                    c = k == best ? ch[ k ] : c;
                 }
-                ch[ best ]     = children[ c * 2 + 0 ];
-                ch[ best + 1 ] = children[ c * 2 + 1 ];
+                const uint32_t l = children[ c * 2 + 0 ], r = children[ c * 2 + 1 ];
+                float          al, arr;
+                bool           el, er;
+                info( l, al, el );
+                info( r, arr, er );
+                // ch[best + 2 ..] = old ch[best + 1 .. nch - 1]; then the two
+                // children take slots best and best + 1
+#pragma unroll
+                for( int k = 7; k >= 1; k-- )
+                {
+                    if( k <= nch && k > best + 1 )
+                    {
+                        ch[ k ] = ch[ k - 1 ];
+                        ar[ k ] = ar[ k - 1 ];
+                        ex[ k ] = ex[ k - 1 ];
+                    }
+                }
+#pragma unroll
+                for( int k = 0; k < 8; k++ )
+                {
+                    if( k == best )
+                    {
+                        ch[ k ] = l;
+                        ar[ k ] = al;
+                        ex[ k ] = el;
+                    }
+                    else if( k == best + 1 )
+                    {
+                        ch[ k ] = r;
+                        ar[ k ] = arr;
+                        ex[ k ] = er;
+                    }
+                }
                 nch++;
             }
         }
         const uint32_t slot = b < n - 1 ? b : 0;
+#pragma unroll
         for( int k = 0; k < 8; k++ )
         {
             expansion[ size_t( slot ) * 8 + k ] = k < nch ? ch[ k ] : 0xFFFFFFFFu;
         }
         isNode8[ slot ] = 1u;
-        for( int k = 0; k < nch; k++ )
+        if( nch > 1 )
         {
-            const uint32_t c = ch[ k ];
-            if( c < n - 1 && NodeCount( c, n, rf, rl ) > leafMax && !( nch == 1 ) )
+#pragma unroll
+            for( int k = 0; k < 8; k++ )
             {
-                frontierOut[ atomicAdd( countOut, 1u ) ] = c;
+                if( k < nch && ex[ k ] )
+                {
+                    frontierOut[ atomicAdd( countOut, 1u ) ] = ch[ k ];
+                }
             }
         }
     }
