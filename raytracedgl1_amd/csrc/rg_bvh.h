@@ -70,6 +70,7 @@ struct BvhObject
     DeviceBuffer levelHist;    // per-depth counts, then offsets and cursors
     DeviceBuffer levelOrder;   // (n-1) internal nodes grouped by depth
     DeviceBuffer expansion;    // max(1,n-1) * 8 u32
+    DeviceBuffer expandInfo;   // max(1,n-1) uint4: children with open bits, their half areas
     DeviceBuffer isNode8;      // max(1,n-1) u32
     DeviceBuffer node8Index;   // max(1,n-1) u32
     DeviceBuffer frontier[ 2 ];

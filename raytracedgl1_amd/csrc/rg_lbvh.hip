@@ -650,6 +650,28 @@ __device__ __forceinline__ uint32_t NodeCount( uint32_t id, uint32_t n, const ui
     return id >= n - 1 ? 1u : rl[ id ] - rf[ id ] + 1u;
 }
 
+constexpr uint32_t OPEN_BIT = 0x80000000u;
+
+// Per internal binary node: its two children, each with OPEN_BIT set when it
+// may be opened (internal, more than leafMax triangles), and their half
+// areas (-1 for the others).  The collapse then reads one 16-byte record per
+// opening instead of a chain of child, range and box loads.
+__global__ __launch_bounds__( THREADS ) void k_expand_info( uint32_t internal, uint32_t n, uint32_t leafMax, const uint32_t* children,
+                                                            const uint32_t* rf, const uint32_t* rl, const float* bounds, uint4* info )
+{
+    const uint32_t b = blockIdx.x * THREADS + threadIdx.x;
+    if( b >= internal )
+    {
+        return;
+    }
+    const uint32_t l  = children[ b * 2 + 0 ], r = children[ b * 2 + 1 ];
+    const bool     el = l < n - 1 && NodeCount( l, n, rf, rl ) > leafMax;
+    const bool     er = r < n - 1 && NodeCount( r, n, rf, rl ) > leafMax;
+    const float    al = el ? HalfArea( bounds + size_t( l ) * 6 ) : -1.0f;
+    const float    ar = er ? HalfArea( bounds + size_t( r ) * 6 ) : -1.0f;
+    info[ b ]         = make_uint4( l | ( el ? OPEN_BIT : 0u ), r | ( er ? OPEN_BIT : 0u ), __float_as_uint( al ), __float_as_uint( ar ) );
+}
+
 // Expands binary node b into up to 8 children by repeatedly opening the
 // child with the largest surface area (first one on ties) among those that
 // are internal and hold more than leafMax triangles.
@@ -659,10 +681,9 @@ __global__ __launch_bounds__( THREADS ) void k_collapse_level( const uint32_t* f
                                                                uint32_t*       countOut,
                                                                uint32_t        n,
                                                                uint32_t        leafMax,
-                                                               const uint32_t* children,
+                                                               const uint4*    info,
                                                                const uint32_t* rf,
                                                                const uint32_t* rl,
-                                                               const float*    bounds,
                                                                uint32_t*       expansion,
                                                                uint32_t*       isNode8 )
 {
@@ -684,10 +705,6 @@ __global__ __launch_bounds__( THREADS ) void k_collapse_level( const uint32_t* f
             ar[ k ] = -1.0f;
             ex[ k ] = false;
         }
-        auto info = [ & ]( uint32_t c, float& a, bool& e ) {
-            e = c < n - 1 && NodeCount( c, n, rf, rl ) > leafMax;
-            a = e ? HalfArea( bounds + size_t( c ) * 6 ) : -1.0f;
-        };
         int nch = 0;
         if( b >= n - 1 || NodeCount( b, n, rf, rl ) <= leafMax )
         {
@@ -696,11 +713,14 @@ __global__ __launch_bounds__( THREADS ) void k_collapse_level( const uint32_t* f
         }
         else
         {
-            ch[ 0 ] = children[ b * 2 + 0 ];
-            ch[ 1 ] = children[ b * 2 + 1 ];
-            info( ch[ 0 ], ar[ 0 ], ex[ 0 ] );
-            info( ch[ 1 ], ar[ 1 ], ex[ 1 ] );
-            nch = 2;
+            const uint4 q = info[ b ];
+            ch[ 0 ]       = q.x & ~OPEN_BIT;
+            ch[ 1 ]       = q.y & ~OPEN_BIT;
+            ex[ 0 ]       = ( q.x & OPEN_BIT ) != 0u;
+            ex[ 1 ]       = ( q.y & OPEN_BIT ) != 0u;
+            ar[ 0 ]       = __uint_as_float( q.z );
+            ar[ 1 ]       = __uint_as_float( q.w );
+            nch           = 2;
             while( nch < 8 )
             {
                 // the first child of largest area among those that open
@@ -725,11 +745,10 @@ __global__ __launch_bounds__( THREADS ) void k_collapse_level( const uint32_t* f
                 {
                     c = k == best ? ch[ k ] : c;
                 }
-                const uint32_t l = children[ c * 2 + 0 ], r = children[ c * 2 + 1 ];
-                float          al, arr;
-                bool           el, er;
-                info( l, al, el );
-                info( r, arr, er );
+                const uint4    q   = info[ c ];
+                const uint32_t l   = q.x & ~OPEN_BIT, r = q.y & ~OPEN_BIT;
+                const bool     el  = ( q.x & OPEN_BIT ) != 0u, er = ( q.y & OPEN_BIT ) != 0u;
+                const float    al  = __uint_as_float( q.z ), arr = __uint_as_float( q.w );
                 // ch[best + 2 ..] = old ch[best + 1 .. nch - 1]; then the two
                 // children take slots best and best + 1
 #pragma unroll
@@ -1141,6 +1160,13 @@ void BuildBvhImpl( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtB
     }
 
     // 6. Collapse to 8-wide nodes, level by level.
+    bvh.expandInfo.Reserve( size_t( internal > 0 ? internal : 1 ) * sizeof( uint4 ) );
+    if( internal > 0 )
+    {
+        k_expand_info<<< DivUp( internal, THREADS ), THREADS, 0, stream >>>( internal, n, bvh.leafMax, children, rf, rl, bounds,
+                                                                            bvh.expandInfo.as< uint4 >() );
+        CheckLaunch( "k_expand_info" );
+    }
     uint32_t* counts = bvh.levelCounts.as< uint32_t >();
     RG_HIP_CHECK( hipMemsetAsync( counts, 0, ( BVH_MAX_LEVELS + 2 ) * 4, stream ) );
     RG_HIP_CHECK( hipMemsetAsync( bvh.isNode8.ptr, 0, size_t( slots ) * 4, stream ) );
@@ -1175,10 +1201,9 @@ void BuildBvhImpl( BvhObject& bvh, const float* d_tris, uint32_t n, const RgExtB
                                                                       counts + level + 1,
                                                                       n,
                                                                       bvh.leafMax,
-                                                                      children,
+                                                                      bvh.expandInfo.as< uint4 >(),
                                                                       rf,
                                                                       rl,
-                                                                      bounds,
                                                                       bvh.expansion.as< uint32_t >(),
                                                                       bvh.isNode8.as< uint32_t >() );
             CheckLaunch( "k_collapse_level" );
