@@ -462,7 +462,8 @@ RGAPI RgResult RGCONV rgExtDenoiseAtrous( RgExtFramebuffers fb, const RgExtFrame
 RGAPI RgResult RGCONV rgExtDenoise( RgExtFramebuffers fb, const RgExtFrameParams* p, void* stream );
 /* No counterpart (the reference's compute shaders have fixed 16x16 groups):
  * the workgroup tile of a-trous iterations >= 1, 256 pixels of tileWidth
- * 16, 32 (the default), 64 or 128 columns.  Results do not depend on it.
+ * 16, 32, 64 (the default: the fastest at 1080p) or 128 columns.  Results
+ * do not depend on it.
  * rgExtDenoiseAutotune times iteration 1 on the set's current planes with
  * each width (it writes that iteration's outputs and synchronises the
  * stream), keeps the fastest for later a-trous calls on the set and returns

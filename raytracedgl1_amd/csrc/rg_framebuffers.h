@@ -68,7 +68,7 @@ struct FramebufferSet
     DeviceBuffer     planes[ RG_EXT_PLANE_COUNT ][ 2 ];
     int              parity = 0;
     TransientPlanes* shared = nullptr; // within-frame planes live there when set
-    uint32_t         atrousTileW = 32; // a-trous iterations >= 1: workgroup tile width (x 256/width rows)
+    uint32_t         atrousTileW = 64; // a-trous iterations >= 1: workgroup tile width (x 256/width rows)
 
     // With `transient`, only the persistent planes are allocated here.
     void   Create( uint32_t w, uint32_t h, uint32_t storageMode = RG_EXT_STORAGE_FP32, TransientPlanes* transient = nullptr );
