@@ -489,6 +489,27 @@ def group_frames(soups, res, eyes, rounds: int = 3) -> dict:
         srv.close()
 
 
+def bvh_build_time(scene, reps: int = 5) -> dict:
+    """The LBVH build of the frame's triangles (one triangle per leaf, FP32
+    nodes, as the instance builds it), best of `reps` synchronised builds
+    after a warm one.  Off the timed frames: static geometry is built once."""
+    import numpy as np
+
+    from raytracedgl1_amd import ffi
+
+    d_tris = ffi.DeviceArray.from_numpy(np.ascontiguousarray(scene.tris, np.float32))
+    b = ffi.Bvh()
+    b.build(d_tris, len(scene.tris), 1)
+    ffi.call("rgExtSynchronize")
+    best = 1e30
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        b.build(d_tris, len(scene.tris), 1)
+        ffi.call("rgExtSynchronize")
+        best = min(best, time.perf_counter() - t0)
+    return {"triangles": len(scene.tris), "ms": best * 1e3, "mtris_per_s": len(scene.tris) / best / 1e6}
+
+
 def load_profile(path):
     """A counter summary committed under profiles/ (None when absent)."""
     try:
@@ -586,6 +607,7 @@ def main(argv=None) -> int:
         },
         "denoise_kernels": None,
         "atrous_autotune": getattr(a, "atrous_tune", None),
+        "bvh_build": bvh_build_time(scene),
         "cpu_baseline": None,
     }
     dprof = load_profile(DENOISE_TRAFFIC_FILE)
