@@ -257,6 +257,34 @@ def test_scene_group_batches_match_alone():
             assert np.array_equal(own[k][p], got[k][p]), (k, p)
 
 
+def test_scene_group_fp16_nodes():
+    """The precision form's FP16 nodes in a group's batched BVH: grouped and
+    batched frames equal each session's frames through its own FP16-node
+    BVH, bit for bit."""
+    print("serving: grouped scenes with FP16 nodes, batches of up to 4", flush=True)
+    specs = list(enumerate(GROUP_SPECS[:4]))
+    sc = _group_scenes()
+    prec = ffi.RG_EXT_PRECISION_FP16_NODES
+    srv = serving.Server(256, 128, streams=1, frames_in_flight=1, batch=4, precision=prec)
+    batches = []
+    try:
+        handles = [srv.scene(s) for s in sc]
+        srv.group(handles)
+        got = _render(srv, [(k, srv.session(handles[si], w, h)) for k, (si, w, h) in specs], FRAMES, batches)
+    finally:
+        srv.close()
+    print(f"  batch sizes of the finished frames: {sorted(batches)}", flush=True)
+    assert max(batches) > 1
+    for k, (si, w, h) in specs:
+        srv = serving.Server(256, 128, streams=1, precision=prec)
+        try:
+            own = _render(srv, [(k, srv.session(srv.scene(sc[si]), w, h))], FRAMES)
+        finally:
+            srv.close()
+        for p in PERSISTENT:
+            assert np.array_equal(own[k][p], got[k][p]), (k, p)
+
+
 def test_scene_group_refuses_other_light_kinds():
     a, _ = _scenes()
     _, b = _scenes()  # a sun and sphere lights
